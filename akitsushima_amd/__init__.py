@@ -1,0 +1,34 @@
+"""akitsushima_amd: an MI355X-native FP64 force engine for akitsushima.
+
+Hand-written HIP kernels for gfx950 behind the reference's force-function
+API: Tersoff, Stillinger-Weber and Zhou EAM energies, forces and virial
+stress, a device cell-list neighbour list, batched evaluation of many
+structures, a device-resident NVE integrator and a Tersoff parameter
+gradient.  The compute path is the HIP library ``libakitsushima_amd.so``
+only; importing works without a GPU, computing does not.
+"""
+from .neighborlist import NeighbourList
+from .potentials import (
+    EAMPotential,
+    PotentialEngine,
+    StillingerWeberPotential,
+    TersoffPotential,
+    call_force_fn,
+    force_fn_from_config,
+    make_force_fn,
+)
+from .zhou import zhou_eam
+
+__version__ = "0.1.0"
+
+__all__ = [
+    "NeighbourList",
+    "TersoffPotential",
+    "StillingerWeberPotential",
+    "EAMPotential",
+    "PotentialEngine",
+    "make_force_fn",
+    "call_force_fn",
+    "force_fn_from_config",
+    "zhou_eam",
+]

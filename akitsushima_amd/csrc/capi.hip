@@ -1,0 +1,539 @@
+// C ABI of the engine (include/akitsushima_amd.h): handles, argument checks,
+// host<->device staging, and the owned-list policy of the reference's force
+// functions.
+#include "../../include/akitsushima_amd.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+
+using namespace ak;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    g_err.clear();
+    return AK_OK;
+  } catch (const Error& e) {
+    return fail(e.code, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(AK_ERR_CAPACITY, "host out of memory");
+  } catch (const std::exception& e) {
+    return fail(AK_ERR_STATE, e.what());
+  }
+}
+
+// Smallest singular value of a 3x3 matrix (eigenvalues of M^T M by Jacobi).
+double sigma_min3(const double* M) {
+  double A[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k) s += M[3 * k + i] * M[3 * k + j];
+      A[i][j] = s;
+    }
+  for (int sweep = 0; sweep < 50; ++sweep) {
+    double off = std::fabs(A[0][1]) + std::fabs(A[0][2]) + std::fabs(A[1][2]);
+    if (off < 1e-300) break;
+    for (int p = 0; p < 2; ++p)
+      for (int q = p + 1; q < 3; ++q) {
+        if (A[p][q] == 0.0) continue;
+        double th = 0.5 * (A[q][q] - A[p][p]) / A[p][q];
+        double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < 3; ++k) {
+          double akp = A[k][p], akq = A[k][q];
+          A[k][p] = c * akp - s * akq;
+          A[k][q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < 3; ++k) {
+          double apk = A[p][k], aqk = A[q][k];
+          A[p][k] = c * apk - s * aqk;
+          A[q][k] = s * apk + c * aqk;
+        }
+      }
+  }
+  double ev = std::min(A[0][0], std::min(A[1][1], A[2][2]));
+  return std::sqrt(std::max(ev, 0.0));
+}
+
+void matmul3(const double* a, const double* b, double* c) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k) s += a[3 * i + k] * b[3 * k + j];
+      c[3 * i + j] = s;
+    }
+}
+
+bool lattice_allclose(const double* a, const double* b) {
+  // np.allclose(lattice, built, atol=1e-12) with its default rtol 1e-5
+  for (int k = 0; k < 9; ++k)
+    if (!(std::fabs(a[k] - b[k]) <= 1e-12 + 1e-5 * std::fabs(b[k]))) return false;
+  return true;
+}
+
+void check_finite(const double* x, size_t n, const char* what) {
+  for (size_t k = 0; k < n; ++k)
+    AK_REQUIRE(std::isfinite(x[k]), AK_ERR_ARG, std::string(what) + " contains non-finite values");
+}
+
+}  // namespace
+
+struct ak_nlist {
+  NList nl;
+  SetDev set;
+  hipStream_t stream = nullptr;
+  DevBuf d_pos, d_dmax;
+  int64_t n_steps = 0;
+  ~ak_nlist() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+struct ak_potential {
+  Potential pot;
+  hipStream_t stream = nullptr;
+  // owned list (engine/neighborlist.py:1103-1163) and its structure
+  ak_nlist owned;
+  double owned_skin = -1.0;
+  // batched layout
+  ak_nlist batch;
+  std::vector<int64_t> batch_off;
+  // staging
+  DevBuf d_pos, d_spc, d_F, d_out;
+  std::vector<int32_t> last_spc;
+  bool spc_valid = false;
+  ~ak_potential() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+static hipStream_t make_stream() {
+  hipStream_t s;
+  AK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  return s;
+}
+
+static void require_device() {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0)
+    throw Error(AK_ERR_NO_DEVICE, "no HIP device is visible: the akitsushima_amd force "
+                                  "engine runs on the GPU only");
+}
+
+// Build or keep `L` valid for positions already on the device
+// (NeighbourList.check_and_rebuild, engine/neighborlist.py:613-673).
+static bool check_and_rebuild(ak_nlist& L, int n, const double* lattice, const double* inv,
+                              const double* d_pos, hipStream_t st) {
+  L.n_steps++;
+  L.nl.n_checks++;
+  L.set.setup(1, std::vector<int64_t>{0, (int64_t)n}.data(), lattice, inv, L.nl.r_list, st);
+  if (!L.nl.valid || L.nl.n_atoms != n) {
+    nl_build(L.nl, L.set, d_pos, st);
+    return true;
+  }
+  const double* Lb = L.nl.built_geoms[0].lat;
+  const double* Ib = L.nl.built_geoms[0].inv;
+  double sig = 1.0;
+  Mat3 M;
+  matmul3(Ib, lattice, M.m);
+  if (!lattice_allclose(lattice, Lb)) sig = sigma_min3(M.m);
+  L.d_dmax.ensure(sizeof(double));
+  nl_check_displacement(L.nl, L.set, d_pos, M, sig, nullptr, L.d_dmax.as<double>(), st);
+  double dmax = 0.0;
+  AK_HIP(hipMemcpyAsync(&dmax, L.d_dmax.p, sizeof(double), hipMemcpyDeviceToHost, st));
+  AK_HIP(hipStreamSynchronize(st));
+  if (sig * L.nl.r_list - 2.0 * dmax <= L.nl.r_cut) {
+    nl_build(L.nl, L.set, d_pos, st);
+    return true;
+  }
+  return false;
+}
+
+static void stage_species(ak_potential* P, int n, const int32_t* species) {
+  std::vector<int32_t> spc(species ? species : nullptr, species ? species + n : nullptr);
+  if (!species) spc.assign(n, 0);
+  for (int a = 0; a < n; ++a)
+    AK_REQUIRE(spc[a] >= 0 && spc[a] < P->pot.ns, AK_ERR_ARG,
+               "species index out of range for this potential");
+  if (P->spc_valid && spc == P->last_spc) return;
+  P->d_spc.ensure(sizeof(int32_t) * std::max(1, n));
+  if (n > 0)
+    AK_HIP(hipMemcpyAsync(P->d_spc.p, spc.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice,
+                          P->stream));
+  AK_HIP(hipStreamSynchronize(P->stream));
+  P->last_spc.swap(spc);
+  P->spc_valid = true;
+}
+
+extern "C" {
+
+int ak_abi_version(void) { return AK_ABI_VERSION; }
+
+const char* ak_last_error(void) { return g_err.c_str(); }
+
+int ak_device_count(int32_t* count) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  if (count) *count = n;
+  return AK_OK;
+}
+
+static int make_potential(ak_potential** out, int kind, int ns, double r_cut,
+                          std::vector<double> params, std::vector<int> map) {
+  return guarded([&] {
+    AK_REQUIRE(out, AK_ERR_ARG, "null output handle");
+    require_device();
+    auto P = std::make_unique<ak_potential>();
+    P->pot.kind = kind;
+    P->pot.ns = ns;
+    P->pot.r_cut = r_cut;
+    P->pot.h_params = std::move(params);
+    P->pot.h_map = std::move(map);
+    P->stream = make_stream();
+    P->pot.d_params.ensure(sizeof(double) * P->pot.h_params.size());
+    AK_HIP(hipMemcpy(P->pot.d_params.p, P->pot.h_params.data(),
+                     sizeof(double) * P->pot.h_params.size(), hipMemcpyHostToDevice));
+    if (!P->pot.h_map.empty()) {
+      P->pot.d_map.ensure(sizeof(int) * P->pot.h_map.size());
+      AK_HIP(hipMemcpy(P->pot.d_map.p, P->pot.h_map.data(), sizeof(int) * P->pot.h_map.size(),
+                       hipMemcpyHostToDevice));
+    }
+    *out = P.release();
+  });
+}
+
+int ak_tersoff_create(int32_t ns, const double* pair_table, const double* triple_table,
+                      double r_cut, ak_potential** out) {
+  if (ns < 1 || ns > 16 || !pair_table || !triple_table || !(r_cut > 0.0))
+    return fail(AK_ERR_ARG, "ak_tersoff_create: bad arguments");
+  std::vector<double> p(pair_table, pair_table + (size_t)ns * ns * 9);
+  p.insert(p.end(), triple_table, triple_table + (size_t)ns * ns * ns * 7);
+  return make_potential(out, KIND_TERSOFF, ns, r_cut, std::move(p), {});
+}
+
+int ak_sw_create(const double* prm, ak_potential** out) {
+  if (!prm) return fail(AK_ERR_ARG, "ak_sw_create: null parameters");
+  if (!(prm[1] > 0.0) || !(prm[6] > 0.0)) return fail(AK_ERR_ARG, "ak_sw_create: sigma, a > 0");
+  return make_potential(out, KIND_SW, 1, prm[6] * prm[1], std::vector<double>(prm, prm + 10),
+                        {});
+}
+
+int ak_eam_create(int32_t ns, int32_t nf, const double* fn, const int32_t* map, double r_cut,
+                  ak_potential** out) {
+  if (ns < 1 || ns > 16 || nf < 1 || !fn || !map || !(r_cut > 0.0))
+    return fail(AK_ERR_ARG, "ak_eam_create: bad arguments");
+  std::vector<int> m(map, map + 2 * ns + ns * ns);
+  for (int v : m)
+    if (v < 0 || v >= nf) return fail(AK_ERR_ARG, "ak_eam_create: index_map out of range");
+  return make_potential(out, KIND_EAM, ns, r_cut, std::vector<double>(fn, fn + (size_t)nf * 44),
+                        std::move(m));
+}
+
+int ak_potential_destroy(ak_potential* pot) {
+  return guarded([&] { delete pot; });
+}
+
+int ak_potential_set_lanes(ak_potential* pot, int32_t lanes) {
+  if (!pot) return fail(AK_ERR_ARG, "null potential");
+  if (lanes != 0 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
+    return fail(AK_ERR_ARG, "lanes must be 0, 4, 8, 16, 32 or 64");
+  pot->pot.eam_lanes = lanes;
+  return AK_OK;
+}
+
+int ak_nlist_create(double r_cut, double skin, ak_nlist** out) {
+  return guarded([&] {
+    AK_REQUIRE(out, AK_ERR_ARG, "null output handle");
+    AK_REQUIRE(r_cut > 0.0, AK_ERR_ARG, "NeighbourList: r_cut must be positive");
+    AK_REQUIRE(skin >= 0.0, AK_ERR_ARG, "NeighbourList: skin_width must be >= 0");
+    require_device();
+    auto L = std::make_unique<ak_nlist>();
+    L->nl.r_cut = r_cut;
+    L->nl.skin = skin;
+    L->nl.r_list = r_cut + skin;
+    L->stream = make_stream();
+    *out = L.release();
+  });
+}
+
+int ak_nlist_destroy(ak_nlist* nl) {
+  return guarded([&] { delete nl; });
+}
+
+int ak_nlist_build(ak_nlist* L, int32_t n, const double* lattice, const double* inv,
+                   const double* positions) {
+  return guarded([&] {
+    AK_REQUIRE(L && lattice && (positions || n == 0) && n >= 0, AK_ERR_ARG, "bad arguments");
+    check_finite(lattice, 9, "lattice");
+    check_finite(positions, 3 * (size_t)n, "positions");
+    L->set.setup(1, std::vector<int64_t>{0, (int64_t)n}.data(), lattice, inv, L->nl.r_list,
+                 L->stream);
+    L->d_pos.ensure(sizeof(double) * 3 * std::max(1, n));
+    if (n > 0)
+      AK_HIP(hipMemcpyAsync(L->d_pos.p, positions, sizeof(double) * 3 * n,
+                            hipMemcpyHostToDevice, L->stream));
+    nl_build(L->nl, L->set, L->d_pos.as<double>(), L->stream);
+    AK_HIP(hipStreamSynchronize(L->stream));
+  });
+}
+
+int ak_nlist_check_and_rebuild(ak_nlist* L, int32_t n, const double* lattice,
+                               const double* inv, const double* positions,
+                               int32_t* rebuilt) {
+  return guarded([&] {
+    AK_REQUIRE(L && lattice && (positions || n == 0) && n >= 0, AK_ERR_ARG, "bad arguments");
+    check_finite(lattice, 9, "lattice");
+    check_finite(positions, 3 * (size_t)n, "positions");
+    L->d_pos.ensure(sizeof(double) * 3 * std::max(1, n));
+    if (n > 0)
+      AK_HIP(hipMemcpyAsync(L->d_pos.p, positions, sizeof(double) * 3 * n,
+                            hipMemcpyHostToDevice, L->stream));
+    bool r = check_and_rebuild(*L, n, lattice, inv, L->d_pos.as<double>(), L->stream);
+    AK_HIP(hipStreamSynchronize(L->stream));
+    if (rebuilt) *rebuilt = r ? 1 : 0;
+  });
+}
+
+int ak_nlist_candidate_pairs(ak_nlist* L, int64_t* n_pairs, int32_t* oi, int32_t* oj,
+                             int64_t capacity) {
+  return guarded([&] {
+    AK_REQUIRE(L && n_pairs, AK_ERR_ARG, "bad arguments");
+    int64_t np = nl_half_pairs(L->nl, L->set, L->stream, nullptr, nullptr);
+    *n_pairs = np;
+    if (oi && oj) {
+      AK_REQUIRE(capacity >= np, AK_ERR_CAPACITY, "pair buffers too small");
+      nl_half_pairs(L->nl, L->set, L->stream, oi, oj);
+    }
+  });
+}
+
+int ak_nlist_counters(ak_nlist* L, int64_t* n_builds, int64_t* n_steps) {
+  if (!L) return fail(AK_ERR_ARG, "null list");
+  if (n_builds) *n_builds = L->nl.n_builds;
+  if (n_steps) *n_steps = L->n_steps;
+  return AK_OK;
+}
+
+int ak_compute(ak_potential* P, ak_nlist* nl, int32_t n, const double* lattice,
+               const double* inv, const double* positions, const int32_t* species,
+               int32_t need_stress, double* energy, double* forces, double* stress) {
+  return guarded([&] {
+    AK_REQUIRE(P && lattice && energy && (forces || n == 0) && n >= 0, AK_ERR_ARG,
+               "bad arguments");
+    AK_REQUIRE(positions || n == 0, AK_ERR_ARG, "null positions");
+    check_finite(lattice, 9, "lattice");
+    check_finite(positions, 3 * (size_t)n, "positions");
+    hipStream_t st = P->stream;
+    stage_species(P, n, species);
+    P->d_pos.ensure(sizeof(double) * 3 * std::max(1, n));
+    P->d_F.ensure(sizeof(double) * 3 * std::max(1, n));
+    P->d_out.ensure(sizeof(double) * 10);
+    if (n > 0)
+      AK_HIP(hipMemcpyAsync(P->d_pos.p, positions, sizeof(double) * 3 * n,
+                            hipMemcpyHostToDevice, st));
+    ak_nlist* L = nl;
+    if (L) {
+      AK_REQUIRE(L->nl.r_cut + 1e-12 >= P->pot.r_cut, AK_ERR_ARG,
+                 "neighbour list r_cut is smaller than the potential's cutoff");
+      // the caller's list, as built or rebuilt by the caller (the reference's
+      // potentials use a passed list's candidates without rebuilding it);
+      // the current cell is used for the minimum image
+      L->set.setup(1, std::vector<int64_t>{0, (int64_t)n}.data(), lattice, inv, L->nl.r_list,
+                   st);
+      AK_REQUIRE(L->nl.valid && L->nl.n_atoms == n, AK_ERR_STATE,
+                 "the neighbour list was not built for this structure");
+    } else {
+      // owned list: skin 0.3 A when the cell allows it, else none
+      L = &P->owned;
+      CellInfo ci = cell_info(lattice);
+      double skin = (P->pot.r_cut + 0.3 < ci.min_width / 2.0) ? 0.3 : 0.0;
+      if (skin != P->owned_skin) {
+        P->owned.nl.valid = false;
+        P->owned.nl.r_cut = P->pot.r_cut;
+        P->owned.nl.skin = skin;
+        P->owned.nl.r_list = P->pot.r_cut + skin;
+        P->owned_skin = skin;
+      }
+      check_and_rebuild(*L, n, lattice, inv, P->d_pos.as<double>(), st);
+    }
+    compute_forces(P->pot, L->nl, L->set, P->d_pos.as<double>(), P->d_spc.as<int>(),
+                   P->d_F.as<double>(), P->d_out.as<double>(), st);
+    double out[10];
+    AK_HIP(hipMemcpyAsync(out, P->d_out.p, sizeof(out), hipMemcpyDeviceToHost, st));
+    if (n > 0)
+      AK_HIP(hipMemcpyAsync(forces, P->d_F.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
+                            st));
+    AK_HIP(hipStreamSynchronize(st));
+    *energy = out[0];
+    if (stress && need_stress) std::memcpy(stress, out + 1, 9 * sizeof(double));
+  });
+}
+
+int ak_compute_device(ak_potential* P, ak_nlist* L, int32_t n, const double* lattice,
+                      const double* inv, const double* d_pos, const int32_t* d_spc,
+                      double* d_F, double* d_es, void* stream) {
+  return guarded([&] {
+    AK_REQUIRE(P && L && lattice && d_F && d_es && n >= 0, AK_ERR_ARG, "bad arguments");
+    AK_REQUIRE(L->nl.r_cut + 1e-12 >= P->pot.r_cut, AK_ERR_ARG,
+               "neighbour list r_cut is smaller than the potential's cutoff");
+    hipStream_t st = stream ? (hipStream_t)stream : P->stream;
+    L->set.setup(1, std::vector<int64_t>{0, (int64_t)n}.data(), lattice, inv, L->nl.r_list, st);
+    AK_REQUIRE(L->nl.valid && L->nl.n_atoms == n, AK_ERR_STATE,
+               "the neighbour list was not built for this structure");
+    const int* spc = d_spc;
+    if (!spc) {
+      stage_species(P, n, nullptr);
+      spc = P->d_spc.as<int>();
+    }
+    compute_forces(P->pot, L->nl, L->set, d_pos, spc, d_F, d_es, st);
+  });
+}
+
+int ak_compute_batched(ak_potential* P, int32_t S, const int64_t* off, const double* lattices,
+                       const double* invs, const double* positions, const int32_t* species,
+                       double* energies, double* forces, double* stresses) {
+  return guarded([&] {
+    AK_REQUIRE(P && S >= 1 && off && lattices && energies, AK_ERR_ARG, "bad arguments");
+    const int n = (int)off[S];
+    AK_REQUIRE(n >= 0 && (positions || n == 0) && (forces || n == 0), AK_ERR_ARG,
+               "bad arguments");
+    check_finite(lattices, 9 * (size_t)S, "lattices");
+    check_finite(positions, 3 * (size_t)n, "positions");
+    hipStream_t st = P->stream;
+    stage_species(P, n, species);
+    P->d_pos.ensure(sizeof(double) * 3 * std::max(1, n));
+    P->d_F.ensure(sizeof(double) * 3 * std::max(1, n));
+    P->d_out.ensure(sizeof(double) * 10 * S);
+    if (n > 0)
+      AK_HIP(hipMemcpyAsync(P->d_pos.p, positions, sizeof(double) * 3 * n,
+                            hipMemcpyHostToDevice, st));
+    ak_nlist& B = P->batch;
+    B.nl.r_cut = P->pot.r_cut;
+    B.nl.skin = 0.0;
+    B.nl.r_list = P->pot.r_cut;
+    B.set.setup(S, off, lattices, invs, B.nl.r_list, st);
+    nl_build(B.nl, B.set, P->d_pos.as<double>(), st);
+    P->batch_off.assign(off, off + S + 1);
+    compute_forces(P->pot, B.nl, B.set, P->d_pos.as<double>(), P->d_spc.as<int>(),
+                   P->d_F.as<double>(), P->d_out.as<double>(), st);
+    std::vector<double> out(10 * (size_t)S);
+    AK_HIP(hipMemcpyAsync(out.data(), P->d_out.p, sizeof(double) * out.size(),
+                          hipMemcpyDeviceToHost, st));
+    if (n > 0)
+      AK_HIP(hipMemcpyAsync(forces, P->d_F.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
+                            st));
+    AK_HIP(hipStreamSynchronize(st));
+    for (int s = 0; s < S; ++s) {
+      energies[s] = out[10 * s];
+      if (stresses) std::memcpy(stresses + 9 * s, out.data() + 10 * s + 1, 9 * sizeof(double));
+    }
+  });
+}
+
+int ak_compute_batched_device(ak_potential* P, int32_t S, const int64_t* off,
+                              const double* lattices, const double* d_pos,
+                              const int32_t* d_spc, double* d_F, double* d_es, int32_t rebuild,
+                              void* stream) {
+  return guarded([&] {
+    AK_REQUIRE(P && S >= 1 && off && lattices && d_F && d_es, AK_ERR_ARG, "bad arguments");
+    hipStream_t st = stream ? (hipStream_t)stream : P->stream;
+    ak_nlist& B = P->batch;
+    bool same = B.nl.valid && P->batch_off.size() == (size_t)S + 1 &&
+                std::equal(P->batch_off.begin(), P->batch_off.end(), off);
+    B.nl.r_cut = P->pot.r_cut;
+    B.nl.skin = 0.0;
+    B.nl.r_list = P->pot.r_cut;
+    B.set.setup(S, off, lattices, nullptr, B.nl.r_list, st);
+    if (!same || rebuild) {
+      nl_build(B.nl, B.set, d_pos, st);
+      P->batch_off.assign(off, off + S + 1);
+    }
+    const int* spc = d_spc;
+    if (!spc) {
+      stage_species(P, (int)off[S], nullptr);
+      spc = P->d_spc.as<int>();
+    }
+    compute_forces(P->pot, B.nl, B.set, d_pos, spc, d_F, d_es, st);
+  });
+}
+
+}  // extern "C"
+
+namespace ak {
+void md_nve_run(Potential& pot, int n, const double* lattice, const double* inv, double* pos,
+                double* vel, const double* mass, const int32_t* spc, double dt, int n_steps,
+                double skin, bool use_graph, double* e_pot, double* e_kin, int* n_rebuilds,
+                double* seconds, hipStream_t st);
+void tersoff_loss_grad_run(Potential& pot, int S, const int64_t* off, const double* lattices,
+                           const double* positions, const double* theta,
+                           const double* ref_energy, const double* ref_forces,
+                           const double* ref_stress, const double* weights, double* loss,
+                           double* grad, double* energies, double* forces, double* stresses,
+                           hipStream_t st);
+}  // namespace ak
+
+extern "C" {
+
+int ak_md_nve(ak_potential* P, int32_t n, const double* lattice, const double* inv,
+              double* positions, double* velocities, const double* masses,
+              const int32_t* species, double dt, int32_t n_steps, double skin,
+              int32_t use_graph, double* e_pot, double* e_kin, int32_t* n_rebuilds,
+              double* seconds) {
+  return guarded([&] {
+    AK_REQUIRE(P && lattice && positions && velocities && masses && n > 0, AK_ERR_ARG,
+               "bad arguments");
+    AK_REQUIRE(skin >= 0.0, AK_ERR_ARG, "skin_width must be >= 0");
+    check_finite(lattice, 9, "lattice");
+    check_finite(positions, 3 * (size_t)n, "positions");
+    check_finite(velocities, 3 * (size_t)n, "velocities");
+    for (int a = 0; a < n; ++a) {
+      AK_REQUIRE(masses[a] > 0.0, AK_ERR_ARG, "masses must be positive");
+      if (species)
+        AK_REQUIRE(species[a] >= 0 && species[a] < P->pot.ns, AK_ERR_ARG,
+                   "species index out of range for this potential");
+    }
+    int nr = 0;
+    md_nve_run(P->pot, n, lattice, inv, positions, velocities, masses, species, dt, n_steps,
+               skin, use_graph != 0, e_pot, e_kin, &nr, seconds, P->stream);
+    if (n_rebuilds) *n_rebuilds = nr;
+  });
+}
+
+int ak_tersoff_loss_grad(ak_potential* P, int32_t S, const int64_t* off,
+                         const double* lattices, const double* positions,
+                         const double* theta, const double* ref_energy,
+                         const double* ref_forces, const double* ref_stress,
+                         const double* weights, double* loss, double* grad,
+                         double* energies, double* forces, double* stresses) {
+  return guarded([&] {
+    AK_REQUIRE(P && S >= 1 && off && lattices && positions && theta && ref_energy &&
+                   ref_forces && ref_stress && weights && loss,
+               AK_ERR_ARG, "bad arguments");
+    AK_REQUIRE(P->pot.kind == KIND_TERSOFF && P->pot.ns == 1, AK_ERR_ARG,
+               "the parameter gradient is defined for a single-species Tersoff potential");
+    tersoff_loss_grad_run(P->pot, S, off, lattices, positions, theta, ref_energy, ref_forces,
+                          ref_stress, weights, loss, grad, energies, forces, stresses,
+                          P->stream);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,108 @@
+// Shared device-side definitions of the akitsushima_amd force engine (gfx950).
+//
+// Layout conventions (all FP64, row-major, C-contiguous):
+//   lattice rows are lattice vectors, cart = frac @ lattice; rij = r_j - r_i;
+//   positions (N, 3); forces (N, 3); stress (3, 3) = -(1/V) sum_bonds G (x) r.
+// A "set" is one or more periodic structures concatenated atom-wise; atom a
+// belongs to structure sid[a], whose geometry is geoms[sid[a]].
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ak {
+
+constexpr int kWave = 64;
+
+struct Mat3 {
+  double m[9];
+};
+
+struct Geom {
+  double lat[9];
+  double inv[9];
+  double volume;
+  int nc[3];      // cell-grid dimensions of the neighbour list
+  int cell_off;   // first global cell id of this structure
+  int atom_off;   // first atom of this structure
+  int natoms;
+  int pad;
+};
+
+// Minimum-image displacement pj - pi in the structure's cell, with the
+// reference's operation order (engine/neighborlist.py:350-361): fractional
+// components rounded half-to-even (rint), no fused multiply-add, so the
+// neighbour-list predicate r2 < rl2 matches the reference bit for bit.
+__device__ __forceinline__ void mic_exact(const double* __restrict__ lat,
+                                          const double* __restrict__ inv, double pix,
+                                          double piy, double piz, double pjx, double pjy,
+                                          double pjz, double& dx, double& dy, double& dz) {
+#pragma clang fp contract(off)
+  double rx = pjx - pix, ry = pjy - piy, rz = pjz - piz;
+  double sx = rx * inv[0] + ry * inv[3] + rz * inv[6];
+  double sy = rx * inv[1] + ry * inv[4] + rz * inv[7];
+  double sz = rx * inv[2] + ry * inv[5] + rz * inv[8];
+  sx -= rint(sx);
+  sy -= rint(sy);
+  sz -= rint(sz);
+  dx = sx * lat[0] + sy * lat[3] + sz * lat[6];
+  dy = sx * lat[1] + sy * lat[4] + sz * lat[7];
+  dz = sx * lat[2] + sy * lat[5] + sz * lat[8];
+}
+
+__device__ __forceinline__ double r2_exact(double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// Clamped exp of the reference (engine/_numba_common.py:12-32).
+__device__ __forceinline__ double nb_exp(double x) {
+  if (x > 500.0) return exp(500.0);
+  if (x < -500.0) return 0.0;
+  return exp(x);
+}
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  return (uint64_t(1) << __lane_id()) - 1;
+}
+
+// Sum over a segment of L consecutive lanes (L a power of two <= 64); every
+// lane of the segment receives the total.
+template <int L>
+__device__ __forceinline__ double seg_sum(double v) {
+#pragma unroll
+  for (int o = L / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// Deterministic block reduction of `n` doubles per thread (blockDim.x a
+// multiple of 64, <= 1024); thread 0 receives the sums in out[].
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* smem /* NV*16 */) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double x = v[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
+    v[k] = x;
+  }
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) smem[k * 16 + wid] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.0;
+      for (int w = 0; w < nw; ++w) s += smem[k * 16 + w];
+      v[k] = s;
+    }
+  }
+}
+
+}  // namespace ak

@@ -1,0 +1,149 @@
+// Host-side state of the akitsushima_amd engine: device buffers, structure
+// sets, neighbour lists, potentials, and the launchers each .hip file exports.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/akitsushima_amd.h"
+#include "common.hpp"
+
+namespace ak {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define AK_HIP(x)                                                                     \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess)                                                             \
+      throw ::ak::Error(AK_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define AK_REQUIRE(cond, code, msg)              \
+  do {                                           \
+    if (!(cond)) throw ::ak::Error((code), (msg)); \
+  } while (0)
+
+// Grow-only device allocation.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void ensure(size_t b) {
+    if (b <= bytes && p) return;
+    if (p) AK_HIP(hipFree(p));
+    p = nullptr;
+    bytes = 0;
+    size_t nb = b < 256 ? 256 : b;
+    AK_HIP(hipMalloc(&p, nb));
+    bytes = nb;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// Host geometry of one periodic cell (C++ restatement of the reference's
+// _cell_metrics, engine/neighborlist.py:170-267).
+struct CellInfo {
+  double perp[3];
+  double volume;
+  double min_width;
+};
+CellInfo cell_info(const double* lat);
+void invert3(const double* m, double* inv);
+
+// A set of structures laid out atom-wise on the device.
+struct SetDev {
+  int n_atoms = 0;
+  int n_struct = 0;
+  int n_cells = 0;
+  std::vector<Geom> geoms;
+  std::vector<int> atom_off;  // n_struct + 1
+  std::vector<int> uploaded_off;  // atom_off of the uploaded sid array
+  std::vector<Geom> uploaded_geoms;
+  DevBuf d_geoms, d_sid, d_cell_offs;
+  // (re)builds geometry for r_list; lattices (S, 9), invs may be null.
+  void setup(int n_struct, const int64_t* offsets, const double* lattices, const double* invs,
+             double r_list, hipStream_t st, bool check_min_image = true);
+  void upload_geoms(hipStream_t st);
+};
+
+// Full (two-direction) neighbour list in ELL layout: row a holds cnt[a]
+// neighbour indices sorted ascending, padded to `stride`; rev[a*stride+k]
+// is the slot of a in the row of its k-th neighbour.
+struct NList {
+  double r_cut = 0, skin = 0, r_list = 0;
+  int stride = 0;   // ELL row stride
+  int maxcnt = 0;   // largest row
+  int cap = 0;      // search capacity per atom
+  int cmax = 0;     // candidate capacity per cell
+  int n_atoms = 0;
+  int n_builds = 0, n_checks = 0;
+  bool valid = false;
+  bool fixed_stride = false;  // graph mode: stride never changes
+  std::vector<Geom> built_geoms;
+  DevBuf cnt, nbr, rev, ref_pos;
+  DevBuf keys_a, vals_a, keys_b, vals_b, hist, cell_start, tmp_nbr, stats, half_off;
+  DevBuf disp_part, flag;
+};
+
+void nl_build(NList& nl, SetDev& set, const double* d_pos, hipStream_t st);
+// Device-gated rebuild for captured MD steps: runs the cell sort and search
+// only when *d_flag is nonzero; rows keep nl.stride.
+void nl_build_gated(NList& nl, SetDev& set, const double* d_pos, const int* d_flag,
+                    hipStream_t st);
+// Largest minimum-image displacement since the last build -> *d_flag = 1 when
+// sigma_min * r_list - 2 d_max <= r_cut (fixed cell: sigma_min = 1, M = I);
+// d_flag and d_dmax may be null.
+void nl_check_displacement(NList& nl, SetDev& set, const double* d_pos, const Mat3& M,
+                           double sigma_min, int* d_flag, double* d_dmax, hipStream_t st);
+int64_t nl_half_pairs(NList& nl, SetDev& set, hipStream_t st, int32_t* h_i, int32_t* h_j);
+int nl_stats_overflow(NList& nl, hipStream_t st);
+
+// Potentials ------------------------------------------------------------
+enum Kind { KIND_TERSOFF = 1, KIND_SW = 2, KIND_EAM = 3 };
+
+struct Potential {
+  int kind = 0;
+  int ns = 0;
+  double r_cut = 0;
+  std::vector<double> h_params;
+  DevBuf d_params;
+  // EAM function-table indices
+  std::vector<int> h_map;
+  DevBuf d_map;
+  // per-call scratch
+  DevBuf G, e_atom, w_atom, aux, out;
+  // launch shape chosen by the autotuner (0 = default)
+  int eam_lanes = 0;
+  int block = 256;
+  // fitting session (fit.hip)
+  std::shared_ptr<void> fit;
+};
+
+// Forces on all atoms of `set`, per-structure energy and stress.
+// d_out: (n_struct, 10) = [E, S00..S22]; d_F (n_atoms, 3).
+void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
+                    const int* d_spc, double* d_F, double* d_out, hipStream_t st);
+
+inline int pow2ceil(int x) {
+  int p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+}  // namespace ak

@@ -1,0 +1,686 @@
+// Many-body force kernels: Tersoff, Stillinger-Weber, Zhou EAM.
+//
+// Tersoff and SW use a site formulation.  The energy is split into site
+// energies E_a = E_a({r_aj}) that depend only on the bond vectors of atom a to
+// its neighbours (for Tersoff E_a = 1/2 sum_j fc [fR + b_aj fA]; the pair sum
+// of the reference, engine/tersoff.py:1399-1414, with b_avg = (b_ij + b_ji)/2,
+// is the same total).  One segment of L lanes handles one atom, one lane per
+// neighbour slot; the lanes exchange bond vectors by wavefront shuffles.  Each
+// lane writes G_aj = dE_a/dr_aj for its bond, and a second pass gathers
+//     F_a = sum_j (G_aj - G_ja),      sigma = -(1/V) sum_bonds G_aj (x) r_aj,
+// reading G_ja through the reverse-slot index of the neighbour list.  No
+// atomics, so results are bitwise reproducible run to run.
+//
+// EAM runs two per-atom passes over the full neighbour rows: density and
+// embedding derivative, then forces (after the reference's numba kernel,
+// engine/_eam_kernels.py:231-307).
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace ak {
+
+// ---------------------------------------------------------------------------
+// Scalar helpers
+// ---------------------------------------------------------------------------
+
+// Tersoff cutoff and its derivative (engine/tersoff.py:48-58, 127-135).
+__device__ __forceinline__ void te_fc(double r, double R, double D, double& fc, double& dfc) {
+  if (r < R - D) {
+    fc = 1.0;
+    dfc = 0.0;
+  } else if (r > R + D) {
+    fc = 0.0;
+    dfc = 0.0;
+  } else {
+    double s, c;
+    sincos(0.5 * M_PI * (r - R) / D, &s, &c);
+    fc = 0.5 - 0.5 * s;
+    dfc = -0.25 * M_PI / D * c;
+  }
+}
+
+// exp(l3^m du^m) and its derivative in du (the zeta-term factor of
+// engine/tersoff.py:161-202); m = 3 and m = 1 are multiplied out.
+__device__ __forceinline__ void te_exp(double l3, double m, double du, double& e, double& de) {
+  if (m == 3.0) {
+    if (l3 == 0.0) {
+      e = 1.0;
+      de = 0.0;
+      return;
+    }
+    double lm = l3 * l3 * l3, u2 = du * du;
+    e = nb_exp(lm * u2 * du);
+    de = e * 3.0 * lm * u2;
+  } else if (m == 1.0) {
+    e = nb_exp(l3 * du);
+    de = e * l3;
+  } else {
+    double lm = pow(l3, m);
+    e = nb_exp(lm * pow(du, m));
+    de = e * m * lm * pow(du, m - 1.0);
+  }
+}
+
+// x^e with small integer exponents multiplied out.
+__device__ __forceinline__ double xpow(double x, double e) {
+  if (e == 0.0) return 1.0;
+  if (e == 4.0) {
+    double x2 = x * x;
+    return x2 * x2;
+  }
+  if (e == -4.0) {
+    double x2 = x * x;
+    return 1.0 / (x2 * x2);
+  }
+  if (e == -5.0) {
+    double x2 = x * x;
+    return 1.0 / (x2 * x2 * x);
+  }
+  if (e == -1.0) return 1.0 / x;
+  if (e == 1.0) return x;
+  return pow(x, e);
+}
+
+// Minimum image with contraction allowed (force evaluation).
+__device__ __forceinline__ void mic_fast(const Geom& g, const double* __restrict__ pos, int i,
+                                         int j, double& dx, double& dy, double& dz) {
+  double rx = pos[3 * j] - pos[3 * i], ry = pos[3 * j + 1] - pos[3 * i + 1],
+         rz = pos[3 * j + 2] - pos[3 * i + 2];
+  double sx = rx * g.inv[0] + ry * g.inv[3] + rz * g.inv[6];
+  double sy = rx * g.inv[1] + ry * g.inv[4] + rz * g.inv[7];
+  double sz = rx * g.inv[2] + ry * g.inv[5] + rz * g.inv[8];
+  sx -= rint(sx);
+  sy -= rint(sy);
+  sz -= rint(sz);
+  dx = sx * g.lat[0] + sy * g.lat[3] + sz * g.lat[6];
+  dy = sx * g.lat[1] + sy * g.lat[4] + sz * g.lat[7];
+  dz = sx * g.lat[2] + sy * g.lat[5] + sz * g.lat[8];
+}
+
+// Segment-level sums of the site energy and virial; lane 0 of the segment
+// stores them.
+template <int L>
+__device__ __forceinline__ void store_site(bool first, bool live, int a, double e, double Gx,
+                                           double Gy, double Gz, double ux, double uy,
+                                           double uz, double* __restrict__ e_atom,
+                                           double* __restrict__ w_atom) {
+  double w[9] = {-Gx * ux, -Gx * uy, -Gx * uz, -Gy * ux, -Gy * uy,
+                 -Gy * uz, -Gz * ux, -Gz * uy, -Gz * uz};
+  e = seg_sum<L>(e);
+#pragma unroll
+  for (int q = 0; q < 9; ++q) w[q] = seg_sum<L>(w[q]);
+  if (first && live) {
+    e_atom[a] = e;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) w_atom[9 * (size_t)a + q] = w[q];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tersoff site kernel
+//   tb : (ns, ns, 9) [valid, A, B, lambda1, lambda2, R, D, beta, n]
+//   tri: (ns, ns, ns, 7) [valid, c, d, h, lambda3, m, gamma]
+// Pair terms use the record of (species of the lower-indexed atom, other),
+// bond orders the record of (centre, neighbour), angular terms the triple
+// (centre, bond neighbour, third atom) with the leg cutoff (centre, third):
+// the table semantics of engine/tersoff.py:528-855 (the flat single-species
+// form is the ns = 1 case with m = 3, gamma = 1).
+// ---------------------------------------------------------------------------
+template <int L>
+__global__ __launch_bounds__(256) void k_tersoff_site(
+    int n, const double* __restrict__ pos, const int* __restrict__ sid,
+    const Geom* __restrict__ geoms, const int* __restrict__ spc, int ns,
+    const double* __restrict__ tb, const double* __restrict__ tri, double rc,
+    const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
+    double* __restrict__ G, double* __restrict__ e_atom, double* __restrict__ w_atom) {
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = gt / L;
+  const int k = gt & (L - 1);
+  const int seg = (threadIdx.x & 63) & ~(L - 1);
+  const bool live = a < n;
+  const int ac = live ? a : 0;
+  const int nk = live ? cnt[ac] : 0;
+  const bool has = live && k < nk;
+  const size_t slot = (size_t)ac * stride + k;
+  const int j = has ? nbr[slot] : ac;
+  const int sa = ns > 1 ? spc[ac] : 0;
+  const int sj = ns > 1 ? spc[j] : 0;
+
+  double ux = 0.0, uy = 0.0, uz = 0.0, r = 0.0, rinv = 0.0;
+  bool act = false;
+  if (has) {
+    const Geom& g = geoms[sid[ac]];
+    mic_fast(g, pos, ac, j, ux, uy, uz);
+    r = sqrt(ux * ux + uy * uy + uz * uz);
+    act = r < rc && r > 1e-12;
+    rinv = act ? 1.0 / r : 0.0;
+  }
+  const int lo = ac < j ? sa : sj, hi = ac < j ? sj : sa;
+  const double* P2 = tb + ((size_t)lo * ns + hi) * 9;
+  const double* PL = tb + ((size_t)sa * ns + sj) * 9;
+  const bool pair_ok = act && P2[0] != 0.0;
+  // cutoff of this bond as the third leg of an angle centred on a
+  double fcl = 0.0, dfcl = 0.0;
+  if (act && PL[0] != 0.0) te_fc(r, PL[5], PL[6], fcl, dfcl);
+
+  // zeta of this bond: sum over the other legs of the segment
+  double zeta = 0.0;
+#pragma unroll 4
+  for (int s = 0; s < L; ++s) {
+    const int src = seg + s;
+    double wx = __shfl(ux, src, 64), wy = __shfl(uy, src, 64), wz = __shfl(uz, src, 64);
+    double wr = __shfl(r, src, 64), wri = __shfl(rinv, src, 64), wfc = __shfl(fcl, src, 64);
+    int wsp = ns > 1 ? __shfl(sj, src, 64) : 0;
+    if (s == k || !pair_ok || wfc == 0.0) continue;
+    const double* T = tri + (((size_t)sa * ns + sj) * ns + wsp) * 7;
+    if (T[0] == 0.0) continue;
+    double cs = (ux * wx + uy * wy + uz * wz) * rinv * wri;
+    double c = T[1], d = T[2], h = T[3];
+    double hc = h - cs;
+    double gt_ = 1.0 + c * c / (d * d) - c * c / (d * d + hc * hc);
+    double ex, dex;
+    te_exp(T[4], T[5], r - wr, ex, dex);
+    zeta += T[6] * wfc * gt_ * ex;
+  }
+
+  // bond order (beta, n of the record (a, j)) and the two-body term
+  double b = 1.0, db = 0.0;
+  if (pair_ok && zeta > 0.0 && PL[7] != 0.0) {
+    double bz = pow(PL[7] * zeta, PL[8]);
+    b = pow(1.0 + bz, -0.5 / PL[8]);
+    db = -0.5 * b * bz / (zeta * (1.0 + bz));
+  }
+  double e = 0.0, dVdr = 0.0, cown = 0.0;
+  if (pair_ok) {
+    double fc2, dfc2;
+    te_fc(r, P2[5], P2[6], fc2, dfc2);
+    if (fc2 != 0.0 || dfc2 != 0.0) {
+      double fr = P2[1] * nb_exp(-P2[3] * r);
+      double fa = -P2[2] * nb_exp(-P2[4] * r);
+      e = 0.5 * fc2 * (fr + b * fa);
+      dVdr = 0.5 * (dfc2 * (fr + b * fa) + fc2 * (-P2[3] * fr + b * (-P2[4] * fa)));
+      if (fabs(db) > 1e-30) cown = 0.5 * fc2 * fa * db;
+    }
+  }
+
+  // gradient of E_a with respect to this bond vector u:
+  //   own zeta (this bond's b depends on the angle to every other leg w) and
+  //   every other bond's zeta in which u is the third leg.
+  double su = dVdr * rinv, swx = 0.0, swy = 0.0, swz = 0.0;
+#pragma unroll 4
+  for (int s = 0; s < L; ++s) {
+    const int src = seg + s;
+    double wx = __shfl(ux, src, 64), wy = __shfl(uy, src, 64), wz = __shfl(uz, src, 64);
+    double wr = __shfl(r, src, 64), wri = __shfl(rinv, src, 64), wfc = __shfl(fcl, src, 64);
+    double wc = __shfl(cown, src, 64);
+    int wsp = ns > 1 ? __shfl(sj, src, 64) : 0;
+    if (s == k || !act || wri == 0.0) continue;
+    const bool termA = cown != 0.0 && wfc != 0.0;
+    const bool termB = wc != 0.0 && fcl != 0.0;
+    if (!termA && !termB) continue;
+    double cs = (ux * wx + uy * wy + uz * wz) * rinv * wri;
+    double alpha = 0.0, beta = 0.0;
+    if (termA) {
+      const double* T = tri + (((size_t)sa * ns + sj) * ns + wsp) * 7;
+      if (T[0] != 0.0) {
+        double c = T[1], d = T[2], hc = T[3] - cs;
+        double q = d * d + hc * hc;
+        double g = 1.0 + c * c / (d * d) - c * c / q;
+        double dg = -2.0 * c * c * hc / (q * q);
+        double ex, dex;
+        te_exp(T[4], T[5], r - wr, ex, dex);
+        alpha += cown * T[6] * wfc * dg * ex;
+        beta += cown * T[6] * wfc * g * dex;
+      }
+    }
+    if (termB) {
+      const double* T = tri + (((size_t)sa * ns + wsp) * ns + sj) * 7;
+      if (T[0] != 0.0) {
+        double c = T[1], d = T[2], hc = T[3] - cs;
+        double q = d * d + hc * hc;
+        double g = 1.0 + c * c / (d * d) - c * c / q;
+        double dg = -2.0 * c * c * hc / (q * q);
+        double ex, dex;
+        te_exp(T[4], T[5], wr - r, ex, dex);
+        alpha += wc * T[6] * fcl * dg * ex;
+        beta += wc * T[6] * (dfcl * g * ex - fcl * g * dex);
+      }
+    }
+    // d cos / du = w/(|u||w|) - cos u/|u|^2
+    su += beta * rinv - alpha * cs * rinv * rinv;
+    double aw = alpha * rinv * wri;
+    swx += aw * wx;
+    swy += aw * wy;
+    swz += aw * wz;
+  }
+  const double Gx = su * ux + swx, Gy = su * uy + swy, Gz = su * uz + swz;
+  if (has) {
+    const size_t plane = (size_t)n * stride;
+    G[slot] = Gx;
+    G[plane + slot] = Gy;
+    G[2 * plane + slot] = Gz;
+  }
+  store_site<L>(k == 0, live, ac, e, Gx, Gy, Gz, ux, uy, uz, e_atom, w_atom);
+}
+
+// ---------------------------------------------------------------------------
+// Stillinger-Weber site kernel (flat single-species form,
+// engine/stillinger_weber.py:23-255):
+//   E_a = 1/2 sum_j V2(r_aj) + 1/2 sum_{j != k} h(r_aj, r_ak)
+// prm = epsilon, sigma, A, B, p, q, a, lambda, gamma, costheta0.
+// ---------------------------------------------------------------------------
+template <int L>
+__global__ __launch_bounds__(256) void k_sw_site(
+    int n, const double* __restrict__ pos, const int* __restrict__ sid,
+    const Geom* __restrict__ geoms, const double* __restrict__ prm,
+    const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
+    double* __restrict__ G, double* __restrict__ e_atom, double* __restrict__ w_atom) {
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = gt / L;
+  const int k = gt & (L - 1);
+  const int seg = (threadIdx.x & 63) & ~(L - 1);
+  const bool live = a < n;
+  const int ac = live ? a : 0;
+  const int nk = live ? cnt[ac] : 0;
+  const bool has = live && k < nk;
+  const size_t slot = (size_t)ac * stride + k;
+  const int j = has ? nbr[slot] : ac;
+  const double eps = prm[0], sig = prm[1], Asw = prm[2], Bsw = prm[3], pp = prm[4],
+               qq = prm[5], asw = prm[6], lam = prm[7], gam = prm[8], cth0 = prm[9];
+  const double rc = asw * sig;
+
+  double ux = 0.0, uy = 0.0, uz = 0.0, r = 0.0, rinv = 0.0;
+  bool act = false;
+  if (has) {
+    const Geom& g = geoms[sid[ac]];
+    mic_fast(g, pos, ac, j, ux, uy, uz);
+    r = sqrt(ux * ux + uy * uy + uz * uz);
+    act = r < rc && r > 1e-12;
+  }
+  double x = act ? r / sig : 0.0;
+  bool leg = act && x < asw;  // leg of a three-body term
+  rinv = leg ? 1.0 / r : 0.0;
+  double E = 0.0, dlnE = 0.0;  // exp(gamma/(x - a)) and d ln / dr
+  double e = 0.0, su = 0.0;
+  if (leg) {
+    double dx_ = x - asw;
+    E = nb_exp(gam / dx_);
+    dlnE = -gam / (sig * dx_ * dx_);
+    double t = 1.0 / dx_;
+    double ex = nb_exp(t);
+    double bx = Bsw * xpow(x, -pp) - xpow(x, -qq);
+    double V2 = eps * Asw * bx * ex;
+    double dV2 = eps * Asw * ex *
+                 ((-pp * Bsw * xpow(x, -pp - 1.0) + qq * xpow(x, -qq - 1.0)) + bx * (-t * t));
+    e = 0.5 * V2;
+    su = 0.5 * (dV2 / sig) * rinv;
+  }
+  const double c3 = lam * eps;
+  double swx = 0.0, swy = 0.0, swz = 0.0;
+#pragma unroll 4
+  for (int s = 0; s < L; ++s) {
+    const int src = seg + s;
+    double wx = __shfl(ux, src, 64), wy = __shfl(uy, src, 64), wz = __shfl(uz, src, 64);
+    double wri = __shfl(rinv, src, 64), wE = __shfl(E, src, 64);
+    if (s == k || !leg || wri == 0.0) continue;
+    double cs = (ux * wx + uy * wy + uz * wz) * rinv * wri;
+    double dc = cs - cth0;
+    double pre = c3 * E * wE * dc;
+    e += 0.5 * pre * dc;
+    // dh/du = pre [dc dlnE u_hat + 2 (w_hat - cos u_hat) / r]
+    double rad = pre * dc * dlnE, ang = 2.0 * pre * rinv;
+    su += (rad - ang * cs) * rinv;
+    double aw = ang * wri;
+    swx += aw * wx;
+    swy += aw * wy;
+    swz += aw * wz;
+  }
+  const double Gx = su * ux + swx, Gy = su * uy + swy, Gz = su * uz + swz;
+  if (has) {
+    const size_t plane = (size_t)n * stride;
+    G[slot] = Gx;
+    G[plane + slot] = Gy;
+    G[2 * plane + slot] = Gz;
+  }
+  store_site<L>(k == 0, live, ac, e, Gx, Gy, Gz, ux, uy, uz, e_atom, w_atom);
+}
+
+// F_a = sum_k G[a, k] - G[rev(a, k)]
+template <int L>
+__global__ __launch_bounds__(256) void k_gather(int n, const int* __restrict__ cnt,
+                                                const int* __restrict__ rev, int stride,
+                                                const double* __restrict__ G,
+                                                double* __restrict__ F) {
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = gt / L;
+  const int k = gt & (L - 1);
+  const bool live = a < n;
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  if (live && k < cnt[a]) {
+    const size_t plane = (size_t)n * stride;
+    const size_t s = (size_t)a * stride + k;
+    const int r = rev[s];
+    if (r >= 0) {
+      const size_t rs = (size_t)r;
+      fx = G[s] - G[rs];
+      fy = G[plane + s] - G[plane + rs];
+      fz = G[2 * plane + s] - G[2 * plane + rs];
+    }
+  }
+  fx = seg_sum<L>(fx);
+  fy = seg_sum<L>(fy);
+  fz = seg_sum<L>(fz);
+  if (live && k == 0) {
+    F[3 * (size_t)a] = fx;
+    F[3 * (size_t)a + 1] = fy;
+    F[3 * (size_t)a + 2] = fz;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Zhou EAM.  A function is a row of 44 doubles [code, r_c, width, chi,
+// pa[20], pb[20]]: code 3 density, 4 pair, 5 cross pair, 6 embedding;
+// element parameters in the order r_e, f_e, rho_e, rho_s, alpha, beta, A, B,
+// kappa, lambda, Fn0..Fn3, F0..F3, eta, F_e (engine/_eam_kernels.py:129-223).
+// ---------------------------------------------------------------------------
+constexpr int ZW = 44;
+
+__device__ __forceinline__ void z_decay(double r, double re, double pre, double expo,
+                                        double shift, double& val, double& dval) {
+  double x = r / re;
+  double e = pre * exp(-expo * (x - 1.0));
+  double u = x - shift;
+  double u2 = u * u, u4 = u2 * u2, u8 = u4 * u4, u16 = u8 * u8;
+  double u19 = u16 * u2 * u;
+  double den = 1.0 + u19 * u;
+  double iden = 1.0 / den;
+  val = e * iden;
+  dval = (-expo * val - e * 20.0 * u19 * iden * iden) / re;
+}
+
+__device__ __forceinline__ void z_embed(const double* __restrict__ p, double rho, double& F,
+                                        double& dF) {
+  double re = p[2], rs = p[3], eta = p[18];
+  double rn = 0.85 * re;
+  if (rho < rn) {
+    double z = rho / rn - 1.0;
+    F = p[10] + p[11] * z + p[12] * z * z + p[13] * z * z * z;
+    dF = (p[11] + 2.0 * p[12] * z + 3.0 * p[13] * z * z) / rn;
+  } else if (rho >= 1.15 * re) {
+    double t = rho / rs, lt = log(t);
+    double te = pow(t, eta);
+    F = p[19] * (1.0 - eta * lt) * te;
+    dF = -p[19] * eta * eta * lt * (te / t) / rs;
+  } else {
+    double z = rho / re - 1.0;
+    F = p[14] + p[15] * z + p[16] * z * z + p[17] * z * z * z;
+    dF = (p[15] + 2.0 * p[16] * z + 3.0 * p[17] * z * z) / re;
+  }
+}
+
+__device__ __forceinline__ void z_feval(const double* __restrict__ f, double r, double& v,
+                                        double& dv) {
+  const int code = (int)f[0];
+  if (code == 6) {
+    z_embed(f + 4, r, v, dv);
+    return;
+  }
+  const double* pa = f + 4;
+  double val, dval;
+  if (code == 3) {
+    z_decay(r, pa[0], pa[1], pa[5], pa[9], val, dval);
+  } else if (code == 4) {
+    double A, dA, B, dB;
+    z_decay(r, pa[0], pa[6], pa[4], pa[8], A, dA);
+    z_decay(r, pa[0], pa[7], pa[5], pa[9], B, dB);
+    val = A - B;
+    dval = dA - dB;
+  } else {
+    const double* pb = f + 24;
+    double fa, dfa, fb, dfb, A, dA, B, dB;
+    z_decay(r, pa[0], pa[1], pa[5], pa[9], fa, dfa);
+    z_decay(r, pb[0], pb[1], pb[5], pb[9], fb, dfb);
+    z_decay(r, pa[0], pa[6], pa[4], pa[8], A, dA);
+    z_decay(r, pa[0], pa[7], pa[5], pa[9], B, dB);
+    double ua = A - B, dua = dA - dB;
+    z_decay(r, pb[0], pb[6], pb[4], pb[8], A, dA);
+    z_decay(r, pb[0], pb[7], pb[5], pb[9], B, dB);
+    double ub = A - B, dub = dA - dB;
+    double kc = f[3];
+    double q = fb / fa;
+    double dq = (dfb * fa - fb * dfa) / (fa * fa);
+    val = 0.5 * kc * (q * ua + ub / q);
+    dval = 0.5 * kc * (dq * ua + q * dua + dub / q - ub * dq / (q * q));
+  }
+  const double rc = f[1], w = f[2];
+  double s = (r - (rc - w)) / w;
+  s = s < 0.0 ? 0.0 : (s > 1.0 ? 1.0 : s);
+  double S = 1.0 - s * s * s * (10.0 - 15.0 * s + 6.0 * s * s);
+  double dS = -30.0 * s * s * (1.0 - s) * (1.0 - s) / w;
+  v = val * S;
+  dv = dval * S + val * dS;
+}
+
+// map: [rho_f[ns], emb_f[ns], phi_f[ns*ns]]
+template <int LE>
+__global__ __launch_bounds__(256) void k_eam_rho(
+    int n, const double* __restrict__ pos, const int* __restrict__ sid,
+    const Geom* __restrict__ geoms, const int* __restrict__ spc, int ns,
+    const double* __restrict__ fn, const int* __restrict__ map, double rc,
+    const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
+    double* __restrict__ dF, double* __restrict__ Fe) {
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = gt / LE;
+  const int l = gt & (LE - 1);
+  const bool live = a < n;
+  double rho = 0.0;
+  if (live) {
+    const Geom& g = geoms[sid[a]];
+    const int nk = cnt[a];
+    for (int k = l; k < nk; k += LE) {
+      const int j = nbr[(size_t)a * stride + k];
+      double ux, uy, uz;
+      mic_fast(g, pos, a, j, ux, uy, uz);
+      double r = sqrt(ux * ux + uy * uy + uz * uz);
+      if (r < rc && r > 1e-12) {
+        const int sj = ns > 1 ? spc[j] : 0;
+        double v, dv;
+        z_feval(fn + (size_t)ZW * map[sj], r, v, dv);
+        rho += v;
+      }
+    }
+  }
+  rho = seg_sum<LE>(rho);
+  if (live && l == 0) {
+    const int sa = ns > 1 ? spc[a] : 0;
+    double F, d;
+    z_feval(fn + (size_t)ZW * map[ns + sa], rho > 0.0 ? rho : 0.0, F, d);
+    Fe[a] = F;
+    dF[a] = d;
+  }
+}
+
+template <int LE>
+__global__ __launch_bounds__(256) void k_eam_force(
+    int n, const double* __restrict__ pos, const int* __restrict__ sid,
+    const Geom* __restrict__ geoms, const int* __restrict__ spc, int ns,
+    const double* __restrict__ fn, const int* __restrict__ map, double rc,
+    const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
+    const double* __restrict__ dF, const double* __restrict__ Fe, double* __restrict__ F,
+    double* __restrict__ e_atom, double* __restrict__ w_atom) {
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = gt / LE;
+  const int l = gt & (LE - 1);
+  const bool live = a < n;
+  double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
+  double w[6] = {0, 0, 0, 0, 0, 0};  // xx, xy, xz, yy, yz, zz
+  if (live) {
+    const Geom& g = geoms[sid[a]];
+    const int nk = cnt[a];
+    const int sa = ns > 1 ? spc[a] : 0;
+    const double dFa = dF[a];
+    const double* fa_rho = fn + (size_t)ZW * map[sa];
+    for (int k = l; k < nk; k += LE) {
+      const int j = nbr[(size_t)a * stride + k];
+      double ux, uy, uz;
+      mic_fast(g, pos, a, j, ux, uy, uz);
+      double r = sqrt(ux * ux + uy * uy + uz * uz);
+      if (r < rc && r > 1e-12) {
+        const int sj = ns > 1 ? spc[j] : 0;
+        double v, dfj, dfa, phi, dphi;
+        z_feval(fn + (size_t)ZW * map[sj], r, v, dfj);
+        if (sj == sa)
+          dfa = dfj;
+        else
+          z_feval(fa_rho, r, v, dfa);
+        z_feval(fn + (size_t)ZW * map[2 * ns + sa * ns + sj], r, phi, dphi);
+        double c = dphi + dFa * dfj + dF[j] * dfa;
+        double s = c / r;
+        fx += s * ux;
+        fy += s * uy;
+        fz += s * uz;
+        e += 0.5 * phi;
+        double hs = -0.5 * s;
+        w[0] += hs * ux * ux;
+        w[1] += hs * ux * uy;
+        w[2] += hs * ux * uz;
+        w[3] += hs * uy * uy;
+        w[4] += hs * uy * uz;
+        w[5] += hs * uz * uz;
+      }
+    }
+  }
+  fx = seg_sum<LE>(fx);
+  fy = seg_sum<LE>(fy);
+  fz = seg_sum<LE>(fz);
+  e = seg_sum<LE>(e);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) w[q] = seg_sum<LE>(w[q]);
+  if (live && l == 0) {
+    F[3 * (size_t)a] = fx;
+    F[3 * (size_t)a + 1] = fy;
+    F[3 * (size_t)a + 2] = fz;
+    e_atom[a] = Fe[a] + e;
+    double* W = w_atom + 9 * (size_t)a;
+    W[0] = w[0];
+    W[1] = w[1];
+    W[2] = w[2];
+    W[3] = w[1];
+    W[4] = w[3];
+    W[5] = w[4];
+    W[6] = w[2];
+    W[7] = w[4];
+    W[8] = w[5];
+  }
+}
+
+// Per-structure energy and stress: out[s] = [E, W/V (9)].
+__global__ __launch_bounds__(1024) void k_struct_reduce(const Geom* __restrict__ geoms,
+                                                        const double* __restrict__ e_atom,
+                                                        const double* __restrict__ w_atom,
+                                                        double* __restrict__ out) {
+  __shared__ double smem[10 * 16];
+  const Geom& g = geoms[blockIdx.x];
+  double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int a = g.atom_off + threadIdx.x; a < g.atom_off + g.natoms; a += blockDim.x) {
+    v[0] += e_atom[a];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) v[1 + q] += w_atom[9 * (size_t)a + q];
+  }
+  block_sum<10>(v, smem);
+  if (threadIdx.x == 0) {
+    double* o = out + 10 * (size_t)blockIdx.x;
+    o[0] = v[0];
+    for (int q = 0; q < 9; ++q) o[1 + q] = v[1 + q] / g.volume;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host dispatch
+// ---------------------------------------------------------------------------
+
+template <int L>
+static void launch_bond_kernels(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
+                                const int* d_spc, double* d_F, hipStream_t st) {
+  const int n = set.n_atoms;
+  const int64_t threads = (int64_t)n * L;
+  const int bs = 256;
+  const unsigned grid = (unsigned)((threads + bs - 1) / bs);
+  double* G = pot.G.as<double>();
+  if (pot.kind == KIND_TERSOFF) {
+    const double* tb = pot.d_params.as<double>();
+    const double* tri = tb + (size_t)pot.ns * pot.ns * 9;
+    hipLaunchKernelGGL(k_tersoff_site<L>, dim3(grid), dim3(bs), 0, st, n, d_pos,
+                       set.d_sid.as<int>(), set.d_geoms.as<Geom>(), d_spc, pot.ns, tb, tri,
+                       pot.r_cut, nl.cnt.as<int>(), nl.nbr.as<int>(), nl.stride, G,
+                       pot.e_atom.as<double>(), pot.w_atom.as<double>());
+  } else {
+    hipLaunchKernelGGL(k_sw_site<L>, dim3(grid), dim3(bs), 0, st, n, d_pos,
+                       set.d_sid.as<int>(), set.d_geoms.as<Geom>(), pot.d_params.as<double>(),
+                       nl.cnt.as<int>(), nl.nbr.as<int>(), nl.stride, G,
+                       pot.e_atom.as<double>(), pot.w_atom.as<double>());
+  }
+  hipLaunchKernelGGL(k_gather<L>, dim3(grid), dim3(bs), 0, st, n, nl.cnt.as<int>(),
+                     nl.rev.as<int>(), nl.stride, (const double*)G, d_F);
+}
+
+template <int LE>
+static void launch_eam(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
+                       const int* d_spc, double* d_F, hipStream_t st) {
+  const int n = set.n_atoms;
+  const int64_t threads = (int64_t)n * LE;
+  const int bs = 256;
+  const unsigned grid = (unsigned)((threads + bs - 1) / bs);
+  double* dF = pot.aux.as<double>();
+  double* Fe = dF + n;
+  hipLaunchKernelGGL(k_eam_rho<LE>, dim3(grid), dim3(bs), 0, st, n, d_pos,
+                     set.d_sid.as<int>(), set.d_geoms.as<Geom>(), d_spc, pot.ns,
+                     pot.d_params.as<double>(), pot.d_map.as<int>(), pot.r_cut,
+                     nl.cnt.as<int>(), nl.nbr.as<int>(), nl.stride, dF, Fe);
+  hipLaunchKernelGGL(k_eam_force<LE>, dim3(grid), dim3(bs), 0, st, n, d_pos,
+                     set.d_sid.as<int>(), set.d_geoms.as<Geom>(), d_spc, pot.ns,
+                     pot.d_params.as<double>(), pot.d_map.as<int>(), pot.r_cut,
+                     nl.cnt.as<int>(), nl.nbr.as<int>(), nl.stride, (const double*)dF,
+                     (const double*)Fe, d_F, pot.e_atom.as<double>(), pot.w_atom.as<double>());
+}
+
+void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
+                    const int* d_spc, double* d_F, double* d_out, hipStream_t st) {
+  const int n = set.n_atoms;
+  AK_REQUIRE(nl.valid, AK_ERR_STATE, "neighbour list has not been built");
+  AK_REQUIRE(nl.n_atoms == n, AK_ERR_STATE, "neighbour list was built for another size");
+  pot.e_atom.ensure(sizeof(double) * std::max(1, n));
+  pot.w_atom.ensure(sizeof(double) * 9 * (size_t)std::max(1, n));
+  if (n > 0) {
+    if (pot.kind == KIND_EAM) {
+      pot.aux.ensure(sizeof(double) * 2 * (size_t)n);
+      int le = pot.eam_lanes > 0 ? pot.eam_lanes : 16;
+      switch (le) {
+        case 4: launch_eam<4>(pot, nl, set, d_pos, d_spc, d_F, st); break;
+        case 8: launch_eam<8>(pot, nl, set, d_pos, d_spc, d_F, st); break;
+        case 32: launch_eam<32>(pot, nl, set, d_pos, d_spc, d_F, st); break;
+        case 64: launch_eam<64>(pot, nl, set, d_pos, d_spc, d_F, st); break;
+        default: launch_eam<16>(pot, nl, set, d_pos, d_spc, d_F, st); break;
+      }
+    } else {
+      AK_REQUIRE(nl.stride <= 64, AK_ERR_CAPACITY,
+                 "more than 64 neighbours within r_cut + skin for a bond-order kernel");
+      pot.G.ensure(sizeof(double) * 3 * (size_t)n * nl.stride);
+      switch (nl.stride) {
+        case 4: launch_bond_kernels<4>(pot, nl, set, d_pos, d_spc, d_F, st); break;
+        case 8: launch_bond_kernels<8>(pot, nl, set, d_pos, d_spc, d_F, st); break;
+        case 16: launch_bond_kernels<16>(pot, nl, set, d_pos, d_spc, d_F, st); break;
+        case 32: launch_bond_kernels<32>(pot, nl, set, d_pos, d_spc, d_F, st); break;
+        case 64: launch_bond_kernels<64>(pot, nl, set, d_pos, d_spc, d_F, st); break;
+        default: throw Error(AK_ERR_STATE, "unsupported neighbour stride");
+      }
+    }
+  }
+  hipLaunchKernelGGL(k_struct_reduce, dim3(set.n_struct), dim3(1024), 0, st,
+                     set.d_geoms.as<Geom>(), (const double*)pot.e_atom.as<double>(),
+                     (const double*)pot.w_atom.as<double>(), d_out);
+}
+
+}  // namespace ak

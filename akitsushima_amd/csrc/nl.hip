@@ -1,0 +1,743 @@
+// Cell-list neighbour list on the device (after the reference's
+// NeighbourList._build_cell_linked, engine/neighborlist.py:935-1024, and its
+// pair search _cell_linked_pairs_numpy :312-373).
+//
+// 1. k_cell_id: per atom, its cell in the structure's grid
+//    (nc_i = floor(d_i / r_list), fractional coordinates wrapped into [0, 1)).
+// 2. Stable LSD radix sort of (cell, atom) by cell with 8-bit digits, free of
+//    atomics: digit counts per wavefront come from ballot match masks, block
+//    histograms are scanned globally, and each item's rank is its position
+//    among equal digits earlier in its block.  Atoms stay in ascending order
+//    inside each cell.
+// 3. k_nl_search: one wavefront per cell.  The wave collects the candidates
+//    of the canonical stencil (per-axis offsets {0} / {0, 1} / {-1, 0, 1} for
+//    nc = 1 / 2 / >= 3, engine/neighborlist.py:270-309), sorts them by atom
+//    index (bitonic, in LDS), then tests every atom of the cell against them
+//    with the reference's exact predicate r2 < rl2 and r2 > 1e-20, compacting
+//    hits by ballot.  Rows therefore come out sorted by neighbour index, the
+//    canonical order of engine/neighborlist.py:1166-1181, with no per-row
+//    sort.
+// 4. k_nl_rev: reverse slot of every entry (binary search in the sorted row).
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace ak {
+
+// ---------------------------------------------------------------------------
+// Geometry (host)
+// ---------------------------------------------------------------------------
+
+void invert3(const double* m, double* inv) {
+  double a = m[0], b = m[1], c = m[2], d = m[3], e = m[4], f = m[5], g = m[6], h = m[7],
+         i = m[8];
+  double A = e * i - f * h, B = -(d * i - f * g), C = d * h - e * g;
+  double det = a * A + b * B + c * C;
+  AK_REQUIRE(det != 0.0 && std::isfinite(det), AK_ERR_ARG, "singular lattice");
+  double id = 1.0 / det;
+  inv[0] = A * id;
+  inv[1] = -(b * i - c * h) * id;
+  inv[2] = (b * f - c * e) * id;
+  inv[3] = B * id;
+  inv[4] = (a * i - c * g) * id;
+  inv[5] = -(a * f - c * d) * id;
+  inv[6] = C * id;
+  inv[7] = -(a * h - b * g) * id;
+  inv[8] = (a * e - b * d) * id;
+}
+
+CellInfo cell_info(const double* L) {
+  CellInfo ci;
+  double det = L[0] * (L[4] * L[8] - L[5] * L[7]) - L[1] * (L[3] * L[8] - L[5] * L[6]) +
+               L[2] * (L[3] * L[7] - L[4] * L[6]);
+  ci.volume = std::fabs(det);
+  double cn[3];
+  for (int ax = 0; ax < 3; ++ax) {
+    const double* u = L + 3 * ((ax + 1) % 3);
+    const double* v = L + 3 * ((ax + 2) % 3);
+    double x = u[1] * v[2] - u[2] * v[1], y = u[2] * v[0] - u[0] * v[2],
+           z = u[0] * v[1] - u[1] * v[0];
+    cn[ax] = std::sqrt(x * x + y * y + z * z);
+  }
+  AK_REQUIRE(ci.volume > 0.0 && cn[0] > 0.0 && cn[1] > 0.0 && cn[2] > 0.0, AK_ERR_ARG,
+             "degenerate (singular) lattice: the three lattice vectors must be linearly "
+             "independent");
+  for (int ax = 0; ax < 3; ++ax) ci.perp[ax] = ci.volume / cn[ax];
+  ci.min_width = std::min(ci.perp[0], std::min(ci.perp[1], ci.perp[2]));
+  return ci;
+}
+
+void SetDev::setup(int S, const int64_t* offsets, const double* lattices, const double* invs,
+                   double r_list, hipStream_t st, bool check_min_image) {
+  AK_REQUIRE(S >= 1, AK_ERR_ARG, "need at least one structure");
+  n_struct = S;
+  geoms.assign(S, Geom{});
+  atom_off.assign(S + 1, 0);
+  int cells = 0;
+  for (int s = 0; s < S; ++s) {
+    Geom& g = geoms[s];
+    std::memcpy(g.lat, lattices + 9 * s, 9 * sizeof(double));
+    if (invs)
+      std::memcpy(g.inv, invs + 9 * s, 9 * sizeof(double));
+    else
+      invert3(g.lat, g.inv);
+    CellInfo ci = cell_info(g.lat);
+    g.volume = ci.volume;
+    if (check_min_image && r_list >= ci.min_width / 2.0) {
+      char buf[512];
+      snprintf(buf, sizeof buf,
+               "structure %d: r_cut + skin_width (%.4f A) must be < half the smallest "
+               "perpendicular cell width (%.4f A) for the minimum-image convention to be "
+               "valid. Perpendicular widths are [%.4f, %.4f, %.4f] A",
+               s, r_list, ci.min_width / 2.0, ci.perp[0], ci.perp[1], ci.perp[2]);
+      throw Error(AK_ERR_MIN_IMAGE, buf);
+    }
+    for (int ax = 0; ax < 3; ++ax) {
+      int nc = (int)std::floor(ci.perp[ax] / r_list);
+      g.nc[ax] = nc < 1 ? 1 : nc;
+    }
+    g.cell_off = cells;
+    long long nc_tot = (long long)g.nc[0] * g.nc[1] * g.nc[2];
+    AK_REQUIRE(cells + nc_tot < (1LL << 30), AK_ERR_CAPACITY, "too many cells");
+    cells += (int)nc_tot;
+    AK_REQUIRE(offsets[s + 1] >= offsets[s], AK_ERR_ARG, "atom offsets must not decrease");
+    g.atom_off = (int)offsets[s];
+    g.natoms = (int)(offsets[s + 1] - offsets[s]);
+    atom_off[s] = (int)offsets[s];
+  }
+  atom_off[S] = (int)offsets[S];
+  AK_REQUIRE(offsets[0] == 0, AK_ERR_ARG, "atom offsets must start at 0");
+  n_atoms = (int)offsets[S];
+  n_cells = cells;
+  upload_geoms(st);
+  if (uploaded_off != atom_off || d_sid.p == nullptr) {
+    uploaded_off = atom_off;
+    std::vector<int> sid(n_atoms > 0 ? n_atoms : 1, 0);
+    for (int s = 0; s < S; ++s)
+      for (int a = atom_off[s]; a < atom_off[s + 1]; ++a) sid[a] = s;
+    d_sid.ensure(sizeof(int) * sid.size());
+    AK_HIP(hipMemcpyAsync(d_sid.p, sid.data(), sizeof(int) * sid.size(),
+                          hipMemcpyHostToDevice, st));
+    AK_HIP(hipStreamSynchronize(st));
+  }
+}
+
+void SetDev::upload_geoms(hipStream_t st) {
+  if (uploaded_geoms.size() == geoms.size() && d_geoms.p &&
+      std::memcmp(uploaded_geoms.data(), geoms.data(), sizeof(Geom) * geoms.size()) == 0)
+    return;
+  uploaded_geoms = geoms;
+  d_geoms.ensure(sizeof(Geom) * geoms.size());
+  AK_HIP(hipMemcpyAsync(d_geoms.p, geoms.data(), sizeof(Geom) * geoms.size(),
+                        hipMemcpyHostToDevice, st));
+  std::vector<int> co(n_struct + 1);
+  for (int s = 0; s < n_struct; ++s) co[s] = geoms[s].cell_off;
+  co[n_struct] = n_cells;
+  d_cell_offs.ensure(sizeof(int) * co.size());
+  AK_HIP(hipMemcpyAsync(d_cell_offs.p, co.data(), sizeof(int) * co.size(),
+                        hipMemcpyHostToDevice, st));
+  // the host vectors are copied by the time the stream reaches them only if
+  // they outlive the call; synchronise to keep it simple
+  AK_HIP(hipStreamSynchronize(st));
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ bool gated_off(const int* gate) { return gate && *gate == 0; }
+
+__global__ void k_cell_id(int n, const double* __restrict__ pos, const int* __restrict__ sid,
+                          const Geom* __restrict__ geoms, unsigned* __restrict__ key,
+                          unsigned* __restrict__ val, const int* gate) {
+  if (gated_off(gate)) return;
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  const Geom& g = geoms[sid[a]];
+  double p0 = pos[3 * a], p1 = pos[3 * a + 1], p2 = pos[3 * a + 2];
+  int c[3];
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    double f = p0 * g.inv[b] + p1 * g.inv[3 + b] + p2 * g.inv[6 + b];
+    f = f - floor(f);
+    double t = floor(f * g.nc[b]);
+    int k = isfinite(t) ? (int)t : 0;
+    k = k < 0 ? 0 : (k > g.nc[b] - 1 ? g.nc[b] - 1 : k);
+    c[b] = k;
+  }
+  key[a] = (unsigned)(g.cell_off + (c[0] * g.nc[1] + c[1]) * g.nc[2] + c[2]);
+  val[a] = (unsigned)a;
+}
+
+// Lanes of this wavefront holding the same 8-bit digit as the calling lane.
+__device__ __forceinline__ uint64_t match_digit(unsigned d, bool valid) {
+  uint64_t m = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    int bit = (d >> b) & 1;
+    uint64_t bb = __ballot(bit);
+    m &= bit ? bb : ~bb;
+  }
+  return m;
+}
+
+constexpr int RS_T = 256;
+
+__global__ __launch_bounds__(RS_T) void k_rs_hist(int n, const unsigned* __restrict__ keys,
+                                                  int shift, int chunk,
+                                                  unsigned* __restrict__ hist, int B,
+                                                  const int* gate) {
+  if (gated_off(gate)) return;
+  __shared__ unsigned cnt[RS_T / 64][256];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  for (int i = tid; i < (RS_T / 64) * 256; i += RS_T) (&cnt[0][0])[i] = 0;
+  __syncthreads();
+  const int base = blockIdx.x * chunk;
+  for (int t = 0; t < chunk; t += RS_T) {
+    int idx = base + t + tid;
+    bool v = idx < n;
+    unsigned d = v ? (keys[idx] >> shift) & 255u : 0u;
+    uint64_t m = match_digit(d, v);
+    if (v && lane == __ffsll((unsigned long long)m) - 1) cnt[wave][d] += __popcll(m);
+  }
+  __syncthreads();
+  for (int d = tid; d < 256; d += RS_T) {
+    unsigned s = 0;
+    for (int w = 0; w < RS_T / 64; ++w) s += cnt[w][d];
+    hist[(size_t)d * B + blockIdx.x] = s;
+  }
+}
+
+// In-place exclusive scan of m unsigned values by one block of 1024 threads;
+// *total (if given) receives the sum.
+__global__ __launch_bounds__(1024) void k_scan_excl(unsigned* __restrict__ data, int m,
+                                                    unsigned* total, const int* gate) {
+  if (gated_off(gate)) return;
+  __shared__ unsigned part[1024];
+  const int tid = threadIdx.x;
+  const int per = (m + 1023) / 1024;
+  const int lo = tid * per, hi = min(m, lo + per);
+  unsigned s = 0;
+  for (int i = lo; i < hi; ++i) s += data[i];
+  part[tid] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    unsigned v = tid >= o ? part[tid - o] : 0u;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  unsigned run = part[tid] - s;  // exclusive prefix of this thread's chunk
+  for (int i = lo; i < hi; ++i) {
+    unsigned x = data[i];
+    data[i] = run;
+    run += x;
+  }
+  if (total && tid == 1023) *total = part[1023];
+}
+
+__global__ __launch_bounds__(RS_T) void k_rs_scatter(int n, const unsigned* __restrict__ kin,
+                                                     const unsigned* __restrict__ vin,
+                                                     unsigned* __restrict__ kout,
+                                                     unsigned* __restrict__ vout, int shift,
+                                                     int chunk, const unsigned* __restrict__ off,
+                                                     int B, const int* gate) {
+  if (gated_off(gate)) return;
+  __shared__ unsigned run[256];
+  __shared__ unsigned wcnt[RS_T / 64][256];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  for (int d = tid; d < 256; d += RS_T) run[d] = 0;
+  const int base = blockIdx.x * chunk;
+  for (int t = 0; t < chunk; t += RS_T) {
+    for (int i = tid; i < (RS_T / 64) * 256; i += RS_T) (&wcnt[0][0])[i] = 0;
+    __syncthreads();
+    int idx = base + t + tid;
+    bool v = idx < n;
+    unsigned key = v ? kin[idx] : 0u;
+    unsigned d = (key >> shift) & 255u;
+    uint64_t m = match_digit(d, v);
+    int rank = __popcll(m & lanemask_lt());
+    if (v && lane == __ffsll((unsigned long long)m) - 1) wcnt[wave][d] = __popcll(m);
+    __syncthreads();
+    if (v) {
+      unsigned p = off[(size_t)d * B + blockIdx.x] + run[d] + rank;
+      for (int w = 0; w < wave; ++w) p += wcnt[w][d];
+      kout[p] = key;
+      vout[p] = vin[idx];
+    }
+    __syncthreads();
+    for (int dd = tid; dd < 256; dd += RS_T) {
+      unsigned s = 0;
+      for (int w = 0; w < RS_T / 64; ++w) s += wcnt[w][dd];
+      run[dd] += s;
+    }
+    __syncthreads();
+  }
+}
+
+// cell_start[c] = first sorted position with key >= c, for c in [0, C].
+__global__ void k_cell_start(int n, int C, const unsigned* __restrict__ skey,
+                             int* __restrict__ cell_start, const int* gate) {
+  if (gated_off(gate)) return;
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n == 0) {
+    for (int c = idx; c <= C; c += gridDim.x * blockDim.x) cell_start[c] = 0;
+    return;
+  }
+  if (idx >= n) return;
+  int k = (int)skey[idx];
+  int prev = idx > 0 ? (int)skey[idx - 1] : -1;
+  for (int c = prev + 1; c <= k; ++c) cell_start[c] = idx;
+  if (idx == n - 1)
+    for (int c = k + 1; c <= C; ++c) cell_start[c] = n;
+}
+
+__device__ __forceinline__ int find_struct(const int* __restrict__ offs, int S, int c) {
+  int lo = 0, hi = S;  // offs[lo] <= c < offs[hi]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (offs[mid] <= c)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// One wavefront (one 64-thread block) per cell.
+__global__ __launch_bounds__(64) void k_nl_search(
+    int C, int S, const int* __restrict__ cell_offs, const Geom* __restrict__ geoms,
+    const int* __restrict__ cell_start, const unsigned* __restrict__ cell_atoms,
+    const double* __restrict__ pos, double rl2, int cap, int cmax, int* __restrict__ cnt,
+    int* __restrict__ nbr, int* __restrict__ stats, const int* gate) {
+  if (gated_off(gate)) return;
+  extern __shared__ unsigned cand[];  // cmax entries (power of two)
+  __shared__ int spre[28];
+  __shared__ int scell[27];
+  const int c = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int s = find_struct(cell_offs, S, c);
+  const Geom& g = geoms[s];
+  const int nc0 = g.nc[0], nc1 = g.nc[1], nc2 = g.nc[2];
+  const int l = c - g.cell_off;
+  const int c0 = l / (nc1 * nc2), c1 = (l / nc2) % nc1, c2 = l % nc2;
+  const int n0 = nc0 == 1 ? 1 : (nc0 == 2 ? 2 : 3);
+  const int n1 = nc1 == 1 ? 1 : (nc1 == 2 ? 2 : 3);
+  const int n2 = nc2 == 1 ? 1 : (nc2 == 2 ? 2 : 3);
+  const int nst = n0 * n1 * n2;
+  int size = 0, cid = 0;
+  if (lane < nst) {
+    int a0 = lane / (n1 * n2), a1 = (lane / n2) % n1, a2 = lane % n2;
+    int o0 = n0 == 3 ? a0 - 1 : a0, o1 = n1 == 3 ? a1 - 1 : a1, o2 = n2 == 3 ? a2 - 1 : a2;
+    int j0 = (c0 + o0 + nc0) % nc0, j1 = (c1 + o1 + nc1) % nc1, j2 = (c2 + o2 + nc2) % nc2;
+    cid = g.cell_off + (j0 * nc1 + j1) * nc2 + j2;
+    size = cell_start[cid + 1] - cell_start[cid];
+  }
+  // inclusive wave prefix of the stencil cell sizes
+  int inc = size;
+#pragma unroll
+  for (int o = 1; o < 32; o <<= 1) {
+    int y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += y;
+  }
+  const int ncand = __shfl(inc, 31, 64);
+  if (lane < nst) {
+    spre[lane + 1] = inc;
+    scell[lane] = cid;
+  }
+  if (lane == 0) spre[0] = 0;
+  const int first = cell_start[c], last = cell_start[c + 1];
+  if (ncand > cmax) {
+    if (lane == 0) stats[1] = max(stats[1], ncand);  // candidate overflow
+    for (int t = first + lane; t < last; t += 64) cnt[cell_atoms[t]] = 0;
+    return;
+  }
+  __syncthreads();
+  int npow = 1;
+  while (npow < ncand) npow <<= 1;
+  for (int q = lane; q < npow; q += 64) {
+    unsigned v = 0xFFFFFFFFu;
+    if (q < ncand) {
+      int st = 0;
+      while (spre[st + 1] <= q) ++st;
+      int cc = scell[st];
+      v = cell_atoms[cell_start[cc] + (q - spre[st])];
+    }
+    cand[q] = v;
+  }
+  __syncthreads();
+  // bitonic sort of cand[0, npow) ascending
+  for (int k = 2; k <= npow; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int e = lane; e < npow; e += 64) {
+        int p = e ^ j;
+        if (p > e) {
+          unsigned x = cand[e], y = cand[p];
+          bool up = (e & k) == 0;
+          if ((x > y) == up) {
+            cand[e] = y;
+            cand[p] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int t = first; t < last; ++t) {
+    const int i = (int)cell_atoms[t];
+    const double pix = pos[3 * i], piy = pos[3 * i + 1], piz = pos[3 * i + 2];
+    int count = 0;
+    for (int q0 = 0; q0 < ncand; q0 += 64) {
+      int q = q0 + lane;
+      bool hit = false;
+      int j = -1;
+      if (q < ncand) {
+        j = (int)cand[q];
+        double dx, dy, dz;
+        mic_exact(g.lat, g.inv, pix, piy, piz, pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], dx,
+                  dy, dz);
+        double r2 = r2_exact(dx, dy, dz);
+        hit = r2 < rl2 && r2 > 1e-20;
+      }
+      uint64_t m = __ballot(hit);
+      if (hit) {
+        int slot = count + __popcll(m & lanemask_lt());
+        if (slot < cap) nbr[(size_t)i * cap + slot] = j;
+      }
+      count += __popcll(m);
+    }
+    if (lane == 0) {
+      cnt[i] = count;
+      if (count > cap) stats[2] = max(stats[2], count);  // row overflow
+    }
+  }
+}
+
+// stats[0] = max row length over atoms (one block).
+__global__ __launch_bounds__(1024) void k_nl_maxcnt(int n, const int* __restrict__ cnt,
+                                                    int* __restrict__ stats, const int* gate) {
+  if (gated_off(gate)) return;
+  __shared__ int m[1024];
+  int v = 0;
+  for (int a = threadIdx.x; a < n; a += 1024) v = max(v, cnt[a]);
+  m[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) m[threadIdx.x] = max(m[threadIdx.x], m[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) stats[0] = max(stats[0], m[0]);
+}
+
+__global__ void k_nl_repack(int n, const int* __restrict__ cnt, const int* __restrict__ src,
+                            int cap, int* __restrict__ dst, int stride) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * stride) return;
+  int a = (int)(t / stride), k = (int)(t % stride);
+  dst[t] = k < cnt[a] ? src[(size_t)a * cap + k] : -1;
+}
+
+__global__ void k_nl_rev(int n, const int* __restrict__ cnt, const int* __restrict__ nbr,
+                         int stride, int* __restrict__ rev, int* stats, const int* gate) {
+  if (gated_off(gate)) return;
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * stride) return;
+  int a = (int)(t / stride), k = (int)(t % stride);
+  if (k >= cnt[a]) {
+    rev[t] = -1;
+    return;
+  }
+  int j = nbr[t];
+  const int* row = nbr + (size_t)j * stride;
+  int lo = 0, hi = cnt[j];
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (row[mid] < a)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  if (lo < cnt[j] && row[lo] == a) {
+    rev[t] = j * stride + lo;
+  } else {
+    rev[t] = -1;
+    stats[3] = 1;  // asymmetric list: should never happen
+  }
+}
+
+__global__ void k_copy_pos(int n, const double* __restrict__ src, double* __restrict__ dst,
+                           const int* gate) {
+  if (gated_off(gate)) return;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < 3 * n) dst[t] = src[t];
+}
+
+// Per-block max of |MIC(pos - ref @ M)|^2: the non-affine displacement since
+// the last build, M mapping the build cell onto the current one
+// (engine/neighborlist.py:662-669).
+__global__ __launch_bounds__(256) void k_disp_max(int n, const double* __restrict__ pos,
+                                                  const double* __restrict__ ref,
+                                                  const int* __restrict__ sid,
+                                                  const Geom* __restrict__ geoms, Mat3 M,
+                                                  double* __restrict__ part) {
+  __shared__ double m[256];
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (a < n) {
+    const Geom& g = geoms[sid[a]];
+    double r0 = ref[3 * a], r1 = ref[3 * a + 1], r2 = ref[3 * a + 2];
+    double q0 = r0 * M.m[0] + r1 * M.m[3] + r2 * M.m[6];
+    double q1 = r0 * M.m[1] + r1 * M.m[4] + r2 * M.m[7];
+    double q2 = r0 * M.m[2] + r1 * M.m[5] + r2 * M.m[8];
+    double dx, dy, dz;
+    mic_exact(g.lat, g.inv, q0, q1, q2, pos[3 * a], pos[3 * a + 1], pos[3 * a + 2], dx, dy,
+              dz);
+    v = dx * dx + dy * dy + dz * dz;
+  }
+  m[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) m[threadIdx.x] = fmax(m[threadIdx.x], m[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = m[0];
+}
+
+__global__ __launch_bounds__(256) void k_disp_flag(int nb, const double* __restrict__ part,
+                                                   double sigma_min, double r_list,
+                                                   double r_cut, int* __restrict__ flag,
+                                                   double* __restrict__ dmax_out) {
+  __shared__ double m[256];
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) v = fmax(v, part[b]);
+  m[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) m[threadIdx.x] = fmax(m[threadIdx.x], m[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double dmax = sqrt(m[0]);
+    if (flag) *flag = (sigma_min * r_list - 2.0 * dmax <= r_cut) ? 1 : 0;
+    if (dmax_out) *dmax_out = dmax;
+  }
+}
+
+__global__ void k_half_count(int n, const int* __restrict__ cnt, const int* __restrict__ nbr,
+                             int stride, unsigned* __restrict__ hc) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  int c = 0;
+  for (int k = 0; k < cnt[a]; ++k) c += nbr[(size_t)a * stride + k] > a;
+  hc[a] = (unsigned)c;
+}
+
+__global__ void k_half_write(int n, const int* __restrict__ cnt, const int* __restrict__ nbr,
+                             int stride, const unsigned* __restrict__ off,
+                             int32_t* __restrict__ oi, int32_t* __restrict__ oj) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  unsigned p = off[a];
+  for (int k = 0; k < cnt[a]; ++k) {
+    int j = nbr[(size_t)a * stride + k];
+    if (j > a) {
+      oi[p] = a;
+      oj[p] = j;
+      ++p;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host drivers
+// ---------------------------------------------------------------------------
+
+static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+static void sort_cells(NList& nl, SetDev& set, const double* d_pos, hipStream_t st,
+                       const int* gate) {
+  const int n = set.n_atoms;
+  const int C = set.n_cells;
+  nl.keys_a.ensure(sizeof(unsigned) * (n + 1));
+  nl.vals_a.ensure(sizeof(unsigned) * (n + 1));
+  nl.keys_b.ensure(sizeof(unsigned) * (n + 1));
+  nl.vals_b.ensure(sizeof(unsigned) * (n + 1));
+  nl.cell_start.ensure(sizeof(int) * (C + 2));
+  if (n > 0)
+    hipLaunchKernelGGL(k_cell_id, dim3(nblk(n, 256)), dim3(256), 0, st, n, d_pos,
+                       set.d_sid.as<int>(), set.d_geoms.as<Geom>(), nl.keys_a.as<unsigned>(),
+                       nl.vals_a.as<unsigned>(), gate);
+  int bits = 0;
+  while ((1LL << bits) < (long long)C) ++bits;
+  int passes = (bits + 7) / 8;
+  int tpb = std::max(1, (n + 256 * 512 - 1) / (256 * 512));
+  int chunk = 256 * tpb;
+  int B = std::max(1, (n + chunk - 1) / chunk);
+  nl.hist.ensure(sizeof(unsigned) * 256 * (size_t)B);
+  unsigned *ka = nl.keys_a.as<unsigned>(), *va = nl.vals_a.as<unsigned>();
+  unsigned *kb = nl.keys_b.as<unsigned>(), *vb = nl.vals_b.as<unsigned>();
+  for (int p = 0; p < passes && n > 0; ++p) {
+    int shift = 8 * p;
+    hipLaunchKernelGGL(k_rs_hist, dim3(B), dim3(RS_T), 0, st, n, ka, shift, chunk,
+                       nl.hist.as<unsigned>(), B, gate);
+    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, nl.hist.as<unsigned>(),
+                       256 * B, (unsigned*)nullptr, gate);
+    hipLaunchKernelGGL(k_rs_scatter, dim3(B), dim3(RS_T), 0, st, n, ka, va, kb, vb, shift,
+                       chunk, nl.hist.as<unsigned>(), B, gate);
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  // sorted keys/values end in ka/va; remember which buffer holds them
+  if (ka != nl.keys_a.as<unsigned>()) {
+    std::swap(nl.keys_a.p, nl.keys_b.p);
+    std::swap(nl.keys_a.bytes, nl.keys_b.bytes);
+    std::swap(nl.vals_a.p, nl.vals_b.p);
+    std::swap(nl.vals_a.bytes, nl.vals_b.bytes);
+  }
+  hipLaunchKernelGGL(k_cell_start, dim3(std::max(1u, nblk(n, 256))), dim3(256), 0, st, n, C,
+                     nl.keys_a.as<unsigned>(), nl.cell_start.as<int>(), gate);
+}
+
+static void launch_search(NList& nl, SetDev& set, const double* d_pos, int* d_cnt, int* d_nbr,
+                          int cap, hipStream_t st, const int* gate) {
+  const int C = set.n_cells;
+  size_t lds = sizeof(unsigned) * (size_t)nl.cmax;
+  hipLaunchKernelGGL(k_nl_search, dim3(C), dim3(64), lds, st, C, set.n_struct,
+                     set.d_cell_offs.as<int>(), set.d_geoms.as<Geom>(), nl.cell_start.as<int>(),
+                     nl.vals_a.as<unsigned>(), d_pos, nl.r_list * nl.r_list, cap, nl.cmax,
+                     d_cnt, d_nbr, nl.stats.as<int>(), gate);
+}
+
+static int est_cap(SetDev& set, double r_list) {
+  double dens = 0.0;
+  for (auto& g : set.geoms) dens = std::max(dens, g.natoms / g.volume);
+  double est = dens * 4.18879020478639 * r_list * r_list * r_list;
+  return pow2ceil(std::max(16, (int)(2.0 * est) + 16));
+}
+
+void nl_build(NList& nl, SetDev& set, const double* d_pos, hipStream_t st) {
+  const int n = set.n_atoms;
+  nl.n_atoms = n;
+  nl.stats.ensure(sizeof(int) * 8);
+  sort_cells(nl, set, d_pos, st, nullptr);
+  nl.cnt.ensure(sizeof(int) * (n + 1));
+  if (nl.cap == 0) nl.cap = est_cap(set, nl.r_list);
+  if (nl.cmax == 0) nl.cmax = 2048;
+  int h_stats[8];
+  for (int attempt = 0; attempt < 8; ++attempt) {
+    AK_HIP(hipMemsetAsync(nl.stats.p, 0, sizeof(int) * 8, st));
+    nl.tmp_nbr.ensure(sizeof(int) * (size_t)std::max(1, n) * nl.cap);
+    if (n > 0) {
+      launch_search(nl, set, d_pos, nl.cnt.as<int>(), nl.tmp_nbr.as<int>(), nl.cap, st,
+                    nullptr);
+      hipLaunchKernelGGL(k_nl_maxcnt, dim3(1), dim3(1024), 0, st, n, nl.cnt.as<int>(),
+                         nl.stats.as<int>(), (const int*)nullptr);
+    }
+    AK_HIP(hipMemcpyAsync(h_stats, nl.stats.p, sizeof(int) * 8, hipMemcpyDeviceToHost, st));
+    AK_HIP(hipStreamSynchronize(st));
+    bool again = false;
+    if (h_stats[1] > 0) {  // some cell's stencil held more candidates than cmax
+      nl.cmax = std::max(2 * nl.cmax, pow2ceil(h_stats[1]));
+      AK_REQUIRE(nl.cmax <= 8192, AK_ERR_CAPACITY,
+                 "more than 8192 candidate atoms around one cell: density too high for "
+                 "the cell list");
+      again = true;
+    }
+    if (h_stats[2] > 0) {  // rows longer than cap; counts are exact, use their max
+      nl.cap = pow2ceil(std::max(h_stats[0], h_stats[2]));
+      again = true;
+    }
+    if (!again) break;
+    AK_REQUIRE(attempt < 7, AK_ERR_CAPACITY, "neighbour list capacity search did not converge");
+  }
+  nl.maxcnt = h_stats[0];
+  int stride = std::max(4, pow2ceil(std::max(1, nl.maxcnt)));
+  if (nl.fixed_stride && nl.stride > 0) {
+    AK_REQUIRE(stride <= nl.stride, AK_ERR_CAPACITY,
+               "neighbour row exceeds the fixed stride of a captured list");
+    stride = nl.stride;
+  }
+  nl.stride = stride;
+  nl.nbr.ensure(sizeof(int) * (size_t)std::max(1, n) * stride);
+  nl.rev.ensure(sizeof(int) * (size_t)std::max(1, n) * stride);
+  if (n > 0) {
+    int64_t tot = (int64_t)n * stride;
+    hipLaunchKernelGGL(k_nl_repack, dim3(nblk(tot, 256)), dim3(256), 0, st, n, nl.cnt.as<int>(),
+                       nl.tmp_nbr.as<int>(), nl.cap, nl.nbr.as<int>(), stride);
+    hipLaunchKernelGGL(k_nl_rev, dim3(nblk(tot, 256)), dim3(256), 0, st, n, nl.cnt.as<int>(),
+                       nl.nbr.as<int>(), stride, nl.rev.as<int>(), nl.stats.as<int>(),
+                       (const int*)nullptr);
+  }
+  nl.ref_pos.ensure(sizeof(double) * 3 * (size_t)std::max(1, n));
+  if (n > 0)
+    AK_HIP(hipMemcpyAsync(nl.ref_pos.p, d_pos, sizeof(double) * 3 * n,
+                          hipMemcpyDeviceToDevice, st));
+  nl.built_geoms = set.geoms;
+  nl.valid = true;
+  nl.n_builds++;
+}
+
+void nl_build_gated(NList& nl, SetDev& set, const double* d_pos, const int* d_flag,
+                    hipStream_t st) {
+  const int n = set.n_atoms;
+  AK_REQUIRE(nl.valid && nl.fixed_stride && nl.stride > 0, AK_ERR_STATE,
+             "gated rebuild needs a list built with a fixed stride");
+  sort_cells(nl, set, d_pos, st, d_flag);
+  // search straight into the final layout (cap = stride)
+  launch_search(nl, set, d_pos, nl.cnt.as<int>(), nl.nbr.as<int>(), nl.stride, st, d_flag);
+  int64_t tot = (int64_t)n * nl.stride;
+  hipLaunchKernelGGL(k_nl_rev, dim3(nblk(tot, 256)), dim3(256), 0, st, n, nl.cnt.as<int>(),
+                     nl.nbr.as<int>(), nl.stride, nl.rev.as<int>(), nl.stats.as<int>(), d_flag);
+  hipLaunchKernelGGL(k_copy_pos, dim3(nblk(3 * (int64_t)n, 256)), dim3(256), 0, st, n, d_pos,
+                     nl.ref_pos.as<double>(), d_flag);
+}
+
+void nl_check_displacement(NList& nl, SetDev& set, const double* d_pos, const Mat3& M,
+                           double sigma_min, int* d_flag, double* d_dmax, hipStream_t st) {
+  const int n = set.n_atoms;
+  int nb = (int)nblk(std::max(1, n), 256);
+  nl.disp_part.ensure(sizeof(double) * nb);
+  hipLaunchKernelGGL(k_disp_max, dim3(nb), dim3(256), 0, st, n, d_pos, nl.ref_pos.as<double>(),
+                     set.d_sid.as<int>(), set.d_geoms.as<Geom>(), M, nl.disp_part.as<double>());
+  hipLaunchKernelGGL(k_disp_flag, dim3(1), dim3(256), 0, st, nb, nl.disp_part.as<double>(),
+                     sigma_min, nl.r_list, nl.r_cut, d_flag, d_dmax);
+}
+
+int nl_stats_overflow(NList& nl, hipStream_t st) {
+  int h[8];
+  AK_HIP(hipMemcpyAsync(h, nl.stats.p, sizeof(int) * 8, hipMemcpyDeviceToHost, st));
+  AK_HIP(hipStreamSynchronize(st));
+  return (h[1] > 0 || h[2] > 0 || h[3] > 0) ? 1 : 0;
+}
+
+int64_t nl_half_pairs(NList& nl, SetDev& set, hipStream_t st, int32_t* h_i, int32_t* h_j) {
+  const int n = set.n_atoms;
+  AK_REQUIRE(nl.valid, AK_ERR_STATE, "neighbour list has not been built");
+  if (n == 0) return 0;
+  nl.half_off.ensure(sizeof(unsigned) * (n + 1));
+  DevBuf total;
+  total.ensure(sizeof(unsigned));
+  hipLaunchKernelGGL(k_half_count, dim3(nblk(n, 256)), dim3(256), 0, st, n, nl.cnt.as<int>(),
+                     nl.nbr.as<int>(), nl.stride, nl.half_off.as<unsigned>());
+  hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, nl.half_off.as<unsigned>(), n,
+                     total.as<unsigned>(), (const int*)nullptr);
+  unsigned np = 0;
+  AK_HIP(hipMemcpyAsync(&np, total.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+  AK_HIP(hipStreamSynchronize(st));
+  if (h_i && np > 0) {
+    DevBuf di, dj;
+    di.ensure(sizeof(int32_t) * np);
+    dj.ensure(sizeof(int32_t) * np);
+    hipLaunchKernelGGL(k_half_write, dim3(nblk(n, 256)), dim3(256), 0, st, n, nl.cnt.as<int>(),
+                       nl.nbr.as<int>(), nl.stride, nl.half_off.as<unsigned>(),
+                       di.as<int32_t>(), dj.as<int32_t>());
+    AK_HIP(hipMemcpyAsync(h_i, di.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost, st));
+    AK_HIP(hipMemcpyAsync(h_j, dj.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost, st));
+    AK_HIP(hipStreamSynchronize(st));
+  }
+  return (int64_t)np;
+}
+
+}  // namespace ak

@@ -1,0 +1,122 @@
+"""Differentiable Tersoff fitting loss.
+
+``TersoffFit`` holds a batch of structures with reference energies (eV/atom),
+forces and stresses and evaluates the squared loss of fitting/loss.py:212-242
+(default weights w_E 1, w_F 1, w_S 0.1, r0 1, eps0 1, fitting/loss.py:184-188)
+for the 14 Tersoff parameters
+
+    theta = A, B, lambda1, lambda2, lambda3, beta, n, c, d, h, R, D, m, gamma
+
+together with dL/dtheta from the engine's hand-written backward kernel (the
+reference has no parameter gradient: its fits use derivative-free or
+finite-difference optimisers, fitting/driver.py:520-541).  All structures go
+through the device in one batched pass.  ``tersoff_loss(theta, fit)`` wraps
+it as a ``torch.autograd.Function``.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+from .potentials import TersoffPotential
+
+THETA_KEYS = ("A", "B", "lambda1", "lambda2", "lambda3", "beta", "n", "c", "d", "h",
+              "R", "D", "m", "gamma")
+DEFAULT_WEIGHTS = {"energy": 1.0, "forces": 1.0, "stress": 0.1, "r0": 1.0, "eps0": 1.0}
+
+
+def theta_from_params(params, m=3.0, gamma=1.0):
+    p = dict(params)
+    p.setdefault("m", m)
+    p.setdefault("gamma", gamma)
+    return np.array([float(p[k]) for k in THETA_KEYS])
+
+
+def params_from_theta(theta):
+    return {k: float(v) for k, v in zip(THETA_KEYS, theta)}
+
+
+class TersoffFit:
+    """structures: list of dicts with lattice, positions, ref_energy (eV per
+    atom), ref_forces (N, 3), ref_stress (3, 3)."""
+
+    def __init__(self, structures, weights=None):
+        w = dict(DEFAULT_WEIGHTS)
+        w.update(weights or {})
+        self.weights = np.array([w["energy"], w["forces"], w["stress"], w["r0"], w["eps0"]])
+        S = len(structures)
+        self.S = S
+        self.off = np.zeros(S + 1, np.int64)
+        for s, st in enumerate(structures):
+            self.off[s + 1] = self.off[s] + len(st["positions"])
+        self.lat = np.ascontiguousarray(np.stack([np.asarray(st["lattice"], float)
+                                                  for st in structures]))
+        self.pos = np.ascontiguousarray(np.concatenate([np.asarray(st["positions"], float)
+                                                        for st in structures]))
+        self.ref_e = np.ascontiguousarray([float(st["ref_energy"]) for st in structures])
+        self.ref_f = np.ascontiguousarray(np.concatenate([np.asarray(st["ref_forces"], float)
+                                                          for st in structures]))
+        self.ref_s = np.ascontiguousarray(np.stack([np.asarray(st["ref_stress"], float)
+                                                    for st in structures]))
+        self._pot = TersoffPotential({"A": 1.0, "B": 1.0, "lambda1": 1.0, "lambda2": 1.0,
+                                      "lambda3": 0.0, "beta": 1.0, "n": 1.0, "c": 1.0,
+                                      "d": 1.0, "h": 0.0, "R": 2.85, "D": 0.15})
+
+    def evaluate(self, theta, want_grad=True, want_outputs=False):
+        """(loss, grad (14,) or None[, (E (S,), F (sum N, 3), sigma (S, 3, 3))])."""
+        th = np.ascontiguousarray(theta, float)
+        if th.shape != (14,):
+            raise ValueError("theta must hold the 14 Tersoff parameters")
+        P = _lib._pf64
+        loss = ctypes.c_double(0.0)
+        grad = np.zeros(14) if want_grad else None
+        E = np.zeros(self.S) if want_outputs else None
+        F = np.zeros_like(self.pos) if want_outputs else None
+        S = np.zeros((self.S, 3, 3)) if want_outputs else None
+        ptr = lambda a: a.ctypes.data_as(P) if a is not None else None
+        check(lib().ak_tersoff_loss_grad(
+            self._pot.handle(), self.S, self.off.ctypes.data_as(_lib._pi64), ptr(self.lat),
+            ptr(self.pos), ptr(th), ptr(self.ref_e), ptr(self.ref_f), ptr(self.ref_s),
+            ptr(self.weights), ctypes.byref(loss), ptr(grad), ptr(E), ptr(F), ptr(S)))
+        if want_outputs:
+            return loss.value, grad, (E, F, S)
+        return loss.value, grad
+
+    def loss(self, theta):
+        return self.evaluate(theta, want_grad=False)[0]
+
+    def loss_and_grad(self, theta):
+        return self.evaluate(theta, want_grad=True)
+
+
+def _function():
+    import torch
+
+    class TersoffLoss(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, theta, fit):
+            L, g = fit.loss_and_grad(theta.detach().cpu().numpy().astype(np.float64))
+            ctx.save_for_backward(torch.as_tensor(g, dtype=theta.dtype, device=theta.device))
+            return theta.new_tensor(L)
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            (g,) = ctx.saved_tensors
+            return grad_out * g, None
+
+    return TersoffLoss
+
+
+_TL = None
+
+
+def tersoff_loss(theta, fit):
+    """Differentiable loss: a torch scalar whose backward is the engine's
+    dL/dtheta.  ``theta`` is a float64 tensor of the 14 parameters."""
+    global _TL
+    if _TL is None:
+        _TL = _function()
+    return _TL.apply(theta, fit)

@@ -1,0 +1,173 @@
+/*
+ * akitsushima_amd.h -- C ABI of the MI355X FP64 force engine.
+ *
+ * Every entry point returns 0 on success or a status code (AK_ERR_*); the
+ * message of the last failure on the calling thread is ak_last_error().
+ * Arrays are C-contiguous: lattice (3, 3) with lattice vectors as ROWS,
+ * positions and forces (N, 3) in Angstrom and eV/Angstrom, stress (3, 3) in
+ * eV/Angstrom^3 with the reference's sign convention
+ * (sigma = -(1/V) sum f (x) r; docs/adr/0001-stress-sign-and-multiplication-
+ * order-conventions.md:7-28).  Species are int32 indices into the species
+ * list the potential was created with.
+ *
+ * Handles are not thread-safe: use one potential handle per thread, as the
+ * reference keeps one owned neighbour list per thread
+ * (engine/neighborlist.py:1098-1100).
+ */
+#ifndef AKITSUSHIMA_AMD_H
+#define AKITSUSHIMA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AK_ABI_VERSION 1
+
+#define AK_OK 0
+#define AK_ERR_ARG 1
+#define AK_ERR_MIN_IMAGE 2 /* cell too thin for r_cut (+ skin): the reference's ValueError */
+#define AK_ERR_HIP 3
+#define AK_ERR_CAPACITY 4
+#define AK_ERR_STATE 5
+#define AK_ERR_NO_DEVICE 6
+
+typedef struct ak_potential ak_potential;
+typedef struct ak_nlist ak_nlist;
+
+int ak_abi_version(void);
+const char *ak_last_error(void);
+/* Number of visible HIP devices (0 without a GPU; never fails). */
+int ak_device_count(int32_t *count);
+
+/* ---- Potentials ------------------------------------------------------- */
+
+/* Tersoff in the table form of TersoffPotential._compute_multi_species
+ * (engine/tersoff.py:1518-1951; the flat single-species dict of
+ * TersoffPotential.__init__, engine/tersoff.py:1135-1169, is n_species = 1
+ * with m = 3, gamma = 1).
+ *   pair_table   (ns, ns, 9): valid, A, B, lambda1, lambda2, R, D, beta, n
+ *                for the record (i, j, j);
+ *   triple_table (ns, ns, ns, 7): valid, c, d, h, lambda3, m, gamma for the
+ *                record (i, j, k);
+ *   r_cut: max R + D over the pair records. */
+int ak_tersoff_create(int32_t n_species, const double *pair_table,
+                      const double *triple_table, double r_cut, ak_potential **out);
+
+/* Stillinger-Weber, flat single-species form of StillingerWeberPotential
+ * (engine/stillinger_weber.py:860-946): params = epsilon, sigma, A, B, p, q,
+ * a, lambda, gamma, costheta0; r_cut = a * sigma. */
+int ak_sw_create(const double *params10, ak_potential **out);
+
+/* Alloy EAM with Zhou-Johnson-Wadley functions (EAMPotential built by
+ * zhou_eam, engine/eam_zhou.py:128-207, evaluated as the numba kernel
+ * engine/_eam_kernels.py:231-307).  functions: (n_functions, 44) rows
+ * [code, r_c, width, chi, pa[20], pb[20]] with code 3 density, 4 pair,
+ * 5 cross pair, 6 embedding; index_map: rho[ns], embedding[ns], phi[ns*ns]
+ * row indices. */
+int ak_eam_create(int32_t n_species, int32_t n_functions, const double *functions,
+                  const int32_t *index_map, double r_cut, ak_potential **out);
+
+int ak_potential_destroy(ak_potential *pot);
+
+/* ---- Neighbour list (NeighbourList, engine/neighborlist.py:390-714) ---- */
+
+int ak_nlist_create(double r_cut, double skin_width, ak_nlist **out);
+int ak_nlist_destroy(ak_nlist *nl);
+/* NeighbourList.build (engine/neighborlist.py:574-585); inv_lattice may be
+ * NULL (then computed here); passing numpy's inverse reproduces the
+ * reference's fractional coordinates bit for bit. */
+int ak_nlist_build(ak_nlist *nl, int32_t n_atoms, const double *lattice,
+                   const double *inv_lattice, const double *positions);
+/* NeighbourList.check_and_rebuild (engine/neighborlist.py:613-673). */
+int ak_nlist_check_and_rebuild(ak_nlist *nl, int32_t n_atoms, const double *lattice,
+                               const double *inv_lattice, const double *positions,
+                               int32_t *rebuilt);
+/* NeighbourList.get_candidate_pairs (engine/neighborlist.py:679-714): pairs
+ * i < j within r_cut + skin at the last build, canonical (i, j) order.
+ * With idx_i == NULL only *n_pairs is set. */
+int ak_nlist_candidate_pairs(ak_nlist *nl, int64_t *n_pairs, int32_t *idx_i,
+                             int32_t *idx_j, int64_t capacity);
+int ak_nlist_counters(ak_nlist *nl, int64_t *n_builds, int64_t *n_steps);
+
+/* ---- Force calls ------------------------------------------------------ */
+
+/* potential.compute(lattice, positions, atom_types, neigh_list)
+ * (engine/tersoff.py:1208-1228, engine/stillinger_weber.py:900-922,
+ * engine/eam.py:174-256).  nl == NULL: the potential keeps a list of its
+ * own with skin 0.3 A (owned_neighbour_list, engine/neighborlist.py:1103-1163).
+ * Host buffers; stress may be NULL when need_stress == 0. */
+int ak_compute(ak_potential *pot, ak_nlist *nl, int32_t n_atoms, const double *lattice,
+               const double *inv_lattice, const double *positions, const int32_t *species,
+               int32_t need_stress, double *energy, double *forces, double *stress);
+
+/* Device-resident force call: d_positions (N, 3), d_species (N), d_forces
+ * (N, 3) and d_energy_stress (10: E, then the 9 stress components) are
+ * device pointers; the list must have been built (ak_nlist_build) for this
+ * structure.  Runs asynchronously on `stream` (a hipStream_t; NULL = the
+ * potential's own stream). */
+int ak_compute_device(ak_potential *pot, ak_nlist *nl, int32_t n_atoms,
+                      const double *lattice, const double *inv_lattice,
+                      const double *d_positions, const int32_t *d_species, double *d_forces,
+                      double *d_energy_stress, void *stream);
+
+/* Many independent structures in one launch sequence (the phonon / elastic /
+ * fitting callers of the force function, solvers/phonon.py:412-482,
+ * solvers/elastic.py:245-291).  atom_offsets (n_struct + 1); lattices
+ * (n_struct, 3, 3); positions/forces concatenated (sum N, 3); energies
+ * (n_struct); stresses (n_struct, 3, 3).  Host buffers. */
+int ak_compute_batched(ak_potential *pot, int32_t n_struct, const int64_t *atom_offsets,
+                       const double *lattices, const double *inv_lattices,
+                       const double *positions, const int32_t *species, double *energies,
+                       double *forces, double *stresses);
+
+/* Batched device-resident call for repeated evaluation: the first call with
+ * a given layout builds the batch neighbour list; `rebuild` != 0 forces a
+ * rebuild.  d_energy_stress is (n_struct, 10). */
+int ak_compute_batched_device(ak_potential *pot, int32_t n_struct,
+                              const int64_t *atom_offsets, const double *lattices,
+                              const double *d_positions, const int32_t *d_species,
+                              double *d_forces, double *d_energy_stress, int32_t rebuild,
+                              void *stream);
+
+/* Launch-shape tuning knob (the autotuner's output): EAM lanes per atom
+ * (4..64, power of two; 0 = default). */
+int ak_potential_set_lanes(ak_potential *pot, int32_t lanes);
+
+/* ---- Molecular dynamics ------------------------------------------------ */
+
+/* NVE velocity Verlet, device-resident (NVESimulation.run,
+ * solvers/molecular_dynamics.py:1969-1996), with the Verlet-skin rebuild
+ * test of engine/neighborlist.py:653-673 evaluated on the device each step.
+ * positions/velocities (N, 3) host buffers, updated in place; masses (N) in
+ * amu; dt in fs.  e_pot/e_kin (n_steps + 1) receive the energies after each
+ * step (index 0 = initial state).  use_graph != 0 replays one captured step
+ * with hipGraphLaunch.  *n_rebuilds counts list rebuilds. */
+int ak_md_nve(ak_potential *pot, int32_t n_atoms, const double *lattice,
+              const double *inv_lattice, double *positions, double *velocities,
+              const double *masses, const int32_t *species, double dt, int32_t n_steps,
+              double skin_width, int32_t use_graph, double *e_pot, double *e_kin,
+              int32_t *n_rebuilds, double *seconds);
+
+/* ---- Tersoff parameter gradient ---------------------------------------- */
+
+/* Loss of fitting/loss.py:184-242 (squared) over a batch of structures with
+ * the 14 Tersoff parameters theta = (A, B, lambda1, lambda2, lambda3, beta,
+ * n, c, d, h, R, D, m, gamma) of one species, and its gradient dL/dtheta by
+ * the hand-written backward kernel (no finite differences).
+ * ref_energy (n_struct) in eV/atom, ref_forces (sum N, 3), ref_stress
+ * (n_struct, 3, 3); weights = w_E, w_F, w_S, r0, eps0.  grad may be NULL.
+ * Per-structure E, F, sigma are written when the pointers are non-NULL. */
+int ak_tersoff_loss_grad(ak_potential *pot, int32_t n_struct, const int64_t *atom_offsets,
+                         const double *lattices, const double *positions,
+                         const double *theta14, const double *ref_energy,
+                         const double *ref_forces, const double *ref_stress,
+                         const double *weights5, double *loss, double *grad14,
+                         double *energies, double *forces, double *stresses);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AKITSUSHIMA_AMD_H */

@@ -1,0 +1,511 @@
+"""Oracle: CPU restatement of the reference's FP64 force path.
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py`` import this package, and only as the
+checker (or the timed CPU baseline).  The product package ``akitsushima_amd``
+never imports it and has no CPU fallback.
+
+The compiled kernels live in ``refcpu.c`` (built by ``oracle/Makefile`` into
+``oracle/liboracle.so``); this module is the NumPy glue around them, written
+after the reference's Python glue:
+
+* ``cell_metrics``/``check_min_cell``: engine/neighborlist.py:170-267, 489-572
+* ``nl_build``: NeighbourList._build_cell_linked, engine/neighborlist.py:935-1024,
+  plus _canonical_pair_order :1166-1181
+* ``candidate_pairs``: owned_neighbour_list, engine/neighborlist.py:1103-1163
+  (skin 0.3, all pairs below 100 atoms or in thin cells)
+* ``tersoff_flat``: TersoffPotential._compute_single_species, engine/tersoff.py:1233-1513
+* ``tersoff_multi``: TersoffPotential._compute_multi_species, engine/tersoff.py:1518-1951
+* ``sw_flat``: StillingerWeberPotential._compute_single_species,
+  engine/stillinger_weber.py:927-1159
+* ``zhou_*`` and ``eam_zhou``: engine/eam_zhou.py:47-207 and the numba kernel
+  engine/_eam_kernels.py:231-307
+* ``tersoff_loss``: fitting/loss.py:51-248 (squared loss)
+* ``nve_run``: NVESimulation.run, solvers/molecular_dynamics.py:1969-1996
+
+Parity pinning of every function is listed in DESIGN-level docs; each test
+file under tests/ names what pins it.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+def build():
+    """Compile refcpu.c with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_nl_build.restype = ctypes.c_int64
+        L.orc_nl_build.argtypes = [ctypes.c_int32, _i32p, _f64p, _f64p, _f64p,
+                                   ctypes.c_double, _i32p, _i32p, _f64p, _f64p]
+        common = [ctypes.c_int32, _f64p, _f64p, _f64p, ctypes.c_double,
+                  ctypes.c_int64, _i32p, _i32p]
+        L.orc_tersoff_flat.restype = ctypes.c_int
+        L.orc_tersoff_flat.argtypes = common + [_f64p, ctypes.c_int, _f64p, _f64p, _f64p]
+        L.orc_tersoff_multi.restype = ctypes.c_int
+        L.orc_tersoff_multi.argtypes = common + [_i32p, ctypes.c_int32, _f64p, _f64p,
+                                                 ctypes.c_double, ctypes.c_int,
+                                                 _f64p, _f64p, _f64p]
+        L.orc_sw_flat.restype = ctypes.c_int
+        L.orc_sw_flat.argtypes = common + [_f64p, ctypes.c_int, _f64p, _f64p, _f64p]
+        L.orc_eam.restype = ctypes.c_int
+        L.orc_eam.argtypes = common + [_i32p, ctypes.c_int32, _i32p, _i32p, _i32p,
+                                       _f64p, ctypes.c_double, ctypes.c_int,
+                                       _f64p, _f64p, _f64p]
+        L.orc_zhou_feval.restype = None
+        L.orc_zhou_feval.argtypes = [_f64p, ctypes.c_int64, _f64p, _f64p, _f64p]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+# ---------------------------------------------------------------------------
+#  Cell geometry (engine/neighborlist.py:170-267, 98-132, 489-572)
+# ---------------------------------------------------------------------------
+
+def cell_metrics(lattice):
+    lat = np.asarray(lattice, dtype=float)
+    volume = abs(float(np.linalg.det(lat)))
+    cross = np.array([np.linalg.norm(np.cross(lat[(i + 1) % 3], lat[(i + 2) % 3]))
+                      for i in range(3)])
+    if volume <= 0.0 or not np.all(cross > 0.0):
+        raise ValueError("degenerate (singular) lattice")
+    perp = volume / cross
+    sigma_min = float(np.linalg.svd(lat, compute_uv=False)[-1])
+    return perp, sigma_min
+
+
+def check_min_cell(lattice, r_list):
+    perp, _ = cell_metrics(lattice)
+    if r_list >= float(np.min(perp)) / 2.0:
+        raise ValueError(
+            f"cutoff radius ({r_list:.4f} A) must be < half the smallest "
+            f"perpendicular cell width ({float(np.min(perp)) / 2.0:.4f} A)")
+
+
+# ---------------------------------------------------------------------------
+#  Neighbour list
+# ---------------------------------------------------------------------------
+
+def nl_build(lattice, positions, r_list):
+    """Candidate pairs i < j with r^2 < r_list^2 in canonical (i, j) order:
+    (idx_i, idx_j, rij, r), as NeighbourList.build with the CLL back-end
+    followed by _canonical_pair_order."""
+    lattice = np.ascontiguousarray(lattice, dtype=float)
+    positions = np.ascontiguousarray(positions, dtype=float)
+    check_min_cell(lattice, r_list)
+    inv = np.ascontiguousarray(np.linalg.inv(lattice))
+    perp, _ = cell_metrics(lattice)
+    nc = np.maximum(1, np.floor(perp / r_list).astype(int)).astype(np.int32)
+    n = len(positions)
+    rl2 = r_list ** 2
+    L = lib()
+    npair = L.orc_nl_build(n, _p(nc, _i32p), _p(positions, _f64p), _p(lattice, _f64p),
+                           _p(inv, _f64p), rl2, None, None, None, None)
+    oi = np.empty(npair, np.int32)
+    oj = np.empty(npair, np.int32)
+    orij = np.empty((npair, 3))
+    orr = np.empty(npair)
+    L.orc_nl_build(n, _p(nc, _i32p), _p(positions, _f64p), _p(lattice, _f64p),
+                   _p(inv, _f64p), rl2, _p(oi, _i32p), _p(oj, _i32p),
+                   _p(orij, _f64p), _p(orr, _f64p))
+    return oi, oj, orij, orr
+
+
+def brute_force_pairs(lattice, positions, r_list):
+    """O(N^2) minimum-image pair set, independent of the cell list
+    (tests/test_neighborlist.py:27-41 style), vectorised per row."""
+    lattice = np.asarray(lattice, float)
+    positions = np.asarray(positions, float)
+    inv = np.linalg.inv(lattice)
+    out_i, out_j = [], []
+    n = len(positions)
+    for i in range(n - 1):
+        d = positions[i + 1:] - positions[i]
+        s = d[:, 0:1] * inv[0] + d[:, 1:2] * inv[1] + d[:, 2:3] * inv[2]
+        s = s - np.round(s)
+        v = s[:, 0:1] * lattice[0] + s[:, 1:2] * lattice[1] + s[:, 2:3] * lattice[2]
+        r2 = v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1] + v[:, 2] * v[:, 2]
+        m = np.nonzero((r2 < r_list * r_list) & (r2 > 1e-20))[0]
+        out_i.append(np.full(len(m), i, np.int32))
+        out_j.append((m + i + 1).astype(np.int32))
+    if not out_i:
+        return np.empty(0, np.int32), np.empty(0, np.int32)
+    return np.concatenate(out_i), np.concatenate(out_j)
+
+
+OWNED_SKIN = 0.3
+OWNED_MIN_ATOMS = 100
+
+
+def candidate_pairs(lattice, positions, r_cut):
+    """The candidates a force function called without a list uses
+    (owned_neighbour_list, engine/neighborlist.py:1103-1163)."""
+    n = len(positions)
+    perp, _ = cell_metrics(lattice)
+    if n < OWNED_MIN_ATOMS or r_cut + OWNED_SKIN >= float(np.min(perp)) / 2.0:
+        check_min_cell(lattice, r_cut)
+        ii, jj = np.triu_indices(n, k=1)
+        return ii.astype(np.int32), jj.astype(np.int32)
+    i, j, _, _ = nl_build(lattice, positions, r_cut + OWNED_SKIN)
+    return i, j
+
+
+def _prep(lattice, positions, pairs, r_cut):
+    lattice = np.ascontiguousarray(lattice, dtype=float)
+    positions = np.ascontiguousarray(positions, dtype=float)
+    if pairs is None:
+        pairs = candidate_pairs(lattice, positions, r_cut)
+    ci = np.ascontiguousarray(pairs[0], dtype=np.int32)
+    cj = np.ascontiguousarray(pairs[1], dtype=np.int32)
+    inv = np.ascontiguousarray(np.linalg.inv(lattice))
+    vol = float(np.abs(np.linalg.det(lattice)))
+    return lattice, positions, ci, cj, inv, vol
+
+
+# ---------------------------------------------------------------------------
+#  Tersoff
+# ---------------------------------------------------------------------------
+
+TERSOFF_KEYS = ("A", "B", "lambda1", "lambda2", "lambda3", "beta", "n",
+                "c", "d", "h", "R", "D")
+
+# benchmarks/bench_force.py:81-98 (T3 Si)
+TERSOFF_SI = {"A": 1830.8, "B": 471.18, "lambda1": 2.4799, "lambda2": 1.7322,
+              "lambda3": 0.0, "beta": 1.1e-6, "n": 0.78734, "c": 100390.0,
+              "d": 16.217, "h": -0.59825, "R": 2.85, "D": 0.15}
+
+
+def tersoff_flat(lattice, positions, params=TERSOFF_SI, pairs=None, nthreads=1):
+    prm = np.array([float(params[k]) for k in TERSOFF_KEYS])
+    rc = prm[10] + prm[11]
+    lattice, positions, ci, cj, inv, vol = _prep(lattice, positions, pairs, rc)
+    n = len(positions)
+    e = ctypes.c_double()
+    f = np.zeros((n, 3))
+    s = np.zeros((3, 3))
+    lib().orc_tersoff_flat(n, _p(positions, _f64p), _p(lattice, _f64p), _p(inv, _f64p),
+                           vol, len(ci), _p(ci, _i32p), _p(cj, _i32p), _p(prm, _f64p),
+                           int(nthreads), ctypes.byref(e), _p(f, _f64p), _p(s, _f64p))
+    return e.value, f, s
+
+
+def tersoff_tables(records, species):
+    """Dense (n_sp, n_sp) pair table and (n_sp)^3 triple table as built in
+    engine/tersoff.py:1606-1653.  records: {(i, j, k): dict}."""
+    ns = len(species)
+    tb = np.zeros((ns, ns, 9))
+    tb[:, :, 5] = 1.0   # R default
+    tb[:, :, 6] = 1.0   # D default
+    tb[:, :, 8] = 1.0   # n default
+    tri = np.zeros((ns, ns, ns, 7))
+    tri[:, :, :, 2] = 1.0   # d default
+    tri[:, :, :, 5] = 3.0   # m default
+    tri[:, :, :, 6] = 1.0   # gamma default
+    for a, sa in enumerate(species):
+        for b, sb in enumerate(species):
+            t = records.get((sa, sb, sb))
+            if t is not None:
+                tb[a, b] = [1.0, float(t.get("A", 0.0)), float(t.get("B", 0.0)),
+                            float(t.get("lambda1", 0.0)), float(t.get("lambda2", 0.0)),
+                            float(t.get("R", 1.0)), float(t.get("D", 1.0)),
+                            float(t.get("beta", 0.0)), float(t.get("n", 1.0))]
+            for c, sc in enumerate(species):
+                t3 = records.get((sa, sb, sc))
+                if t3 is not None:
+                    tri[a, b, c] = [1.0, float(t3["c"]), float(t3["d"]), float(t3["h"]),
+                                    float(t3["lambda3"]), float(t3.get("m", 3.0)),
+                                    float(t3.get("gamma", 1.0))]
+    rc = 0.0
+    for (i, j, k), v in records.items():
+        if j == k and "R" in v and "D" in v:
+            rc = max(rc, float(v["R"]) + float(v["D"]))
+    return tb, tri, (rc if rc > 0 else 3.0)
+
+
+def tersoff_multi(lattice, positions, types, records, species, pairs=None, nthreads=1):
+    tb, tri, rc = tersoff_tables(records, species)
+    lattice, positions, ci, cj, inv, vol = _prep(lattice, positions, pairs, rc)
+    idx = {s: k for k, s in enumerate(species)}
+    tid = np.array([idx[t] for t in types], np.int32)
+    n = len(positions)
+    e = ctypes.c_double()
+    f = np.zeros((n, 3))
+    s = np.zeros((3, 3))
+    tb = np.ascontiguousarray(tb)
+    tri = np.ascontiguousarray(tri)
+    lib().orc_tersoff_multi(n, _p(positions, _f64p), _p(lattice, _f64p), _p(inv, _f64p),
+                            vol, len(ci), _p(ci, _i32p), _p(cj, _i32p), _p(tid, _i32p),
+                            len(species), _p(tb, _f64p), _p(tri, _f64p), rc, int(nthreads),
+                            ctypes.byref(e), _p(f, _f64p), _p(s, _f64p))
+    return e.value, f, s
+
+
+def tersoff_flat_as_records(params, sp="Si", m=3.0, gamma=1.0):
+    """The single-species flat dict as one (sp, sp, sp) record with explicit
+    m and gamma (the 14-parameter form of the multi-species path)."""
+    rec = {k: float(params[k]) for k in TERSOFF_KEYS}
+    rec["m"] = float(m)
+    rec["gamma"] = float(gamma)
+    return {(sp, sp, sp): rec}
+
+
+# ---------------------------------------------------------------------------
+#  Stillinger-Weber (defaults: engine/stillinger_weber.py:933-946)
+# ---------------------------------------------------------------------------
+
+SW_KEYS = ("epsilon", "sigma", "A", "B", "p", "q", "a", "lambda", "gamma", "costheta0")
+SW_DEFAULTS = {"epsilon": 2.1682, "sigma": 2.0951, "A": 7.049556, "B": 0.6022246,
+               "p": 4.0, "q": 0.0, "a": 1.80, "lambda": 21.0, "gamma": 1.20,
+               "costheta0": -1.0 / 3.0}
+# benchmarks/bench_force.py:64-79
+SW_SI = {"epsilon": 2.1683, "sigma": 2.0951, "a": 1.80, "gamma": 1.20, "lambda": 21.0,
+         "A": 7.049556277, "B": 0.6022245584, "p": 4.0, "q": 0.0, "costheta0": -1 / 3}
+
+
+def sw_prm(params):
+    return np.array([float(params.get(k, SW_DEFAULTS[k])) for k in SW_KEYS])
+
+
+def sw_flat(lattice, positions, params=SW_SI, pairs=None, nthreads=1):
+    prm = sw_prm(params)
+    rc = prm[6] * prm[1]
+    lattice, positions, ci, cj, inv, vol = _prep(lattice, positions, pairs, rc)
+    n = len(positions)
+    e = ctypes.c_double()
+    f = np.zeros((n, 3))
+    s = np.zeros((3, 3))
+    lib().orc_sw_flat(n, _p(positions, _f64p), _p(lattice, _f64p), _p(inv, _f64p), vol,
+                      len(ci), _p(ci, _i32p), _p(cj, _i32p), _p(prm, _f64p), int(nthreads),
+                      ctypes.byref(e), _p(f, _f64p), _p(s, _f64p))
+    return e.value, f, s
+
+
+# ---------------------------------------------------------------------------
+#  Zhou 2004 EAM (engine/eam_zhou.py:47-207)
+# ---------------------------------------------------------------------------
+
+ZHOU_KEYS = ("r_e", "f_e", "rho_e", "rho_s", "alpha", "beta", "A", "B", "kappa",
+             "lambda", "Fn0", "Fn1", "Fn2", "Fn3", "F0", "F1", "F2", "F3", "eta", "F_e")
+
+# Zhou, Johnson and Wadley, PRB 69, 144113 (2004), Table III rows used by the
+# checks (engine/published_parameters.py:116-180).
+ZHOU_CU = dict(zip(ZHOU_KEYS, (2.556162, 1.554485, 21.175871, 21.175395, 8.127620, 4.334731,
+                               0.396620, 0.548085, 0.308782, 0.756515, -2.170269, -0.263788,
+                               1.088878, -0.817603, -2.19, 0.0, 0.561830, -2.100595, 0.310490,
+                               -2.186568)))
+ZHOU_NI = dict(zip(ZHOU_KEYS, (2.488746, 2.007018, 27.562015, 27.930410, 8.383453, 4.471175,
+                               0.429046, 0.633531, 0.443599, 0.820658, -2.693513, -0.076445,
+                               0.241442, -2.375626, -2.70, 0.0, 0.265390, -0.152856, 0.469000,
+                               -2.699486)))
+
+
+def zhou_continuous(p):
+    """engine/eam_zhou.py:47-64."""
+    q = {k: float(v) for k, v in p.items()}
+    re, rs, eta = q["rho_e"], q["rho_s"], q["eta"]
+    F1, F2, F3 = q["F1"], q["F2"], q["F3"]
+    y, x = 0.15, -0.15
+    t0 = 1.15 * re / rs
+    slope0 = (F1 + 2 * F2 * y + 3 * F3 * y * y) / re
+    dg = -eta * eta * math.log(t0) * t0 ** (eta - 1.0) / rs
+    q["F_e"] = slope0 / dg
+    q["F0"] = q["F_e"] * (1 - eta * math.log(t0)) * t0 ** eta - (F1 * y + F2 * y * y + F3 * y ** 3)
+    q["Fn0"] = q["F0"] + F1 * x + F2 * x * x + F3 * x ** 3
+    q["Fn1"] = 0.85 * (F1 + 2 * F2 * x + 3 * F3 * x * x)
+    return q
+
+
+def _zdecay(r, re, pre, expo, shift):
+    x = r / re
+    e = pre * np.exp(-expo * (x - 1.0))
+    u = x - shift
+    den = 1.0 + u ** 20
+    return e / den
+
+
+def zhou_cutoff(p, tol=1e-6):
+    """engine/eam_zhou.py:117-125."""
+    r = np.linspace(p["r_e"], 5.0 * p["r_e"], 40001)
+    pair = _zdecay(r, p["r_e"], p["A"], p["alpha"], p["kappa"]) - \
+        _zdecay(r, p["r_e"], p["B"], p["beta"], p["lambda"])
+    dens = _zdecay(r, p["r_e"], p["f_e"], p["beta"], p["lambda"])
+    ok = (np.abs(pair) < tol * max(p["A"], p["B"])) & (dens < tol * p["f_e"])
+    bad = np.nonzero(~ok)[0]
+    return float(r[bad[-1] + 1]) if len(bad) else float(r[0])
+
+
+ZW = 44
+
+
+def zhou_function_table(elements, species, r_cut=None, chi=None, width=0.5):
+    """Rows [code, rc, width, chi, pa[20], pb[20]] and the index maps
+    RHO[s], EMB[s], PHI[s, t] of the numba kernel tables
+    (engine/_eam_kernels.py:77-121, engine/eam_zhou.py:128-207)."""
+    P = {s: zhou_continuous(elements[s]) for s in species}
+    if r_cut is None:
+        r_cut = max(zhou_cutoff(P[s]) for s in species)
+    chi = chi or {}
+    rows = []
+
+    def row(code, pa, pb=None, k=1.0):
+        v = np.zeros(ZW)
+        v[0], v[1], v[2], v[3] = code, r_cut, width, k
+        v[4:24] = [pa[kk] for kk in ZHOU_KEYS]
+        if pb is not None:
+            v[24:44] = [pb[kk] for kk in ZHOU_KEYS]
+        rows.append(v)
+        return len(rows) - 1
+
+    ns = len(species)
+    rho_f = np.array([row(3, P[s]) for s in species], np.int32)
+    emb_f = np.array([row(6, P[s]) for s in species], np.int32)
+    phi_f = np.zeros((ns, ns), np.int32)
+    for a in range(ns):
+        for b in range(a, ns):
+            sa, sb = species[a], species[b]
+            if a == b:
+                k = row(4, P[sa])
+            else:
+                cf = float(chi.get(f"{sa}-{sb}", chi.get(f"{sb}-{sa}", 1.0)))
+                k = row(5, P[sa], P[sb], cf)
+            phi_f[a, b] = phi_f[b, a] = k
+    return np.array(rows), rho_f, emb_f, phi_f, float(r_cut), P
+
+
+def eam_zhou(lattice, positions, types, elements, species, pairs=None, nthreads=1,
+             r_cut=None, chi=None):
+    fn, rho_f, emb_f, phi_f, rc, _ = zhou_function_table(elements, species, r_cut, chi)
+    lattice, positions, ci, cj, inv, vol = _prep(lattice, positions, pairs, rc)
+    idx = {s: k for k, s in enumerate(species)}
+    spc = np.array([idx[t] for t in types], np.int32)
+    n = len(positions)
+    e = ctypes.c_double()
+    f = np.zeros((n, 3))
+    s = np.zeros((3, 3))
+    fn = np.ascontiguousarray(fn)
+    phi_f = np.ascontiguousarray(phi_f.reshape(-1))
+    lib().orc_eam(n, _p(positions, _f64p), _p(lattice, _f64p), _p(inv, _f64p), vol,
+                  len(ci), _p(ci, _i32p), _p(cj, _i32p), _p(spc, _i32p), len(species),
+                  _p(rho_f, _i32p), _p(emb_f, _i32p), _p(phi_f, _i32p), _p(fn, _f64p),
+                  rc, int(nthreads), ctypes.byref(e), _p(f, _f64p), _p(s, _f64p))
+    return e.value, f, s
+
+
+def zhou_feval(row, r):
+    r = np.ascontiguousarray(r, dtype=float)
+    v = np.empty_like(r)
+    dv = np.empty_like(r)
+    row = np.ascontiguousarray(row, dtype=float)
+    lib().orc_zhou_feval(_p(row, _f64p), len(r), _p(r, _f64p), _p(v, _f64p), _p(dv, _f64p))
+    return v, dv
+
+
+# ---------------------------------------------------------------------------
+#  Fit loss (fitting/loss.py:184-242, squared loss, default weights)
+# ---------------------------------------------------------------------------
+
+DEFAULT_WEIGHTS = {"energy": 1.0, "forces": 1.0, "stress": 0.1, "r0": 1.0, "eps0": 1.0}
+
+
+def tersoff_loss(structures, records, species, weights=None, nthreads=1):
+    """Sum over structures of w_E (N dE/N)^2 + w_F r0^2 mean dF^2
+    + w_S (V eps0)^2 mean dsigma^2."""
+    w = dict(DEFAULT_WEIGHTS)
+    w.update(weights or {})
+    total = 0.0
+    for s in structures:
+        lat, pos, types = s["lattice"], s["positions"], s["atom_types"]
+        N = len(pos)
+        vol = abs(np.linalg.det(lat))
+        e, f, st = tersoff_multi(lat, pos, types, records, species, nthreads=nthreads)
+        tot = w["energy"] * (N * (e / N - s["ref_energy"])) ** 2
+        tot += w["forces"] * w["r0"] ** 2 * float(np.mean((f - s["ref_forces"]) ** 2))
+        tot += w["stress"] * (vol * w["eps0"]) ** 2 * float(np.mean((st - s["ref_stress"]) ** 2))
+        total += tot
+    return total
+
+
+# ---------------------------------------------------------------------------
+#  NVE velocity Verlet (solvers/molecular_dynamics.py:195-221, 1969-1996)
+# ---------------------------------------------------------------------------
+
+K_B = 1.380649e-23 * 6.241509074e18
+EV_ANG_TO_AMU_ANG_FS2 = (1.60217663e-19 / 1.0e-10) / (1.66053906660e-27) * 1.0e-20
+
+
+def mb_velocities(T, masses, seed):
+    rng = np.random.default_rng(seed)
+    N = len(masses)
+    sigma = np.sqrt(K_B * T * EV_ANG_TO_AMU_ANG_FS2 / masses)
+    v = rng.standard_normal((N, 3)) * sigma[:, None]
+    v -= (np.sum(masses[:, None] * v, axis=0) / masses.sum())
+    e_kin = 0.5 * float(np.sum(masses[:, None] * v ** 2)) / EV_ANG_TO_AMU_ANG_FS2
+    n_dof = max(3 * N - 3, 1)
+    T_act = 2.0 * e_kin / (n_dof * K_B)
+    if T_act > 1e-12:
+        v *= np.sqrt(T / T_act)
+    return v
+
+
+def nve_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, skin):
+    """Velocity Verlet with PBC wrap and the Verlet-skin rebuild test of
+    engine/neighborlist.py:613-673 (fixed cell: rebuild when
+    r_list - 2 d_max <= r_cut).  ``force(lattice, pos, (i, j))`` returns
+    (E, F, S).  Returns per-step E_pot, E_kin arrays and the final state."""
+    lattice = np.asarray(lattice, float)
+    pos = np.array(positions, float)
+    vel = np.array(velocities, float)
+    inv = np.linalg.inv(lattice)
+    r_list = r_cut + skin
+    ref = pos.copy()
+    pairs = nl_build(lattice, pos, r_list)[:2]
+    e, f, _ = force(lattice, pos, pairs)
+    acc = f * EV_ANG_TO_AMU_ANG_FS2 / masses[:, None]
+    ek = lambda v: 0.5 * float(np.sum(masses[:, None] * v ** 2)) / EV_ANG_TO_AMU_ANG_FS2
+    epot = [e]
+    ekin = [ek(vel)]
+    rebuilds = 0
+    for _ in range(n_steps):
+        vel += 0.5 * dt * acc
+        pos += dt * vel
+        frac = pos @ inv
+        frac -= np.floor(frac)
+        frac[frac >= 1.0] = 0.0
+        pos = frac @ lattice
+        dr = pos - ref
+        fr = dr @ inv
+        fr -= np.round(fr)
+        dmax = float(np.max(np.linalg.norm(fr @ lattice, axis=1)))
+        if r_list - 2.0 * dmax <= r_cut:
+            pairs = nl_build(lattice, pos, r_list)[:2]
+            ref = pos.copy()
+            rebuilds += 1
+        e, f, _ = force(lattice, pos, pairs)
+        acc = f * EV_ANG_TO_AMU_ANG_FS2 / masses[:, None]
+        vel += 0.5 * dt * acc
+        epot.append(e)
+        ekin.append(ek(vel))
+    return np.array(epot), np.array(ekin), pos, vel, rebuilds
