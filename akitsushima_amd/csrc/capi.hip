@@ -537,3 +537,34 @@ int ak_tersoff_loss_grad(ak_potential* P, int32_t S, const int64_t* off,
 }
 
 }  // extern "C"
+
+extern "C" {
+
+int ak_profile_enable(ak_potential* P, int32_t on) {
+  return guarded([&] {
+    AK_REQUIRE(P, AK_ERR_ARG, "null potential");
+    P->pot.prof_on = on != 0;
+    P->pot.prof_used = 0;
+  });
+}
+
+int ak_profile_read(ak_potential* P, double* total_ms, double* min_ms, int64_t* count) {
+  return guarded([&] {
+    AK_REQUIRE(P && total_ms && count, AK_ERR_ARG, "bad arguments");
+    double tot = 0.0, mn = 0.0;
+    for (size_t k = 0; k < P->pot.prof_used; ++k) {
+      auto& e = P->pot.prof_pool[k];
+      AK_HIP(hipEventSynchronize(e.b));
+      float ms = 0.0f;
+      AK_HIP(hipEventElapsedTime(&ms, e.a, e.b));
+      tot += ms;
+      mn = (k == 0 || ms < mn) ? ms : mn;
+    }
+    *total_ms = tot;
+    if (min_ms) *min_ms = mn;
+    *count = (int64_t)P->pot.prof_used;
+    P->pot.prof_used = 0;
+  });
+}
+
+}  // extern "C"

@@ -68,12 +68,69 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
   return (uint64_t(1) << __lane_id()) - 1;
 }
 
+// DPP quad permutation of a double: lane q of every quad receives lane
+// ((CTRL >> 2q) & 3) of its quad.  A VALU move, no LDS round trip.
+template <int CTRL>
+__device__ __forceinline__ double dpp_quad(double x) {
+  int lo = __double2loint(x), hi = __double2hiint(x);
+  lo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, false);
+  hi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+template <int CTRL>
+__device__ __forceinline__ int dpp_quad(int x) {
+  return __builtin_amdgcn_mov_dpp(x, CTRL, 0xF, 0xF, false);
+}
+
+// Value of lane `seg + s` of the calling lane's segment of L lanes.  For
+// L = 4 the segment is a DPP quad and s must be a compile-time constant after
+// unrolling (broadcast of lane s: quad_perm [s, s, s, s]).
+template <int L, class T>
+__device__ __forceinline__ T seg_get(T x, int seg, int s) {
+  if constexpr (L == 4) {
+    switch (s) {
+      case 0: return dpp_quad<0x00>(x);
+      case 1: return dpp_quad<0x55>(x);
+      case 2: return dpp_quad<0xAA>(x);
+      default: return dpp_quad<0xFF>(x);
+    }
+  } else {
+    return __shfl(x, seg + s, kWave);
+  }
+}
+
 // Sum over a segment of L consecutive lanes (L a power of two <= 64); every
-// lane of the segment receives the total.
+// lane of the segment receives the total.  The last two butterfly steps are
+// DPP quad permutations (xor 1: [1,0,3,2], xor 2: [2,3,0,1]).
 template <int L>
 __device__ __forceinline__ double seg_sum(double v) {
 #pragma unroll
-  for (int o = L / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  for (int o = L / 2; o >= 4; o >>= 1) v += __shfl_xor(v, o, kWave);
+  if constexpr (L >= 4) v += dpp_quad<0x4E>(v);
+  if constexpr (L >= 2) v += dpp_quad<0xB1>(v);
+  return v;
+}
+
+// DPP row rotation of a double within rows of 16 lanes (row_ror:N is
+// 0x120 + N).
+template <int CTRL>
+__device__ __forceinline__ double dpp_row(double x) {
+  int lo = __double2loint(x), hi = __double2hiint(x);
+  lo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, false);
+  hi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wavefront, every lane receives it: DPP quad
+// butterflies, row rotations by 4 and 8, then two cross-row shuffles.
+// Fixed order, so bitwise reproducible.
+__device__ __forceinline__ double wave_sum(double v) {
+  v += dpp_quad<0xB1>(v);
+  v += dpp_quad<0x4E>(v);
+  v += dpp_row<0x124>(v);
+  v += dpp_row<0x128>(v);
+  v += __shfl_xor(v, 16, kWave);
+  v += __shfl_xor(v, 32, kWave);
   return v;
 }
 

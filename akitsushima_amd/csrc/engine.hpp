@@ -75,7 +75,18 @@ struct SetDev {
   std::vector<int> atom_off;  // n_struct + 1
   std::vector<int> uploaded_off;  // atom_off of the uploaded sid array
   std::vector<Geom> uploaded_geoms;
-  DevBuf d_geoms, d_sid, d_cell_offs;
+  DevBuf d_sid, d_geoms, d_cell_offs;
+  // Launch chunks: contiguous atom ranges of at most `size` atoms inside one
+  // structure, one thread block each, so that block partial sums never mix
+  // structures.  rng (n, 2) = [first atom, end); first (S + 1) = first chunk
+  // of each structure; partial (n, 10) = block sums of energy and virial.
+  struct Chunks {
+    int size = 0;
+    int n = 0;
+    DevBuf rng, first, partial;
+  };
+  std::vector<std::unique_ptr<Chunks>> chunk_tables;
+  Chunks& chunks(int size, hipStream_t st);
   // (re)builds geometry for r_list; lattices (S, 9), invs may be null.
   void setup(int n_struct, const int64_t* offsets, const double* lattices, const double* invs,
              double r_list, hipStream_t st, bool check_min_image = true);
@@ -127,12 +138,36 @@ struct Potential {
   std::vector<int> h_map;
   DevBuf d_map;
   // per-call scratch
-  DevBuf G, e_atom, w_atom, aux, out;
+  DevBuf G, aux, out;
   // launch shape chosen by the autotuner (0 = default)
   int eam_lanes = 0;
   int block = 256;
   // fitting session (fit.hip)
   std::shared_ptr<void> fit;
+  // optional HIP-event timing of the dominant kernel (site kernel for
+  // Tersoff/SW, force pass for EAM), recorded on the launch stream
+  struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+  };
+  bool prof_on = false;
+  std::vector<EventPair> prof_pool;
+  size_t prof_used = 0;
+  EventPair* prof_next() {
+    if (!prof_on) return nullptr;
+    if (prof_used == prof_pool.size()) {
+      EventPair p;
+      AK_HIP(hipEventCreate(&p.a));
+      AK_HIP(hipEventCreate(&p.b));
+      prof_pool.push_back(p);
+    }
+    return &prof_pool[prof_used++];
+  }
+  ~Potential() {
+    for (auto& p : prof_pool) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+  }
 };
 
 // Forces on all atoms of `set`, per-structure energy and stress.

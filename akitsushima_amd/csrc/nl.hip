@@ -121,7 +121,36 @@ void SetDev::setup(int S, const int64_t* offsets, const double* lattices, const 
     AK_HIP(hipMemcpyAsync(d_sid.p, sid.data(), sizeof(int) * sid.size(),
                           hipMemcpyHostToDevice, st));
     AK_HIP(hipStreamSynchronize(st));
+    chunk_tables.clear();
   }
+}
+
+SetDev::Chunks& SetDev::chunks(int size, hipStream_t st) {
+  for (auto& c : chunk_tables)
+    if (c->size == size) return *c;
+  auto c = std::make_unique<Chunks>();
+  c->size = size;
+  std::vector<int> rng, first(n_struct + 1, 0);
+  for (int s = 0; s < n_struct; ++s) {
+    first[s] = (int)rng.size() / 2;
+    for (int a = atom_off[s]; a < atom_off[s + 1]; a += size) {
+      rng.push_back(a);
+      rng.push_back(std::min(a + size, atom_off[s + 1]));
+    }
+  }
+  first[n_struct] = (int)rng.size() / 2;
+  c->n = first[n_struct];
+  c->rng.ensure(sizeof(int) * std::max<size_t>(2, rng.size()));
+  c->first.ensure(sizeof(int) * first.size());
+  c->partial.ensure(sizeof(double) * 40 * std::max(1, c->n));  // 4 wavefronts per chunk
+  if (!rng.empty())
+    AK_HIP(hipMemcpyAsync(c->rng.p, rng.data(), sizeof(int) * rng.size(), hipMemcpyHostToDevice,
+                          st));
+  AK_HIP(hipMemcpyAsync(c->first.p, first.data(), sizeof(int) * first.size(),
+                        hipMemcpyHostToDevice, st));
+  AK_HIP(hipStreamSynchronize(st));
+  chunk_tables.push_back(std::move(c));
+  return *chunk_tables.back();
 }
 
 void SetDev::upload_geoms(hipStream_t st) {

@@ -1,0 +1,284 @@
+"""Benchmark of the MI355X force engine.
+
+Headline: FP64 Tersoff Si force evaluations at 32768 atoms (diamond 16^3
+conventional cells, a = 5.431 A, Gaussian rattle 0.05 A from seed 0, T3
+parameters with R = 2.85, D = 0.15), as the reference's force benchmark
+builds its geometry (benchmarks/bench_force.py:152-158).  A step is one
+force call -- energy, forces, virial stress -- over a neighbour list built
+beforehand, with positions already in device memory (the reference's
+``call_nl_s``).  value = atoms x steps / seconds.
+
+The line also carries:
+  roofline      the Tersoff site kernel's algorithmic FP64 rate (flops counted
+                for the reference's algorithm on this geometry, see
+                ``tersoff_flops``) over its average duration, timed with HIP
+                events on its launch stream during the timed region;
+                ``traffic`` from the committed rocprofv3 counter summary;
+  cpu_baseline  the oracle (C/OpenMP restatement of the reference path,
+                kind "port") on this host's cores, same workload;
+  fit           batched Tersoff fit-gradient throughput (256 structures of
+                64-512 atoms) against a per-structure GPU loop.
+Run ``python bench.py --gpus 1 --steps K --warmup W``.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = ("atom-force-evals/sec (FP64) at 32k-atom Tersoff Si vs host Numba; "
+          "batched fit grads/sec")
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix), datasheet
+HBM_PEAK_GBS = 8000.0
+W_TRANSCENDENTAL = 20     # flops charged per exp / log / pow / sin / cos
+
+
+def tersoff_flops(n_pairs, n_triplets, lambda3):
+    """Algorithmic FP64 work of one Tersoff force call, counted on the
+    reference's algorithm (engine/tersoff.py:1233-1513 and the numba kernels
+    :161-202, :208-325, :464-509) with +,-,*,/,sqrt as 1 flop and
+    exp/log/pow/sin/cos as W_TRANSCENDENTAL flops.
+      per pair (i < j within r_c): minimum image 45 + 1 sqrt; bond orders of
+        both directions 18 + 6 T; two-body energy, force, virial 48 + 2 T;
+      per ordered triplet (i; j, k): zeta term 22, force term 136 + 2 sqrt,
+        plus 9 flops + 7 T for the exp(lambda3^3 (r_ij - r_ik)^3) factor and
+        its derivative when lambda3 != 0 (identically 1 when lambda3 = 0).
+    """
+    per_pair = 45 + 1 + 18 + 48 + 8 * W_TRANSCENDENTAL
+    per_trip = 22 + 136 + 2
+    if lambda3 != 0.0:
+        per_trip += 9 + 7 * W_TRANSCENDENTAL
+    return n_pairs * per_pair + n_triplets * per_trip
+
+
+def geometry_counts(lat, pos, r_cut):
+    """Pairs within r_cut and ordered triplets sum_i n_i (n_i - 1)."""
+    from akitsushima_amd import NeighbourList
+
+    nl = NeighbourList(r_cut, skin_width=0.0)
+    nl.build(lat, pos)
+    i, j = nl.get_candidate_pairs()
+    d = pos[j] - pos[i]
+    inv = np.linalg.inv(lat)
+    s = d @ inv
+    s -= np.round(s)
+    r = np.linalg.norm(s @ lat, axis=1)
+    m = r < r_cut
+    cnt = np.bincount(np.concatenate([i[m], j[m]]), minlength=len(pos))
+    return int(m.sum()), int(np.sum(cnt * (cnt - 1)))
+
+
+def read_traffic():
+    """HBM bytes per launch of the site kernel from the committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel)."""
+    path = os.path.join(HERE, "profiles", "tersoff_site_32768_traffic.json")
+    try:
+        with open(path) as fh:
+            return json.load(fh).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(lat, pos, params, budget_s, threads):
+    import oracle as O
+
+    ci, cj, _, _ = O.nl_build(lat, pos, params["R"] + params["D"] + 0.3)
+    O.tersoff_flat(lat, pos, params, pairs=(ci, cj), nthreads=threads)  # warm
+    times = []
+    t_end = time.perf_counter() + budget_s
+    while time.perf_counter() < t_end and len(times) < 5000:
+        t0 = time.perf_counter()
+        O.tersoff_flat(lat, pos, params, pairs=(ci, cj), nthreads=threads)
+        times.append(time.perf_counter() - t0)
+    return min(times), len(times), float(np.sum(times))
+
+
+def fit_throughput(n_struct, reps, seed=17):
+    """Batched loss+gradient evaluations per second over n_struct Si
+    structures (64-512 atoms, rattle 0.15 A, +-3 % strain), targets from the
+    engine at parameters perturbed by +-5 % (m held at 3), gradient at the
+    unperturbed T3 parameters; against a loop over single-structure calls."""
+    from akitsushima_amd.fit import TersoffFit, theta_from_params
+    from akitsushima_amd.published import TERSOFF_SI_T3
+    from akitsushima_amd.structures import diamond_batch
+
+    structs = diamond_batch(n_struct, 64, 512, 0.03, 0.15, seed=seed, rep_range=(2, 4))
+    theta0 = theta_from_params(dict(TERSOFF_SI_T3, lambda3=1.7322))
+    rng = np.random.default_rng(seed)
+    pert = theta0 * (1.0 + rng.uniform(-0.05, 0.05, 14))
+    pert[12] = 3.0
+    dummy = [{"lattice": l, "positions": p, "ref_energy": 0.0,
+              "ref_forces": np.zeros_like(p), "ref_stress": np.zeros((3, 3))}
+             for l, p, _ in structs]
+    _, _, (E, F, S) = TersoffFit(dummy).evaluate(pert, want_grad=False, want_outputs=True)
+    off = np.cumsum([0] + [len(p) for _, p, _ in structs])
+    data = [{"lattice": l, "positions": p, "ref_energy": E[k] / len(p),
+             "ref_forces": F[off[k]:off[k + 1]], "ref_stress": S[k]}
+            for k, (l, p, _) in enumerate(structs)]
+    fit = TersoffFit(data)
+    L, g = fit.loss_and_grad(theta0)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fit.loss_and_grad(theta0)
+    batched = reps / (time.perf_counter() - t0)
+    singles = [TersoffFit([d]) for d in data]
+    for f in singles:
+        f.loss_and_grad(theta0)
+    t0 = time.perf_counter()
+    Ls, gs = 0.0, np.zeros(14)
+    for f in singles:
+        l1, g1 = f.loss_and_grad(theta0)
+        Ls += l1
+        gs += g1
+    loop = 1.0 / (time.perf_counter() - t0)
+    agree = float(np.max(np.abs(gs - g)) / np.max(np.abs(g)))
+    return {"workload": f"{n_struct} diamond Si structures, 64-512 atoms, rattle 0.15 A, "
+                        "strain +-3 %, 14 Tersoff parameters",
+            "atoms": int(off[-1]), "gradients_per_s": batched,
+            "per_structure_loop_gradients_per_s": loop, "speedup_vs_loop": batched / loop,
+            "loss": L, "batched_vs_loop_grad_rel_diff": agree}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cells", type=int, default=16, help="n of the n^3 diamond cells")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fit-reps", type=int, default=20)
+    ap.add_argument("--no-fit", action="store_true")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="only the force calls (for rocprofv3 runs)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of it "
+              "(use --gpus 1)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from akitsushima_amd import NeighbourList, TersoffPotential
+    from akitsushima_amd import _lib
+    from akitsushima_amd.published import TERSOFF_SI_T3
+    from akitsushima_amd.structures import DIAMOND, cubic
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        return 1
+    dev = torch.device("cuda:0")
+    params = dict(TERSOFF_SI_T3)
+    lat, pos, types = cubic(DIAMOND, 5.431, args.cells, ["Si"] * 8, rattle=0.05, seed=0)
+    n = len(pos)
+    pot = TersoffPotential(params, species=["Si"])
+    nl = NeighbourList(pot.r_cut, skin_width=0.3)
+    nl.build(lat, pos)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        nl.build(lat, pos)
+    nl_build_ms = (time.perf_counter() - t0) / 5 * 1e3
+
+    d_pos = torch.tensor(pos, dtype=torch.float64, device=dev)
+    d_spc = torch.zeros(n, dtype=torch.int32, device=dev)
+    d_F = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    d_es = torch.empty(10, dtype=torch.float64, device=dev)
+    latc = np.ascontiguousarray(lat)
+    invc = np.ascontiguousarray(np.linalg.inv(lat))
+    P = _lib._pf64
+    stream = torch.cuda.current_stream(dev)
+    L = _lib.lib()
+    h = pot.handle()
+
+    def step():
+        _lib.check(L.ak_compute_device(h, nl.handle(), n, latc.ctypes.data_as(P),
+                                       invc.ctypes.data_as(P), d_pos.data_ptr(),
+                                       d_spc.data_ptr(), d_F.data_ptr(), d_es.data_ptr(),
+                                       stream.cuda_stream))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    _lib.check(L.ak_profile_enable(h, 1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    tot_ms = ctypes.c_double()
+    min_ms = ctypes.c_double()
+    cnt = ctypes.c_int64()
+    _lib.check(L.ak_profile_read(h, ctypes.byref(tot_ms), ctypes.byref(min_ms),
+                                 ctypes.byref(cnt)))
+    _lib.check(L.ak_profile_enable(h, 0))
+    elapsed = t1 - t0
+    value = n * args.steps / elapsed
+    es = d_es.cpu().numpy()
+    F = d_F.cpu().numpy()
+    if args.profile_only:
+        print(json.dumps({"profile_only": True, "atoms": n, "steps": args.steps,
+                          "ms_per_step": elapsed / args.steps * 1e3}))
+        return 0
+
+    # host round trip (positions in, forces out over PCIe) for reference
+    e_h, f_h, s_h = pot.compute(lat, pos, types, neigh_list=nl)
+    t0 = time.perf_counter()
+    for _ in range(10):
+        pot.compute(lat, pos, types, neigh_list=nl)
+    host_rate = n * 10 / (time.perf_counter() - t0)
+    assert abs(e_h - es[0]) <= 1e-10 * abs(es[0]), (e_h, es[0])
+
+    n_pairs, n_trip = geometry_counts(lat, pos, pot.r_cut)
+    flops = tersoff_flops(n_pairs, n_trip, params["lambda3"])
+    avg_site_s = tot_ms.value / max(cnt.value, 1) * 1e-3
+    achieved = flops / avg_site_s / 1e12
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                "traffic": read_traffic(),
+                "kernel": "k_tersoff_site<4>", "kernel_avg_us": avg_site_s * 1e6,
+                "kernel_min_us": min_ms.value * 1e3, "launches_timed": int(cnt.value),
+                "flops_per_launch": flops,
+                "flops_per_atom": flops / n,
+                "compute_unit": "FP64 VALU (MI355X FP64 vector peak = 78.6 TFLOP/s)"}
+
+    out = {"metric": METRIC, "value": value, "unit": "atom-force-evals/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (rattled diamond Si, seed 0)",
+           "config": {"workload": f"Tersoff Si T3 diamond {args.cells}^3 cells ({n} atoms), "
+                                  "a=5.431 A, rattle 0.05 A seed 0, R=2.85 D=0.15, "
+                                  "force call with a prebuilt list (skin 0.3 A)",
+                      "atoms": n, "pairs_within_rc": n_pairs, "triplets": n_trip},
+           "energy_per_atom": float(es[0] / n),
+           "sum_force_max": float(np.abs(F.sum(axis=0)).max()),
+           "nl_build_ms": nl_build_ms,
+           "host_roundtrip_atom_evals_per_s": host_rate,
+           "roofline": roofline}
+
+    if not args.no_cpu:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        best, ncalls, spent = cpu_baseline(lat, pos, params, args.cpu_seconds, threads)
+        out["cpu_baseline"] = {
+            "value": n / best, "unit": "atom-force-evals/s", "cores": threads, "kind": "port",
+            "sample": f"best of {ncalls} warm Tersoff force calls on the same {n}-atom "
+                      f"workload with a prebuilt list ({spent:.1f} s of CPU time), C/OpenMP "
+                      "restatement of the reference Numba path (oracle/refcpu.c)"}
+        out["gpu_over_cpu"] = value / (n / best)
+    if not args.no_fit:
+        out["fit"] = fit_throughput(256, args.fit_reps)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -166,6 +166,16 @@ int ak_tersoff_loss_grad(ak_potential *pot, int32_t n_struct, const int64_t *ato
                          const double *weights5, double *loss, double *grad14,
                          double *energies, double *forces, double *stresses);
 
+/* ---- Measurement -------------------------------------------------------- */
+
+/* No counterpart in the reference: timing support for benchmarks.  While
+ * enabled, every force call records HIP events around its dominant kernel
+ * (the Tersoff/SW site kernel, the EAM force pass) on the launch stream;
+ * ak_profile_read waits for them and returns the summed and the smallest
+ * duration in milliseconds and the number of launches, then resets. */
+int ak_profile_enable(ak_potential *pot, int32_t on);
+int ak_profile_read(ak_potential *pot, double *total_ms, double *min_ms, int64_t *count);
+
 #ifdef __cplusplus
 }
 #endif
