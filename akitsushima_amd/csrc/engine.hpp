@@ -71,6 +71,9 @@ struct SetDev {
   int n_atoms = 0;
   int n_struct = 0;
   int n_cells = 0;
+  double setup_r_list = -1.0;
+  bool setup_checked = true;
+  bool setup_had_inv = false;
   std::vector<Geom> geoms;
   std::vector<int> atom_off;  // n_struct + 1
   std::vector<int> uploaded_off;  // atom_off of the uploaded sid array

@@ -44,7 +44,15 @@ __device__ __forceinline__ void te_fc(double r, double R, double D, double& fc, 
 }
 
 // exp(l3^m du^m) and its derivative in du (the zeta-term factor of
-// engine/tersoff.py:161-202); m = 3 and m = 1 are multiplied out.
+// engine/tersoff.py:161-202); m = 3 and m = 1 are multiplied out, other m go
+// through pow out of line (rare, and kept out of the unrolled partner loops).
+__device__ __attribute__((noinline)) void te_exp_general(double l3, double m, double du,
+                                                         double& e, double& de) {
+  double lm = pow(l3, m);
+  e = nb_exp(lm * pow(du, m));
+  de = e * m * lm * pow(du, m - 1.0);
+}
+
 __device__ __forceinline__ void te_exp(double l3, double m, double du, double& e, double& de) {
   if (m == 3.0) {
     if (l3 == 0.0) {
@@ -59,11 +67,12 @@ __device__ __forceinline__ void te_exp(double l3, double m, double du, double& e
     e = nb_exp(l3 * du);
     de = e * l3;
   } else {
-    double lm = pow(l3, m);
-    e = nb_exp(lm * pow(du, m));
-    de = e * m * lm * pow(du, m - 1.0);
+    te_exp_general(l3, m, du, e, de);
   }
 }
+
+// pow for a base that may be non-positive (reference semantics), out of line
+__device__ __attribute__((noinline)) double pow_any(double x, double y) { return pow(x, y); }
 
 // x^e with small integer exponents multiplied out.
 __device__ __forceinline__ double xpow(double x, double e) {
@@ -101,16 +110,23 @@ __device__ __forceinline__ void mic_fast(const Geom& g, const double* __restrict
   dz = sx * g.lat[2] + sy * g.lat[5] + sz * g.lat[8];
 }
 
-// Wavefront sums of 10 values (energy, virial); lane 0 of each wavefront
-// writes them to partial[(block * 4 + wave) * 10 ...] (blocks of 256
-// threads are one launch chunk: atoms of one structure).
+// Block sums of 10 values (energy, virial): DPP wavefront sums, one LDS
+// exchange of the four wavefront totals, combined in fixed order; thread q < 10
+// writes value q to partial[q * n_blocks + block] (blocks of 256 threads are
+// one launch chunk: atoms of one structure).
 __device__ __forceinline__ void wave_partial(double (&v)[10], double* __restrict__ partial) {
+  __shared__ double sm[10][4];
 #pragma unroll
   for (int q = 0; q < 10; ++q) v[q] = wave_sum(v[q]);
+  const int wid = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
-    double* o = partial + 10 * ((size_t)blockIdx.x * 4 + (threadIdx.x >> 6));
 #pragma unroll
-    for (int q = 0; q < 10; ++q) o[q] = v[q];
+    for (int q = 0; q < 10; ++q) sm[q][wid] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < 10) {
+    const int q = threadIdx.x;
+    partial[(size_t)q * gridDim.x + blockIdx.x] = ((sm[q][0] + sm[q][1]) + sm[q][2]) + sm[q][3];
   }
 }
 
@@ -225,7 +241,7 @@ __global__ __launch_bounds__(256) void k_tersoff_site(
   if (pair_ok && zeta > 0.0 && PL[7] != 0.0) {
     const double beta = PL[7], nn = PL[8];
     double x = beta * zeta;
-    double bz = x > 0.0 ? exp(nn * log(x)) : pow(x, nn);
+    double bz = x > 0.0 ? exp(nn * log(x)) : pow_any(x, nn);
     b = exp((-0.5 / nn) * log1p(bz));
     db = -0.5 * b * bz / (zeta * (1.0 + bz));
   }
@@ -407,22 +423,70 @@ __global__ __launch_bounds__(256) void k_sw_site(
   block_partial(e, Gx, Gy, Gz, ux, uy, uz, partial);
 }
 
-// F_a = sum_k G[a, k] - G[rev(a, k)]
+// Per-structure energy and stress of structure s by one block of 256
+// threads: the chunk partials of s in fixed order; out[s] = [E, W/V (9)].
+__device__ __forceinline__ void struct_reduce_block(int s, const Geom* __restrict__ geoms,
+                                                    const int* __restrict__ first,
+                                                    int n_chunks,
+                                                    const double* __restrict__ partial,
+                                                    double* __restrict__ out) {
+  __shared__ double sm[10][4];
+  const int c0 = first[s], c1 = first[s + 1];
+  double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int c = c0 + threadIdx.x; c < c1; c += 256) {
+#pragma unroll
+    for (int q = 0; q < 10; ++q) v[q] += partial[(size_t)q * n_chunks + c];
+  }
+#pragma unroll
+  for (int q = 0; q < 10; ++q) v[q] = wave_sum(v[q]);
+  const int wid = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int q = 0; q < 10; ++q) sm[q][wid] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < 10) {
+    const int q = threadIdx.x;
+    double t = ((sm[q][0] + sm[q][1]) + sm[q][2]) + sm[q][3];
+    out[10 * (size_t)s + q] = q == 0 ? t : t / geoms[s].volume;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_struct_final(const Geom* __restrict__ geoms,
+                                                      const int* __restrict__ first,
+                                                      int n_chunks,
+                                                      const double* __restrict__ partial,
+                                                      double* __restrict__ out) {
+  struct_reduce_block(blockIdx.x, geoms, first, n_chunks, partial, out);
+}
+
+// F_a = sum_k G[a, k] - G[rev(a, k)].  Blocks past the atom blocks reduce
+// the site kernel's partials to per-structure energy and stress (the site
+// kernel has completed, so any block may read them).
 template <int L>
 __global__ __launch_bounds__(256) void k_gather(int n, const int* __restrict__ cnt,
                                                 const int* __restrict__ rev, int stride,
                                                 const double* __restrict__ G,
-                                                double* __restrict__ F) {
+                                                double* __restrict__ F, int atom_blocks,
+                                                const Geom* __restrict__ geoms,
+                                                const int* __restrict__ first, int n_chunks,
+                                                const double* __restrict__ partial,
+                                                double* __restrict__ out) {
+  if ((int)blockIdx.x >= atom_blocks) {
+    struct_reduce_block((int)blockIdx.x - atom_blocks, geoms, first, n_chunks, partial, out);
+    return;
+  }
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
   const int a = gt / L;
   const int k = gt & (L - 1);
   const bool live = a < n;
   double fx = 0.0, fy = 0.0, fz = 0.0;
-  if (live && k < cnt[a]) {
-    const size_t plane = (size_t)n * stride;
+  if (live) {
     const size_t s = (size_t)a * stride + k;
-    const int r = rev[s];
-    if (r >= 0) {
+    const int c = cnt[a];
+    const int r = rev[s];  // issued alongside cnt[]; used when k < c
+    if (k < c && r >= 0) {
+      const size_t plane = (size_t)n * stride;
       const size_t rs = (size_t)r;
       fx = G[s] - G[rs];
       fy = G[plane + s] - G[plane + rs];
@@ -624,30 +688,6 @@ __global__ __launch_bounds__(256) void k_eam_force(
   wave_partial(v, partial);
 }
 
-// Per-structure energy and stress: one wavefront per structure sums the
-// block partials of its chunks in order; out[s] = [E, W/V (9)].
-__global__ __launch_bounds__(256) void k_struct_final(const Geom* __restrict__ geoms,
-                                                      const int* __restrict__ first,
-                                                      const double* __restrict__ partial,
-                                                      double* __restrict__ out) {
-  __shared__ double smem[10 * 16];
-  const int s = blockIdx.x;
-  const int c0 = 4 * first[s], c1 = 4 * first[s + 1];  // 4 wavefront partials per chunk
-  double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int c = c0 + threadIdx.x; c < c1; c += 256) {
-#pragma unroll
-    for (int q = 0; q < 10; ++q) v[q] += partial[10 * (size_t)c + q];
-  }
-  block_sum<10>(v, smem);
-  if (threadIdx.x == 0) {
-    const double vol = geoms[s].volume;
-    double* o = out + 10 * (size_t)s;
-    o[0] = v[0];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) o[1 + q] = v[1 + q] / vol;
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Host dispatch
 // ---------------------------------------------------------------------------
@@ -667,8 +707,8 @@ static void launch_timed(Potential& pot, K kernel, dim3 grid, dim3 block, hipStr
 
 template <int L>
 static void launch_bond_kernels(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
-                                const int* d_spc, double* d_F, SetDev::Chunks& ch,
-                                hipStream_t st) {
+                                const int* d_spc, double* d_F, double* d_out,
+                                SetDev::Chunks& ch, hipStream_t st) {
   const int n = set.n_atoms;
   const int single = set.n_struct == 1 ? 1 : 0;
   double* G = pot.G.as<double>();
@@ -688,9 +728,12 @@ static void launch_bond_kernels(Potential& pot, NList& nl, SetDev& set, const do
                  ch.partial.as<double>());
   }
   const int64_t threads = (int64_t)n * L;
-  const unsigned grid = (unsigned)((threads + 255) / 256);
-  hipLaunchKernelGGL(k_gather<L>, dim3(grid), dim3(256), 0, st, n, nl.cnt.as<int>(),
-                     nl.rev.as<int>(), nl.stride, (const double*)G, d_F);
+  const int atom_blocks = (int)((threads + 255) / 256);
+  hipLaunchKernelGGL(k_gather<L>, dim3(atom_blocks + set.n_struct), dim3(256), 0, st, n,
+                     (const int*)nl.cnt.as<int>(), (const int*)nl.rev.as<int>(), nl.stride,
+                     (const double*)G, d_F, atom_blocks, (const Geom*)set.d_geoms.as<Geom>(),
+                     (const int*)ch.first.as<int>(), ch.n, (const double*)ch.partial.as<double>(),
+                     d_out);
 }
 
 template <int LE>
@@ -741,18 +784,19 @@ void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
     } else {
       pot.G.ensure(sizeof(double) * 3 * (size_t)n * nl.stride);
       switch (lanes) {
-        case 4: launch_bond_kernels<4>(pot, nl, set, d_pos, d_spc, d_F, ch, st); break;
-        case 8: launch_bond_kernels<8>(pot, nl, set, d_pos, d_spc, d_F, ch, st); break;
-        case 16: launch_bond_kernels<16>(pot, nl, set, d_pos, d_spc, d_F, ch, st); break;
-        case 32: launch_bond_kernels<32>(pot, nl, set, d_pos, d_spc, d_F, ch, st); break;
-        case 64: launch_bond_kernels<64>(pot, nl, set, d_pos, d_spc, d_F, ch, st); break;
+        case 4: launch_bond_kernels<4>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, st); break;
+        case 8: launch_bond_kernels<8>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, st); break;
+        case 16: launch_bond_kernels<16>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, st); break;
+        case 32: launch_bond_kernels<32>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, st); break;
+        case 64: launch_bond_kernels<64>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, st); break;
         default: throw Error(AK_ERR_STATE, "unsupported neighbour stride");
       }
     }
   }
-  hipLaunchKernelGGL(k_struct_final, dim3(set.n_struct), dim3(256), 0, st,
-                     set.d_geoms.as<Geom>(), (const int*)ch.first.as<int>(),
-                     (const double*)ch.partial.as<double>(), d_out);
+  if (pot.kind == KIND_EAM || n == 0)
+    hipLaunchKernelGGL(k_struct_final, dim3(set.n_struct), dim3(256), 0, st,
+                       set.d_geoms.as<Geom>(), (const int*)ch.first.as<int>(), ch.n,
+                       (const double*)ch.partial.as<double>(), d_out);
 }
 
 }  // namespace ak

@@ -159,6 +159,9 @@ def main():
     ap.add_argument("--no-fit", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the force calls (for rocprofv3 runs)")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="with --profile-only: also run a MiB-sized elementwise copy "
+                         "kernel (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of it "
@@ -198,11 +201,16 @@ def main():
     L = _lib.lib()
     h = pot.handle()
 
+    call_args = (h, nl.handle(), n, latc.ctypes.data_as(P), invc.ctypes.data_as(P),
+                 ctypes.c_void_p(d_pos.data_ptr()), ctypes.c_void_p(d_spc.data_ptr()),
+                 ctypes.c_void_p(d_F.data_ptr()), ctypes.c_void_p(d_es.data_ptr()),
+                 ctypes.c_void_p(stream.cuda_stream))
+    compute = L.ak_compute_device
+
     def step():
-        _lib.check(L.ak_compute_device(h, nl.handle(), n, latc.ctypes.data_as(P),
-                                       invc.ctypes.data_as(P), d_pos.data_ptr(),
-                                       d_spc.data_ptr(), d_F.data_ptr(), d_es.data_ptr(),
-                                       stream.cuda_stream))
+        rc = compute(*call_args)
+        if rc:
+            _lib.check(rc)
 
     for _ in range(args.warmup):
         step()
@@ -225,8 +233,20 @@ def main():
     es = d_es.cpu().numpy()
     F = d_F.cpu().numpy()
     if args.profile_only:
+        if args.copy_mb > 0:
+            # calibration for counter-based traffic: an elementwise kernel that
+            # reads and writes copy_mb MiB each
+            # (larger than the last-level cache, so every byte comes from HBM)
+            x = torch.ones(args.copy_mb * 131072, dtype=torch.float64, device=dev)
+            y = torch.empty_like(x)
+            for _ in range(5):
+                torch.neg(x, out=y)
+                torch.neg(y, out=x)
+            torch.cuda.synchronize()
         print(json.dumps({"profile_only": True, "atoms": n, "steps": args.steps,
-                          "ms_per_step": elapsed / args.steps * 1e3}))
+                          "ms_per_step": elapsed / args.steps * 1e3,
+                          "kernel_avg_us": tot_ms.value / max(cnt.value, 1) * 1e3,
+                          "copy_mb": args.copy_mb}))
         return 0
 
     # host round trip (positions in, forces out over PCIe) for reference
