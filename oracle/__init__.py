@@ -23,8 +23,17 @@ after the reference's Python glue:
 * ``tersoff_loss``: fitting/loss.py:51-248 (squared loss)
 * ``nve_run``: NVESimulation.run, solvers/molecular_dynamics.py:1969-1996
 
-Parity pinning of every function is listed in DESIGN-level docs; each test
-file under tests/ names what pins it.
+What pins each function (none is left unpinned):
+
+* cell list: brute-force O(N^2) pair sets (tests/test_oracle_derivatives.py)
+* Tersoff flat / multi, SW, Zhou EAM: the reference's ledger energies
+  (benchmarks/adr0047.jsonl, tests/golden/), published cohesive energies,
+  brute-force double sums, 4th-order finite differences of forces and stress
+  (tests/test_oracle_golden.py, tests/test_oracle_derivatives.py)
+* tersoff_loss: closed-form normalisation on hand-set residuals, and an
+  independent torch-autograd restatement (tests/test_fit_reference.py)
+* mb_velocities, nve_run: zero COM momentum and exact temperature; energy
+  conservation and time reversibility.
 """
 from __future__ import annotations
 
