@@ -44,6 +44,9 @@ SIGNATURES = {
     "ak_eam_create": (ctypes.c_int, [_i32, _i32, _pf64, _pi32, _f64, _phandle]),
     "ak_potential_destroy": (ctypes.c_int, [_handle]),
     "ak_potential_set_lanes": (ctypes.c_int, [_handle, _i32]),
+    "ak_potential_set_launch": (ctypes.c_int, [_handle, _i32, _i32, _i32]),
+    "ak_autotune_batched": (ctypes.c_int, [_handle, _i32, _pi64, _pf64, _pf64, _pi32, _i32,
+                                           _pi32, _pf64, _i32, _pi32, _pf64, _pi32]),
     "ak_nlist_create": (ctypes.c_int, [_f64, _f64, _phandle]),
     "ak_nlist_destroy": (ctypes.c_int, [_handle]),
     "ak_nlist_build": (ctypes.c_int, [_handle, _i32, _pf64, _pf64, _pf64]),

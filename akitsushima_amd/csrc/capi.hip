@@ -4,6 +4,7 @@
 #include "../../include/akitsushima_amd.h"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -260,6 +261,21 @@ int ak_potential_set_lanes(ak_potential* pot, int32_t lanes) {
   return AK_OK;
 }
 
+int ak_potential_set_launch(ak_potential* pot, int32_t bond_tile, int32_t block_threads,
+                            int32_t grid_cap) {
+  if (!pot) return fail(AK_ERR_ARG, "null potential");
+  if (bond_tile < 0 || bond_tile > 2)
+    return fail(AK_ERR_ARG, "bond_tile must be 0 (automatic), 1 (lane per bond) or 2 (lane "
+                            "per bond and partner leg)");
+  if (block_threads != 0 && block_threads != 64 && block_threads != 128 && block_threads != 256)
+    return fail(AK_ERR_ARG, "block_threads must be 0 (default 256), 64, 128 or 256");
+  if (grid_cap < 0) return fail(AK_ERR_ARG, "grid_cap must be >= 0");
+  pot->pot.bond_tile = bond_tile;
+  pot->pot.block = block_threads ? block_threads : 256;
+  pot->pot.grid_cap = grid_cap;
+  return AK_OK;
+}
+
 int ak_nlist_create(double r_cut, double skin, ak_nlist** out) {
   return guarded([&] {
     AK_REQUIRE(out, AK_ERR_ARG, "null output handle");
@@ -446,6 +462,112 @@ int ak_compute_batched(ak_potential* P, int32_t S, const int64_t* off, const dou
       energies[s] = out[10 * s];
       if (stresses) std::memcpy(stresses + 9 * s, out.data() + 10 * s + 1, 9 * sizeof(double));
     }
+  });
+}
+
+int ak_autotune_batched(ak_potential* P, int32_t S, const int64_t* off, const double* lattices,
+                        const double* positions, const int32_t* species, int32_t reps,
+                        int32_t* best, double* best_ms, int32_t capacity, int32_t* shapes,
+                        double* times_ms, int32_t* n_tried) {
+  return guarded([&] {
+    AK_REQUIRE(P && S >= 1 && off && lattices && best && best_ms, AK_ERR_ARG, "bad arguments");
+    const int n = (int)off[S];
+    AK_REQUIRE(n > 0 && positions, AK_ERR_ARG, "the batch holds no atoms");
+    check_finite(positions, 3 * (size_t)n, "positions");
+    hipStream_t st = P->stream;
+    stage_species(P, n, species);
+    P->d_pos.ensure(sizeof(double) * 3 * n);
+    P->d_F.ensure(sizeof(double) * 3 * n);
+    P->d_out.ensure(sizeof(double) * 10 * S);
+    AK_HIP(hipMemcpyAsync(P->d_pos.p, positions, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+                          st));
+    ak_nlist& B = P->batch;
+    B.nl.r_cut = P->pot.r_cut;
+    B.nl.skin = 0.0;
+    B.nl.r_list = P->pot.r_cut;
+    B.set.setup(S, off, lattices, nullptr, B.nl.r_list, st);
+    nl_build(B.nl, B.set, P->d_pos.as<double>(), st);
+    P->batch_off.assign(off, off + S + 1);
+
+    // candidate launch shapes {bond_tile, block, grid_cap, eam lanes}
+    int dev = 0, cus = 256;
+    AK_HIP(hipGetDevice(&dev));
+    AK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    std::vector<std::array<int, 4>> cand;
+    const int blocks[3] = {64, 128, 256};
+    const int caps[3] = {0, 2 * cus, 4 * cus};
+    if (P->pot.kind == KIND_EAM) {
+      for (int lanes : {8, 16, 32, 64})
+        for (int b : blocks)
+          if (lanes <= b) cand.push_back({0, b, 0, lanes});
+    } else {
+      const int L = B.nl.stride;
+      std::vector<int> tiles = {1};
+      if (P->pot.kind == KIND_TERSOFF && L * L <= 64) tiles.push_back(2);
+      for (int t : tiles)
+        for (int b : blocks)
+          for (int c : caps)
+            if ((t == 2 ? L * L : L) <= b) cand.push_back({t, b, c, 0});
+    }
+    Potential& pot = P->pot;
+    const int saved[4] = {pot.bond_tile, pot.block, pot.grid_cap, pot.eam_lanes};
+    const bool saved_prof = pot.prof_on;
+    pot.prof_on = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    AK_HIP(hipEventCreate(&e0));
+    AK_HIP(hipEventCreate(&e1));
+    double best_t = 1e300;
+    int bi = -1;
+    int tried = 0;
+    const int R = std::max(1, (int)reps);
+    try {
+      for (size_t k = 0; k < cand.size(); ++k) {
+        pot.bond_tile = cand[k][0];
+        pot.block = cand[k][1];
+        pot.grid_cap = cand[k][2];
+        pot.eam_lanes = cand[k][3];
+        // first call: chunk tables for this block size, caches warm
+        compute_forces(pot, B.nl, B.set, P->d_pos.as<double>(), P->d_spc.as<int>(),
+                       P->d_F.as<double>(), P->d_out.as<double>(), st);
+        AK_HIP(hipEventRecord(e0, st));
+        for (int r = 0; r < R; ++r)
+          compute_forces(pot, B.nl, B.set, P->d_pos.as<double>(), P->d_spc.as<int>(),
+                         P->d_F.as<double>(), P->d_out.as<double>(), st);
+        AK_HIP(hipEventRecord(e1, st));
+        AK_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        AK_HIP(hipEventElapsedTime(&ms, e0, e1));
+        const double t = (double)ms / R;
+        if (shapes && times_ms && tried < capacity) {
+          for (int q = 0; q < 4; ++q) shapes[4 * tried + q] = cand[k][q];
+          times_ms[tried] = t;
+        }
+        ++tried;
+        if (t < best_t) {
+          best_t = t;
+          bi = (int)k;
+        }
+      }
+    } catch (...) {
+      pot.bond_tile = saved[0];
+      pot.block = saved[1];
+      pot.grid_cap = saved[2];
+      pot.eam_lanes = saved[3];
+      pot.prof_on = saved_prof;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    pot.prof_on = saved_prof;
+    pot.bond_tile = cand[bi][0];
+    pot.block = cand[bi][1];
+    pot.grid_cap = cand[bi][2];
+    pot.eam_lanes = cand[bi][3];
+    for (int q = 0; q < 4; ++q) best[q] = cand[bi][q];
+    *best_ms = best_t;
+    if (n_tried) *n_tried = tried;
   });
 }
 

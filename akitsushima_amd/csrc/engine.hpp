@@ -144,7 +144,13 @@ struct Potential {
   DevBuf G, aux, out;
   // launch shape chosen by the autotuner (0 = default)
   int eam_lanes = 0;
+  // Tersoff site kernel form: 2 = one lane per (bond, partner) pair (list
+  // strides up to 8), otherwise one lane per bond
+  int bond_tile = 0;
+  // threads per block of the site / force kernels (64, 128, 256), and the
+  // largest grid (0 = one block per chunk; else blocks stride over chunks)
   int block = 256;
+  int grid_cap = 0;
   // fitting session (fit.hip)
   std::shared_ptr<void> fit;
   // optional HIP-event timing of the dominant kernel (site kernel for

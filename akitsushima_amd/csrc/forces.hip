@@ -110,15 +110,16 @@ __device__ __forceinline__ void mic_fast(const Geom& g, const double* __restrict
   dz = sx * g.lat[2] + sy * g.lat[5] + sz * g.lat[8];
 }
 
-// Block sums of 10 values (energy, virial): DPP wavefront sums, one LDS
-// exchange of the four wavefront totals, combined in fixed order; thread q < 10
-// writes value q to partial[q * n_blocks + block] (blocks of 256 threads are
-// one launch chunk: atoms of one structure).
-__device__ __forceinline__ void wave_partial(double (&v)[10], double* __restrict__ partial) {
+// Block sums of 10 values (energy, virial) of launch chunk c: DPP wavefront
+// sums, one LDS exchange of the wavefront totals, combined in fixed order;
+// thread q < 10 writes value q to partial[q * n_chunks + c].  A chunk is the
+// atoms of one block (of one structure).
+__device__ __forceinline__ void wave_partial(double (&v)[10], double* __restrict__ partial, int c,
+                                             int n_chunks) {
   __shared__ double sm[10][4];
 #pragma unroll
   for (int q = 0; q < 10; ++q) v[q] = wave_sum(v[q]);
-  const int wid = threadIdx.x >> 6;
+  const int wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
   if ((threadIdx.x & 63) == 0) {
 #pragma unroll
     for (int q = 0; q < 10; ++q) sm[q][wid] = v[q];
@@ -126,17 +127,21 @@ __device__ __forceinline__ void wave_partial(double (&v)[10], double* __restrict
   __syncthreads();
   if (threadIdx.x < 10) {
     const int q = threadIdx.x;
-    partial[(size_t)q * gridDim.x + blockIdx.x] = ((sm[q][0] + sm[q][1]) + sm[q][2]) + sm[q][3];
+    double t = sm[q][0];
+    for (int w = 1; w < nw; ++w) t += sm[q][w];
+    partial[(size_t)q * n_chunks + c] = t;
   }
+  __syncthreads();  // sm is written again by the block's next chunk
 }
 
 // Bond energies and virials (e, -G (x) u) of the lanes.
 __device__ __forceinline__ void block_partial(double e, double Gx, double Gy, double Gz,
                                               double ux, double uy, double uz,
-                                              double* __restrict__ partial) {
+                                              double* __restrict__ partial, int c,
+                                              int n_chunks) {
   double v[10] = {e,        -Gx * ux, -Gx * uy, -Gx * uz, -Gy * ux,
                   -Gy * uy, -Gy * uz, -Gz * ux, -Gz * uy, -Gz * uz};
-  wave_partial(v, partial);
+  wave_partial(v, partial, c, n_chunks);
 }
 
 // ---------------------------------------------------------------------------
@@ -149,17 +154,20 @@ __device__ __forceinline__ void block_partial(double e, double Gx, double Gy, do
 // the table semantics of engine/tersoff.py:528-855 (the flat single-species
 // form is the ns = 1 case with m = 3, gamma = 1).
 // ---------------------------------------------------------------------------
-template <int L>
-__global__ __launch_bounds__(256) void k_tersoff_site(
+template <int L, bool SP1>
+__device__ __forceinline__ void k_tersoff_site_chunk(int c, int n_chunks, 
     int n, const double* __restrict__ pos, const int* __restrict__ sid,
-    const Geom* __restrict__ geoms, int single, const int* __restrict__ spc, int ns,
+    const Geom* __restrict__ geoms, int single, const int* __restrict__ spc, int ns_arg,
     const double* __restrict__ tb, const double* __restrict__ tri, double rc,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial) {
-  const int a = chunk_rng[2 * blockIdx.x] + (int)threadIdx.x / L;
+  // one species: every table row is the first, read through uniform (scalar)
+  // loads, and the leg cutoff equals the pair cutoff
+  const int ns = SP1 ? 1 : ns_arg;
+  const int a = chunk_rng[2 * c] + (int)threadIdx.x / L;
   const int k = threadIdx.x & (L - 1);
   const int seg = (threadIdx.x & 63) & ~(L - 1);
-  const bool live = a < chunk_rng[2 * blockIdx.x + 1];
+  const bool live = a < chunk_rng[2 * c + 1];
   const int ac = live ? a : 0;
   const int nk = live ? cnt[ac] : 0;
   const bool has = live && k < nk;
@@ -247,8 +255,8 @@ __global__ __launch_bounds__(256) void k_tersoff_site(
   }
   double e = 0.0, dVdr = 0.0, cown = 0.0;
   if (pair_ok) {
-    double fc2, dfc2;
-    te_fc(r, P2[5], P2[6], fc2, dfc2);
+    double fc2 = fcl, dfc2 = dfcl;
+    if (!SP1) te_fc(r, P2[5], P2[6], fc2, dfc2);
     if (fc2 != 0.0 || dfc2 != 0.0) {
       double fr = P2[1] * nb_exp(-P2[3] * r);
       double fa = -P2[2] * nb_exp(-P2[4] * r);
@@ -339,7 +347,189 @@ __global__ __launch_bounds__(256) void k_tersoff_site(
     G[plane + slot] = Gy;
     G[2 * plane + slot] = Gz;
   }
-  block_partial(e, Gx, Gy, Gz, ux, uy, uz, partial);
+  block_partial(e, Gx, Gy, Gz, ux, uy, uz, partial, c, n_chunks);
+}
+
+template <int L, bool SP1>
+__global__ __launch_bounds__(256) void k_tersoff_site(int n_chunks, int n, const double* __restrict__ pos, const int* __restrict__ sid,
+    const Geom* __restrict__ geoms, int single, const int* __restrict__ spc, int ns_arg,
+    const double* __restrict__ tb, const double* __restrict__ tri, double rc,
+    const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
+    const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial) {
+  for (int c = blockIdx.x; c < n_chunks; c += gridDim.x)
+    k_tersoff_site_chunk<L, SP1>(c, n_chunks, n, pos, sid, geoms, single, spc, ns_arg, tb, tri, rc, cnt, nbr, stride, chunk_rng, G, partial);
+}
+
+// ---------------------------------------------------------------------------
+// Tersoff site kernel, one lane per (bond, partner) pair.
+// L*L lanes serve one atom (L = list stride, L*L <= 64): lane (k, p) of the
+// atom handles bond k and its partner leg s = (k + p) mod L.  Lane p = 0 of
+// each bond group has no partner and evaluates the bond order and the
+// two-body term of bond k instead.  zeta_k and the gradient of bond k are
+// sums over the group's L lanes (DPP butterflies); the coefficient of the
+// partner's bond order comes from lane (s, 0) by one shuffle.  Same
+// arithmetic and gating as k_tersoff_site; only the summation order of
+// zeta and of the gradient differs.  Four times more wavefronts than the
+// lane-per-bond form for the same atoms, so the latency of the partner
+// loop is hidden by occupancy instead of being serialised in one lane.
+// ---------------------------------------------------------------------------
+struct Bond {
+  double x = 0.0, y = 0.0, z = 0.0, r = 0.0, rinv = 0.0, fcl = 0.0, dfcl = 0.0;
+  int j = 0, sj = 0;
+  bool act = false;
+};
+
+__device__ __forceinline__ Bond te_bond(const Geom& g, const double* __restrict__ pos,
+                                        const int* __restrict__ spc, int ns,
+                                        const double* __restrict__ tb, double rc, int a, int sa,
+                                        int jr, bool has) {
+  Bond b;
+  b.j = has ? jr : a;
+  b.sj = ns > 1 ? spc[b.j] : 0;
+  if (has) {
+    mic_fast(g, pos, a, b.j, b.x, b.y, b.z);
+    b.r = sqrt(b.x * b.x + b.y * b.y + b.z * b.z);
+    b.act = b.r < rc && b.r > 1e-12;
+    b.rinv = b.act ? 1.0 / b.r : 0.0;
+    const double* PL = tb + ((size_t)sa * ns + b.sj) * 9;
+    if (b.act && PL[0] != 0.0) te_fc(b.r, PL[5], PL[6], b.fcl, b.dfcl);
+  }
+  return b;
+}
+
+template <int L, bool SP1>
+__device__ __forceinline__ void k_tersoff_trip_chunk(int c, int n_chunks, 
+    int n, const double* __restrict__ pos, const int* __restrict__ sid,
+    const Geom* __restrict__ geoms, int single, const int* __restrict__ spc, int ns_arg,
+    const double* __restrict__ tb, const double* __restrict__ tri, double rc,
+    const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
+    const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial) {
+  static_assert(L * L <= 64, "one atom must fit a wavefront");
+  constexpr int LL = L * L;
+  const int ns = SP1 ? 1 : ns_arg;
+  const int a = chunk_rng[2 * c] + (int)threadIdx.x / LL;
+  const int k = ((int)threadIdx.x / L) & (L - 1);
+  const int p = threadIdx.x & (L - 1);
+  const int s = (k + p) & (L - 1);
+  const int wl = threadIdx.x & 63;
+  const int gseg = wl & ~(L - 1);    // first lane of this bond group
+  const int aseg = wl & ~(LL - 1);   // first lane of this atom
+  const bool live = a < chunk_rng[2 * c + 1];
+  const int ac = live ? a : 0;
+  const int nk = live ? cnt[ac] : 0;
+  const bool has = live && k < nk;
+  const bool has_s = live && p != 0 && s < nk;
+  const size_t slot = (size_t)ac * stride + k;
+  const int jr = live ? nbr[slot] : 0;
+  const int jsr = live ? nbr[(size_t)ac * stride + s] : 0;
+  const int sa = ns > 1 ? spc[ac] : 0;
+  const Geom& g = geoms[single ? 0 : sid[ac]];
+  const Bond u = te_bond(g, pos, spc, ns, tb, rc, ac, sa, jr, has);
+  const Bond w = te_bond(g, pos, spc, ns, tb, rc, ac, sa, jsr, has_s);
+
+  const int lo = ac < u.j ? sa : u.sj, hi = ac < u.j ? u.sj : sa;
+  const double* P2 = tb + ((size_t)lo * ns + hi) * 9;
+  const double* PL = tb + ((size_t)sa * ns + u.sj) * 9;
+  const bool pair_ok = u.act && P2[0] != 0.0;
+
+  // angle (a; k, s): cos, g, g', exponential factor and its derivative
+  const bool tri_ok = p != 0 && u.act && w.act;
+  const double* TA = tri + (((size_t)sa * ns + u.sj) * ns + w.sj) * 7;
+  double cs = 0.0, gA = 0.0, dgA = 0.0, eA = 0.0, deA = 0.0, zt = 0.0;
+  if (tri_ok) {
+    cs = (u.x * w.x + u.y * w.y + u.z * w.z) * u.rinv * w.rinv;
+    if (TA[0] != 0.0) {
+      double c = TA[1], d = TA[2], hc = TA[3] - cs;
+      double c2 = c * c, d2 = d * d;
+      double iq = 1.0 / (d2 + hc * hc);
+      gA = 1.0 + c2 / d2 - c2 * iq;
+      dgA = -2.0 * c2 * hc * iq * iq;
+      te_exp(TA[4], TA[5], u.r - w.r, eA, deA);
+      if (pair_ok && w.fcl != 0.0) zt = TA[6] * w.fcl * gA * eA;
+    }
+  }
+  const double zeta = seg_sum<L>(zt);
+
+  // bond order and two-body term of bond k, in lane p = 0
+  double e = 0.0, dVdr = 0.0, cown = 0.0;
+  if (p == 0 && pair_ok) {
+    double b = 1.0, db = 0.0;
+    if (zeta > 0.0 && PL[7] != 0.0) {
+      const double beta = PL[7], nn = PL[8];
+      double x = beta * zeta;
+      double bz = x > 0.0 ? exp(nn * log(x)) : pow_any(x, nn);
+      b = exp((-0.5 / nn) * log1p(bz));
+      db = -0.5 * b * bz / (zeta * (1.0 + bz));
+    }
+    double fc2 = u.fcl, dfc2 = u.dfcl;
+    if (!SP1) te_fc(u.r, P2[5], P2[6], fc2, dfc2);
+    if (fc2 != 0.0 || dfc2 != 0.0) {
+      double fr = P2[1] * nb_exp(-P2[3] * u.r);
+      double fa = -P2[2] * nb_exp(-P2[4] * u.r);
+      e = 0.5 * fc2 * (fr + b * fa);
+      dVdr = 0.5 * (dfc2 * (fr + b * fa) + fc2 * (-P2[3] * fr + b * (-P2[4] * fa)));
+      if (fabs(db) > 1e-30) cown = 0.5 * fc2 * fa * db;
+    }
+  }
+  const double ck = seg_get<L>(cown, gseg, 0);              // bond k's coefficient
+  const double cw = __shfl(cown, aseg + s * L, kWave);       // bond s's coefficient
+
+  // gradient share of this lane: own zeta through the angle to leg s (A),
+  // and the zeta of bond s in which bond k is the third leg (B)
+  double su = p == 0 ? dVdr * u.rinv : 0.0, aw = 0.0;
+  if (tri_ok) {
+    const double* TB = tri + (((size_t)sa * ns + w.sj) * ns + u.sj) * 7;
+    double alpha = 0.0, beta = 0.0;
+    if (ck != 0.0 && w.fcl != 0.0 && TA[0] != 0.0) {
+      alpha += ck * TA[6] * w.fcl * dgA * eA;
+      beta += ck * TA[6] * w.fcl * gA * deA;
+    }
+    if (cw != 0.0 && u.fcl != 0.0 && TB[0] != 0.0) {
+      double gB, dgB;
+      if (TB == TA) {
+        gB = gA;
+        dgB = dgA;
+      } else {
+        double c = TB[1], d = TB[2], hc = TB[3] - cs;
+        double c2 = c * c, iq = 1.0 / (d * d + hc * hc);
+        gB = 1.0 + c2 / (d * d) - c2 * iq;
+        dgB = -2.0 * c2 * hc * iq * iq;
+      }
+      double eB, deB;
+      te_exp(TB[4], TB[5], w.r - u.r, eB, deB);
+      alpha += cw * TB[6] * u.fcl * dgB * eB;
+      beta += cw * TB[6] * (u.dfcl * gB * eB - u.fcl * gB * deB);
+    }
+    // d cos / du = w/(|u||w|) - cos u/|u|^2
+    su = beta * u.rinv - alpha * cs * u.rinv * u.rinv;
+    aw = alpha * u.rinv * w.rinv;
+  }
+  su = seg_sum<L>(su);
+  const double swx = seg_sum<L>(aw * w.x), swy = seg_sum<L>(aw * w.y),
+               swz = seg_sum<L>(aw * w.z);
+  double Gx = 0.0, Gy = 0.0, Gz = 0.0;
+  if (p == 0) {
+    Gx = su * u.x + swx;
+    Gy = su * u.y + swy;
+    Gz = su * u.z + swz;
+    if (has) {
+      const size_t plane = (size_t)n * stride;
+      G[slot] = Gx;
+      G[plane + slot] = Gy;
+      G[2 * plane + slot] = Gz;
+    }
+  }
+  block_partial(e, Gx, Gy, Gz, u.x, u.y, u.z, partial, c, n_chunks);
+}
+
+template <int L, bool SP1>
+__global__ __launch_bounds__(256) void k_tersoff_trip(int n_chunks, int n, const double* __restrict__ pos, const int* __restrict__ sid,
+    const Geom* __restrict__ geoms, int single, const int* __restrict__ spc, int ns_arg,
+    const double* __restrict__ tb, const double* __restrict__ tri, double rc,
+    const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
+    const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial) {
+  for (int c = blockIdx.x; c < n_chunks; c += gridDim.x)
+    k_tersoff_trip_chunk<L, SP1>(c, n_chunks, n, pos, sid, geoms, single, spc, ns_arg, tb, tri, rc, cnt, nbr, stride, chunk_rng, G, partial);
 }
 
 // ---------------------------------------------------------------------------
@@ -349,15 +539,15 @@ __global__ __launch_bounds__(256) void k_tersoff_site(
 // prm = epsilon, sigma, A, B, p, q, a, lambda, gamma, costheta0.
 // ---------------------------------------------------------------------------
 template <int L>
-__global__ __launch_bounds__(256) void k_sw_site(
+__device__ __forceinline__ void k_sw_site_chunk(int c, int n_chunks, 
     int n, const double* __restrict__ pos, const int* __restrict__ sid,
     const Geom* __restrict__ geoms, int single, const double* __restrict__ prm,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial) {
-  const int a = chunk_rng[2 * blockIdx.x] + (int)threadIdx.x / L;
+  const int a = chunk_rng[2 * c] + (int)threadIdx.x / L;
   const int k = threadIdx.x & (L - 1);
   const int seg = (threadIdx.x & 63) & ~(L - 1);
-  const bool live = a < chunk_rng[2 * blockIdx.x + 1];
+  const bool live = a < chunk_rng[2 * c + 1];
   const int ac = live ? a : 0;
   const int nk = live ? cnt[ac] : 0;
   const bool has = live && k < nk;
@@ -420,7 +610,16 @@ __global__ __launch_bounds__(256) void k_sw_site(
     G[plane + slot] = Gy;
     G[2 * plane + slot] = Gz;
   }
-  block_partial(e, Gx, Gy, Gz, ux, uy, uz, partial);
+  block_partial(e, Gx, Gy, Gz, ux, uy, uz, partial, c, n_chunks);
+}
+
+template <int L>
+__global__ __launch_bounds__(256) void k_sw_site(int n_chunks, int n, const double* __restrict__ pos, const int* __restrict__ sid,
+    const Geom* __restrict__ geoms, int single, const double* __restrict__ prm,
+    const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
+    const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial) {
+  for (int c = blockIdx.x; c < n_chunks; c += gridDim.x)
+    k_sw_site_chunk<L>(c, n_chunks, n, pos, sid, geoms, single, prm, cnt, nbr, stride, chunk_rng, G, partial);
 }
 
 // Per-structure energy and stress of structure s by one block of 256
@@ -627,16 +826,16 @@ __global__ __launch_bounds__(256) void k_eam_rho(
 }
 
 template <int LE>
-__global__ __launch_bounds__(256) void k_eam_force(
+__device__ __forceinline__ void k_eam_force_chunk(int c, int n_chunks, 
     const double* __restrict__ pos, const int* __restrict__ sid,
     const Geom* __restrict__ geoms, int single, const int* __restrict__ spc, int ns,
     const double* __restrict__ fn, const int* __restrict__ map, double rc,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const double* __restrict__ dF, const double* __restrict__ Fe,
     const int* __restrict__ chunk_rng, double* __restrict__ F, double* __restrict__ partial) {
-  const int a = chunk_rng[2 * blockIdx.x] + (int)threadIdx.x / LE;
+  const int a = chunk_rng[2 * c] + (int)threadIdx.x / LE;
   const int l = threadIdx.x & (LE - 1);
-  const bool live = a < chunk_rng[2 * blockIdx.x + 1];
+  const bool live = a < chunk_rng[2 * c + 1];
   double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
   double w[6] = {0, 0, 0, 0, 0, 0};  // xx, xy, xz, yy, yz, zz
   if (live) {
@@ -685,7 +884,18 @@ __global__ __launch_bounds__(256) void k_eam_force(
     F[3 * (size_t)a + 2] = fz;
   }
   double v[10] = {e, w[0], w[1], w[2], w[1], w[3], w[4], w[2], w[4], w[5]};
-  wave_partial(v, partial);
+  wave_partial(v, partial, c, n_chunks);
+}
+
+template <int LE>
+__global__ __launch_bounds__(256) void k_eam_force(int n_chunks, const double* __restrict__ pos, const int* __restrict__ sid,
+    const Geom* __restrict__ geoms, int single, const int* __restrict__ spc, int ns,
+    const double* __restrict__ fn, const int* __restrict__ map, double rc,
+    const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
+    const double* __restrict__ dF, const double* __restrict__ Fe,
+    const int* __restrict__ chunk_rng, double* __restrict__ F, double* __restrict__ partial) {
+  for (int c = blockIdx.x; c < n_chunks; c += gridDim.x)
+    k_eam_force_chunk<LE>(c, n_chunks, pos, sid, geoms, single, spc, ns, fn, map, rc, cnt, nbr, stride, dF, Fe, chunk_rng, F, partial);
 }
 
 // ---------------------------------------------------------------------------
@@ -705,35 +915,54 @@ static void launch_timed(Potential& pot, K kernel, dim3 grid, dim3 block, hipStr
     hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
 }
 
+// Site-kernel grid: one block per chunk, or at most grid_cap blocks that
+// stride over the chunks (persistent form).
+static inline dim3 chunk_grid(const Potential& pot, const SetDev::Chunks& ch) {
+  int g = ch.n;
+  if (pot.grid_cap > 0 && g > pot.grid_cap) g = pot.grid_cap;
+  return dim3(g > 0 ? g : 1);
+}
+
 template <int L>
 static void launch_bond_kernels(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
                                 const int* d_spc, double* d_F, double* d_out,
-                                SetDev::Chunks& ch, hipStream_t st) {
+                                SetDev::Chunks& ch, bool trip, hipStream_t st) {
   const int n = set.n_atoms;
   const int single = set.n_struct == 1 ? 1 : 0;
   double* G = pot.G.as<double>();
+  const dim3 grid = chunk_grid(pot, ch), block(pot.block);
+  const int* sid = set.d_sid.as<int>();
+  const Geom* geoms = set.d_geoms.as<Geom>();
+  const int* cnt = nl.cnt.as<int>();
+  const int* nbr = nl.nbr.as<int>();
+  const int* rng = ch.rng.as<int>();
+  double* part = ch.partial.as<double>();
   if (pot.kind == KIND_TERSOFF) {
     const double* tb = pot.d_params.as<double>();
     const double* tri = tb + (size_t)pot.ns * pot.ns * 9;
-    launch_timed(pot, k_tersoff_site<L>, dim3(ch.n), dim3(256), st, n, d_pos,
-                 (const int*)set.d_sid.as<int>(), (const Geom*)set.d_geoms.as<Geom>(), single, d_spc,
-                 pot.ns, tb, tri, pot.r_cut, (const int*)nl.cnt.as<int>(),
-                 (const int*)nl.nbr.as<int>(), nl.stride, (const int*)ch.rng.as<int>(), G,
-                 ch.partial.as<double>());
+    const bool sp1 = pot.ns == 1;
+    bool done = false;
+    if constexpr (L * L <= 64) {
+      if (trip) {
+        launch_timed(pot, sp1 ? k_tersoff_trip<L, true> : k_tersoff_trip<L, false>, grid, block,
+                     st, ch.n, n, d_pos, sid, geoms, single, d_spc, pot.ns, tb, tri, pot.r_cut,
+                     cnt, nbr, nl.stride, rng, G, part);
+        done = true;
+      }
+    }
+    if (!done)
+      launch_timed(pot, sp1 ? k_tersoff_site<L, true> : k_tersoff_site<L, false>, grid, block, st,
+                   ch.n, n, d_pos, sid, geoms, single, d_spc, pot.ns, tb, tri, pot.r_cut, cnt,
+                   nbr, nl.stride, rng, G, part);
   } else {
-    launch_timed(pot, k_sw_site<L>, dim3(ch.n), dim3(256), st, n, d_pos,
-                 (const int*)set.d_sid.as<int>(), (const Geom*)set.d_geoms.as<Geom>(), single,
-                 (const double*)pot.d_params.as<double>(), (const int*)nl.cnt.as<int>(),
-                 (const int*)nl.nbr.as<int>(), nl.stride, (const int*)ch.rng.as<int>(), G,
-                 ch.partial.as<double>());
+    launch_timed(pot, k_sw_site<L>, grid, block, st, ch.n, n, d_pos, sid, geoms, single,
+                 (const double*)pot.d_params.as<double>(), cnt, nbr, nl.stride, rng, G, part);
   }
   const int64_t threads = (int64_t)n * L;
   const int atom_blocks = (int)((threads + 255) / 256);
-  hipLaunchKernelGGL(k_gather<L>, dim3(atom_blocks + set.n_struct), dim3(256), 0, st, n,
-                     (const int*)nl.cnt.as<int>(), (const int*)nl.rev.as<int>(), nl.stride,
-                     (const double*)G, d_F, atom_blocks, (const Geom*)set.d_geoms.as<Geom>(),
-                     (const int*)ch.first.as<int>(), ch.n, (const double*)ch.partial.as<double>(),
-                     d_out);
+  hipLaunchKernelGGL(k_gather<L>, dim3(atom_blocks + set.n_struct), dim3(256), 0, st, n, cnt,
+                     (const int*)nl.rev.as<int>(), nl.stride, (const double*)G, d_F, atom_blocks,
+                     geoms, (const int*)ch.first.as<int>(), ch.n, (const double*)part, d_out);
 }
 
 template <int LE>
@@ -749,7 +978,7 @@ static void launch_eam(Potential& pot, NList& nl, SetDev& set, const double* d_p
                      set.d_sid.as<int>(), set.d_geoms.as<Geom>(), single, d_spc, pot.ns,
                      pot.d_params.as<double>(), pot.d_map.as<int>(), pot.r_cut,
                      nl.cnt.as<int>(), nl.nbr.as<int>(), nl.stride, dF, Fe);
-  launch_timed(pot, k_eam_force<LE>, dim3(ch.n), dim3(256), st, d_pos,
+  launch_timed(pot, k_eam_force<LE>, chunk_grid(pot, ch), dim3(pot.block), st, ch.n, d_pos,
                (const int*)set.d_sid.as<int>(), (const Geom*)set.d_geoms.as<Geom>(), single, d_spc,
                pot.ns, (const double*)pot.d_params.as<double>(),
                (const int*)pot.d_map.as<int>(), pot.r_cut, (const int*)nl.cnt.as<int>(),
@@ -762,6 +991,8 @@ void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
   const int n = set.n_atoms;
   AK_REQUIRE(nl.valid, AK_ERR_STATE, "neighbour list has not been built");
   AK_REQUIRE(nl.n_atoms == n, AK_ERR_STATE, "neighbour list was built for another size");
+  AK_REQUIRE(pot.block == 64 || pot.block == 128 || pot.block == 256, AK_ERR_STATE,
+             "block size must be 64, 128 or 256 threads");
   int lanes;
   if (pot.kind == KIND_EAM) {
     lanes = pot.eam_lanes > 0 ? pot.eam_lanes : 16;
@@ -770,7 +1001,12 @@ void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
                "more than 64 neighbours within r_cut + skin for a bond-order kernel");
     lanes = nl.stride;
   }
-  SetDev::Chunks& ch = set.chunks(256 / lanes, st);
+  // lanes per atom of the site kernel: L, or L * L for the Tersoff form with
+  // one lane per (bond, partner) pair
+  const bool trip = pot.kind == KIND_TERSOFF && lanes * lanes <= 64 && pot.bond_tile == 2;
+  const int atom_lanes = trip ? lanes * lanes : lanes;
+  AK_REQUIRE(atom_lanes <= pot.block, AK_ERR_STATE, "block smaller than one atom's lanes");
+  SetDev::Chunks& ch = set.chunks(pot.block / atom_lanes, st);
   if (n > 0) {
     if (pot.kind == KIND_EAM) {
       pot.aux.ensure(sizeof(double) * 2 * (size_t)n);
@@ -784,11 +1020,11 @@ void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
     } else {
       pot.G.ensure(sizeof(double) * 3 * (size_t)n * nl.stride);
       switch (lanes) {
-        case 4: launch_bond_kernels<4>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, st); break;
-        case 8: launch_bond_kernels<8>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, st); break;
-        case 16: launch_bond_kernels<16>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, st); break;
-        case 32: launch_bond_kernels<32>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, st); break;
-        case 64: launch_bond_kernels<64>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, st); break;
+        case 4: launch_bond_kernels<4>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, st); break;
+        case 8: launch_bond_kernels<8>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, st); break;
+        case 16: launch_bond_kernels<16>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, st); break;
+        case 32: launch_bond_kernels<32>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, st); break;
+        case 64: launch_bond_kernels<64>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, st); break;
         default: throw Error(AK_ERR_STATE, "unsupported neighbour stride");
       }
     }

@@ -54,6 +54,7 @@ class _GpuPotential:
     def __init__(self):
         self._tls = threading.local()
         self._lanes = 0
+        self._launch = (0, 0, 0)
 
     def _create(self):  # -> c_void_p
         raise NotImplementedError
@@ -64,8 +65,24 @@ class _GpuPotential:
             h = _Handle(self._create())
             if self._lanes:
                 check(lib().ak_potential_set_lanes(h.ptr, self._lanes))
+            if any(self._launch):
+                check(lib().ak_potential_set_launch(h.ptr, *self._launch))
             self._tls.h = h
         return h.ptr
+
+    def set_launch(self, bond_tile=0, block_threads=0, grid_cap=0):
+        """Launch shape of the site / force kernels (the autotuner's knobs):
+        Tersoff site form (1 = lane per bond, 2 = lane per (bond, partner)),
+        threads per block (64, 128, 256), and a grid cap (0 = one block per
+        chunk of atoms, > 0 = persistent blocks looping over chunks)."""
+        self._launch = (int(bond_tile), int(block_threads), int(grid_cap))
+        h = getattr(self._tls, "h", None)
+        if h is not None:
+            check(lib().ak_potential_set_launch(h.ptr, *self._launch))
+
+    def set_bond_tile(self, tile):
+        """Tersoff site kernel form only (see set_launch)."""
+        self.set_launch(tile, self._launch[1], self._launch[2])
 
     def set_lanes(self, lanes):
         """Lanes per atom of the EAM kernels (the autotuner's knob)."""
@@ -73,6 +90,44 @@ class _GpuPotential:
         h = getattr(self._tls, "h", None)
         if h is not None:
             check(lib().ak_potential_set_lanes(h.ptr, self._lanes))
+
+    def autotune(self, structures, reps=5):
+        """Pick the launch shape for a batch of (lattice, positions,
+        atom_types) by timing every candidate on the device (the native
+        autotuner, ``ak_autotune_batched``).  The choice stays on this
+        potential; returns {"best": {...}, "best_ms": t, "tried": [...]}."""
+        off, lats, _, pos, spc = self._pack(structures)
+        cap = 64
+        best = np.zeros(4, np.int32)
+        best_ms = ctypes.c_double()
+        shapes = np.zeros((cap, 4), np.int32)
+        times = np.zeros(cap)
+        tried = ctypes.c_int32()
+        check(lib().ak_autotune_batched(
+            self.handle(), len(structures), off.ctypes.data_as(_lib._pi64),
+            lats.ctypes.data_as(_lib._pf64), pos.ctypes.data_as(_lib._pf64),
+            spc.ctypes.data_as(_lib._pi32), int(reps), best.ctypes.data_as(_lib._pi32),
+            ctypes.byref(best_ms), cap, shapes.ctypes.data_as(_lib._pi32),
+            times.ctypes.data_as(_lib._pf64), ctypes.byref(tried)))
+        keys = ("bond_tile", "block_threads", "grid_cap", "eam_lanes")
+        self._launch = (int(best[0]), int(best[1]), int(best[2]))
+        self._lanes = int(best[3])
+        k = min(int(tried.value), cap)
+        return {"best": dict(zip(keys, map(int, best))), "best_ms": float(best_ms.value),
+                "tried": [dict(zip(keys, map(int, shapes[i])), ms=float(times[i]))
+                          for i in range(k)]}
+
+    def _pack(self, structures):
+        S = len(structures)
+        off = np.zeros(S + 1, np.int64)
+        for s, (_, p, _) in enumerate(structures):
+            off[s + 1] = off[s] + len(p)
+        lats = np.ascontiguousarray(np.stack([np.asarray(l, float) for l, _, _ in structures]))
+        invs = np.ascontiguousarray(np.linalg.inv(lats))
+        pos = np.ascontiguousarray(np.concatenate([np.asarray(p, float).reshape(-1, 3)
+                                                   for _, p, _ in structures]))
+        spc = np.concatenate([self.species_ids(t) for _, _, t in structures]).astype(np.int32)
+        return off, lats, invs, pos, spc
 
     def species_ids(self, atom_types):
         if len(self.species) == 1:
@@ -111,14 +166,7 @@ class _GpuPotential:
         """E (S,), forces (list of (N_s, 3)), stress (S, 3, 3) for a list of
         (lattice, positions, atom_types) in one device pass."""
         S = len(structures)
-        off = np.zeros(S + 1, np.int64)
-        for s, (_, p, _) in enumerate(structures):
-            off[s + 1] = off[s] + len(p)
-        lats = np.ascontiguousarray(np.stack([np.asarray(l, float) for l, _, _ in structures]))
-        invs = np.ascontiguousarray(np.linalg.inv(lats))
-        pos = np.ascontiguousarray(np.concatenate([np.asarray(p, float).reshape(-1, 3)
-                                                   for _, p, _ in structures]))
-        spc = np.concatenate([self.species_ids(t) for _, _, t in structures]).astype(np.int32)
+        off, lats, invs, pos, spc = self._pack(structures)
         E = np.zeros(S)
         F = np.zeros((int(off[-1]), 3))
         Sg = np.zeros((S, 3, 3))

@@ -12,7 +12,9 @@ The line also carries:
   roofline      the Tersoff site kernel's algorithmic FP64 rate (flops counted
                 for the reference's algorithm on this geometry, see
                 ``tersoff_flops``) over its average duration, timed with HIP
-                events on its launch stream during the timed region;
+                start/stop events of each of its dispatches on its launch
+                stream, over a second pass of the K timed steps (the events
+                cost host time, so the throughput pass runs without them);
                 ``traffic`` from the committed rocprofv3 counter summary;
   cpu_baseline  the oracle (C/OpenMP restatement of the reference path,
                 kind "port") on this host's cores, same workload;
@@ -147,6 +149,164 @@ def fit_throughput(n_struct, reps, seed=17):
             "loss": L, "batched_vs_loop_grad_rel_diff": agree}
 
 
+def device_force_rate(pot, lat, pos, types, steps, warmup, skin=0.3):
+    """Device-resident force calls on a prebuilt list: (atoms x calls / s,
+    ms per call, site/force-kernel average from HIP events in a second pass)."""
+    import torch
+
+    from akitsushima_amd import NeighbourList
+    from akitsushima_amd import _lib
+
+    dev = torch.device("cuda:0")
+    n = len(pos)
+    nl = NeighbourList(pot.r_cut, skin_width=skin)
+    nl.build(lat, pos)
+    d_pos = torch.tensor(pos, dtype=torch.float64, device=dev)
+    d_spc = torch.tensor(pot.species_ids(types), dtype=torch.int32, device=dev)
+    d_F = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    d_es = torch.empty(10, dtype=torch.float64, device=dev)
+    latc = np.ascontiguousarray(lat)
+    invc = np.ascontiguousarray(np.linalg.inv(lat))
+    L = _lib.lib()
+    h = pot.handle()
+    stream = torch.cuda.current_stream(dev)
+    call_args = (h, nl.handle(), n, latc.ctypes.data_as(_lib._pf64),
+                 invc.ctypes.data_as(_lib._pf64), ctypes.c_void_p(d_pos.data_ptr()),
+                 ctypes.c_void_p(d_spc.data_ptr()), ctypes.c_void_p(d_F.data_ptr()),
+                 ctypes.c_void_p(d_es.data_ptr()), ctypes.c_void_p(stream.cuda_stream))
+    for _ in range(warmup):
+        _lib.check(L.ak_compute_device(*call_args))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _lib.check(L.ak_compute_device(*call_args))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    _lib.check(L.ak_profile_enable(h, 1))
+    for _ in range(steps):
+        _lib.check(L.ak_compute_device(*call_args))
+    torch.cuda.synchronize()
+    tot, mn, cnt = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+    _lib.check(L.ak_profile_read(h, ctypes.byref(tot), ctypes.byref(mn), ctypes.byref(cnt)))
+    _lib.check(L.ak_profile_enable(h, 0))
+    es = d_es.cpu().numpy()
+    return {"atom_evals_per_s": n * steps / dt, "ms_per_call": dt / steps * 1e3,
+            "kernel_avg_us": tot.value / max(cnt.value, 1) * 1e3,
+            "energy_per_atom": float(es[0] / n)}
+
+
+def sw_md(n_steps=1000):
+    """Stillinger-Weber Si NVE: diamond 20^3 cells (64000 atoms), Maxwell-
+    Boltzmann velocities at 1000 K (seed 0), dt 1 fs, skin 0.3 A, one step
+    captured in a hipGraph and replayed."""
+    from akitsushima_amd import StillingerWeberPotential
+    from akitsushima_amd.md import masses_of, maxwell_boltzmann, nve_run
+    from akitsushima_amd.published import SW_SI
+    from akitsushima_amd.structures import DIAMOND, cubic
+
+    lat, pos, types = cubic(DIAMOND, 5.431, 20, ["Si"] * 8, rattle=0.0, seed=0)
+    pot = StillingerWeberPotential(dict(SW_SI), species=["Si"])
+    masses = masses_of(types)
+    vel = maxwell_boltzmann(1000.0, masses, 0)
+    nve_run(pot, lat, pos, vel, types, dt=1.0, n_steps=10, skin_width=0.3)  # warm
+    res = nve_run(pot, lat, pos, vel, types, dt=1.0, n_steps=n_steps, skin_width=0.3,
+                  use_graph=True)
+    n = len(pos)
+    drift = res["drift"] / n
+    return {"workload": f"SW Si diamond 20^3 ({n} atoms), 1000 K, dt 1 fs, {n_steps} steps, "
+                        "skin 0.3 A, hipGraph-captured step",
+            "atoms": n, "steps": n_steps, "seconds": res["seconds"],
+            "ms_per_step": res["seconds"] / n_steps * 1e3,
+            "atom_steps_per_s": n * n_steps / res["seconds"],
+            "n_rebuilds": res["n_rebuilds"],
+            "drift_per_atom_final_eV": float(drift[-1]),
+            "drift_per_atom_max_abs_eV": float(np.abs(drift).max()),
+            "e_tot_per_atom_start_eV": float(res["e_tot"][0] / n)}
+
+
+def eam_cuni(steps, warmup):
+    """Zhou 2004 Cu-Ni random alloy: fcc 40^3 (256000 atoms), a = 3.56 A,
+    species drawn 50/50 from seed 0, rattle 0.03 A (seed 1)."""
+    from akitsushima_amd import zhou_eam
+    from akitsushima_amd.published import ZHOU_2004
+    from akitsushima_amd.structures import FCC, cubic, random_species
+
+    lat, pos, _ = cubic(FCC, 3.56, 40, ["X"] * 4, rattle=0.03, seed=1)
+    types = random_species(np.random.default_rng(0), ["Cu", "Ni"], len(pos))
+    pot = zhou_eam({"Cu": dict(ZHOU_2004["Cu"]), "Ni": dict(ZHOU_2004["Ni"])}, ["Cu", "Ni"])
+    out = {"workload": f"Zhou EAM Cu-Ni fcc 40^3 ({len(pos)} atoms), a 3.56 A, rattle 0.03 A",
+           "atoms": len(pos), "r_cut": pot.r_cut}
+    out.update(device_force_rate(pot, lat, pos, types, steps, warmup))
+    return out
+
+
+def batched_tersoff(reps):
+    """512 diamond Si structures of 64-2048 atoms (seeded supercell repeats,
+    +-2 % symmetric strain, rattle 0.02 A) in one batched call, autotuned,
+    against a loop of single-structure device calls."""
+    import torch
+
+    from akitsushima_amd import NeighbourList, TersoffPotential
+    from akitsushima_amd import _lib
+    from akitsushima_amd.published import TERSOFF_SI_T3
+    from akitsushima_amd.structures import diamond_batch
+
+    structs = diamond_batch(512, 64, 2048, 0.02, 0.02, seed=5, rep_range=(2, 8))
+    pot = TersoffPotential(dict(TERSOFF_SI_T3), species=["Si"])
+    tune = pot.autotune(structs, reps=3)
+    off, lats, _, pos, spc = pot._pack(structs)
+    n = int(off[-1])
+    S = len(structs)
+    dev = torch.device("cuda:0")
+    d_pos = torch.tensor(pos, dtype=torch.float64, device=dev)
+    d_spc = torch.tensor(spc, dtype=torch.int32, device=dev)
+    d_F = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    d_es = torch.empty((S, 10), dtype=torch.float64, device=dev)
+    L = _lib.lib()
+    h = pot.handle()
+    stream = torch.cuda.current_stream(dev)
+    args = (h, S, off.ctypes.data_as(_lib._pi64), lats.ctypes.data_as(_lib._pf64),
+            ctypes.c_void_p(d_pos.data_ptr()), ctypes.c_void_p(d_spc.data_ptr()),
+            ctypes.c_void_p(d_F.data_ptr()), ctypes.c_void_p(d_es.data_ptr()), 0,
+            ctypes.c_void_p(stream.cuda_stream))
+    _lib.check(L.ak_compute_batched_device(*args))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _lib.check(L.ak_compute_batched_device(*args))
+    torch.cuda.synchronize()
+    batched = (time.perf_counter() - t0) / reps
+    # per-structure loop: one prebuilt list and one device call per structure
+    loop_pot = TersoffPotential(dict(TERSOFF_SI_T3), species=["Si"])
+    lh = loop_pot.handle()
+    calls = []
+    for k, (lat, p, _) in enumerate(structs):
+        nl = NeighbourList(loop_pot.r_cut, skin_width=0.0)
+        nl.build(lat, p)
+        latc = np.ascontiguousarray(lat)
+        invc = np.ascontiguousarray(np.linalg.inv(lat))
+        calls.append((nl, latc, invc, (lh, nl.handle(), len(p), latc.ctypes.data_as(_lib._pf64),
+                      invc.ctypes.data_as(_lib._pf64),
+                      ctypes.c_void_p(d_pos[off[k]:].data_ptr()),
+                      ctypes.c_void_p(d_spc[off[k]:].data_ptr()),
+                      ctypes.c_void_p(d_F[off[k]:].data_ptr()),
+                      ctypes.c_void_p(d_es[k].data_ptr()), ctypes.c_void_p(stream.cuda_stream))))
+    for c in calls:
+        _lib.check(L.ak_compute_device(*c[3]))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for c in calls:
+        _lib.check(L.ak_compute_device(*c[3]))
+    torch.cuda.synchronize()
+    loop = time.perf_counter() - t0
+    return {"workload": f"{S} diamond Si structures, 64-2048 atoms ({n} atoms), strain +-2 %, "
+                        "rattle 0.02 A, Tersoff T3",
+            "structures": S, "atoms": n, "autotuned": tune["best"],
+            "candidates_tried": len(tune["tried"]),
+            "batched_ms_per_call": batched * 1e3, "atom_evals_per_s": n / batched,
+            "per_structure_loop_ms": loop * 1e3, "speedup_vs_loop": loop / batched}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,6 +319,11 @@ def main():
     ap.add_argument("--no-fit", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the force calls (for rocprofv3 runs)")
+    ap.add_argument("--bond-tile", type=int, default=0,
+                    help="Tersoff site kernel form: 1 lane per bond, 2 lane per "
+                         "(bond, partner), 0 automatic")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the SW MD, Zhou EAM and batched Tersoff lines")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="with --profile-only: also run a MiB-sized elementwise copy "
                          "kernel (counter calibration)")
@@ -183,6 +348,7 @@ def main():
     lat, pos, types = cubic(DIAMOND, 5.431, args.cells, ["Si"] * 8, rattle=0.05, seed=0)
     n = len(pos)
     pot = TersoffPotential(params, species=["Si"])
+    pot.set_bond_tile(args.bond_tile)
     nl = NeighbourList(pot.r_cut, skin_width=0.3)
     nl.build(lat, pos)
     t0 = time.perf_counter()
@@ -214,22 +380,32 @@ def main():
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
-    _lib.check(L.ak_profile_enable(h, 1))
+    # throughput: K steps back to back, no per-kernel instrumentation
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    elapsed = t1 - t0
+    value = n * args.steps / elapsed
+    # kernel duration: the same K steps again with start/stop events around
+    # every site-kernel dispatch (hipExtLaunchKernelGGL on the launch stream).
+    # The events cost ~20 us of host time per call, so they get a pass of
+    # their own instead of slowing the throughput pass.
+    _lib.check(L.ak_profile_enable(h, 1))
+    torch.cuda.synchronize()
+    te0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    events_ms_per_step = (time.perf_counter() - te0) / args.steps * 1e3
     tot_ms = ctypes.c_double()
     min_ms = ctypes.c_double()
     cnt = ctypes.c_int64()
     _lib.check(L.ak_profile_read(h, ctypes.byref(tot_ms), ctypes.byref(min_ms),
                                  ctypes.byref(cnt)))
     _lib.check(L.ak_profile_enable(h, 0))
-    elapsed = t1 - t0
-    value = n * args.steps / elapsed
     es = d_es.cpu().numpy()
     F = d_F.cpu().numpy()
     if args.profile_only:
@@ -257,6 +433,12 @@ def main():
     host_rate = n * 10 / (time.perf_counter() - t0)
     assert abs(e_h - es[0]) <= 1e-10 * abs(es[0]), (e_h, es[0])
 
+    ci, cj = nl.get_candidate_pairs()
+    stride = 4
+    while stride < int(np.bincount(np.concatenate([ci, cj]), minlength=n).max()):
+        stride *= 2
+    site_kernel = (f"k_tersoff_trip<{stride}>" if args.bond_tile == 2 and stride <= 8
+                   else f"k_tersoff_site<{stride}>")
     n_pairs, n_trip = geometry_counts(lat, pos, pot.r_cut)
     flops = tersoff_flops(n_pairs, n_trip, params["lambda3"])
     avg_site_s = tot_ms.value / max(cnt.value, 1) * 1e-3
@@ -264,8 +446,11 @@ def main():
     roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                 "traffic": read_traffic(),
-                "kernel": "k_tersoff_site<4>", "kernel_avg_us": avg_site_s * 1e6,
+                "kernel": site_kernel, "kernel_avg_us": avg_site_s * 1e6,
                 "kernel_min_us": min_ms.value * 1e3, "launches_timed": int(cnt.value),
+                "timing": "HIP start/stop events of each site-kernel dispatch "
+                          "(hipExtLaunchKernelGGL) over a second pass of the K timed steps",
+                "events_pass_ms_per_step": events_ms_per_step,
                 "flops_per_launch": flops,
                 "flops_per_atom": flops / n,
                 "compute_unit": "FP64 VALU (MI355X FP64 vector peak = 78.6 TFLOP/s)"}
@@ -296,6 +481,10 @@ def main():
         out["gpu_over_cpu"] = value / (n / best)
     if not args.no_fit:
         out["fit"] = fit_throughput(256, args.fit_reps)
+    if not args.no_extras:
+        out["sw_md"] = sw_md(1000)
+        out["eam_cuni"] = eam_cuni(20, 3)
+        out["batched_tersoff"] = batched_tersoff(10)
     print(json.dumps(out))
     return 0
 

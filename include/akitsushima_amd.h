@@ -135,6 +135,31 @@ int ak_compute_batched_device(ak_potential *pot, int32_t n_struct,
  * (4..64, power of two; 0 = default). */
 int ak_potential_set_lanes(ak_potential *pot, int32_t lanes);
 
+/* Launch shape of the site / force kernels: the autotuner's output (no
+ * counterpart in the reference, whose kernels are CPU loops).
+ *   bond_tile     Tersoff site kernel form: 1 = one lane per bond, 2 = one
+ *                 lane per (bond, partner leg) pair for list strides up to 8;
+ *                 0 = automatic (one lane per bond);
+ *   block_threads 64, 128 or 256 threads per block (0 = 256); atoms per
+ *                 block = block_threads / lanes per atom;
+ *   grid_cap      0 = one block per chunk of atoms; > 0 = at most this many
+ *                 blocks, each looping over chunks (persistent form). */
+int ak_potential_set_launch(ak_potential *pot, int32_t bond_tile, int32_t block_threads,
+                            int32_t grid_cap);
+
+/* Project-written autotuner (no counterpart in the reference): builds the
+ * batch list for the given structures (as ak_compute_batched), times every
+ * candidate launch shape -- Tersoff site form, threads per block, grid cap,
+ * EAM lanes per atom -- over `reps` device force calls with HIP events, and
+ * leaves the fastest set on the potential.  best (4) = bond_tile,
+ * block_threads, grid_cap, eam_lanes; *best_ms = its time per call.  When
+ * shapes (capacity, 4) and times_ms (capacity) are given they receive every
+ * candidate tried; *n_tried counts them. */
+int ak_autotune_batched(ak_potential *pot, int32_t n_struct, const int64_t *atom_offsets,
+                        const double *lattices, const double *positions,
+                        const int32_t *species, int32_t reps, int32_t *best, double *best_ms,
+                        int32_t capacity, int32_t *shapes, double *times_ms, int32_t *n_tried);
+
 /* ---- Molecular dynamics ------------------------------------------------ */
 
 /* NVE velocity Verlet, device-resident (NVESimulation.run,

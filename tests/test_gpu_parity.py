@@ -65,10 +65,15 @@ def test_neighbour_list_too_thin_cell_raises():
 #  Tersoff
 # --------------------------------------------------------------------------
 
+TILES = [1, 2]   # site kernel: lane per bond / lane per (bond, partner)
+
+
+@pytest.mark.parametrize("tile", TILES)
 @pytest.mark.parametrize("n", [4, 8])
-def test_tersoff_si_matches_ledger_and_oracle(n):
+def test_tersoff_si_matches_ledger_and_oracle(n, tile):
     lat, pos, t = si(n)
     pot = TersoffPotential(TERSOFF_SI, species=["Si"])
+    pot.set_bond_tile(tile)
     e, f, s = pot.compute(lat, pos, t)
     N = len(pos)
     ref = LEDGER[f"tersoff_si/{N}"]["e_per_atom"]
@@ -77,10 +82,12 @@ def test_tersoff_si_matches_ledger_and_oracle(n):
     _check(e, f, s, e0, f0, s0, N, f"tersoff si{n}")
 
 
+@pytest.mark.parametrize("tile", TILES)
 @pytest.mark.parametrize("n", [4, 8])
-def test_tersoff_two_species_matches_ledger_and_oracle(n):
+def test_tersoff_two_species_matches_ledger_and_oracle(n, tile):
     lat, pos, t = xy(n)
     pot = TersoffPotential(TERSOFF_XY, species=["X", "Y"])
+    pot.set_bond_tile(tile)
     e, f, s = pot.compute(lat, pos, t)
     N = len(pos)
     ref = LEDGER[f"tersoff_xy/{N}"]["e_per_atom"]
@@ -89,14 +96,16 @@ def test_tersoff_two_species_matches_ledger_and_oracle(n):
     _check(e, f, s, e0, f0, s0, N, f"tersoff xy{n}")
 
 
+@pytest.mark.parametrize("tile", TILES)
 @pytest.mark.parametrize("m,gamma,l3", [(3.0, 1.0, 1.7322), (3.0, 1.07, 1.3), (1.0, 0.93, 0.9)])
-def test_tersoff_fourteen_parameter_form(m, gamma, l3):
+def test_tersoff_fourteen_parameter_form(m, gamma, l3, tile):
     """Explicit m, gamma, lambda3 (the multi-species record form) on a
     liquid-like rattled, sheared cell."""
     lat, pos, t = sheared_si(3, shear=0.05, rattle=0.15, seed=11)
     rec = dict(TERSOFF_SI, lambda3=l3, m=m, gamma=gamma)
     recs = {("Si", "Si", "Si"): rec}
     pot = TersoffPotential(recs, species=["Si"])
+    pot.set_bond_tile(tile)
     e, f, s = pot.compute(lat, pos, t)
     e0, f0, s0 = O.tersoff_multi(lat, pos, t, recs, ["Si"])
     _check(e, f, s, e0, f0, s0, len(pos), f"tersoff m={m}")
