@@ -140,8 +140,8 @@ struct Potential {
   // EAM function-table indices
   std::vector<int> h_map;
   DevBuf d_map;
-  // per-call scratch
-  DevBuf G, aux, out;
+  // per-call scratch; amask (n) = bonds inside r_c per atom (rows >= 8)
+  DevBuf G, aux, out, amask;
   // launch shape chosen by the autotuner (0 = default)
   int eam_lanes = 0;
   // Tersoff site kernel form: 2 = one lane per (bond, partner) pair (list

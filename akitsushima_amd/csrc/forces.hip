@@ -144,6 +144,26 @@ __device__ __forceinline__ void block_partial(double e, double Gx, double Gy, do
   wave_partial(v, partial, c, n_chunks);
 }
 
+// Active legs of an atom's segment of L lanes (L >= 8): the segment's
+// ballot bits, this lane's rank among the active ones (slot order), their
+// count, and the largest count over the wavefront's segments (the bound of
+// a wave-uniform partner loop).
+template <int L>
+struct ActiveLegs {
+  uint64_t mask;
+  int rank, count, wave_max;
+  __device__ __forceinline__ ActiveLegs(bool active, int k, int seg) {
+    const uint64_t bal = __ballot(active);
+    mask = L == 64 ? bal : (bal >> seg) & ((uint64_t(1) << (L & 63)) - 1);
+    rank = __popcll(mask & ((uint64_t(1) << k) - 1));
+    count = __popcll(mask);
+    int m = count;
+#pragma unroll
+    for (int o = L; o < 64; o <<= 1) m = max(m, __shfl_xor(m, o, kWave));
+    wave_max = m;
+  }
+};
+
 // ---------------------------------------------------------------------------
 // Tersoff site kernel
 //   tb : (ns, ns, 9) [valid, A, B, lambda1, lambda2, R, D, beta, n]
@@ -160,7 +180,8 @@ __device__ __forceinline__ void k_tersoff_site_chunk(int c, int n_chunks,
     const Geom* __restrict__ geoms, int single, const int* __restrict__ spc, int ns_arg,
     const double* __restrict__ tb, const double* __restrict__ tri, double rc,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
-    const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial) {
+    const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
+    uint64_t* __restrict__ amask) {
   // one species: every table row is the first, read through uniform (scalar)
   // loads, and the leg cutoff equals the pair cutoff
   const int ns = SP1 ? 1 : ns_arg;
@@ -341,7 +362,13 @@ __device__ __forceinline__ void k_tersoff_site_chunk(int c, int n_chunks,
     swz += aw * wz;
   }
   const double Gx = su * ux + swx, Gy = su * uy + swy, Gz = su * uz + swz;
-  if (has) {
+  // bonds outside r_c have G = 0 exactly; with rows of 8 or more slots only
+  // the active ones are written and the gather skips the rest
+  if constexpr (L >= 8) {
+    const ActiveLegs<L> al(act, k, seg);
+    if (live && k == 0) amask[ac] = al.mask;
+  }
+  if (has && (L == 4 || act)) {
     const size_t plane = (size_t)n * stride;
     G[slot] = Gx;
     G[plane + slot] = Gy;
@@ -355,9 +382,11 @@ __global__ __launch_bounds__(256) void k_tersoff_site(int n_chunks, int n, const
     const Geom* __restrict__ geoms, int single, const int* __restrict__ spc, int ns_arg,
     const double* __restrict__ tb, const double* __restrict__ tri, double rc,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
-    const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial) {
+    const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
+    uint64_t* __restrict__ amask) {
   for (int c = blockIdx.x; c < n_chunks; c += gridDim.x)
-    k_tersoff_site_chunk<L, SP1>(c, n_chunks, n, pos, sid, geoms, single, spc, ns_arg, tb, tri, rc, cnt, nbr, stride, chunk_rng, G, partial);
+    k_tersoff_site_chunk<L, SP1>(c, n_chunks, n, pos, sid, geoms, single, spc, ns_arg, tb, tri, rc, cnt,
+                                nbr, stride, chunk_rng, G, partial, amask);
 }
 
 // ---------------------------------------------------------------------------
@@ -403,7 +432,8 @@ __device__ __forceinline__ void k_tersoff_trip_chunk(int c, int n_chunks,
     const Geom* __restrict__ geoms, int single, const int* __restrict__ spc, int ns_arg,
     const double* __restrict__ tb, const double* __restrict__ tri, double rc,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
-    const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial) {
+    const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
+    uint64_t* __restrict__ amask) {
   static_assert(L * L <= 64, "one atom must fit a wavefront");
   constexpr int LL = L * L;
   const int ns = SP1 ? 1 : ns_arg;
@@ -508,11 +538,19 @@ __device__ __forceinline__ void k_tersoff_trip_chunk(int c, int n_chunks,
   const double swx = seg_sum<L>(aw * w.x), swy = seg_sum<L>(aw * w.y),
                swz = seg_sum<L>(aw * w.z);
   double Gx = 0.0, Gy = 0.0, Gz = 0.0;
+  if constexpr (L >= 8) {
+    // bit k of the atom's mask: bond k is inside r_c (lane (k, 0) holds it)
+    const uint64_t bal = __ballot(p == 0 && u.act);
+    uint64_t m = 0;
+#pragma unroll
+    for (int kk = 0; kk < L; ++kk) m |= ((bal >> (aseg + kk * L)) & 1) << kk;
+    if (live && wl == aseg) amask[ac] = m;
+  }
   if (p == 0) {
     Gx = su * u.x + swx;
     Gy = su * u.y + swy;
     Gz = su * u.z + swz;
-    if (has) {
+    if (has && (L == 4 || u.act)) {
       const size_t plane = (size_t)n * stride;
       G[slot] = Gx;
       G[plane + slot] = Gy;
@@ -527,9 +565,11 @@ __global__ __launch_bounds__(256) void k_tersoff_trip(int n_chunks, int n, const
     const Geom* __restrict__ geoms, int single, const int* __restrict__ spc, int ns_arg,
     const double* __restrict__ tb, const double* __restrict__ tri, double rc,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
-    const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial) {
+    const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
+    uint64_t* __restrict__ amask) {
   for (int c = blockIdx.x; c < n_chunks; c += gridDim.x)
-    k_tersoff_trip_chunk<L, SP1>(c, n_chunks, n, pos, sid, geoms, single, spc, ns_arg, tb, tri, rc, cnt, nbr, stride, chunk_rng, G, partial);
+    k_tersoff_trip_chunk<L, SP1>(c, n_chunks, n, pos, sid, geoms, single, spc, ns_arg, tb, tri, rc, cnt,
+                                nbr, stride, chunk_rng, G, partial, amask);
 }
 
 // ---------------------------------------------------------------------------
@@ -543,7 +583,8 @@ __device__ __forceinline__ void k_sw_site_chunk(int c, int n_chunks,
     int n, const double* __restrict__ pos, const int* __restrict__ sid,
     const Geom* __restrict__ geoms, int single, const double* __restrict__ prm,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
-    const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial) {
+    const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
+    uint64_t* __restrict__ amask) {
   const int a = chunk_rng[2 * c] + (int)threadIdx.x / L;
   const int k = threadIdx.x & (L - 1);
   const int seg = (threadIdx.x & 63) & ~(L - 1);
@@ -586,11 +627,8 @@ __device__ __forceinline__ void k_sw_site_chunk(int c, int n_chunks,
   }
   const double c3 = lam * eps;
   double swx = 0.0, swy = 0.0, swz = 0.0;
-#pragma unroll
-  for (int s = 0; s < L; ++s) {
-    double wx = seg_get<L>(ux, seg, s), wy = seg_get<L>(uy, seg, s), wz = seg_get<L>(uz, seg, s);
-    double wri = seg_get<L>(rinv, seg, s), wE = seg_get<L>(E, seg, s);
-    if (s == k || !leg || wri == 0.0) continue;
+  // one three-body term h(a; this leg, leg w)
+  auto add_leg = [&](double wx, double wy, double wz, double wri, double wE) {
     double cs = (ux * wx + uy * wy + uz * wz) * rinv * wri;
     double dc = cs - cth0;
     double pre = c3 * E * wE * dc;
@@ -602,9 +640,40 @@ __device__ __forceinline__ void k_sw_site_chunk(int c, int n_chunks,
     swx += aw * wx;
     swy += aw * wy;
     swz += aw * wz;
+  };
+  if constexpr (L == 4) {
+#pragma unroll
+    for (int s = 0; s < L; ++s) {
+      double wx = seg_get<L>(ux, seg, s), wy = seg_get<L>(uy, seg, s), wz = seg_get<L>(uz, seg, s);
+      double wri = seg_get<L>(rinv, seg, s), wE = seg_get<L>(E, seg, s);
+      if (s == k || !leg || wri == 0.0) continue;
+      add_leg(wx, wy, wz, wri, wE);
+    }
+  } else {
+    // Long rows (a Verlet skin, or second shells beyond r_c): the legs
+    // inside r_c are compacted into LDS in slot order, and the partner loop
+    // runs over them only -- the same terms in the same order as a loop over
+    // every slot that skips the inactive ones.
+    const ActiveLegs<L> al(leg, k, seg);
+    __shared__ double s_x[256], s_y[256], s_z[256], s_ri[256], s_E[256];
+    const int base = (int)threadIdx.x & ~(L - 1);
+    if (leg) {
+      s_x[base + al.rank] = ux;
+      s_y[base + al.rank] = uy;
+      s_z[base + al.rank] = uz;
+      s_ri[base + al.rank] = rinv;
+      s_E[base + al.rank] = E;
+    }
+    __syncthreads();
+    for (int t = 0; t < al.wave_max; ++t) {
+      if (!leg || t >= al.count || t == al.rank) continue;
+      add_leg(s_x[base + t], s_y[base + t], s_z[base + t], s_ri[base + t], s_E[base + t]);
+    }
+    if (live && k == 0) amask[ac] = al.mask;
   }
   const double Gx = su * ux + swx, Gy = su * uy + swy, Gz = su * uz + swz;
-  if (has) {
+  // inactive legs have G = 0 exactly; with long rows the gather skips them
+  if (has && (L == 4 || leg)) {
     const size_t plane = (size_t)n * stride;
     G[slot] = Gx;
     G[plane + slot] = Gy;
@@ -614,12 +683,15 @@ __device__ __forceinline__ void k_sw_site_chunk(int c, int n_chunks,
 }
 
 template <int L>
-__global__ __launch_bounds__(256) void k_sw_site(int n_chunks, int n, const double* __restrict__ pos, const int* __restrict__ sid,
+__global__ __launch_bounds__(256) void k_sw_site(
+    int n_chunks, int n, const double* __restrict__ pos, const int* __restrict__ sid,
     const Geom* __restrict__ geoms, int single, const double* __restrict__ prm,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
-    const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial) {
+    const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
+    uint64_t* __restrict__ amask) {
   for (int c = blockIdx.x; c < n_chunks; c += gridDim.x)
-    k_sw_site_chunk<L>(c, n_chunks, n, pos, sid, geoms, single, prm, cnt, nbr, stride, chunk_rng, G, partial);
+    k_sw_site_chunk<L>(c, n_chunks, n, pos, sid, geoms, single, prm, cnt, nbr, stride,
+                       chunk_rng, G, partial, amask);
 }
 
 // Per-structure energy and stress of structure s by one block of 256
@@ -670,7 +742,8 @@ __global__ __launch_bounds__(256) void k_gather(int n, const int* __restrict__ c
                                                 const Geom* __restrict__ geoms,
                                                 const int* __restrict__ first, int n_chunks,
                                                 const double* __restrict__ partial,
-                                                double* __restrict__ out) {
+                                                double* __restrict__ out,
+                                                const uint64_t* __restrict__ amask) {
   if ((int)blockIdx.x >= atom_blocks) {
     struct_reduce_block((int)blockIdx.x - atom_blocks, geoms, first, n_chunks, partial, out);
     return;
@@ -687,9 +760,24 @@ __global__ __launch_bounds__(256) void k_gather(int n, const int* __restrict__ c
     if (k < c && r >= 0) {
       const size_t plane = (size_t)n * stride;
       const size_t rs = (size_t)r;
-      fx = G[s] - G[rs];
-      fy = G[plane + s] - G[plane + rs];
-      fz = G[2 * plane + s] - G[2 * plane + rs];
+      if constexpr (L == 4) {
+        fx = G[s] - G[rs];
+        fy = G[plane + s] - G[plane + rs];
+        fz = G[2 * plane + s] - G[2 * plane + rs];
+      } else {
+        // rows of stride L: slot r is entry r % L of atom r / L; only the
+        // bonds inside r_c were written (the others are zero)
+        if ((amask[a] >> k) & 1) {
+          fx = G[s];
+          fy = G[plane + s];
+          fz = G[2 * plane + s];
+        }
+        if ((amask[r / L] >> (r & (L - 1))) & 1) {
+          fx -= G[rs];
+          fy -= G[plane + rs];
+          fz -= G[2 * plane + rs];
+        }
+      }
     }
   }
   fx = seg_sum<L>(fx);
@@ -937,6 +1025,7 @@ static void launch_bond_kernels(Potential& pot, NList& nl, SetDev& set, const do
   const int* nbr = nl.nbr.as<int>();
   const int* rng = ch.rng.as<int>();
   double* part = ch.partial.as<double>();
+  uint64_t* amask = pot.amask.as<uint64_t>();
   if (pot.kind == KIND_TERSOFF) {
     const double* tb = pot.d_params.as<double>();
     const double* tri = tb + (size_t)pot.ns * pot.ns * 9;
@@ -946,23 +1035,25 @@ static void launch_bond_kernels(Potential& pot, NList& nl, SetDev& set, const do
       if (trip) {
         launch_timed(pot, sp1 ? k_tersoff_trip<L, true> : k_tersoff_trip<L, false>, grid, block,
                      st, ch.n, n, d_pos, sid, geoms, single, d_spc, pot.ns, tb, tri, pot.r_cut,
-                     cnt, nbr, nl.stride, rng, G, part);
+                     cnt, nbr, nl.stride, rng, G, part, amask);
         done = true;
       }
     }
     if (!done)
       launch_timed(pot, sp1 ? k_tersoff_site<L, true> : k_tersoff_site<L, false>, grid, block, st,
                    ch.n, n, d_pos, sid, geoms, single, d_spc, pot.ns, tb, tri, pot.r_cut, cnt,
-                   nbr, nl.stride, rng, G, part);
+                   nbr, nl.stride, rng, G, part, amask);
   } else {
     launch_timed(pot, k_sw_site<L>, grid, block, st, ch.n, n, d_pos, sid, geoms, single,
-                 (const double*)pot.d_params.as<double>(), cnt, nbr, nl.stride, rng, G, part);
+                 (const double*)pot.d_params.as<double>(), cnt, nbr, nl.stride, rng, G, part,
+                 amask);
   }
   const int64_t threads = (int64_t)n * L;
   const int atom_blocks = (int)((threads + 255) / 256);
   hipLaunchKernelGGL(k_gather<L>, dim3(atom_blocks + set.n_struct), dim3(256), 0, st, n, cnt,
                      (const int*)nl.rev.as<int>(), nl.stride, (const double*)G, d_F, atom_blocks,
-                     geoms, (const int*)ch.first.as<int>(), ch.n, (const double*)part, d_out);
+                     geoms, (const int*)ch.first.as<int>(), ch.n, (const double*)part, d_out,
+                     (const uint64_t*)amask);
 }
 
 template <int LE>
@@ -1019,6 +1110,7 @@ void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
       }
     } else {
       pot.G.ensure(sizeof(double) * 3 * (size_t)n * nl.stride);
+      pot.amask.ensure(sizeof(uint64_t) * (size_t)n);
       switch (lanes) {
         case 4: launch_bond_kernels<4>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, st); break;
         case 8: launch_bond_kernels<8>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, st); break;

@@ -17,32 +17,55 @@ namespace ak {
 // (1.60217663e-19 / 1e-10) / 1.66053906660e-27 * 1e-20
 constexpr double kConv = (1.60217663e-19 / 1.0e-10) / (1.66053906660e-27) * 1.0e-20;
 
-__global__ void k_md_drift(int n, double* __restrict__ pos, double* __restrict__ vel,
-                           const double* __restrict__ acc, double dt, Geom g) {
-  int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= n) return;
-  double x[3];
+// Half kick, drift and wrap into the cell, and this block's largest squared
+// displacement since the list was built (minimum image of x - x_build):
+//   v += dt/2 a; x += dt v; frac = x @ inv; frac -= floor(frac);
+//   frac[frac >= 1] = 0; x = frac @ lat.
+__global__ __launch_bounds__(256) void k_md_drift(int n, double* __restrict__ pos,
+                                                  double* __restrict__ vel,
+                                                  const double* __restrict__ acc, double dt,
+                                                  Geom g, const double* __restrict__ ref,
+                                                  double* __restrict__ part) {
+  __shared__ double m[256];
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  double d2 = 0.0;
+  if (a < n) {
+    double x[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    double v = vel[3 * a + d] + 0.5 * dt * acc[3 * a + d];
-    vel[3 * a + d] = v;
-    x[d] = pos[3 * a + d] + dt * v;
+    for (int d = 0; d < 3; ++d) {
+      double v = vel[3 * a + d] + 0.5 * dt * acc[3 * a + d];
+      vel[3 * a + d] = v;
+      x[d] = pos[3 * a + d] + dt * v;
+    }
+    double f[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      double s = x[0] * g.inv[b] + x[1] * g.inv[3 + b] + x[2] * g.inv[6 + b];
+      s -= floor(s);
+      f[b] = s >= 1.0 ? 0.0 : s;
+    }
+    double y[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      y[b] = f[0] * g.lat[b] + f[1] * g.lat[3 + b] + f[2] * g.lat[6 + b];
+      pos[3 * a + b] = y[b];
+    }
+    double dx, dy, dz;
+    mic_exact(g.lat, g.inv, ref[3 * a], ref[3 * a + 1], ref[3 * a + 2], y[0], y[1], y[2], dx,
+              dy, dz);
+    d2 = dx * dx + dy * dy + dz * dz;
   }
-  // frac = x @ inv; frac -= floor(frac); frac[frac >= 1] = 0; x = frac @ lat
-  double f[3];
-#pragma unroll
-  for (int b = 0; b < 3; ++b) {
-    double s = x[0] * g.inv[b] + x[1] * g.inv[3 + b] + x[2] * g.inv[6 + b];
-    s -= floor(s);
-    f[b] = s >= 1.0 ? 0.0 : s;
+  m[threadIdx.x] = d2;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) m[threadIdx.x] = fmax(m[threadIdx.x], m[threadIdx.x + o]);
+    __syncthreads();
   }
-#pragma unroll
-  for (int b = 0; b < 3; ++b)
-    pos[3 * a + b] = f[0] * g.lat[b] + f[1] * g.lat[3 + b] + f[2] * g.lat[6 + b];
+  if (threadIdx.x == 0) part[blockIdx.x] = m[0];
 }
 
-// a = F conv / m; v += dt/2 a; per-block kinetic-energy partial sums
-// (0.5 m v^2 / conv).
+// a = F conv / m; v += dt/2 a (kick != 0); per-block kinetic-energy sums
+// 0.5 m v^2 / conv.
 __global__ __launch_bounds__(256) void k_md_kick(int n, double* __restrict__ vel,
                                                  double* __restrict__ acc,
                                                  const double* __restrict__ F,
@@ -71,23 +94,45 @@ __global__ __launch_bounds__(256) void k_md_kick(int n, double* __restrict__ vel
   if (threadIdx.x == 0) ke_part[blockIdx.x] = ke[0];
 }
 
-// Record E_pot, E_kin at the current step and count rebuilds (one block).
-__global__ __launch_bounds__(256) void k_md_record(int nb, const double* __restrict__ ke_part,
-                                                   const double* __restrict__ es,
-                                                   const int* __restrict__ flag,
-                                                   int* __restrict__ counters,
-                                                   double* __restrict__ e_pot,
-                                                   double* __restrict__ e_kin) {
+// One block.  Records the state the previous kick left -- E_pot = es[0] and
+// E_kin = sum of the kick's block sums in fixed order -- at index
+// counters[0] and advances it; with disp_part != nullptr, also the Verlet-
+// skin test (engine/neighborlist.py:653-673, fixed cell: rebuild when
+// r_list - 2 max|MIC(x - x_build)| <= r_cut) -> *flag, counted in
+// counters[1].
+__global__ __launch_bounds__(256) void k_md_flag(int nb, const double* __restrict__ ke_part,
+                                                 const double* __restrict__ es,
+                                                 const double* __restrict__ disp_part,
+                                                 double r_list, double r_cut,
+                                                 int* __restrict__ flag,
+                                                 int* __restrict__ counters,
+                                                 double* __restrict__ e_pot,
+                                                 double* __restrict__ e_kin) {
   __shared__ double smem[16];
+  __shared__ double m[256];
   double v[1] = {0.0};
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += ke_part[b];
+  double d = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    v[0] += ke_part[b];
+    if (disp_part) d = fmax(d, disp_part[b]);
+  }
   block_sum<1>(v, smem);
+  m[threadIdx.x] = d;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) m[threadIdx.x] = fmax(m[threadIdx.x], m[threadIdx.x + o]);
+    __syncthreads();
+  }
   if (threadIdx.x == 0) {
     int step = counters[0];
     e_pot[step] = es[0];
     e_kin[step] = v[0];
     counters[0] = step + 1;
-    if (flag) counters[1] += *flag;
+    if (disp_part) {
+      const int fl = (r_list - 2.0 * sqrt(m[0]) <= r_cut) ? 1 : 0;
+      *flag = fl;
+      counters[1] += fl;
+    }
   }
 }
 
@@ -95,23 +140,35 @@ struct MDState {
   DevBuf pos, vel, acc, F, mass, spc, es, ke, hist_ep, hist_ek, counters, flag;
 };
 
+// One step: drift (+ block displacement maxima); record of the previous state
+// and the skin test; the gated list rebuild; forces; kick.  The state after
+// the last step is recorded by one more k_md_flag after the loop.
 static void md_step(Potential& pot, NList& nl, SetDev& set, MDState& S, int n, double dt,
                     hipStream_t st) {
   const unsigned nb = (unsigned)((n + 255) / 256);
+  nl.disp_part.ensure(sizeof(double) * nb);
   hipLaunchKernelGGL(k_md_drift, dim3(nb), dim3(256), 0, st, n, S.pos.as<double>(),
-                     S.vel.as<double>(), (const double*)S.acc.as<double>(), dt, set.geoms[0]);
-  Mat3 I = {{1, 0, 0, 0, 1, 0, 0, 0, 1}};
-  nl_check_displacement(nl, set, S.pos.as<double>(), I, 1.0, S.flag.as<int>(), nullptr, st);
+                     S.vel.as<double>(), (const double*)S.acc.as<double>(), dt, set.geoms[0],
+                     (const double*)nl.ref_pos.as<double>(), nl.disp_part.as<double>());
+  hipLaunchKernelGGL(k_md_flag, dim3(1), dim3(256), 0, st, (int)nb,
+                     (const double*)S.ke.as<double>(), (const double*)S.es.as<double>(),
+                     (const double*)nl.disp_part.as<double>(), nl.r_list, nl.r_cut,
+                     S.flag.as<int>(), S.counters.as<int>(), S.hist_ep.as<double>(),
+                     S.hist_ek.as<double>());
   nl_build_gated(nl, set, S.pos.as<double>(), S.flag.as<int>(), st);
   compute_forces(pot, nl, set, S.pos.as<double>(), S.spc.as<int>(), S.F.as<double>(),
                  S.es.as<double>(), st);
   hipLaunchKernelGGL(k_md_kick, dim3(nb), dim3(256), 0, st, n, S.vel.as<double>(),
                      S.acc.as<double>(), (const double*)S.F.as<double>(),
                      (const double*)S.mass.as<double>(), dt, 1, S.ke.as<double>());
-  hipLaunchKernelGGL(k_md_record, dim3(1), dim3(256), 0, st, (int)nb,
+}
+
+static void md_record(MDState& S, int n, hipStream_t st) {
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_md_flag, dim3(1), dim3(256), 0, st, (int)nb,
                      (const double*)S.ke.as<double>(), (const double*)S.es.as<double>(),
-                     (const int*)S.flag.as<int>(), S.counters.as<int>(), S.hist_ep.as<double>(),
-                     S.hist_ek.as<double>());
+                     (const double*)nullptr, 0.0, 0.0, (int*)nullptr, S.counters.as<int>(),
+                     S.hist_ep.as<double>(), S.hist_ek.as<double>());
 }
 
 void md_nve_run(Potential& pot, int n, const double* lattice, const double* inv, double* pos,
@@ -161,22 +218,20 @@ void md_nve_run(Potential& pot, int n, const double* lattice, const double* inv,
   nl.valid = false;
   nl_build(nl, set, S.pos.as<double>(), st);
 
-  // initial forces and the step-0 record
+  // initial forces: acceleration and the kinetic-energy sums of state 0
   compute_forces(pot, nl, set, S.pos.as<double>(), S.spc.as<int>(), S.F.as<double>(),
                  S.es.as<double>(), st);
   hipLaunchKernelGGL(k_md_kick, dim3(nb), dim3(256), 0, st, n, S.vel.as<double>(),
                      S.acc.as<double>(), (const double*)S.F.as<double>(),
                      (const double*)S.mass.as<double>(), dt, 0, S.ke.as<double>());
-  hipLaunchKernelGGL(k_md_record, dim3(1), dim3(256), 0, st, (int)nb,
-                     (const double*)S.ke.as<double>(), (const double*)S.es.as<double>(),
-                     (const int*)nullptr, S.counters.as<int>(), S.hist_ep.as<double>(),
-                     S.hist_ek.as<double>());
   AK_HIP(hipMemsetAsync(nl.stats.p, 0, sizeof(int) * 8, st));
-  // make sure every scratch buffer a step touches exists before capture
-  md_step(pot, nl, set, S, n, 0.0, st);  // dt = 0: positions unchanged, record overwritten
+  // make sure every scratch buffer a step touches exists before capture: a
+  // step with dt = 0 leaves positions, velocities and the state-0 sums as
+  // they were
+  md_step(pot, nl, set, S, n, 0.0, st);
   AK_HIP(hipStreamSynchronize(st));
-  // the warm-up step recorded at index 1: restart the record after step 0
-  int reset[4] = {1, 0, 0, 0};
+  // restart the record and the rebuild count: step k records state k - 1
+  int reset[4] = {0, 0, 0, 0};
   AK_HIP(hipMemcpy(S.counters.as<int>(), reset, sizeof(reset), hipMemcpyHostToDevice));
 
   auto t0 = std::chrono::steady_clock::now();
@@ -189,6 +244,7 @@ void md_nve_run(Potential& pot, int n, const double* lattice, const double* inv,
     AK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     t0 = std::chrono::steady_clock::now();
     for (int s = 0; s < n_steps; ++s) AK_HIP(hipGraphLaunch(exec, st));
+    md_record(S, n, st);
     AK_HIP(hipStreamSynchronize(st));
     auto t1 = std::chrono::steady_clock::now();
     if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
@@ -196,6 +252,7 @@ void md_nve_run(Potential& pot, int n, const double* lattice, const double* inv,
     (void)hipGraphDestroy(graph);
   } else {
     for (int s = 0; s < n_steps; ++s) md_step(pot, nl, set, S, n, dt, st);
+    md_record(S, n, st);
     AK_HIP(hipStreamSynchronize(st));
     auto t1 = std::chrono::steady_clock::now();
     if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();

@@ -2,8 +2,9 @@
 // NeighbourList._build_cell_linked, engine/neighborlist.py:935-1024, and its
 // pair search _cell_linked_pairs_numpy :312-373).
 //
-// 1. k_cell_id: per atom, its cell in the structure's grid
-//    (nc_i = floor(d_i / r_list), fractional coordinates wrapped into [0, 1)).
+// 1. cell_of (in the first histogram pass): per atom, its cell in the
+//    structure's grid (nc_i = floor(d_i / r_list), fractional coordinates
+//    wrapped into [0, 1)).
 // 2. Stable LSD radix sort of (cell, atom) by cell with 8-bit digits, free of
 //    atomics: digit counts per wavefront come from ballot match masks, block
 //    histograms are scanned globally, and each item's rank is its position
@@ -194,12 +195,10 @@ void SetDev::upload_geoms(hipStream_t st) {
 
 __device__ __forceinline__ bool gated_off(const int* gate) { return gate && *gate == 0; }
 
-__global__ void k_cell_id(int n, const double* __restrict__ pos, const int* __restrict__ sid,
-                          const Geom* __restrict__ geoms, unsigned* __restrict__ key,
-                          unsigned* __restrict__ val, const int* gate) {
-  if (gated_off(gate)) return;
-  int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= n) return;
+// Cell of atom a in its structure's grid.
+__device__ __forceinline__ unsigned cell_of(int a, const double* __restrict__ pos,
+                                            const int* __restrict__ sid,
+                                            const Geom* __restrict__ geoms) {
   const Geom& g = geoms[sid[a]];
   double p0 = pos[3 * a], p1 = pos[3 * a + 1], p2 = pos[3 * a + 2];
   int c[3];
@@ -212,8 +211,7 @@ __global__ void k_cell_id(int n, const double* __restrict__ pos, const int* __re
     k = k < 0 ? 0 : (k > g.nc[b] - 1 ? g.nc[b] - 1 : k);
     c[b] = k;
   }
-  key[a] = (unsigned)(g.cell_off + (c[0] * g.nc[1] + c[1]) * g.nc[2] + c[2]);
-  val[a] = (unsigned)a;
+  return (unsigned)(g.cell_off + (c[0] * g.nc[1] + c[1]) * g.nc[2] + c[2]);
 }
 
 // Lanes of this wavefront holding the same 8-bit digit as the calling lane.
@@ -230,10 +228,16 @@ __device__ __forceinline__ uint64_t match_digit(unsigned d, bool valid) {
 
 constexpr int RS_T = 256;
 
-__global__ __launch_bounds__(RS_T) void k_rs_hist(int n, const unsigned* __restrict__ keys,
+// Digit histogram of one radix pass.  With pos != nullptr (first pass) the
+// keys are computed here -- the cell of each atom -- and stored with the atom
+// index as the value.
+__global__ __launch_bounds__(RS_T) void k_rs_hist(int n, unsigned* __restrict__ keys,
                                                   int shift, int chunk,
                                                   unsigned* __restrict__ hist, int B,
-                                                  const int* gate) {
+                                                  const int* gate, const double* __restrict__ pos,
+                                                  const int* __restrict__ sid,
+                                                  const Geom* __restrict__ geoms,
+                                                  unsigned* __restrict__ vals) {
   if (gated_off(gate)) return;
   __shared__ unsigned cnt[RS_T / 64][256];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -243,7 +247,17 @@ __global__ __launch_bounds__(RS_T) void k_rs_hist(int n, const unsigned* __restr
   for (int t = 0; t < chunk; t += RS_T) {
     int idx = base + t + tid;
     bool v = idx < n;
-    unsigned d = v ? (keys[idx] >> shift) & 255u : 0u;
+    unsigned key = 0u;
+    if (v) {
+      if (pos) {
+        key = cell_of(idx, pos, sid, geoms);
+        keys[idx] = key;
+        vals[idx] = (unsigned)idx;
+      } else {
+        key = keys[idx];
+      }
+    }
+    unsigned d = (key >> shift) & 255u;
     uint64_t m = match_digit(d, v);
     if (v && lane == __ffsll((unsigned long long)m) - 1) cnt[wave][d] += __popcll(m);
   }
@@ -256,31 +270,47 @@ __global__ __launch_bounds__(RS_T) void k_rs_hist(int n, const unsigned* __restr
 }
 
 // In-place exclusive scan of m unsigned values by one block of 1024 threads;
-// *total (if given) receives the sum.
+// *total (if given) receives the sum.  Thread t owns the contiguous run
+// [t * per, (t + 1) * per) and reads it eight independent loads at a time;
+// the per-thread sums are scanned with wavefront shuffles and one LDS pass.
 __global__ __launch_bounds__(1024) void k_scan_excl(unsigned* __restrict__ data, int m,
                                                     unsigned* total, const int* gate) {
   if (gated_off(gate)) return;
-  __shared__ unsigned part[1024];
-  const int tid = threadIdx.x;
+  __shared__ unsigned wsum[16];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int per = (m + 1023) / 1024;
-  const int lo = tid * per, hi = min(m, lo + per);
+  const int lo = min(m, tid * per), hi = min(m, lo + per);
   unsigned s = 0;
-  for (int i = lo; i < hi; ++i) s += data[i];
-  part[tid] = s;
+  for (int i = lo; i < hi; i += 8) {
+    unsigned v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = i + q < hi ? data[i + q] : 0u;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s += v[q];
+  }
+  // inclusive scan of s over the block
+  unsigned inc = s;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    unsigned y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += y;
+  }
+  if (lane == 63) wsum[wid] = inc;
   __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    unsigned v = tid >= o ? part[tid - o] : 0u;
-    __syncthreads();
-    part[tid] += v;
-    __syncthreads();
+  unsigned before = 0;
+  for (int w = 0; w < wid; ++w) before += wsum[w];
+  unsigned run = before + inc - s;  // exclusive prefix of this thread's run
+  for (int i = lo; i < hi; i += 8) {
+    unsigned v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = i + q < hi ? data[i + q] : 0u;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      if (i + q < hi) data[i + q] = run;
+      run += v[q];
+    }
   }
-  unsigned run = part[tid] - s;  // exclusive prefix of this thread's chunk
-  for (int i = lo; i < hi; ++i) {
-    unsigned x = data[i];
-    data[i] = run;
-    run += x;
-  }
-  if (total && tid == 1023) *total = part[1023];
+  if (total && tid == 1023) *total = before + inc;
 }
 
 __global__ __launch_bounds__(RS_T) void k_rs_scatter(int n, const unsigned* __restrict__ kin,
@@ -405,10 +435,15 @@ __global__ __launch_bounds__(64) void k_nl_search(
   for (int q = lane; q < npow; q += 64) {
     unsigned v = 0xFFFFFFFFu;
     if (q < ncand) {
-      int st = 0;
-      while (spre[st + 1] <= q) ++st;
-      int cc = scell[st];
-      v = cell_atoms[cell_start[cc] + (q - spre[st])];
+      int lo = 0, hi = nst;  // largest st with spre[st] <= q
+      while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (spre[mid] <= q)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      v = cell_atoms[cell_start[scell[lo]] + (q - spre[lo])];
     }
     cand[q] = v;
   }
@@ -460,6 +495,143 @@ __global__ __launch_bounds__(64) void k_nl_search(
   }
 }
 
+// Register-resident variant of k_nl_search for stencils of at most 64 * EPL
+// candidates: lane l holds candidates l, l + 64, ...; the bitonic network
+// runs on registers (shuffles across lanes, swaps within a lane), and each
+// candidate's position is loaded once per cell.  Same pairs, same order.
+template <int EPL>
+__global__ __launch_bounds__(64) void k_nl_search_reg(
+    int C, int S, const int* __restrict__ cell_offs, const Geom* __restrict__ geoms,
+    const int* __restrict__ cell_start, const unsigned* __restrict__ cell_atoms,
+    const double* __restrict__ pos, double rl2, int cap, int* __restrict__ cnt,
+    int* __restrict__ nbr, int* __restrict__ stats, const int* gate) {
+  if (gated_off(gate)) return;
+  constexpr int NP = 64 * EPL;
+  __shared__ int spre[28];
+  __shared__ int scell[27];
+  const int c = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int s = find_struct(cell_offs, S, c);
+  const Geom& g = geoms[s];
+  const int nc0 = g.nc[0], nc1 = g.nc[1], nc2 = g.nc[2];
+  const int l = c - g.cell_off;
+  const int c0 = l / (nc1 * nc2), c1 = (l / nc2) % nc1, c2 = l % nc2;
+  const int n0 = nc0 == 1 ? 1 : (nc0 == 2 ? 2 : 3);
+  const int n1 = nc1 == 1 ? 1 : (nc1 == 2 ? 2 : 3);
+  const int n2 = nc2 == 1 ? 1 : (nc2 == 2 ? 2 : 3);
+  const int nst = n0 * n1 * n2;
+  int size = 0, cid = 0;
+  if (lane < nst) {
+    int a0 = lane / (n1 * n2), a1 = (lane / n2) % n1, a2 = lane % n2;
+    int o0 = n0 == 3 ? a0 - 1 : a0, o1 = n1 == 3 ? a1 - 1 : a1, o2 = n2 == 3 ? a2 - 1 : a2;
+    int j0 = (c0 + o0 + nc0) % nc0, j1 = (c1 + o1 + nc1) % nc1, j2 = (c2 + o2 + nc2) % nc2;
+    cid = g.cell_off + (j0 * nc1 + j1) * nc2 + j2;
+    size = cell_start[cid + 1] - cell_start[cid];
+  }
+  int inc = size;
+#pragma unroll
+  for (int o = 1; o < 32; o <<= 1) {
+    int y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += y;
+  }
+  const int ncand = __shfl(inc, 31, 64);
+  if (lane < nst) {
+    spre[lane + 1] = inc;
+    scell[lane] = cid;
+  }
+  if (lane == 0) spre[0] = 0;
+  const int first = cell_start[c], last = cell_start[c + 1];
+  if (ncand > NP) {
+    if (lane == 0) stats[1] = max(stats[1], ncand);  // candidate overflow
+    for (int t = first + lane; t < last; t += 64) cnt[cell_atoms[t]] = 0;
+    return;
+  }
+  __syncthreads();
+  unsigned v[EPL];
+#pragma unroll
+  for (int r = 0; r < EPL; ++r) {
+    const int q = lane + 64 * r;
+    v[r] = 0xFFFFFFFFu;
+    if (q < ncand) {
+      // stencil cell holding candidate q: largest st with spre[st] <= q
+      int lo = 0, hi = nst;
+      while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (spre[mid] <= q)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      v[r] = cell_atoms[cell_start[scell[lo]] + (q - spre[lo])];
+    }
+  }
+  // bitonic sort of the NP elements e = lane + 64 r, ascending
+#pragma unroll
+  for (int k = 2; k <= NP; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j >= 64) {
+#pragma unroll
+        for (int r = 0; r < EPL; ++r) {
+          const int pr = r ^ (j >> 6);
+          if (pr > r) {
+            const bool up = ((lane + 64 * r) & k) == 0;
+            const unsigned x = v[r], y = v[pr];
+            if ((x > y) == up) {
+              v[r] = y;
+              v[pr] = x;
+            }
+          }
+        }
+      } else {
+        const bool lower = (lane & j) == 0;
+#pragma unroll
+        for (int r = 0; r < EPL; ++r) {
+          const bool up = ((lane + 64 * r) & k) == 0;
+          const unsigned x = v[r];
+          const unsigned y = (unsigned)__shfl_xor((int)x, j, 64);
+          v[r] = (lower == up) ? min(x, y) : max(x, y);
+        }
+      }
+    }
+  }
+  double px[EPL], py[EPL], pz[EPL];
+#pragma unroll
+  for (int r = 0; r < EPL; ++r) {
+    const bool ok = lane + 64 * r < ncand;
+    const int j = ok ? (int)v[r] : 0;
+    px[r] = ok ? pos[3 * j] : 0.0;
+    py[r] = ok ? pos[3 * j + 1] : 0.0;
+    pz[r] = ok ? pos[3 * j + 2] : 0.0;
+  }
+  for (int t = first; t < last; ++t) {
+    const int i = (int)cell_atoms[t];
+    const double pix = pos[3 * i], piy = pos[3 * i + 1], piz = pos[3 * i + 2];
+    int count = 0;
+#pragma unroll
+    for (int r = 0; r < EPL; ++r) {
+      if (64 * r >= ncand) break;
+      bool hit = false;
+      if (lane + 64 * r < ncand) {
+        double dx, dy, dz;
+        mic_exact(g.lat, g.inv, pix, piy, piz, px[r], py[r], pz[r], dx, dy, dz);
+        double r2 = r2_exact(dx, dy, dz);
+        hit = r2 < rl2 && r2 > 1e-20;
+      }
+      uint64_t m = __ballot(hit);
+      if (hit) {
+        int slot = count + __popcll(m & lanemask_lt());
+        if (slot < cap) nbr[(size_t)i * cap + slot] = (int)v[r];
+      }
+      count += __popcll(m);
+    }
+    if (lane == 0) {
+      cnt[i] = count;
+      if (count > cap) stats[2] = max(stats[2], count);  // row overflow
+    }
+  }
+}
+
 // stats[0] = max row length over atoms (one block).
 __global__ __launch_bounds__(1024) void k_nl_maxcnt(int n, const int* __restrict__ cnt,
                                                     int* __restrict__ stats, const int* gate) {
@@ -484,10 +656,14 @@ __global__ void k_nl_repack(int n, const int* __restrict__ cnt, const int* __res
   dst[t] = k < cnt[a] ? src[(size_t)a * cap + k] : -1;
 }
 
+// Also copies the build positions (copy_src -> copy_dst, 3n doubles) when
+// given: the reference positions of the next displacement test.
 __global__ void k_nl_rev(int n, const int* __restrict__ cnt, const int* __restrict__ nbr,
-                         int stride, int* __restrict__ rev, int* stats, const int* gate) {
+                         int stride, int* __restrict__ rev, int* stats, const int* gate,
+                         const double* __restrict__ copy_src, double* __restrict__ copy_dst) {
   if (gated_off(gate)) return;
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (copy_src && t < 3 * (int64_t)n) copy_dst[t] = copy_src[t];
   if (t >= (int64_t)n * stride) return;
   int a = (int)(t / stride), k = (int)(t % stride);
   if (k >= cnt[a]) {
@@ -510,13 +686,6 @@ __global__ void k_nl_rev(int n, const int* __restrict__ cnt, const int* __restri
     rev[t] = -1;
     stats[3] = 1;  // asymmetric list: should never happen
   }
-}
-
-__global__ void k_copy_pos(int n, const double* __restrict__ src, double* __restrict__ dst,
-                           const int* gate) {
-  if (gated_off(gate)) return;
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < 3 * n) dst[t] = src[t];
 }
 
 // Per-block max of |MIC(pos - ref @ M)|^2: the non-affine displacement since
@@ -610,13 +779,9 @@ static void sort_cells(NList& nl, SetDev& set, const double* d_pos, hipStream_t 
   nl.keys_b.ensure(sizeof(unsigned) * (n + 1));
   nl.vals_b.ensure(sizeof(unsigned) * (n + 1));
   nl.cell_start.ensure(sizeof(int) * (C + 2));
-  if (n > 0)
-    hipLaunchKernelGGL(k_cell_id, dim3(nblk(n, 256)), dim3(256), 0, st, n, d_pos,
-                       set.d_sid.as<int>(), set.d_geoms.as<Geom>(), nl.keys_a.as<unsigned>(),
-                       nl.vals_a.as<unsigned>(), gate);
   int bits = 0;
   while ((1LL << bits) < (long long)C) ++bits;
-  int passes = (bits + 7) / 8;
+  int passes = std::max(1, (bits + 7) / 8);
   int tpb = std::max(1, (n + 256 * 512 - 1) / (256 * 512));
   int chunk = 256 * tpb;
   int B = std::max(1, (n + chunk - 1) / chunk);
@@ -626,7 +791,8 @@ static void sort_cells(NList& nl, SetDev& set, const double* d_pos, hipStream_t 
   for (int p = 0; p < passes && n > 0; ++p) {
     int shift = 8 * p;
     hipLaunchKernelGGL(k_rs_hist, dim3(B), dim3(RS_T), 0, st, n, ka, shift, chunk,
-                       nl.hist.as<unsigned>(), B, gate);
+                       nl.hist.as<unsigned>(), B, gate, p == 0 ? d_pos : nullptr,
+                       (const int*)set.d_sid.as<int>(), (const Geom*)set.d_geoms.as<Geom>(), va);
     hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, nl.hist.as<unsigned>(),
                        256 * B, (unsigned*)nullptr, gate);
     hipLaunchKernelGGL(k_rs_scatter, dim3(B), dim3(RS_T), 0, st, n, ka, va, kb, vb, shift,
@@ -648,11 +814,50 @@ static void sort_cells(NList& nl, SetDev& set, const double* d_pos, hipStream_t 
 static void launch_search(NList& nl, SetDev& set, const double* d_pos, int* d_cnt, int* d_nbr,
                           int cap, hipStream_t st, const int* gate) {
   const int C = set.n_cells;
-  size_t lds = sizeof(unsigned) * (size_t)nl.cmax;
-  hipLaunchKernelGGL(k_nl_search, dim3(C), dim3(64), lds, st, C, set.n_struct,
-                     set.d_cell_offs.as<int>(), set.d_geoms.as<Geom>(), nl.cell_start.as<int>(),
-                     nl.vals_a.as<unsigned>(), d_pos, nl.r_list * nl.r_list, cap, nl.cmax,
-                     d_cnt, d_nbr, nl.stats.as<int>(), gate);
+  const double rl2 = nl.r_list * nl.r_list;
+  const int* co = set.d_cell_offs.as<int>();
+  const Geom* ge = set.d_geoms.as<Geom>();
+  const int* cs = nl.cell_start.as<int>();
+  const unsigned* ca = nl.vals_a.as<unsigned>();
+  int* stt = nl.stats.as<int>();
+  const int S = set.n_struct;
+  switch (nl.cmax) {
+    case 64:
+      hipLaunchKernelGGL(k_nl_search_reg<1>, dim3(C), dim3(64), 0, st, C, S, co, ge, cs, ca,
+                         d_pos, rl2, cap, d_cnt, d_nbr, stt, gate);
+      return;
+    case 128:
+      hipLaunchKernelGGL(k_nl_search_reg<2>, dim3(C), dim3(64), 0, st, C, S, co, ge, cs, ca,
+                         d_pos, rl2, cap, d_cnt, d_nbr, stt, gate);
+      return;
+    case 256:
+      hipLaunchKernelGGL(k_nl_search_reg<4>, dim3(C), dim3(64), 0, st, C, S, co, ge, cs, ca,
+                         d_pos, rl2, cap, d_cnt, d_nbr, stt, gate);
+      return;
+    case 512:
+      hipLaunchKernelGGL(k_nl_search_reg<8>, dim3(C), dim3(64), 0, st, C, S, co, ge, cs, ca,
+                         d_pos, rl2, cap, d_cnt, d_nbr, stt, gate);
+      return;
+    default: {
+      size_t lds = sizeof(unsigned) * (size_t)nl.cmax;
+      hipLaunchKernelGGL(k_nl_search, dim3(C), dim3(64), lds, st, C, S, co, ge, cs, ca, d_pos,
+                         rl2, cap, nl.cmax, d_cnt, d_nbr, stt, gate);
+    }
+  }
+}
+
+// Candidate capacity per cell: twice the expected stencil population
+// (27 cells, or fewer along axes with fewer than three cells), a power of two
+// of at least 64.
+static int est_cmax(SetDev& set) {
+  double most = 0.0;
+  for (auto& g : set.geoms) {
+    const long long cells = (long long)g.nc[0] * g.nc[1] * g.nc[2];
+    if (cells <= 0) continue;
+    const int n0 = std::min(3, g.nc[0]), n1 = std::min(3, g.nc[1]), n2 = std::min(3, g.nc[2]);
+    most = std::max(most, (double)g.natoms / (double)cells * n0 * n1 * n2);
+  }
+  return pow2ceil(std::max(64, (int)(2.0 * most) + 8));
 }
 
 static int est_cap(SetDev& set, double r_list) {
@@ -669,7 +874,7 @@ void nl_build(NList& nl, SetDev& set, const double* d_pos, hipStream_t st) {
   sort_cells(nl, set, d_pos, st, nullptr);
   nl.cnt.ensure(sizeof(int) * (n + 1));
   if (nl.cap == 0) nl.cap = est_cap(set, nl.r_list);
-  if (nl.cmax == 0) nl.cmax = 2048;
+  if (nl.cmax == 0) nl.cmax = est_cmax(set);
   int h_stats[8];
   for (int attempt = 0; attempt < 8; ++attempt) {
     AK_HIP(hipMemsetAsync(nl.stats.p, 0, sizeof(int) * 8, st));
@@ -713,7 +918,7 @@ void nl_build(NList& nl, SetDev& set, const double* d_pos, hipStream_t st) {
                        nl.tmp_nbr.as<int>(), nl.cap, nl.nbr.as<int>(), stride);
     hipLaunchKernelGGL(k_nl_rev, dim3(nblk(tot, 256)), dim3(256), 0, st, n, nl.cnt.as<int>(),
                        nl.nbr.as<int>(), stride, nl.rev.as<int>(), nl.stats.as<int>(),
-                       (const int*)nullptr);
+                       (const int*)nullptr, (const double*)nullptr, (double*)nullptr);
   }
   nl.ref_pos.ensure(sizeof(double) * 3 * (size_t)std::max(1, n));
   if (n > 0)
@@ -734,9 +939,8 @@ void nl_build_gated(NList& nl, SetDev& set, const double* d_pos, const int* d_fl
   launch_search(nl, set, d_pos, nl.cnt.as<int>(), nl.nbr.as<int>(), nl.stride, st, d_flag);
   int64_t tot = (int64_t)n * nl.stride;
   hipLaunchKernelGGL(k_nl_rev, dim3(nblk(tot, 256)), dim3(256), 0, st, n, nl.cnt.as<int>(),
-                     nl.nbr.as<int>(), nl.stride, nl.rev.as<int>(), nl.stats.as<int>(), d_flag);
-  hipLaunchKernelGGL(k_copy_pos, dim3(nblk(3 * (int64_t)n, 256)), dim3(256), 0, st, n, d_pos,
-                     nl.ref_pos.as<double>(), d_flag);
+                     nl.nbr.as<int>(), nl.stride, nl.rev.as<int>(), nl.stats.as<int>(), d_flag,
+                     d_pos, nl.ref_pos.as<double>());
 }
 
 void nl_check_displacement(NList& nl, SetDev& set, const double* d_pos, const Mat3& M,
