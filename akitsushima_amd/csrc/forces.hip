@@ -791,6 +791,194 @@ __global__ __launch_bounds__(256) void k_gather(int n, const int* __restrict__ c
 }
 
 // ---------------------------------------------------------------------------
+// Stillinger-Weber with long rows (stride L >= 16, e.g. a Verlet skin that
+// reaches the second shell): LA lanes per atom instead of L.  Phase 1: lane q
+// tests slots q, q + LA, ... and the legs inside r_c are appended, in slot
+// order, to the atom's region of LDS (bond vector, length, exp(gamma/(x-a)),
+// slot).  Phase 2: lane q evaluates legs q, q + LA, ...: two-body term and
+// the three-body terms against every other active leg of the atom, in slot
+// order -- the same terms in the same order as k_sw_site -- and writes G to
+// the leg's slot.  Dynamic LDS: (block / LA) * L legs of 5 doubles + 1 int.
+// ---------------------------------------------------------------------------
+template <int L, int LA>
+__device__ __forceinline__ void k_sw_site_c_chunk(
+    int c, int n_chunks, int n, const double* __restrict__ pos, const int* __restrict__ sid,
+    const Geom* __restrict__ geoms, int single, const double* __restrict__ prm,
+    const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
+    const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
+    uint64_t* __restrict__ amask, double* __restrict__ lds) {
+  static_assert(L >= LA && L % LA == 0 && LA <= 64, "LA lanes cover the row in L / LA rounds");
+  const int apb = blockDim.x / LA;          // atoms per block
+  const int al = (int)threadIdx.x / LA;     // atom of this lane within the block
+  const int q = threadIdx.x & (LA - 1);
+  const int seg = (threadIdx.x & 63) & ~(LA - 1);
+  double* lx = lds;
+  double* ly = lx + apb * L;
+  double* lz = ly + apb * L;
+  double* lr = lz + apb * L;
+  double* lE = lr + apb * L;
+  int* lslot = (int*)(lE + apb * L);
+  const int reg = al * L;
+  const int a = chunk_rng[2 * c] + al;
+  const bool live = a < chunk_rng[2 * c + 1];
+  const int ac = live ? a : 0;
+  const int nk = live ? cnt[ac] : 0;
+  const double eps = prm[0], sig = prm[1], Asw = prm[2], Bsw = prm[3], pp = prm[4],
+               qq = prm[5], asw = prm[6], lam = prm[7], gam = prm[8], cth0 = prm[9];
+  const double rc = asw * sig;
+  const Geom& g = geoms[single ? 0 : sid[ac]];
+
+  // phase 1: active legs, compacted in slot order
+  int count = 0;
+  uint64_t mask = 0;
+#pragma unroll
+  for (int i = 0; i < L / LA; ++i) {
+    const int k = q + LA * i;
+    const bool has = live && k < nk;
+    double ux = 0.0, uy = 0.0, uz = 0.0, r = 0.0;
+    bool leg = false;
+    if (has) {
+      const int j = nbr[(size_t)ac * stride + k];
+      mic_fast(g, pos, ac, j, ux, uy, uz);
+      r = sqrt(ux * ux + uy * uy + uz * uz);
+      leg = r < rc && r > 1e-12 && r / sig < asw;
+    }
+    const uint64_t bal = __ballot(leg);
+    const uint64_t sm = LA == 64 ? bal : (bal >> seg) & ((uint64_t(1) << (LA & 63)) - 1);
+    if (leg) {
+      const int t = reg + count + __popcll(sm & ((uint64_t(1) << q) - 1));
+      lx[t] = ux;
+      ly[t] = uy;
+      lz[t] = uz;
+      lr[t] = r;
+      lE[t] = nb_exp(gam / (r / sig - asw));
+      lslot[t] = k;
+    }
+    mask |= sm << (LA * i);
+    count += __popcll(sm);
+  }
+  if (live && q == 0) amask[ac] = mask;
+  __syncthreads();
+
+  // phase 2: legs q, q + LA, ... of this atom
+  const double c3 = lam * eps;
+  double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int t = q; t < count; t += LA) {
+    const double ux = lx[reg + t], uy = ly[reg + t], uz = lz[reg + t], r = lr[reg + t];
+    const double E = lE[reg + t];
+    const double x = r / sig;
+    const double rinv = 1.0 / r;
+    const double dx_ = x - asw;
+    const double dlnE = -gam / (sig * dx_ * dx_);
+    const double tt = 1.0 / dx_;
+    const double ex = nb_exp(tt);
+    const double bx = Bsw * xpow(x, -pp) - xpow(x, -qq);
+    const double V2 = eps * Asw * bx * ex;
+    const double dV2 = eps * Asw * ex *
+                       ((-pp * Bsw * xpow(x, -pp - 1.0) + qq * xpow(x, -qq - 1.0)) + bx * (-tt * tt));
+    double e = 0.5 * V2;
+    double su = 0.5 * (dV2 / sig) * rinv;
+    double swx = 0.0, swy = 0.0, swz = 0.0;
+    for (int s2 = 0; s2 < count; ++s2) {
+      if (s2 == t) continue;
+      const double wx = lx[reg + s2], wy = ly[reg + s2], wz = lz[reg + s2];
+      const double wri = 1.0 / lr[reg + s2], wE = lE[reg + s2];
+      double cs = (ux * wx + uy * wy + uz * wz) * rinv * wri;
+      double dc = cs - cth0;
+      double pre = c3 * E * wE * dc;
+      e += 0.5 * pre * dc;
+      double rad = pre * dc * dlnE, ang = 2.0 * pre * rinv;
+      su += (rad - ang * cs) * rinv;
+      double aw = ang * wri;
+      swx += aw * wx;
+      swy += aw * wy;
+      swz += aw * wz;
+    }
+    const double Gx = su * ux + swx, Gy = su * uy + swy, Gz = su * uz + swz;
+    const size_t slot = (size_t)ac * stride + lslot[reg + t];
+    const size_t plane = (size_t)n * stride;
+    G[slot] = Gx;
+    G[plane + slot] = Gy;
+    G[2 * plane + slot] = Gz;
+    v[0] += e;
+    v[1] -= Gx * ux;
+    v[2] -= Gx * uy;
+    v[3] -= Gx * uz;
+    v[4] -= Gy * ux;
+    v[5] -= Gy * uy;
+    v[6] -= Gy * uz;
+    v[7] -= Gz * ux;
+    v[8] -= Gz * uy;
+    v[9] -= Gz * uz;
+  }
+  wave_partial(v, partial, c, n_chunks);
+}
+
+template <int L, int LA>
+__global__ __launch_bounds__(256) void k_sw_site_c(
+    int n_chunks, int n, const double* __restrict__ pos, const int* __restrict__ sid,
+    const Geom* __restrict__ geoms, int single, const double* __restrict__ prm,
+    const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
+    const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
+    uint64_t* __restrict__ amask) {
+  extern __shared__ double lds_sw[];
+  for (int c = blockIdx.x; c < n_chunks; c += gridDim.x)
+    k_sw_site_c_chunk<L, LA>(c, n_chunks, n, pos, sid, geoms, single, prm, cnt, nbr, stride,
+                             chunk_rng, G, partial, amask, lds_sw);
+}
+
+// Gather for long rows (stride L >= 16): LA lanes per atom, lane q summing
+// slots q, q + LA, ... (only bonds inside r_c hold G; see the masks).
+template <int L, int LA>
+__global__ __launch_bounds__(256) void k_gather_c(int n, const int* __restrict__ cnt,
+                                                  const int* __restrict__ rev, int stride,
+                                                  const double* __restrict__ G,
+                                                  double* __restrict__ F, int atom_blocks,
+                                                  const Geom* __restrict__ geoms,
+                                                  const int* __restrict__ first, int n_chunks,
+                                                  const double* __restrict__ partial,
+                                                  double* __restrict__ out,
+                                                  const uint64_t* __restrict__ amask) {
+  if ((int)blockIdx.x >= atom_blocks) {
+    struct_reduce_block((int)blockIdx.x - atom_blocks, geoms, first, n_chunks, partial, out);
+    return;
+  }
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = gt / LA;
+  const int q = gt & (LA - 1);
+  const bool live = a < n;
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  if (live) {
+    const int c = cnt[a];
+    const uint64_t ma = amask[a];
+    const size_t plane = (size_t)n * stride;
+    for (int k = q; k < c; k += LA) {
+      const size_t s = (size_t)a * stride + k;
+      const int r = rev[s];
+      if (r < 0) continue;
+      if ((ma >> k) & 1) {
+        fx += G[s];
+        fy += G[plane + s];
+        fz += G[2 * plane + s];
+      }
+      if ((amask[r / L] >> (r & (L - 1))) & 1) {
+        fx -= G[r];
+        fy -= G[plane + r];
+        fz -= G[2 * plane + r];
+      }
+    }
+  }
+  fx = seg_sum<LA>(fx);
+  fy = seg_sum<LA>(fy);
+  fz = seg_sum<LA>(fz);
+  if (live && q == 0) {
+    F[3 * (size_t)a] = fx;
+    F[3 * (size_t)a + 1] = fy;
+    F[3 * (size_t)a + 2] = fz;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Zhou EAM.  A function is a row of 44 doubles [code, r_c, width, chi,
 // pa[20], pb[20]]: code 3 density, 4 pair, 5 cross pair, 6 embedding;
 // element parameters in the order r_e, f_e, rho_e, rho_s, alpha, beta, A, B,
@@ -995,12 +1183,20 @@ __global__ __launch_bounds__(256) void k_eam_force(int n_chunks, const double* _
 // from the kernel's own dispatch packet (no extra barrier packets).
 template <typename K, typename... Args>
 static void launch_timed(Potential& pot, K kernel, dim3 grid, dim3 block, hipStream_t st,
-                         Args... args) {
+                         size_t lds, Args... args) {
   Potential::EventPair* ev = pot.prof_next();
   if (ev)
-    hipExtLaunchKernelGGL(kernel, grid, block, 0, st, ev->a, ev->b, 0, args...);
+    hipExtLaunchKernelGGL(kernel, grid, block, lds, st, ev->a, ev->b, 0, args...);
   else
-    hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
+    hipLaunchKernelGGL(kernel, grid, block, lds, st, args...);
+}
+
+// Rows of at least kLongRow slots run the SW site kernel and the gather with
+// kRowLanes lanes per atom (k_sw_site_c, k_gather_c).
+constexpr int kLongRow = 16;
+constexpr int kRowLanes = 8;
+static inline size_t sw_c_lds(int blk, int L) {
+  return (size_t)(blk / kRowLanes) * L * (5 * sizeof(double) + sizeof(int));
 }
 
 // Site-kernel grid: one block per chunk, or at most grid_cap blocks that
@@ -1014,11 +1210,11 @@ static inline dim3 chunk_grid(const Potential& pot, const SetDev::Chunks& ch) {
 template <int L>
 static void launch_bond_kernels(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
                                 const int* d_spc, double* d_F, double* d_out,
-                                SetDev::Chunks& ch, bool trip, hipStream_t st) {
+                                SetDev::Chunks& ch, bool trip, int blk, hipStream_t st) {
   const int n = set.n_atoms;
   const int single = set.n_struct == 1 ? 1 : 0;
   double* G = pot.G.as<double>();
-  const dim3 grid = chunk_grid(pot, ch), block(pot.block);
+  const dim3 grid = chunk_grid(pot, ch), block(blk);
   const int* sid = set.d_sid.as<int>();
   const Geom* geoms = set.d_geoms.as<Geom>();
   const int* cnt = nl.cnt.as<int>();
@@ -1034,26 +1230,38 @@ static void launch_bond_kernels(Potential& pot, NList& nl, SetDev& set, const do
     if constexpr (L * L <= 64) {
       if (trip) {
         launch_timed(pot, sp1 ? k_tersoff_trip<L, true> : k_tersoff_trip<L, false>, grid, block,
-                     st, ch.n, n, d_pos, sid, geoms, single, d_spc, pot.ns, tb, tri, pot.r_cut,
+                     st, (size_t)0, ch.n, n, d_pos, sid, geoms, single, d_spc, pot.ns, tb, tri, pot.r_cut,
                      cnt, nbr, nl.stride, rng, G, part, amask);
         done = true;
       }
     }
     if (!done)
       launch_timed(pot, sp1 ? k_tersoff_site<L, true> : k_tersoff_site<L, false>, grid, block, st,
-                   ch.n, n, d_pos, sid, geoms, single, d_spc, pot.ns, tb, tri, pot.r_cut, cnt,
+                   (size_t)0, ch.n, n, d_pos, sid, geoms, single, d_spc, pot.ns, tb, tri, pot.r_cut, cnt,
                    nbr, nl.stride, rng, G, part, amask);
+  } else if constexpr (L >= kLongRow) {
+    const size_t lds = sw_c_lds(blk, L);
+    launch_timed(pot, k_sw_site_c<L, kRowLanes>, grid, block, st, lds, ch.n, n, d_pos, sid,
+                 geoms, single, (const double*)pot.d_params.as<double>(), cnt, nbr, nl.stride,
+                 rng, G, part, amask);
   } else {
-    launch_timed(pot, k_sw_site<L>, grid, block, st, ch.n, n, d_pos, sid, geoms, single,
-                 (const double*)pot.d_params.as<double>(), cnt, nbr, nl.stride, rng, G, part,
-                 amask);
+    launch_timed(pot, k_sw_site<L>, grid, block, st, (size_t)0, ch.n, n, d_pos, sid, geoms,
+                 single, (const double*)pot.d_params.as<double>(), cnt, nbr, nl.stride, rng, G,
+                 part, amask);
   }
-  const int64_t threads = (int64_t)n * L;
-  const int atom_blocks = (int)((threads + 255) / 256);
-  hipLaunchKernelGGL(k_gather<L>, dim3(atom_blocks + set.n_struct), dim3(256), 0, st, n, cnt,
-                     (const int*)nl.rev.as<int>(), nl.stride, (const double*)G, d_F, atom_blocks,
-                     geoms, (const int*)ch.first.as<int>(), ch.n, (const double*)part, d_out,
-                     (const uint64_t*)amask);
+  const int* rev = nl.rev.as<int>();
+  const int* first = ch.first.as<int>();
+  if constexpr (L >= kLongRow) {
+    const int atom_blocks = (int)(((int64_t)n * kRowLanes + 255) / 256);
+    hipLaunchKernelGGL((k_gather_c<L, kRowLanes>), dim3(atom_blocks + set.n_struct), dim3(256),
+                       0, st, n, cnt, rev, nl.stride, (const double*)G, d_F, atom_blocks, geoms,
+                       first, ch.n, (const double*)part, d_out, (const uint64_t*)amask);
+  } else {
+    const int atom_blocks = (int)(((int64_t)n * L + 255) / 256);
+    hipLaunchKernelGGL(k_gather<L>, dim3(atom_blocks + set.n_struct), dim3(256), 0, st, n, cnt,
+                       rev, nl.stride, (const double*)G, d_F, atom_blocks, geoms, first, ch.n,
+                       (const double*)part, d_out, (const uint64_t*)amask);
+  }
 }
 
 template <int LE>
@@ -1069,7 +1277,8 @@ static void launch_eam(Potential& pot, NList& nl, SetDev& set, const double* d_p
                      set.d_sid.as<int>(), set.d_geoms.as<Geom>(), single, d_spc, pot.ns,
                      pot.d_params.as<double>(), pot.d_map.as<int>(), pot.r_cut,
                      nl.cnt.as<int>(), nl.nbr.as<int>(), nl.stride, dF, Fe);
-  launch_timed(pot, k_eam_force<LE>, chunk_grid(pot, ch), dim3(pot.block), st, ch.n, d_pos,
+  launch_timed(pot, k_eam_force<LE>, chunk_grid(pot, ch), dim3(pot.block), st, (size_t)0, ch.n,
+               d_pos,
                (const int*)set.d_sid.as<int>(), (const Geom*)set.d_geoms.as<Geom>(), single, d_spc,
                pot.ns, (const double*)pot.d_params.as<double>(),
                (const int*)pot.d_map.as<int>(), pot.r_cut, (const int*)nl.cnt.as<int>(),
@@ -1095,9 +1304,14 @@ void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
   // lanes per atom of the site kernel: L, or L * L for the Tersoff form with
   // one lane per (bond, partner) pair
   const bool trip = pot.kind == KIND_TERSOFF && lanes * lanes <= 64 && pot.bond_tile == 2;
-  const int atom_lanes = trip ? lanes * lanes : lanes;
-  AK_REQUIRE(atom_lanes <= pot.block, AK_ERR_STATE, "block smaller than one atom's lanes");
-  SetDev::Chunks& ch = set.chunks(pot.block / atom_lanes, st);
+  const bool sw_long = pot.kind == KIND_SW && lanes >= kLongRow;
+  const int atom_lanes = trip ? lanes * lanes : (sw_long ? kRowLanes : lanes);
+  int blk = pot.block;
+  // the long-row SW kernel stages every leg of its atoms in LDS: keep it
+  // within 48 KiB per block
+  while (sw_long && blk > 64 && sw_c_lds(blk, lanes) > 48 * 1024) blk >>= 1;
+  AK_REQUIRE(atom_lanes <= blk, AK_ERR_STATE, "block smaller than one atom's lanes");
+  SetDev::Chunks& ch = set.chunks(blk / atom_lanes, st);
   if (n > 0) {
     if (pot.kind == KIND_EAM) {
       pot.aux.ensure(sizeof(double) * 2 * (size_t)n);
@@ -1112,11 +1326,11 @@ void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
       pot.G.ensure(sizeof(double) * 3 * (size_t)n * nl.stride);
       pot.amask.ensure(sizeof(uint64_t) * (size_t)n);
       switch (lanes) {
-        case 4: launch_bond_kernels<4>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, st); break;
-        case 8: launch_bond_kernels<8>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, st); break;
-        case 16: launch_bond_kernels<16>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, st); break;
-        case 32: launch_bond_kernels<32>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, st); break;
-        case 64: launch_bond_kernels<64>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, st); break;
+        case 4: launch_bond_kernels<4>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, blk, st); break;
+        case 8: launch_bond_kernels<8>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, blk, st); break;
+        case 16: launch_bond_kernels<16>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, blk, st); break;
+        case 32: launch_bond_kernels<32>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, blk, st); break;
+        case 64: launch_bond_kernels<64>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, blk, st); break;
         default: throw Error(AK_ERR_STATE, "unsupported neighbour stride");
       }
     }

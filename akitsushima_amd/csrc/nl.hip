@@ -313,17 +313,48 @@ __global__ __launch_bounds__(1024) void k_scan_excl(unsigned* __restrict__ data,
   if (total && tid == 1023) *total = before + inc;
 }
 
+// Stable scatter of one radix pass.  Each block derives its own output
+// offsets from the digit histograms (hist[d * B + b]): thread d sums digit
+// d over all blocks and over the blocks before this one, and an exclusive
+// scan of the digit totals over the 256 threads gives the digit bases -- no
+// separate scan kernel.
 __global__ __launch_bounds__(RS_T) void k_rs_scatter(int n, const unsigned* __restrict__ kin,
                                                      const unsigned* __restrict__ vin,
                                                      unsigned* __restrict__ kout,
                                                      unsigned* __restrict__ vout, int shift,
-                                                     int chunk, const unsigned* __restrict__ off,
+                                                     int chunk, const unsigned* __restrict__ hist,
                                                      int B, const int* gate) {
+  static_assert(RS_T == 256, "one thread per digit");
   if (gated_off(gate)) return;
   __shared__ unsigned run[256];
   __shared__ unsigned wcnt[RS_T / 64][256];
+  __shared__ unsigned wtot[RS_T / 64];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  for (int d = tid; d < 256; d += RS_T) run[d] = 0;
+  {
+    const unsigned* row = hist + (size_t)tid * B;
+    unsigned tot = 0, pre = 0;
+    for (int b0 = 0; b0 < B; b0 += 4) {
+      unsigned h[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) h[q] = b0 + q < B ? row[b0 + q] : 0u;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        tot += h[q];
+        if (b0 + q < (int)blockIdx.x) pre += h[q];
+      }
+    }
+    unsigned inc = tot;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      unsigned y = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += y;
+    }
+    if (lane == 63) wtot[wave] = inc;
+    __syncthreads();
+    unsigned before = 0;
+    for (int w = 0; w < wave; ++w) before += wtot[w];
+    run[tid] = before + inc - tot + pre;  // first output position of digit tid
+  }
   const int base = blockIdx.x * chunk;
   for (int t = 0; t < chunk; t += RS_T) {
     for (int i = tid; i < (RS_T / 64) * 256; i += RS_T) (&wcnt[0][0])[i] = 0;
@@ -337,7 +368,7 @@ __global__ __launch_bounds__(RS_T) void k_rs_scatter(int n, const unsigned* __re
     if (v && lane == __ffsll((unsigned long long)m) - 1) wcnt[wave][d] = __popcll(m);
     __syncthreads();
     if (v) {
-      unsigned p = off[(size_t)d * B + blockIdx.x] + run[d] + rank;
+      unsigned p = run[d] + rank;
       for (int w = 0; w < wave; ++w) p += wcnt[w][d];
       kout[p] = key;
       vout[p] = vin[idx];
@@ -565,32 +596,40 @@ __global__ __launch_bounds__(64) void k_nl_search_reg(
       v[r] = cell_atoms[cell_start[scell[lo]] + (q - spre[lo])];
     }
   }
-  // bitonic sort of the NP elements e = lane + 64 r, ascending
+  // bitonic sort, ascending, of the elements e = lane + 64 r below P, the
+  // power of two covering the candidates (the rest hold the sentinel and
+  // are never paired with an element below P)
+  int P = 1;
+  while (P < ncand) P <<= 1;
 #pragma unroll
   for (int k = 2; k <= NP; k <<= 1) {
+    if (k <= P) {
 #pragma unroll
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      if (j >= 64) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        if (j >= 64) {
 #pragma unroll
-        for (int r = 0; r < EPL; ++r) {
-          const int pr = r ^ (j >> 6);
-          if (pr > r) {
-            const bool up = ((lane + 64 * r) & k) == 0;
-            const unsigned x = v[r], y = v[pr];
-            if ((x > y) == up) {
-              v[r] = y;
-              v[pr] = x;
+          for (int r = 0; r < EPL; ++r) {
+            const int pr = r ^ (j >> 6);
+            if (pr > r && 64 * pr < P) {
+              const bool up = ((lane + 64 * r) & k) == 0;
+              const unsigned x = v[r], y = v[pr];
+              if ((x > y) == up) {
+                v[r] = y;
+                v[pr] = x;
+              }
             }
           }
-        }
-      } else {
-        const bool lower = (lane & j) == 0;
+        } else {
+          const bool lower = (lane & j) == 0;
 #pragma unroll
-        for (int r = 0; r < EPL; ++r) {
-          const bool up = ((lane + 64 * r) & k) == 0;
-          const unsigned x = v[r];
-          const unsigned y = (unsigned)__shfl_xor((int)x, j, 64);
-          v[r] = (lower == up) ? min(x, y) : max(x, y);
+          for (int r = 0; r < EPL; ++r) {
+            if (64 * r < P) {
+              const bool up = ((lane + 64 * r) & k) == 0;
+              const unsigned x = v[r];
+              const unsigned y = (unsigned)__shfl_xor((int)x, j, 64);
+              v[r] = (lower == up) ? min(x, y) : max(x, y);
+            }
+          }
         }
       }
     }
@@ -610,7 +649,7 @@ __global__ __launch_bounds__(64) void k_nl_search_reg(
     int count = 0;
 #pragma unroll
     for (int r = 0; r < EPL; ++r) {
-      if (64 * r >= ncand) break;
+      if (64 * r >= ncand) continue;
       bool hit = false;
       if (lane + 64 * r < ncand) {
         double dx, dy, dz;
@@ -782,7 +821,9 @@ static void sort_cells(NList& nl, SetDev& set, const double* d_pos, hipStream_t 
   int bits = 0;
   while ((1LL << bits) < (long long)C) ++bits;
   int passes = std::max(1, (bits + 7) / 8);
-  int tpb = std::max(1, (n + 256 * 512 - 1) / (256 * 512));
+  // about 64 blocks of whole 256-atom tiles: few enough that every scatter
+  // block can sum the digit histograms itself
+  int tpb = std::max(1, (n + 256 * 64 - 1) / (256 * 64));
   int chunk = 256 * tpb;
   int B = std::max(1, (n + chunk - 1) / chunk);
   nl.hist.ensure(sizeof(unsigned) * 256 * (size_t)B);
@@ -793,8 +834,6 @@ static void sort_cells(NList& nl, SetDev& set, const double* d_pos, hipStream_t 
     hipLaunchKernelGGL(k_rs_hist, dim3(B), dim3(RS_T), 0, st, n, ka, shift, chunk,
                        nl.hist.as<unsigned>(), B, gate, p == 0 ? d_pos : nullptr,
                        (const int*)set.d_sid.as<int>(), (const Geom*)set.d_geoms.as<Geom>(), va);
-    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, nl.hist.as<unsigned>(),
-                       256 * B, (unsigned*)nullptr, gate);
     hipLaunchKernelGGL(k_rs_scatter, dim3(B), dim3(RS_T), 0, st, n, ka, va, kb, vb, shift,
                        chunk, nl.hist.as<unsigned>(), B, gate);
     std::swap(ka, kb);
