@@ -218,6 +218,15 @@ static int make_potential(ak_potential** out, int kind, int ns, double r_cut,
       AK_HIP(hipMemcpy(P->pot.d_map.p, P->pot.h_map.data(), sizeof(int) * P->pot.h_map.size(),
                        hipMemcpyHostToDevice));
     }
+    if (kind == KIND_EAM) {
+      std::vector<double> fast;
+      if (eam_fast_table(ns, P->pot.h_params, P->pot.h_map, fast)) {
+        P->pot.d_eam_fast.ensure(sizeof(double) * fast.size());
+        AK_HIP(hipMemcpy(P->pot.d_eam_fast.p, fast.data(), sizeof(double) * fast.size(),
+                         hipMemcpyHostToDevice));
+        P->pot.eam_fast = true;
+      }
+    }
     *out = P.release();
   });
 }

@@ -137,9 +137,12 @@ struct Potential {
   double r_cut = 0;
   std::vector<double> h_params;
   DevBuf d_params;
-  // EAM function-table indices
+  // EAM function-table indices; eam_fast: the table is the Zhou alloy form
+  // (per-species density and pair rows, cross rows of two species), so the
+  // force pass may evaluate each pair's functions with shared exponentials
   std::vector<int> h_map;
-  DevBuf d_map;
+  DevBuf d_map, d_eam_fast;
+  bool eam_fast = false;
   // per-call scratch; amask (n) = bonds inside r_c per atom (rows >= 8)
   DevBuf G, aux, out, amask;
   // launch shape chosen by the autotuner (0 = default)
@@ -178,6 +181,11 @@ struct Potential {
     }
   }
 };
+
+// Per-species / per-pair table of the shared-exponential EAM path, or false
+// when the function table is not of the Zhou alloy form (forces.hip).
+bool eam_fast_table(int ns, const std::vector<double>& rows, const std::vector<int>& map,
+                    std::vector<double>& out);
 
 // Forces on all atoms of `set`, per-structure energy and stress.
 // d_out: (n_struct, 10) = [E, S00..S22]; d_F (n_atoms, 3).

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 namespace ak {
 
@@ -952,16 +953,31 @@ __global__ __launch_bounds__(256) void k_gather_c(int n, const int* __restrict__
     const int c = cnt[a];
     const uint64_t ma = amask[a];
     const size_t plane = (size_t)n * stride;
-    for (int k = q; k < c; k += LA) {
-      const size_t s = (size_t)a * stride + k;
-      const int r = rev[s];
-      if (r < 0) continue;
-      if ((ma >> k) & 1) {
+    // all rounds unrolled so that the reverse-slot, mask and G loads of the
+    // rounds are in flight together
+    constexpr int R = L / LA;
+    int rv[R];
+    bool own[R], oth[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const int k = q + LA * i;
+      rv[i] = k < c ? rev[(size_t)a * stride + k] : -1;
+      own[i] = k < c && ((ma >> k) & 1);
+    }
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+      oth[i] = rv[i] >= 0 && ((amask[rv[i] / L] >> (rv[i] & (L - 1))) & 1);
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      if (rv[i] < 0) continue;
+      const size_t s = (size_t)a * stride + q + LA * i;
+      if (own[i]) {
         fx += G[s];
         fy += G[plane + s];
         fz += G[2 * plane + s];
       }
-      if ((amask[r / L] >> (r & (L - 1))) & 1) {
+      if (oth[i]) {
+        const size_t r = (size_t)rv[i];
         fx -= G[r];
         fy -= G[plane + r];
         fz -= G[2 * plane + r];
@@ -1062,6 +1078,94 @@ __device__ __forceinline__ void z_feval(const double* __restrict__ f, double r, 
   dv = dval * S + val * dS;
 }
 
+// Shared-exponential evaluation of the Zhou functions of one pair.  For
+// species s at distance r (x = r / r_e):
+//   density  f   = f_e e^{-beta (x-1)} / (1 + (x - lambda)^20)
+//   pair     phi = A e^{-alpha (x-1)} / (1 + (x - kappa)^20)
+//                - B e^{-beta (x-1)} / (1 + (x - lambda)^20)
+// share e^{-beta (x-1)} and the lambda denominator: two exponentials per
+// species instead of one per function term (a cross pair needs both species'
+// densities and pair terms: 4 instead of 8).  Every value is formed with the
+// operations, in the order, of z_decay / z_feval.
+// Fast table: per species (ZF_SP doubles) r_e, f_e, beta, lambda, A, alpha,
+// kappa, B, density-row r_c and width; per ordered pair (sa, sj) (ZF_PR
+// doubles) code (4 pair, 5 cross), r_c, width, chi, swap (the cross row's
+// first species is sj).
+constexpr int ZF_SP = 10, ZF_PR = 5;
+
+struct ZBase {
+  double fv, fd, pv, pd;  // unswitched density and pair value, derivative
+};
+
+__device__ __forceinline__ ZBase z_base(const double* __restrict__ sp, double r) {
+  const double re = sp[0], fe = sp[1], beta = sp[2], lam = sp[3], A = sp[4], alpha = sp[5],
+               kap = sp[6], B = sp[7];
+  const double x = r / re;
+  const double Eb = exp(-beta * (x - 1.0));
+  const double Ea = exp(-alpha * (x - 1.0));
+  double ul = x - lam, uk = x - kap;
+  double ul2 = ul * ul, ul4 = ul2 * ul2, ul8 = ul4 * ul4, ul16 = ul8 * ul8;
+  double ul19 = ul16 * ul2 * ul;
+  double idl = 1.0 / (1.0 + ul19 * ul);
+  double uk2 = uk * uk, uk4 = uk2 * uk2, uk8 = uk4 * uk4, uk16 = uk8 * uk8;
+  double uk19 = uk16 * uk2 * uk;
+  double idk = 1.0 / (1.0 + uk19 * uk);
+  ZBase b;
+  const double ef = fe * Eb;
+  b.fv = ef * idl;
+  b.fd = (-beta * b.fv - ef * 20.0 * ul19 * idl * idl) / re;
+  const double eA = A * Ea;
+  const double vA = eA * idk;
+  const double dA = (-alpha * vA - eA * 20.0 * uk19 * idk * idk) / re;
+  const double eB = B * Eb;
+  const double vB = eB * idl;
+  const double dB = (-beta * vB - eB * 20.0 * ul19 * idl * idl) / re;
+  b.pv = vA - vB;
+  b.pd = dA - dB;
+  return b;
+}
+
+__device__ __forceinline__ void z_switch(double r, double rc, double w, double val, double dval,
+                                         double& v, double& dv) {
+  double s = (r - (rc - w)) / w;
+  s = s < 0.0 ? 0.0 : (s > 1.0 ? 1.0 : s);
+  double S = 1.0 - s * s * s * (10.0 - 15.0 * s + 6.0 * s * s);
+  double dS = -30.0 * s * s * (1.0 - s) * (1.0 - s) / w;
+  v = val * S;
+  dv = dval * S + val * dS;
+}
+
+// d f_sj / dr, d f_sa / dr and phi_{sa sj}, phi' at r.
+__device__ __forceinline__ void z_pair_fast(const double* __restrict__ fast, int ns, int sa,
+                                            int sj, double r, double& dfj, double& dfa,
+                                            double& phi, double& dphi) {
+  const double* spj = fast + (size_t)ZF_SP * sj;
+  const double* spa = fast + (size_t)ZF_SP * sa;
+  const double* pr = fast + (size_t)ZF_SP * ns + (size_t)ZF_PR * (sa * ns + sj);
+  const ZBase bj = z_base(spj, r);
+  const ZBase ba = sa == sj ? bj : z_base(spa, r);
+  double v;
+  z_switch(r, spj[8], spj[9], bj.fv, bj.fd, v, dfj);
+  if (sa == sj)
+    dfa = dfj;
+  else
+    z_switch(r, spa[8], spa[9], ba.fv, ba.fd, v, dfa);
+  double val, dval;
+  if (pr[0] == 4.0) {
+    val = bj.pv;
+    dval = bj.pd;
+  } else {
+    const ZBase& X = pr[4] != 0.0 ? bj : ba;  // species of the row's first block
+    const ZBase& Y = pr[4] != 0.0 ? ba : bj;
+    const double kc = pr[3];
+    double q = Y.fv / X.fv;
+    double dq = (Y.fd * X.fv - Y.fv * X.fd) / (X.fv * X.fv);
+    val = 0.5 * kc * (q * X.pv + Y.pv / q);
+    dval = 0.5 * kc * (dq * X.pv + q * X.pd + Y.pd / q - Y.pv * dq / (q * q));
+  }
+  z_switch(r, pr[1], pr[2], val, dval, phi, dphi);
+}
+
 // map: [rho_f[ns], emb_f[ns], phi_f[ns*ns]]
 template <int LE>
 __global__ __launch_bounds__(256) void k_eam_rho(
@@ -1108,7 +1212,8 @@ __device__ __forceinline__ void k_eam_force_chunk(int c, int n_chunks,
     const double* __restrict__ fn, const int* __restrict__ map, double rc,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const double* __restrict__ dF, const double* __restrict__ Fe,
-    const int* __restrict__ chunk_rng, double* __restrict__ F, double* __restrict__ partial) {
+    const int* __restrict__ chunk_rng, double* __restrict__ F, double* __restrict__ partial,
+    const double* __restrict__ fast) {
   const int a = chunk_rng[2 * c] + (int)threadIdx.x / LE;
   const int l = threadIdx.x & (LE - 1);
   const bool live = a < chunk_rng[2 * c + 1];
@@ -1129,12 +1234,16 @@ __device__ __forceinline__ void k_eam_force_chunk(int c, int n_chunks,
       if (r < rc && r > 1e-12) {
         const int sj = ns > 1 ? spc[j] : 0;
         double v, dfj, dfa, phi, dphi;
-        z_feval(fn + (size_t)ZW * map[sj], r, v, dfj);
-        if (sj == sa)
-          dfa = dfj;
-        else
-          z_feval(fa_rho, r, v, dfa);
-        z_feval(fn + (size_t)ZW * map[2 * ns + sa * ns + sj], r, phi, dphi);
+        if (fast) {
+          z_pair_fast(fast, ns, sa, sj, r, dfj, dfa, phi, dphi);
+        } else {
+          z_feval(fn + (size_t)ZW * map[sj], r, v, dfj);
+          if (sj == sa)
+            dfa = dfj;
+          else
+            z_feval(fa_rho, r, v, dfa);
+          z_feval(fn + (size_t)ZW * map[2 * ns + sa * ns + sj], r, phi, dphi);
+        }
         double c = dphi + dFa * dfj + dF[j] * dfa;
         double s = c / r;
         fx += s * ux;
@@ -1169,9 +1278,10 @@ __global__ __launch_bounds__(256) void k_eam_force(int n_chunks, const double* _
     const double* __restrict__ fn, const int* __restrict__ map, double rc,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const double* __restrict__ dF, const double* __restrict__ Fe,
-    const int* __restrict__ chunk_rng, double* __restrict__ F, double* __restrict__ partial) {
+    const int* __restrict__ chunk_rng, double* __restrict__ F, double* __restrict__ partial,
+    const double* __restrict__ fast) {
   for (int c = blockIdx.x; c < n_chunks; c += gridDim.x)
-    k_eam_force_chunk<LE>(c, n_chunks, pos, sid, geoms, single, spc, ns, fn, map, rc, cnt, nbr, stride, dF, Fe, chunk_rng, F, partial);
+    k_eam_force_chunk<LE>(c, n_chunks, pos, sid, geoms, single, spc, ns, fn, map, rc, cnt, nbr, stride, dF, Fe, chunk_rng, F, partial, fast);
 }
 
 // ---------------------------------------------------------------------------
@@ -1283,7 +1393,58 @@ static void launch_eam(Potential& pot, NList& nl, SetDev& set, const double* d_p
                pot.ns, (const double*)pot.d_params.as<double>(),
                (const int*)pot.d_map.as<int>(), pot.r_cut, (const int*)nl.cnt.as<int>(),
                (const int*)nl.nbr.as<int>(), nl.stride, (const double*)dF, (const double*)Fe,
-               (const int*)ch.rng.as<int>(), d_F, ch.partial.as<double>());
+               (const int*)ch.rng.as<int>(), d_F, ch.partial.as<double>(),
+               (const double*)(pot.eam_fast ? pot.d_eam_fast.as<double>() : nullptr));
+}
+
+bool eam_fast_table(int ns, const std::vector<double>& rows, const std::vector<int>& map,
+                    std::vector<double>& out) {
+  auto row = [&](int k) { return rows.data() + (size_t)ZW * k; };
+  auto same = [](const double* x, const double* y) {
+    return std::memcmp(x, y, 20 * sizeof(double)) == 0;
+  };
+  out.assign((size_t)ZF_SP * ns + (size_t)ZF_PR * ns * ns, 0.0);
+  for (int s = 0; s < ns; ++s) {
+    const double* d = row(map[s]);
+    if (d[0] != 3.0) return false;
+    const double* pa = d + 4;
+    double* o = out.data() + (size_t)ZF_SP * s;
+    o[0] = pa[0];  // r_e
+    o[1] = pa[1];  // f_e
+    o[2] = pa[5];  // beta
+    o[3] = pa[9];  // lambda
+    o[4] = pa[6];  // A
+    o[5] = pa[4];  // alpha
+    o[6] = pa[8];  // kappa
+    o[7] = pa[7];  // B
+    o[8] = d[1];
+    o[9] = d[2];
+  }
+  for (int a = 0; a < ns; ++a)
+    for (int b = 0; b < ns; ++b) {
+      const double* f = row(map[2 * ns + a * ns + b]);
+      const double* ea = row(map[a]) + 4;
+      const double* eb = row(map[b]) + 4;
+      double* o = out.data() + (size_t)ZF_SP * ns + (size_t)ZF_PR * (a * ns + b);
+      o[0] = f[0];
+      o[1] = f[1];
+      o[2] = f[2];
+      o[3] = f[3];
+      if (f[0] == 4.0) {
+        if (a != b || !same(f + 4, ea)) return false;
+      } else if (f[0] == 5.0) {
+        if (a == b) return false;
+        if (same(f + 4, ea) && same(f + 24, eb))
+          o[4] = 0.0;
+        else if (same(f + 4, eb) && same(f + 24, ea))
+          o[4] = 1.0;
+        else
+          return false;
+      } else {
+        return false;
+      }
+    }
+  return true;
 }
 
 void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,

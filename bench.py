@@ -234,8 +234,9 @@ def eam_cuni(steps, warmup):
     lat, pos, _ = cubic(FCC, 3.56, 40, ["X"] * 4, rattle=0.03, seed=1)
     types = random_species(np.random.default_rng(0), ["Cu", "Ni"], len(pos))
     pot = zhou_eam({"Cu": dict(ZHOU_2004["Cu"]), "Ni": dict(ZHOU_2004["Ni"])}, ["Cu", "Ni"])
+    tune = pot.autotune([(lat, pos, types)], reps=3)
     out = {"workload": f"Zhou EAM Cu-Ni fcc 40^3 ({len(pos)} atoms), a 3.56 A, rattle 0.03 A",
-           "atoms": len(pos), "r_cut": pot.r_cut}
+           "atoms": len(pos), "r_cut": pot.r_cut, "autotuned": tune["best"]}
     out.update(device_force_rate(pot, lat, pos, types, steps, warmup))
     return out
 
@@ -348,7 +349,13 @@ def main():
     lat, pos, types = cubic(DIAMOND, 5.431, args.cells, ["Si"] * 8, rattle=0.05, seed=0)
     n = len(pos)
     pot = TersoffPotential(params, species=["Si"])
-    pot.set_bond_tile(args.bond_tile)
+    # launch shape: the autotuner's pick on this structure, unless a site
+    # kernel form is forced
+    tuned = None
+    if args.bond_tile:
+        pot.set_bond_tile(args.bond_tile)
+    else:
+        tuned = pot.autotune([(lat, pos, types)], reps=5)["best"]
     nl = NeighbourList(pot.r_cut, skin_width=0.3)
     nl.build(lat, pos)
     t0 = time.perf_counter()
@@ -437,7 +444,8 @@ def main():
     stride = 4
     while stride < int(np.bincount(np.concatenate([ci, cj]), minlength=n).max()):
         stride *= 2
-    site_kernel = (f"k_tersoff_trip<{stride}>" if args.bond_tile == 2 and stride <= 8
+    tile = args.bond_tile or (tuned or {}).get("bond_tile", 1)
+    site_kernel = (f"k_tersoff_trip<{stride}>" if tile == 2 and stride <= 8
                    else f"k_tersoff_site<{stride}>")
     n_pairs, n_trip = geometry_counts(lat, pos, pot.r_cut)
     flops = tersoff_flops(n_pairs, n_trip, params["lambda3"])
@@ -446,7 +454,8 @@ def main():
     roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                 "traffic": read_traffic(),
-                "kernel": site_kernel, "kernel_avg_us": avg_site_s * 1e6,
+                "kernel": site_kernel, "launch_shape": tuned,
+                "kernel_avg_us": avg_site_s * 1e6,
                 "kernel_min_us": min_ms.value * 1e3, "launches_timed": int(cnt.value),
                 "timing": "HIP start/stop events of each site-kernel dispatch "
                           "(hipExtLaunchKernelGGL) over a second pass of the K timed steps",
