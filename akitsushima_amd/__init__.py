@@ -3,8 +3,9 @@
 Hand-written HIP kernels for gfx950 behind the reference's force-function
 API: Tersoff, Stillinger-Weber and Zhou EAM energies, forces and virial
 stress, a device cell-list neighbour list, batched evaluation of many
-structures, a device-resident NVE integrator and a Tersoff parameter
-gradient.  The compute path is the HIP library ``libakitsushima_amd.so``
+structures, a device-resident NVE integrator, a Tersoff parameter
+gradient, and the batched callers of the force call: force constants and
+Gamma-point phonons, and elastic constants.  The compute path is the HIP library ``libakitsushima_amd.so``
 only; importing works without a GPU, computing does not.
 """
 from .neighborlist import NeighbourList
@@ -17,6 +18,8 @@ from .potentials import (
     force_fn_from_config,
     make_force_fn,
 )
+from .elastic import elastic_constants
+from .phonon import force_constants, gamma_frequencies
 from .zhou import zhou_eam
 
 __version__ = "0.1.0"
@@ -31,4 +34,7 @@ __all__ = [
     "call_force_fn",
     "force_fn_from_config",
     "zhou_eam",
+    "elastic_constants",
+    "force_constants",
+    "gamma_frequencies",
 ]
