@@ -68,6 +68,10 @@ SIGNATURES = {
     "ak_tersoff_loss_grad": (ctypes.c_int, [_handle, _i32, _pi64, _pf64, _pf64, _pf64, _pf64,
                                             _pf64, _pf64, _pf64, _pf64, _pf64, _pf64, _pf64,
                                             _pf64]),
+    "ak_tersoff_fit_load": (ctypes.c_int, [_handle, _i32, _pi64, _pf64, _pf64, _pf64, _pf64,
+                                           _pf64]),
+    "ak_tersoff_fit_eval": (ctypes.c_int, [_handle, _pf64, _pf64, _pf64, _pf64, _pf64, _pf64,
+                                           _pf64]),
 }
 
 _lib = None

@@ -620,6 +620,12 @@ void tersoff_loss_grad_run(Potential& pot, int S, const int64_t* off, const doub
                            const double* ref_stress, const double* weights, double* loss,
                            double* grad, double* energies, double* forces, double* stresses,
                            hipStream_t st);
+void tersoff_fit_set(Potential& pot, int S, const int64_t* off, const double* lattices,
+                     const double* positions, const double* ref_energy,
+                     const double* ref_forces, const double* ref_stress, hipStream_t st);
+void tersoff_fit_eval(Potential& pot, const double* theta, const double* weights, double* loss,
+                      double* grad, double* energies, double* forces, double* stresses,
+                      hipStream_t st);
 }  // namespace ak
 
 extern "C" {
@@ -664,6 +670,34 @@ int ak_tersoff_loss_grad(ak_potential* P, int32_t S, const int64_t* off,
     tersoff_loss_grad_run(P->pot, S, off, lattices, positions, theta, ref_energy, ref_forces,
                           ref_stress, weights, loss, grad, energies, forces, stresses,
                           P->stream);
+  });
+}
+
+int ak_tersoff_fit_load(ak_potential* P, int32_t S, const int64_t* off,
+                        const double* lattices, const double* positions,
+                        const double* ref_energy, const double* ref_forces,
+                        const double* ref_stress) {
+  return guarded([&] {
+    AK_REQUIRE(P && S >= 1 && off && lattices && positions && ref_energy && ref_forces &&
+                   ref_stress,
+               AK_ERR_ARG, "bad arguments");
+    AK_REQUIRE(P->pot.kind == KIND_TERSOFF && P->pot.ns == 1, AK_ERR_ARG,
+               "the parameter gradient is defined for a single-species Tersoff potential");
+    check_finite(lattices, 9 * (size_t)S, "lattices");
+    check_finite(positions, 3 * (size_t)off[S], "positions");
+    tersoff_fit_set(P->pot, S, off, lattices, positions, ref_energy, ref_forces, ref_stress,
+                    P->stream);
+  });
+}
+
+int ak_tersoff_fit_eval(ak_potential* P, const double* theta, const double* weights,
+                        double* loss, double* grad, double* energies, double* forces,
+                        double* stresses) {
+  return guarded([&] {
+    AK_REQUIRE(P && theta && weights && loss, AK_ERR_ARG, "bad arguments");
+    AK_REQUIRE(P->pot.kind == KIND_TERSOFF && P->pot.ns == 1, AK_ERR_ARG,
+               "the parameter gradient is defined for a single-species Tersoff potential");
+    tersoff_fit_eval(P->pot, theta, weights, loss, grad, energies, forces, stresses, P->stream);
   });
 }
 

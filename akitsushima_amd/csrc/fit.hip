@@ -11,10 +11,10 @@
 // which is the derivative of the site energies when every bond vector moves
 // along w_ij.  Hence
 //   dL/dtheta_p = sum_s kappa_s dE_s/dtheta_p + sum_i d^2 E_i / (d eps d theta_p),
-// and both terms come from one evaluation of each site energy in hyper-dual
-// numbers (value, d/d eps, d/d theta_p, d^2/d eps d theta_p), one parameter
-// per grid row: the mixed second derivative needs no stored bond-order
-// intermediates, the bond orders are recomputed per structure and parameter.
+// and both terms come from one pass over the bonds that carries the 14
+// analytic parameter derivatives of each site energy as dual numbers along
+// eps (value, d/d eps): the mixed second derivative needs no stored
+// bond-order intermediates, the bond orders are recomputed per structure.
 //
 // theta = A, B, lambda1, lambda2, lambda3, beta, n, c, d, h, R, D, m, gamma
 // (the single-species record of engine/tersoff.py:528-855 with explicit m and
@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <utility>
 #include <vector>
 
 #include "engine.hpp"
@@ -31,94 +32,92 @@
 namespace ak {
 
 // ---------------------------------------------------------------------------
-// Hyper-dual numbers: x = v + e eps + t tau + et eps tau
+// Dual numbers over eps: x = v + e eps
 // ---------------------------------------------------------------------------
-struct HD {
-  double v, e, t, et;
+struct Du {
+  double v, e;
 };
 
-__device__ __forceinline__ HD hd(double v) { return {v, 0.0, 0.0, 0.0}; }
-__device__ __forceinline__ HD operator+(HD a, HD b) {
-  return {a.v + b.v, a.e + b.e, a.t + b.t, a.et + b.et};
+__device__ __forceinline__ Du du(double v) { return {v, 0.0}; }
+__device__ __forceinline__ Du operator+(Du a, Du b) { return {a.v + b.v, a.e + b.e}; }
+__device__ __forceinline__ Du operator-(Du a, Du b) { return {a.v - b.v, a.e - b.e}; }
+__device__ __forceinline__ Du operator-(Du a) { return {-a.v, -a.e}; }
+__device__ __forceinline__ Du operator+(Du a, double b) { return {a.v + b, a.e}; }
+__device__ __forceinline__ Du operator+(double b, Du a) { return {a.v + b, a.e}; }
+__device__ __forceinline__ Du operator-(double b, Du a) { return {b - a.v, -a.e}; }
+__device__ __forceinline__ Du operator*(Du a, Du b) { return {a.v * b.v, a.e * b.v + a.v * b.e}; }
+__device__ __forceinline__ Du operator*(double s, Du a) { return {s * a.v, s * a.e}; }
+__device__ __forceinline__ Du operator*(Du a, double s) { return {s * a.v, s * a.e}; }
+__device__ __forceinline__ Du& operator+=(Du& a, Du b) {
+  a.v += b.v;
+  a.e += b.e;
+  return a;
 }
-__device__ __forceinline__ HD operator-(HD a, HD b) {
-  return {a.v - b.v, a.e - b.e, a.t - b.t, a.et - b.et};
+__device__ __forceinline__ Du dinv(Du a) {
+  const double i = 1.0 / a.v;
+  return {i, -a.e * i * i};
 }
-__device__ __forceinline__ HD operator-(HD a) { return {-a.v, -a.e, -a.t, -a.et}; }
-__device__ __forceinline__ HD operator+(HD a, double b) { return {a.v + b, a.e, a.t, a.et}; }
-__device__ __forceinline__ HD operator+(double b, HD a) { return a + b; }
-__device__ __forceinline__ HD operator-(double b, HD a) { return {b - a.v, -a.e, -a.t, -a.et}; }
-__device__ __forceinline__ HD operator*(HD a, HD b) {
-  return {a.v * b.v, a.e * b.v + a.v * b.e, a.t * b.v + a.v * b.t,
-          a.et * b.v + a.e * b.t + a.t * b.e + a.v * b.et};
+__device__ __forceinline__ Du operator/(Du a, Du b) { return a * dinv(b); }
+__device__ __forceinline__ Du dexp(Du a) {
+  const double x = exp(a.v);
+  return {x, x * a.e};
 }
-__device__ __forceinline__ HD operator*(double s, HD a) {
-  return {s * a.v, s * a.e, s * a.t, s * a.et};
-}
-__device__ __forceinline__ HD operator*(HD a, double s) { return s * a; }
-// f(a) from f, f', f''
-__device__ __forceinline__ HD chain(HD a, double f0, double f1, double f2) {
-  return {f0, f1 * a.e, f1 * a.t, f1 * a.et + f2 * a.e * a.t};
-}
-__device__ __forceinline__ HD hinv(HD a) {
-  double i = 1.0 / a.v;
-  return chain(a, i, -i * i, 2.0 * i * i * i);
-}
-__device__ __forceinline__ HD operator/(HD a, HD b) { return a * hinv(b); }
-__device__ __forceinline__ HD hexp(HD a) {
-  double x = exp(a.v);
-  return chain(a, x, x, x);
-}
-__device__ __forceinline__ HD hlog(HD a) {
-  double i = 1.0 / a.v;
-  return chain(a, log(a.v), i, -i * i);
-}
-__device__ __forceinline__ HD hsqrt(HD a) {
-  double s = sqrt(a.v);
-  return chain(a, s, 0.5 / s, -0.25 / (s * a.v));
-}
-__device__ __forceinline__ HD hsin(HD a) {
-  double s, c;
-  sincos(a.v, &s, &c);
-  return chain(a, s, c, -s);
-}
-// sign(x) |x|^m for m > 0
-__device__ __forceinline__ HD oddpow(HD x, HD m) {
-  if (x.v == 0.0) return m.v == 1.0 ? x : hd(0.0);
-  double sg = x.v > 0.0 ? 1.0 : -1.0;
-  HD ax = sg * x;
-  return sg * hexp(m * hlog(ax));
+__device__ __forceinline__ Du dlog(Du a) { return {log(a.v), a.e / a.v}; }
+__device__ __forceinline__ Du dlog1p(Du a) { return {log1p(a.v), a.e / (1.0 + a.v)}; }
+__device__ __forceinline__ Du dsqrt(Du a) {
+  const double s = sqrt(a.v);
+  return {s, 0.5 * a.e / s};
 }
 
-__device__ __forceinline__ HD fc_hd(HD r, HD R, HD D) {
-  if (r.v < R.v - D.v) return hd(1.0);
-  if (r.v > R.v + D.v) return hd(0.0);
-  return 0.5 - 0.5 * hsin((0.5 * M_PI) * (r - R) / D);
+// Tersoff cutoff of a bond length r (dual) and its derivatives in R and D.
+struct Fc {
+  Du f, fR, fD;
+};
+__device__ __forceinline__ Fc fc_du(Du r, double R, double D) {
+  Fc o;
+  if (r.v < R - D) {
+    o.f = du(1.0);
+    o.fR = o.fD = du(0.0);
+  } else if (r.v > R + D) {
+    o.f = o.fR = o.fD = du(0.0);
+  } else {
+    const Du phi = (0.5 * M_PI / D) * (r - du(R));
+    double sn, cs;
+    sincos(phi.v, &sn, &cs);
+    const Du s_ = {sn, cs * phi.e}, c_ = {cs, -sn * phi.e};
+    o.f = 0.5 - 0.5 * s_;
+    o.fR = (0.25 * M_PI / D) * c_;       // -1/2 cos(phi) dphi/dR
+    o.fD = (0.5 / D) * (c_ * phi);       // -1/2 cos(phi) dphi/dD
+  }
+  return o;
 }
 
-struct Theta {
-  HD A, B, l1, l2, l3, beta, n, c, d, h, R, D, m, gam;
+// Parameter indices in theta.
+enum {
+  TH_A, TH_B, TH_L1, TH_L2, TH_L3, TH_BETA, TH_N, TH_C, TH_D, TH_H, TH_R, TH_DD, TH_M, TH_GAM
 };
 
-__device__ __forceinline__ Theta load_theta(const double* __restrict__ th, int p) {
-  HD x[14];
-#pragma unroll
-  for (int q = 0; q < 14; ++q) x[q] = {th[q], 0.0, q == p ? 1.0 : 0.0, 0.0};
-  return {x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], x[8], x[9], x[10], x[11], x[12],
-          x[13]};
-}
-
-// One parameter per grid row (blockIdx.y); L lanes per atom as the forward
-// site kernel.  Writes dE[p*n + a] = dE_a/dtheta_p and dD[p*n + a] =
-// d^2 E_a / (d eps d theta_p).
+// All 14 parameter derivatives of every site energy in one pass.  Each lane
+// owns one bond (a, j) of the site energy
+//   V_aj = 1/2 fc(r_aj) [fR(r_aj) + b_aj fA(r_aj)],
+//   b = (1 + (beta zeta)^n)^(-1/2n),
+//   zeta_aj = sum_k gamma fc(r_ak) g(cos theta_jk) exp(sign(x)|x|^m),
+//   x = lambda3 (r_aj - r_ak),
+// and accumulates, over its partner legs k, zeta and its derivatives in R,
+// D, c, d, h, lambda3, m and gamma; the chain through b and the
+// two-body terms gives dV_aj/dtheta_p for all p.  Every quantity is a dual
+// number along eps (each bond vector moving along w_aj), so the e parts are
+// the mixed second derivatives d^2 E_a / (d eps d theta_p).  The atom's L
+// lanes are summed: dE[p n + a] = dE_a/dtheta_p, dD[p n + a] =
+// d^2 E_a/(d eps d theta_p).  The bond orders are recomputed per structure
+// (no stored intermediates).
 template <int L>
-__global__ __launch_bounds__(256) void k_tersoff_hd(
+__global__ __launch_bounds__(256) void k_tersoff_grad(
     int n, const double* __restrict__ pos, const int* __restrict__ sid,
     const Geom* __restrict__ geoms, const double* __restrict__ theta,
     const double* __restrict__ lam, const double* __restrict__ muv,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     double* __restrict__ dE, double* __restrict__ dD) {
-  const int p = blockIdx.y;
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
   const int a = gt / L;
   const int k = gt & (L - 1);
@@ -128,8 +127,11 @@ __global__ __launch_bounds__(256) void k_tersoff_hd(
   const int nk = live ? cnt[ac] : 0;
   const bool has = live && k < nk;
   const int j = has ? nbr[(size_t)ac * stride + k] : ac;
-  const Theta P = load_theta(theta, p);
-  const double rc = theta[10] + theta[11];
+  const double A = theta[TH_A], B = theta[TH_B], l1 = theta[TH_L1], l2 = theta[TH_L2],
+               l3 = theta[TH_L3], beta = theta[TH_BETA], nn = theta[TH_N], c = theta[TH_C],
+               d = theta[TH_D], h = theta[TH_H], R = theta[TH_R], D = theta[TH_DD],
+               m = theta[TH_M], gam = theta[TH_GAM];
+  const double rc = R + D;
 
   double ux = 0, uy = 0, uz = 0, wx = 0, wy = 0, wz = 0;
   bool act = false;
@@ -158,48 +160,122 @@ __global__ __launch_bounds__(256) void k_tersoff_hd(
       ux = uy = uz = 0.0;
     }
   }
-  // |u + eps w|
-  const double uu = ux * ux + uy * uy + uz * uz;
-  // eps^2 terms drop out: only eps, tau and eps*tau orders are carried
-  HD r = act ? hsqrt(HD{uu, 2.0 * (ux * wx + uy * wy + uz * wz), 0.0, 0.0}) : hd(0.0);
-  HD fcl = act ? fc_hd(r, P.R, P.D) : hd(0.0);
+  // |u + eps w| to first order in eps
+  const Du r = act ? dsqrt(Du{ux * ux + uy * uy + uz * uz, 2.0 * (ux * wx + uy * wy + uz * wz)})
+                   : du(0.0);
+  const Fc fr = act ? fc_du(r, R, D) : Fc{du(0.0), du(0.0), du(0.0)};
 
-  HD zeta = hd(0.0);
+  const double c2 = c * c, d2 = d * d, c2d2 = c2 / d2;
+  Du z = du(0.0), zR = du(0.0), zD = du(0.0), zc = du(0.0), zd = du(0.0), zh = du(0.0),
+     zl3 = du(0.0), zm = du(0.0), zg = du(0.0);
   for (int s = 0; s < L; ++s) {
     const int src = seg + s;
-    double vx = __shfl(ux, src, 64), vy = __shfl(uy, src, 64), vz = __shfl(uz, src, 64);
-    double qx = __shfl(wx, src, 64), qy = __shfl(wy, src, 64), qz = __shfl(wz, src, 64);
-    int pact = __shfl((int)act, src, 64);
+    const double vx = __shfl(ux, src, 64), vy = __shfl(uy, src, 64), vz = __shfl(uz, src, 64);
+    const double qx = __shfl(wx, src, 64), qy = __shfl(wy, src, 64), qz = __shfl(wz, src, 64);
+    const int pact = __shfl((int)act, src, 64);
     if (s == k || !act || !pact) continue;
-    double vv = vx * vx + vy * vy + vz * vz;
-    HD rs = hsqrt(HD{vv, 2.0 * (vx * qx + vy * qy + vz * qz), 0.0, 0.0});
-    HD fcs = fc_hd(rs, P.R, P.D);
-    if (fcs.v == 0.0) continue;
-    HD dot = {ux * vx + uy * vy + uz * vz, wx * vx + wy * vy + wz * vz + ux * qx + uy * qy +
-                                               uz * qz,
-              0.0, 0.0};
-    HD cs = dot / (r * rs);
-    HD hc = P.h - cs;
-    HD c2 = P.c * P.c, d2 = P.d * P.d;
-    HD g = 1.0 + c2 / d2 - c2 / (d2 + hc * hc);
-    HD ex = hexp(oddpow(P.l3 * (r - rs), P.m));
-    zeta = zeta + P.gam * fcs * g * ex;
-  }
-  HD Ek = hd(0.0);
-  if (act) {
-    HD b = hd(1.0);
-    if (zeta.v > 0.0 && P.beta.v != 0.0) {
-      HD bz = hexp(P.n * hlog(P.beta * zeta));
-      b = hexp(-(hlog(1.0 + bz) / (2.0 * P.n)));
+    const Du rs = dsqrt(Du{vx * vx + vy * vy + vz * vz, 2.0 * (vx * qx + vy * qy + vz * qz)});
+    const Fc fk = fc_du(rs, R, D);
+    if (fk.f.v == 0.0) continue;
+    const Du dot = {ux * vx + uy * vy + uz * vz,
+                    wx * vx + wy * vy + wz * vz + ux * qx + uy * qy + uz * qz};
+    const Du cs = dot / (r * rs);
+    const Du q = h - cs;
+    // g = 1 + c^2/d^2 - c^2/(d^2 + q^2), q = h - cos, and dg/dc, dg/dd, dg/dh
+    const Du iden = dinv((q * q) + d2);
+    const Du iden2 = iden * iden;
+    const Du g = (1.0 + c2d2) - c2 * iden;
+    const Du gc = (2.0 * c / d2) - (2.0 * c) * iden;
+    const Du gd = (2.0 * c2 * d) * iden2 + (-2.0 * c2 / (d2 * d));
+    const Du gh = (2.0 * c2) * (q * iden2);
+    // exponential factor E = exp(sign(x) |x|^m) and dE/dlambda3, dE/dm
+    const Du dr = r - rs;
+    Du E = du(1.0), El3 = du(0.0), Em = du(0.0);
+    if (l3 != 0.0) {
+      const Du x = l3 * dr;
+      if (x.v != 0.0) {
+        const double sg = x.v > 0.0 ? 1.0 : -1.0;
+        const Du ax = sg * x;
+        Du pw, dpw;  // sign(x)|x|^m and d/dx of it
+        if (m == 3.0) {
+          pw = x * x * x;
+          dpw = 3.0 * (x * x);
+        } else if (m == 1.0) {
+          pw = x;
+          dpw = du(1.0);
+        } else {
+          const Du lx = dlog(ax);
+          pw = sg * dexp(m * lx);
+          dpw = m * dexp((m - 1.0) * lx);
+        }
+        E = dexp(pw);
+        El3 = E * (dpw * dr);
+        Em = E * (pw * dlog(ax));
+      } else if (m == 1.0) {
+        E = dexp(x);
+        El3 = E * dr;
+      }
+    } else if (m == 1.0) {
+      El3 = dr;  // d/dlambda3 exp(lambda3 dr) at lambda3 = 0
     }
-    HD fr = P.A * hexp(-(P.l1 * r));
-    HD fa = -(P.B * hexp(-(P.l2 * r)));
-    Ek = 0.5 * fcl * (fr + b * fa);
+    const Du fgn = fk.f * g;
+    const Du fg = gam * fgn;
+    zg += fgn * E;
+    z += fg * E;
+    zR += gam * (fk.fR * g) * E;
+    zD += gam * (fk.fD * g) * E;
+    const Du fe = gam * (fk.f * E);
+    zc += fe * gc;
+    zd += fe * gd;
+    zh += fe * gh;
+    zl3 += fg * El3;
+    zm += fg * Em;
   }
-  double de = seg_sum<L>(Ek.t), dd = seg_sum<L>(Ek.et);
-  if (live && k == 0) {
-    dE[(size_t)p * n + ac] = de;
-    dD[(size_t)p * n + ac] = dd;
+
+  Du gp[14];
+#pragma unroll
+  for (int p = 0; p < 14; ++p) gp[p] = du(0.0);
+  if (act) {
+    Du b = du(1.0), bz = du(0.0), bb = du(0.0), bn = du(0.0);
+    if (z.v > 0.0 && beta != 0.0) {
+      const Du X = beta * z;
+      const Du lX = dlog(X);
+      const Du P = dexp(nn * lX);
+      const Du l1P = dlog1p(P);
+      b = dexp((-0.5 / nn) * l1P);
+      const Du q = P * dinv(1.0 + P);          // P / (1 + P)
+      bz = (-0.5) * (b * q * dinv(z));          // db/dzeta
+      bb = (-0.5 / beta) * (b * q);             // db/dbeta
+      bn = b * ((0.5 / (nn * nn)) * l1P - (0.5 / nn) * (q * lX));  // db/dn
+    }
+    const Du eR = dexp(-l1 * r), eA = dexp(-l2 * r);
+    const Du fR = A * eR, fA = -B * eA;
+    const Du half = 0.5 * fr.f;
+    const Du sumf = fR + b * fA;
+    gp[TH_A] = half * eR;
+    gp[TH_B] = -(half * (b * eA));
+    gp[TH_L1] = -(half * (r * fR));
+    gp[TH_L2] = -(half * (b * (r * fA)));
+    const Du cb = half * fA;                    // dV/db
+    gp[TH_BETA] = cb * bb;
+    gp[TH_N] = cb * bn;
+    const Du cz = cb * bz;                      // dV/dzeta
+    gp[TH_R] = 0.5 * (fr.fR * sumf) + cz * zR;
+    gp[TH_DD] = 0.5 * (fr.fD * sumf) + cz * zD;
+    gp[TH_C] = cz * zc;
+    gp[TH_D] = cz * zd;
+    gp[TH_H] = cz * zh;
+    gp[TH_L3] = cz * zl3;
+    gp[TH_M] = cz * zm;
+    gp[TH_GAM] = cz * zg;
+  }
+#pragma unroll
+  for (int p = 0; p < 14; ++p) {
+    const double de = seg_sum<L>(gp[p].v), dd = seg_sum<L>(gp[p].e);
+    if (live && k == 0) {
+      dE[(size_t)p * n + ac] = de;
+      dD[(size_t)p * n + ac] = dd;
+    }
   }
 }
 
@@ -278,11 +354,28 @@ __global__ __launch_bounds__(256) void k_fit_reduce(const Geom* __restrict__ geo
 
 struct FitState {
   int S = 0, n = 0;
+  bool loaded = false;
   std::vector<int64_t> off;
   std::vector<double> h_lat, h_pos, h_ref, h_fref;
   SetDev set;
   NList nl;
   DevBuf pos, spc, fref, ref, F, es, lam, lf, loss_s, kappa, muv, dE, dD, grad_s, theta;
+  // pinned staging: [params 16 | theta 14 | loss_s S | grad_s 14 S | es 10 S]
+  double* pin = nullptr;
+  size_t pin_n = 0;
+  double* pinned(size_t n_doubles) {
+    if (n_doubles > pin_n) {
+      if (pin) AK_HIP(hipHostFree(pin));
+      pin = nullptr;
+      pin_n = 0;
+      AK_HIP(hipHostMalloc((void**)&pin, sizeof(double) * n_doubles, hipHostMallocDefault));
+      pin_n = n_doubles;
+    }
+    return pin;
+  }
+  ~FitState() {
+    if (pin) (void)hipHostFree(pin);
+  }
 };
 
 static FitState& fit_state(Potential& pot) {
@@ -293,19 +386,56 @@ static FitState& fit_state(Potential& pot) {
 template <int L>
 static void launch_hd(FitState& F, int n, hipStream_t st) {
   const int64_t threads = (int64_t)n * L;
-  dim3 grid((unsigned)((threads + 255) / 256), 14);
-  hipLaunchKernelGGL(k_tersoff_hd<L>, grid, dim3(256), 0, st, n, F.pos.as<double>(),
-                     F.set.d_sid.as<int>(), F.set.d_geoms.as<Geom>(), F.theta.as<double>(),
-                     F.lam.as<double>(), F.muv.as<double>(), F.nl.cnt.as<int>(),
-                     F.nl.nbr.as<int>(), F.nl.stride, F.dE.as<double>(), F.dD.as<double>());
+  hipLaunchKernelGGL(k_tersoff_grad<L>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
+                     st, n, F.pos.as<double>(), F.set.d_sid.as<int>(), F.set.d_geoms.as<Geom>(),
+                     F.theta.as<double>(), F.lam.as<double>(), F.muv.as<double>(),
+                     F.nl.cnt.as<int>(), F.nl.nbr.as<int>(), F.nl.stride, F.dE.as<double>(),
+                     F.dD.as<double>());
 }
 
-void tersoff_loss_grad_run(Potential& pot, int S, const int64_t* off, const double* lattices,
-                           const double* positions, const double* theta,
-                           const double* ref_energy, const double* ref_forces,
-                           const double* ref_stress, const double* weights, double* loss,
-                           double* grad, double* energies, double* forces, double* stresses,
-                           hipStream_t st) {
+// Upload a batch of structures and their references; the list is built at
+// the first evaluation (its reach depends on R + D).
+void tersoff_fit_set(Potential& pot, int S, const int64_t* off, const double* lattices,
+                     const double* positions, const double* ref_energy,
+                     const double* ref_forces, const double* ref_stress, hipStream_t st) {
+  AK_REQUIRE(S >= 1 && off[0] == 0, AK_ERR_ARG, "bad structure offsets");
+  const int n = (int)off[S];
+  AK_REQUIRE(n > 0, AK_ERR_ARG, "no atoms");
+  FitState& F = fit_state(pot);
+  std::vector<double> ref(10 * (size_t)S);
+  for (int s = 0; s < S; ++s) {
+    ref[10 * s] = ref_energy[s];
+    std::memcpy(ref.data() + 10 * s + 1, ref_stress + 9 * s, 9 * sizeof(double));
+  }
+  F.S = S;
+  F.n = n;
+  F.off.assign(off, off + S + 1);
+  F.h_lat.assign(lattices, lattices + 9 * (size_t)S);
+  F.h_pos.assign(positions, positions + 3 * (size_t)n);
+  F.h_fref.assign(ref_forces, ref_forces + 3 * (size_t)n);
+  F.h_ref = ref;
+  F.pos.ensure(sizeof(double) * 3 * n);
+  F.fref.ensure(sizeof(double) * 3 * n);
+  F.ref.ensure(sizeof(double) * 10 * S);
+  F.spc.ensure(sizeof(int) * n);
+  AK_HIP(hipMemcpyAsync(F.pos.p, positions, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+                        st));
+  AK_HIP(hipMemcpyAsync(F.fref.p, ref_forces, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+                        st));
+  AK_HIP(hipMemcpyAsync(F.ref.p, ref.data(), sizeof(double) * 10 * S, hipMemcpyHostToDevice,
+                        st));
+  AK_HIP(hipMemsetAsync(F.spc.p, 0, sizeof(int) * n, st));
+  AK_HIP(hipStreamSynchronize(st));
+  F.nl.valid = false;
+  F.loaded = true;
+}
+
+// Loss and gradient at theta for the loaded batch.
+void tersoff_fit_eval(Potential& pot, const double* theta, const double* weights, double* loss,
+                      double* grad, double* energies, double* forces, double* stresses,
+                      hipStream_t st) {
+  FitState& F = fit_state(pot);
+  AK_REQUIRE(F.loaded, AK_ERR_STATE, "no structures loaded for the fit");
   for (int q = 0; q < 14; ++q)
     AK_REQUIRE(std::isfinite(theta[q]), AK_ERR_ARG, "non-finite Tersoff parameter");
   const double A = theta[0], B = theta[1], beta = theta[5], nn = theta[6], c = theta[7],
@@ -313,60 +443,31 @@ void tersoff_loss_grad_run(Potential& pot, int S, const int64_t* off, const doub
   AK_REQUIRE(A > 0 && B > 0 && beta > 0 && nn > 0 && c > 0 && d > 0 && D > 0 && R > D &&
                  m > 0 && theta[4] >= 0,
              AK_ERR_ARG, "Tersoff parameters out of their domain");
-  const int n = (int)off[S];
-  AK_REQUIRE(n > 0, AK_ERR_ARG, "no atoms");
-  FitState& F = fit_state(pot);
+  const int S = F.S, n = F.n;
   const double rc = R + D;
+  double* pin = F.pinned(30 + 25 * (size_t)S);
+  double* pin_ls = pin + 30;
+  double* pin_gs = pin_ls + S;
+  double* pin_es = pin_gs + 14 * (size_t)S;
 
-  // parameter tables of the forward kernels (one species)
-  double tb[9] = {1.0, A, B, theta[2], theta[3], R, D, beta, nn};
-  double tri[7] = {1.0, c, d, theta[9], theta[4], m, theta[13]};
+  // parameter tables of the forward kernels (one species) and theta, staged
+  // through pinned memory
+  const double tb[9] = {1.0, A, B, theta[2], theta[3], R, D, beta, nn};
+  const double tri[7] = {1.0, c, d, theta[9], theta[4], m, theta[13]};
   std::memcpy(pot.h_params.data(), tb, sizeof tb);
   std::memcpy(pot.h_params.data() + 9, tri, sizeof tri);
-  AK_HIP(hipMemcpyAsync(pot.d_params.p, pot.h_params.data(), sizeof(double) * 16,
-                        hipMemcpyHostToDevice, st));
+  std::memcpy(pin, pot.h_params.data(), sizeof(double) * 16);
+  std::memcpy(pin + 16, theta, sizeof(double) * 14);
+  AK_HIP(hipMemcpyAsync(pot.d_params.p, pin, sizeof(double) * 16, hipMemcpyHostToDevice, st));
   pot.r_cut = rc;
   F.theta.ensure(sizeof(double) * 14);
-  AK_HIP(hipMemcpyAsync(F.theta.p, theta, sizeof(double) * 14, hipMemcpyHostToDevice, st));
+  AK_HIP(hipMemcpyAsync(F.theta.p, pin + 16, sizeof(double) * 14, hipMemcpyHostToDevice, st));
 
-  // structures: upload once, reuse while the inputs are unchanged
-  std::vector<double> ref(10 * (size_t)S);
-  for (int s = 0; s < S; ++s) {
-    ref[10 * s] = ref_energy[s];
-    std::memcpy(ref.data() + 10 * s + 1, ref_stress + 9 * s, 9 * sizeof(double));
-  }
-  bool same = F.S == S && F.n == n && std::equal(F.off.begin(), F.off.end(), off) &&
-              F.off.size() == (size_t)S + 1 &&
-              std::memcmp(F.h_lat.data(), lattices, sizeof(double) * 9 * S) == 0 &&
-              std::memcmp(F.h_pos.data(), positions, sizeof(double) * 3 * n) == 0 &&
-              std::memcmp(F.h_fref.data(), ref_forces, sizeof(double) * 3 * n) == 0 &&
-              F.h_ref == ref;
-  if (!same) {
-    F.S = S;
-    F.n = n;
-    F.off.assign(off, off + S + 1);
-    F.h_lat.assign(lattices, lattices + 9 * (size_t)S);
-    F.h_pos.assign(positions, positions + 3 * (size_t)n);
-    F.h_fref.assign(ref_forces, ref_forces + 3 * (size_t)n);
-    F.h_ref = ref;
-    F.pos.ensure(sizeof(double) * 3 * n);
-    F.fref.ensure(sizeof(double) * 3 * n);
-    F.ref.ensure(sizeof(double) * 10 * S);
-    F.spc.ensure(sizeof(int) * n);
-    AK_HIP(hipMemcpyAsync(F.pos.p, positions, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
-                          st));
-    AK_HIP(hipMemcpyAsync(F.fref.p, ref_forces, sizeof(double) * 3 * n,
-                          hipMemcpyHostToDevice, st));
-    AK_HIP(hipMemcpyAsync(F.ref.p, ref.data(), sizeof(double) * 10 * S, hipMemcpyHostToDevice,
-                          st));
-    AK_HIP(hipMemsetAsync(F.spc.p, 0, sizeof(int) * n, st));
-    F.nl.valid = false;
-  }
   if (!F.nl.valid || F.nl.r_cut < rc) {
     F.nl.r_cut = rc * 1.05;  // head room for fitted R and D
     F.nl.skin = 0.0;
     F.nl.r_list = F.nl.r_cut;
-    F.set.setup(S, off, lattices, nullptr, F.nl.r_list, st);
+    F.set.setup(S, F.off.data(), F.h_lat.data(), nullptr, F.nl.r_list, st);
     nl_build(F.nl, F.set, F.pos.as<double>(), st);
   }
   AK_REQUIRE(F.nl.stride <= 64, AK_ERR_CAPACITY, "more than 64 neighbours per atom");
@@ -392,7 +493,6 @@ void tersoff_loss_grad_run(Potential& pot, int S, const int64_t* off, const doub
                      (const double*)F.es.as<double>(), (const double*)F.ref.as<double>(),
                      (const double*)F.lf.as<double>(), wE, wS, eps0, F.loss_s.as<double>(),
                      F.kappa.as<double>(), F.muv.as<double>());
-  std::vector<double> ls(S), gs;
   if (grad) {
     F.dE.ensure(sizeof(double) * 14 * (size_t)n);
     F.dD.ensure(sizeof(double) * 14 * (size_t)n);
@@ -408,32 +508,54 @@ void tersoff_loss_grad_run(Potential& pot, int S, const int64_t* off, const doub
     hipLaunchKernelGGL(k_fit_reduce, dim3(S, 14), dim3(256), 0, st, F.set.d_geoms.as<Geom>(),
                        (const double*)F.dE.as<double>(), (const double*)F.dD.as<double>(),
                        (const double*)F.kappa.as<double>(), n, F.grad_s.as<double>());
-    gs.resize(14 * (size_t)S);
-    AK_HIP(hipMemcpyAsync(gs.data(), F.grad_s.p, sizeof(double) * gs.size(),
+    AK_HIP(hipMemcpyAsync(pin_gs, F.grad_s.p, sizeof(double) * 14 * (size_t)S,
                           hipMemcpyDeviceToHost, st));
   }
-  AK_HIP(hipMemcpyAsync(ls.data(), F.loss_s.p, sizeof(double) * S, hipMemcpyDeviceToHost, st));
-  std::vector<double> es;
-  if (energies || stresses) {
-    es.resize(10 * (size_t)S);
-    AK_HIP(hipMemcpyAsync(es.data(), F.es.p, sizeof(double) * es.size(), hipMemcpyDeviceToHost,
-                          st));
-  }
+  AK_HIP(hipMemcpyAsync(pin_ls, F.loss_s.p, sizeof(double) * S, hipMemcpyDeviceToHost, st));
+  if (energies || stresses)
+    AK_HIP(hipMemcpyAsync(pin_es, F.es.p, sizeof(double) * 10 * (size_t)S,
+                          hipMemcpyDeviceToHost, st));
   if (forces)
     AK_HIP(hipMemcpyAsync(forces, F.F.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
   AK_HIP(hipStreamSynchronize(st));
   double L = 0.0;
-  for (int s = 0; s < S; ++s) L += ls[s];
+  for (int s = 0; s < S; ++s) L += pin_ls[s];
   *loss = L;
   if (grad) {
     for (int q = 0; q < 14; ++q) grad[q] = 0.0;
     for (int s = 0; s < S; ++s)
-      for (int q = 0; q < 14; ++q) grad[q] += gs[14 * (size_t)s + q];
+      for (int q = 0; q < 14; ++q) grad[q] += pin_gs[14 * (size_t)s + q];
   }
   for (int s = 0; s < S; ++s) {
-    if (energies) energies[s] = es[10 * s];
-    if (stresses) std::memcpy(stresses + 9 * s, es.data() + 10 * s + 1, 9 * sizeof(double));
+    if (energies) energies[s] = pin_es[10 * s];
+    if (stresses) std::memcpy(stresses + 9 * s, pin_es + 10 * s + 1, 9 * sizeof(double));
   }
+}
+
+// One-shot form: (re)load when the batch differs from the loaded one, then
+// evaluate.
+void tersoff_loss_grad_run(Potential& pot, int S, const int64_t* off, const double* lattices,
+                           const double* positions, const double* theta,
+                           const double* ref_energy, const double* ref_forces,
+                           const double* ref_stress, const double* weights, double* loss,
+                           double* grad, double* energies, double* forces, double* stresses,
+                           hipStream_t st) {
+  FitState& F = fit_state(pot);
+  const int n = (int)off[S];
+  bool same = F.loaded && F.S == S && F.n == n && F.off.size() == (size_t)S + 1 &&
+              std::equal(F.off.begin(), F.off.end(), off) &&
+              std::memcmp(F.h_lat.data(), lattices, sizeof(double) * 9 * S) == 0 &&
+              std::memcmp(F.h_pos.data(), positions, sizeof(double) * 3 * n) == 0 &&
+              std::memcmp(F.h_fref.data(), ref_forces, sizeof(double) * 3 * n) == 0;
+  if (same) {
+    for (int s = 0; s < S && same; ++s)
+      same = F.h_ref[10 * s] == ref_energy[s] &&
+             std::memcmp(F.h_ref.data() + 10 * s + 1, ref_stress + 9 * s,
+                         9 * sizeof(double)) == 0;
+  }
+  if (!same)
+    tersoff_fit_set(pot, S, off, lattices, positions, ref_energy, ref_forces, ref_stress, st);
+  tersoff_fit_eval(pot, theta, weights, loss, grad, energies, forces, stresses, st);
 }
 
 }  // namespace ak

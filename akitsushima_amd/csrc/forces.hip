@@ -219,7 +219,7 @@ __device__ __forceinline__ void k_tersoff_site_chunk(int c, int n_chunks,
   // Per partner leg s: cos of the angle, and g, g', the exponential factor
   // of the triple (a; this bond, s).  Kept in registers for the gradient
   // pass when the segment is short; recomputed otherwise.
-  constexpr bool kCache = L <= 8;
+  constexpr bool kCache = L <= 4;
   constexpr int NC = kCache ? L : 1;
   double c_cs[NC], c_g[NC], c_dg[NC], c_e[NC], c_de[NC];
   double c_wx[NC], c_wy[NC], c_wz[NC], c_wr[NC], c_wri[NC], c_wfc[NC];

@@ -16,6 +16,7 @@ it as a ``torch.autograd.Function``.
 from __future__ import annotations
 
 import ctypes
+import threading
 
 import numpy as np
 
@@ -43,7 +44,7 @@ class TersoffFit:
     """structures: list of dicts with lattice, positions, ref_energy (eV per
     atom), ref_forces (N, 3), ref_stress (3, 3)."""
 
-    def __init__(self, structures, weights=None):
+    def __init__(self, structures, weights=None, autotune=True):
         w = dict(DEFAULT_WEIGHTS)
         w.update(weights or {})
         self.weights = np.array([w["energy"], w["forces"], w["stress"], w["r0"], w["eps0"]])
@@ -64,6 +65,26 @@ class TersoffFit:
         self._pot = TersoffPotential({"A": 1.0, "B": 1.0, "lambda1": 1.0, "lambda2": 1.0,
                                       "lambda3": 0.0, "beta": 1.0, "n": 1.0, "c": 1.0,
                                       "d": 1.0, "h": 0.0, "R": 2.85, "D": 0.15})
+        self._tls = threading.local()   # per thread: the handle holding the batch
+        self._autotune = bool(autotune) and S > 1
+        self.launch_shape = None
+
+    def _handle(self):
+        """This thread's potential handle, with the batch uploaded once."""
+        h = self._pot.handle()
+        if getattr(self._tls, "loaded", None) != h:
+            P = _lib._pf64
+            check(lib().ak_tersoff_fit_load(
+                h, self.S, self.off.ctypes.data_as(_lib._pi64), self.lat.ctypes.data_as(P),
+                self.pos.ctypes.data_as(P), self.ref_e.ctypes.data_as(P),
+                self.ref_f.ctypes.data_as(P), self.ref_s.ctypes.data_as(P)))
+            self._tls.loaded = h
+            if self._autotune and self.launch_shape is None:
+                # launch shape of the forward force pass, timed on this batch
+                cells = [(self.lat[k], self.pos[self.off[k]:self.off[k + 1]],
+                          ["Si"] * int(self.off[k + 1] - self.off[k])) for k in range(self.S)]
+                self.launch_shape = self._pot.autotune(cells, reps=2)["best"]
+        return h
 
     def evaluate(self, theta, want_grad=True, want_outputs=False):
         """(loss, grad (14,) or None[, (E (S,), F (sum N, 3), sigma (S, 3, 3))])."""
@@ -77,10 +98,8 @@ class TersoffFit:
         F = np.zeros_like(self.pos) if want_outputs else None
         S = np.zeros((self.S, 3, 3)) if want_outputs else None
         ptr = lambda a: a.ctypes.data_as(P) if a is not None else None
-        check(lib().ak_tersoff_loss_grad(
-            self._pot.handle(), self.S, self.off.ctypes.data_as(_lib._pi64), ptr(self.lat),
-            ptr(self.pos), ptr(th), ptr(self.ref_e), ptr(self.ref_f), ptr(self.ref_s),
-            ptr(self.weights), ctypes.byref(loss), ptr(grad), ptr(E), ptr(F), ptr(S)))
+        check(lib().ak_tersoff_fit_eval(self._handle(), ptr(th), ptr(self.weights),
+                                        ctypes.byref(loss), ptr(grad), ptr(E), ptr(F), ptr(S)))
         if want_outputs:
             return loss.value, grad, (E, F, S)
         return loss.value, grad

@@ -191,6 +191,19 @@ int ak_tersoff_loss_grad(ak_potential *pot, int32_t n_struct, const int64_t *ato
                          const double *weights5, double *loss, double *grad14,
                          double *energies, double *forces, double *stresses);
 
+/* The same loss in two steps, for optimisers that call it many times on one
+ * batch: ak_tersoff_fit_load uploads the structures and references once
+ * (kept on the potential handle), ak_tersoff_fit_eval evaluates loss and
+ * gradient at theta14 (grad14 may be NULL) with no host-side staging of the
+ * batch.  No counterpart in the reference, which has no gradient. */
+int ak_tersoff_fit_load(ak_potential *pot, int32_t n_struct, const int64_t *atom_offsets,
+                        const double *lattices, const double *positions,
+                        const double *ref_energy, const double *ref_forces,
+                        const double *ref_stress);
+int ak_tersoff_fit_eval(ak_potential *pot, const double *theta14, const double *weights5,
+                        double *loss, double *grad14, double *energies, double *forces,
+                        double *stresses);
+
 /* ---- Measurement -------------------------------------------------------- */
 
 /* No counterpart in the reference: timing support for benchmarks.  While

@@ -31,6 +31,21 @@ def test_loss_and_gradient_match_torch_reference(seed, lambda3, gamma):
         assert abs(g[p] - gt[p]) <= 1e-9 * max(abs(gt[p]), 1e-6 * scale), (TT.KEYS[p], g[p], gt[p])
 
 
+def test_gradient_with_a_linear_exponent_m():
+    """m = 1 takes the multiplied-out power in the rows of the other
+    parameters and the general power in the row of m itself."""
+    theta, data = make_set(n_struct=2, seed=17, lambda3=1.1, gamma=1.0)
+    theta = theta.copy()
+    theta[12] = 1.0
+    fit, (L, g) = _gpu(theta, data)
+    L0 = oracle_loss(theta, data)
+    assert abs(L - L0) <= 1e-10 * abs(L0)
+    Lt, gt = TT.loss_and_grad(theta, data)
+    scale = np.abs(gt).max()
+    for p in range(14):
+        assert abs(g[p] - gt[p]) <= 1e-9 * max(abs(gt[p]), 1e-6 * scale), (TT.KEYS[p], g[p], gt[p])
+
+
 def test_gradient_matches_central_differences():
     theta, data = make_set(n_struct=2, seed=21)
     _, (L, g) = _gpu(theta, data)
