@@ -348,6 +348,33 @@ __global__ __launch_bounds__(256) void k_fit_reduce(const Geom* __restrict__ geo
   if (threadIdx.x == 0) grad_s[14 * (size_t)s + p] = kappa[s] * v[0] + v[1];
 }
 
+// out[0] = sum_s loss_s, out[1 + p] = sum_s grad_s[14 s + p] (fixed order,
+// one block); with es != nullptr also out[15 + q] = es[q] for the 10 S
+// per-structure values, so that one copy brings everything back.
+__global__ __launch_bounds__(256) void k_fit_total(int S, const double* __restrict__ loss_s,
+                                                   const double* __restrict__ grad_s,
+                                                   const double* __restrict__ es,
+                                                   double* __restrict__ out) {
+  __shared__ double smem[15 * 16];
+  double v[15];
+#pragma unroll
+  for (int q = 0; q < 15; ++q) v[q] = 0.0;
+  for (int st = threadIdx.x; st < S; st += blockDim.x) {
+    v[0] += loss_s[st];
+    if (grad_s) {
+#pragma unroll
+      for (int p = 0; p < 14; ++p) v[1 + p] += grad_s[14 * (size_t)st + p];
+    }
+  }
+  block_sum<15>(v, smem);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < 15; ++q) out[q] = v[q];
+  }
+  if (es)
+    for (int q = threadIdx.x; q < 10 * S; q += blockDim.x) out[15 + q] = es[q];
+}
+
 // ---------------------------------------------------------------------------
 // Host driver
 // ---------------------------------------------------------------------------
@@ -359,8 +386,8 @@ struct FitState {
   std::vector<double> h_lat, h_pos, h_ref, h_fref;
   SetDev set;
   NList nl;
-  DevBuf pos, spc, fref, ref, F, es, lam, lf, loss_s, kappa, muv, dE, dD, grad_s, theta;
-  // pinned staging: [params 16 | theta 14 | loss_s S | grad_s 14 S | es 10 S]
+  DevBuf pos, spc, fref, ref, F, es, lam, lf, loss_s, kappa, muv, dE, dD, grad_s, cbuf, tot;
+  // pinned staging: [params 16 | theta 14 | loss, grad 15 | es 10 S]
   double* pin = nullptr;
   size_t pin_n = 0;
   double* pinned(size_t n_doubles) {
@@ -388,7 +415,8 @@ static void launch_hd(FitState& F, int n, hipStream_t st) {
   const int64_t threads = (int64_t)n * L;
   hipLaunchKernelGGL(k_tersoff_grad<L>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
                      st, n, F.pos.as<double>(), F.set.d_sid.as<int>(), F.set.d_geoms.as<Geom>(),
-                     F.theta.as<double>(), F.lam.as<double>(), F.muv.as<double>(),
+                     (const double*)(F.cbuf.as<double>() + 16), F.lam.as<double>(),
+                     F.muv.as<double>(),
                      F.nl.cnt.as<int>(), F.nl.nbr.as<int>(), F.nl.stride, F.dE.as<double>(),
                      F.dD.as<double>());
 }
@@ -445,23 +473,34 @@ void tersoff_fit_eval(Potential& pot, const double* theta, const double* weights
              AK_ERR_ARG, "Tersoff parameters out of their domain");
   const int S = F.S, n = F.n;
   const double rc = R + D;
-  double* pin = F.pinned(30 + 25 * (size_t)S);
-  double* pin_ls = pin + 30;
-  double* pin_gs = pin_ls + S;
-  double* pin_es = pin_gs + 14 * (size_t)S;
+  double* pin = F.pinned(45 + 10 * (size_t)S);
+  double* pin_out = pin + 30;
 
-  // parameter tables of the forward kernels (one species) and theta, staged
-  // through pinned memory
+  // parameter tables of the forward kernels (one species) and theta: one
+  // copy from pinned memory into cbuf = [tb 9 | tri 7 | theta 14]; the
+  // forward pass reads its tables from there for the duration of the call
   const double tb[9] = {1.0, A, B, theta[2], theta[3], R, D, beta, nn};
   const double tri[7] = {1.0, c, d, theta[9], theta[4], m, theta[13]};
   std::memcpy(pot.h_params.data(), tb, sizeof tb);
   std::memcpy(pot.h_params.data() + 9, tri, sizeof tri);
   std::memcpy(pin, pot.h_params.data(), sizeof(double) * 16);
   std::memcpy(pin + 16, theta, sizeof(double) * 14);
-  AK_HIP(hipMemcpyAsync(pot.d_params.p, pin, sizeof(double) * 16, hipMemcpyHostToDevice, st));
+  F.cbuf.ensure(sizeof(double) * 30);
+  AK_HIP(hipMemcpyAsync(F.cbuf.p, pin, sizeof(double) * 30, hipMemcpyHostToDevice, st));
   pot.r_cut = rc;
-  F.theta.ensure(sizeof(double) * 14);
-  AK_HIP(hipMemcpyAsync(F.theta.p, pin + 16, sizeof(double) * 14, hipMemcpyHostToDevice, st));
+  struct ParamView {  // pot.d_params -> cbuf while this call runs
+    DevBuf& b;
+    void* p0;
+    size_t n0;
+    ParamView(DevBuf& buf, void* p) : b(buf), p0(buf.p), n0(buf.bytes) {
+      b.p = p;
+      b.bytes = sizeof(double) * 16;
+    }
+    ~ParamView() {
+      b.p = p0;
+      b.bytes = n0;
+    }
+  } view(pot.d_params, F.cbuf.p);
 
   if (!F.nl.valid || F.nl.r_cut < rc) {
     F.nl.r_cut = rc * 1.05;  // head room for fitted R and D
@@ -508,27 +547,25 @@ void tersoff_fit_eval(Potential& pot, const double* theta, const double* weights
     hipLaunchKernelGGL(k_fit_reduce, dim3(S, 14), dim3(256), 0, st, F.set.d_geoms.as<Geom>(),
                        (const double*)F.dE.as<double>(), (const double*)F.dD.as<double>(),
                        (const double*)F.kappa.as<double>(), n, F.grad_s.as<double>());
-    AK_HIP(hipMemcpyAsync(pin_gs, F.grad_s.p, sizeof(double) * 14 * (size_t)S,
-                          hipMemcpyDeviceToHost, st));
   }
-  AK_HIP(hipMemcpyAsync(pin_ls, F.loss_s.p, sizeof(double) * S, hipMemcpyDeviceToHost, st));
-  if (energies || stresses)
-    AK_HIP(hipMemcpyAsync(pin_es, F.es.p, sizeof(double) * 10 * (size_t)S,
-                          hipMemcpyDeviceToHost, st));
+  F.tot.ensure(sizeof(double) * (15 + 10 * (size_t)S));
+  const bool want_es = energies || stresses;
+  hipLaunchKernelGGL(k_fit_total, dim3(1), dim3(256), 0, st, S,
+                     (const double*)F.loss_s.as<double>(),
+                     (const double*)(grad ? F.grad_s.as<double>() : nullptr),
+                     (const double*)(want_es ? F.es.as<double>() : nullptr), F.tot.as<double>());
+  AK_HIP(hipMemcpyAsync(pin_out, F.tot.p, sizeof(double) * (15 + (want_es ? 10 * (size_t)S : 0)),
+                        hipMemcpyDeviceToHost, st));
   if (forces)
     AK_HIP(hipMemcpyAsync(forces, F.F.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
   AK_HIP(hipStreamSynchronize(st));
-  double L = 0.0;
-  for (int s = 0; s < S; ++s) L += pin_ls[s];
-  *loss = L;
-  if (grad) {
-    for (int q = 0; q < 14; ++q) grad[q] = 0.0;
-    for (int s = 0; s < S; ++s)
-      for (int q = 0; q < 14; ++q) grad[q] += pin_gs[14 * (size_t)s + q];
-  }
+  *loss = pin_out[0];
+  if (grad)
+    for (int q = 0; q < 14; ++q) grad[q] = pin_out[1 + q];
+  const double* pes = pin_out + 15;
   for (int s = 0; s < S; ++s) {
-    if (energies) energies[s] = pin_es[10 * s];
-    if (stresses) std::memcpy(stresses + 9 * s, pin_es + 10 * s + 1, 9 * sizeof(double));
+    if (energies) energies[s] = pes[10 * s];
+    if (stresses) std::memcpy(stresses + 9 * s, pes + 10 * s + 1, 9 * sizeof(double));
   }
 }
 
