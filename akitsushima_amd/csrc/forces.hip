@@ -219,49 +219,91 @@ __device__ __forceinline__ void k_tersoff_site_chunk(int c, int n_chunks,
   // Per partner leg s: cos of the angle, and g, g', the exponential factor
   // of the triple (a; this bond, s).  Kept in registers for the gradient
   // pass when the segment is short; recomputed otherwise.
-  constexpr bool kCache = L <= 4;
+  constexpr bool kCache = L == 4;
   constexpr int NC = kCache ? L : 1;
   double c_cs[NC], c_g[NC], c_dg[NC], c_e[NC], c_de[NC];
   double c_wx[NC], c_wy[NC], c_wz[NC], c_wr[NC], c_wri[NC], c_wfc[NC];
   int c_wsp[NC];
 
+  // Rows of 8 or more slots (a skin, or liquid-like cells): the active legs
+  // are staged in LDS in slot order and both partner loops run over them
+  // only -- the same terms in the same order as a loop over every slot that
+  // skips the inactive ones.
+  const ActiveLegs<L> al(act, k, seg);
+  __shared__ double t_x[256], t_y[256], t_z[256], t_r[256], t_ri[256], t_fc[256], t_cw[256];
+  __shared__ int t_sp[256];
+  const int lbase = (int)threadIdx.x & ~(L - 1);
+
   // zeta of this bond: sum over the other legs of the segment
   double zeta = 0.0;
-#pragma unroll
-  for (int s = 0; s < L; ++s) {
-    double wx = seg_get<L>(ux, seg, s), wy = seg_get<L>(uy, seg, s), wz = seg_get<L>(uz, seg, s);
-    double wr = seg_get<L>(r, seg, s), wri = seg_get<L>(rinv, seg, s);
-    double wfc = seg_get<L>(fcl, seg, s);
-    int wsp = ns > 1 ? seg_get<L>(sj, seg, s) : 0;
-    double cs = 0.0, g = 0.0, dg = 0.0, ex = 0.0, dex = 0.0;
-    const bool geo = s != k && act && wri != 0.0;
-    if (geo) {
-      cs = (ux * wx + uy * wy + uz * wz) * rinv * wri;
+  if constexpr (L == 4) {
+  #pragma unroll
+    for (int s = 0; s < L; ++s) {
+      double wx = seg_get<L>(ux, seg, s), wy = seg_get<L>(uy, seg, s), wz = seg_get<L>(uz, seg, s);
+      double wr = seg_get<L>(r, seg, s), wri = seg_get<L>(rinv, seg, s);
+      double wfc = seg_get<L>(fcl, seg, s);
+      int wsp = ns > 1 ? seg_get<L>(sj, seg, s) : 0;
+      double cs = 0.0, g = 0.0, dg = 0.0, ex = 0.0, dex = 0.0;
+      const bool geo = s != k && act && wri != 0.0;
+      if (geo) {
+        cs = (ux * wx + uy * wy + uz * wz) * rinv * wri;
+        const double* T = tri + (((size_t)sa * ns + sj) * ns + wsp) * 7;
+        if (T[0] != 0.0) {
+          double c = T[1], d = T[2], hc = T[3] - cs;
+          double c2 = c * c, d2 = d * d;
+          double q = d2 + hc * hc;
+          double iq = 1.0 / q;
+          g = 1.0 + c2 / d2 - c2 * iq;
+          dg = -2.0 * c2 * hc * iq * iq;
+          te_exp(T[4], T[5], r - wr, ex, dex);
+          if (pair_ok && wfc != 0.0) zeta += T[6] * wfc * g * ex;
+        }
+      }
+      if constexpr (kCache) {
+        c_cs[s] = cs;
+        c_g[s] = g;
+        c_dg[s] = dg;
+        c_e[s] = ex;
+        c_de[s] = dex;
+        c_wx[s] = wx;
+        c_wy[s] = wy;
+        c_wz[s] = wz;
+        c_wr[s] = wr;
+        c_wri[s] = wri;
+        c_wfc[s] = wfc;
+        c_wsp[s] = wsp;
+      }
+    }
+  } else {
+    if (act) {
+      const int q = lbase + al.rank;
+      t_x[q] = ux;
+      t_y[q] = uy;
+      t_z[q] = uz;
+      t_r[q] = r;
+      t_ri[q] = rinv;
+      t_fc[q] = fcl;
+      t_sp[q] = sj;
+    }
+    __syncthreads();
+    for (int t = 0; t < al.wave_max; ++t) {
+      if (!act || t >= al.count || t == al.rank) continue;
+      const int q = lbase + t;
+      const double wx = t_x[q], wy = t_y[q], wz = t_z[q], wr = t_r[q], wri = t_ri[q];
+      const double wfc = t_fc[q];
+      const int wsp = ns > 1 ? t_sp[q] : 0;
+      const double cs = (ux * wx + uy * wy + uz * wz) * rinv * wri;
       const double* T = tri + (((size_t)sa * ns + sj) * ns + wsp) * 7;
       if (T[0] != 0.0) {
         double c = T[1], d = T[2], hc = T[3] - cs;
         double c2 = c * c, d2 = d * d;
-        double q = d2 + hc * hc;
-        double iq = 1.0 / q;
-        g = 1.0 + c2 / d2 - c2 * iq;
-        dg = -2.0 * c2 * hc * iq * iq;
+        double qq = d2 + hc * hc;
+        double iq = 1.0 / qq;
+        double g = 1.0 + c2 / d2 - c2 * iq;
+        double ex, dex;
         te_exp(T[4], T[5], r - wr, ex, dex);
         if (pair_ok && wfc != 0.0) zeta += T[6] * wfc * g * ex;
       }
-    }
-    if constexpr (kCache) {
-      c_cs[s] = cs;
-      c_g[s] = g;
-      c_dg[s] = dg;
-      c_e[s] = ex;
-      c_de[s] = dex;
-      c_wx[s] = wx;
-      c_wy[s] = wy;
-      c_wz[s] = wz;
-      c_wr[s] = wr;
-      c_wri[s] = wri;
-      c_wfc[s] = wfc;
-      c_wsp[s] = wsp;
     }
   }
 
@@ -292,81 +334,129 @@ __device__ __forceinline__ void k_tersoff_site_chunk(int c, int n_chunks,
   //   own zeta (this bond's b depends on the angle to every other leg w) and
   //   every other bond's zeta in which u is the third leg.
   double su = dVdr * rinv, swx = 0.0, swy = 0.0, swz = 0.0;
-#pragma unroll
-  for (int s = 0; s < L; ++s) {
-    const double wc = seg_get<L>(cown, seg, s);
-    double wx, wy, wz, wr, wri, wfc, cs, gA, dgA, eA, deA;
-    int wsp;
-    if constexpr (kCache) {
-      wx = c_wx[s];
-      wy = c_wy[s];
-      wz = c_wz[s];
-      wr = c_wr[s];
-      wri = c_wri[s];
-      wfc = c_wfc[s];
-      wsp = c_wsp[s];
-      cs = c_cs[s];
-      gA = c_g[s];
-      dgA = c_dg[s];
-      eA = c_e[s];
-      deA = c_de[s];
-    } else {
-      wx = seg_get<L>(ux, seg, s);
-      wy = seg_get<L>(uy, seg, s);
-      wz = seg_get<L>(uz, seg, s);
-      wr = seg_get<L>(r, seg, s);
-      wri = seg_get<L>(rinv, seg, s);
-      wfc = seg_get<L>(fcl, seg, s);
-      wsp = ns > 1 ? seg_get<L>(sj, seg, s) : 0;
-      cs = gA = dgA = eA = deA = 0.0;
-      if (s != k && act && wri != 0.0) {
-        cs = (ux * wx + uy * wy + uz * wz) * rinv * wri;
-        const double* T = tri + (((size_t)sa * ns + sj) * ns + wsp) * 7;
-        if (T[0] != 0.0) {
-          double c = T[1], d = T[2], hc = T[3] - cs;
-          double c2 = c * c, q = d * d + hc * hc, iq = 1.0 / q;
-          gA = 1.0 + c2 / (d * d) - c2 * iq;
-          dgA = -2.0 * c2 * hc * iq * iq;
-          te_exp(T[4], T[5], r - wr, eA, deA);
+  if constexpr (L == 4) {
+  #pragma unroll
+    for (int s = 0; s < L; ++s) {
+      const double wc = seg_get<L>(cown, seg, s);
+      double wx, wy, wz, wr, wri, wfc, cs, gA, dgA, eA, deA;
+      int wsp;
+      if constexpr (kCache) {
+        wx = c_wx[s];
+        wy = c_wy[s];
+        wz = c_wz[s];
+        wr = c_wr[s];
+        wri = c_wri[s];
+        wfc = c_wfc[s];
+        wsp = c_wsp[s];
+        cs = c_cs[s];
+        gA = c_g[s];
+        dgA = c_dg[s];
+        eA = c_e[s];
+        deA = c_de[s];
+      } else {
+        wx = seg_get<L>(ux, seg, s);
+        wy = seg_get<L>(uy, seg, s);
+        wz = seg_get<L>(uz, seg, s);
+        wr = seg_get<L>(r, seg, s);
+        wri = seg_get<L>(rinv, seg, s);
+        wfc = seg_get<L>(fcl, seg, s);
+        wsp = ns > 1 ? seg_get<L>(sj, seg, s) : 0;
+        cs = gA = dgA = eA = deA = 0.0;
+        if (s != k && act && wri != 0.0) {
+          cs = (ux * wx + uy * wy + uz * wz) * rinv * wri;
+          const double* T = tri + (((size_t)sa * ns + sj) * ns + wsp) * 7;
+          if (T[0] != 0.0) {
+            double c = T[1], d = T[2], hc = T[3] - cs;
+            double c2 = c * c, q = d * d + hc * hc, iq = 1.0 / q;
+            gA = 1.0 + c2 / (d * d) - c2 * iq;
+            dgA = -2.0 * c2 * hc * iq * iq;
+            te_exp(T[4], T[5], r - wr, eA, deA);
+          }
         }
       }
-    }
-    if (s == k || !act || wri == 0.0) continue;
-    const double* TA = tri + (((size_t)sa * ns + sj) * ns + wsp) * 7;
-    const double* TB = tri + (((size_t)sa * ns + wsp) * ns + sj) * 7;
-    double alpha = 0.0, beta = 0.0;
-    if (cown != 0.0 && wfc != 0.0 && TA[0] != 0.0) {
-      alpha += cown * TA[6] * wfc * dgA * eA;
-      beta += cown * TA[6] * wfc * gA * deA;
-    }
-    if (wc != 0.0 && fcl != 0.0 && TB[0] != 0.0) {
-      double gB, dgB;
-      if (TB == TA) {
-        gB = gA;
-        dgB = dgA;
-      } else {
-        double c = TB[1], d = TB[2], hc = TB[3] - cs;
-        double c2 = c * c, q = d * d + hc * hc, iq = 1.0 / q;
-        gB = 1.0 + c2 / (d * d) - c2 * iq;
-        dgB = -2.0 * c2 * hc * iq * iq;
+      if (s == k || !act || wri == 0.0) continue;
+      const double* TA = tri + (((size_t)sa * ns + sj) * ns + wsp) * 7;
+      const double* TB = tri + (((size_t)sa * ns + wsp) * ns + sj) * 7;
+      double alpha = 0.0, beta = 0.0;
+      if (cown != 0.0 && wfc != 0.0 && TA[0] != 0.0) {
+        alpha += cown * TA[6] * wfc * dgA * eA;
+        beta += cown * TA[6] * wfc * gA * deA;
       }
-      double eB, deB;
-      te_exp(TB[4], TB[5], wr - r, eB, deB);
-      alpha += wc * TB[6] * fcl * dgB * eB;
-      beta += wc * TB[6] * (dfcl * gB * eB - fcl * gB * deB);
+      if (wc != 0.0 && fcl != 0.0 && TB[0] != 0.0) {
+        double gB, dgB;
+        if (TB == TA) {
+          gB = gA;
+          dgB = dgA;
+        } else {
+          double c = TB[1], d = TB[2], hc = TB[3] - cs;
+          double c2 = c * c, q = d * d + hc * hc, iq = 1.0 / q;
+          gB = 1.0 + c2 / (d * d) - c2 * iq;
+          dgB = -2.0 * c2 * hc * iq * iq;
+        }
+        double eB, deB;
+        te_exp(TB[4], TB[5], wr - r, eB, deB);
+        alpha += wc * TB[6] * fcl * dgB * eB;
+        beta += wc * TB[6] * (dfcl * gB * eB - fcl * gB * deB);
+      }
+      // d cos / du = w/(|u||w|) - cos u/|u|^2
+      su += beta * rinv - alpha * cs * rinv * rinv;
+      double aw = alpha * rinv * wri;
+      swx += aw * wx;
+      swy += aw * wy;
+      swz += aw * wz;
     }
-    // d cos / du = w/(|u||w|) - cos u/|u|^2
-    su += beta * rinv - alpha * cs * rinv * rinv;
-    double aw = alpha * rinv * wri;
-    swx += aw * wx;
-    swy += aw * wy;
-    swz += aw * wz;
+  } else {
+    if (act) t_cw[lbase + al.rank] = cown;
+    __syncthreads();
+    for (int t = 0; t < al.wave_max; ++t) {
+      if (!act || t >= al.count || t == al.rank) continue;
+      const int q = lbase + t;
+      const double wx = t_x[q], wy = t_y[q], wz = t_z[q], wr = t_r[q], wri = t_ri[q];
+      const double wfc = t_fc[q], wc = t_cw[q];
+      const int wsp = ns > 1 ? t_sp[q] : 0;
+      const double cs = (ux * wx + uy * wy + uz * wz) * rinv * wri;
+      const double* TA = tri + (((size_t)sa * ns + sj) * ns + wsp) * 7;
+      const double* TB = tri + (((size_t)sa * ns + wsp) * ns + sj) * 7;
+      double gA = 0.0, dgA = 0.0, eA = 0.0, deA = 0.0;
+      if (TA[0] != 0.0) {
+        double c = TA[1], d = TA[2], hc = TA[3] - cs;
+        double c2 = c * c, qq = d * d + hc * hc, iq = 1.0 / qq;
+        gA = 1.0 + c2 / (d * d) - c2 * iq;
+        dgA = -2.0 * c2 * hc * iq * iq;
+        te_exp(TA[4], TA[5], r - wr, eA, deA);
+      }
+      double alpha = 0.0, beta = 0.0;
+      if (cown != 0.0 && wfc != 0.0 && TA[0] != 0.0) {
+        alpha += cown * TA[6] * wfc * dgA * eA;
+        beta += cown * TA[6] * wfc * gA * deA;
+      }
+      if (wc != 0.0 && fcl != 0.0 && TB[0] != 0.0) {
+        double gB, dgB;
+        if (TB == TA) {
+          gB = gA;
+          dgB = dgA;
+        } else {
+          double c = TB[1], d = TB[2], hc = TB[3] - cs;
+          double c2 = c * c, qq = d * d + hc * hc, iq = 1.0 / qq;
+          gB = 1.0 + c2 / (d * d) - c2 * iq;
+          dgB = -2.0 * c2 * hc * iq * iq;
+        }
+        double eB, deB;
+        te_exp(TB[4], TB[5], wr - r, eB, deB);
+        alpha += wc * TB[6] * fcl * dgB * eB;
+        beta += wc * TB[6] * (dfcl * gB * eB - fcl * gB * deB);
+      }
+      su += beta * rinv - alpha * cs * rinv * rinv;
+      double aw = alpha * rinv * wri;
+      swx += aw * wx;
+      swy += aw * wy;
+      swz += aw * wz;
+    }
   }
   const double Gx = su * ux + swx, Gy = su * uy + swy, Gz = su * uz + swz;
   // bonds outside r_c have G = 0 exactly; with rows of 8 or more slots only
   // the active ones are written and the gather skips the rest
   if constexpr (L >= 8) {
-    const ActiveLegs<L> al(act, k, seg);
     if (live && k == 0) amask[ac] = al.mask;
   }
   if (has && (L == 4 || act)) {
