@@ -134,6 +134,17 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// The dispatcher hands consecutive workgroups to the 8 XCDs in turn, and each
+// XCD has its own L2.  Chunks (and atom blocks) are numbered so that XCD x
+// gets the contiguous x-th eighth of them: neighbouring atoms, whose
+// positions and bond gradients a chunk reads, then stay in one L2.  A
+// bijection on [0, nb): the last nb mod 8 blocks keep their numbers.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  const int q = nb / kXcds;
+  return b < q * kXcds ? (b % kXcds) * q + b / kXcds : b;
+}
+
 // Deterministic block reduction of `n` doubles per thread (blockDim.x a
 // multiple of 64, <= 1024); thread 0 receives the sums in out[].
 template <int NV>

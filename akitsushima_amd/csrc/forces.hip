@@ -475,8 +475,8 @@ __global__ __launch_bounds__(256) void k_tersoff_site(int n_chunks, int n, const
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
     uint64_t* __restrict__ amask) {
-  for (int c = blockIdx.x; c < n_chunks; c += gridDim.x)
-    k_tersoff_site_chunk<L, SP1>(c, n_chunks, n, pos, sid, geoms, single, spc, ns_arg, tb, tri, rc, cnt,
+  for (int cc = blockIdx.x; cc < n_chunks; cc += gridDim.x)
+    k_tersoff_site_chunk<L, SP1>(xcd_remap(cc, n_chunks), n_chunks, n, pos, sid, geoms, single, spc, ns_arg, tb, tri, rc, cnt,
                                 nbr, stride, chunk_rng, G, partial, amask);
 }
 
@@ -658,8 +658,8 @@ __global__ __launch_bounds__(256) void k_tersoff_trip(int n_chunks, int n, const
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
     uint64_t* __restrict__ amask) {
-  for (int c = blockIdx.x; c < n_chunks; c += gridDim.x)
-    k_tersoff_trip_chunk<L, SP1>(c, n_chunks, n, pos, sid, geoms, single, spc, ns_arg, tb, tri, rc, cnt,
+  for (int cc = blockIdx.x; cc < n_chunks; cc += gridDim.x)
+    k_tersoff_trip_chunk<L, SP1>(xcd_remap(cc, n_chunks), n_chunks, n, pos, sid, geoms, single, spc, ns_arg, tb, tri, rc, cnt,
                                 nbr, stride, chunk_rng, G, partial, amask);
 }
 
@@ -780,8 +780,8 @@ __global__ __launch_bounds__(256) void k_sw_site(
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
     uint64_t* __restrict__ amask) {
-  for (int c = blockIdx.x; c < n_chunks; c += gridDim.x)
-    k_sw_site_chunk<L>(c, n_chunks, n, pos, sid, geoms, single, prm, cnt, nbr, stride,
+  for (int cc = blockIdx.x; cc < n_chunks; cc += gridDim.x)
+    k_sw_site_chunk<L>(xcd_remap(cc, n_chunks), n_chunks, n, pos, sid, geoms, single, prm, cnt, nbr, stride,
                        chunk_rng, G, partial, amask);
 }
 
@@ -839,7 +839,7 @@ __global__ __launch_bounds__(256) void k_gather(int n, const int* __restrict__ c
     struct_reduce_block((int)blockIdx.x - atom_blocks, geoms, first, n_chunks, partial, out);
     return;
   }
-  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gt = xcd_remap(blockIdx.x, atom_blocks) * blockDim.x + threadIdx.x;
   const int a = gt / L;
   const int k = gt & (L - 1);
   const bool live = a < n;
@@ -1013,8 +1013,8 @@ __global__ __launch_bounds__(256) void k_sw_site_c(
     const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
     uint64_t* __restrict__ amask) {
   extern __shared__ double lds_sw[];
-  for (int c = blockIdx.x; c < n_chunks; c += gridDim.x)
-    k_sw_site_c_chunk<L, LA>(c, n_chunks, n, pos, sid, geoms, single, prm, cnt, nbr, stride,
+  for (int cc = blockIdx.x; cc < n_chunks; cc += gridDim.x)
+    k_sw_site_c_chunk<L, LA>(xcd_remap(cc, n_chunks), n_chunks, n, pos, sid, geoms, single, prm, cnt, nbr, stride,
                              chunk_rng, G, partial, amask, lds_sw);
 }
 
@@ -1034,7 +1034,7 @@ __global__ __launch_bounds__(256) void k_gather_c(int n, const int* __restrict__
     struct_reduce_block((int)blockIdx.x - atom_blocks, geoms, first, n_chunks, partial, out);
     return;
   }
-  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gt = xcd_remap(blockIdx.x, atom_blocks) * blockDim.x + threadIdx.x;
   const int a = gt / LA;
   const int q = gt & (LA - 1);
   const bool live = a < n;
@@ -1264,7 +1264,7 @@ __global__ __launch_bounds__(256) void k_eam_rho(
     const double* __restrict__ fn, const int* __restrict__ map, double rc,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     double* __restrict__ dF, double* __restrict__ Fe) {
-  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gt = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int a = gt / LE;
   const int l = gt & (LE - 1);
   const bool live = a < n;
@@ -1370,8 +1370,8 @@ __global__ __launch_bounds__(256) void k_eam_force(int n_chunks, const double* _
     const double* __restrict__ dF, const double* __restrict__ Fe,
     const int* __restrict__ chunk_rng, double* __restrict__ F, double* __restrict__ partial,
     const double* __restrict__ fast) {
-  for (int c = blockIdx.x; c < n_chunks; c += gridDim.x)
-    k_eam_force_chunk<LE>(c, n_chunks, pos, sid, geoms, single, spc, ns, fn, map, rc, cnt, nbr, stride, dF, Fe, chunk_rng, F, partial, fast);
+  for (int cc = blockIdx.x; cc < n_chunks; cc += gridDim.x)
+    k_eam_force_chunk<LE>(xcd_remap(cc, n_chunks), n_chunks, pos, sid, geoms, single, spc, ns, fn, map, rc, cnt, nbr, stride, dF, Fe, chunk_rng, F, partial, fast);
 }
 
 // ---------------------------------------------------------------------------

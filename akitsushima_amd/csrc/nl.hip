@@ -422,7 +422,7 @@ __global__ __launch_bounds__(64) void k_nl_search(
   extern __shared__ unsigned cand[];  // cmax entries (power of two)
   __shared__ int spre[28];
   __shared__ int scell[27];
-  const int c = blockIdx.x;
+  const int c = xcd_remap(blockIdx.x, gridDim.x);
   const int lane = threadIdx.x;
   const int s = find_struct(cell_offs, S, c);
   const Geom& g = geoms[s];
@@ -540,7 +540,7 @@ __global__ __launch_bounds__(64) void k_nl_search_reg(
   constexpr int NP = 64 * EPL;
   __shared__ int spre[28];
   __shared__ int scell[27];
-  const int c = blockIdx.x;
+  const int c = xcd_remap(blockIdx.x, gridDim.x);  // neighbouring cells on one XCD
   const int lane = threadIdx.x;
   const int s = find_struct(cell_offs, S, c);
   const Geom& g = geoms[s];
