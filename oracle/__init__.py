@@ -18,6 +18,9 @@ after the reference's Python glue:
 * ``tersoff_multi``: TersoffPotential._compute_multi_species, engine/tersoff.py:1518-1951
 * ``sw_flat``: StillingerWeberPotential._compute_single_species,
   engine/stillinger_weber.py:927-1159
+* ``sw_multi``: StillingerWeberPotential._compute_multi_species (tables
+  :1255-1315, kernels _sw_multi_2body_numpy :414-447 and
+  _sw_multi_3body_numpy :476-560)
 * ``zhou_*`` and ``eam_zhou``: engine/eam_zhou.py:47-207 and the numba kernel
   engine/_eam_kernels.py:231-307
 * ``tersoff_loss``: fitting/loss.py:51-248 (squared loss)
@@ -26,10 +29,11 @@ after the reference's Python glue:
 What pins each function (none is left unpinned):
 
 * cell list: brute-force O(N^2) pair sets (tests/test_oracle_derivatives.py)
-* Tersoff flat / multi, SW, Zhou EAM: the reference's ledger energies
+* Tersoff flat / multi, SW flat / multi, Zhou EAM: the reference's ledger energies
   (benchmarks/adr0047.jsonl, tests/golden/), published cohesive energies,
   brute-force double sums, 4th-order finite differences of forces and stress
-  (tests/test_oracle_golden.py, tests/test_oracle_derivatives.py)
+  (tests/test_oracle_golden.py, tests/test_oracle_derivatives.py); SW multi
+  also one-species records == flat, and invariance under atom relabelling
 * tersoff_loss: closed-form normalisation on hand-set residuals, and an
   independent torch-autograd restatement (tests/test_fit_reference.py)
 * mb_velocities, nve_run: zero COM momentum and exact temperature; energy
@@ -76,6 +80,10 @@ def lib():
                                                  _f64p, _f64p, _f64p]
         L.orc_sw_flat.restype = ctypes.c_int
         L.orc_sw_flat.argtypes = common + [_f64p, ctypes.c_int, _f64p, _f64p, _f64p]
+        L.orc_sw_multi.restype = ctypes.c_int
+        L.orc_sw_multi.argtypes = common + [ctypes.c_int32, _i32p, _f64p, _f64p, _f64p,
+                                            ctypes.c_double, ctypes.c_int,
+                                            _f64p, _f64p, _f64p]
         L.orc_eam.restype = ctypes.c_int
         L.orc_eam.argtypes = common + [_i32p, ctypes.c_int32, _i32p, _i32p, _i32p,
                                        _f64p, ctypes.c_double, ctypes.c_int,
@@ -310,6 +318,70 @@ def sw_flat(lattice, positions, params=SW_SI, pairs=None, nthreads=1):
                       len(ci), _p(ci, _i32p), _p(cj, _i32p), _p(prm, _f64p), int(nthreads),
                       ctypes.byref(e), _p(f, _f64p), _p(s, _f64p))
     return e.value, f, s
+
+
+def sw_tables(records, species):
+    """Dense tables of the multi-species SW records
+    (engine/stillinger_weber.py:1255-1315, :874-890): tb (ns, ns, 8) =
+    valid, epsilon, sigma, a, A, B, p, q of (i, j, j) when it exists with
+    A != 0; leg (ns, ns, 3) = sigma, a, gamma of (i, j, j); tri (ns, ns, ns,
+    4) = valid, epsilon, lambda, costheta0 of (i, j, k) when (i, j, k),
+    (i, j, j) and (i, k, k) all exist.  r_cut = max a sigma over records
+    with j == k, else 4 (:860-871)."""
+    ns = len(species)
+    tb = np.zeros((ns, ns, 8))
+    tb[:, :, 2] = 1.0
+    leg = np.zeros((ns, ns, 3))
+    leg[:, :, 0] = 1.0
+    tri = np.zeros((ns, ns, ns, 4))
+    tri[..., 3] = -1.0 / 3.0
+    for a, sa in enumerate(species):
+        for b, sb in enumerate(species):
+            t = records.get((sa, sb, sb))
+            if t is None:
+                continue
+            leg[a, b] = [float(t["sigma"]), float(t["a"]), float(t["gamma"])]
+            if float(t.get("A", 0.0)) != 0.0:
+                tb[a, b] = [1.0, float(t.get("epsilon", 1.0)), float(t["sigma"]),
+                            float(t["a"]), float(t["A"]), float(t.get("B", 0.0)),
+                            float(t.get("p", 4)), float(t.get("q", 0))]
+    for a, sa in enumerate(species):
+        for b, sb in enumerate(species):
+            for c, sc in enumerate(species):
+                t3 = records.get((sa, sb, sc))
+                if (t3 is None or records.get((sa, sb, sb)) is None
+                        or records.get((sa, sc, sc)) is None):
+                    continue
+                tri[a, b, c] = [1.0, float(t3.get("epsilon", 1.0)), float(t3["lambda"]),
+                                float(t3.get("costheta0", -1.0 / 3.0))]
+    rc = 0.0
+    for (i, j, k), v in records.items():
+        if j == k:
+            rc = max(rc, float(v["a"]) * float(v["sigma"]))
+    return tb, leg, tri, (rc if rc > 0 else 4.0)
+
+
+def sw_multi(lattice, positions, types, records, species, pairs=None, nthreads=1):
+    tb, leg, tri, rc = sw_tables(records, species)
+    lattice, positions, ci, cj, inv, vol = _prep(lattice, positions, pairs, rc)
+    idx = {s: k for k, s in enumerate(species)}
+    tid = np.array([idx[t] for t in types], np.int32)
+    n = len(positions)
+    e = ctypes.c_double()
+    f = np.zeros((n, 3))
+    s = np.zeros((3, 3))
+    lib().orc_sw_multi(n, _p(positions, _f64p), _p(lattice, _f64p), _p(inv, _f64p), vol,
+                       len(ci), _p(ci, _i32p), _p(cj, _i32p), len(species), _p(tid, _i32p),
+                       _p(tb, _f64p), _p(leg, _f64p), _p(tri, _f64p), rc, int(nthreads),
+                       ctypes.byref(e), _p(f, _f64p), _p(s, _f64p))
+    return e.value, f, s
+
+
+def sw_flat_as_records(params, sp="Si"):
+    """The single-species flat dict as one (sp, sp, sp) record (the
+    multi-species form of the same potential)."""
+    rec = {k: float(params.get(k, SW_DEFAULTS[k])) for k in SW_KEYS}
+    return {(sp, sp, sp): rec}
 
 
 # ---------------------------------------------------------------------------

@@ -779,6 +779,147 @@ int orc_sw_flat(int32_t n, const double *pos, const double *lat, const double *i
 }
 
 /* ------------------------------------------------------------------------
+ * Stillinger-Weber, multi-species table path:
+ * StillingerWeberPotential._compute_multi_species
+ * (engine/stillinger_weber.py:1165-1451) with _sw_multi_2body_numpy
+ * (:414-447) and _sw_multi_3body_numpy (:476-560), _sw_h3_grad (:96-154).
+ *   tb  (ns, ns, 8): valid, epsilon, sigma, a, A, B, p, q of the two-body
+ *       record (i, j, j), valid when it exists and A != 0;
+ *   leg (ns, ns, 3): sigma, a, gamma of the leg record (i, j, j);
+ *   tri (ns, ns, ns, 4): valid, epsilon, lambda, costheta0 of (i, j, k),
+ *       valid when (i, j, k), (i, j, j) and (i, k, k) exist.
+ * ---------------------------------------------------------------------- */
+static inline void sw_h3_grad_legs(const double *rij, const double *rik, double sig_j,
+                                   double a_j, double gam_j, double sig_k, double a_k,
+                                   double gam_k, double c3, double costh0, double *h,
+                                   double *gj, double *gk)
+{
+    double r_ij = sqrt(rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2]);
+    double r_ik = sqrt(rik[0] * rik[0] + rik[1] * rik[1] + rik[2] * rik[2]);
+    double xij = r_ij / sig_j, xik = r_ik / sig_k;
+    if (xij >= a_j || xik >= a_k) {
+        *h = 0.0;
+        gj[0] = gj[1] = gj[2] = gk[0] = gk[1] = gk[2] = 0.0;
+        return;
+    }
+    double dj = xij - a_j, dk = xik - a_k;
+    double eij = nb_exp(gam_j / dj), eik = nb_exp(gam_k / dk);
+    double cs = (rij[0] * rik[0] + rij[1] * rik[1] + rij[2] * rik[2]) / (r_ij * r_ik);
+    double dc = cs - costh0;
+    *h = c3 * eij * eik * dc * dc;
+    double pre = c3 * eij * eik * dc;
+    double inv_ij = 1.0 / r_ij, inv_ik = 1.0 / r_ik;
+    double uij[3] = {rij[0] * inv_ij, rij[1] * inv_ij, rij[2] * inv_ij};
+    double uik[3] = {rik[0] * inv_ik, rik[1] * inv_ik, rik[2] * inv_ik};
+    double rad_j = pre * dc * (-gam_j / (sig_j * dj * dj));
+    double rad_k = pre * dc * (-gam_k / (sig_k * dk * dk));
+    double ang_j = 2.0 * pre * inv_ij, ang_k = 2.0 * pre * inv_ik;
+    for (int x = 0; x < 3; ++x) {
+        gj[x] = rad_j * uij[x] + ang_j * (uik[x] - cs * uij[x]);
+        gk[x] = rad_k * uik[x] + ang_k * (uij[x] - cs * uik[x]);
+    }
+}
+
+int orc_sw_multi(int32_t n, const double *pos, const double *lat, const double *inv,
+                 double volume, int64_t ncand, const int32_t *ci, const int32_t *cj,
+                 int32_t ns, const int32_t *type_ids, const double *tb, const double *leg,
+                 const double *tri, double rc, int nthreads, double *energy, double *forces,
+                 double *stress)
+{
+    memset(forces, 0, sizeof(double) * 3 * (size_t)n);
+    memset(stress, 0, sizeof(double) * 9);
+    *energy = 0.0;
+    pairs_t pr;
+    pairs_make(&pr, n, pos, lat, inv, ncand, ci, cj, rc);
+    if (pr.P == 0) { pairs_free(&pr); return 0; }
+    int T = orc_threads(nthreads);
+    double e = 0.0, S[9] = {0};
+    /* _sw_multi_2body_numpy: each pair i < j once, record (s_i, s_j) */
+    for (int64_t p = 0; p < pr.P; ++p) {
+        int si = type_ids[pr.ii[p]], sj = type_ids[pr.jj[p]];
+        const double *t2 = tb + 8 * ((size_t)si * ns + sj);
+        if (t2[0] == 0.0) continue;
+        double eps = t2[1], sig = t2[2], a = t2[3], Asw = t2[4], Bsw = t2[5], pp = t2[6],
+               qq = t2[7];
+        double r = pr.r[p], xi = r / sig;
+        if (xi >= a) continue;
+        double t = 1.0 / (xi - a);
+        double ex = nb_exp(t);
+        double V2 = eps * Asw * (Bsw * pow(xi, -pp) - pow(xi, -qq)) * ex;
+        double dV2 = eps * Asw * ex * ((-pp * Bsw * pow(xi, -pp - 1) + qq * pow(xi, -qq - 1)) +
+                                       (Bsw * pow(xi, -pp) - pow(xi, -qq)) * (-t * t));
+        double cx = (dV2 / sig) / r;
+        e += V2;
+        const double *v = pr.rij + 3 * p;
+        for (int x = 0; x < 3; ++x) {
+            forces[3 * (size_t)pr.ii[p] + x] += cx * v[x];
+            forces[3 * (size_t)pr.jj[p] + x] += -cx * v[x];
+        }
+        for (int x = 0; x < 3; ++x)
+            for (int y = 0; y < 3; ++y) S[3 * x + y] -= cx * v[x] * v[y];
+    }
+    /* _sw_multi_3body_numpy: ordered (j, k), each record weighted 1/2 */
+    double *Ft = (double *)calloc((size_t)T * 3 * (size_t)n, sizeof(double));
+    double *St = (double *)calloc((size_t)T * 9, sizeof(double));
+    double *Et = (double *)calloc((size_t)T, sizeof(double));
+#pragma omp parallel num_threads(T)
+    {
+#ifdef _OPENMP
+        int tid = omp_get_thread_num();
+#else
+        int tid = 0;
+#endif
+        double *Fb = Ft + (size_t)tid * 3 * (size_t)n;
+        double *Sb = St + (size_t)tid * 9;
+        double eb = 0.0;
+#pragma omp for schedule(static)
+        for (int32_t i = 0; i < n; ++i) {
+            int64_t s0 = pr.nb_start[i], s1 = pr.nb_start[i + 1];
+            if (s1 - s0 < 2) continue;
+            int isp = type_ids[i];
+            for (int64_t pj = s0; pj < s1; ++pj) {
+                int32_t ja = pr.nb_idx[pj];
+                int jsp = type_ids[ja];
+                const double *vj = pr.nb_rij + 3 * pj;
+                for (int64_t pk = s0; pk < s1; ++pk) {
+                    if (pk == pj) continue;
+                    int32_t ka = pr.nb_idx[pk];
+                    int ksp = type_ids[ka];
+                    const double *t3 = tri + 4 * (((size_t)isp * ns + jsp) * ns + ksp);
+                    if (t3[0] == 0.0) continue;
+                    const double *lj = leg + 3 * ((size_t)isp * ns + jsp);
+                    const double *lk = leg + 3 * ((size_t)isp * ns + ksp);
+                    const double *vk = pr.nb_rij + 3 * pk;
+                    double eps_t = t3[1], lam_t = 0.5 * t3[2], cos_t = t3[3];
+                    double h, gj[3], gk[3];
+                    sw_h3_grad_legs(vj, vk, lj[0], lj[1], lj[2], lk[0], lk[1], lk[2],
+                                    eps_t * lam_t, cos_t, &h, gj, gk);
+                    if (h == 0.0) continue;
+                    eb += h;
+                    for (int x = 0; x < 3; ++x) {
+                        Fb[3 * (size_t)i + x] += gj[x] + gk[x];
+                        Fb[3 * (size_t)ja + x] += -gj[x];
+                        Fb[3 * (size_t)ka + x] += -gk[x];
+                        for (int y = 0; y < 3; ++y) Sb[3 * x + y] += -gj[x] * vj[y] - gk[x] * vk[y];
+                    }
+                }
+            }
+        }
+        Et[tid] = eb;
+    }
+    for (int t = 0; t < T; ++t) {
+        e += Et[t];
+        for (size_t k = 0; k < 3 * (size_t)n; ++k) forces[k] += Ft[(size_t)t * 3 * (size_t)n + k];
+        for (int k = 0; k < 9; ++k) S[k] += St[(size_t)t * 9 + k];
+    }
+    for (int k = 0; k < 9; ++k) stress[k] = S[k] / volume;
+    *energy = e;
+    free(Ft); free(St); free(Et);
+    pairs_free(&pr);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------
  * Zhou-Johnson-Wadley EAM: engine/_eam_kernels.py:148-223 (evaluators) and
  * :231-307 (_eam_kernel, four phases).  A function is a row of ZW doubles:
  * [code, rc, width, chi, pa[20], pb[20]] with code 3 = density,

@@ -27,6 +27,23 @@ TERSOFF_XY = {
                       "h": -0.55, "lambda3": 1.5},
 }
 
+# Tomita et al., Jpn. J. Appl. Phys. 57, 04FB04 (2018), Table I, as
+# benchmarks/bench_force.py:104-119 gives it (epsilon 1, costheta0 -0.333333).
+_TOMITA_KEYS = ("sigma", "a", "lambda", "gamma", "A", "B", "p", "q")
+_TOMITA_ROWS = [
+    ("Ge", "Ge", "Ge", 2.181, 1.8110, 32.101, 1.1718, 10.093, 0.6862, 4.6973, 0.6588),
+    ("Sn", "Sn", "Sn", 2.503, 1.7825, 28.747, 1.1610, 9.4315, 0.7012, 4.9827, 0.6252),
+    ("Ge", "Ge", "Sn", 2.337, 1.7988, 29.858, 1.8136, 0.0, 0.0, 0.0, 0.0),
+    ("Ge", "Sn", "Ge", 2.337, 1.7988, 29.858, 0.7232, 0.0, 0.0, 0.0, 0.0),
+    ("Ge", "Sn", "Sn", 2.337, 1.7988, 24.460, 1.1484, 9.8864, 0.7021, 4.8537, 0.6482),
+    ("Sn", "Ge", "Ge", 2.337, 1.7988, 24.460, 1.1484, 9.8864, 0.7021, 4.8537, 0.6482),
+    ("Sn", "Ge", "Sn", 2.337, 1.7988, 26.074, 1.3564, 0.0, 0.0, 0.0, 0.0),
+    ("Sn", "Sn", "Ge", 2.337, 1.7988, 26.074, 1.0924, 0.0, 0.0, 0.0, 0.0),
+]
+SW_GESN = {tuple(r[:3]): dict(zip(_TOMITA_KEYS, r[3:]), epsilon=1.0, costheta0=-0.333333)
+           for r in _TOMITA_ROWS}
+GESN_SPECIES = ["Ge", "Sn"]
+
 
 def si(n):
     return cubic(DIAMOND, 5.431, n, ["Si"] * 8)
@@ -39,6 +56,17 @@ def ni(n):
 def xy(n):
     lat, pos, _ = si(n)
     return lat, pos, (["X"] * 4 + ["Y"] * 4) * (len(pos) // 8)
+
+
+def gesn_all(cells_diamond=(4, 8)):
+    """The Ge0.875Sn0.125 cells of bench_force.build_cases (:223-229)."""
+    rng = np.random.default_rng(1)
+    out = {}
+    for n in cells_diamond:
+        lat, pos, types = cubic(DIAMOND, 5.762, n, ["Ge"] * 8)
+        out[n] = (lat, pos, [str(x) for x in rng.choice(["Ge", "Sn"], len(types),
+                                                         p=[0.875, 0.125])])
+    return out
 
 
 def cuni_all(cells_diamond=(4, 8), cells_fcc=(5, 10)):

@@ -20,7 +20,8 @@ import pytest
 import oracle as O
 from akitsushima_amd.structures import DIAMOND, FCC, cubic
 
-from _cases import SW_SI, TERSOFF_SI, TERSOFF_XY, cuni_all, rattled, sheared_si, si
+from _cases import (GESN_SPECIES, SW_GESN, SW_SI, TERSOFF_SI, TERSOFF_XY, cuni_all, rattled,
+                    sheared_si, si)
 
 
 # --------------------------------------------------------------------------
@@ -115,6 +116,53 @@ def brute_sw(lat, pos, p):
     return E
 
 
+def brute_sw_multi(lat, pos, types, recs):
+    """Multi-species SW (Tomita et al. 2018 eq. 4, LAMMPS pair_sw): each pair
+    once with the record (s_i, s_j, s_j) of its lower-indexed atom when it
+    has A != 0; each angle j-i-k (j < k) with the mean of the records
+    (i, j, k) and (i, k, j), each leg i-x taking sigma, a, gamma from
+    (s_i, s_x, s_x)."""
+    rc = max(r["a"] * r["sigma"] for (i, j, k), r in recs.items() if j == k)
+    nb = _neighbours(lat, pos, rc)
+    E = 0.0
+    for i, lst in enumerate(nb):
+        si = types[i]
+        for j, v in lst:
+            lo, hi = (i, j) if i < j else (j, i)
+            t = recs.get((types[lo], types[hi], types[hi]))
+            if t is None or t["A"] == 0.0:
+                continue
+            x = np.linalg.norm(v) / t["sigma"]
+            if x < t["a"]:
+                E += 0.5 * t["epsilon"] * t["A"] * (t["B"] * x ** -t["p"] - x ** -t["q"]) \
+                    * math.exp(1 / (x - t["a"]))
+        for (j, vj), (k, vk) in itertools.combinations(lst, 2):
+            lj = recs.get((si, types[j], types[j]))
+            lk = recs.get((si, types[k], types[k]))
+            if lj is None or lk is None:
+                continue
+            xj = np.linalg.norm(vj) / lj["sigma"]
+            xk = np.linalg.norm(vk) / lk["sigma"]
+            if xj >= lj["a"] or xk >= lk["a"]:
+                continue
+            ct = vj @ vk / (np.linalg.norm(vj) * np.linalg.norm(vk))
+            ex = math.exp(lj["gamma"] / (xj - lj["a"]) + lk["gamma"] / (xk - lk["a"]))
+            for t3 in (recs.get((si, types[j], types[k])), recs.get((si, types[k], types[j]))):
+                if t3 is not None:
+                    E += 0.5 * t3["epsilon"] * t3["lambda"] * ex * (ct - t3["costheta0"]) ** 2
+    return E
+
+
+def _gesn_cell(seed, shear=0.06, rattle=0.12):
+    """A sheared, rattled 64-atom Ge-Sn diamond cell with Sn at random."""
+    lat, pos, _ = sheared_si(2, shear=shear, rattle=rattle, seed=seed)
+    lat = lat * (5.762 / 5.431)
+    pos = pos * (5.762 / 5.431)
+    rng = np.random.default_rng(seed + 100)
+    types = [str(x) for x in rng.choice(GESN_SPECIES, len(pos), p=[0.7, 0.3])]
+    return lat, pos, types
+
+
 @pytest.mark.parametrize("m,gamma,l3", [(3.0, 1.0, 0.0), (3.0, 1.0, 1.7322),
                                         (3.0, 1.12, 1.2), (1.0, 0.9, 0.8)])
 def test_tersoff_matches_brute_force(m, gamma, l3):
@@ -134,6 +182,42 @@ def test_sw_matches_brute_force():
     e_bf = brute_sw(lat, pos, SW_SI)
     e, _, _ = O.sw_flat(lat, pos, SW_SI)
     assert abs(e - e_bf) < 1e-10 * abs(e_bf)
+
+
+def test_sw_multi_matches_brute_force():
+    lat, pos, t = _gesn_cell(11)
+    e_bf = brute_sw_multi(lat, pos, t, SW_GESN)
+    e, _, _ = O.sw_multi(lat, pos, t, SW_GESN, GESN_SPECIES)
+    assert abs(e - e_bf) < 1e-10 * abs(e_bf)
+
+
+def test_sw_multi_with_unequal_mixed_records_matches_brute_force():
+    """(i, j, k) and (i, k, j) with different lambda and costheta0: each order
+    weighs one half, so the energy does not depend on neighbour order."""
+    recs = {k: dict(v) for k, v in SW_GESN.items()}
+    recs[("Ge", "Sn", "Ge")]["costheta0"] = -0.25
+    recs[("Sn", "Ge", "Sn")]["epsilon"] = 1.3
+    lat, pos, t = _gesn_cell(12)
+    e_bf = brute_sw_multi(lat, pos, t, recs)
+    e, _, _ = O.sw_multi(lat, pos, t, recs, GESN_SPECIES)
+    assert abs(e - e_bf) < 1e-10 * abs(e_bf)
+
+
+def test_sw_multi_one_species_equals_flat():
+    lat, pos, t = sheared_si(2, shear=0.05, rattle=0.12, seed=13)
+    e1, f1, s1 = O.sw_flat(lat, pos, SW_SI)
+    e2, f2, s2 = O.sw_multi(lat, pos, t, O.sw_flat_as_records(SW_SI), ["Si"])
+    assert abs(e1 - e2) < 1e-13 * abs(e1)
+    assert np.abs(f1 - f2).max() < 1e-12 and np.abs(s1 - s2).max() < 1e-12
+
+
+def test_sw_multi_is_invariant_under_atom_relabelling():
+    lat, pos, t = _gesn_cell(14)
+    perm = np.random.default_rng(3).permutation(len(pos))
+    e1, f1, s1 = O.sw_multi(lat, pos, t, SW_GESN, GESN_SPECIES)
+    e2, f2, s2 = O.sw_multi(lat, pos[perm], [t[k] for k in perm], SW_GESN, GESN_SPECIES)
+    assert abs(e1 - e2) < 1e-12 * abs(e1)
+    assert np.abs(f1[perm] - f2).max() < 1e-11 and np.abs(s1 - s2).max() < 1e-12
 
 
 # --------------------------------------------------------------------------
@@ -222,6 +306,15 @@ def test_sw_forces_stress_fd():
     _fd_check(lambda q: O.sw_flat(lat, q, SW_SI, pairs=pr)[0],
               lambda l, q: O.sw_flat(l, q, SW_SI, pairs=pr)[0],
               lat, pos, f, s, 2e-8, 1e-9, atoms=range(3, 64, 9))
+
+
+def test_sw_multi_forces_stress_fd():
+    lat, pos, t = _gesn_cell(15)
+    pr = _pairs_all(len(pos))
+    e, f, s = O.sw_multi(lat, pos, t, SW_GESN, GESN_SPECIES, pairs=pr)
+    _fd_check(lambda q: O.sw_multi(lat, q, t, SW_GESN, GESN_SPECIES, pairs=pr)[0],
+              lambda l, q: O.sw_multi(l, q, t, SW_GESN, GESN_SPECIES, pairs=pr)[0],
+              lat, pos, f, s, 2e-8, 1e-9, atoms=range(4, 64, 9))
 
 
 def test_eam_cuni_forces_stress_fd():

@@ -2,9 +2,9 @@
 
 * Ledger energies per atom of the reference's force benchmark
   (benchmarks/adr0047.jsonl, single-thread rows; geometry of
-  benchmarks/bench_force.py:152-158), for all five families: Tersoff Si,
-  two-species Tersoff X-Y, SW Si, Zhou EAM Ni and Cu-Ni, including the
-  Zhou cutoffs the ledger records.
+  benchmarks/bench_force.py:152-158), for all six families: Tersoff Si,
+  two-species Tersoff X-Y, SW Si, SW Ge-Sn (Tomita), Zhou EAM Ni and Cu-Ni,
+  including the SW Ge-Sn and Zhou cutoffs the ledger records.
 * Published values: Tersoff T3 Si a0 = 5.432 A, E_coh = 4.63 eV/atom; SW Si
   E = -2 epsilon per atom in perfect diamond (tests/test_engine_forces.py:
   1274-1338 of the reference); all forces zero in perfect diamond.
@@ -15,7 +15,8 @@ import pytest
 import oracle as O
 from akitsushima_amd.structures import DIAMOND, cubic
 
-from _cases import LEDGER, SW_SI, TERSOFF_SI, TERSOFF_XY, cuni_all, ni, si, xy
+from _cases import (GESN_SPECIES, LEDGER, SW_GESN, SW_SI, TERSOFF_SI, TERSOFF_XY, cuni_all,
+                    gesn_all, ni, si, xy)
 
 TOL = 1e-12
 
@@ -43,6 +44,17 @@ def test_sw_si_ledger(n):
     e, f, s = O.sw_flat(lat, pos, SW_SI)
     ref = LEDGER[f"sw_si/{len(pos)}"]
     assert abs(e / len(pos) - ref["e_per_atom"]) < TOL * abs(ref["e_per_atom"])
+
+
+@pytest.mark.parametrize("n", [4, 8])
+def test_sw_gesn_ledger(n):
+    lat, pos, t = gesn_all()[n]
+    ref = LEDGER[f"sw_gesn_tomita/{len(pos)}"]
+    _, _, _, rc = O.sw_tables(SW_GESN, GESN_SPECIES)
+    assert rc == ref["r_cut"]
+    e, f, s = O.sw_multi(lat, pos, t, SW_GESN, GESN_SPECIES, nthreads=4)
+    assert abs(e / len(pos) - ref["e_per_atom"]) < TOL * abs(ref["e_per_atom"])
+    assert np.abs(f.sum(axis=0)).max() < 1e-12
 
 
 @pytest.mark.parametrize("n", [5, 10])
