@@ -41,6 +41,7 @@ SIGNATURES = {
     "ak_device_count": (ctypes.c_int, [_pi32]),
     "ak_tersoff_create": (ctypes.c_int, [_i32, _pf64, _pf64, _f64, _phandle]),
     "ak_sw_create": (ctypes.c_int, [_pf64, _phandle]),
+    "ak_sw_multi_create": (ctypes.c_int, [_i32, _pf64, _pf64, _pf64, _f64, _phandle]),
     "ak_eam_create": (ctypes.c_int, [_i32, _i32, _pf64, _pi32, _f64, _phandle]),
     "ak_potential_destroy": (ctypes.c_int, [_handle]),
     "ak_potential_set_lanes": (ctypes.c_int, [_handle, _i32]),

@@ -247,6 +247,42 @@ int ak_sw_create(const double* prm, ak_potential** out) {
                         {});
 }
 
+int ak_sw_multi_create(int32_t ns, const double* pair, const double* leg, const double* tri,
+                       double r_cut, ak_potential** out) {
+  if (ns < 1 || ns > 8 || !pair || !leg || !tri || !(r_cut > 0.0))
+    return fail(AK_ERR_ARG, "ak_sw_multi_create: bad arguments");
+  const size_t n2 = (size_t)ns * ns, n3 = n2 * ns;
+  // device layout: pair (ns, ns, 7) with a = 0 where there is no two-body
+  // term, leg (ns, ns, 3) with a = 0 where the record is absent, triple
+  // (ns, ns, ns, 2) = epsilon lambda / 2 (0 where absent), cos0
+  std::vector<double> t(7 * n2 + 3 * n2 + 2 * n3, 0.0);
+  for (size_t p = 0; p < n2; ++p) {
+    const double* s = pair + 8 * p;
+    double* d = t.data() + 7 * p;
+    const bool valid = s[0] != 0.0;
+    if (valid && !(s[2] > 0.0)) return fail(AK_ERR_ARG, "ak_sw_multi_create: pair sigma > 0");
+    d[0] = s[1];
+    d[1] = valid ? s[2] : 1.0;
+    d[2] = valid ? s[3] : 0.0;
+    for (int k = 3; k < 7; ++k) d[k] = s[k + 1];
+    const double* l = leg + 3 * p;
+    double* dl = t.data() + 7 * n2 + 3 * p;
+    if (l[1] > 0.0 && !(l[0] > 0.0)) return fail(AK_ERR_ARG, "ak_sw_multi_create: leg sigma > 0");
+    dl[0] = l[1] > 0.0 ? l[0] : 1.0;
+    dl[1] = l[1] > 0.0 ? l[1] : 0.0;
+    dl[2] = l[2];
+  }
+  for (size_t q = 0; q < n3; ++q) {
+    const double* s = tri + 4 * q;
+    double* d = t.data() + 10 * n2 + 2 * q;
+    d[0] = s[0] != 0.0 ? s[1] * (0.5 * s[2]) : 0.0;
+    d[1] = s[3];
+  }
+  int rc = make_potential(out, KIND_SW, ns, r_cut, std::move(t), {});
+  if (rc == AK_OK) (*out)->pot.sw_multi = true;
+  return rc;
+}
+
 int ak_eam_create(int32_t ns, int32_t nf, const double* fn, const int32_t* map, double r_cut,
                   ak_potential** out) {
   if (ns < 1 || ns > 16 || nf < 1 || !fn || !map || !(r_cut > 0.0))

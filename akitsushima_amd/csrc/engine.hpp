@@ -143,6 +143,8 @@ struct Potential {
   std::vector<int> h_map;
   DevBuf d_map, d_eam_fast;
   bool eam_fast = false;
+  // SW with multi-species tables (pair, leg and triple records) in d_params
+  bool sw_multi = false;
   // per-call scratch; amask (n) = bonds inside r_c per atom (rows >= 8)
   DevBuf G, aux, out, amask;
   // launch shape chosen by the autotuner (0 = default)

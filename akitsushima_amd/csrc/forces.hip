@@ -1085,6 +1085,200 @@ __global__ __launch_bounds__(256) void k_gather_c(int n, const int* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Stillinger-Weber, multi-species tables (Tomita et al., Jpn. J. Appl. Phys.
+// 57, 04FB04 (2018), eq. 4; StillingerWeberPotential._compute_multi_species,
+// engine/stillinger_weber.py:1165-1451, kernels :414-560):
+//   E_a = 1/2 sum_j V2_{p(a,j)}(r_aj)
+//       + sum_{j != k} c_(s_a,s_j,s_k) E_aj E_ak (cos - cos0_(s_a,s_j,s_k))^2,
+// with the pair record p(a, j) = (s_lo, s_hi) of the lower-indexed atom, the
+// leg factor E_aj = exp(gamma / (r / sigma - a)) of the leg record
+// (s_a, s_j), and c = epsilon lambda / 2 of the triple record (each ordered
+// partner pair is one term).  The lane of leg j takes the terms (j, k) for
+// its energy and both (j, k) and (k, j) for G_aj.
+// Tables (d_params): pair (ns, ns, 7) = epsilon, sigma, a, A, B, p, q with
+// a = 0 where there is no two-body term; leg (ns, ns, 3) = sigma, a, gamma
+// with a = 0 where the record is absent; triple (ns, ns, ns, 2) = c, cos0
+// with c = 0 where the term is absent.  The triple table is staged in LDS.
+// Same two phases as k_sw_site_c: LA lanes per atom, legs compacted in slot
+// order into LDS; dynamic LDS = legs (6 doubles + 3 ints) + triple table.
+// ---------------------------------------------------------------------------
+static inline size_t sw_m_lds(int blk, int L, int LA, int ns) {
+  const size_t legs = (size_t)(blk / LA) * L;
+  return legs * 6 * sizeof(double) + (legs * 3 * sizeof(int) + 7) / 8 * 8 +
+         sizeof(double) * 2 * (size_t)ns * ns * ns;
+}
+
+template <int L, int LA>
+__device__ __forceinline__ void k_sw_multi_chunk(
+    int c, int n_chunks, int n, const double* __restrict__ pos, const int* __restrict__ sid,
+    const Geom* __restrict__ geoms, int single, const int* __restrict__ spc, int ns,
+    const double* __restrict__ tpair, const double* __restrict__ tleg, double rc,
+    const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
+    const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
+    uint64_t* __restrict__ amask, double* __restrict__ lds, const double* __restrict__ s_tri) {
+  static_assert(L >= LA && L % LA == 0 && LA <= 64, "LA lanes cover the row in L / LA rounds");
+  const int apb = blockDim.x / LA;
+  const int al = (int)threadIdx.x / LA;
+  const int q = threadIdx.x & (LA - 1);
+  const int seg = (threadIdx.x & 63) & ~(LA - 1);
+  double* lx = lds;
+  double* ly = lx + apb * L;
+  double* lz = ly + apb * L;
+  double* lr = lz + apb * L;
+  double* lE = lr + apb * L;
+  double* lD = lE + apb * L;
+  int* lslot = (int*)(lD + apb * L);
+  int* lsp = lslot + apb * L;
+  int* lpr = lsp + apb * L;
+  const int reg = al * L;
+  const int a = chunk_rng[2 * c] + al;
+  const bool live = a < chunk_rng[2 * c + 1];
+  const int ac = live ? a : 0;
+  const int nk = live ? cnt[ac] : 0;
+  const int si = spc[ac];
+  const Geom& g = geoms[single ? 0 : sid[ac]];
+
+  // phase 1: legs inside a two-body or a three-body cutoff, in slot order
+  int count = 0;
+  uint64_t mask = 0;
+#pragma unroll
+  for (int i = 0; i < L / LA; ++i) {
+    const int k = q + LA * i;
+    const bool has = live && k < nk;
+    double ux = 0.0, uy = 0.0, uz = 0.0, r = 0.0, E = 0.0, dlnE = 0.0;
+    int sj = 0, pr = -1;
+    bool act = false;
+    if (has) {
+      const int j = nbr[(size_t)ac * stride + k];
+      mic_fast(g, pos, ac, j, ux, uy, uz);
+      r = sqrt(ux * ux + uy * uy + uz * uz);
+      sj = spc[j];
+      if (r < rc && r > 1e-12) {
+        const int p = ac < j ? si * ns + sj : sj * ns + si;
+        const double* tp = tpair + 7 * (size_t)p;
+        if (r / tp[1] < tp[2]) pr = p;
+        const double* tl = tleg + 3 * ((size_t)si * ns + sj);
+        const double xl = r / tl[0];
+        if (xl < tl[1]) {
+          const double d = xl - tl[1];
+          E = nb_exp(tl[2] / d);
+          dlnE = -tl[2] / (tl[0] * d * d);
+        }
+        act = pr >= 0 || E != 0.0;
+      }
+    }
+    const uint64_t bal = __ballot(act);
+    const uint64_t sm = LA == 64 ? bal : (bal >> seg) & ((uint64_t(1) << (LA & 63)) - 1);
+    if (act) {
+      const int t = reg + count + __popcll(sm & ((uint64_t(1) << q) - 1));
+      lx[t] = ux;
+      ly[t] = uy;
+      lz[t] = uz;
+      lr[t] = r;
+      lE[t] = E;
+      lD[t] = dlnE;
+      lslot[t] = k;
+      lsp[t] = sj;
+      lpr[t] = pr;
+    }
+    mask |= sm << (LA * i);
+    count += __popcll(sm);
+  }
+  if (live && q == 0) amask[ac] = mask;
+  __syncthreads();
+
+  double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int t = q; t < count; t += LA) {
+    const double ux = lx[reg + t], uy = ly[reg + t], uz = lz[reg + t], r = lr[reg + t];
+    const double E = lE[reg + t], dlnE = lD[reg + t];
+    const int sj = lsp[reg + t], pr = lpr[reg + t];
+    const double rinv = 1.0 / r;
+    double e = 0.0, su = 0.0;
+    if (pr >= 0) {
+      const double* tp = tpair + 7 * (size_t)pr;
+      const double eps = tp[0], sig = tp[1], asw = tp[2], Asw = tp[3], Bsw = tp[4], pp = tp[5],
+                   qq = tp[6];
+      const double x = r / sig;
+      const double tt = 1.0 / (x - asw);
+      const double ex = nb_exp(tt);
+      const double bx = Bsw * xpow(x, -pp) - xpow(x, -qq);
+      const double V2 = eps * Asw * bx * ex;
+      const double dV2 = eps * Asw * ex *
+                         ((-pp * Bsw * xpow(x, -pp - 1.0) + qq * xpow(x, -qq - 1.0)) +
+                          bx * (-tt * tt));
+      e = 0.5 * V2;
+      su = 0.5 * (dV2 / sig) * rinv;
+    }
+    double swx = 0.0, swy = 0.0, swz = 0.0;
+    if (E != 0.0) {
+      const double* row1 = s_tri + 2 * (size_t)((si * ns + sj) * ns);  // (s_a, s_j, s_k)
+      const double* col2 = s_tri + 2 * (size_t)(si * ns * ns + sj);    // (s_a, s_k, s_j)
+      for (int s2 = 0; s2 < count; ++s2) {
+        const double wE = lE[reg + s2];
+        if (s2 == t || wE == 0.0) continue;
+        const int sk = lsp[reg + s2];
+        const double wx = lx[reg + s2], wy = ly[reg + s2], wz = lz[reg + s2];
+        const double wri = 1.0 / lr[reg + s2];
+        const double c1 = row1[2 * sk], q1 = row1[2 * sk + 1];
+        const double c2 = col2[2 * ns * sk], q2 = col2[2 * ns * sk + 1];
+        const double cs = (ux * wx + uy * wy + uz * wz) * rinv * wri;
+        const double d1 = cs - q1, d2 = cs - q2;
+        const double EE = E * wE;
+        e += c1 * EE * d1 * d1;
+        // dh/du over both orders: E E [(c1 d1^2 + c2 d2^2) dlnE u_hat
+        //                              + 2 (c1 d1 + c2 d2) (w_hat - cos u_hat) / r]
+        const double rad = EE * (c1 * d1 * d1 + c2 * d2 * d2) * dlnE;
+        const double ang = 2.0 * EE * (c1 * d1 + c2 * d2) * rinv;
+        su += (rad - ang * cs) * rinv;
+        const double aw = ang * wri;
+        swx += aw * wx;
+        swy += aw * wy;
+        swz += aw * wz;
+      }
+    }
+    const double Gx = su * ux + swx, Gy = su * uy + swy, Gz = su * uz + swz;
+    const size_t slot = (size_t)ac * stride + lslot[reg + t];
+    const size_t plane = (size_t)n * stride;
+    G[slot] = Gx;
+    G[plane + slot] = Gy;
+    G[2 * plane + slot] = Gz;
+    v[0] += e;
+    v[1] -= Gx * ux;
+    v[2] -= Gx * uy;
+    v[3] -= Gx * uz;
+    v[4] -= Gy * ux;
+    v[5] -= Gy * uy;
+    v[6] -= Gy * uz;
+    v[7] -= Gz * ux;
+    v[8] -= Gz * uy;
+    v[9] -= Gz * uz;
+  }
+  wave_partial(v, partial, c, n_chunks);
+}
+
+template <int L, int LA>
+__global__ __launch_bounds__(256) void k_sw_multi(
+    int n_chunks, int n, const double* __restrict__ pos, const int* __restrict__ sid,
+    const Geom* __restrict__ geoms, int single, const int* __restrict__ spc, int ns,
+    const double* __restrict__ tables, double rc, const int* __restrict__ cnt,
+    const int* __restrict__ nbr, int stride, const int* __restrict__ chunk_rng,
+    double* __restrict__ G, double* __restrict__ partial, uint64_t* __restrict__ amask) {
+  extern __shared__ double lds_swm[];
+  const int n2 = ns * ns;
+  const double* tpair = tables;
+  const double* tleg = tpair + 7 * (size_t)n2;
+  const double* ttri = tleg + 3 * (size_t)n2;
+  const int apb = blockDim.x / LA;
+  double* s_tri = lds_swm + (size_t)apb * L * 6 + ((size_t)apb * L * 3 * sizeof(int) + 7) / 8;
+  for (int i = threadIdx.x; i < 2 * n2 * ns; i += blockDim.x) s_tri[i] = ttri[i];
+  __syncthreads();
+  for (int cc = blockIdx.x; cc < n_chunks; cc += gridDim.x)
+    k_sw_multi_chunk<L, LA>(xcd_remap(cc, n_chunks), n_chunks, n, pos, sid, geoms, single, spc,
+                            ns, tpair, tleg, rc, cnt, nbr, stride, chunk_rng, G, partial, amask,
+                            lds_swm, s_tri);
+}
+
+// ---------------------------------------------------------------------------
 // Zhou EAM.  A function is a row of 44 doubles [code, r_c, width, chi,
 // pa[20], pb[20]]: code 3 density, 4 pair, 5 cross pair, 6 embedding;
 // element parameters in the order r_e, f_e, rho_e, rho_s, alpha, beta, A, B,
@@ -1439,6 +1633,12 @@ static void launch_bond_kernels(Potential& pot, NList& nl, SetDev& set, const do
       launch_timed(pot, sp1 ? k_tersoff_site<L, true> : k_tersoff_site<L, false>, grid, block, st,
                    (size_t)0, ch.n, n, d_pos, sid, geoms, single, d_spc, pot.ns, tb, tri, pot.r_cut, cnt,
                    nbr, nl.stride, rng, G, part, amask);
+  } else if (pot.sw_multi) {
+    constexpr int LA = L < kRowLanes ? L : kRowLanes;
+    launch_timed(pot, k_sw_multi<L, LA>, grid, block, st, sw_m_lds(blk, L, LA, pot.ns), ch.n, n,
+                 d_pos, sid, geoms, single, d_spc, pot.ns,
+                 (const double*)pot.d_params.as<double>(), pot.r_cut, cnt, nbr, nl.stride, rng, G,
+                 part, amask);
   } else if constexpr (L >= kLongRow) {
     const size_t lds = sw_c_lds(blk, L);
     launch_timed(pot, k_sw_site_c<L, kRowLanes>, grid, block, st, lds, ch.n, n, d_pos, sid,
@@ -1451,7 +1651,13 @@ static void launch_bond_kernels(Potential& pot, NList& nl, SetDev& set, const do
   }
   const int* rev = nl.rev.as<int>();
   const int* first = ch.first.as<int>();
-  if constexpr (L >= kLongRow) {
+  if (pot.sw_multi) {
+    constexpr int LA = L < kRowLanes ? L : kRowLanes;
+    const int atom_blocks = (int)(((int64_t)n * LA + 255) / 256);
+    hipLaunchKernelGGL((k_gather_c<L, LA>), dim3(atom_blocks + set.n_struct), dim3(256), 0, st,
+                       n, cnt, rev, nl.stride, (const double*)G, d_F, atom_blocks, geoms, first,
+                       ch.n, (const double*)part, d_out, (const uint64_t*)amask);
+  } else if constexpr (L >= kLongRow) {
     const int atom_blocks = (int)(((int64_t)n * kRowLanes + 255) / 256);
     hipLaunchKernelGGL((k_gather_c<L, kRowLanes>), dim3(atom_blocks + set.n_struct), dim3(256),
                        0, st, n, cnt, rev, nl.stride, (const double*)G, d_F, atom_blocks, geoms,
@@ -1555,12 +1761,15 @@ void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
   // lanes per atom of the site kernel: L, or L * L for the Tersoff form with
   // one lane per (bond, partner) pair
   const bool trip = pot.kind == KIND_TERSOFF && lanes * lanes <= 64 && pot.bond_tile == 2;
-  const bool sw_long = pot.kind == KIND_SW && lanes >= kLongRow;
-  const int atom_lanes = trip ? lanes * lanes : (sw_long ? kRowLanes : lanes);
+  const bool swm = pot.kind == KIND_SW && pot.sw_multi;
+  const bool sw_long = pot.kind == KIND_SW && !swm && lanes >= kLongRow;
+  const int atom_lanes = trip ? lanes * lanes
+                              : (sw_long || (swm && lanes > kRowLanes) ? kRowLanes : lanes);
   int blk = pot.block;
-  // the long-row SW kernel stages every leg of its atoms in LDS: keep it
-  // within 48 KiB per block
+  // the long-row and multi-species SW kernels stage every leg of their atoms
+  // in LDS: keep it within 48 KiB per block
   while (sw_long && blk > 64 && sw_c_lds(blk, lanes) > 48 * 1024) blk >>= 1;
+  while (swm && blk > 64 && sw_m_lds(blk, lanes, atom_lanes, pot.ns) > 48 * 1024) blk >>= 1;
   AK_REQUIRE(atom_lanes <= blk, AK_ERR_STATE, "block smaller than one atom's lanes");
   SetDev::Chunks& ch = set.chunks(blk / atom_lanes, st);
   if (n > 0) {

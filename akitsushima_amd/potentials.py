@@ -275,31 +275,133 @@ SW_DEFAULTS = {"epsilon": 2.1682, "sigma": 2.0951, "A": 7.049556, "B": 0.6022246
                "costheta0": -1.0 / 3.0}
 
 
+def _effective_two_body(rec):
+    """The two-body inputs a record supplies, or None when it supplies none
+    (absent, or A == 0), engine/stillinger_weber.py:700-710."""
+    if rec is None or float(rec.get("A", 0.0)) == 0.0:
+        return None
+    return {"epsilon": float(rec.get("epsilon", 1.0)), "sigma": float(rec["sigma"]),
+            "a": float(rec["a"]), "A": float(rec["A"]), "B": float(rec.get("B", 0.0)),
+            "p": float(rec.get("p", 4)), "q": float(rec.get("q", 0))}
+
+
+def check_two_body_symmetry(params):
+    """Refuse records whose two-body term depends on atom order
+    (engine/stillinger_weber.py:713-762): a pair is evaluated from the
+    record (s_i, s_j, s_j) of its lower-indexed atom, so (si, sj, sj) and
+    (sj, si, si) must supply the same two-body parameters, or both none."""
+    species = sorted({e for t in params for e in t})
+    for x, si in enumerate(species):
+        for sj in species[x + 1:]:
+            r1 = _effective_two_body(params.get((si, sj, sj)))
+            r2 = _effective_two_body(params.get((sj, si, si)))
+            if r1 is None and r2 is None:
+                continue
+            if r1 is not None and r2 is not None and all(
+                    math.isclose(r1[k], r2[k], rel_tol=1e-12, abs_tol=0.0) for k in r1):
+                continue
+            if r1 is None or r2 is None:
+                have, miss = (((si, sj, sj), (sj, si, si)) if r2 is None
+                              else ((sj, si, si), (si, sj, sj)))
+                raise ValueError(
+                    f"StillingerWeberPotential: {have} supplies a two-body {si}-{sj} term "
+                    f"but {miss} does not (absent, or A = 0). The pair is evaluated from "
+                    f"whichever of the two belongs to the lower-indexed atom, so the energy "
+                    f"would depend on atom order.  Give {miss} the same two-body parameters "
+                    f"(epsilon, sigma, a, A, B, p, q).")
+            diff = [k for k in r1 if not math.isclose(r1[k], r2[k], rel_tol=1e-12, abs_tol=0.0)]
+            raise ValueError(
+                f"StillingerWeberPotential: ({si},{sj},{sj}) and ({sj},{si},{si}) disagree on "
+                f"the {si}-{sj} two-body parameter(s) {diff}: "
+                + ", ".join(f"{k} {r1[k]} vs {r2[k]}" for k in diff)
+                + ".  The pair is evaluated from whichever record belongs to the "
+                "lower-indexed atom, so the energy would depend on atom order.  Their "
+                "lambda/gamma/costheta0 may differ; these may not.")
+
+
+def sw_tables(records, species):
+    """Dense tables of the multi-species records, as
+    engine/stillinger_weber.py:1255-1315 resolves them: pair (ns, ns, 8) =
+    valid, epsilon, sigma, a, A, B, p, q of (i, j, j) when it exists with
+    A != 0; leg (ns, ns, 3) = sigma, a, gamma of (i, j, j) (a = 0 when
+    absent); triple (ns, ns, ns, 4) = valid, epsilon, lambda, costheta0 of
+    (i, j, k) when (i, j, k), (i, j, j) and (i, k, k) exist."""
+    ns = len(species)
+    tb = np.zeros((ns, ns, 8))
+    tb[:, :, 2] = 1.0
+    leg = np.zeros((ns, ns, 3))
+    leg[:, :, 0] = 1.0
+    tri = np.zeros((ns, ns, ns, 4))
+    tri[..., 3] = -1.0 / 3.0
+    for a, sa in enumerate(species):
+        for b, sb in enumerate(species):
+            t = records.get((sa, sb, sb))
+            if t is None:
+                continue
+            leg[a, b] = [float(t["sigma"]), float(t["a"]), float(t["gamma"])]
+            two = _effective_two_body(t)
+            if two is not None:
+                tb[a, b] = [1.0, two["epsilon"], two["sigma"], two["a"], two["A"], two["B"],
+                            two["p"], two["q"]]
+    for a, sa in enumerate(species):
+        for b, sb in enumerate(species):
+            for c, sc in enumerate(species):
+                t3 = records.get((sa, sb, sc))
+                if (t3 is None or records.get((sa, sb, sb)) is None
+                        or records.get((sa, sc, sc)) is None):
+                    continue
+                tri[a, b, c] = [1.0, float(t3.get("epsilon", 1.0)), float(t3["lambda"]),
+                                float(t3.get("costheta0", -1.0 / 3.0))]
+    return tb, leg, tri
+
+
 class StillingerWeberPotential(_GpuPotential):
-    """Stillinger-Weber (engine/stillinger_weber.py:765-1159), flat
-    single-species form.  Multi-species record tables are not built here."""
+    """Stillinger-Weber (engine/stillinger_weber.py:765-1451): the flat
+    single-species dict, or records keyed by ordered triples (i, j, k) as
+    LAMMPS pair_sw and Tomita et al. (2018) write them, with ``species``."""
 
     def __init__(self, params, r_cut=None, species=None):
         super().__init__()
+        self.species = list(species) if species else None
         flat = ("epsilon" in params or "sigma" in params) and not any(
             isinstance(k, tuple) for k in params)
-        if not flat and params:
-            raise NotImplementedError(
-                "StillingerWeberPotential: the multi-species record form is not part of "
-                "this engine; pass the flat single-species parameters")
-        sp0 = species[0] if species else "__X__"
-        self.species = [sp0]
-        self._multi = False
-        self.p = dict(params)
-        self._prm = np.array([float(params.get(k, SW_DEFAULTS[k])) for k in SW_KEYS])
-        rc_nat = self._prm[6] * self._prm[1]
-        if r_cut is not None and abs(float(r_cut) - rc_nat) > 1e-12:
-            raise NotImplementedError("StillingerWeberPotential: r_cut other than a*sigma")
-        self.r_cut = float(rc_nat)
+        self.p = params
+        if flat:
+            sp0 = self.species[0] if self.species else "__X__"
+            self.species = [sp0]
+            self._multi = False
+            self._prm = np.array([float(params.get(k, SW_DEFAULTS[k])) for k in SW_KEYS])
+            rc_nat = self._prm[6] * self._prm[1]
+            if r_cut is not None and abs(float(r_cut) - rc_nat) > 1e-12:
+                raise NotImplementedError("StillingerWeberPotential: r_cut other than a*sigma")
+            self.r_cut = float(rc_nat)
+        else:
+            if not self.species:
+                raise ValueError("StillingerWeberPotential: 'species' is required when using "
+                                 "the multi-species (nested-triple) parameter format.")
+            if len(self.species) > 8:
+                raise NotImplementedError("StillingerWeberPotential: at most 8 species")
+            self._multi = True
+            check_two_body_symmetry(params)
+            rc_max = 0.0
+            for (i, j, k), v in params.items():
+                if j == k:
+                    rc_max = max(rc_max, float(v["a"]) * float(v["sigma"]))
+            self.r_cut = rc_max if rc_max > 0 else 4.0
+            tb, leg, tri = sw_tables(params, self.species)
+            self._tb = np.ascontiguousarray(tb)
+            self._leg = np.ascontiguousarray(leg)
+            self._tri = np.ascontiguousarray(tri)
 
     def _create(self):
         h = ctypes.c_void_p()
-        check(lib().ak_sw_create(self._prm.ctypes.data_as(_lib._pf64), ctypes.byref(h)))
+        if self._multi:
+            check(lib().ak_sw_multi_create(len(self.species), self._tb.ctypes.data_as(_lib._pf64),
+                                           self._leg.ctypes.data_as(_lib._pf64),
+                                           self._tri.ctypes.data_as(_lib._pf64),
+                                           float(self.r_cut), ctypes.byref(h)))
+        else:
+            check(lib().ak_sw_create(self._prm.ctypes.data_as(_lib._pf64), ctypes.byref(h)))
         return h
 
 
@@ -411,7 +513,7 @@ def force_fn_from_config(cfg):
     elif kind in ("sw", "stillingerweber"):
         params = cfg.get("params", {})
         if isinstance(params, list):
-            raise NotImplementedError("multi-species SW records are not part of this engine")
+            params = _records_from_list(params)
         pot = StillingerWeberPotential(params, species=species)
     elif kind == "eam":
         from .published import ZHOU_2004

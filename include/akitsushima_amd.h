@@ -60,6 +60,22 @@ int ak_tersoff_create(int32_t n_species, const double *pair_table,
  * a, lambda, gamma, costheta0; r_cut = a * sigma. */
 int ak_sw_create(const double *params10, ak_potential **out);
 
+/* Stillinger-Weber, multi-species tables of StillingerWeberPotential
+ * (engine/stillinger_weber.py:1255-1315, _compute_multi_species :1165-1451;
+ * records as :874-890, Tomita et al. 2018 eq. 4), n_species <= 8:
+ *   pair_table (ns, ns, 8): valid, epsilon, sigma, a, A, B, p, q of the
+ *                record (i, j, j), valid when it exists with A != 0; a pair
+ *                of atoms reads the row of its lower-indexed atom;
+ *   leg_table (ns, ns, 3): sigma, a, gamma of the record (i, j, j), a = 0
+ *                where that record is absent;
+ *   triple_table (ns, ns, ns, 4): valid, epsilon, lambda, costheta0 of the
+ *                record (i, j, k), valid when (i, j, k), (i, j, j) and
+ *                (i, k, k) exist; both orders of a neighbour pair are terms,
+ *                each weighted 1/2;
+ *   r_cut: max a * sigma over the records (i, j, j). */
+int ak_sw_multi_create(int32_t n_species, const double *pair_table, const double *leg_table,
+                       const double *triple_table, double r_cut, ak_potential **out);
+
 /* Alloy EAM with Zhou-Johnson-Wadley functions (EAMPotential built by
  * zhou_eam, engine/eam_zhou.py:128-207, evaluated as the numba kernel
  * engine/_eam_kernels.py:231-307).  functions: (n_functions, 44) rows

@@ -80,4 +80,4 @@ def test_autograd_function_backward_is_engine_gradient():
     L.backward()
     _, g = fit.loss_and_grad(theta)
     assert np.array_equal(th.grad.numpy(), g)
-    assert abs(float(L) - fit.loss(theta)) == 0.0
+    assert abs(float(L.detach()) - fit.loss(theta)) == 0.0
