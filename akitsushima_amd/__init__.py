@@ -17,6 +17,7 @@ from .potentials import (
     call_force_fn,
     force_fn_from_config,
     make_force_fn,
+    tersoff1989_to_triplets,
 )
 from .elastic import elastic_constants
 from .phonon import force_constants, gamma_frequencies
@@ -34,6 +35,7 @@ __all__ = [
     "call_force_fn",
     "force_fn_from_config",
     "zhou_eam",
+    "tersoff1989_to_triplets",
     "elastic_constants",
     "force_constants",
     "gamma_frequencies",

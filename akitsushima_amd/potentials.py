@@ -220,6 +220,59 @@ def tersoff_tables(records, species):
     return tb, tri, (rc if rc > 0 else 3.0)
 
 
+TERSOFF1989_ELEMENT_KEYS = ("A", "B", "lambda", "mu", "beta", "n", "c", "d", "h", "R", "S")
+
+
+def _pair_value(a, b, table, default):
+    for k in (f"{a}-{b}", f"{b}-{a}"):
+        if k in table:
+            return float(table[k])
+    return default
+
+
+def tersoff1989_to_triplets(elements, chi, species=None, omega=None):
+    """(i, j, k) records of the multicomponent Tersoff form (Tersoff, PRB
+    39, 5566 (1989), eqs. 1a-1e), as engine/tersoff.py:1968-2034 expands
+    it: A_ij, B_ij, R_ij, S_ij geometric means, lambda_ij, mu_ij arithmetic
+    means, chi_ij folded into B_ij (chi_ii = 1), beta, n, c, d, h of the
+    centre i, lambda3 = 0, m = 3, gamma = omega_ik (default 1); the cutoff
+    in the engine's form R = (R_ij + S_ij) / 2, D = (S_ij - R_ij) / 2.  A
+    missing chi of two distinct species is an error."""
+    species = list(species) if species is not None else list(elements)
+    omega = omega or {}
+    for s in species:
+        if s not in elements:
+            raise ValueError(f"tersoff1989: no parameters for element '{s}'")
+        missing = [k for k in TERSOFF1989_ELEMENT_KEYS if k not in elements[s]]
+        if missing:
+            raise ValueError(f"tersoff1989: element '{s}' lacks {missing}")
+    out = {}
+    for i in species:
+        pi = elements[i]
+        for j in species:
+            pj = elements[j]
+            x = 1.0 if i == j else _pair_value(i, j, chi, None)
+            if x is None:
+                raise ValueError(f"tersoff1989: chi for the {i}-{j} pair is not given "
+                                 "(Tersoff 1989 tabulates only C-Si and Si-Ge).")
+            r_ij = math.sqrt(float(pi["R"]) * float(pj["R"]))
+            s_ij = math.sqrt(float(pi["S"]) * float(pj["S"]))
+            if not s_ij > r_ij:
+                raise ValueError(f"tersoff1989: S must exceed R ({i}-{j})")
+            for k in species:
+                rec = {"c": float(pi["c"]), "d": float(pi["d"]), "h": float(pi["h"]),
+                       "lambda3": 0.0, "m": 3.0, "gamma": _pair_value(i, k, omega, 1.0)}
+                if j == k:
+                    rec.update(A=math.sqrt(float(pi["A"]) * float(pj["A"])),
+                               B=x * math.sqrt(float(pi["B"]) * float(pj["B"])),
+                               lambda1=0.5 * (float(pi["lambda"]) + float(pj["lambda"])),
+                               lambda2=0.5 * (float(pi["mu"]) + float(pj["mu"])),
+                               R=0.5 * (r_ij + s_ij), D=0.5 * (s_ij - r_ij),
+                               beta=float(pi["beta"]), n=float(pi["n"]))
+                out[(i, j, k)] = rec
+    return out
+
+
 class TersoffPotential(_GpuPotential):
     """Tersoff bond-order potential (engine/tersoff.py:1071-1229): a flat
     single-species dict (m = 3, gamma = 1) or records keyed by (i, j, k)."""
@@ -507,6 +560,12 @@ def force_fn_from_config(cfg):
     species = cfg.get("species", [])
     if kind == "tersoff":
         params = cfg.get("params", {})
+        if isinstance(params, dict) and str(params.get("mixing", "")).lower() == "tersoff1989":
+            # {mixing: tersoff1989, elements, chi, omega} (fitting/registry.py:639-641,
+            # 687-694)
+            species = species or list(params["elements"])
+            params = tersoff1989_to_triplets(params["elements"], params.get("chi", {}),
+                                             species, params.get("omega"))
         if isinstance(params, list):
             params = _records_from_list(params)
         pot = TersoffPotential(params, species=species)

@@ -241,6 +241,53 @@ def eam_cuni(steps, warmup):
     return out
 
 
+def sw_gesn(steps, warmup, cpu=True):
+    """Multi-species SW Ge0.875Sn0.125 (Tomita et al. 2018 records): the
+    reference's ledger cell (diamond 8^3, 4096 atoms, a = 5.762 A, rattle
+    0.05 A seed 0, species from default_rng(1) after the 512-atom cell's
+    draw, benchmarks/bench_force.py:223-229) and a 20^3 cell (64000 atoms,
+    species from default_rng(1)), autotuned; the ledger cell also on the
+    host oracle (16 threads)."""
+    from akitsushima_amd import StillingerWeberPotential
+    from akitsushima_amd.structures import DIAMOND, cubic
+
+    rows = [
+        ("Ge", "Ge", "Ge", 2.181, 1.8110, 32.101, 1.1718, 10.093, 0.6862, 4.6973, 0.6588),
+        ("Sn", "Sn", "Sn", 2.503, 1.7825, 28.747, 1.1610, 9.4315, 0.7012, 4.9827, 0.6252),
+        ("Ge", "Ge", "Sn", 2.337, 1.7988, 29.858, 1.8136, 0.0, 0.0, 0.0, 0.0),
+        ("Ge", "Sn", "Ge", 2.337, 1.7988, 29.858, 0.7232, 0.0, 0.0, 0.0, 0.0),
+        ("Ge", "Sn", "Sn", 2.337, 1.7988, 24.460, 1.1484, 9.8864, 0.7021, 4.8537, 0.6482),
+        ("Sn", "Ge", "Ge", 2.337, 1.7988, 24.460, 1.1484, 9.8864, 0.7021, 4.8537, 0.6482),
+        ("Sn", "Ge", "Sn", 2.337, 1.7988, 26.074, 1.3564, 0.0, 0.0, 0.0, 0.0),
+        ("Sn", "Sn", "Ge", 2.337, 1.7988, 26.074, 1.0924, 0.0, 0.0, 0.0, 0.0),
+    ]
+    keys = ("sigma", "a", "lambda", "gamma", "A", "B", "p", "q")
+    recs = {r[:3]: dict(zip(keys, r[3:]), epsilon=1.0, costheta0=-0.333333) for r in rows}
+    sp = ["Ge", "Sn"]
+    out = {"workload": "SW Ge0.875Sn0.125 (Tomita 2018), diamond a 5.762 A, rattle 0.05 A"}
+    for cells in (8, 20):
+        lat, pos, _ = cubic(DIAMOND, 5.762, cells, ["Ge"] * 8)
+        rng = np.random.default_rng(1)
+        if cells == 8:  # the ledger's 4096-atom cell takes the second draw
+            rng.choice(sp, 512, p=[0.875, 0.125])
+        types = [str(x) for x in rng.choice(sp, len(pos), p=[0.875, 0.125])]
+        pot = StillingerWeberPotential(recs, species=sp)
+        tune = pot.autotune([(lat, pos, types)], reps=3)
+        r = {"atoms": len(pos), "autotuned": tune["best"]}
+        r.update(device_force_rate(pot, lat, pos, types, steps, warmup))
+        if cells == 8 and cpu:
+            import oracle
+
+            oracle.sw_multi(lat, pos, types, recs, sp, nthreads=16)
+            t0 = time.perf_counter()
+            for _ in range(5):
+                oracle.sw_multi(lat, pos, types, recs, sp, nthreads=16)
+            r["cpu_oracle_ms_per_call_16_threads"] = (time.perf_counter() - t0) / 5 * 1e3
+            r["reference_ledger_call_ms_1_thread"] = 33.9447  # benchmarks/adr0047.jsonl:8
+        out[f"cells_{cells}"] = r
+    return out
+
+
 def batched_tersoff(reps):
     """512 diamond Si structures of 64-2048 atoms (seeded supercell repeats,
     +-2 % symmetric strain, rattle 0.02 A) in one batched call, autotuned,
@@ -532,6 +579,7 @@ def main():
         out["eam_cuni"] = eam_cuni(20, 3)
         out["batched_tersoff"] = batched_tersoff(10)
         out["sw_elastic"] = sw_elastic()
+        out["sw_gesn"] = sw_gesn(50, 5, cpu=not args.no_cpu)
     print(json.dumps(out))
     return 0
 

@@ -230,3 +230,27 @@ def test_batched_equals_single_calls():
     for k, (lat, pos, t) in enumerate(structs):
         e0, f0, s0 = O.tersoff_flat(lat, pos, TERSOFF_SI)
         _check(E[k], F[k], S[k], e0, f0, s0, len(pos), f"batch {k}")
+
+
+# --------------------------------------------------------------------------
+#  Tersoff (1989) mixing records (engine/tersoff.py:1968-2034)
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("tile", TILES)
+def test_tersoff1989_sic_matches_oracle_and_table_ii(tile):
+    from scipy.optimize import minimize_scalar
+
+    from akitsushima_amd import tersoff1989_to_triplets
+    from akitsushima_amd.published import TERSOFF_1989
+    from akitsushima_amd.structures import DIAMOND, cubic
+
+    recs = tersoff1989_to_triplets(TERSOFF_1989["elements"], TERSOFF_1989["chi"], ["Si", "C"])
+    pot = TersoffPotential(recs, species=["Si", "C"])
+    pot.set_bond_tile(tile)
+    lat, pos, t = cubic(DIAMOND, 4.32, 3, ["Si"] * 4 + ["C"] * 4, rattle=0.06, seed=3)
+    e, f, s = pot.compute(lat, pos, t)
+    e0, f0, s0 = O.tersoff_multi(lat, pos, t, recs, ["Si", "C"])
+    _check(e, f, s, e0, f0, s0, len(pos), "SiC 1989")
+    efn = lambda a: pot.compute(*cubic(DIAMOND, a, 2, ["Si"] * 4 + ["C"] * 4, rattle=0.0))[0] / 64
+    r = minimize_scalar(efn, bounds=(4.2, 4.45), method="bounded", options={"xatol": 1e-8})
+    assert abs(r.x - 4.32) < 0.005 and abs(-r.fun - 6.165) < 0.0058
