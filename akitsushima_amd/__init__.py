@@ -1,12 +1,14 @@
 """akitsushima_amd: an MI355X-native FP64 force engine for akitsushima.
 
 Hand-written HIP kernels for gfx950 behind the reference's force-function
-API: Tersoff, Stillinger-Weber and Zhou EAM energies, forces and virial
-stress, a device cell-list neighbour list, batched evaluation of many
-structures, a device-resident NVE integrator, a Tersoff parameter
-gradient, and the batched callers of the force call: force constants and
-Gamma-point phonons, and elastic constants.  The compute path is the HIP library ``libakitsushima_amd.so``
-only; importing works without a GPU, computing does not.
+API: Tersoff (one or many species, Tersoff 1989 mixing), Stillinger-Weber
+(flat or multi-species records) and alloy EAM (Zhou forms or setfl tables)
+energies, forces and virial stress, a device cell-list neighbour list,
+batched evaluation of many structures, a device-resident NVE integrator, a
+Tersoff parameter gradient, and the batched callers of the force call:
+force constants and Gamma-point phonons, and elastic constants.  The compute
+path is the HIP library ``libakitsushima_amd.so`` only; importing works
+without a GPU, computing does not.
 """
 from .neighborlist import NeighbourList
 from .potentials import (
@@ -19,6 +21,7 @@ from .potentials import (
     make_force_fn,
     tersoff1989_to_triplets,
 )
+from .setfl import read_setfl, write_setfl
 from .elastic import elastic_constants
 from .phonon import force_constants, gamma_frequencies
 from .zhou import zhou_eam
@@ -35,6 +38,8 @@ __all__ = [
     "call_force_fn",
     "force_fn_from_config",
     "zhou_eam",
+    "read_setfl",
+    "write_setfl",
     "tersoff1989_to_triplets",
     "elastic_constants",
     "force_constants",

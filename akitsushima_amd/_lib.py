@@ -43,6 +43,9 @@ SIGNATURES = {
     "ak_sw_create": (ctypes.c_int, [_pf64, _phandle]),
     "ak_sw_multi_create": (ctypes.c_int, [_i32, _pf64, _pf64, _pf64, _f64, _phandle]),
     "ak_eam_create": (ctypes.c_int, [_i32, _i32, _pf64, _pi32, _f64, _phandle]),
+    "ak_eam_tables_create": (ctypes.c_int, [_i32, _i32, _pf64, _pi32, _i64, _pf64, _pf64, _f64,
+                                            _phandle]),
+    "ak_eam_eval": (ctypes.c_int, [ctypes.c_void_p, _i32, _i64, _pf64, _pf64, _pf64]),
     "ak_potential_destroy": (ctypes.c_int, [_handle]),
     "ak_potential_set_lanes": (ctypes.c_int, [_handle, _i32]),
     "ak_potential_set_launch": (ctypes.c_int, [_handle, _i32, _i32, _i32]),

@@ -290,8 +290,80 @@ int ak_eam_create(int32_t ns, int32_t nf, const double* fn, const int32_t* map, 
   std::vector<int> m(map, map + 2 * ns + ns * ns);
   for (int v : m)
     if (v < 0 || v >= nf) return fail(AK_ERR_ARG, "ak_eam_create: index_map out of range");
-  return make_potential(out, KIND_EAM, ns, r_cut, std::vector<double>(fn, fn + (size_t)nf * 44),
-                        std::move(m));
+  int rc = make_potential(out, KIND_EAM, ns, r_cut,
+                          std::vector<double>(fn, fn + (size_t)nf * 44), std::move(m));
+  if (rc == AK_OK) (*out)->pot.eam_nf = nf;
+  return rc;
+}
+
+int ak_eam_tables_create(int32_t ns, int32_t nf, const double* fn, const int32_t* map,
+                         int64_t n_knots, const double* knots, const double* coeffs,
+                         double r_cut, ak_potential** out) {
+  if (ns < 1 || ns > 16 || nf < 1 || !fn || !map || !(r_cut > 0.0) || n_knots < 0 ||
+      (n_knots > 0 && (!knots || !coeffs)))
+    return fail(AK_ERR_ARG, "ak_eam_tables_create: bad arguments");
+  std::vector<int> m(map, map + 2 * ns + ns * ns);
+  for (int v : m)
+    if (v < 0 || v >= nf) return fail(AK_ERR_ARG, "ak_eam_tables_create: index_map out of range");
+  // device layout: rows (nf, 44) | knots | coefficients (4 per knot, at a
+  // 4-double boundary); spline rows get absolute offsets, 1/h and x_0
+  const size_t nr = (size_t)nf * 44, nk = (size_t)n_knots;
+  const size_t xo = nr, co = (nr + nk + 3) / 4 * 4;
+  std::vector<double> t(co + 4 * nk, 0.0);
+  std::copy(fn, fn + nr, t.begin());
+  std::copy(knots, knots + nk, t.begin() + xo);
+  std::copy(coeffs, coeffs + 4 * nk, t.begin() + co);
+  for (int32_t f = 0; f < nf; ++f) {
+    double* row = t.data() + 44 * (size_t)f;
+    const int code = (int)row[0];
+    if (code >= 3 && code <= 6 && row[0] == (double)code) continue;
+    if (code != 1 && code != 2)
+      return fail(AK_ERR_ARG, "ak_eam_tables_create: function code must be 1-6");
+    const double off = row[1], cnt = row[2];
+    if (!(off >= 0.0) || !(cnt >= 2.0) || off != std::floor(off) || cnt != std::floor(cnt) ||
+        off + cnt > (double)nk)
+      return fail(AK_ERR_ARG, "ak_eam_tables_create: spline knots out of range");
+    const size_t o = (size_t)off, n = (size_t)cnt;
+    const double* x = knots + o;
+    for (size_t i = 1; i < n; ++i)
+      if (!(x[i] > x[i - 1]))
+        return fail(AK_ERR_ARG, "ak_eam_tables_create: spline knots must increase");
+    // uniform grid: the search may start at floor((r - x_0) / h)
+    const double h = (x[n - 1] - x[0]) / (double)(n - 1);
+    bool uniform = true;
+    for (size_t i = 0; i < n && uniform; ++i)
+      uniform = std::fabs(x[i] - (x[0] + (double)i * h)) <= 1e-9 * h;
+    row[1] = (double)(xo + o);
+    row[2] = (double)n;
+    row[3] = (double)(co + 4 * o);
+    row[4] = uniform ? 1.0 / h : 0.0;
+    row[5] = x[0];
+  }
+  int rc = make_potential(out, KIND_EAM, ns, r_cut, std::move(t), std::move(m));
+  if (rc == AK_OK) (*out)->pot.eam_nf = nf;
+  return rc;
+}
+
+int ak_eam_eval(ak_potential* pot, int32_t function, int64_t m, const double* x, double* v,
+                double* dv) {
+  return guarded([&] {
+    AK_REQUIRE(pot && pot->pot.kind == KIND_EAM, AK_ERR_ARG, "ak_eam_eval: not an EAM potential");
+    AK_REQUIRE(m >= 0 && (m == 0 || (x && v && dv)), AK_ERR_ARG, "ak_eam_eval: bad arguments");
+    AK_REQUIRE(function >= 0 && function < pot->pot.eam_nf, AK_ERR_ARG,
+               "ak_eam_eval: no such function");
+    if (m == 0) return;
+    DevBuf dx, dvv, ddv;
+    dx.ensure(sizeof(double) * (size_t)m);
+    dvv.ensure(sizeof(double) * (size_t)m);
+    ddv.ensure(sizeof(double) * (size_t)m);
+    hipStream_t st = pot->stream;
+    AK_HIP(hipMemcpyAsync(dx.p, x, sizeof(double) * (size_t)m, hipMemcpyHostToDevice, st));
+    eam_feval(pot->pot, function, m, dx.as<double>(), dvv.as<double>(), ddv.as<double>(), st);
+    AK_HIP(hipGetLastError());
+    AK_HIP(hipMemcpyAsync(v, dvv.p, sizeof(double) * (size_t)m, hipMemcpyDeviceToHost, st));
+    AK_HIP(hipMemcpyAsync(dv, ddv.p, sizeof(double) * (size_t)m, hipMemcpyDeviceToHost, st));
+    AK_HIP(hipStreamSynchronize(st));
+  });
 }
 
 int ak_potential_destroy(ak_potential* pot) {

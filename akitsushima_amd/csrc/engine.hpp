@@ -143,6 +143,7 @@ struct Potential {
   std::vector<int> h_map;
   DevBuf d_map, d_eam_fast;
   bool eam_fast = false;
+  int eam_nf = 0;  // function rows of an EAM table
   // SW with multi-species tables (pair, leg and triple records) in d_params
   bool sw_multi = false;
   // per-call scratch; amask (n) = bonds inside r_c per atom (rows >= 8)
@@ -188,6 +189,10 @@ struct Potential {
 // when the function table is not of the Zhou alloy form (forces.hip).
 bool eam_fast_table(int ns, const std::vector<double>& rows, const std::vector<int>& map,
                     std::vector<double>& out);
+
+// Function `f` of an EAM table at m device arguments (forces.hip).
+void eam_feval(const Potential& pot, int f, int64_t m, const double* d_x, double* d_v,
+               double* d_dv, hipStream_t st);
 
 // Forces on all atoms of `set`, per-structure energy and stress.
 // d_out: (n_struct, 10) = [E, S00..S22]; d_F (n_atoms, 3).

@@ -1279,12 +1279,54 @@ __global__ __launch_bounds__(256) void k_sw_multi(
 }
 
 // ---------------------------------------------------------------------------
-// Zhou EAM.  A function is a row of 44 doubles [code, r_c, width, chi,
-// pa[20], pb[20]]: code 3 density, 4 pair, 5 cross pair, 6 embedding;
-// element parameters in the order r_e, f_e, rho_e, rho_s, alpha, beta, A, B,
-// kappa, lambda, Fn0..Fn3, F0..F3, eta, F_e (engine/_eam_kernels.py:129-223).
+// EAM functions.  A function is a row of 44 doubles.  Zhou forms: [code, r_c,
+// width, chi, pa[20], pb[20]] with code 3 density, 4 pair, 5 cross pair,
+// 6 embedding; element parameters in the order r_e, f_e, rho_e, rho_s,
+// alpha, beta, A, B, kappa, lambda, Fn0..Fn3, F0..F3, eta, F_e
+// (engine/_eam_kernels.py:129-223).  Cubic splines (scipy PPoly, the
+// tables of a setfl file): [code, X offset, n, C offset, 1/h, x_0] with code
+// 1 = s(r), 2 = s(r) / r (the r*phi table); X (n knots) and C (4 doubles per
+// knot: c0..c3 of the interval starting there) sit after the rows in the same
+// buffer, the offsets counted from its start (engine/_eam_kernels.py:128-145,
+// 199-205).
 // ---------------------------------------------------------------------------
 constexpr int ZW = 44;
+
+// Interval i with X[i] <= r < X[i+1] as searchsorted(X, r, 'right') - 1,
+// clipped to [0, n - 2] (extrapolation by the end polynomials), then Horner.
+// With a uniform grid (1/h > 0) the search starts from floor((r - x_0) / h)
+// and steps to the exact interval; otherwise it bisects.
+__device__ __forceinline__ void spline_eval(const double* __restrict__ fn,
+                                            const double* __restrict__ f, double r, double& v,
+                                            double& dv) {
+  const double* X = fn + (size_t)f[1];
+  const int n = (int)f[2];
+  const double* C = fn + (size_t)f[3];
+  int i;
+  if (f[4] > 0.0) {
+    double g = floor((r - f[5]) * f[4]);
+    g = fmin(fmax(g, -1.0), (double)(n - 1));  // NaN -> -1
+    i = (int)g;
+    while (i + 1 < n && X[i + 1] <= r) ++i;
+    while (i >= 0 && X[i] > r) --i;
+  } else {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (X[mid] <= r)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    i = lo - 1;
+  }
+  i = i < 0 ? 0 : i;
+  i = i > n - 2 ? n - 2 : i;
+  const double dx = r - X[i];
+  const double4 c = *reinterpret_cast<const double4*>(C + 4 * (size_t)i);
+  v = ((c.x * dx + c.y) * dx + c.z) * dx + c.w;
+  dv = (3.0 * c.x * dx + 2.0 * c.y) * dx + c.z;
+}
 
 __device__ __forceinline__ void z_decay(double r, double re, double pre, double expo,
                                         double shift, double& val, double& dval) {
@@ -1319,9 +1361,22 @@ __device__ __forceinline__ void z_embed(const double* __restrict__ p, double rho
   }
 }
 
-__device__ __forceinline__ void z_feval(const double* __restrict__ f, double r, double& v,
+__device__ __forceinline__ void z_feval(const double* __restrict__ fn,
+                                        const double* __restrict__ f, double r, double& v,
                                         double& dv) {
   const int code = (int)f[0];
+  if (code <= 2) {
+    double sv, sd;
+    spline_eval(fn, f, r, sv, sd);
+    if (code == 1) {
+      v = sv;
+      dv = sd;
+    } else {
+      v = sv / r;
+      dv = (sd * r - sv) / (r * r);
+    }
+    return;
+  }
   if (code == 6) {
     z_embed(f + 4, r, v, dv);
     return;
@@ -1474,7 +1529,7 @@ __global__ __launch_bounds__(256) void k_eam_rho(
       if (r < rc && r > 1e-12) {
         const int sj = ns > 1 ? spc[j] : 0;
         double v, dv;
-        z_feval(fn + (size_t)ZW * map[sj], r, v, dv);
+        z_feval(fn, fn + (size_t)ZW * map[sj], r, v, dv);
         rho += v;
       }
     }
@@ -1483,7 +1538,7 @@ __global__ __launch_bounds__(256) void k_eam_rho(
   if (live && l == 0) {
     const int sa = ns > 1 ? spc[a] : 0;
     double F, d;
-    z_feval(fn + (size_t)ZW * map[ns + sa], rho > 0.0 ? rho : 0.0, F, d);
+    z_feval(fn, fn + (size_t)ZW * map[ns + sa], rho > 0.0 ? rho : 0.0, F, d);
     Fe[a] = F;
     dF[a] = d;
   }
@@ -1521,12 +1576,12 @@ __device__ __forceinline__ void k_eam_force_chunk(int c, int n_chunks,
         if (fast) {
           z_pair_fast(fast, ns, sa, sj, r, dfj, dfa, phi, dphi);
         } else {
-          z_feval(fn + (size_t)ZW * map[sj], r, v, dfj);
+          z_feval(fn, fn + (size_t)ZW * map[sj], r, v, dfj);
           if (sj == sa)
             dfa = dfj;
           else
-            z_feval(fa_rho, r, v, dfa);
-          z_feval(fn + (size_t)ZW * map[2 * ns + sa * ns + sj], r, phi, dphi);
+            z_feval(fn, fa_rho, r, v, dfa);
+          z_feval(fn, fn + (size_t)ZW * map[2 * ns + sa * ns + sj], r, phi, dphi);
         }
         double c = dphi + dFa * dfj + dF[j] * dfa;
         double s = c / r;
@@ -1566,6 +1621,22 @@ __global__ __launch_bounds__(256) void k_eam_force(int n_chunks, const double* _
     const double* __restrict__ fast) {
   for (int cc = blockIdx.x; cc < n_chunks; cc += gridDim.x)
     k_eam_force_chunk<LE>(xcd_remap(cc, n_chunks), n_chunks, pos, sid, geoms, single, spc, ns, fn, map, rc, cnt, nbr, stride, dF, Fe, chunk_rng, F, partial, fast);
+}
+
+// Function f of the table at m arguments (tabulation of a potential).
+__global__ __launch_bounds__(256) void k_eam_feval(const double* __restrict__ fn, int f, int64_t m,
+                                                   const double* __restrict__ x,
+                                                   double* __restrict__ v,
+                                                   double* __restrict__ dv) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < m) z_feval(fn, fn + (size_t)ZW * f, x[k], v[k], dv[k]);
+}
+
+void eam_feval(const Potential& pot, int f, int64_t m, const double* d_x, double* d_v,
+               double* d_dv, hipStream_t st) {
+  if (m <= 0) return;
+  hipLaunchKernelGGL(k_eam_feval, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st,
+                     (const double*)pot.d_params.as<double>(), f, m, d_x, d_v, d_dv);
 }
 
 // ---------------------------------------------------------------------------

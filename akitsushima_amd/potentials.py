@@ -464,11 +464,18 @@ class StillingerWeberPotential(_GpuPotential):
 
 
 class EAMPotential(_GpuPotential):
-    """Alloy EAM whose functions are Zhou-Johnson-Wadley forms (the
-    EAMPotential that engine/eam_zhou.py:zhou_eam builds).  Use
-    :func:`akitsushima_amd.zhou.zhou_eam` to construct one."""
+    """Alloy EAM over the GPU engine: function rows of Zhou-Johnson-Wadley
+    forms (the EAMPotential that engine/eam_zhou.py:zhou_eam builds; use
+    :func:`akitsushima_amd.zhou.zhou_eam`) or cubic splines of tabulated
+    functions (the one read_setfl builds, engine/eam_zhou.py:257-315; use
+    :func:`akitsushima_amd.setfl.read_setfl`).
 
-    def __init__(self, functions, index_map, r_cut, species):
+    ``functions`` (n, 44) rows, ``index_map`` = [rho[ns], F[ns], phi[ns*ns]]
+    row indices; spline rows are [code (1 = s(r), 2 = s(r)/r), first knot,
+    number of knots, 0...] into ``knots`` (n_knots,) and ``coeffs``
+    (n_knots, 4), the PPoly coefficients c0..c3 of each knot's interval."""
+
+    def __init__(self, functions, index_map, r_cut, species, knots=None, coeffs=None):
         super().__init__()
         self.species = list(species)
         self.r_cut = float(r_cut)
@@ -476,14 +483,53 @@ class EAMPotential(_GpuPotential):
         self._map = np.ascontiguousarray(index_map, dtype=np.int32)
         if self._fn.ndim != 2 or self._fn.shape[1] != 44:
             raise ValueError("EAM function table must be (n, 44)")
+        ns = len(self.species)
+        if len(self._map) != 2 * ns + ns * ns:
+            raise ValueError("EAM index_map must hold rho[ns], F[ns], phi[ns*ns]")
+        self._knots = None if knots is None else np.ascontiguousarray(knots, dtype=float)
+        self._coeffs = None if coeffs is None else np.ascontiguousarray(coeffs, dtype=float)
+        if (self._knots is None) != (self._coeffs is None):
+            raise ValueError("EAM spline tables need both knots and coeffs")
+        if self._knots is not None and self._coeffs.shape != (len(self._knots), 4):
+            raise ValueError("EAM spline coeffs must be (n_knots, 4)")
 
     def _create(self):
         h = ctypes.c_void_p()
-        check(lib().ak_eam_create(len(self.species), len(self._fn),
-                                  self._fn.ctypes.data_as(_lib._pf64),
-                                  self._map.ctypes.data_as(_lib._pi32), self.r_cut,
-                                  ctypes.byref(h)))
+        if self._knots is None:
+            check(lib().ak_eam_create(len(self.species), len(self._fn),
+                                      self._fn.ctypes.data_as(_lib._pf64),
+                                      self._map.ctypes.data_as(_lib._pi32), self.r_cut,
+                                      ctypes.byref(h)))
+        else:
+            check(lib().ak_eam_tables_create(
+                len(self.species), len(self._fn), self._fn.ctypes.data_as(_lib._pf64),
+                self._map.ctypes.data_as(_lib._pi32), len(self._knots),
+                self._knots.ctypes.data_as(_lib._pf64), self._coeffs.ctypes.data_as(_lib._pf64),
+                self.r_cut, ctypes.byref(h)))
         return h
+
+    def function_index(self, kind, a, b=None):
+        """Row of F ("F"), rho ("rho") of species a, or phi ("phi") of a-b."""
+        ns = len(self.species)
+        ia = self.species.index(a)
+        if kind == "rho":
+            return int(self._map[ia])
+        if kind == "F":
+            return int(self._map[ns + ia])
+        if kind == "phi":
+            return int(self._map[2 * ns + ia * ns + self.species.index(b)])
+        raise ValueError(f"unknown EAM function kind {kind!r}")
+
+    def evaluate(self, function, x):
+        """(values, derivatives) of function row ``function`` at ``x``,
+        evaluated on the device."""
+        x = np.ascontiguousarray(x, dtype=float).reshape(-1)
+        v = np.empty_like(x)
+        dv = np.empty_like(x)
+        check(lib().ak_eam_eval(self.handle(), int(function), len(x),
+                                x.ctypes.data_as(_lib._pf64), v.ctypes.data_as(_lib._pf64),
+                                dv.ctypes.data_as(_lib._pf64)))
+        return v, dv
 
 
 # ---------------------------------------------------------------------------
@@ -578,8 +624,15 @@ def force_fn_from_config(cfg):
         from .published import ZHOU_2004
         from .zhou import zhou_eam
 
+        if "setfl" in cfg:
+            # {type: EAM, setfl: file.eam.alloy} (fitting/registry.py:779-786)
+            from .setfl import read_setfl
+
+            pot = read_setfl(cfg["setfl"], species=species or None)
+            return make_force_fn(PotentialEngine().add(pot))
         if str(cfg.get("form", "")).lower() != "zhou2004":
-            raise NotImplementedError("EAM: only form 'zhou2004' is part of this engine")
+            raise ValueError("EAM: give 'setfl: <file>' (a LAMMPS eam/alloy table) or "
+                             "'form: zhou2004' with elements")
         el = cfg["elements"]
         if isinstance(el, (list, tuple)):
             el = {s: dict(ZHOU_2004[s]) for s in el}

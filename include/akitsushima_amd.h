@@ -85,6 +85,26 @@ int ak_sw_multi_create(int32_t n_species, const double *pair_table, const double
 int ak_eam_create(int32_t n_species, int32_t n_functions, const double *functions,
                   const int32_t *index_map, double r_cut, ak_potential **out);
 
+/* Alloy EAM from tabulated functions as well: the EAMPotential that
+ * read_setfl builds (engine/eam_zhou.py:257-315), whose functions are scipy
+ * CubicSpline PPolys evaluated as the numba kernel does
+ * (engine/_eam_kernels.py:128-145, 199-205: interval from
+ * searchsorted(x, r, 'right') - 1 clipped to the end intervals, Horner).
+ * functions: (n_functions, 44) rows; besides the Zhou rows of ak_eam_create,
+ * code 1 = spline s(r) and code 2 = s(r) / r (a setfl r*phi table) as
+ * [code, first knot, number of knots, 0...] indexing knots (n_knots) and
+ * coeffs (n_knots, 4) = c0..c3 (PPoly order, highest power first) of the
+ * interval starting at that knot (the last knot's row is unused). */
+int ak_eam_tables_create(int32_t n_species, int32_t n_functions, const double *functions,
+                         const int32_t *index_map, int64_t n_knots, const double *knots,
+                         const double *coeffs, double r_cut, ak_potential **out);
+
+/* Value and derivative of function row `function` of an EAM potential at m
+ * arguments (the tabulation write_setfl does, engine/eam_zhou.py:216-254,
+ * evaluated on the device). */
+int ak_eam_eval(ak_potential *pot, int32_t function, int64_t m, const double *x, double *v,
+                double *dv);
+
 int ak_potential_destroy(ak_potential *pot);
 
 /* ---- Neighbour list (NeighbourList, engine/neighborlist.py:390-714) ---- */

@@ -23,6 +23,9 @@ after the reference's Python glue:
   _sw_multi_3body_numpy :476-560)
 * ``zhou_*`` and ``eam_zhou``: engine/eam_zhou.py:47-207 and the numba kernel
   engine/_eam_kernels.py:231-307
+* ``read_setfl`` / ``write_setfl_zhou`` / ``eam_table``: setfl tables as
+  engine/eam_zhou.py:210-315 reads and writes them, evaluated as
+  engine/_eam_kernels.py:128-145, 199-205 (scipy CubicSpline coefficients)
 * ``tersoff_loss``: fitting/loss.py:51-248 (squared loss)
 * ``nve_run``: NVESimulation.run, solvers/molecular_dynamics.py:1969-1996
 
@@ -85,9 +88,14 @@ def lib():
                                             ctypes.c_double, ctypes.c_int,
                                             _f64p, _f64p, _f64p]
         L.orc_eam.restype = ctypes.c_int
+        _i64p = ctypes.POINTER(ctypes.c_int64)
         L.orc_eam.argtypes = common + [_i32p, ctypes.c_int32, _i32p, _i32p, _i32p,
-                                       _f64p, ctypes.c_double, ctypes.c_int,
+                                       _f64p, _i64p, _i64p, _f64p, _f64p, ctypes.c_int64,
+                                       ctypes.c_double, ctypes.c_int,
                                        _f64p, _f64p, _f64p]
+        L.orc_eam_feval.restype = None
+        L.orc_eam_feval.argtypes = [_f64p, _i64p, _i64p, _f64p, _f64p, ctypes.c_int64,
+                                    ctypes.c_int32, ctypes.c_int64, _f64p, _f64p, _f64p]
         L.orc_zhou_feval.restype = None
         L.orc_zhou_feval.argtypes = [_f64p, ctypes.c_int64, _f64p, _f64p, _f64p]
         _lib = L
@@ -476,23 +484,159 @@ def zhou_function_table(elements, species, r_cut=None, chi=None, width=0.5):
     return np.array(rows), rho_f, emb_f, phi_f, float(r_cut), P
 
 
-def eam_zhou(lattice, positions, types, elements, species, pairs=None, nthreads=1,
-             r_cut=None, chi=None):
-    fn, rho_f, emb_f, phi_f, rc, _ = zhou_function_table(elements, species, r_cut, chi)
-    lattice, positions, ci, cj, inv, vol = _prep(lattice, positions, pairs, rc)
+class EamTable:
+    """Function rows plus the spline tables of build_tables
+    (engine/_eam_kernels.py:77-121): xo, nx per function, X (sum nx), C (4,
+    sum nx) with a zero column at each function's last knot."""
+
+    def __init__(self, rows, rho_f, emb_f, phi_f, r_cut, splines=None):
+        self.rows = np.ascontiguousarray(rows, float)
+        nf = len(self.rows)
+        self.rho_f = np.ascontiguousarray(rho_f, np.int32)
+        self.emb_f = np.ascontiguousarray(emb_f, np.int32)
+        self.phi_f = np.ascontiguousarray(np.asarray(phi_f, np.int32).reshape(-1))
+        self.r_cut = float(r_cut)
+        self.xo = np.zeros(nf, np.int64)
+        self.nx = np.zeros(nf, np.int64)
+        Xs, Cs, off = [], [], 0
+        for f, (x, c) in (splines or {}).items():
+            x = np.asarray(x, float)
+            cc = np.zeros((4, len(x)))
+            cc[:, :len(x) - 1] = c
+            self.xo[f], self.nx[f] = off, len(x)
+            Xs.append(x)
+            Cs.append(cc)
+            off += len(x)
+        self.X = np.ascontiguousarray(np.concatenate(Xs) if Xs else np.zeros(1))
+        self.C = np.ascontiguousarray(np.concatenate(Cs, axis=1) if Cs else np.zeros((4, 1)))
+
+    def args(self):
+        i64 = ctypes.POINTER(ctypes.c_int64)
+        return (_p(self.rows, _f64p), _p(self.xo, i64), _p(self.nx, i64), _p(self.X, _f64p),
+                _p(self.C, _f64p), self.C.shape[1])
+
+    def feval(self, f, r):
+        r = np.ascontiguousarray(r, dtype=float)
+        v = np.empty_like(r)
+        dv = np.empty_like(r)
+        lib().orc_eam_feval(*self.args(), int(f), len(r), _p(r, _f64p), _p(v, _f64p),
+                            _p(dv, _f64p))
+        return v, dv
+
+
+def eam_table(lattice, positions, types, tab, species, pairs=None, nthreads=1):
+    """engine/_eam_kernels.py:231-307 over an EamTable."""
+    lattice, positions, ci, cj, inv, vol = _prep(lattice, positions, pairs, tab.r_cut)
     idx = {s: k for k, s in enumerate(species)}
     spc = np.array([idx[t] for t in types], np.int32)
     n = len(positions)
     e = ctypes.c_double()
     f = np.zeros((n, 3))
     s = np.zeros((3, 3))
-    fn = np.ascontiguousarray(fn)
-    phi_f = np.ascontiguousarray(phi_f.reshape(-1))
+    rows, xo, nx, X, C, NX = tab.args()
     lib().orc_eam(n, _p(positions, _f64p), _p(lattice, _f64p), _p(inv, _f64p), vol,
                   len(ci), _p(ci, _i32p), _p(cj, _i32p), _p(spc, _i32p), len(species),
-                  _p(rho_f, _i32p), _p(emb_f, _i32p), _p(phi_f, _i32p), _p(fn, _f64p),
-                  rc, int(nthreads), ctypes.byref(e), _p(f, _f64p), _p(s, _f64p))
+                  _p(tab.rho_f, _i32p), _p(tab.emb_f, _i32p), _p(tab.phi_f, _i32p), rows,
+                  xo, nx, X, C, NX, tab.r_cut, int(nthreads), ctypes.byref(e), _p(f, _f64p),
+                  _p(s, _f64p))
     return e.value, f, s
+
+
+def zhou_table(elements, species, r_cut=None, chi=None):
+    fn, rho_f, emb_f, phi_f, rc, _ = zhou_function_table(elements, species, r_cut, chi)
+    return EamTable(fn, rho_f, emb_f, phi_f, rc)
+
+
+def eam_zhou(lattice, positions, types, elements, species, pairs=None, nthreads=1,
+             r_cut=None, chi=None):
+    return eam_table(lattice, positions, types, zhou_table(elements, species, r_cut, chi),
+                     species, pairs=pairs, nthreads=nthreads)
+
+
+# setfl files (engine/eam_zhou.py:210-315)
+
+def _setfl_block(vals):
+    vals = list(vals)
+    return "\n".join(" ".join(f"{v: .16e}" for v in vals[i:i + 5])
+                     for i in range(0, len(vals), 5)) + "\n"
+
+
+def write_setfl_zhou(path, elements, species, masses, n_rho=5000, rho_max=None, n_r=5000,
+                     r_cut=None, chi=None):
+    """write_setfl (engine/eam_zhou.py:216-254) of the Zhou potential, the
+    functions tabulated by this oracle."""
+    fn, rho_f, emb_f, phi_f, rc, P = zhou_function_table(elements, species, r_cut, chi)
+    if rho_max is None:
+        rho_max = 2.0 * max(p["rho_e"] for p in P.values())
+    drho = rho_max / (n_rho - 1)
+    dr = rc / (n_r - 1)
+    rho = np.arange(n_rho) * drho
+    r = np.arange(n_r) * dr
+    with open(path, "w") as fh:
+        fh.write("\n\n\n")
+        fh.write(f"{len(species)} " + " ".join(species) + "\n")
+        fh.write(f"{n_rho} {drho:.16e} {n_r} {dr:.16e} {rc:.16e}\n")
+        for k, s in enumerate(species):
+            fh.write(f"0 {masses[s]:.6f} {0.0:.6f} fcc\n")
+            fh.write(_setfl_block(zhou_feval(fn[emb_f[k]], rho)[0]))
+            fh.write(_setfl_block(zhou_feval(fn[rho_f[k]], r)[0]))
+        for i in range(len(species)):
+            for j in range(i + 1):
+                fh.write(_setfl_block(r * zhou_feval(fn[phi_f[i, j]], r)[0]))
+
+
+def read_setfl(path):
+    """read_setfl (engine/eam_zhou.py:257-315): each table a scipy
+    CubicSpline (default not-a-knot ends), r*phi stored, phi = (r phi) / r.
+    Returns (EamTable, species)."""
+    from scipy.interpolate import CubicSpline
+
+    with open(path) as fh:
+        lines = fh.read().splitlines()
+    head = lines[3].split()
+    n_el = int(head[0])
+    names = head[1:1 + n_el]
+    n_rho, drho, n_r, dr, r_cut = lines[4].split()[:5]
+    n_rho, n_r = int(n_rho), int(n_r)
+    drho, dr, r_cut = float(drho), float(dr), float(r_cut)
+    pos = 5
+    tables = {}
+
+    def take(n):
+        nonlocal pos
+        out = []
+        while len(out) < n:
+            out.extend(float(v) for v in lines[pos].split())
+            pos += 1
+        return np.array(out[:n])
+
+    for s in names:
+        pos += 1
+        tables[("F", s)] = take(n_rho)
+        tables[("rho", s)] = take(n_r)
+    for i, a in enumerate(names):
+        for b in names[:i + 1]:
+            tables[("rphi", a, b)] = take(n_r)
+    rho_g = np.arange(n_rho) * drho
+    r_g = np.arange(n_r) * dr
+    ns = len(names)
+    rows, splines = [], {}
+
+    def add(code, x, y):
+        cs = CubicSpline(x, y)
+        row = np.zeros(ZW)
+        row[0] = code
+        rows.append(row)
+        splines[len(rows) - 1] = (cs.x, cs.c)
+        return len(rows) - 1
+
+    rho_f = [add(1, r_g, tables[("rho", s)]) for s in names]
+    emb_f = [add(1, rho_g, tables[("F", s)]) for s in names]
+    phi_f = np.zeros((ns, ns), np.int32)
+    for i, a in enumerate(names):
+        for j, b in enumerate(names[:i + 1]):
+            phi_f[i, j] = phi_f[j, i] = add(2, r_g, tables[("rphi", a, b)])
+    return EamTable(np.array(rows), rho_f, emb_f, phi_f, r_cut, splines), names
 
 
 def zhou_feval(row, r):

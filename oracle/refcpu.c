@@ -972,7 +972,37 @@ static inline void z_embed(const double *p, double rho, double *F, double *dF)
     }
 }
 
-static inline void z_feval(const double *f, double r, double *v, double *dv)
+/* Tabulated functions (engine/_eam_kernels.py:77-121 build_tables): function
+ * f with code 1 (spline) or 2 (spline / r) owns knots X[xo[f] .. xo[f] +
+ * nx[f]) and the PPoly coefficients C[0..3][same range] (C has NX columns). */
+typedef struct {
+    const double *fn;
+    const int64_t *xo, *nx;
+    const double *X, *C;
+    int64_t NX;
+} eam_tab_t;
+
+/* engine/_eam_kernels.py:128-145 */
+static inline void spline_eval(const eam_tab_t *T, int f, double r, double *v, double *dv)
+{
+    int64_t o = T->xo[f], n = T->nx[f];
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {  /* searchsorted side='right' */
+        int64_t mid = (lo + hi) / 2;
+        if (T->X[o + mid] <= r) lo = mid + 1;
+        else hi = mid;
+    }
+    int64_t i = lo - 1;
+    if (i < 0) i = 0;
+    if (i > n - 2) i = n - 2;
+    double dx = r - T->X[o + i];
+    double c0 = T->C[o + i], c1 = T->C[T->NX + o + i], c2 = T->C[2 * T->NX + o + i],
+           c3 = T->C[3 * T->NX + o + i];
+    *v = ((c0 * dx + c1) * dx + c2) * dx + c3;
+    *dv = (3.0 * c0 * dx + 2.0 * c1) * dx + c2;
+}
+
+static inline void z_feval_row(const double *f, double r, double *v, double *dv)
 {
     int code = (int)f[0];
     if (code == 6) { z_embed(f + 4, r, v, dv); return; }
@@ -1003,14 +1033,31 @@ static inline void z_feval(const double *f, double r, double *v, double *dv)
     *dv = dval * S + val * dS;
 }
 
-/* rho_f[s], emb_f[s], phi_f[s*ns+t]: function row indices into fn (ZW each) */
+/* engine/_eam_kernels.py:199-205 (_feval, spline codes) and the Zhou rows */
+static inline void z_feval(const eam_tab_t *T, int f, double r, double *v, double *dv)
+{
+    int code = (int)T->fn[(size_t)ZW * f];
+    if (code == 1 || code == 2) {
+        double s_, ds;
+        spline_eval(T, f, r, &s_, &ds);
+        if (code == 1) { *v = s_; *dv = ds; }
+        else { *v = s_ / r; *dv = (ds * r - s_) / (r * r); }
+        return;
+    }
+    z_feval_row(T->fn + (size_t)ZW * f, r, v, dv);
+}
+
+/* rho_f[s], emb_f[s], phi_f[s*ns+t]: function row indices into fn (ZW each);
+ * xo, nx, X, C, NX: the spline tables (unused when no row has code 1 or 2) */
 int orc_eam(int32_t n, const double *pos, const double *lat, const double *inv, double volume,
             int64_t ncand, const int32_t *ci, const int32_t *cj, const int32_t *spc,
             int32_t ns, const int32_t *rho_f, const int32_t *emb_f, const int32_t *phi_f,
-            const double *fn, double rc, int nthreads, double *energy, double *forces,
-            double *stress)
+            const double *fn, const int64_t *xo, const int64_t *nx, const double *X,
+            const double *C, int64_t NX, double rc, int nthreads, double *energy,
+            double *forces, double *stress)
 {
-    (void)ns;
+    eam_tab_t TT = {fn, xo, nx, X, C, NX};
+    const eam_tab_t *Tb = &TT;
     memset(forces, 0, sizeof(double) * 3 * (size_t)n);
     memset(stress, 0, sizeof(double) * 9);
     *energy = 0.0;
@@ -1030,9 +1077,9 @@ int orc_eam(int32_t n, const double *pos, const double *lat, const double *inv, 
     for (int64_t p = 0; p < P; ++p) {
         int si = spc[pr.ii[p]], sj = spc[pr.jj[p]];
         double r = pr.r[p];
-        z_feval(fn + (size_t)ZW * rho_f[sj], r, &rho_i[p], &drho_j[p]);
-        z_feval(fn + (size_t)ZW * rho_f[si], r, &rho_j[p], &drho_i[p]);
-        z_feval(fn + (size_t)ZW * phi_f[si * ns + sj], r, &phi[p], &dphi[p]);
+        z_feval(Tb, rho_f[sj], r, &rho_i[p], &drho_j[p]);
+        z_feval(Tb, rho_f[si], r, &rho_j[p], &drho_i[p]);
+        z_feval(Tb, phi_f[si * ns + sj], r, &phi[p], &dphi[p]);
     }
     /* two-way CSR: atom -> its pairs with side +1 (a = i) or -1 (a = j) */
     int64_t *cnt = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
@@ -1059,7 +1106,7 @@ int orc_eam(int32_t n, const double *pos, const double *lat, const double *inv, 
             rb += side[k] == 1 ? rho_i[p] : rho_j[p];
         }
         if (rb < 0.0) rb = 0.0;
-        z_feval(fn + (size_t)ZW * emb_f[spc[a]], rb, &Fv[a], &dF[a]);
+        z_feval(Tb, emb_f[spc[a]], rb, &Fv[a], &dF[a]);
     }
     /* phase 3: per pair force */
     double *fij = (double *)malloc(sizeof(double) * 3 * (size_t)P);
@@ -1097,5 +1144,14 @@ int orc_eam(int32_t n, const double *pos, const double *lat, const double *inv, 
 /* One function value, for host-side checks of the Zhou construction. */
 void orc_zhou_feval(const double *f, int64_t m, const double *r, double *v, double *dv)
 {
-    for (int64_t k = 0; k < m; ++k) z_feval(f, r[k], &v[k], &dv[k]);
+    for (int64_t k = 0; k < m; ++k) z_feval_row(f, r[k], &v[k], &dv[k]);
+}
+
+/* Function f of a table (any code) at m arguments. */
+void orc_eam_feval(const double *fn, const int64_t *xo, const int64_t *nx, const double *X,
+                   const double *C, int64_t NX, int32_t f, int64_t m, const double *r,
+                   double *v, double *dv)
+{
+    eam_tab_t TT = {fn, xo, nx, X, C, NX};
+    for (int64_t k = 0; k < m; ++k) z_feval(&TT, f, r[k], &v[k], &dv[k]);
 }
