@@ -67,6 +67,14 @@ SIGNATURES = {
                                                  _vp, _i32, _vp]),
     "ak_md_nve": (ctypes.c_int, [_handle, _i32, _pf64, _pf64, _pf64, _pf64, _pf64, _pi32, _f64,
                                  _i32, _f64, _i32, _pf64, _pf64, _pi32, _pf64]),
+    "ak_md_run": (ctypes.c_int, [_handle, _i32, _pf64, _pf64, _pf64, _pf64, _pf64, _pi32, _f64,
+                                 _i32, _f64, _i32, _pf64, _i32, _pf64, _pf64, _pf64, _pf64, _pi32,
+                                 _pf64]),
+    "ak_site_terms": (ctypes.c_int, [_handle, _handle, _i32, _pf64, _pf64, _pf64, _pi32, _pf64,
+                                     _i64, _pi32, _pi32, _pf64, _pf64,
+                                     ctypes.POINTER(ctypes.c_int64)]),
+    "ak_heat_flux": (ctypes.c_int, [_handle, _handle, _i32, _pf64, _pf64, _pf64, _pf64, _pf64,
+                                    _pi32, _pf64, _pf64]),
     "ak_profile_enable": (ctypes.c_int, [_handle, _i32]),
     "ak_profile_read": (ctypes.c_int, [_handle, _pf64, _pf64, _pi64]),
     "ak_tersoff_loss_grad": (ctypes.c_int, [_handle, _i32, _pi64, _pf64, _pf64, _pf64, _pf64,

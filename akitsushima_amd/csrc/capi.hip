@@ -185,6 +185,52 @@ static void stage_species(ak_potential* P, int n, const int32_t* species) {
   P->spc_valid = true;
 }
 
+// The list a single-structure call uses, positions already in P->d_pos: the
+// caller's (as built or rebuilt by the caller -- the reference's potentials
+// use a passed list's candidates without rebuilding it; the current cell is
+// used for the minimum image), or the potential's owned list (skin 0.3 A
+// when the cell allows it, else none), checked and rebuilt.
+static ak_nlist* select_list(ak_potential* P, ak_nlist* L, int n, const double* lattice,
+                             const double* inv, hipStream_t st) {
+  if (L) {
+    AK_REQUIRE(L->nl.r_cut + 1e-12 >= P->pot.r_cut, AK_ERR_ARG,
+               "neighbour list r_cut is smaller than the potential's cutoff");
+    L->set.setup(1, std::vector<int64_t>{0, (int64_t)n}.data(), lattice, inv, L->nl.r_list, st);
+    AK_REQUIRE(L->nl.valid && L->nl.n_atoms == n, AK_ERR_STATE,
+               "the neighbour list was not built for this structure");
+    return L;
+  }
+  L = &P->owned;
+  CellInfo ci = cell_info(lattice);
+  double skin = (P->pot.r_cut + 0.3 < ci.min_width / 2.0) ? 0.3 : 0.0;
+  if (skin != P->owned_skin) {
+    P->owned.nl.valid = false;
+    P->owned.nl.r_cut = P->pot.r_cut;
+    P->owned.nl.skin = skin;
+    P->owned.nl.r_list = P->pot.r_cut + skin;
+    P->owned_skin = skin;
+  }
+  check_and_rebuild(*L, n, lattice, inv, P->d_pos.as<double>(), st);
+  return L;
+}
+
+// Minimum-image bond vectors of every slot below cnt (3 planes n x stride).
+__global__ __launch_bounds__(256) void k_site_vec(int n, const double* __restrict__ pos,
+                                                  Geom g, const int* __restrict__ cnt,
+                                                  const int* __restrict__ nbr, int stride,
+                                                  double* __restrict__ vec) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * stride) return;
+  const int a = (int)(t / stride), k = (int)(t % stride);
+  if (k >= cnt[a]) return;
+  double ux, uy, uz;
+  mic_fast(g, pos, a, nbr[t], ux, uy, uz);
+  const size_t plane = (size_t)n * stride;
+  vec[t] = ux;
+  vec[plane + t] = uy;
+  vec[2 * plane + t] = uz;
+}
+
 extern "C" {
 
 int ak_abi_version(void) { return AK_ABI_VERSION; }
@@ -483,31 +529,7 @@ int ak_compute(ak_potential* P, ak_nlist* nl, int32_t n, const double* lattice,
     if (n > 0)
       AK_HIP(hipMemcpyAsync(P->d_pos.p, positions, sizeof(double) * 3 * n,
                             hipMemcpyHostToDevice, st));
-    ak_nlist* L = nl;
-    if (L) {
-      AK_REQUIRE(L->nl.r_cut + 1e-12 >= P->pot.r_cut, AK_ERR_ARG,
-                 "neighbour list r_cut is smaller than the potential's cutoff");
-      // the caller's list, as built or rebuilt by the caller (the reference's
-      // potentials use a passed list's candidates without rebuilding it);
-      // the current cell is used for the minimum image
-      L->set.setup(1, std::vector<int64_t>{0, (int64_t)n}.data(), lattice, inv, L->nl.r_list,
-                   st);
-      AK_REQUIRE(L->nl.valid && L->nl.n_atoms == n, AK_ERR_STATE,
-                 "the neighbour list was not built for this structure");
-    } else {
-      // owned list: skin 0.3 A when the cell allows it, else none
-      L = &P->owned;
-      CellInfo ci = cell_info(lattice);
-      double skin = (P->pot.r_cut + 0.3 < ci.min_width / 2.0) ? 0.3 : 0.0;
-      if (skin != P->owned_skin) {
-        P->owned.nl.valid = false;
-        P->owned.nl.r_cut = P->pot.r_cut;
-        P->owned.nl.skin = skin;
-        P->owned.nl.r_list = P->pot.r_cut + skin;
-        P->owned_skin = skin;
-      }
-      check_and_rebuild(*L, n, lattice, inv, P->d_pos.as<double>(), st);
-    }
+    ak_nlist* L = select_list(P, nl, n, lattice, inv, st);
     compute_forces(P->pot, L->nl, L->set, P->d_pos.as<double>(), P->d_spc.as<int>(),
                    P->d_F.as<double>(), P->d_out.as<double>(), st);
     double out[10];
@@ -518,6 +540,118 @@ int ak_compute(ak_potential* P, ak_nlist* nl, int32_t n, const double* lattice,
     AK_HIP(hipStreamSynchronize(st));
     *energy = out[0];
     if (stress && need_stress) std::memcpy(stress, out + 1, 9 * sizeof(double));
+  });
+}
+
+int ak_site_terms(ak_potential* P, ak_nlist* nl, int32_t n, const double* lattice,
+                  const double* inv, const double* positions, const int32_t* species,
+                  double* e_site, int64_t capacity, int32_t* centre, int32_t* neighbour,
+                  double* vec, double* dudv, int64_t* n_entries) {
+  return guarded([&] {
+    AK_REQUIRE(P && lattice && n > 0 && positions && n_entries, AK_ERR_ARG, "bad arguments");
+    AK_REQUIRE(!centre || (neighbour && vec && dudv && capacity >= 0), AK_ERR_ARG,
+               "bad entry buffers");
+    check_finite(lattice, 9, "lattice");
+    check_finite(positions, 3 * (size_t)n, "positions");
+    hipStream_t st = P->stream;
+    stage_species(P, n, species);
+    P->d_pos.ensure(sizeof(double) * 3 * n);
+    P->d_F.ensure(sizeof(double) * 3 * n);
+    P->d_out.ensure(sizeof(double) * 10);
+    AK_HIP(hipMemcpyAsync(P->d_pos.p, positions, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+                          st));
+    ak_nlist* L = select_list(P, nl, n, lattice, inv, st);
+    DevBuf d_es, d_vec;
+    d_es.ensure(sizeof(double) * n);
+    site_grads(P->pot, L->nl, L->set, P->d_pos.as<double>(), P->d_spc.as<int>(),
+               P->d_F.as<double>(), P->d_out.as<double>(), d_es.as<double>(), st);
+    const int S = L->nl.stride;
+    const size_t ns = (size_t)n * S;
+    d_vec.ensure(sizeof(double) * 3 * ns);
+    hipLaunchKernelGGL(k_site_vec, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st, n,
+                       (const double*)P->d_pos.as<double>(), L->set.geoms[0],
+                       (const int*)L->nl.cnt.as<int>(), (const int*)L->nl.nbr.as<int>(), S,
+                       d_vec.as<double>());
+    AK_HIP(hipGetLastError());
+    std::vector<int> cnt(n), nbr(ns);
+    std::vector<double> G(3 * ns), V(3 * ns);
+    std::vector<uint64_t> mask(n, ~uint64_t(0));
+    const bool masked = site_masked(P->pot, L->nl);
+    AK_HIP(hipMemcpyAsync(cnt.data(), L->nl.cnt.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+    AK_HIP(hipMemcpyAsync(nbr.data(), L->nl.nbr.p, sizeof(int) * ns, hipMemcpyDeviceToHost, st));
+    AK_HIP(hipMemcpyAsync(G.data(), P->pot.G.p, sizeof(double) * 3 * ns, hipMemcpyDeviceToHost,
+                          st));
+    AK_HIP(hipMemcpyAsync(V.data(), d_vec.p, sizeof(double) * 3 * ns, hipMemcpyDeviceToHost, st));
+    if (masked)
+      AK_HIP(hipMemcpyAsync(mask.data(), P->pot.amask.p, sizeof(uint64_t) * n,
+                            hipMemcpyDeviceToHost, st));
+    if (e_site)
+      AK_HIP(hipMemcpyAsync(e_site, d_es.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    AK_HIP(hipStreamSynchronize(st));
+    // entries: every bond within r_cut (engine/site_terms.py:84-97), atoms in
+    // order, each atom's bonds in list order (ascending neighbour index)
+    const double rc = P->pot.r_cut;
+    int64_t m = 0;
+    for (int a = 0; a < n; ++a)
+      for (int k = 0; k < cnt[a]; ++k) {
+        const size_t t = (size_t)a * S + k;
+        const double x = V[t], y = V[ns + t], z = V[2 * ns + t];
+        const double r = std::sqrt(x * x + y * y + z * z);
+        if (!(r < rc && r > 1e-12)) continue;
+        if (centre && m < capacity) {
+          const bool has_g = !masked || ((mask[a] >> k) & 1);
+          centre[m] = a;
+          neighbour[m] = nbr[t];
+          vec[3 * m] = x;
+          vec[3 * m + 1] = y;
+          vec[3 * m + 2] = z;
+          dudv[3 * m] = has_g ? G[t] : 0.0;
+          dudv[3 * m + 1] = has_g ? G[ns + t] : 0.0;
+          dudv[3 * m + 2] = has_g ? G[2 * ns + t] : 0.0;
+        }
+        ++m;
+      }
+    *n_entries = m;
+    AK_REQUIRE(!centre || m <= capacity, AK_ERR_CAPACITY, "site-term entry buffers too small");
+  });
+}
+
+int ak_heat_flux(ak_potential* P, ak_nlist* nl, int32_t n, const double* lattice,
+                 const double* inv, const double* positions, const double* velocities,
+                 const double* masses, const int32_t* species, double* flux, double* e_site) {
+  return guarded([&] {
+    AK_REQUIRE(P && lattice && n > 0 && positions && velocities && masses && flux, AK_ERR_ARG,
+               "bad arguments");
+    check_finite(lattice, 9, "lattice");
+    check_finite(positions, 3 * (size_t)n, "positions");
+    check_finite(velocities, 3 * (size_t)n, "velocities");
+    hipStream_t st = P->stream;
+    stage_species(P, n, species);
+    P->d_pos.ensure(sizeof(double) * 3 * n);
+    P->d_F.ensure(sizeof(double) * 3 * n);
+    P->d_out.ensure(sizeof(double) * 10);
+    AK_HIP(hipMemcpyAsync(P->d_pos.p, positions, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+                          st));
+    ak_nlist* L = select_list(P, nl, n, lattice, inv, st);
+    DevBuf d_es, d_vel, d_m, d_part, d_J;
+    d_es.ensure(sizeof(double) * n);
+    d_vel.ensure(sizeof(double) * 3 * n);
+    d_m.ensure(sizeof(double) * n);
+    d_part.ensure(sizeof(double) * 3 * ((size_t)n / 32 + 1));
+    d_J.ensure(sizeof(double) * 3);
+    AK_HIP(hipMemcpyAsync(d_vel.p, velocities, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+                          st));
+    AK_HIP(hipMemcpyAsync(d_m.p, masses, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    site_grads(P->pot, L->nl, L->set, P->d_pos.as<double>(), P->d_spc.as<int>(),
+               P->d_F.as<double>(), P->d_out.as<double>(), d_es.as<double>(), st);
+    heat_flux(P->pot, L->nl, L->set, P->d_pos.as<double>(), d_vel.as<double>(),
+              d_m.as<double>(), d_es.as<double>(), nullptr, kConvAmu, nullptr, 1,
+              d_part.as<double>(), d_J.as<double>(), st);
+    AK_HIP(hipGetLastError());
+    AK_HIP(hipMemcpyAsync(flux, d_J.p, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
+    if (e_site)
+      AK_HIP(hipMemcpyAsync(e_site, d_es.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    AK_HIP(hipStreamSynchronize(st));
   });
 }
 
@@ -718,10 +852,6 @@ int ak_compute_batched_device(ak_potential* P, int32_t S, const int64_t* off,
 }  // extern "C"
 
 namespace ak {
-void md_nve_run(Potential& pot, int n, const double* lattice, const double* inv, double* pos,
-                double* vel, const double* mass, const int32_t* spc, double dt, int n_steps,
-                double skin, bool use_graph, double* e_pot, double* e_kin, int* n_rebuilds,
-                double* seconds, hipStream_t st);
 void tersoff_loss_grad_run(Potential& pot, int S, const int64_t* off, const double* lattices,
                            const double* positions, const double* theta,
                            const double* ref_energy, const double* ref_forces,
@@ -737,6 +867,43 @@ void tersoff_fit_eval(Potential& pot, const double* theta, const double* weights
 }  // namespace ak
 
 extern "C" {
+
+int ak_md_run(ak_potential* P, int32_t n, const double* lattice, const double* inv,
+              double* positions, double* velocities, const double* masses,
+              const int32_t* species, double dt, int32_t n_steps, double skin,
+              int32_t use_graph, const double* thermostat, int32_t flux_every, double* e_pot,
+              double* e_kin, double* e_cons, double* flux, int32_t* n_rebuilds,
+              double* seconds) {
+  return guarded([&] {
+    AK_REQUIRE(P && lattice && positions && velocities && masses && n > 0, AK_ERR_ARG,
+               "bad arguments");
+    AK_REQUIRE(skin >= 0.0, AK_ERR_ARG, "skin_width must be >= 0");
+    AK_REQUIRE(flux_every >= 0, AK_ERR_ARG, "flux_every must be >= 0");
+    check_finite(lattice, 9, "lattice");
+    check_finite(positions, 3 * (size_t)n, "positions");
+    check_finite(velocities, 3 * (size_t)n, "velocities");
+    for (int a = 0; a < n; ++a) {
+      AK_REQUIRE(masses[a] > 0.0, AK_ERR_ARG, "masses must be positive");
+      if (species)
+        AK_REQUIRE(species[a] >= 0 && species[a] < P->pot.ns, AK_ERR_ARG,
+                   "species index out of range for this potential");
+    }
+    MDOptions opt;
+    if (thermostat) {
+      opt.thermostat = (int)thermostat[0];
+      AK_REQUIRE(thermostat[0] == (double)opt.thermostat, AK_ERR_ARG, "bad thermostat kind");
+      opt.temperature = thermostat[1];
+      opt.tau = thermostat[2];
+      opt.chain = (int)thermostat[3];
+      AK_REQUIRE(thermostat[3] == (double)opt.chain, AK_ERR_ARG, "bad chain length");
+    }
+    opt.flux_every = flux_every;
+    int nr = 0;
+    md_run(P->pot, n, lattice, inv, positions, velocities, masses, species, dt, n_steps, skin,
+           use_graph != 0, opt, e_pot, e_kin, e_cons, flux, &nr, seconds, P->stream);
+    if (n_rebuilds) *n_rebuilds = nr;
+  });
+}
 
 int ak_md_nve(ak_potential* P, int32_t n, const double* lattice, const double* inv,
               double* positions, double* velocities, const double* masses,
@@ -757,8 +924,9 @@ int ak_md_nve(ak_potential* P, int32_t n, const double* lattice, const double* i
                    "species index out of range for this potential");
     }
     int nr = 0;
-    md_nve_run(P->pot, n, lattice, inv, positions, velocities, masses, species, dt, n_steps,
-               skin, use_graph != 0, e_pot, e_kin, &nr, seconds, P->stream);
+    md_run(P->pot, n, lattice, inv, positions, velocities, masses, species, dt, n_steps, skin,
+           use_graph != 0, MDOptions{}, e_pot, e_kin, nullptr, nullptr, &nr, seconds,
+           P->stream);
     if (n_rebuilds) *n_rebuilds = nr;
   });
 }

@@ -29,6 +29,22 @@ struct Geom {
   int pad;
 };
 
+// Minimum image with contraction allowed (force evaluation).
+__device__ __forceinline__ void mic_fast(const Geom& g, const double* __restrict__ pos, int i,
+                                         int j, double& dx, double& dy, double& dz) {
+  double rx = pos[3 * j] - pos[3 * i], ry = pos[3 * j + 1] - pos[3 * i + 1],
+         rz = pos[3 * j + 2] - pos[3 * i + 2];
+  double sx = rx * g.inv[0] + ry * g.inv[3] + rz * g.inv[6];
+  double sy = rx * g.inv[1] + ry * g.inv[4] + rz * g.inv[7];
+  double sz = rx * g.inv[2] + ry * g.inv[5] + rz * g.inv[8];
+  sx -= rint(sx);
+  sy -= rint(sy);
+  sz -= rint(sz);
+  dx = sx * g.lat[0] + sy * g.lat[3] + sz * g.lat[6];
+  dy = sx * g.lat[1] + sy * g.lat[4] + sz * g.lat[7];
+  dz = sx * g.lat[2] + sy * g.lat[5] + sz * g.lat[8];
+}
+
 // Minimum-image displacement pj - pi in the structure's cell, with the
 // reference's operation order (engine/neighborlist.py:350-361): fractional
 // components rounded half-to-even (rint), no fused multiply-add, so the

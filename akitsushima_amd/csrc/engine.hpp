@@ -196,8 +196,50 @@ void eam_feval(const Potential& pot, int f, int64_t m, const double* d_x, double
 
 // Forces on all atoms of `set`, per-structure energy and stress.
 // d_out: (n_struct, 10) = [E, S00..S22]; d_F (n_atoms, 3).
+// Boltzmann constant in eV/K, K_B_SI * EV_PER_JOULE (constants.py:4-13)
+constexpr double kBoltzmannEv = 1.380649e-23 * 6.241509074e18;
+
+// eV/A per amu -> A/fs^2: (1.60217663e-19 / 1e-10) / 1.66053906660e-27 * 1e-20
+// (the reference's EV_ANG_TO_AMU_ANG_FS2, constants.py:20)
+constexpr double kConvAmu = (1.60217663e-19 / 1.0e-10) / (1.66053906660e-27) * 1.0e-20;
+
+// d_esite (n, nullable): per-atom site energies U_a.
 void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
-                    const int* d_spc, double* d_F, double* d_out, hipStream_t st);
+                    const int* d_spc, double* d_F, double* d_out, hipStream_t st,
+                    double* d_esite = nullptr);
+
+// Forces, per-atom site energies and the site gradients G[a, k] = dU_a/dv_ak
+// in pot.G for every kind (forces.hip); site_masked: whether only the slots
+// whose amask bit is set hold G (else every slot below cnt does).
+void site_grads(Potential& pot, NList& nl, SetDev& set, const double* d_pos, const int* d_spc,
+                double* d_F, double* d_out, double* d_esite, hipStream_t st);
+bool site_masked(const Potential& pot, const NList& nl);
+
+// Device MD options: thermostat 0 = none (NVE), 1 = Nose-Hoover chain of
+// `chain` links at `temperature` K with time constant `tau` fs; heat flux of
+// the state after every step s with s % flux_every == 0 (0 = none).
+struct MDOptions {
+  int thermostat = 0;
+  double temperature = 0.0, tau = 0.0;
+  int chain = 3;
+  int flux_every = 0;
+};
+
+// md.hip: n_steps of velocity Verlet on one cell, positions and velocities
+// updated in place; e_pot, e_kin, e_cons (n_steps + 1) and flux
+// ((n_steps + flux_every - 1) / flux_every, 3) as recorded on the device.
+void md_run(Potential& pot, int n, const double* lattice, const double* inv, double* pos,
+            double* vel, const double* mass, const int32_t* spc, double dt, int n_steps,
+            double skin, bool use_graph, const MDOptions& opt, double* e_pot, double* e_kin,
+            double* e_cons, double* flux, int* n_rebuilds, double* seconds, hipStream_t st);
+
+// Heat flux J = [sum_a E_a v_a - sum_(a,k) v_ak (G[a,k] . v_nbr)] / V of one
+// cell after site_grads (d_part: one triple per 32 atoms).  With d_gate, only
+// on steps counters[2] % every == 0, into d_out[3 * (step / every)].
+void heat_flux(Potential& pot, NList& nl, SetDev& set, const double* d_pos, const double* d_vel,
+               const double* d_mass, const double* d_esite, const double* d_vscale,
+               double conv, int* d_gate, int every, double* d_part, double* d_out,
+               hipStream_t st);
 
 inline int pow2ceil(int x) {
   int p = 1;

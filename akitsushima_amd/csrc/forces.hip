@@ -95,22 +95,6 @@ __device__ __forceinline__ double xpow(double x, double e) {
   return pow(x, e);
 }
 
-// Minimum image with contraction allowed (force evaluation).
-__device__ __forceinline__ void mic_fast(const Geom& g, const double* __restrict__ pos, int i,
-                                         int j, double& dx, double& dy, double& dz) {
-  double rx = pos[3 * j] - pos[3 * i], ry = pos[3 * j + 1] - pos[3 * i + 1],
-         rz = pos[3 * j + 2] - pos[3 * i + 2];
-  double sx = rx * g.inv[0] + ry * g.inv[3] + rz * g.inv[6];
-  double sy = rx * g.inv[1] + ry * g.inv[4] + rz * g.inv[7];
-  double sz = rx * g.inv[2] + ry * g.inv[5] + rz * g.inv[8];
-  sx -= rint(sx);
-  sy -= rint(sy);
-  sz -= rint(sz);
-  dx = sx * g.lat[0] + sy * g.lat[3] + sz * g.lat[6];
-  dy = sx * g.lat[1] + sy * g.lat[4] + sz * g.lat[7];
-  dz = sx * g.lat[2] + sy * g.lat[5] + sz * g.lat[8];
-}
-
 // Block sums of 10 values (energy, virial) of launch chunk c: DPP wavefront
 // sums, one LDS exchange of the wavefront totals, combined in fixed order;
 // thread q < 10 writes value q to partial[q * n_chunks + c].  A chunk is the
@@ -182,7 +166,7 @@ __device__ __forceinline__ void k_tersoff_site_chunk(int c, int n_chunks,
     const double* __restrict__ tb, const double* __restrict__ tri, double rc,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
-    uint64_t* __restrict__ amask) {
+    uint64_t* __restrict__ amask, double* __restrict__ esite) {
   // one species: every table row is the first, read through uniform (scalar)
   // loads, and the leg cutoff equals the pair cutoff
   const int ns = SP1 ? 1 : ns_arg;
@@ -465,6 +449,10 @@ __device__ __forceinline__ void k_tersoff_site_chunk(int c, int n_chunks,
     G[plane + slot] = Gy;
     G[2 * plane + slot] = Gz;
   }
+  if (esite) {  // per-atom site energy U_a (uniform branch: every lane joins the sum)
+    const double ea = seg_sum<L>(e);
+    if (live && k == 0) esite[a] = ea;
+  }
   block_partial(e, Gx, Gy, Gz, ux, uy, uz, partial, c, n_chunks);
 }
 
@@ -474,10 +462,10 @@ __global__ __launch_bounds__(256) void k_tersoff_site(int n_chunks, int n, const
     const double* __restrict__ tb, const double* __restrict__ tri, double rc,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
-    uint64_t* __restrict__ amask) {
+    uint64_t* __restrict__ amask, double* __restrict__ esite) {
   for (int cc = blockIdx.x; cc < n_chunks; cc += gridDim.x)
     k_tersoff_site_chunk<L, SP1>(xcd_remap(cc, n_chunks), n_chunks, n, pos, sid, geoms, single, spc, ns_arg, tb, tri, rc, cnt,
-                                nbr, stride, chunk_rng, G, partial, amask);
+                                nbr, stride, chunk_rng, G, partial, amask, esite);
 }
 
 // ---------------------------------------------------------------------------
@@ -675,7 +663,7 @@ __device__ __forceinline__ void k_sw_site_chunk(int c, int n_chunks,
     const Geom* __restrict__ geoms, int single, const double* __restrict__ prm,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
-    uint64_t* __restrict__ amask) {
+    uint64_t* __restrict__ amask, double* __restrict__ esite) {
   const int a = chunk_rng[2 * c] + (int)threadIdx.x / L;
   const int k = threadIdx.x & (L - 1);
   const int seg = (threadIdx.x & 63) & ~(L - 1);
@@ -770,6 +758,10 @@ __device__ __forceinline__ void k_sw_site_chunk(int c, int n_chunks,
     G[plane + slot] = Gy;
     G[2 * plane + slot] = Gz;
   }
+  if (esite) {  // per-atom site energy U_a (uniform branch: every lane joins the sum)
+    const double ea = seg_sum<L>(e);
+    if (live && k == 0) esite[a] = ea;
+  }
   block_partial(e, Gx, Gy, Gz, ux, uy, uz, partial, c, n_chunks);
 }
 
@@ -779,10 +771,10 @@ __global__ __launch_bounds__(256) void k_sw_site(
     const Geom* __restrict__ geoms, int single, const double* __restrict__ prm,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
-    uint64_t* __restrict__ amask) {
+    uint64_t* __restrict__ amask, double* __restrict__ esite) {
   for (int cc = blockIdx.x; cc < n_chunks; cc += gridDim.x)
     k_sw_site_chunk<L>(xcd_remap(cc, n_chunks), n_chunks, n, pos, sid, geoms, single, prm, cnt, nbr, stride,
-                       chunk_rng, G, partial, amask);
+                       chunk_rng, G, partial, amask, esite);
 }
 
 // Per-structure energy and stress of structure s by one block of 256
@@ -897,7 +889,7 @@ __device__ __forceinline__ void k_sw_site_c_chunk(
     const Geom* __restrict__ geoms, int single, const double* __restrict__ prm,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
-    uint64_t* __restrict__ amask, double* __restrict__ lds) {
+    uint64_t* __restrict__ amask, double* __restrict__ lds, double* __restrict__ esite) {
   static_assert(L >= LA && L % LA == 0 && LA <= 64, "LA lanes cover the row in L / LA rounds");
   const int apb = blockDim.x / LA;          // atoms per block
   const int al = (int)threadIdx.x / LA;     // atom of this lane within the block
@@ -1002,6 +994,10 @@ __device__ __forceinline__ void k_sw_site_c_chunk(
     v[8] -= Gz * uy;
     v[9] -= Gz * uz;
   }
+  if (esite) {  // per-atom site energy U_a
+    const double ea = seg_sum<LA>(v[0]);
+    if (live && q == 0) esite[a] = ea;
+  }
   wave_partial(v, partial, c, n_chunks);
 }
 
@@ -1011,11 +1007,11 @@ __global__ __launch_bounds__(256) void k_sw_site_c(
     const Geom* __restrict__ geoms, int single, const double* __restrict__ prm,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
-    uint64_t* __restrict__ amask) {
+    uint64_t* __restrict__ amask, double* __restrict__ esite) {
   extern __shared__ double lds_sw[];
   for (int cc = blockIdx.x; cc < n_chunks; cc += gridDim.x)
     k_sw_site_c_chunk<L, LA>(xcd_remap(cc, n_chunks), n_chunks, n, pos, sid, geoms, single, prm, cnt, nbr, stride,
-                             chunk_rng, G, partial, amask, lds_sw);
+                             chunk_rng, G, partial, amask, lds_sw, esite);
 }
 
 // Gather for long rows (stride L >= 16): LA lanes per atom, lane q summing
@@ -1115,7 +1111,8 @@ __device__ __forceinline__ void k_sw_multi_chunk(
     const double* __restrict__ tpair, const double* __restrict__ tleg, double rc,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
-    uint64_t* __restrict__ amask, double* __restrict__ lds, const double* __restrict__ s_tri) {
+    uint64_t* __restrict__ amask, double* __restrict__ lds, const double* __restrict__ s_tri,
+    double* __restrict__ esite) {
   static_assert(L >= LA && L % LA == 0 && LA <= 64, "LA lanes cover the row in L / LA rounds");
   const int apb = blockDim.x / LA;
   const int al = (int)threadIdx.x / LA;
@@ -1253,6 +1250,10 @@ __device__ __forceinline__ void k_sw_multi_chunk(
     v[8] -= Gz * uy;
     v[9] -= Gz * uz;
   }
+  if (esite) {  // per-atom site energy U_a
+    const double ea = seg_sum<LA>(v[0]);
+    if (live && q == 0) esite[a] = ea;
+  }
   wave_partial(v, partial, c, n_chunks);
 }
 
@@ -1262,7 +1263,8 @@ __global__ __launch_bounds__(256) void k_sw_multi(
     const Geom* __restrict__ geoms, int single, const int* __restrict__ spc, int ns,
     const double* __restrict__ tables, double rc, const int* __restrict__ cnt,
     const int* __restrict__ nbr, int stride, const int* __restrict__ chunk_rng,
-    double* __restrict__ G, double* __restrict__ partial, uint64_t* __restrict__ amask) {
+    double* __restrict__ G, double* __restrict__ partial, uint64_t* __restrict__ amask,
+    double* __restrict__ esite) {
   extern __shared__ double lds_swm[];
   const int n2 = ns * ns;
   const double* tpair = tables;
@@ -1275,7 +1277,7 @@ __global__ __launch_bounds__(256) void k_sw_multi(
   for (int cc = blockIdx.x; cc < n_chunks; cc += gridDim.x)
     k_sw_multi_chunk<L, LA>(xcd_remap(cc, n_chunks), n_chunks, n, pos, sid, geoms, single, spc,
                             ns, tpair, tleg, rc, cnt, nbr, stride, chunk_rng, G, partial, amask,
-                            lds_swm, s_tri);
+                            lds_swm, s_tri, esite);
 }
 
 // ---------------------------------------------------------------------------
@@ -1552,7 +1554,7 @@ __device__ __forceinline__ void k_eam_force_chunk(int c, int n_chunks,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const double* __restrict__ dF, const double* __restrict__ Fe,
     const int* __restrict__ chunk_rng, double* __restrict__ F, double* __restrict__ partial,
-    const double* __restrict__ fast) {
+    const double* __restrict__ fast, double* __restrict__ esite) {
   const int a = chunk_rng[2 * c] + (int)threadIdx.x / LE;
   const int l = threadIdx.x & (LE - 1);
   const bool live = a < chunk_rng[2 * c + 1];
@@ -1607,6 +1609,10 @@ __device__ __forceinline__ void k_eam_force_chunk(int c, int n_chunks,
     F[3 * (size_t)a + 1] = fy;
     F[3 * (size_t)a + 2] = fz;
   }
+  if (esite) {  // per-atom site energy U_a = F_a + 1/2 sum_j phi_aj
+    const double ea = seg_sum<LE>(e);
+    if (live && l == 0) esite[a] = ea;
+  }
   double v[10] = {e, w[0], w[1], w[2], w[1], w[3], w[4], w[2], w[4], w[5]};
   wave_partial(v, partial, c, n_chunks);
 }
@@ -1618,9 +1624,9 @@ __global__ __launch_bounds__(256) void k_eam_force(int n_chunks, const double* _
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const double* __restrict__ dF, const double* __restrict__ Fe,
     const int* __restrict__ chunk_rng, double* __restrict__ F, double* __restrict__ partial,
-    const double* __restrict__ fast) {
+    const double* __restrict__ fast, double* __restrict__ esite) {
   for (int cc = blockIdx.x; cc < n_chunks; cc += gridDim.x)
-    k_eam_force_chunk<LE>(xcd_remap(cc, n_chunks), n_chunks, pos, sid, geoms, single, spc, ns, fn, map, rc, cnt, nbr, stride, dF, Fe, chunk_rng, F, partial, fast);
+    k_eam_force_chunk<LE>(xcd_remap(cc, n_chunks), n_chunks, pos, sid, geoms, single, spc, ns, fn, map, rc, cnt, nbr, stride, dF, Fe, chunk_rng, F, partial, fast, esite);
 }
 
 // Function f of the table at m arguments (tabulation of a potential).
@@ -1675,7 +1681,8 @@ static inline dim3 chunk_grid(const Potential& pot, const SetDev::Chunks& ch) {
 template <int L>
 static void launch_bond_kernels(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
                                 const int* d_spc, double* d_F, double* d_out,
-                                SetDev::Chunks& ch, bool trip, int blk, hipStream_t st) {
+                                SetDev::Chunks& ch, bool trip, int blk, double* esite,
+                                hipStream_t st) {
   const int n = set.n_atoms;
   const int single = set.n_struct == 1 ? 1 : 0;
   double* G = pot.G.as<double>();
@@ -1703,22 +1710,22 @@ static void launch_bond_kernels(Potential& pot, NList& nl, SetDev& set, const do
     if (!done)
       launch_timed(pot, sp1 ? k_tersoff_site<L, true> : k_tersoff_site<L, false>, grid, block, st,
                    (size_t)0, ch.n, n, d_pos, sid, geoms, single, d_spc, pot.ns, tb, tri, pot.r_cut, cnt,
-                   nbr, nl.stride, rng, G, part, amask);
+                   nbr, nl.stride, rng, G, part, amask, esite);
   } else if (pot.sw_multi) {
     constexpr int LA = L < kRowLanes ? L : kRowLanes;
     launch_timed(pot, k_sw_multi<L, LA>, grid, block, st, sw_m_lds(blk, L, LA, pot.ns), ch.n, n,
                  d_pos, sid, geoms, single, d_spc, pot.ns,
                  (const double*)pot.d_params.as<double>(), pot.r_cut, cnt, nbr, nl.stride, rng, G,
-                 part, amask);
+                 part, amask, esite);
   } else if constexpr (L >= kLongRow) {
     const size_t lds = sw_c_lds(blk, L);
     launch_timed(pot, k_sw_site_c<L, kRowLanes>, grid, block, st, lds, ch.n, n, d_pos, sid,
                  geoms, single, (const double*)pot.d_params.as<double>(), cnt, nbr, nl.stride,
-                 rng, G, part, amask);
+                 rng, G, part, amask, esite);
   } else {
     launch_timed(pot, k_sw_site<L>, grid, block, st, (size_t)0, ch.n, n, d_pos, sid, geoms,
                  single, (const double*)pot.d_params.as<double>(), cnt, nbr, nl.stride, rng, G,
-                 part, amask);
+                 part, amask, esite);
   }
   const int* rev = nl.rev.as<int>();
   const int* first = ch.first.as<int>();
@@ -1743,7 +1750,8 @@ static void launch_bond_kernels(Potential& pot, NList& nl, SetDev& set, const do
 
 template <int LE>
 static void launch_eam(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
-                       const int* d_spc, double* d_F, SetDev::Chunks& ch, hipStream_t st) {
+                       const int* d_spc, double* d_F, SetDev::Chunks& ch, double* esite,
+                       hipStream_t st) {
   const int n = set.n_atoms;
   const int64_t threads = (int64_t)n * LE;
   const unsigned grid = (unsigned)((threads + 255) / 256);
@@ -1761,7 +1769,7 @@ static void launch_eam(Potential& pot, NList& nl, SetDev& set, const double* d_p
                (const int*)pot.d_map.as<int>(), pot.r_cut, (const int*)nl.cnt.as<int>(),
                (const int*)nl.nbr.as<int>(), nl.stride, (const double*)dF, (const double*)Fe,
                (const int*)ch.rng.as<int>(), d_F, ch.partial.as<double>(),
-               (const double*)(pot.eam_fast ? pot.d_eam_fast.as<double>() : nullptr));
+               (const double*)(pot.eam_fast ? pot.d_eam_fast.as<double>() : nullptr), esite);
 }
 
 bool eam_fast_table(int ns, const std::vector<double>& rows, const std::vector<int>& map,
@@ -1815,7 +1823,8 @@ bool eam_fast_table(int ns, const std::vector<double>& rows, const std::vector<i
 }
 
 void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
-                    const int* d_spc, double* d_F, double* d_out, hipStream_t st) {
+                    const int* d_spc, double* d_F, double* d_out, hipStream_t st,
+                    double* esite) {
   const int n = set.n_atoms;
   AK_REQUIRE(nl.valid, AK_ERR_STATE, "neighbour list has not been built");
   AK_REQUIRE(nl.n_atoms == n, AK_ERR_STATE, "neighbour list was built for another size");
@@ -1831,7 +1840,9 @@ void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
   }
   // lanes per atom of the site kernel: L, or L * L for the Tersoff form with
   // one lane per (bond, partner) pair
-  const bool trip = pot.kind == KIND_TERSOFF && lanes * lanes <= 64 && pot.bond_tile == 2;
+  // per-atom site energies come from the lane-per-bond Tersoff form
+  const bool trip =
+      pot.kind == KIND_TERSOFF && lanes * lanes <= 64 && pot.bond_tile == 2 && !esite;
   const bool swm = pot.kind == KIND_SW && pot.sw_multi;
   const bool sw_long = pot.kind == KIND_SW && !swm && lanes >= kLongRow;
   const int atom_lanes = trip ? lanes * lanes
@@ -1847,21 +1858,21 @@ void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
     if (pot.kind == KIND_EAM) {
       pot.aux.ensure(sizeof(double) * 2 * (size_t)n);
       switch (lanes) {
-        case 4: launch_eam<4>(pot, nl, set, d_pos, d_spc, d_F, ch, st); break;
-        case 8: launch_eam<8>(pot, nl, set, d_pos, d_spc, d_F, ch, st); break;
-        case 32: launch_eam<32>(pot, nl, set, d_pos, d_spc, d_F, ch, st); break;
-        case 64: launch_eam<64>(pot, nl, set, d_pos, d_spc, d_F, ch, st); break;
-        default: launch_eam<16>(pot, nl, set, d_pos, d_spc, d_F, ch, st); break;
+        case 4: launch_eam<4>(pot, nl, set, d_pos, d_spc, d_F, ch, esite, st); break;
+        case 8: launch_eam<8>(pot, nl, set, d_pos, d_spc, d_F, ch, esite, st); break;
+        case 32: launch_eam<32>(pot, nl, set, d_pos, d_spc, d_F, ch, esite, st); break;
+        case 64: launch_eam<64>(pot, nl, set, d_pos, d_spc, d_F, ch, esite, st); break;
+        default: launch_eam<16>(pot, nl, set, d_pos, d_spc, d_F, ch, esite, st); break;
       }
     } else {
       pot.G.ensure(sizeof(double) * 3 * (size_t)n * nl.stride);
       pot.amask.ensure(sizeof(uint64_t) * (size_t)n);
       switch (lanes) {
-        case 4: launch_bond_kernels<4>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, blk, st); break;
-        case 8: launch_bond_kernels<8>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, blk, st); break;
-        case 16: launch_bond_kernels<16>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, blk, st); break;
-        case 32: launch_bond_kernels<32>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, blk, st); break;
-        case 64: launch_bond_kernels<64>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, blk, st); break;
+        case 4: launch_bond_kernels<4>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, blk, esite, st); break;
+        case 8: launch_bond_kernels<8>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, blk, esite, st); break;
+        case 16: launch_bond_kernels<16>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, blk, esite, st); break;
+        case 32: launch_bond_kernels<32>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, blk, esite, st); break;
+        case 64: launch_bond_kernels<64>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, blk, esite, st); break;
         default: throw Error(AK_ERR_STATE, "unsupported neighbour stride");
       }
     }
@@ -1870,6 +1881,174 @@ void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
     hipLaunchKernelGGL(k_struct_final, dim3(set.n_struct), dim3(256), 0, st,
                        set.d_geoms.as<Geom>(), (const int*)ch.first.as<int>(), ch.n,
                        (const double*)ch.partial.as<double>(), d_out);
+}
+
+// ---------------------------------------------------------------------------
+// Site terms and the heat flux (engine/site_terms.py:1-60, 179-475;
+// solvers/thermal_conductivity.py:168-303).  With E = sum_a U_a and U_a a
+// function of the bond vectors v_ak = r_nbr(a,k) - r_a, the bond kernels'
+// G[a, k] = dU_a / dv_ak is the reference's dUdv of the entry (a -> nbr).
+// EAM: dU_a / dv_ak = (F'_a rho'_{s_j}(r) + phi'_aj(r) / 2) v / r.
+// ---------------------------------------------------------------------------
+template <int LE>
+__global__ __launch_bounds__(256) void k_eam_site_g(
+    int n, const double* __restrict__ pos, const int* __restrict__ sid,
+    const Geom* __restrict__ geoms, int single, const int* __restrict__ spc, int ns,
+    const double* __restrict__ fn, const int* __restrict__ map, double rc,
+    const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
+    const double* __restrict__ dF, double* __restrict__ G) {
+  const int gt = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int a = gt / LE;
+  const int l = gt & (LE - 1);
+  if (a >= n) return;
+  const Geom& g = geoms[single ? 0 : sid[a]];
+  const int nk = cnt[a];
+  const int sa = ns > 1 ? spc[a] : 0;
+  const double dFa = dF[a];
+  const size_t plane = (size_t)n * stride;
+  for (int k = l; k < nk; k += LE) {
+    const size_t slot = (size_t)a * stride + k;
+    const int j = nbr[slot];
+    double ux, uy, uz;
+    mic_fast(g, pos, a, j, ux, uy, uz);
+    const double r = sqrt(ux * ux + uy * uy + uz * uz);
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    if (r < rc && r > 1e-12) {
+      const int sj = ns > 1 ? spc[j] : 0;
+      double v, dfj, phi, dphi;
+      z_feval(fn, fn + (size_t)ZW * map[sj], r, v, dfj);
+      z_feval(fn, fn + (size_t)ZW * map[2 * ns + sa * ns + sj], r, phi, dphi);
+      const double s = (dFa * dfj + 0.5 * dphi) / r;
+      gx = s * ux;
+      gy = s * uy;
+      gz = s * uz;
+    }
+    G[slot] = gx;
+    G[plane + slot] = gy;
+    G[2 * plane + slot] = gz;
+  }
+}
+
+// Per-block sums of V J = sum_a E_a v_a - sum_(a,k) v_ak (G[a, k] . vel_nbr),
+// E_a = m_a |v_a|^2 / (2 conv) + U_a (Fan et al., PRB 92, 094301 (2015),
+// eq. 21, per neighbour entry).  vscale (nullable) multiplies every
+// velocity; masked: only slots whose amask bit is set hold G.  With
+// gate != nullptr the kernel runs only when counters[2] % every == 0.
+template <int LE>
+__global__ __launch_bounds__(256) void k_heat_flux(
+    int n, const double* __restrict__ pos, const double* __restrict__ vel,
+    const double* __restrict__ mass, const double* __restrict__ esite,
+    const double* __restrict__ vscale, const int* __restrict__ sid,
+    const Geom* __restrict__ geoms, int single, double rc, const int* __restrict__ cnt,
+    const int* __restrict__ nbr, int stride, const double* __restrict__ G,
+    const uint64_t* __restrict__ amask, int masked, double conv,
+    const int* __restrict__ gate, int every, double* __restrict__ part) {
+  if (gate && gate[2] % every != 0) return;
+  __shared__ double smem[16 * 3];
+  const int a = (int)(blockIdx.x * blockDim.x + threadIdx.x) / LE;
+  const int l = threadIdx.x & (LE - 1);
+  double J[3] = {0.0, 0.0, 0.0};
+  if (a < n) {
+    const double sc = vscale ? vscale[0] : 1.0;
+    const Geom& g = geoms[single ? 0 : sid[a]];
+    const int nk = cnt[a];
+    const size_t plane = (size_t)n * stride;
+    const uint64_t ma = masked ? amask[a] : ~uint64_t(0);
+    for (int k = l; k < nk; k += LE) {
+      if (masked && !((ma >> k) & 1)) continue;
+      const size_t slot = (size_t)a * stride + k;
+      const int j = nbr[slot];
+      double ux, uy, uz;
+      mic_fast(g, pos, a, j, ux, uy, uz);
+      const double r2 = ux * ux + uy * uy + uz * uz;
+      if (!(r2 < rc * rc) || !(r2 > 1e-24)) continue;
+      const double gv = (G[slot] * vel[3 * (size_t)j] + G[plane + slot] * vel[3 * (size_t)j + 1] +
+                         G[2 * plane + slot] * vel[3 * (size_t)j + 2]) *
+                        sc;
+      J[0] -= ux * gv;
+      J[1] -= uy * gv;
+      J[2] -= uz * gv;
+    }
+    if (l == 0) {
+      const double vx = vel[3 * (size_t)a] * sc, vy = vel[3 * (size_t)a + 1] * sc,
+                   vz = vel[3 * (size_t)a + 2] * sc;
+      const double E = 0.5 * mass[a] * (vx * vx + vy * vy + vz * vz) / conv + esite[a];
+      J[0] += E * vx;
+      J[1] += E * vy;
+      J[2] += E * vz;
+    }
+  }
+  block_sum<3>(J, smem);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = J[0];
+    part[3 * blockIdx.x + 1] = J[1];
+    part[3 * blockIdx.x + 2] = J[2];
+  }
+}
+
+// J = (sum of the block sums, fixed order) / V -> out[0..2]; with gate,
+// into out[3 * (counters[2] / every)] on record steps, and counters[2]
+// advances every call.
+__global__ __launch_bounds__(256) void k_heat_flux_final(int nb, const double* __restrict__ part,
+                                                         double inv_volume, int* __restrict__ gate,
+                                                         int every, double* __restrict__ out) {
+  __shared__ double smem[16 * 3];
+  const int step = gate ? gate[2] : 0;
+  const bool rec = !gate || step % every == 0;
+  double J[3] = {0.0, 0.0, 0.0};
+  if (rec)
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+      J[0] += part[3 * b];
+      J[1] += part[3 * b + 1];
+      J[2] += part[3 * b + 2];
+    }
+  block_sum<3>(J, smem);
+  if (threadIdx.x == 0) {
+    if (rec) {
+      double* o = out + (gate ? 3 * (size_t)(step / every) : 0);
+      o[0] = J[0] * inv_volume;
+      o[1] = J[1] * inv_volume;
+      o[2] = J[2] * inv_volume;
+    }
+    if (gate) gate[2] = step + 1;
+  }
+}
+
+bool site_masked(const Potential& pot, const NList& nl) {
+  return pot.kind != KIND_EAM && (nl.stride >= 8 || pot.sw_multi);
+}
+
+void site_grads(Potential& pot, NList& nl, SetDev& set, const double* d_pos, const int* d_spc,
+                double* d_F, double* d_out, double* d_esite, hipStream_t st) {
+  compute_forces(pot, nl, set, d_pos, d_spc, d_F, d_out, st, d_esite);
+  if (pot.kind != KIND_EAM || set.n_atoms == 0) return;
+  const int n = set.n_atoms;
+  pot.G.ensure(sizeof(double) * 3 * (size_t)n * nl.stride);
+  constexpr int LE = 8;
+  const unsigned grid = (unsigned)(((int64_t)n * LE + 255) / 256);
+  hipLaunchKernelGGL(k_eam_site_g<LE>, dim3(grid), dim3(256), 0, st, n, d_pos,
+                     (const int*)set.d_sid.as<int>(), (const Geom*)set.d_geoms.as<Geom>(),
+                     set.n_struct == 1 ? 1 : 0, d_spc, pot.ns,
+                     (const double*)pot.d_params.as<double>(), (const int*)pot.d_map.as<int>(),
+                     pot.r_cut, (const int*)nl.cnt.as<int>(), (const int*)nl.nbr.as<int>(),
+                     nl.stride, (const double*)pot.aux.as<double>(), pot.G.as<double>());
+}
+
+void heat_flux(Potential& pot, NList& nl, SetDev& set, const double* d_pos, const double* d_vel,
+               const double* d_mass, const double* d_esite, const double* d_vscale,
+               double conv, int* d_gate, int every, double* d_part, double* d_out,
+               hipStream_t st) {
+  const int n = set.n_atoms;
+  constexpr int LE = 8;
+  const unsigned grid = (unsigned)(((int64_t)n * LE + 255) / 256);
+  hipLaunchKernelGGL(k_heat_flux<LE>, dim3(grid), dim3(256), 0, st, n, d_pos, d_vel, d_mass,
+                     d_esite, d_vscale, (const int*)set.d_sid.as<int>(),
+                     (const Geom*)set.d_geoms.as<Geom>(), set.n_struct == 1 ? 1 : 0, pot.r_cut,
+                     (const int*)nl.cnt.as<int>(), (const int*)nl.nbr.as<int>(), nl.stride,
+                     (const double*)pot.G.as<double>(), (const uint64_t*)pot.amask.as<uint64_t>(),
+                     site_masked(pot, nl) ? 1 : 0, conv, (const int*)d_gate, every, d_part);
+  hipLaunchKernelGGL(k_heat_flux_final, dim3(1), dim3(256), 0, st, (int)grid, (const double*)d_part,
+                     1.0 / set.geoms[0].volume, d_gate, every, d_out);
 }
 
 }  // namespace ak

@@ -1,11 +1,16 @@
-// Device-resident NVE molecular dynamics (velocity Verlet), after
+// Device-resident molecular dynamics (velocity Verlet), after
 // NVESimulation.run (solvers/molecular_dynamics.py:1969-1996):
 //     v += dt/2 a;  x += dt v;  wrap into the cell;  list check/rebuild;
 //     F = F(x);  a = F conv / m;  v += dt/2 a
-// with conv = EV_ANG_TO_AMU_ANG_FS2 (constants.py:20).  The whole step runs on
-// the device: the Verlet-skin test (engine/neighborlist.py:653-673, fixed
-// cell) sets a device flag and the list kernels return at once when it is
-// clear, so a step can be captured once in a hipGraph and replayed.
+// with conv = EV_ANG_TO_AMU_ANG_FS2 (constants.py:20), and NVTSimulation.run
+// (:1113-1160): the same step between two half-steps of an M-link
+// Nose-Hoover chain (_nhc_step, :635-727).  The whole step runs on the
+// device: the Verlet-skin test (engine/neighborlist.py:653-673, fixed cell)
+// sets a device flag and the list kernels return at once when it is clear,
+// so a step can be captured once in a hipGraph and replayed.  Optionally the
+// heat flux of the state after each step is recorded on the device every
+// `every` steps (ThermalConductivityCalculator.run_md,
+// solvers/thermal_conductivity.py:575-610).
 #include <chrono>
 #include <cstring>
 #include <vector>
@@ -14,26 +19,29 @@
 
 namespace ak {
 
-// (1.60217663e-19 / 1e-10) / 1.66053906660e-27 * 1e-20
-constexpr double kConv = (1.60217663e-19 / 1.0e-10) / (1.66053906660e-27) * 1.0e-20;
+constexpr double kConv = kConvAmu;
 
 // Half kick, drift and wrap into the cell, and this block's largest squared
 // displacement since the list was built (minimum image of x - x_build):
 //   v += dt/2 a; x += dt v; frac = x @ inv; frac -= floor(frac);
 //   frac[frac >= 1] = 0; x = frac @ lat.
+// vscale (nullable): the thermostat's velocity scale, applied first.
 __global__ __launch_bounds__(256) void k_md_drift(int n, double* __restrict__ pos,
                                                   double* __restrict__ vel,
                                                   const double* __restrict__ acc, double dt,
                                                   Geom g, const double* __restrict__ ref,
-                                                  double* __restrict__ part) {
+                                                  double* __restrict__ part,
+                                                  const double* __restrict__ vscale) {
   __shared__ double m[256];
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   double d2 = 0.0;
   if (a < n) {
+    const double sc = vscale ? vscale[0] : 1.0;
     double x[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      double v = vel[3 * a + d] + 0.5 * dt * acc[3 * a + d];
+      const double v0 = vscale ? vel[3 * a + d] * sc : vel[3 * a + d];
+      double v = v0 + 0.5 * dt * acc[3 * a + d];
       vel[3 * a + d] = v;
       x[d] = pos[3 * a + d] + dt * v;
     }
@@ -94,12 +102,12 @@ __global__ __launch_bounds__(256) void k_md_kick(int n, double* __restrict__ vel
   if (threadIdx.x == 0) ke_part[blockIdx.x] = ke[0];
 }
 
-// One block.  Records the state the previous kick left -- E_pot = es[0] and
-// E_kin = sum of the kick's block sums in fixed order -- at index
-// counters[0] and advances it; with disp_part != nullptr, also the Verlet-
-// skin test (engine/neighborlist.py:653-673, fixed cell: rebuild when
-// r_list - 2 max|MIC(x - x_build)| <= r_cut) -> *flag, counted in
-// counters[1].
+// One block.  With e_pot != nullptr, records the state the previous kick
+// left -- E_pot = es[0] and E_kin = sum of the kick's block sums in fixed
+// order -- at index counters[0] and advances it; with disp_part != nullptr,
+// also the Verlet-skin test (engine/neighborlist.py:653-673, fixed cell:
+// rebuild when r_list - 2 max|MIC(x - x_build)| <= r_cut) -> *flag, counted
+// in counters[1].
 __global__ __launch_bounds__(256) void k_md_flag(int nb, const double* __restrict__ ke_part,
                                                  const double* __restrict__ es,
                                                  const double* __restrict__ disp_part,
@@ -107,7 +115,8 @@ __global__ __launch_bounds__(256) void k_md_flag(int nb, const double* __restric
                                                  int* __restrict__ flag,
                                                  int* __restrict__ counters,
                                                  double* __restrict__ e_pot,
-                                                 double* __restrict__ e_kin) {
+                                                 double* __restrict__ e_kin,
+                                                 double* __restrict__ e_cons) {
   __shared__ double smem[16];
   __shared__ double m[256];
   double v[1] = {0.0};
@@ -124,10 +133,13 @@ __global__ __launch_bounds__(256) void k_md_flag(int nb, const double* __restric
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    int step = counters[0];
-    e_pot[step] = es[0];
-    e_kin[step] = v[0];
-    counters[0] = step + 1;
+    if (e_pot) {
+      int step = counters[0];
+      e_pot[step] = es[0];
+      e_kin[step] = v[0];
+      e_cons[step] = es[0] + v[0];
+      counters[0] = step + 1;
+    }
     if (disp_part) {
       const int fl = (r_list - 2.0 * sqrt(m[0]) <= r_cut) ? 1 : 0;
       *flag = fl;
@@ -136,46 +148,169 @@ __global__ __launch_bounds__(256) void k_md_flag(int nb, const double* __restric
   }
 }
 
+// Nose-Hoover chain state on the device: nhc[0] = velocity scale of the last
+// end-of-step half-step not yet applied to the velocities, nhc[1] = the
+// scale the next drift applies, then Q[M], xi[M], vxi[M].
+constexpr int kNhcMax = 16;
+
+// One M-link chain half-step of dt/2 (_nhc_step, solvers/molecular_dynamics.py:
+// 635-727, the same operations in the same order) for kinetic energy ek;
+// returns the particle velocity scale exp(-vxi_1 dt/2).  E_kin after the
+// scaling (the reference recomputes it from the velocities) is ek scale^2.
+__device__ double nhc_half(double* __restrict__ nhc, int M, double dt, double kT, double ndof,
+                           double ek) {
+  double* Q = nhc + 2;
+  double* xi = Q + M;
+  double* v = xi + M;
+  auto force = [&](int j, double e) {
+    return j == 0 ? (2.0 * e - ndof * kT) / Q[0] : (Q[j - 1] * v[j - 1] * v[j - 1] - kT) / Q[j];
+  };
+  v[M - 1] += force(M - 1, ek) * (dt * 0.25);
+  double s[kNhcMax];
+  for (int j = M - 2; j >= 0; --j) {
+    const double arg = fmin(fmax(-v[j + 1] * dt * 0.125, -350.0), 350.0);
+    s[j] = exp(arg);
+    v[j] = s[j] * (s[j] * v[j] + force(j, ek) * dt * 0.25);
+  }
+  const double scale = exp(fmin(fmax(-v[0] * dt * 0.5, -350.0), 350.0));
+  for (int j = 0; j < M; ++j) xi[j] += v[j] * dt * 0.5;
+  ek *= scale * scale;
+  for (int j = 0; j < M - 1; ++j) v[j] = s[j] * (s[j] * v[j] + force(j, ek) * dt * 0.25);
+  v[M - 1] += force(M - 1, ek) * (dt * 0.25);
+  return scale;
+}
+
+// One block.  E_kin of the velocities the last kick left = the kick's block
+// sums in fixed order.  mode 1 (start of a step): half-step with E_kin of the
+// velocities after the pending scale; the drift applies pending x new scale.
+// mode 2 (end of a step): half-step; its scale becomes pending; records E_pot,
+// E_kin and H_NHC = E_kin + E_pot + sum Q vxi^2 / 2 + n_dof kT xi_1
+// + sum_{j>1} kT xi_j (_nhc_conserved, :947-969) at counters[0].  mode 0:
+// records only (the initial state).
+__global__ __launch_bounds__(256) void k_nhc(int nb, const double* __restrict__ ke_part,
+                                             double* __restrict__ nhc, int M, double dt,
+                                             double kT, double ndof, int mode,
+                                             const double* __restrict__ es,
+                                             int* __restrict__ counters,
+                                             double* __restrict__ e_pot,
+                                             double* __restrict__ e_kin,
+                                             double* __restrict__ e_cons) {
+  __shared__ double smem[16];
+  double v[1] = {0.0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += ke_part[b];
+  block_sum<1>(v, smem);
+  if (threadIdx.x != 0) return;
+  double ek = v[0];
+  if (mode == 1) {
+    const double p = nhc[0];
+    ek *= p * p;
+    const double s = nhc_half(nhc, M, dt, kT, ndof, ek);
+    nhc[1] = p * s;
+    nhc[0] = 1.0;
+    return;
+  }
+  if (mode == 2) {
+    const double s = nhc_half(nhc, M, dt, kT, ndof, ek);
+    nhc[0] = s;
+    ek *= s * s;
+  }
+  const double* Q = nhc + 2;
+  const double* xi = Q + M;
+  const double* w = xi + M;
+  double t = 0.5 * Q[0] * w[0] * w[0];
+  for (int j = 1; j < M; ++j) t = t + 0.5 * Q[j] * w[j] * w[j];
+  double u = ndof * kT * xi[0];
+  for (int j = 1; j < M; ++j) u = u + kT * xi[j];
+  const int step = counters[0];
+  e_pot[step] = es[0];
+  e_kin[step] = ek;
+  e_cons[step] = ek + es[0] + t + u;
+  counters[0] = step + 1;
+}
+
+// v *= scale[0] (the pending thermostat scale at the end of a run).
+__global__ __launch_bounds__(256) void k_scale_vel(int n, double* __restrict__ vel,
+                                                   const double* __restrict__ scale) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < 3 * n) vel[t] *= scale[0];
+}
+
 struct MDState {
-  DevBuf pos, vel, acc, F, mass, spc, es, ke, hist_ep, hist_ek, counters, flag;
+  DevBuf pos, vel, acc, F, mass, spc, es, ke, hist_ep, hist_ek, hist_ec, counters, flag;
+  DevBuf nhc, esite, flux_part, hist_J;
+  // thermostat: 0 none (NVE), 1 Nose-Hoover chain
+  int thermo = 0, M = 0, every = 0;
+  double kT = 0.0, ndof = 0.0;
 };
 
-// One step: drift (+ block displacement maxima); record of the previous state
-// and the skin test; the gated list rebuild; forces; kick.  The state after
-// the last step is recorded by one more k_md_flag after the loop.
+// One step: [chain half-step;] drift (+ block displacement maxima); the skin
+// test (NVE: also the record of the previous state); the gated list rebuild;
+// forces; kick; [chain half-step and record;] [heat flux].  For NVE the
+// state after the last step is recorded by md_record after the loop.
 static void md_step(Potential& pot, NList& nl, SetDev& set, MDState& S, int n, double dt,
                     hipStream_t st) {
   const unsigned nb = (unsigned)((n + 255) / 256);
+  const bool nhc = S.thermo == 1;
   nl.disp_part.ensure(sizeof(double) * nb);
+  if (nhc)
+    hipLaunchKernelGGL(k_nhc, dim3(1), dim3(256), 0, st, (int)nb,
+                       (const double*)S.ke.as<double>(), S.nhc.as<double>(), S.M, dt, S.kT,
+                       S.ndof, 1, (const double*)S.es.as<double>(), S.counters.as<int>(),
+                       S.hist_ep.as<double>(), S.hist_ek.as<double>(), S.hist_ec.as<double>());
   hipLaunchKernelGGL(k_md_drift, dim3(nb), dim3(256), 0, st, n, S.pos.as<double>(),
                      S.vel.as<double>(), (const double*)S.acc.as<double>(), dt, set.geoms[0],
-                     (const double*)nl.ref_pos.as<double>(), nl.disp_part.as<double>());
+                     (const double*)nl.ref_pos.as<double>(), nl.disp_part.as<double>(),
+                     (const double*)(nhc ? S.nhc.as<double>() + 1 : nullptr));
   hipLaunchKernelGGL(k_md_flag, dim3(1), dim3(256), 0, st, (int)nb,
                      (const double*)S.ke.as<double>(), (const double*)S.es.as<double>(),
                      (const double*)nl.disp_part.as<double>(), nl.r_list, nl.r_cut,
-                     S.flag.as<int>(), S.counters.as<int>(), S.hist_ep.as<double>(),
-                     S.hist_ek.as<double>());
+                     S.flag.as<int>(), S.counters.as<int>(),
+                     nhc ? (double*)nullptr : S.hist_ep.as<double>(), S.hist_ek.as<double>(),
+                     S.hist_ec.as<double>());
   nl_build_gated(nl, set, S.pos.as<double>(), S.flag.as<int>(), st);
-  compute_forces(pot, nl, set, S.pos.as<double>(), S.spc.as<int>(), S.F.as<double>(),
-                 S.es.as<double>(), st);
+  if (S.every > 0)
+    site_grads(pot, nl, set, S.pos.as<double>(), S.spc.as<int>(), S.F.as<double>(),
+               S.es.as<double>(), S.esite.as<double>(), st);
+  else
+    compute_forces(pot, nl, set, S.pos.as<double>(), S.spc.as<int>(), S.F.as<double>(),
+                   S.es.as<double>(), st);
   hipLaunchKernelGGL(k_md_kick, dim3(nb), dim3(256), 0, st, n, S.vel.as<double>(),
                      S.acc.as<double>(), (const double*)S.F.as<double>(),
                      (const double*)S.mass.as<double>(), dt, 1, S.ke.as<double>());
+  if (nhc)
+    hipLaunchKernelGGL(k_nhc, dim3(1), dim3(256), 0, st, (int)nb,
+                       (const double*)S.ke.as<double>(), S.nhc.as<double>(), S.M, dt, S.kT,
+                       S.ndof, 2, (const double*)S.es.as<double>(), S.counters.as<int>(),
+                       S.hist_ep.as<double>(), S.hist_ek.as<double>(), S.hist_ec.as<double>());
+  if (S.every > 0)
+    heat_flux(pot, nl, set, S.pos.as<double>(), S.vel.as<double>(), S.mass.as<double>(),
+              S.esite.as<double>(), nhc ? S.nhc.as<double>() : nullptr, kConv,
+              S.counters.as<int>(), S.every, S.flux_part.as<double>(), S.hist_J.as<double>(), st);
 }
 
 static void md_record(MDState& S, int n, hipStream_t st) {
   const unsigned nb = (unsigned)((n + 255) / 256);
+  if (S.thermo == 1) {
+    hipLaunchKernelGGL(k_nhc, dim3(1), dim3(256), 0, st, (int)nb,
+                       (const double*)S.ke.as<double>(), S.nhc.as<double>(), S.M, 0.0, S.kT,
+                       S.ndof, 0, (const double*)S.es.as<double>(), S.counters.as<int>(),
+                       S.hist_ep.as<double>(), S.hist_ek.as<double>(), S.hist_ec.as<double>());
+    return;
+  }
   hipLaunchKernelGGL(k_md_flag, dim3(1), dim3(256), 0, st, (int)nb,
                      (const double*)S.ke.as<double>(), (const double*)S.es.as<double>(),
                      (const double*)nullptr, 0.0, 0.0, (int*)nullptr, S.counters.as<int>(),
-                     S.hist_ep.as<double>(), S.hist_ek.as<double>());
+                     S.hist_ep.as<double>(), S.hist_ek.as<double>(), S.hist_ec.as<double>());
 }
 
-void md_nve_run(Potential& pot, int n, const double* lattice, const double* inv, double* pos,
-                double* vel, const double* mass, const int32_t* spc, double dt, int n_steps,
-                double skin, bool use_graph, double* e_pot, double* e_kin, int* n_rebuilds,
-                double* seconds, hipStream_t st) {
+void md_run(Potential& pot, int n, const double* lattice, const double* inv, double* pos,
+            double* vel, const double* mass, const int32_t* spc, double dt, int n_steps,
+            double skin, bool use_graph, const MDOptions& opt, double* e_pot, double* e_kin,
+            double* e_cons, double* flux, int* n_rebuilds, double* seconds, hipStream_t st) {
   AK_REQUIRE(n > 0 && n_steps >= 0 && dt > 0.0, AK_ERR_ARG, "bad MD arguments");
+  AK_REQUIRE(opt.thermostat == 0 || opt.thermostat == 1, AK_ERR_ARG,
+             "thermostat must be 0 (NVE) or 1 (Nose-Hoover chain)");
+  AK_REQUIRE(opt.flux_every >= 0, AK_ERR_ARG, "flux_every must be >= 0");
   SetDev set;
   NList nl;
   nl.r_cut = pot.r_cut;
@@ -183,6 +318,8 @@ void md_nve_run(Potential& pot, int n, const double* lattice, const double* inv,
   nl.r_list = pot.r_cut + skin;
   set.setup(1, std::vector<int64_t>{0, (int64_t)n}.data(), lattice, inv, nl.r_list, st);
   MDState S;
+  S.thermo = opt.thermostat;
+  S.every = opt.flux_every;
   S.pos.ensure(sizeof(double) * 3 * n);
   S.vel.ensure(sizeof(double) * 3 * n);
   S.acc.ensure(sizeof(double) * 3 * n);
@@ -194,8 +331,34 @@ void md_nve_run(Potential& pot, int n, const double* lattice, const double* inv,
   S.ke.ensure(sizeof(double) * nb);
   S.hist_ep.ensure(sizeof(double) * (n_steps + 1));
   S.hist_ek.ensure(sizeof(double) * (n_steps + 1));
+  S.hist_ec.ensure(sizeof(double) * (n_steps + 1));
   S.counters.ensure(sizeof(int) * 4);
   S.flag.ensure(sizeof(int));
+  const int frames = S.every > 0 ? (n_steps + S.every - 1) / S.every : 0;
+  if (S.every > 0) {
+    S.esite.ensure(sizeof(double) * n);
+    S.flux_part.ensure(sizeof(double) * 3 * ((size_t)n / 32 + 1));
+    S.hist_J.ensure(sizeof(double) * 3 * (size_t)std::max(frames, 1));
+  }
+  if (S.thermo == 1) {
+    // Martyna, Tuckerman & Klein masses: Q_1 = n_dof kT tau^2, Q_j = kT tau^2
+    // (_init_nhc_chain, solvers/molecular_dynamics.py:575-588), n_dof = 3N - 3
+    AK_REQUIRE(opt.chain >= 2 && opt.chain <= kNhcMax, AK_ERR_ARG,
+               "the Nose-Hoover chain needs 2 to 16 links");
+    AK_REQUIRE(opt.temperature > 0.0 && opt.tau > 0.0, AK_ERR_ARG,
+               "thermostat temperature and tau must be positive");
+    S.M = opt.chain;
+    S.kT = kBoltzmannEv * opt.temperature;
+    S.ndof = (double)std::max(3 * n - 3, 1);
+    std::vector<double> h(2 + 3 * S.M, 0.0);
+    h[0] = h[1] = 1.0;
+    h[2] = S.ndof * kBoltzmannEv * opt.temperature * opt.tau * opt.tau;
+    for (int k = 1; k < S.M; ++k) h[2 + k] = kBoltzmannEv * opt.temperature * opt.tau * opt.tau;
+    S.nhc.ensure(sizeof(double) * h.size());
+    AK_HIP(hipMemcpyAsync(S.nhc.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
+                          st));
+    AK_HIP(hipStreamSynchronize(st));
+  }
   AK_HIP(hipMemcpyAsync(S.pos.p, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
   AK_HIP(hipMemcpyAsync(S.vel.p, vel, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
   AK_HIP(hipMemcpyAsync(S.mass.p, mass, sizeof(double) * n, hipMemcpyHostToDevice, st));
@@ -226,13 +389,15 @@ void md_nve_run(Potential& pot, int n, const double* lattice, const double* inv,
                      (const double*)S.mass.as<double>(), dt, 0, S.ke.as<double>());
   AK_HIP(hipMemsetAsync(nl.stats.p, 0, sizeof(int) * 8, st));
   // make sure every scratch buffer a step touches exists before capture: a
-  // step with dt = 0 leaves positions, velocities and the state-0 sums as
-  // they were
+  // step with dt = 0 leaves positions, velocities, the chain and the state-0
+  // sums as they were
   md_step(pot, nl, set, S, n, 0.0, st);
   AK_HIP(hipStreamSynchronize(st));
-  // restart the record and the rebuild count: step k records state k - 1
+  // restart the record, the rebuild count and the flux gate: step k records
+  // state k - 1 (NVE) or k (chain; state 0 recorded here)
   int reset[4] = {0, 0, 0, 0};
   AK_HIP(hipMemcpy(S.counters.as<int>(), reset, sizeof(reset), hipMemcpyHostToDevice));
+  if (S.thermo == 1) md_record(S, n, st);
 
   auto t0 = std::chrono::steady_clock::now();
   if (use_graph && n_steps > 0) {
@@ -242,9 +407,10 @@ void md_nve_run(Potential& pot, int n, const double* lattice, const double* inv,
     md_step(pot, nl, set, S, n, dt, st);
     AK_HIP(hipStreamEndCapture(st, &graph));
     AK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    AK_HIP(hipStreamSynchronize(st));
     t0 = std::chrono::steady_clock::now();
     for (int s = 0; s < n_steps; ++s) AK_HIP(hipGraphLaunch(exec, st));
-    md_record(S, n, st);
+    if (S.thermo == 0) md_record(S, n, st);
     AK_HIP(hipStreamSynchronize(st));
     auto t1 = std::chrono::steady_clock::now();
     if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
@@ -252,16 +418,21 @@ void md_nve_run(Potential& pot, int n, const double* lattice, const double* inv,
     (void)hipGraphDestroy(graph);
   } else {
     for (int s = 0; s < n_steps; ++s) md_step(pot, nl, set, S, n, dt, st);
-    md_record(S, n, st);
+    if (S.thermo == 0) md_record(S, n, st);
     AK_HIP(hipStreamSynchronize(st));
     auto t1 = std::chrono::steady_clock::now();
     if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
   }
+  if (S.thermo == 1)
+    hipLaunchKernelGGL(k_scale_vel, dim3((unsigned)((3 * n + 255) / 256)), dim3(256), 0, st, n,
+                       S.vel.as<double>(), (const double*)S.nhc.as<double>());
+  AK_HIP(hipStreamSynchronize(st));
   AK_REQUIRE(nl_stats_overflow(nl, st) == 0, AK_ERR_CAPACITY,
              "neighbour capacity of the captured MD step was exceeded");
   int counters[4];
   AK_HIP(hipMemcpy(counters, S.counters.p, sizeof(counters), hipMemcpyDeviceToHost));
   AK_REQUIRE(counters[0] == n_steps + 1, AK_ERR_STATE, "MD step counter mismatch");
+  AK_REQUIRE(S.every == 0 || counters[2] == n_steps, AK_ERR_STATE, "heat-flux gate mismatch");
   AK_HIP(hipMemcpy(pos, S.pos.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
   AK_HIP(hipMemcpy(vel, S.vel.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
   if (e_pot)
@@ -270,6 +441,11 @@ void md_nve_run(Potential& pot, int n, const double* lattice, const double* inv,
   if (e_kin)
     AK_HIP(hipMemcpy(e_kin, S.hist_ek.p, sizeof(double) * (n_steps + 1),
                      hipMemcpyDeviceToHost));
+  if (e_cons)
+    AK_HIP(hipMemcpy(e_cons, S.hist_ec.p, sizeof(double) * (n_steps + 1),
+                     hipMemcpyDeviceToHost));
+  if (flux && frames > 0)
+    AK_HIP(hipMemcpy(flux, S.hist_J.p, sizeof(double) * 3 * frames, hipMemcpyDeviceToHost));
   if (n_rebuilds) *n_rebuilds = counters[1];
 }
 

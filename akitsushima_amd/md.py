@@ -1,11 +1,13 @@
-"""Device-resident NVE molecular dynamics over the HIP engine.
+"""Device-resident molecular dynamics over the HIP engine.
 
 ``nve_run`` integrates velocity Verlet entirely on the GPU
 (NVESimulation.run, solvers/molecular_dynamics.py:1969-1996): half kick,
 drift, wrap into the cell, Verlet-skin list test and rebuild
 (engine/neighborlist.py:653-673), forces, half kick; one step is captured in
-a hipGraph and replayed.  E_pot and E_kin of every step are recorded on the
-device and returned at the end.
+a hipGraph and replayed.  ``nvt_run`` adds the two half-steps of a
+Nose-Hoover chain (NVTSimulation.run, :1113-1160).  E_pot, E_kin and the
+conserved quantity of every step, and optionally the heat flux every k
+steps, are recorded on the device and returned at the end.
 """
 from __future__ import annotations
 
@@ -43,12 +45,15 @@ def masses_of(atom_types):
     return np.array([ATOMIC_MASS.get(t, 1.0) for t in atom_types], float)
 
 
-def nve_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_steps=1000,
-            skin_width=0.3, masses=None, use_graph=True):
-    """Run n_steps of NVE on the device.  Returns a dict with the final
-    positions and velocities, e_pot, e_kin and e_tot per step (n_steps + 1,
-    index 0 the initial state), drift = e_tot - e_tot[0], the number of list
-    rebuilds and the wall time of the stepping loop in seconds."""
+def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_steps=1000,
+           skin_width=0.3, masses=None, use_graph=True, thermostat=None, flux_every=0):
+    """n_steps of velocity Verlet on the device (``ak_md_run``).  thermostat:
+    None (NVE) or ("nhc", temperature K, tau fs, chain links).  Returns a
+    dict with the final positions and velocities, e_pot, e_kin and e_cons
+    (E_tot, or H_NHC with a chain) per step (n_steps + 1, index 0 the
+    initial state), the number of list rebuilds, the loop's wall time and,
+    with flux_every > 0, the heat flux J (frames, 3) [eV/(A^2 fs)] of the
+    state after every step s with s % flux_every == 0."""
     lat = np.ascontiguousarray(lattice, float)
     pos = np.ascontiguousarray(positions, float).copy()
     vel = np.ascontiguousarray(velocities, float).copy()
@@ -58,14 +63,54 @@ def nve_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_ste
     spc = np.ascontiguousarray(potential.species_ids(atom_types), np.int32)
     ep = np.zeros(n_steps + 1)
     ek = np.zeros(n_steps + 1)
+    ec = np.zeros(n_steps + 1)
+    every = int(flux_every)
+    frames = (n_steps + every - 1) // every if every > 0 else 0
+    J = np.zeros((max(frames, 1), 3))
+    th = None
+    if thermostat is not None:
+        kind, T, tau, chain = thermostat
+        if kind != "nhc":
+            raise ValueError("thermostat: only 'nhc' (Nose-Hoover chain) runs on the device")
+        th = np.array([1.0, float(T), float(tau), float(chain)])
     nreb = ctypes.c_int32(0)
     secs = ctypes.c_double(0.0)
     P = _lib._pf64
-    check(lib().ak_md_nve(potential.handle(), n, lat.ctypes.data_as(P), inv.ctypes.data_as(P),
+    check(lib().ak_md_run(potential.handle(), n, lat.ctypes.data_as(P), inv.ctypes.data_as(P),
                           pos.ctypes.data_as(P), vel.ctypes.data_as(P), m.ctypes.data_as(P),
                           spc.ctypes.data_as(_lib._pi32), float(dt), int(n_steps),
-                          float(skin_width), 1 if use_graph else 0, ep.ctypes.data_as(P),
-                          ek.ctypes.data_as(P), ctypes.byref(nreb), ctypes.byref(secs)))
+                          float(skin_width), 1 if use_graph else 0,
+                          th.ctypes.data_as(P) if th is not None else None, every,
+                          ep.ctypes.data_as(P), ek.ctypes.data_as(P), ec.ctypes.data_as(P),
+                          J.ctypes.data_as(P), ctypes.byref(nreb), ctypes.byref(secs)))
     et = ep + ek
-    return {"positions": pos, "velocities": vel, "e_pot": ep, "e_kin": ek, "e_tot": et,
-            "drift": et - et[0], "n_rebuilds": int(nreb.value), "seconds": float(secs.value)}
+    out = {"positions": pos, "velocities": vel, "e_pot": ep, "e_kin": ek, "e_tot": et,
+           "e_cons": ec, "drift": et - et[0], "n_rebuilds": int(nreb.value),
+           "seconds": float(secs.value)}
+    if every > 0:
+        out["flux"] = J[:frames]
+    return out
+
+
+def nve_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_steps=1000,
+            skin_width=0.3, masses=None, use_graph=True, flux_every=0):
+    """Run n_steps of NVE on the device (NVESimulation.run,
+    solvers/molecular_dynamics.py:1969-1996).  Returns md_run's dict;
+    drift = e_tot - e_tot[0]."""
+    return md_run(potential, lattice, positions, velocities, atom_types, dt, n_steps,
+                  skin_width, masses, use_graph, None, flux_every)
+
+
+def nvt_run(potential, lattice, positions, velocities, atom_types, temperature, dt=1.0,
+            n_steps=1000, tau_nhc=100.0, nhc_chain=3, skin_width=0.5, masses=None,
+            use_graph=True, flux_every=0):
+    """NVT with an M-link Nose-Hoover chain on the device (NVTSimulation.run,
+    solvers/molecular_dynamics.py:1113-1160, _nhc_step :635-727; defaults of
+    NVTSimulation: tau_nhc 100 fs, 3 links, skin 0.5 A).  e_cons is H_NHC
+    (:947-969); T_inst = 2 E_kin / (n_dof k_B), n_dof = 3N - 3."""
+    out = md_run(potential, lattice, positions, velocities, atom_types, dt, n_steps,
+                 skin_width, masses, use_graph, ("nhc", temperature, tau_nhc, nhc_chain),
+                 flux_every)
+    n_dof = max(3 * len(positions) - 3, 1)
+    out["T"] = 2.0 * out["e_kin"] / (n_dof * K_B)
+    return out

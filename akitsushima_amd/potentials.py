@@ -162,6 +162,19 @@ class _GpuPotential:
                                stress.ctypes.data_as(_lib._pf64)))
         return float(e.value), forces, stress
 
+    def site_terms(self, lattice, positions, atom_types, neigh_list=None):
+        """Site-energy decomposition (engine/site_terms.py, ``site_terms``
+        of the reference's potentials): a :class:`SiteTerms`."""
+        from .site_terms import site_terms
+
+        return site_terms(self, lattice, positions, atom_types, neigh_list)
+
+    def heat_flux(self, lattice, positions, velocities, atom_types, masses, neigh_list=None):
+        """(J [eV/(A^2 fs)], per-atom site energies) of one state."""
+        from .site_terms import heat_flux
+
+        return heat_flux(self, lattice, positions, velocities, atom_types, masses, neigh_list)
+
     def compute_batched(self, structures):
         """E (S,), forces (list of (N_s, 3)), stress (S, 3, 3) for a list of
         (lattice, positions, atom_types) in one device pass."""

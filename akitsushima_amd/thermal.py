@@ -1,0 +1,168 @@
+"""Green-Kubo thermal conductivity with the MD and the heat flux on the device.
+
+``ThermalConductivityCalculator`` keeps the reference's interface
+(solvers/thermal_conductivity.py:50-130, run_md :414-677, compute_kappa
+:679-831): ``run_md`` equilibrates with a Nose-Hoover chain and records the
+exact many-body heat flux of an NVE production run every
+``heat_flux_every`` steps -- both runs device-resident, one hipGraph step
+each, the flux formed on the device from the bond kernels' site terms --
+and ``compute_kappa`` integrates the heat-flux autocorrelation on the host:
+
+    kappa_ab = V / (k_B T^2) int_0^tau_max <J_a J_b>(t) dt   [W/(m K)],
+
+with the reference's FFT correlation, trapezoid running integral and block
+error estimate.
+"""
+from __future__ import annotations
+
+import os
+import warnings
+
+import numpy as np
+
+from .md import K_B, maxwell_boltzmann, masses_of, md_run, nvt_run
+
+EV_ANG_K_FS_TO_W_MK = (1.60217663e-19) / (1e-10 * 1e-15)
+
+
+def autocorr_fft(a, b, n_lag, subtract_mean=True):
+    """c[k] = sum_t a[t + k] b[t] / (n - k), k < n_lag, by zero-padded FFTs
+    (the lag convention of the reference's _autocorr_fft,
+    solvers/thermal_conductivity.py:1107-1170); series mean-subtracted by
+    default."""
+    a = np.asarray(a, float)
+    b = np.asarray(b, float)
+    n = len(a)
+    if subtract_mean:
+        a = a - a.mean()
+        b = b - b.mean()
+    nfft = 1
+    while nfft < 2 * n:
+        nfft *= 2
+    c = np.fft.irfft(np.fft.rfft(a, n=nfft) * np.conj(np.fft.rfft(b, n=nfft)), n=nfft)[:n]
+    return (c / (n - np.arange(n)))[:n_lag]
+
+
+def _running_integral(h, dt):
+    run = np.zeros_like(h)
+    for k in range(1, len(h)):
+        run[k] = run[k - 1] + 0.5 * dt * (h[k - 1] + h[k])
+    return run
+
+
+def _hcacf(J, n_lag):
+    h = np.zeros((n_lag, 3, 3))
+    for a in range(3):
+        for b in range(3):
+            h[:, a, b] = autocorr_fft(J[:, a], J[:, b], n_lag)
+    return h
+
+
+class ThermalConductivityCalculator:
+    """Green-Kubo kappa of one cell with a GPU potential (any of the
+    engine's potential classes, or a force function of ``make_force_fn``
+    whose engine holds one of them)."""
+
+    def __init__(self, lattice, positions, atom_types, potential, masses=None,
+                 temperature=300.0):
+        self.lattice = np.asarray(lattice, float).copy()
+        self.positions = np.asarray(positions, float).copy()
+        self.atom_types = list(atom_types)
+        pots = getattr(getattr(potential, "engine", None), "_potentials", None)
+        if pots is not None:
+            if len(pots) != 1:
+                raise ValueError("ThermalConductivityCalculator: the engine must hold one "
+                                 "potential (its site terms give the heat flux)")
+            potential = pots[0]
+        if not hasattr(potential, "site_terms"):
+            raise ValueError("ThermalConductivityCalculator: the potential has no site-energy "
+                             "decomposition for the heat flux")
+        self.potential = potential
+        self.temperature = float(temperature)
+        self.N = len(self.atom_types)
+        self.masses = (np.asarray(masses, float) if masses is not None
+                       else masses_of(self.atom_types))
+        self._J_series = None
+        self._dt_hf = None
+        self._hcacf = self._tau_axis = self._kappa = self._kappa_run = self._kappa_std = None
+
+    def run_md(self, n_steps=100_000, timestep=0.5, equil_steps=10_000, velocities=None,
+               heat_flux_every=5, output_dir=None, seed=42, tau_nhc=100.0, nhc_chain=3,
+               skin_width=0.5, verbose=False):
+        """NVT (Nose-Hoover chain) equilibration for equil_steps, then NVE
+        for n_steps recording J after every step s with s %
+        heat_flux_every == 0.  Without ``velocities`` the start is a
+        Maxwell-Boltzmann draw at the temperature from ``seed``.  Returns
+        {"J_series" (frames, 3) [eV/(A^2 fs)], "dt_hf" fs, "positions",
+        "velocities", "e_tot" (NVE), "equil" (the NVT run's record or
+        None)}."""
+        pos = self.positions.copy()
+        vel = (np.asarray(velocities, float).copy() if velocities is not None
+               else maxwell_boltzmann(self.temperature, self.masses, seed))
+        eq = None
+        if equil_steps > 0:
+            eq = nvt_run(self.potential, self.lattice, pos, vel, self.atom_types,
+                         self.temperature, dt=timestep, n_steps=equil_steps, tau_nhc=tau_nhc,
+                         nhc_chain=nhc_chain, skin_width=skin_width, masses=self.masses)
+            pos, vel = eq["positions"], eq["velocities"]
+        prod = md_run(self.potential, self.lattice, pos, vel, self.atom_types, dt=timestep,
+                      n_steps=n_steps, skin_width=skin_width, masses=self.masses,
+                      flux_every=int(heat_flux_every))
+        J = prod["flux"]
+        self._J_series = J
+        self._dt_hf = float(timestep * heat_flux_every)
+        if output_dir:
+            os.makedirs(output_dir, exist_ok=True)
+            with open(os.path.join(output_dir, "heat_flux.dat"), "w") as fh:
+                fh.write("step  Jx_eV_Ang2_fs  Jy  Jz\n")
+                for k, j in enumerate(J):
+                    fh.write(f"{k * heat_flux_every + 1}  {j[0]:.8e}  {j[1]:.8e}  {j[2]:.8e}\n")
+        if verbose:
+            print(f"  NVE production: {n_steps} steps, {len(J)} heat-flux frames, "
+                  f"{prod['seconds']:.2f} s on the device")
+        return {"J_series": J, "dt_hf": self._dt_hf, "positions": prod["positions"],
+                "velocities": prod["velocities"], "e_tot": prod["e_tot"], "equil": eq}
+
+    def compute_kappa(self, J_series=None, dt_hf=None, tau_max=20.0, n_blocks=5, volume=None,
+                      verbose=False):
+        """kappa (3, 3) [W/(m K)] from the heat-flux ACF integrated to tau_max
+        ps (compute_kappa, solvers/thermal_conductivity.py:679-831); block
+        standard error in ``self._kappa_std``."""
+        if J_series is None:
+            if self._J_series is None:
+                raise RuntimeError("compute_kappa: no J_series available. Call run_md() first "
+                                   "or supply J_series=.")
+            J_series = self._J_series
+        if dt_hf is None:
+            dt_hf = self._dt_hf if self._dt_hf else 1.0
+        J = np.asarray(J_series, float)
+        n = len(J)
+        dt = float(dt_hf)
+        T = self.temperature
+        V = float(volume) if volume is not None else abs(np.linalg.det(self.lattice))
+        n_lag = min(int(tau_max * 1e3 / dt), n // 2)
+        pref = V / (K_B * T ** 2)
+        h = _hcacf(J, n_lag)
+        run = _running_integral(h, dt) * pref
+        kappa = run[-1] * EV_ANG_K_FS_TO_W_MK
+        block = max(n // n_blocks, n_lag)
+        nb = n // block if block > 0 else 0
+        if nb < 2:
+            warnings.warn("compute_kappa: trajectory too short relative to tau_max and n_blocks "
+                          "for a block-averaged uncertainty; kappa_std is NaN", UserWarning,
+                          stacklevel=2)
+            std = np.full((3, 3), np.nan)
+        else:
+            kb = np.array([_running_integral(_hcacf(J[i * block:(i + 1) * block], n_lag), dt)[-1]
+                           * pref * EV_ANG_K_FS_TO_W_MK for i in range(nb)])
+            std = np.std(kb, axis=0) / np.sqrt(nb)
+        self._hcacf = h
+        self._tau_axis = np.arange(n_lag) * dt * 1e-3
+        self._kappa = kappa
+        self._kappa_run = run * EV_ANG_K_FS_TO_W_MK
+        self._kappa_std = std
+        if verbose:
+            d = np.diag(kappa)
+            print(f"  Green-Kubo (T={T:.0f} K, tau_max={tau_max:.1f} ps): k = {d} W/(m K), "
+                  f"mean {d.mean():.3f}")
+        return kappa

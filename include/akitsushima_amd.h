@@ -211,6 +211,47 @@ int ak_md_nve(ak_potential *pot, int32_t n_atoms, const double *lattice,
               double skin_width, int32_t use_graph, double *e_pot, double *e_kin,
               int32_t *n_rebuilds, double *seconds);
 
+/* The same loop with a thermostat and the heat flux, for NVTSimulation.run
+ * (solvers/molecular_dynamics.py:1113-1160: velocity Verlet between two
+ * half-steps of an M-link Nose-Hoover chain, _nhc_step :635-727) and the
+ * production run of ThermalConductivityCalculator.run_md
+ * (solvers/thermal_conductivity.py:575-610).  thermostat: NULL for NVE, or
+ * {kind (1 = Nose-Hoover chain), temperature K, tau fs, chain links 2-16},
+ * n_dof = 3N - 3.  e_cons (n_steps + 1): E_tot (NVE) or H_NHC (:947-969).
+ * flux_every > 0: J [eV/(A^2 fs)] of the state after every step s with
+ * s % flux_every == 0 into flux ((n_steps + flux_every - 1) / flux_every, 3),
+ * the exact site-term heat flux of ak_heat_flux.  Any output may be NULL. */
+int ak_md_run(ak_potential *pot, int32_t n_atoms, const double *lattice,
+              const double *inv_lattice, double *positions, double *velocities,
+              const double *masses, const int32_t *species, double dt, int32_t n_steps,
+              double skin_width, int32_t use_graph, const double *thermostat4,
+              int32_t flux_every, double *e_pot, double *e_kin, double *e_cons, double *flux,
+              int32_t *n_rebuilds, double *seconds);
+
+/* ---- Site terms and heat flux ------------------------------------------ */
+
+/* Site-energy decomposition E = sum_a U_a (potential.site_terms,
+ * engine/site_terms.py:1-60, 179-475): e_site (n) = U_a and, for every bond
+ * a -> nbr within r_cut (both directions, atoms in order, each atom's bonds
+ * by ascending neighbour), centre, neighbour, vec = r_nbr - r_a (minimum
+ * image) and dudv = dU_centre / dvec.  Pass centre = NULL to learn the number
+ * of entries (*n_entries) first; AK_ERR_CAPACITY if capacity is too small.
+ * nl as in ak_compute (NULL: the owned list). */
+int ak_site_terms(ak_potential *pot, ak_nlist *nl, int32_t n_atoms, const double *lattice,
+                  const double *inv_lattice, const double *positions, const int32_t *species,
+                  double *e_site, int64_t capacity, int32_t *centre, int32_t *neighbour,
+                  double *vec, double *dudv, int64_t *n_entries);
+
+/* Heat flux J = [sum_a E_a v_a - sum_m vec_m (dudv_m . v_nbr)] / V
+ * [eV/(A^2 fs)], E_a = m_a |v_a|^2 / 2 + U_a (Fan et al., PRB 92, 094301
+ * (2015), eq. 21), as ThermalConductivityCalculator._compute_heat_flux with
+ * the site-term virial flux (solvers/thermal_conductivity.py:168-303);
+ * velocities A/fs, masses amu.  e_site (n) may be NULL. */
+int ak_heat_flux(ak_potential *pot, ak_nlist *nl, int32_t n_atoms, const double *lattice,
+                 const double *inv_lattice, const double *positions,
+                 const double *velocities, const double *masses, const int32_t *species,
+                 double *flux3, double *e_site);
+
 /* ---- Tersoff parameter gradient ---------------------------------------- */
 
 /* Loss of fitting/loss.py:184-242 (squared) over a batch of structures with

@@ -27,7 +27,10 @@ after the reference's Python glue:
   engine/eam_zhou.py:210-315 reads and writes them, evaluated as
   engine/_eam_kernels.py:128-145, 199-205 (scipy CubicSpline coefficients)
 * ``tersoff_loss``: fitting/loss.py:51-248 (squared loss)
-* ``nve_run``: NVESimulation.run, solvers/molecular_dynamics.py:1969-1996
+* ``nve_run``, ``md_run``: NVESimulation.run, solvers/molecular_dynamics.py:1969-1996,
+  and NVTSimulation.run with the Nose-Hoover chain (:635-727, 1113-1160)
+* ``site_terms``: engine/site_terms.py (site energies, per-bond gradients,
+  heat flux)
 
 What pins each function (none is left unpinned):
 
@@ -39,8 +42,11 @@ What pins each function (none is left unpinned):
   also one-species records == flat, and invariance under atom relabelling
 * tersoff_loss: closed-form normalisation on hand-set residuals, and an
   independent torch-autograd restatement (tests/test_fit_reference.py)
-* mb_velocities, nve_run: zero COM momentum and exact temperature; energy
-  conservation and time reversibility.
+* mb_velocities, nve_run, md_run: zero COM momentum and exact temperature;
+  energy conservation and time reversibility; with the chain, conservation
+  of H_NHC, time reversibility and the canonical mean temperature
+* site_terms: rebuilt forces/stress/energy = the force path; heat flux =
+  d/dt sum_i r_i E_i on isolated clusters (tests/test_oracle_site_terms.py)
 """
 from __future__ import annotations
 
@@ -695,11 +701,44 @@ def mb_velocities(T, masses, seed):
     return v
 
 
-def nve_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, skin):
+def nhc_half(state, dt, kT, n_dof, vel, masses):
+    """One M-link chain half-step (_nhc_step, solvers/molecular_dynamics.py:
+    635-727): top-link quarter kick; down the chain damp, kick, damp; scale
+    the velocities by exp(-vxi_1 dt/2); advance xi; E_kin recomputed; up
+    the chain with the same damping factors; top-link quarter kick.  state =
+    (Q, xi, vxi) lists, changed in place; returns the scaled velocities."""
+    Q, xi, v = state
+    M = len(Q)
+    ek = 0.5 * float(np.sum(masses[:, None] * vel ** 2)) / EV_ANG_TO_AMU_ANG_FS2
+
+    def G(j):
+        return (2.0 * ek - n_dof * kT) / Q[0] if j == 0 else (Q[j - 1] * v[j - 1] ** 2 - kT) / Q[j]
+
+    v[M - 1] += G(M - 1) * (dt * 0.25)
+    s = [1.0] * (M - 1)
+    for j in range(M - 2, -1, -1):
+        s[j] = float(np.exp(np.clip(-v[j + 1] * dt * 0.125, -350.0, 350.0)))
+        v[j] = s[j] * (s[j] * v[j] + G(j) * dt * 0.25)
+    vel = vel * float(np.exp(np.clip(-v[0] * dt * 0.5, -350.0, 350.0)))
+    for j in range(M):
+        xi[j] += v[j] * dt * 0.5
+    ek = 0.5 * float(np.sum(masses[:, None] * vel ** 2)) / EV_ANG_TO_AMU_ANG_FS2
+    for j in range(M - 1):
+        v[j] = s[j] * (s[j] * v[j] + G(j) * dt * 0.25)
+    v[M - 1] += G(M - 1) * (dt * 0.25)
+    return vel
+
+
+def md_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, skin,
+           nhc=None, flux=None, every=0):
     """Velocity Verlet with PBC wrap and the Verlet-skin rebuild test of
-    engine/neighborlist.py:613-673 (fixed cell: rebuild when
-    r_list - 2 d_max <= r_cut).  ``force(lattice, pos, (i, j))`` returns
-    (E, F, S).  Returns per-step E_pot, E_kin arrays and the final state."""
+    engine/neighborlist.py:613-673 (fixed cell: rebuild when r_list - 2 d_max
+    <= r_cut); ``force(lattice, pos, (i, j))`` returns (E, F, S).  nhc =
+    (T, tau, M): NVTSimulation.run (solvers/molecular_dynamics.py:1113-1160),
+    the step between two chain half-steps, Q_1 = n_dof kT tau^2, Q_j = kT
+    tau^2, n_dof = 3N - 3.  flux(pos, vel) -> J of the state after steps s
+    with s % every == 0.  Returns E_pot, E_kin, conserved quantity (E_tot or
+    H_NHC, :947-969), final positions and velocities, rebuilds, J frames."""
     lattice = np.asarray(lattice, float)
     pos = np.array(positions, float)
     vel = np.array(velocities, float)
@@ -710,10 +749,31 @@ def nve_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, s
     e, f, _ = force(lattice, pos, pairs)
     acc = f * EV_ANG_TO_AMU_ANG_FS2 / masses[:, None]
     ek = lambda v: 0.5 * float(np.sum(masses[:, None] * v ** 2)) / EV_ANG_TO_AMU_ANG_FS2
-    epot = [e]
-    ekin = [ek(vel)]
+    n_dof = max(3 * len(pos) - 3, 1)
+    state = None
+    if nhc is not None:
+        T, tau, M = nhc
+        kT = K_B * T
+        state = ([n_dof * K_B * T * tau ** 2] + [K_B * T * tau ** 2] * (M - 1), [0.0] * M,
+                 [0.0] * M)
+
+    def cons(e_pot, e_kin):
+        if state is None:
+            return e_pot + e_kin
+        Q, xi, v = state
+        t = 0.5 * Q[0] * v[0] ** 2
+        for j in range(1, len(Q)):
+            t = t + 0.5 * Q[j] * v[j] ** 2
+        u = n_dof * kT * xi[0]
+        for j in range(1, len(Q)):
+            u = u + kT * xi[j]
+        return e_kin + e_pot + t + u
+
+    epot, ekin, econs, J = [e], [ek(vel)], [cons(e, ek(vel))], []
     rebuilds = 0
-    for _ in range(n_steps):
+    for step in range(n_steps):
+        if state is not None:
+            vel = nhc_half(state, dt, kT, n_dof, vel, masses)
         vel += 0.5 * dt * acc
         pos += dt * vel
         frac = pos @ inv
@@ -731,6 +791,19 @@ def nve_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, s
         e, f, _ = force(lattice, pos, pairs)
         acc = f * EV_ANG_TO_AMU_ANG_FS2 / masses[:, None]
         vel += 0.5 * dt * acc
+        if state is not None:
+            vel = nhc_half(state, dt, kT, n_dof, vel, masses)
         epot.append(e)
         ekin.append(ek(vel))
-    return np.array(epot), np.array(ekin), pos, vel, rebuilds
+        econs.append(cons(e, ek(vel)))
+        if flux is not None and every > 0 and step % every == 0:
+            J.append(flux(pos, vel))
+    return (np.array(epot), np.array(ekin), np.array(econs), pos, vel, rebuilds,
+            np.array(J).reshape(-1, 3))
+
+
+def nve_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, skin):
+    """md_run without a thermostat: E_pot, E_kin, final state, rebuilds."""
+    ep, ek, _, pos, vel, rb, _ = md_run(force, lattice, positions, velocities, masses, dt,
+                                        n_steps, r_cut, skin)
+    return ep, ek, pos, vel, rb
