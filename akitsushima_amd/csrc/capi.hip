@@ -896,6 +896,10 @@ int ak_md_run(ak_potential* P, int32_t n, const double* lattice, const double* i
       opt.tau = thermostat[2];
       opt.chain = (int)thermostat[3];
       AK_REQUIRE(thermostat[3] == (double)opt.chain, AK_ERR_ARG, "bad chain length");
+      AK_REQUIRE(thermostat[4] >= 0.0 && thermostat[4] < 9007199254740992.0 &&
+                     thermostat[4] == std::floor(thermostat[4]),
+                 AK_ERR_ARG, "seed must be an integer in [0, 2^53)");
+      opt.seed = (uint64_t)thermostat[4];
     }
     opt.flux_every = flux_every;
     int nr = 0;

@@ -216,12 +216,14 @@ void site_grads(Potential& pot, NList& nl, SetDev& set, const double* d_pos, con
 bool site_masked(const Potential& pot, const NList& nl);
 
 // Device MD options: thermostat 0 = none (NVE), 1 = Nose-Hoover chain of
-// `chain` links at `temperature` K with time constant `tau` fs; heat flux of
-// the state after every step s with s % flux_every == 0 (0 = none).
+// `chain` links at `temperature` K with time constant `tau` fs, 2 = Langevin
+// and 3 = CSVR with damping `tau` fs; heat flux of the state after every step
+// s with s % flux_every == 0 (0 = none).
 struct MDOptions {
-  int thermostat = 0;
-  double temperature = 0.0, tau = 0.0;
+  int thermostat = 0;  // 0 NVE, 1 Nose-Hoover chain, 2 Langevin (BAOAB), 3 CSVR
+  double temperature = 0.0, tau = 0.0;  // tau: chain time constant or damping, fs
   int chain = 3;
+  uint64_t seed = 0;  // random stream of the stochastic thermostats
   int flux_every = 0;
 };
 

@@ -228,6 +228,277 @@ __global__ __launch_bounds__(256) void k_nhc(int nb, const double* __restrict__ 
   counters[0] = step + 1;
 }
 
+// ---------------------------------------------------------------------------
+// Stochastic thermostats.  Random numbers come from Philox4x32-10 (Salmon,
+// Moraes, Dror & Shaw, SC'11), a counter-based generator: the draw of atom a
+// at step s is a pure function of (seed, a, s, stream), so the device and a
+// host restatement produce the same numbers in any order.  Normals by the
+// Box-Muller transform of two 53-bit uniforms.
+// ---------------------------------------------------------------------------
+struct U4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ U4 philox4x32(U4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+    c = U4{(uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.w ^ k1,
+           (uint32_t)p0};
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// 53-bit uniform in [0, 1) from two words.
+__device__ __forceinline__ double u53(uint32_t a, uint32_t b) {
+  return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+// Two standard normals from one Philox block: r = sqrt(-2 ln(1 - u1)),
+// (r cos 2 pi u2, r sin 2 pi u2).
+__device__ __forceinline__ void normal2(U4 q, double& z0, double& z1) {
+  const double u1 = u53(q.x, q.y), u2 = u53(q.z, q.w);
+  const double r = sqrt(-2.0 * log(1.0 - u1));
+  const double t = 6.283185307179586 * u2;
+  z0 = r * cos(t);
+  z1 = r * sin(t);
+}
+
+// Streams of the counter's third word.
+constexpr uint32_t kStreamLangevin = 1, kStreamCsvr = 3;
+
+// Gamma(a) deviate, a >= 1 (Marsaglia & Tsang, ACM TOMS 26, 363 (2000)),
+// drawing block k of the counter (k, h, stream) for its k-th normal and
+// (k + 1) for its uniform; at most 64 rounds.
+__device__ double gamma_mt(double a, uint32_t h, uint32_t k0, uint32_t k1) {
+  const double d = a - 1.0 / 3.0, c = 1.0 / sqrt(9.0 * d);
+  for (uint32_t it = 0; it < 64; ++it) {
+    double x, unused;
+    normal2(philox4x32(U4{2 * it + 1, h, kStreamCsvr, 0}, k0, k1), x, unused);
+    double v = 1.0 + c * x;
+    if (v <= 0.0) continue;
+    v = v * v * v;
+    const U4 q = philox4x32(U4{2 * it + 2, h, kStreamCsvr, 0}, k0, k1);
+    const double u = u53(q.x, q.y);
+    if (u < 1.0 - 0.0331 * (x * x) * (x * x)) return d * v;
+    if (log(u) < 0.5 * x * x + d * (1.0 - v + log(v))) return d * v;
+  }
+  return d;
+}
+
+// CSVR half-step (CSVRSimulation._rescale, solvers/molecular_dynamics.py:
+// 1648-1667; Bussi, Donadio & Parrinello, JCP 126, 014101 (2007)): the
+// kinetic energy K propagated over dt/2 with c = exp(-dt/2 / damping);
+// returns alpha, with *dq = alpha^2 K - K.  Draws: R_1 = normal of block
+// (0, h), S = 2 Gamma((n_f - 1)/2).
+__device__ double csvr_alpha(double K, double c, double nf, double kT, uint32_t h, uint32_t k0,
+                             uint32_t k1, double* dq) {
+  *dq = 0.0;
+  if (!(K > 0.0) || nf < 1.0) return 1.0;
+  const double Kt = 0.5 * nf * kT;
+  double r1, unused;
+  normal2(philox4x32(U4{0, h, kStreamCsvr, 0}, k0, k1), r1, unused);
+  const double sq = nf > 1.0 ? 2.0 * gamma_mt(0.5 * (nf - 1.0), h, k0, k1) : 0.0;
+  const double Kn = c * K + (1.0 - c) * Kt * (r1 * r1 + sq) / nf +
+                    2.0 * r1 * sqrt(c * (1.0 - c) * K * Kt / nf);
+  double alpha = sqrt(fmax(Kn, 0.0) / K);
+  if (r1 + sqrt(c * nf * K / ((1.0 - c) * Kt)) < 0.0) alpha = -alpha;
+  *dq = alpha * alpha * K - K;
+  return alpha;
+}
+
+// One block, CSVR (R B A F B R, CSVRSimulation._step :1669-1687) with the
+// pending-scale bookkeeping of k_nhc; st[0] pending, st[1] drift scale, st[2]
+// heat Q added by the rescalings.  counters[3] counts the half-steps (the
+// random stream).  mode 2 records E_pot, E_kin and E_tot - Q.
+__global__ __launch_bounds__(256) void k_csvr(int nb, const double* __restrict__ ke_part,
+                                              double* __restrict__ th, double c, double nf,
+                                              double kT, uint32_t k0, uint32_t k1, int mode,
+                                              const double* __restrict__ es,
+                                              int* __restrict__ counters,
+                                              double* __restrict__ e_pot,
+                                              double* __restrict__ e_kin,
+                                              double* __restrict__ e_cons) {
+  __shared__ double smem[16];
+  double v[1] = {0.0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += ke_part[b];
+  block_sum<1>(v, smem);
+  if (threadIdx.x != 0) return;
+  double ek = v[0];
+  if (mode == 1 || mode == 2) {
+    const double p = mode == 1 ? th[0] : 1.0;
+    ek *= p * p;
+    double dq;
+    const double a = csvr_alpha(ek, c, nf, kT, (uint32_t)counters[3], k0, k1, &dq);
+    counters[3] += 1;
+    th[2] += dq;
+    if (mode == 1) {
+      th[1] = p * a;
+      th[0] = 1.0;
+      return;
+    }
+    th[0] = a;
+    ek *= a * a;
+  }
+  const int step = counters[0];
+  e_pot[step] = es[0];
+  e_kin[step] = ek;
+  e_cons[step] = ek + es[0] - th[2];
+  counters[0] = step + 1;
+}
+
+// Langevin BAOAB (LangevinSimulation._step, :1570-1587): B, A and O of one
+// step: v += dt/2 a; x += dt/2 v; (kinetic-energy block sums here);
+// v = c v + sqrt((1 - c^2) kT / m) xi, xi of Philox blocks (a, s, 1, 0/1);
+// block sums of m v for the centre-of-mass removal.
+__global__ __launch_bounds__(256) void k_lang_bao(int n, double* __restrict__ pos,
+                                                  double* __restrict__ vel,
+                                                  const double* __restrict__ acc,
+                                                  const double* __restrict__ mass, double dt,
+                                                  double c, double kTc, uint32_t k0, uint32_t k1,
+                                                  const int* __restrict__ counters,
+                                                  double* __restrict__ ke_part,
+                                                  double* __restrict__ p_part) {
+  __shared__ double smem[16 * 4];
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  double r[4] = {0.0, 0.0, 0.0, 0.0};
+  if (a < n) {
+    const double m = mass[a];
+    double v[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      v[d] = vel[3 * a + d] + 0.5 * dt * acc[3 * a + d];
+      pos[3 * a + d] += 0.5 * dt * v[d];
+    }
+    r[0] = 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / kConv;
+    const uint32_t s = (uint32_t)counters[3];
+    double z[4];
+    normal2(philox4x32(U4{(uint32_t)a, s, kStreamLangevin, 0}, k0, k1), z[0], z[1]);
+    normal2(philox4x32(U4{(uint32_t)a, s, kStreamLangevin, 1}, k0, k1), z[2], z[3]);
+    const double sig = sqrt((1.0 - c * c) * kTc / m);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      v[d] = c * v[d] + sig * z[d];
+      vel[3 * a + d] = v[d];
+      r[1 + d] = m * v[d];
+    }
+  }
+  block_sum<4>(r, smem);
+  if (threadIdx.x == 0) {
+    ke_part[blockIdx.x] = r[0];
+    p_part[3 * blockIdx.x] = r[1];
+    p_part[3 * blockIdx.x + 1] = r[2];
+    p_part[3 * blockIdx.x + 2] = r[3];
+  }
+}
+
+// One block: centre-of-mass velocity P / M into th[4..6] (zero when remove
+// is 0, the setup step).
+__global__ __launch_bounds__(256) void k_lang_com(int nb, const double* __restrict__ p_part,
+                                                  double inv_mass, int remove,
+                                                  double* __restrict__ th) {
+  __shared__ double smem[16 * 3];
+  double p[3] = {0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    p[0] += p_part[3 * b];
+    p[1] += p_part[3 * b + 1];
+    p[2] += p_part[3 * b + 2];
+  }
+  block_sum<3>(p, smem);
+  if (threadIdx.x == 0)
+    for (int d = 0; d < 3; ++d) th[4 + d] = remove ? p[d] * inv_mass : 0.0;
+}
+
+// v -= v_com (the end of the O step); block sums of the kinetic energy after
+// it; x += dt/2 v; wrap; the block's largest squared displacement (as
+// k_md_drift).
+__global__ __launch_bounds__(256) void k_lang_a2(int n, double* __restrict__ pos,
+                                                 double* __restrict__ vel,
+                                                 const double* __restrict__ mass, double dt,
+                                                 Geom g, const double* __restrict__ ref,
+                                                 const double* __restrict__ th,
+                                                 double* __restrict__ part,
+                                                 double* __restrict__ ke_part) {
+  __shared__ double m[256];
+  __shared__ double smem[16];
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  double d2 = 0.0;
+  double ke[1] = {0.0};
+  if (a < n) {
+    double x[3], v[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      v[d] = vel[3 * a + d] - th[4 + d];
+      vel[3 * a + d] = v[d];
+      x[d] = pos[3 * a + d] + 0.5 * dt * v[d];
+    }
+    ke[0] = 0.5 * mass[a] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / kConv;
+    double f[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      double sb = x[0] * g.inv[b] + x[1] * g.inv[3 + b] + x[2] * g.inv[6 + b];
+      sb -= floor(sb);
+      f[b] = sb >= 1.0 ? 0.0 : sb;
+    }
+    double y[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      y[b] = f[0] * g.lat[b] + f[1] * g.lat[3 + b] + f[2] * g.lat[6 + b];
+      pos[3 * a + b] = y[b];
+    }
+    double dx, dy, dz;
+    mic_exact(g.lat, g.inv, ref[3 * a], ref[3 * a + 1], ref[3 * a + 2], y[0], y[1], y[2], dx,
+              dy, dz);
+    d2 = dx * dx + dy * dy + dz * dz;
+  }
+  block_sum<1>(ke, smem);
+  if (threadIdx.x == 0) ke_part[blockIdx.x] = ke[0];
+  m[threadIdx.x] = d2;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) m[threadIdx.x] = fmax(m[threadIdx.x], m[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = m[0];
+}
+
+// One block: Q += K(after O) - K(before O); records E_pot, E_kin (the kick's
+// sums) and E_tot - Q (the E-Q column); advances the random step.  mode 0:
+// records only.
+__global__ __launch_bounds__(256) void k_lang_rec(int nb, const double* __restrict__ ke_kick,
+                                                  const double* __restrict__ ke_b,
+                                                  const double* __restrict__ ke_a, int mode,
+                                                  double* __restrict__ th,
+                                                  const double* __restrict__ es,
+                                                  int* __restrict__ counters,
+                                                  double* __restrict__ e_pot,
+                                                  double* __restrict__ e_kin,
+                                                  double* __restrict__ e_cons) {
+  __shared__ double smem[16 * 3];
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    v[0] += ke_kick[b];
+    if (mode) {
+      v[1] += ke_b[b];
+      v[2] += ke_a[b];
+    }
+  }
+  block_sum<3>(v, smem);
+  if (threadIdx.x != 0) return;
+  if (mode) {
+    th[2] += v[2] - v[1];
+    counters[3] += 1;
+  }
+  const int step = counters[0];
+  e_pot[step] = es[0];
+  e_kin[step] = v[0];
+  e_cons[step] = v[0] + es[0] - th[2];
+  counters[0] = step + 1;
+}
+
 // v *= scale[0] (the pending thermostat scale at the end of a run).
 __global__ __launch_bounds__(256) void k_scale_vel(int n, double* __restrict__ vel,
                                                    const double* __restrict__ scale) {
@@ -237,36 +508,70 @@ __global__ __launch_bounds__(256) void k_scale_vel(int n, double* __restrict__ v
 
 struct MDState {
   DevBuf pos, vel, acc, F, mass, spc, es, ke, hist_ep, hist_ek, hist_ec, counters, flag;
-  DevBuf nhc, esite, flux_part, hist_J;
-  // thermostat: 0 none (NVE), 1 Nose-Hoover chain
+  DevBuf nhc, esite, flux_part, hist_J, ke_b, ke_a, p_part;
+  // thermostat: 0 none (NVE), 1 Nose-Hoover chain, 2 Langevin, 3 CSVR
   int thermo = 0, M = 0, every = 0;
-  double kT = 0.0, ndof = 0.0;
+  double kT = 0.0, ndof = 0.0, damping = 0.0, inv_mass = 0.0;
+  uint32_t k0 = 0, k1 = 0;
+  // velocity-scaling thermostats keep a pending scale in nhc[0]
+  bool scaled() const { return thermo == 1 || thermo == 3; }
 };
 
-// One step: [chain half-step;] drift (+ block displacement maxima); the skin
-// test (NVE: also the record of the previous state); the gated list rebuild;
-// forces; kick; [chain half-step and record;] [heat flux].  For NVE the
-// state after the last step is recorded by md_record after the loop.
+static void thermo_kernel(MDState& S, int mode, double dt, hipStream_t st, unsigned nb) {
+  if (S.thermo == 1)
+    hipLaunchKernelGGL(k_nhc, dim3(1), dim3(256), 0, st, (int)nb,
+                       (const double*)S.ke.as<double>(), S.nhc.as<double>(), S.M, dt, S.kT,
+                       S.ndof, mode, (const double*)S.es.as<double>(), S.counters.as<int>(),
+                       S.hist_ep.as<double>(), S.hist_ek.as<double>(), S.hist_ec.as<double>());
+  else if (S.thermo == 3)
+    hipLaunchKernelGGL(k_csvr, dim3(1), dim3(256), 0, st, (int)nb,
+                       (const double*)S.ke.as<double>(), S.nhc.as<double>(),
+                       exp(-0.5 * dt / S.damping), S.ndof, S.kT, S.k0, S.k1, mode,
+                       (const double*)S.es.as<double>(), S.counters.as<int>(),
+                       S.hist_ep.as<double>(), S.hist_ek.as<double>(), S.hist_ec.as<double>());
+  else if (S.thermo == 2)
+    hipLaunchKernelGGL(k_lang_rec, dim3(1), dim3(256), 0, st, (int)nb,
+                       (const double*)S.ke.as<double>(), (const double*)S.ke_b.as<double>(),
+                       (const double*)S.ke_a.as<double>(), mode, S.nhc.as<double>(),
+                       (const double*)S.es.as<double>(), S.counters.as<int>(),
+                       S.hist_ep.as<double>(), S.hist_ek.as<double>(), S.hist_ec.as<double>());
+}
+
+// One step: [thermostat half-step;] drift (+ block displacement maxima) --
+// for Langevin the B A O A sequence; the skin test (NVE: also the record of
+// the previous state); the gated list rebuild; forces; kick; [thermostat
+// half-step and record;] [heat flux].  For NVE the state after the last step
+// is recorded by md_record after the loop.
 static void md_step(Potential& pot, NList& nl, SetDev& set, MDState& S, int n, double dt,
                     hipStream_t st) {
   const unsigned nb = (unsigned)((n + 255) / 256);
-  const bool nhc = S.thermo == 1;
   nl.disp_part.ensure(sizeof(double) * nb);
-  if (nhc)
-    hipLaunchKernelGGL(k_nhc, dim3(1), dim3(256), 0, st, (int)nb,
-                       (const double*)S.ke.as<double>(), S.nhc.as<double>(), S.M, dt, S.kT,
-                       S.ndof, 1, (const double*)S.es.as<double>(), S.counters.as<int>(),
-                       S.hist_ep.as<double>(), S.hist_ek.as<double>(), S.hist_ec.as<double>());
-  hipLaunchKernelGGL(k_md_drift, dim3(nb), dim3(256), 0, st, n, S.pos.as<double>(),
-                     S.vel.as<double>(), (const double*)S.acc.as<double>(), dt, set.geoms[0],
-                     (const double*)nl.ref_pos.as<double>(), nl.disp_part.as<double>(),
-                     (const double*)(nhc ? S.nhc.as<double>() + 1 : nullptr));
+  if (S.scaled()) thermo_kernel(S, 1, dt, st, nb);
+  if (S.thermo == 2) {
+    hipLaunchKernelGGL(k_lang_bao, dim3(nb), dim3(256), 0, st, n, S.pos.as<double>(),
+                       S.vel.as<double>(), (const double*)S.acc.as<double>(),
+                       (const double*)S.mass.as<double>(), dt, exp(-dt / S.damping),
+                       S.kT * kConv, S.k0, S.k1, (const int*)S.counters.as<int>(),
+                       S.ke_b.as<double>(), S.p_part.as<double>());
+    hipLaunchKernelGGL(k_lang_com, dim3(1), dim3(256), 0, st, (int)nb,
+                       (const double*)S.p_part.as<double>(), S.inv_mass, dt > 0.0 ? 1 : 0,
+                       S.nhc.as<double>());
+    hipLaunchKernelGGL(k_lang_a2, dim3(nb), dim3(256), 0, st, n, S.pos.as<double>(),
+                       S.vel.as<double>(), (const double*)S.mass.as<double>(), dt, set.geoms[0],
+                       (const double*)nl.ref_pos.as<double>(), (const double*)S.nhc.as<double>(),
+                       nl.disp_part.as<double>(), S.ke_a.as<double>());
+  } else {
+    hipLaunchKernelGGL(k_md_drift, dim3(nb), dim3(256), 0, st, n, S.pos.as<double>(),
+                       S.vel.as<double>(), (const double*)S.acc.as<double>(), dt, set.geoms[0],
+                       (const double*)nl.ref_pos.as<double>(), nl.disp_part.as<double>(),
+                       (const double*)(S.scaled() ? S.nhc.as<double>() + 1 : nullptr));
+  }
   hipLaunchKernelGGL(k_md_flag, dim3(1), dim3(256), 0, st, (int)nb,
                      (const double*)S.ke.as<double>(), (const double*)S.es.as<double>(),
                      (const double*)nl.disp_part.as<double>(), nl.r_list, nl.r_cut,
                      S.flag.as<int>(), S.counters.as<int>(),
-                     nhc ? (double*)nullptr : S.hist_ep.as<double>(), S.hist_ek.as<double>(),
-                     S.hist_ec.as<double>());
+                     S.thermo ? (double*)nullptr : S.hist_ep.as<double>(),
+                     S.hist_ek.as<double>(), S.hist_ec.as<double>());
   nl_build_gated(nl, set, S.pos.as<double>(), S.flag.as<int>(), st);
   if (S.every > 0)
     site_grads(pot, nl, set, S.pos.as<double>(), S.spc.as<int>(), S.F.as<double>(),
@@ -277,24 +582,17 @@ static void md_step(Potential& pot, NList& nl, SetDev& set, MDState& S, int n, d
   hipLaunchKernelGGL(k_md_kick, dim3(nb), dim3(256), 0, st, n, S.vel.as<double>(),
                      S.acc.as<double>(), (const double*)S.F.as<double>(),
                      (const double*)S.mass.as<double>(), dt, 1, S.ke.as<double>());
-  if (nhc)
-    hipLaunchKernelGGL(k_nhc, dim3(1), dim3(256), 0, st, (int)nb,
-                       (const double*)S.ke.as<double>(), S.nhc.as<double>(), S.M, dt, S.kT,
-                       S.ndof, 2, (const double*)S.es.as<double>(), S.counters.as<int>(),
-                       S.hist_ep.as<double>(), S.hist_ek.as<double>(), S.hist_ec.as<double>());
+  if (S.thermo) thermo_kernel(S, S.thermo == 2 ? 1 : 2, dt, st, nb);
   if (S.every > 0)
     heat_flux(pot, nl, set, S.pos.as<double>(), S.vel.as<double>(), S.mass.as<double>(),
-              S.esite.as<double>(), nhc ? S.nhc.as<double>() : nullptr, kConv,
+              S.esite.as<double>(), S.scaled() ? S.nhc.as<double>() : nullptr, kConv,
               S.counters.as<int>(), S.every, S.flux_part.as<double>(), S.hist_J.as<double>(), st);
 }
 
 static void md_record(MDState& S, int n, hipStream_t st) {
   const unsigned nb = (unsigned)((n + 255) / 256);
-  if (S.thermo == 1) {
-    hipLaunchKernelGGL(k_nhc, dim3(1), dim3(256), 0, st, (int)nb,
-                       (const double*)S.ke.as<double>(), S.nhc.as<double>(), S.M, 0.0, S.kT,
-                       S.ndof, 0, (const double*)S.es.as<double>(), S.counters.as<int>(),
-                       S.hist_ep.as<double>(), S.hist_ek.as<double>(), S.hist_ec.as<double>());
+  if (S.thermo) {
+    thermo_kernel(S, 0, 0.0, st, nb);
     return;
   }
   hipLaunchKernelGGL(k_md_flag, dim3(1), dim3(256), 0, st, (int)nb,
@@ -308,8 +606,8 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
             double skin, bool use_graph, const MDOptions& opt, double* e_pot, double* e_kin,
             double* e_cons, double* flux, int* n_rebuilds, double* seconds, hipStream_t st) {
   AK_REQUIRE(n > 0 && n_steps >= 0 && dt > 0.0, AK_ERR_ARG, "bad MD arguments");
-  AK_REQUIRE(opt.thermostat == 0 || opt.thermostat == 1, AK_ERR_ARG,
-             "thermostat must be 0 (NVE) or 1 (Nose-Hoover chain)");
+  AK_REQUIRE(opt.thermostat >= 0 && opt.thermostat <= 3, AK_ERR_ARG,
+             "thermostat must be 0 (NVE), 1 (Nose-Hoover chain), 2 (Langevin) or 3 (CSVR)");
   AK_REQUIRE(opt.flux_every >= 0, AK_ERR_ARG, "flux_every must be >= 0");
   SetDev set;
   NList nl;
@@ -358,6 +656,29 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
     AK_HIP(hipMemcpyAsync(S.nhc.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
                           st));
     AK_HIP(hipStreamSynchronize(st));
+  } else if (S.thermo == 2 || S.thermo == 3) {
+    // Langevin (damping = 1/gamma) and CSVR (damping = tau of K): state
+    // [pending scale, drift scale, heat Q, -, v_com (3)]; n_dof = 3N - 3
+    AK_REQUIRE(opt.temperature > 0.0 && opt.tau > 0.0, AK_ERR_ARG,
+               "thermostat temperature and damping must be positive");
+    AK_REQUIRE(n >= 2, AK_ERR_ARG, "a stochastic thermostat needs at least two atoms");
+    S.kT = kBoltzmannEv * opt.temperature;
+    S.ndof = (double)std::max(3 * n - 3, 1);
+    S.damping = opt.tau;
+    S.k0 = (uint32_t)(opt.seed & 0xffffffffu);
+    S.k1 = (uint32_t)(opt.seed >> 32);
+    double M = 0.0;
+    for (int a = 0; a < n; ++a) M += mass[a];
+    S.inv_mass = 1.0 / M;
+    std::vector<double> h(8, 0.0);
+    h[0] = h[1] = 1.0;
+    S.nhc.ensure(sizeof(double) * h.size());
+    AK_HIP(hipMemcpyAsync(S.nhc.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
+                          st));
+    S.ke_b.ensure(sizeof(double) * nb);
+    S.ke_a.ensure(sizeof(double) * nb);
+    S.p_part.ensure(sizeof(double) * 3 * nb);
+    AK_HIP(hipStreamSynchronize(st));
   }
   AK_HIP(hipMemcpyAsync(S.pos.p, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
   AK_HIP(hipMemcpyAsync(S.vel.p, vel, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
@@ -397,7 +718,7 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
   // state k - 1 (NVE) or k (chain; state 0 recorded here)
   int reset[4] = {0, 0, 0, 0};
   AK_HIP(hipMemcpy(S.counters.as<int>(), reset, sizeof(reset), hipMemcpyHostToDevice));
-  if (S.thermo == 1) md_record(S, n, st);
+  if (S.thermo) md_record(S, n, st);
 
   auto t0 = std::chrono::steady_clock::now();
   if (use_graph && n_steps > 0) {
@@ -423,7 +744,7 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
     auto t1 = std::chrono::steady_clock::now();
     if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
   }
-  if (S.thermo == 1)
+  if (S.scaled())
     hipLaunchKernelGGL(k_scale_vel, dim3((unsigned)((3 * n + 255) / 256)), dim3(256), 0, st, n,
                        S.vel.as<double>(), (const double*)S.nhc.as<double>());
   AK_HIP(hipStreamSynchronize(st));

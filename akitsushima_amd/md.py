@@ -48,7 +48,8 @@ def masses_of(atom_types):
 def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_steps=1000,
            skin_width=0.3, masses=None, use_graph=True, thermostat=None, flux_every=0):
     """n_steps of velocity Verlet on the device (``ak_md_run``).  thermostat:
-    None (NVE) or ("nhc", temperature K, tau fs, chain links).  Returns a
+    None (NVE), ("nhc", temperature K, tau fs, chain links), ("langevin",
+    temperature, damping fs, seed) or ("csvr", temperature, damping, seed).  Returns a
     dict with the final positions and velocities, e_pot, e_kin and e_cons
     (E_tot, or H_NHC with a chain) per step (n_steps + 1, index 0 the
     initial state), the number of list rebuilds, the loop's wall time and,
@@ -69,10 +70,16 @@ def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_step
     J = np.zeros((max(frames, 1), 3))
     th = None
     if thermostat is not None:
-        kind, T, tau, chain = thermostat
-        if kind != "nhc":
-            raise ValueError("thermostat: only 'nhc' (Nose-Hoover chain) runs on the device")
-        th = np.array([1.0, float(T), float(tau), float(chain)])
+        kind = thermostat[0]
+        if kind == "nhc":
+            _, T, tau, chain = thermostat
+            th = np.array([1.0, float(T), float(tau), float(chain), 0.0])
+        elif kind in ("langevin", "csvr"):
+            _, T, damping, seed = thermostat
+            th = np.array([2.0 if kind == "langevin" else 3.0, float(T), float(damping), 0.0,
+                           float(int(seed))])
+        else:
+            raise ValueError(f"thermostat: unknown kind {kind!r} (nhc, langevin, csvr)")
     nreb = ctypes.c_int32(0)
     secs = ctypes.c_double(0.0)
     P = _lib._pf64
@@ -113,4 +120,31 @@ def nvt_run(potential, lattice, positions, velocities, atom_types, temperature, 
                  flux_every)
     n_dof = max(3 * len(positions) - 3, 1)
     out["T"] = 2.0 * out["e_kin"] / (n_dof * K_B)
+    return out
+
+
+def langevin_run(potential, lattice, positions, velocities, atom_types, temperature, dt=1.0,
+                 n_steps=1000, damping=100.0, seed=42, skin_width=0.5, masses=None,
+                 use_graph=True, flux_every=0):
+    """NVT with a Langevin thermostat integrated by BAOAB on the device
+    (LangevinSimulation, solvers/molecular_dynamics.py:1511-1587): e_cons is
+    E_tot - Q, Q the heat the O steps and the centre-of-mass removal added.
+    The noise is Philox4x32-10 keyed by ``seed`` (the reference draws from
+    numpy's generator: same process, other realisations)."""
+    out = md_run(potential, lattice, positions, velocities, atom_types, dt, n_steps,
+                 skin_width, masses, use_graph, ("langevin", temperature, damping, seed),
+                 flux_every)
+    out["T"] = 2.0 * out["e_kin"] / (max(3 * len(positions) - 3, 1) * K_B)
+    return out
+
+
+def csvr_run(potential, lattice, positions, velocities, atom_types, temperature, dt=1.0,
+             n_steps=1000, damping=100.0, seed=42, skin_width=0.5, masses=None,
+             use_graph=True, flux_every=0):
+    """NVT with the Bussi-Donadio-Parrinello velocity rescaling on the device
+    (CSVRSimulation, solvers/molecular_dynamics.py:1589-1687): R B A F B R,
+    e_cons = E_tot - Q; random numbers as in :func:`langevin_run`."""
+    out = md_run(potential, lattice, positions, velocities, atom_types, dt, n_steps,
+                 skin_width, masses, use_graph, ("csvr", temperature, damping, seed), flux_every)
+    out["T"] = 2.0 * out["e_kin"] / (max(3 * len(positions) - 3, 1) * K_B)
     return out

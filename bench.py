@@ -224,6 +224,36 @@ def sw_md(n_steps=1000):
             "e_tot_per_atom_start_eV": float(res["e_tot"][0] / n)}
 
 
+def sw_gk(n_steps=1000):
+    """The device MD variants on the SW Si 20^3 cell (64000 atoms, 1000 K
+    start, dt 1 fs): NVE with the heat flux recorded every step (a Green-Kubo
+    production run) and NVT with a 3-link Nose-Hoover chain (900 K, tau 100
+    fs); ms per hipGraph step and the conserved quantity's range."""
+    from akitsushima_amd import StillingerWeberPotential
+    from akitsushima_amd.md import masses_of, maxwell_boltzmann, md_run, nvt_run
+    from akitsushima_amd.published import SW_SI
+    from akitsushima_amd.structures import DIAMOND, cubic
+
+    lat, pos, types = cubic(DIAMOND, 5.431, 20, ["Si"] * 8, rattle=0.0, seed=0)
+    pot = StillingerWeberPotential(dict(SW_SI), species=["Si"])
+    m = masses_of(types)
+    vel = maxwell_boltzmann(1000.0, m, 0)
+    n = len(pos)
+    md_run(pot, lat, pos, vel, types, dt=1.0, n_steps=10, skin_width=0.3, flux_every=1)
+    a = md_run(pot, lat, pos, vel, types, dt=1.0, n_steps=n_steps, skin_width=0.3,
+               flux_every=1)
+    b = nvt_run(pot, lat, pos, vel, types, 900.0, dt=1.0, n_steps=n_steps, tau_nhc=100.0,
+                nhc_chain=3, skin_width=0.3)
+    return {"workload": f"SW Si diamond 20^3 ({n} atoms), dt 1 fs, {n_steps} steps, "
+                        "skin 0.3 A, hipGraph-captured steps",
+            "nve_flux_every_step_ms_per_step": a["seconds"] / n_steps * 1e3,
+            "nve_flux_frames": int(len(a["flux"])),
+            "nve_flux_drift_per_atom_max_abs_eV": float(np.abs(a["drift"]).max() / n),
+            "nvt_nhc_ms_per_step": b["seconds"] / n_steps * 1e3,
+            "nvt_h_nhc_range_per_atom_eV": float((b["e_cons"].max() - b["e_cons"].min()) / n),
+            "nvt_T_final_K": float(b["T"][-1])}
+
+
 def eam_cuni(steps, warmup):
     """Zhou 2004 Cu-Ni random alloy: fcc 40^3 (256000 atoms), a = 3.56 A,
     species drawn 50/50 from seed 0, rattle 0.03 A (seed 1)."""
@@ -580,6 +610,7 @@ def main():
         out["batched_tersoff"] = batched_tersoff(10)
         out["sw_elastic"] = sw_elastic()
         out["sw_gesn"] = sw_gesn(50, 5, cpu=not args.no_cpu)
+        out["sw_gk"] = sw_gk(1000)
     print(json.dumps(out))
     return 0
 

@@ -729,16 +729,99 @@ def nhc_half(state, dt, kT, n_dof, vel, masses):
     return vel
 
 
+# Philox4x32-10 (Salmon, Moraes, Dror & Shaw, SC'11), the counter-based
+# generator of the device's stochastic thermostats; numpy over uint64 lanes.
+_PM0, _PM1 = np.uint64(0xD2511F53), np.uint64(0xCD9E8D57)
+_PW0, _PW1 = np.uint64(0x9E3779B9), np.uint64(0xBB67AE85)
+_M32 = np.uint64(0xFFFFFFFF)
+
+
+def philox4x32(c, k0, k1):
+    """c: (4, ...) counters; returns the (4, ...) output block."""
+    c = [np.asarray(x, np.uint64) & _M32 for x in c]
+    k0 = np.uint64(k0) & _M32
+    k1 = np.uint64(k1) & _M32
+    for _ in range(10):
+        p0 = _PM0 * c[0]
+        p1 = _PM1 * c[2]
+        c = [(p1 >> np.uint64(32)) ^ c[1] ^ k0, p1 & _M32, (p0 >> np.uint64(32)) ^ c[3] ^ k1,
+             p0 & _M32]
+        k0 = (k0 + _PW0) & _M32
+        k1 = (k1 + _PW1) & _M32
+    return c
+
+
+def _u53(a, b):
+    return ((a >> np.uint64(5)).astype(float) * 67108864.0
+            + (b >> np.uint64(6)).astype(float)) * (1.0 / 9007199254740992.0)
+
+
+def normal2(q):
+    """Box-Muller of one Philox block: r = sqrt(-2 ln(1 - u1)), angle 2 pi u2."""
+    u1, u2 = _u53(q[0], q[1]), _u53(q[2], q[3])
+    r = np.sqrt(-2.0 * np.log(1.0 - u1))
+    t = 6.283185307179586 * u2
+    return r * np.cos(t), r * np.sin(t)
+
+
+def gamma_mt(a, h, k0, k1):
+    """Marsaglia-Tsang Gamma(a >= 1) with the device's draw order."""
+    d = a - 1.0 / 3.0
+    c = 1.0 / math.sqrt(9.0 * d)
+    for it in range(64):
+        x = float(normal2(philox4x32([2 * it + 1, h, 3, 0], k0, k1))[0])
+        v = 1.0 + c * x
+        if v <= 0.0:
+            continue
+        v = v * v * v
+        q = philox4x32([2 * it + 2, h, 3, 0], k0, k1)
+        u = float(_u53(q[0], q[1]))
+        if u < 1.0 - 0.0331 * (x * x) * (x * x):
+            return d * v
+        if math.log(u) < 0.5 * x * x + d * (1.0 - v + math.log(v)):
+            return d * v
+    return d
+
+
+def csvr_alpha(K, c, nf, kT, h, k0, k1):
+    """CSVRSimulation._rescale (solvers/molecular_dynamics.py:1648-1667) with
+    R_1 and S drawn from the Philox stream h: (alpha, alpha^2 K - K)."""
+    if not K > 0.0 or nf < 1:
+        return 1.0, 0.0
+    Kt = 0.5 * nf * kT
+    r1 = float(normal2(philox4x32([0, h, 3, 0], k0, k1))[0])
+    sq = 2.0 * gamma_mt(0.5 * (nf - 1.0), h, k0, k1) if nf > 1 else 0.0
+    Kn = c * K + (1.0 - c) * Kt * (r1 * r1 + sq) / nf + 2.0 * r1 * math.sqrt(
+        c * (1.0 - c) * K * Kt / nf)
+    alpha = math.sqrt(max(Kn, 0.0) / K)
+    if r1 + math.sqrt(c * nf * K / ((1.0 - c) * Kt)) < 0.0:
+        alpha = -alpha
+    return alpha, alpha * alpha * K - K
+
+
+def langevin_noise(n, step, k0, k1):
+    """(n, 3) standard normals of the Langevin O step at `step`: Philox
+    blocks (a, step, 1, 0) and (a, step, 1, 1), Box-Muller."""
+    a = np.arange(n, dtype=np.uint64)
+    z0, z1 = normal2(philox4x32([a, np.full(n, step), np.ones(n), np.zeros(n)], k0, k1))
+    z2, _ = normal2(philox4x32([a, np.full(n, step), np.ones(n), np.ones(n)], k0, k1))
+    return np.stack([z0, z1, z2], axis=1)
+
+
 def md_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, skin,
-           nhc=None, flux=None, every=0):
+           nhc=None, flux=None, every=0, langevin=None, csvr=None):
     """Velocity Verlet with PBC wrap and the Verlet-skin rebuild test of
     engine/neighborlist.py:613-673 (fixed cell: rebuild when r_list - 2 d_max
     <= r_cut); ``force(lattice, pos, (i, j))`` returns (E, F, S).  nhc =
     (T, tau, M): NVTSimulation.run (solvers/molecular_dynamics.py:1113-1160),
     the step between two chain half-steps, Q_1 = n_dof kT tau^2, Q_j = kT
-    tau^2, n_dof = 3N - 3.  flux(pos, vel) -> J of the state after steps s
-    with s % every == 0.  Returns E_pot, E_kin, conserved quantity (E_tot or
-    H_NHC, :947-969), final positions and velocities, rebuilds, J frames."""
+    tau^2, n_dof = 3N - 3.  langevin = (T, damping, seed): BAOAB
+    (LangevinSimulation._step, :1552-1587); csvr = (T, damping, seed): R B A
+    F B R (CSVRSimulation._step, :1645-1687); both with the Philox streams of
+    the device.  flux(pos, vel) -> J of the state after steps s with s %
+    every == 0.  Returns E_pot, E_kin, conserved quantity (E_tot, H_NHC
+    (:947-969) or E_tot - Q), final positions and velocities, rebuilds, J
+    frames."""
     lattice = np.asarray(lattice, float)
     pos = np.array(positions, float)
     vel = np.array(velocities, float)
@@ -751,13 +834,35 @@ def md_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, sk
     ek = lambda v: 0.5 * float(np.sum(masses[:, None] * v ** 2)) / EV_ANG_TO_AMU_ANG_FS2
     n_dof = max(3 * len(pos) - 3, 1)
     state = None
+    heat = [0.0]
     if nhc is not None:
         T, tau, M = nhc
         kT = K_B * T
         state = ([n_dof * K_B * T * tau ** 2] + [K_B * T * tau ** 2] * (M - 1), [0.0] * M,
                  [0.0] * M)
+    stoch = langevin if langevin is not None else csvr
+    if stoch is not None:
+        T, damping, seed = stoch
+        kT = K_B * T
+        k0, k1 = int(seed) & 0xFFFFFFFF, int(seed) >> 32
+        M_tot = float(masses.sum())
+    half = [0]
+
+    def rescale(v):
+        a, dq = csvr_alpha(ek(v), math.exp(-0.5 * dt / damping), n_dof, kT, half[0], k0, k1)
+        half[0] += 1
+        heat[0] += dq
+        return v * a
+
+    def wrap(p):
+        f = p @ inv
+        f -= np.floor(f)
+        f[f >= 1.0] = 0.0
+        return f @ lattice
 
     def cons(e_pot, e_kin):
+        if stoch is not None:
+            return e_kin + e_pot - heat[0]
         if state is None:
             return e_pot + e_kin
         Q, xi, v = state
@@ -774,12 +879,21 @@ def md_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, sk
     for step in range(n_steps):
         if state is not None:
             vel = nhc_half(state, dt, kT, n_dof, vel, masses)
-        vel += 0.5 * dt * acc
-        pos += dt * vel
-        frac = pos @ inv
-        frac -= np.floor(frac)
-        frac[frac >= 1.0] = 0.0
-        pos = frac @ lattice
+        if csvr is not None:
+            vel = rescale(vel)
+        if langevin is not None:
+            c = math.exp(-dt / damping)
+            vel = vel + 0.5 * dt * acc
+            pos = pos + 0.5 * dt * vel
+            before = ek(vel)
+            sig = np.sqrt((1.0 - c * c) * kT * EV_ANG_TO_AMU_ANG_FS2 / masses)
+            vel = c * vel + sig[:, None] * langevin_noise(len(pos), step, k0, k1)
+            vel = vel - np.sum(masses[:, None] * vel, axis=0) / M_tot
+            heat[0] += ek(vel) - before
+            pos = wrap(pos + 0.5 * dt * vel)
+        else:
+            vel += 0.5 * dt * acc
+            pos = wrap(pos + dt * vel)
         dr = pos - ref
         fr = dr @ inv
         fr -= np.round(fr)
@@ -793,6 +907,8 @@ def md_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, sk
         vel += 0.5 * dt * acc
         if state is not None:
             vel = nhc_half(state, dt, kT, n_dof, vel, masses)
+        if csvr is not None:
+            vel = rescale(vel)
         epot.append(e)
         ekin.append(ek(vel))
         econs.append(cons(e, ek(vel)))

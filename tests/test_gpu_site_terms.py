@@ -15,7 +15,7 @@ import oracle as O
 import oracle.site_terms as ST
 from akitsushima_amd import (NeighbourList, StillingerWeberPotential, TersoffPotential,
                              ThermalConductivityCalculator, read_setfl, zhou_eam)
-from akitsushima_amd.md import maxwell_boltzmann, md_run, nvt_run
+from akitsushima_amd.md import csvr_run, langevin_run, maxwell_boltzmann, md_run, nvt_run
 from akitsushima_amd.published import ATOMIC_MASS, ZHOU_2004
 from akitsushima_amd.structures import DIAMOND, FCC, cubic
 
@@ -166,9 +166,17 @@ def test_device_heat_flux_is_the_time_derivative_of_the_energy_moment(name):
     assert np.abs(J - J_fd).max() < 1e-5 * np.abs(J_fd).max(), (J, J_fd)
 
 
-@pytest.mark.parametrize("thermo", [None, (700.0, 40.0, 3)])
+THERMOSTATS = {
+    "nve": None,
+    "nhc": ("nhc", 700.0, 40.0, 3),
+    "langevin": ("langevin", 700.0, 40.0, 12345),
+    "csvr": ("csvr", 700.0, 40.0, (1 << 40) + 99),
+}
+
+
+@pytest.mark.parametrize("thermo", list(THERMOSTATS))
 @pytest.mark.parametrize("name", ["sw", "tersoff_xy", "eam_cuni"])
-def test_md_flux_series_and_nvt_match_the_oracle(name, thermo):
+def test_md_flux_series_and_thermostats_match_the_oracle(name, thermo):
     make, terms, cell = CASES[name]
     pot = make()
     lat, pos, t = cell()
@@ -176,13 +184,9 @@ def test_md_flux_series_and_nvt_match_the_oracle(name, thermo):
         [63.546 if x == "Cu" else 58.693 for x in t])
     vel = maxwell_boltzmann(500.0, m, 4)
     n_steps, every, skin = 40, 3, 0.3
-    if thermo is None:
-        res = md_run(pot, lat, pos, vel, t, dt=1.0, n_steps=n_steps, skin_width=skin, masses=m,
-                     flux_every=every)
-    else:
-        res = nvt_run(pot, lat, pos, vel, t, thermo[0], dt=1.0, n_steps=n_steps,
-                      tau_nhc=thermo[1], nhc_chain=thermo[2], skin_width=skin, masses=m,
-                      flux_every=every)
+    th = THERMOSTATS[thermo]
+    res = md_run(pot, lat, pos, vel, t, dt=1.0, n_steps=n_steps, skin_width=skin, masses=m,
+                 thermostat=th, flux_every=every)
     if name == "sw":
         force = lambda l, p, pr: O.sw_flat(l, p, SW_SI, pairs=pr)
     elif name == "tersoff_xy":
@@ -191,8 +195,11 @@ def test_md_flux_series_and_nvt_match_the_oracle(name, thermo):
         force = lambda l, p, pr: O.eam_zhou(l, p, t, CUNI, ["Cu", "Ni"], pairs=pr)
     V = abs(np.linalg.det(lat))
     flux = lambda p, v: terms(lat, p, t).heat_flux(v, m, V)
+    kw = {}
+    if th is not None:
+        kw = {"nhc": th[1:]} if th[0] == "nhc" else {th[0]: th[1:]}
     ep, ek, ec, p1, v1, nreb, J = O.md_run(force, lat, pos, vel, m, 1.0, n_steps, pot.r_cut,
-                                           skin, nhc=thermo, flux=flux, every=every)
+                                           skin, flux=flux, every=every, **kw)
     assert np.abs(res["e_pot"] - ep).max() < 1e-9 * np.abs(ep).max()
     assert np.abs(res["e_kin"] - ek).max() < 1e-9 * np.abs(ek).max()
     assert np.abs(res["e_cons"] - ec).max() < 1e-9 * np.abs(ec).max()
@@ -216,6 +223,21 @@ def test_nvt_conserves_h_and_reaches_the_temperature():
     ec = res["e_cons"]
     assert ec.max() - ec.min() < 2e-3 * (et.max() - et.min())
     assert abs(res["T"][3000:].mean() - 900.0) < 0.05 * 900.0
+
+
+@pytest.mark.parametrize("run", ["langevin", "csvr"])
+def test_stochastic_thermostats_sample_the_temperature(run):
+    lat, pos, t = cubic(DIAMOND, 5.431, 4, ["Si"] * 8, rattle=0.02, seed=1)
+    pot = StillingerWeberPotential(SW_SI, species=["Si"])
+    m = np.full(len(pos), 28.086)
+    vel = maxwell_boltzmann(300.0, m, 2)
+    fn = langevin_run if run == "langevin" else csvr_run
+    res = fn(pot, lat, pos, vel, t, 900.0, dt=1.0, n_steps=4000, damping=50.0, seed=3)
+    et, ec = res["e_tot"], res["e_cons"]
+    assert ec.max() - ec.min() < 5e-3 * (et.max() - et.min())
+    assert abs(res["T"][1000:].mean() - 900.0) < 0.03 * 900.0
+    P = (m[:, None] * res["velocities"]).sum(axis=0)
+    assert np.abs(P).max() < 1e-9 * np.abs(m[:, None] * res["velocities"]).sum()
 
 
 def test_green_kubo_run_uses_the_device_flux():

@@ -78,3 +78,59 @@ def test_kappa_of_an_ar1_flux_matches_its_closed_form():
     assert np.all(np.isfinite(tc._kappa_std))
     kc = tc.compute_kappa(np.ones((1000, 3)) * 3e-5, dt_hf=dt, tau_max=0.2, n_blocks=2)
     assert np.abs(kc).max() < 1e-20
+
+
+# --------------------------------------------------------------------------
+#  stochastic thermostats: the Philox stream and its distributions, then the
+#  integrators' E - Q conservation and canonical statistics
+# --------------------------------------------------------------------------
+
+def test_philox_known_answers():
+    """Known-answer vectors of Philox4x32-10 (Random123's kat_vectors)."""
+    for c, k, out in [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+                      ((0xffffffff,) * 4, (0xffffffff, 0xffffffff),
+                       (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+                      ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+                       (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1))]:
+        got = O.philox4x32(list(c), *k)
+        assert [int(x) for x in got] == list(out)
+
+
+def test_normals_and_gamma_have_their_moments():
+    n = 400_000
+    a = np.arange(n, dtype=np.uint64)
+    z0, z1 = O.normal2(O.philox4x32([a, np.zeros(n), np.ones(n), np.zeros(n)], 11, 0))
+    z = np.concatenate([z0, z1])
+    assert abs(z.mean()) < 5e-3 and abs(z.var() - 1.0) < 1e-2
+    assert abs(np.mean(z ** 4) - 3.0) < 0.05
+    for shape in (1.5, 40.0):
+        g = np.array([O.gamma_mt(shape, h, 5, 9) for h in range(4000)])
+        assert abs(g.mean() - shape) < 4 * np.sqrt(shape / 4000)
+        assert abs(g.var() / shape - 1.0) < 0.15
+
+
+def _stoch(kind, dt, n):
+    lat, pos, t = si(2)
+    m = np.full(len(pos), 28.086)
+    vel = O.mb_velocities(300.0, m, 3)
+    force = lambda l, p, pr: O.sw_flat(l, p, SW_SI, pairs=pr)
+    out = O.md_run(force, lat, pos, vel, m, dt, n, 3.77118, 0.3, **{kind: (600.0, 50.0, 7)})
+    return out, len(pos)
+
+
+@pytest.mark.parametrize("kind", ["langevin", "csvr"])
+def test_stochastic_thermostats_conserve_e_minus_q_and_sample_600k(kind):
+    ranges = []
+    for dt in (1.0, 0.5):
+        (ep, ek, ec, *_), n = _stoch(kind, dt, int(2000 / dt))
+        et = ep + ek
+        ranges.append(ec.max() - ec.min())
+        assert ranges[-1] < 2e-3 * (et.max() - et.min())
+    assert 3.0 < ranges[0] / ranges[1] < 5.5
+    (ep, ek, ec, *_), n = _stoch(kind, 1.0, 4000)
+    nf = 3 * n - 3
+    T = 2 * ek / (nf * O.K_B)
+    assert abs(T[1000:].mean() - 600.0) < 0.05 * 600.0
+    if kind == "csvr":   # canonical kinetic-energy variance (n_f / 2) (kT)^2
+        r = ek[1000:].var() / (0.5 * nf * (O.K_B * 600.0) ** 2)
+        assert 0.6 < r < 1.6
