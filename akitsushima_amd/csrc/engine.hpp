@@ -71,7 +71,7 @@ struct SetDev {
   int n_atoms = 0;
   int n_struct = 0;
   int n_cells = 0;
-  double setup_r_list = -1.0;
+  double setup_r_list = -1.0, setup_r_grid = 0.0;
   bool setup_checked = true;
   bool setup_had_inv = false;
   std::vector<Geom> geoms;
@@ -90,9 +90,10 @@ struct SetDev {
   };
   std::vector<std::unique_ptr<Chunks>> chunk_tables;
   Chunks& chunks(int size, hipStream_t st);
-  // (re)builds geometry for r_list; lattices (S, 9), invs may be null.
+  // (re)builds geometry for r_list; lattices (S, 9), invs may be null;
+  // r_grid > r_list sizes the cell grid for cells that may shrink later.
   void setup(int n_struct, const int64_t* offsets, const double* lattices, const double* invs,
-             double r_list, hipStream_t st, bool check_min_image = true);
+             double r_list, hipStream_t st, bool check_min_image = true, double r_grid = 0.0);
   void upload_geoms(hipStream_t st);
 };
 
@@ -217,15 +218,20 @@ bool site_masked(const Potential& pot, const NList& nl);
 
 // Device MD options: thermostat 0 = none (NVE), 1 = Nose-Hoover chain of
 // `chain` links at `temperature` K with time constant `tau` fs, 2 = Langevin
-// and 3 = CSVR with damping `tau` fs; heat flux of the state after every step
-// s with s % flux_every == 0 (0 = none).
+// and 3 = CSVR with damping `tau` fs, 4 = NPT (isotropic barostat at
+// pressure_gpa with time constant tau_baro, and the chain of 1); heat flux
+// of the state after every step s with s % flux_every == 0 (0 = none).
 struct MDOptions {
-  int thermostat = 0;  // 0 NVE, 1 Nose-Hoover chain, 2 Langevin (BAOAB), 3 CSVR
+  int thermostat = 0;  // 0 NVE, 1 Nose-Hoover chain, 2 Langevin (BAOAB), 3 CSVR, 4 NPT
   double temperature = 0.0, tau = 0.0;  // tau: chain time constant or damping, fs
   int chain = 3;
   uint64_t seed = 0;  // random stream of the stochastic thermostats
+  double pressure_gpa = 0.0, tau_baro = 0.0;  // NPT target pressure and barostat time
   int flux_every = 0;
 };
+
+// eV/A^3 -> GPa (constants.py EV_ANG3_TO_GPA)
+constexpr double kEvAng3ToGpa = 160.2176621;
 
 // md.hip: n_steps of velocity Verlet on one cell, positions and velocities
 // updated in place; e_pot, e_kin, e_cons (n_steps + 1) and flux
@@ -233,7 +239,8 @@ struct MDOptions {
 void md_run(Potential& pot, int n, const double* lattice, const double* inv, double* pos,
             double* vel, const double* mass, const int32_t* spc, double dt, int n_steps,
             double skin, bool use_graph, const MDOptions& opt, double* e_pot, double* e_kin,
-            double* e_cons, double* flux, int* n_rebuilds, double* seconds, hipStream_t st);
+            double* e_cons, double* flux, double* lattice_out, double* volume,
+            double* pressure, int* n_rebuilds, double* seconds, hipStream_t st);
 
 // Heat flux J = [sum_a E_a v_a - sum_(a,k) v_ak (G[a,k] . v_nbr)] / V of one
 // cell after site_grads (d_part: one triple per 32 atoms).  With d_gate, only

@@ -499,6 +499,204 @@ __global__ __launch_bounds__(256) void k_lang_rec(int nb, const double* __restri
   counters[0] = step + 1;
 }
 
+// ---------------------------------------------------------------------------
+// NPT: isotropic MTTK barostat with a 2-link chain on its momentum and the
+// particle Nose-Hoover chain (NPTSimulation, solvers/molecular_dynamics.py:
+// 2141-2555, cell_dof "iso", baro_thermostat "nhc"; step :2871-2895).  The
+// cell is s L_0; the device Geom of the set is scaled in place each step, so
+// the force and list kernels see the current cell.  State np[]: 0 s, 1 s at
+// the last list build, 2 p_eps, 3 xi1_b, 4 xi2_b, 5 vxi1_b, 6 vxi2_b,
+// 7 this step's cell scale exp(eps_dot dt), 8 its velocity factor
+// exp(-(1 + 3/n_dof) eps_dot dt), 9 the count of steps whose cell left the
+// range the cell grid was sized for.
+// ---------------------------------------------------------------------------
+struct NptParams {
+  double kT, ndof, W, Qb, P_ext, s_min;
+};
+
+// _nhc_baro_step (:2456-2494): two links on the barostat momentum.
+__device__ void npt_baro_nhc(double* __restrict__ np, const NptParams& p, double dt) {
+  const double e1 = 0.5 * np[2] * np[2] / p.W;
+  double G1 = (2.0 * e1 - p.kT) / p.Qb;
+  double G2 = (p.Qb * np[5] * np[5] - p.kT) / p.Qb;
+  np[6] += G2 * (dt * 0.25);
+  const double sc = exp(fmin(fmax(-np[6] * dt * 0.125, -350.0), 350.0));
+  np[5] = sc * (sc * np[5] + G1 * dt * 0.25);
+  np[2] *= exp(fmin(fmax(-np[5] * dt * 0.5, -350.0), 350.0));
+  np[3] += np[5] * dt * 0.5;
+  np[4] += np[6] * dt * 0.5;
+  const double e2 = 0.5 * np[2] * np[2] / p.W;
+  G1 = (2.0 * e2 - p.kT) / p.Qb;
+  np[5] = sc * (sc * np[5] + G1 * dt * 0.25);
+  G2 = (p.Qb * np[5] * np[5] - p.kT) / p.Qb;
+  np[6] += G2 * (dt * 0.25);
+}
+
+// _barostat_half_step (:2496-2515): p_eps += dt/2 [3 V (P - P_ext) + (3/n_f) 2 E_kin].
+__device__ __forceinline__ void npt_baro_half(double* __restrict__ np, const NptParams& p,
+                                              double dt, double V, double P, double ek) {
+  np[2] += 0.5 * dt * (3.0 * V * (P - p.P_ext) + (3.0 / p.ndof) * 2.0 * ek);
+}
+
+// One block.  mode 1 (start of a step): P_inst of the current state, the
+// particle chain, the barostat chain and half-step, and this step's cell and
+// velocity factors.  mode 2 (end): P_inst of the new state, barostat
+// half-step and chain, particle chain (its scale pending), record of E_pot,
+// E_kin, H_NPT (_npt_conserved :2425-2454), V and P_inst.  mode 0: record.
+__global__ __launch_bounds__(256) void k_npt(int nb, const double* __restrict__ ke_part,
+                                             double* __restrict__ nhc, int M,
+                                             double* __restrict__ np, NptParams p, double dt,
+                                             int mode, const double* __restrict__ es,
+                                             const Geom* __restrict__ geom,
+                                             int* __restrict__ counters,
+                                             double* __restrict__ e_pot,
+                                             double* __restrict__ e_kin,
+                                             double* __restrict__ e_cons,
+                                             double* __restrict__ h_vol,
+                                             double* __restrict__ h_press) {
+  __shared__ double smem[16];
+  double v[1] = {0.0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += ke_part[b];
+  block_sum<1>(v, smem);
+  if (threadIdx.x != 0) return;
+  const double V = geom->volume;
+  const double trs = (es[1] + es[5]) + es[9];
+  double ek = v[0];
+  if (mode == 1) {
+    const double pend = nhc[0];
+    ek *= pend * pend;
+    const double P = 2.0 * ek / (3.0 * V) + trs / 3.0;
+    const double s1 = nhc_half(nhc, M, dt, p.kT, p.ndof, ek);
+    ek *= s1 * s1;
+    npt_baro_nhc(np, p, dt);
+    npt_baro_half(np, p, dt, V, P, ek);
+    const double ed = np[2] / p.W;
+    np[7] = exp(ed * dt);
+    np[8] = exp(-(1.0 + 3.0 / p.ndof) * ed * dt);
+    nhc[1] = pend * s1;
+    nhc[0] = 1.0;
+    return;
+  }
+  const double P = 2.0 * ek / (3.0 * V) + trs / 3.0;
+  if (mode == 2) {
+    npt_baro_half(np, p, dt, V, P, ek);
+    npt_baro_nhc(np, p, dt);
+    const double s2 = nhc_half(nhc, M, dt, p.kT, p.ndof, ek);
+    nhc[0] = s2;
+    ek *= s2 * s2;
+  }
+  const double* Q = nhc + 2;
+  const double* xi = Q + M;
+  const double* w = xi + M;
+  double t = 0.5 * Q[0] * w[0] * w[0];
+  for (int j = 1; j < M; ++j) t = t + 0.5 * Q[j] * w[j] * w[j];
+  double u = p.ndof * p.kT * xi[0];
+  for (int j = 1; j < M; ++j) u = u + p.kT * xi[j];
+  const double ed = np[2] / p.W;
+  const double hb = 0.5 * p.Qb * np[5] * np[5] + 0.5 * p.Qb * np[6] * np[6] + p.kT * np[3] +
+                    p.kT * np[4];
+  const int step = counters[0];
+  e_pot[step] = es[0];
+  e_kin[step] = ek;
+  e_cons[step] = ek + es[0] + p.P_ext * V + 0.5 * p.W * ed * ed + t + u + hb;
+  h_vol[step] = V;
+  h_press[step] = P;
+  counters[0] = step + 1;
+}
+
+// Drift of the NPT step (_scale_step :2517-2555): v = v d + dt/2 a;
+// x += dt v; wrapped in the current cell; frac @ (cell x scale); v *= vfac;
+// and the block's largest squared non-affine displacement from the build
+// positions mapped to the new cell (engine/neighborlist.py:646-668).
+__global__ __launch_bounds__(256) void k_npt_drift(int n, double* __restrict__ pos,
+                                                   double* __restrict__ vel,
+                                                   const double* __restrict__ acc, double dt,
+                                                   const Geom* __restrict__ geom,
+                                                   const double* __restrict__ ref,
+                                                   const double* __restrict__ nhc,
+                                                   const double* __restrict__ np,
+                                                   double* __restrict__ part) {
+  __shared__ double m[256];
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  double d2 = 0.0;
+  if (a < n) {
+    const double d = nhc[1], sc = np[7], vf = np[8];
+    const double aff = np[0] * sc / np[1];
+    double lat[9], inv[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      lat[k] = geom->lat[k] * sc;
+      inv[k] = geom->inv[k] / sc;
+    }
+    double x[3], v[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      v[q] = vel[3 * a + q] * d + 0.5 * dt * acc[3 * a + q];
+      x[q] = pos[3 * a + q] + dt * v[q];
+    }
+    double f[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      double sb = x[0] * geom->inv[b] + x[1] * geom->inv[3 + b] + x[2] * geom->inv[6 + b];
+      sb -= floor(sb);
+      f[b] = sb >= 1.0 ? 0.0 : sb;
+    }
+    double y[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      y[b] = f[0] * lat[b] + f[1] * lat[3 + b] + f[2] * lat[6 + b];
+      pos[3 * a + b] = y[b];
+      vel[3 * a + b] = v[b] * vf;
+    }
+    double dx, dy, dz;
+    mic_exact(lat, inv, ref[3 * a] * aff, ref[3 * a + 1] * aff, ref[3 * a + 2] * aff, y[0],
+              y[1], y[2], dx, dy, dz);
+    d2 = dx * dx + dy * dy + dz * dz;
+  }
+  m[threadIdx.x] = d2;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) m[threadIdx.x] = fmax(m[threadIdx.x], m[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = m[0];
+}
+
+// One block: the cell of the set scaled in place (lattice x sc, inverse /
+// sc, volume x sc^3), s advanced, and the list test with the affine bound
+// sigma_min = s / s_build (engine/neighborlist.py:613-673) -> *flag; a
+// rebuild records s_build = s.  Cells below the grid's range are counted.
+__global__ __launch_bounds__(256) void k_npt_flag(int nb, const double* __restrict__ disp_part,
+                                                  double r_list, double r_cut, double s_min,
+                                                  Geom* __restrict__ geom,
+                                                  double* __restrict__ np,
+                                                  int* __restrict__ flag,
+                                                  int* __restrict__ counters) {
+  __shared__ double m[256];
+  double d = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) d = fmax(d, disp_part[b]);
+  m[threadIdx.x] = d;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) m[threadIdx.x] = fmax(m[threadIdx.x], m[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const double sc = np[7];
+  for (int k = 0; k < 9; ++k) {
+    geom->lat[k] *= sc;
+    geom->inv[k] /= sc;
+  }
+  geom->volume *= sc * sc * sc;
+  const double s_new = np[0] * sc;
+  np[0] = s_new;
+  const int fl = ((s_new / np[1]) * r_list - 2.0 * sqrt(m[0]) <= r_cut) ? 1 : 0;
+  if (fl) np[1] = s_new;
+  if (s_new < s_min) np[9] += 1.0;
+  *flag = fl;
+  counters[1] += fl;
+}
+
 // v *= scale[0] (the pending thermostat scale at the end of a run).
 __global__ __launch_bounds__(256) void k_scale_vel(int n, double* __restrict__ vel,
                                                    const double* __restrict__ scale) {
@@ -508,13 +706,16 @@ __global__ __launch_bounds__(256) void k_scale_vel(int n, double* __restrict__ v
 
 struct MDState {
   DevBuf pos, vel, acc, F, mass, spc, es, ke, hist_ep, hist_ek, hist_ec, counters, flag;
-  DevBuf nhc, esite, flux_part, hist_J, ke_b, ke_a, p_part;
-  // thermostat: 0 none (NVE), 1 Nose-Hoover chain, 2 Langevin, 3 CSVR
+  DevBuf nhc, esite, flux_part, hist_J, ke_b, ke_a, p_part, np, hist_V, hist_P;
+  // thermostat: 0 none (NVE), 1 Nose-Hoover chain, 2 Langevin, 3 CSVR,
+  // 4 NPT (isotropic barostat + Nose-Hoover chain)
   int thermo = 0, M = 0, every = 0;
   double kT = 0.0, ndof = 0.0, damping = 0.0, inv_mass = 0.0;
   uint32_t k0 = 0, k1 = 0;
+  NptParams npt{};
+  Geom* geom = nullptr;  // the set's device cell
   // velocity-scaling thermostats keep a pending scale in nhc[0]
-  bool scaled() const { return thermo == 1 || thermo == 3; }
+  bool scaled() const { return thermo == 1 || thermo == 3 || thermo == 4; }
 };
 
 static void thermo_kernel(MDState& S, int mode, double dt, hipStream_t st, unsigned nb) {
@@ -529,6 +730,13 @@ static void thermo_kernel(MDState& S, int mode, double dt, hipStream_t st, unsig
                        exp(-0.5 * dt / S.damping), S.ndof, S.kT, S.k0, S.k1, mode,
                        (const double*)S.es.as<double>(), S.counters.as<int>(),
                        S.hist_ep.as<double>(), S.hist_ek.as<double>(), S.hist_ec.as<double>());
+  else if (S.thermo == 4)
+    hipLaunchKernelGGL(k_npt, dim3(1), dim3(256), 0, st, (int)nb,
+                       (const double*)S.ke.as<double>(), S.nhc.as<double>(), S.M,
+                       S.np.as<double>(), S.npt, dt, mode, (const double*)S.es.as<double>(),
+                       (const Geom*)S.geom, S.counters.as<int>(), S.hist_ep.as<double>(),
+                       S.hist_ek.as<double>(), S.hist_ec.as<double>(), S.hist_V.as<double>(),
+                       S.hist_P.as<double>());
   else if (S.thermo == 2)
     hipLaunchKernelGGL(k_lang_rec, dim3(1), dim3(256), 0, st, (int)nb,
                        (const double*)S.ke.as<double>(), (const double*)S.ke_b.as<double>(),
@@ -560,18 +768,30 @@ static void md_step(Potential& pot, NList& nl, SetDev& set, MDState& S, int n, d
                        S.vel.as<double>(), (const double*)S.mass.as<double>(), dt, set.geoms[0],
                        (const double*)nl.ref_pos.as<double>(), (const double*)S.nhc.as<double>(),
                        nl.disp_part.as<double>(), S.ke_a.as<double>());
+  } else if (S.thermo == 4) {
+    hipLaunchKernelGGL(k_npt_drift, dim3(nb), dim3(256), 0, st, n, S.pos.as<double>(),
+                       S.vel.as<double>(), (const double*)S.acc.as<double>(), dt,
+                       (const Geom*)S.geom, (const double*)nl.ref_pos.as<double>(),
+                       (const double*)S.nhc.as<double>(), (const double*)S.np.as<double>(),
+                       nl.disp_part.as<double>());
   } else {
     hipLaunchKernelGGL(k_md_drift, dim3(nb), dim3(256), 0, st, n, S.pos.as<double>(),
                        S.vel.as<double>(), (const double*)S.acc.as<double>(), dt, set.geoms[0],
                        (const double*)nl.ref_pos.as<double>(), nl.disp_part.as<double>(),
                        (const double*)(S.scaled() ? S.nhc.as<double>() + 1 : nullptr));
   }
-  hipLaunchKernelGGL(k_md_flag, dim3(1), dim3(256), 0, st, (int)nb,
-                     (const double*)S.ke.as<double>(), (const double*)S.es.as<double>(),
-                     (const double*)nl.disp_part.as<double>(), nl.r_list, nl.r_cut,
-                     S.flag.as<int>(), S.counters.as<int>(),
-                     S.thermo ? (double*)nullptr : S.hist_ep.as<double>(),
-                     S.hist_ek.as<double>(), S.hist_ec.as<double>());
+  if (S.thermo == 4)
+    hipLaunchKernelGGL(k_npt_flag, dim3(1), dim3(256), 0, st, (int)nb,
+                       (const double*)nl.disp_part.as<double>(), nl.r_list, nl.r_cut,
+                       S.npt.s_min, S.geom, S.np.as<double>(), S.flag.as<int>(),
+                       S.counters.as<int>());
+  else
+    hipLaunchKernelGGL(k_md_flag, dim3(1), dim3(256), 0, st, (int)nb,
+                       (const double*)S.ke.as<double>(), (const double*)S.es.as<double>(),
+                       (const double*)nl.disp_part.as<double>(), nl.r_list, nl.r_cut,
+                       S.flag.as<int>(), S.counters.as<int>(),
+                       S.thermo ? (double*)nullptr : S.hist_ep.as<double>(),
+                       S.hist_ek.as<double>(), S.hist_ec.as<double>());
   nl_build_gated(nl, set, S.pos.as<double>(), S.flag.as<int>(), st);
   if (S.every > 0)
     site_grads(pot, nl, set, S.pos.as<double>(), S.spc.as<int>(), S.F.as<double>(),
@@ -604,18 +824,26 @@ static void md_record(MDState& S, int n, hipStream_t st) {
 void md_run(Potential& pot, int n, const double* lattice, const double* inv, double* pos,
             double* vel, const double* mass, const int32_t* spc, double dt, int n_steps,
             double skin, bool use_graph, const MDOptions& opt, double* e_pot, double* e_kin,
-            double* e_cons, double* flux, int* n_rebuilds, double* seconds, hipStream_t st) {
+            double* e_cons, double* flux, double* lattice_out, double* volume,
+            double* pressure, int* n_rebuilds, double* seconds, hipStream_t st) {
   AK_REQUIRE(n > 0 && n_steps >= 0 && dt > 0.0, AK_ERR_ARG, "bad MD arguments");
-  AK_REQUIRE(opt.thermostat >= 0 && opt.thermostat <= 3, AK_ERR_ARG,
-             "thermostat must be 0 (NVE), 1 (Nose-Hoover chain), 2 (Langevin) or 3 (CSVR)");
+  AK_REQUIRE(opt.thermostat >= 0 && opt.thermostat <= 4, AK_ERR_ARG,
+             "thermostat must be 0 (NVE), 1 (Nose-Hoover chain), 2 (Langevin), 3 (CSVR) or "
+             "4 (NPT)");
   AK_REQUIRE(opt.flux_every >= 0, AK_ERR_ARG, "flux_every must be >= 0");
+  AK_REQUIRE(opt.thermostat != 4 || opt.flux_every == 0, AK_ERR_ARG,
+             "the heat flux is recorded at fixed volume (not under NPT)");
   SetDev set;
   NList nl;
   nl.r_cut = pot.r_cut;
   nl.skin = skin;
   nl.r_list = pot.r_cut + skin;
-  set.setup(1, std::vector<int64_t>{0, (int64_t)n}.data(), lattice, inv, nl.r_list, st);
+  // NPT: a cell grid that stays valid while the cell shrinks by up to 5 %
+  const double kNptShrink = 0.95;
+  set.setup(1, std::vector<int64_t>{0, (int64_t)n}.data(), lattice, inv, nl.r_list, st, true,
+            opt.thermostat == 4 ? nl.r_list / kNptShrink : 0.0);
   MDState S;
+  S.geom = set.d_geoms.as<Geom>();
   S.thermo = opt.thermostat;
   S.every = opt.flux_every;
   S.pos.ensure(sizeof(double) * 3 * n);
@@ -638,7 +866,7 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
     S.flux_part.ensure(sizeof(double) * 3 * ((size_t)n / 32 + 1));
     S.hist_J.ensure(sizeof(double) * 3 * (size_t)std::max(frames, 1));
   }
-  if (S.thermo == 1) {
+  if (S.thermo == 1 || S.thermo == 4) {
     // Martyna, Tuckerman & Klein masses: Q_1 = n_dof kT tau^2, Q_j = kT tau^2
     // (_init_nhc_chain, solvers/molecular_dynamics.py:575-588), n_dof = 3N - 3
     AK_REQUIRE(opt.chain >= 2 && opt.chain <= kNhcMax, AK_ERR_ARG,
@@ -655,6 +883,30 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
     S.nhc.ensure(sizeof(double) * h.size());
     AK_HIP(hipMemcpyAsync(S.nhc.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
                           st));
+    if (S.thermo == 4) {
+      // barostat mass W = (n_dof + 3) kT tau_b^2 and chain masses kT tau_b^2
+      // (NPTSimulation.__init__, :2370-2395); P_ext GPa -> eV/A^3
+      AK_REQUIRE(opt.tau_baro > 0.0, AK_ERR_ARG, "tau_baro must be positive");
+      S.npt.kT = S.kT;
+      S.npt.ndof = S.ndof;
+      S.npt.W = (S.ndof + 3.0) * kBoltzmannEv * opt.temperature * opt.tau_baro * opt.tau_baro;
+      S.npt.Qb = kBoltzmannEv * opt.temperature * opt.tau_baro * opt.tau_baro;
+      S.npt.P_ext = opt.pressure_gpa / kEvAng3ToGpa;
+      // the cell may shrink while the grid and the minimum image hold
+      const Geom& g = set.geoms[0];
+      CellInfo ci = cell_info(g.lat);
+      double s_min = 2.0 * nl.r_list / ci.min_width;
+      for (int ax = 0; ax < 3; ++ax)
+        s_min = std::max(s_min, nl.r_list * g.nc[ax] / ci.perp[ax]);
+      S.npt.s_min = s_min;
+      std::vector<double> q(16, 0.0);
+      q[0] = q[1] = q[7] = q[8] = 1.0;
+      S.np.ensure(sizeof(double) * q.size());
+      AK_HIP(hipMemcpyAsync(S.np.p, q.data(), sizeof(double) * q.size(), hipMemcpyHostToDevice,
+                            st));
+      S.hist_V.ensure(sizeof(double) * (n_steps + 1));
+      S.hist_P.ensure(sizeof(double) * (n_steps + 1));
+    }
     AK_HIP(hipStreamSynchronize(st));
   } else if (S.thermo == 2 || S.thermo == 3) {
     // Langevin (damping = 1/gamma) and CSVR (damping = tau of K): state
@@ -754,6 +1006,24 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
   AK_HIP(hipMemcpy(counters, S.counters.p, sizeof(counters), hipMemcpyDeviceToHost));
   AK_REQUIRE(counters[0] == n_steps + 1, AK_ERR_STATE, "MD step counter mismatch");
   AK_REQUIRE(S.every == 0 || counters[2] == n_steps, AK_ERR_STATE, "heat-flux gate mismatch");
+  if (S.thermo == 4) {
+    double q[16];
+    AK_HIP(hipMemcpy(q, S.np.p, sizeof(q), hipMemcpyDeviceToHost));
+    AK_REQUIRE(q[9] == 0.0, AK_ERR_CAPACITY,
+               "the NPT cell shrank beyond the range its neighbour grid and minimum image "
+               "were set up for (5 % linear); restart from the current cell");
+    Geom g;
+    AK_HIP(hipMemcpy(&g, S.geom, sizeof(Geom), hipMemcpyDeviceToHost));
+    if (lattice_out) std::memcpy(lattice_out, g.lat, sizeof(g.lat));
+    if (volume)
+      AK_HIP(hipMemcpy(volume, S.hist_V.p, sizeof(double) * (n_steps + 1),
+                       hipMemcpyDeviceToHost));
+    if (pressure)
+      AK_HIP(hipMemcpy(pressure, S.hist_P.p, sizeof(double) * (n_steps + 1),
+                       hipMemcpyDeviceToHost));
+  } else if (lattice_out) {
+    std::memcpy(lattice_out, lattice, 9 * sizeof(double));
+  }
   AK_HIP(hipMemcpy(pos, S.pos.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
   AK_HIP(hipMemcpy(vel, S.vel.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
   if (e_pot)

@@ -71,11 +71,12 @@ CellInfo cell_info(const double* L) {
 }
 
 void SetDev::setup(int S, const int64_t* offsets, const double* lattices, const double* invs,
-                   double r_list, hipStream_t st, bool check_min_image) {
+                   double r_list, hipStream_t st, bool check_min_image, double r_grid) {
   AK_REQUIRE(S >= 1, AK_ERR_ARG, "need at least one structure");
   // Repeated calls on an unchanged set (the common MD / benchmark case)
   // skip the host-side geometry work entirely.
-  if (S == n_struct && r_list == setup_r_list && check_min_image == setup_checked &&
+  if (S == n_struct && r_list == setup_r_list && r_grid == setup_r_grid &&
+      check_min_image == setup_checked &&
       (invs != nullptr) == setup_had_inv && (int)geoms.size() == S && d_geoms.p && d_sid.p) {
     bool same = true;
     for (int s = 0; s < S && same; ++s) {
@@ -109,7 +110,7 @@ void SetDev::setup(int S, const int64_t* offsets, const double* lattices, const 
       throw Error(AK_ERR_MIN_IMAGE, buf);
     }
     for (int ax = 0; ax < 3; ++ax) {
-      int nc = (int)std::floor(ci.perp[ax] / r_list);
+      int nc = (int)std::floor(ci.perp[ax] / std::max(r_list, r_grid));
       g.nc[ax] = nc < 1 ? 1 : nc;
     }
     g.cell_off = cells;
@@ -138,6 +139,7 @@ void SetDev::setup(int S, const int64_t* offsets, const double* lattices, const 
     chunk_tables.clear();
   }
   setup_r_list = r_list;
+  setup_r_grid = r_grid;
   setup_checked = check_min_image;
   setup_had_inv = invs != nullptr;
 }

@@ -20,6 +20,7 @@ from ._lib import check, lib
 from .published import ATOMIC_MASS
 
 K_B = 1.380649e-23 * 6.241509074e18                                        # eV/K
+EV_ANG3_TO_GPA = 160.2176621
 EV_ANG_TO_AMU_ANG_FS2 = (1.60217663e-19 / 1.0e-10) / (1.66053906660e-27) * 1.0e-20
 
 
@@ -49,7 +50,8 @@ def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_step
            skin_width=0.3, masses=None, use_graph=True, thermostat=None, flux_every=0):
     """n_steps of velocity Verlet on the device (``ak_md_run``).  thermostat:
     None (NVE), ("nhc", temperature K, tau fs, chain links), ("langevin",
-    temperature, damping fs, seed) or ("csvr", temperature, damping, seed).  Returns a
+    temperature, damping fs, seed), ("csvr", temperature, damping, seed) or
+    ("npt", temperature, tau_nhc, chain links, pressure GPa, tau_baro fs).  Returns a
     dict with the final positions and velocities, e_pot, e_kin and e_cons
     (E_tot, or H_NHC with a chain) per step (n_steps + 1, index 0 the
     initial state), the number of list rebuilds, the loop's wall time and,
@@ -73,13 +75,20 @@ def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_step
         kind = thermostat[0]
         if kind == "nhc":
             _, T, tau, chain = thermostat
-            th = np.array([1.0, float(T), float(tau), float(chain), 0.0])
+            th = np.array([1.0, float(T), float(tau), float(chain), 0.0, 0.0, 0.0])
         elif kind in ("langevin", "csvr"):
             _, T, damping, seed = thermostat
             th = np.array([2.0 if kind == "langevin" else 3.0, float(T), float(damping), 0.0,
-                           float(int(seed))])
+                           float(int(seed)), 0.0, 0.0])
+        elif kind == "npt":
+            _, T, tau, chain, p_gpa, tau_b = thermostat
+            th = np.array([4.0, float(T), float(tau), float(chain), 0.0, float(p_gpa),
+                           float(tau_b)])
         else:
             raise ValueError(f"thermostat: unknown kind {kind!r} (nhc, langevin, csvr)")
+    lat_out = np.zeros((3, 3))
+    vol = np.zeros(n_steps + 1)
+    prs = np.zeros(n_steps + 1)
     nreb = ctypes.c_int32(0)
     secs = ctypes.c_double(0.0)
     P = _lib._pf64
@@ -89,13 +98,18 @@ def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_step
                           float(skin_width), 1 if use_graph else 0,
                           th.ctypes.data_as(P) if th is not None else None, every,
                           ep.ctypes.data_as(P), ek.ctypes.data_as(P), ec.ctypes.data_as(P),
-                          J.ctypes.data_as(P), ctypes.byref(nreb), ctypes.byref(secs)))
+                          J.ctypes.data_as(P), lat_out.ctypes.data_as(P), vol.ctypes.data_as(P),
+                          prs.ctypes.data_as(P), ctypes.byref(nreb), ctypes.byref(secs)))
     et = ep + ek
     out = {"positions": pos, "velocities": vel, "e_pot": ep, "e_kin": ek, "e_tot": et,
            "e_cons": ec, "drift": et - et[0], "n_rebuilds": int(nreb.value),
            "seconds": float(secs.value)}
     if every > 0:
         out["flux"] = J[:frames]
+    if th is not None and th[0] == 4.0:
+        out["lattice"] = lat_out
+        out["volume"] = vol
+        out["pressure_gpa"] = prs * EV_ANG3_TO_GPA
     return out
 
 
@@ -146,5 +160,22 @@ def csvr_run(potential, lattice, positions, velocities, atom_types, temperature,
     e_cons = E_tot - Q; random numbers as in :func:`langevin_run`."""
     out = md_run(potential, lattice, positions, velocities, atom_types, dt, n_steps,
                  skin_width, masses, use_graph, ("csvr", temperature, damping, seed), flux_every)
+    out["T"] = 2.0 * out["e_kin"] / (max(3 * len(positions) - 3, 1) * K_B)
+    return out
+
+
+def npt_run(potential, lattice, positions, velocities, atom_types, temperature, pressure=0.0,
+            dt=1.0, n_steps=1000, tau_nhc=100.0, nhc_chain=3, tau_baro=1000.0, skin_width=0.5,
+            masses=None, use_graph=True):
+    """NPT with the isotropic MTTK barostat (a 2-link chain on its momentum)
+    and the particle Nose-Hoover chain, on the device (NPTSimulation with
+    cell_dof "iso", baro_thermostat "nhc", solvers/molecular_dynamics.py:
+    2141-2555; defaults tau_nhc 100 fs, 3 links, tau_baro 1000 fs, skin
+    0.5 A).  pressure in GPa.  Returns md_run's dict with the final
+    "lattice", "volume" [A^3] and "pressure_gpa" (P_inst) per step, e_cons =
+    H_NPT and T.  The cell may shrink by up to 5 % linear in one call."""
+    out = md_run(potential, lattice, positions, velocities, atom_types, dt, n_steps,
+                 skin_width, masses, use_graph,
+                 ("npt", temperature, tau_nhc, nhc_chain, pressure, tau_baro), 0)
     out["T"] = 2.0 * out["e_kin"] / (max(3 * len(positions) - 3, 1) * K_B)
     return out

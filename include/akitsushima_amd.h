@@ -215,22 +215,28 @@ int ak_md_nve(ak_potential *pot, int32_t n_atoms, const double *lattice,
  * (solvers/molecular_dynamics.py:1113-1160: velocity Verlet between two
  * half-steps of an M-link Nose-Hoover chain, _nhc_step :635-727) and the
  * production run of ThermalConductivityCalculator.run_md
- * (solvers/thermal_conductivity.py:575-610), and LangevinSimulation (BAOAB,
- * :1511-1587) and CSVRSimulation (:1589-1687).  thermostat: NULL for NVE,
- * or {kind, temperature K, tau fs, chain links, seed}: kind 1 = Nose-Hoover
- * chain (tau_nhc, 2-16 links), 2 = Langevin, 3 = CSVR (tau = damping; seed
- * an integer < 2^53 keys the Philox4x32-10 stream); n_dof = 3N - 3.
- * e_cons (n_steps + 1): E_tot (NVE), H_NHC (:947-969) or E_tot - Q, Q the
- * heat the stochastic thermostat added (the E-Q column).
+ * (solvers/thermal_conductivity.py:575-610), LangevinSimulation (BAOAB,
+ * :1511-1587), CSVRSimulation (:1589-1687) and NPTSimulation with the
+ * isotropic barostat (:2141-2555, cell_dof "iso", baro_thermostat "nhc").
+ * thermostat: NULL for NVE, or {kind, temperature K, tau fs, chain links,
+ * seed, pressure GPa, tau_baro fs}: kind 1 = Nose-Hoover chain (tau_nhc,
+ * 2-16 links), 2 = Langevin, 3 = CSVR (tau = damping; seed an integer
+ * < 2^53 keys the Philox4x32-10 stream), 4 = NPT (the chain of kind 1 plus
+ * the barostat; the cell may shrink by up to 5 % linear); n_dof = 3N - 3.
+ * e_cons (n_steps + 1): E_tot (NVE), H_NHC (:947-969), E_tot - Q, Q the heat
+ * a stochastic thermostat added (the E-Q column), or H_NPT (:2425-2454).
+ * lattice_out (9): the final cell; volume, pressure (n_steps + 1): V [A^3]
+ * and P_inst [eV/A^3] per step under NPT.
  * flux_every > 0: J [eV/(A^2 fs)] of the state after every step s with
  * s % flux_every == 0 into flux ((n_steps + flux_every - 1) / flux_every, 3),
  * the exact site-term heat flux of ak_heat_flux.  Any output may be NULL. */
 int ak_md_run(ak_potential *pot, int32_t n_atoms, const double *lattice,
               const double *inv_lattice, double *positions, double *velocities,
               const double *masses, const int32_t *species, double dt, int32_t n_steps,
-              double skin_width, int32_t use_graph, const double *thermostat5,
+              double skin_width, int32_t use_graph, const double *thermostat7,
               int32_t flux_every, double *e_pot, double *e_kin, double *e_cons, double *flux,
-              int32_t *n_rebuilds, double *seconds);
+              double *lattice_out, double *volume, double *pressure, int32_t *n_rebuilds,
+              double *seconds);
 
 /* ---- Site terms and heat flux ------------------------------------------ */
 

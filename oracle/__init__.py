@@ -29,6 +29,7 @@ after the reference's Python glue:
 * ``tersoff_loss``: fitting/loss.py:51-248 (squared loss)
 * ``nve_run``, ``md_run``: NVESimulation.run, solvers/molecular_dynamics.py:1969-1996,
   and NVTSimulation.run with the Nose-Hoover chain (:635-727, 1113-1160)
+* ``npt_run``: NPTSimulation.run, isotropic barostat (:2141-2555)
 * ``site_terms``: engine/site_terms.py (site energies, per-bond gradients,
   heat flux)
 
@@ -916,6 +917,122 @@ def md_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, sk
             J.append(flux(pos, vel))
     return (np.array(epot), np.array(ekin), np.array(econs), pos, vel, rebuilds,
             np.array(J).reshape(-1, 3))
+
+
+def npt_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, skin, T, tau, M,
+            p_gpa, tau_baro):
+    """NPTSimulation.run with cell_dof "iso" and baro_thermostat "nhc"
+    (solvers/molecular_dynamics.py:2871-2895): particle chain and barostat
+    chain half-steps, barostat half-step at the current P_inst, half kick,
+    drift with the isotropic cell scaling and the (1 + 3/n_dof) velocity
+    factor (_scale_step :2517-2555), forces, half kick, barostat half-step at
+    the new P_inst, barostat chain, particle chain.  The list follows
+    check_and_rebuild with cell changes (engine/neighborlist.py:613-673).
+    ``force(lattice, pos, pairs)`` -> (E, F, sigma).  Returns E_pot, E_kin,
+    H_NPT (:2425-2454), V, P_inst [GPa] per step, final positions,
+    velocities and lattice, rebuilds."""
+    lattice = np.array(lattice, float)
+    pos = np.array(positions, float)
+    vel = np.array(velocities, float)
+    r_list = r_cut + skin
+    n_dof = max(3 * len(pos) - 3, 1)
+    kT = K_B * T
+    ek = lambda v: 0.5 * float(np.sum(masses[:, None] * v ** 2)) / EV_ANG_TO_AMU_ANG_FS2
+    state = ([n_dof * kT * tau ** 2] + [kT * tau ** 2] * (M - 1), [0.0] * M, [0.0] * M)
+    W = (n_dof + 3) * kT * tau_baro ** 2
+    Qb = kT * tau_baro ** 2
+    P_ext = p_gpa / 160.2176621
+    b = {"p": 0.0, "xi1": 0.0, "xi2": 0.0, "v1": 0.0, "v2": 0.0}
+    vol = lambda L: float(abs(np.linalg.det(L)))
+
+    def p_inst(v, S, L):
+        return 2.0 * ek(v) / (3.0 * vol(L)) + np.trace(S) / 3.0
+
+    def baro_nhc():
+        e1 = 0.5 * b["p"] ** 2 / W
+        G1 = (2.0 * e1 - kT) / Qb
+        G2 = (Qb * b["v1"] ** 2 - kT) / Qb
+        b["v2"] += G2 * (dt * 0.25)
+        sc = float(np.exp(np.clip(-b["v2"] * dt * 0.125, -350.0, 350.0)))
+        b["v1"] = sc * (sc * b["v1"] + G1 * dt * 0.25)
+        b["p"] *= float(np.exp(np.clip(-b["v1"] * dt * 0.5, -350.0, 350.0)))
+        b["xi1"] += b["v1"] * dt * 0.5
+        b["xi2"] += b["v2"] * dt * 0.5
+        e1 = 0.5 * b["p"] ** 2 / W
+        G1 = (2.0 * e1 - kT) / Qb
+        b["v1"] = sc * (sc * b["v1"] + G1 * dt * 0.25)
+        G2 = (Qb * b["v1"] ** 2 - kT) / Qb
+        b["v2"] += G2 * (dt * 0.25)
+
+    def baro_half(P, v, L):
+        b["p"] += 0.5 * dt * (3.0 * vol(L) * (P - P_ext) + (3.0 / n_dof) * 2.0 * ek(v))
+
+    def H(e, v, L):
+        Q, xi, w = state
+        t = 0.5 * Q[0] * w[0] ** 2
+        for j in range(1, M):
+            t = t + 0.5 * Q[j] * w[j] ** 2
+        u = n_dof * kT * xi[0]
+        for j in range(1, M):
+            u = u + kT * xi[j]
+        ed = b["p"] / W
+        hb = 0.5 * Qb * b["v1"] ** 2 + 0.5 * Qb * b["v2"] ** 2 + kT * b["xi1"] + kT * b["xi2"]
+        return ek(v) + e + P_ext * vol(L) + 0.5 * W * ed ** 2 + t + u + hb
+
+    ref = pos.copy()
+    lat_b = lattice.copy()
+    pairs = nl_build(lattice, pos, r_list)[:2]
+    e, f, S = force(lattice, pos, pairs)
+    acc = f * EV_ANG_TO_AMU_ANG_FS2 / masses[:, None]
+    rec = {"ep": [e], "ek": [ek(vel)], "h": [H(e, vel, lattice)], "v": [vol(lattice)],
+           "p": [p_inst(vel, S, lattice) * 160.2176621]}
+    rebuilds = 0
+    for _ in range(n_steps):
+        P = p_inst(vel, S, lattice)
+        vel = nhc_half(state, dt, kT, n_dof, vel, masses)
+        baro_nhc()
+        baro_half(P, vel, lattice)
+        vel = vel + 0.5 * dt * acc
+        ed = b["p"] / W
+        s = float(np.exp(ed * dt))
+        pos = pos + dt * vel
+        fr = pos @ np.linalg.inv(lattice)
+        fr -= np.floor(fr)
+        fr[fr >= 1.0] = 0.0
+        pos = fr @ lattice
+        fr = pos @ np.linalg.inv(lattice)
+        lattice = lattice * s
+        pos = fr @ lattice
+        vel = vel * float(np.exp(-(1.0 + 3.0 / n_dof) * ed * dt))
+        if np.allclose(lattice, lat_b, atol=1e-12):
+            Mm, sig = np.eye(3), 1.0
+        else:
+            Mm = np.linalg.inv(lat_b) @ lattice
+            sig = float(np.linalg.svd(Mm, compute_uv=False)[-1])
+        dr = pos - ref @ Mm
+        fr = dr @ np.linalg.inv(lattice)
+        fr -= np.round(fr)
+        dmax = float(np.max(np.linalg.norm(fr @ lattice, axis=1)))
+        if sig * r_list - 2.0 * dmax <= r_cut:
+            pairs = nl_build(lattice, pos, r_list)[:2]
+            ref = pos.copy()
+            lat_b = lattice.copy()
+            rebuilds += 1
+        e, f, S = force(lattice, pos, pairs)
+        acc = f * EV_ANG_TO_AMU_ANG_FS2 / masses[:, None]
+        vel = vel + 0.5 * dt * acc
+        Pn = p_inst(vel, S, lattice)
+        baro_half(Pn, vel, lattice)
+        baro_nhc()
+        vel = nhc_half(state, dt, kT, n_dof, vel, masses)
+        rec["ep"].append(e)
+        rec["ek"].append(ek(vel))
+        rec["h"].append(H(e, vel, lattice))
+        rec["v"].append(vol(lattice))
+        rec["p"].append(Pn * 160.2176621)
+    out = {k: np.array(v) for k, v in rec.items()}
+    out.update(positions=pos, velocities=vel, lattice=lattice, rebuilds=rebuilds)
+    return out
 
 
 def nve_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, skin):

@@ -252,3 +252,70 @@ def test_green_kubo_run_uses_the_device_flux():
     assert rel_err(J[-1], J1) < 1e-10
     kappa = tc.compute_kappa(tau_max=0.1, n_blocks=2)
     assert kappa.shape == (3, 3) and np.all(np.isfinite(kappa))
+
+
+# --------------------------------------------------------------------------
+#  NPT (isotropic barostat) on the device
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("name", ["sw", "tersoff_xy", "eam_cuni"])
+def test_npt_matches_the_oracle(name):
+    from akitsushima_amd.md import npt_run
+
+    make, _, cell = CASES[name]
+    pot = make()
+    lat, pos, t = cell()
+    lat, pos = lat * 0.985, pos * 0.985          # compressed: the cell moves
+    m = np.full(len(pos), 28.086) if name != "eam_cuni" else np.array(
+        [63.546 if x == "Cu" else 58.693 for x in t])
+    vel = maxwell_boltzmann(500.0, m, 4)
+    n_steps, skin = 40, 0.5
+    res = npt_run(pot, lat, pos, vel, t, 500.0, pressure=0.5, dt=1.0, n_steps=n_steps,
+                  tau_nhc=40.0, nhc_chain=3, tau_baro=150.0, skin_width=skin, masses=m)
+    if name == "sw":
+        force = lambda l, p, pr: O.sw_flat(l, p, SW_SI, pairs=pr)
+    elif name == "tersoff_xy":
+        force = lambda l, p, pr: O.tersoff_multi(l, p, t, TERSOFF_XY, ["X", "Y"], pairs=pr)
+    else:
+        force = lambda l, p, pr: O.eam_zhou(l, p, t, CUNI, ["Cu", "Ni"], pairs=pr)
+    r = O.npt_run(force, lat, pos, vel, m, 1.0, n_steps, pot.r_cut, skin, 500.0, 40.0, 3, 0.5,
+                  150.0)
+    for a, b in (("e_pot", "ep"), ("e_kin", "ek"), ("e_cons", "h"), ("volume", "v"),
+                 ("pressure_gpa", "p")):
+        assert np.abs(res[a] - r[b]).max() < 1e-9 * np.abs(r[b]).max(), a
+    assert res["n_rebuilds"] == r["rebuilds"]
+    assert np.abs(res["lattice"] - r["lattice"]).max() < 1e-10 * np.abs(r["lattice"]).max()
+    d = res["positions"] - r["positions"]
+    d -= np.round(d @ np.linalg.inv(res["lattice"])) @ res["lattice"]
+    assert np.abs(d).max() < 1e-8
+    assert np.abs(res["velocities"] - r["velocities"]).max() < 1e-9 * np.abs(r["velocities"]).max()
+    assert abs(res["volume"][-1] - res["volume"][0]) > 1e-4 * res["volume"][0]
+
+
+def test_npt_with_a_frozen_barostat_is_the_device_nvt():
+    from akitsushima_amd.md import npt_run
+
+    lat, pos, t = cubic(DIAMOND, 5.431, 3, ["Si"] * 8, rattle=0.03, seed=1)
+    pot = StillingerWeberPotential(SW_SI, species=["Si"])
+    m = np.full(len(pos), 28.086)
+    vel = maxwell_boltzmann(600.0, m, 2)
+    a = npt_run(pot, lat, pos, vel, t, 600.0, dt=1.0, n_steps=300, tau_nhc=50.0, tau_baro=1e12)
+    b = nvt_run(pot, lat, pos, vel, t, 600.0, dt=1.0, n_steps=300, tau_nhc=50.0)
+    assert np.abs(a["e_pot"] - b["e_pot"]).max() < 1e-9 * np.abs(b["e_pot"]).max()
+    assert np.abs(a["positions"] - b["positions"]).max() < 1e-8
+    assert np.array_equal(a["lattice"], lat)
+
+
+def test_npt_relaxes_to_the_target_pressure():
+    from akitsushima_amd.md import npt_run
+
+    lat, pos, t = cubic(DIAMOND, 5.40, 4, ["Si"] * 8, rattle=0.02, seed=1)
+    pot = StillingerWeberPotential(SW_SI, species=["Si"])
+    m = np.full(len(pos), 28.086)
+    vel = maxwell_boltzmann(600.0, m, 2)
+    r = npt_run(pot, lat, pos, vel, t, 600.0, pressure=0.0, dt=1.0, n_steps=4000, tau_nhc=50.0,
+                tau_baro=300.0)
+    et, h = r["e_tot"], r["e_cons"]
+    assert h.max() - h.min() < 3e-3 * (et.max() - et.min())
+    assert r["pressure_gpa"][0] > 2.0 and r["volume"][-1] > 1.02 * r["volume"][0]
+    assert abs(r["pressure_gpa"][2000:].mean()) < 0.2

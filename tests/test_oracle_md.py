@@ -134,3 +134,44 @@ def test_stochastic_thermostats_conserve_e_minus_q_and_sample_600k(kind):
     if kind == "csvr":   # canonical kinetic-energy variance (n_f / 2) (kT)^2
         r = ek[1000:].var() / (0.5 * nf * (O.K_B * 600.0) ** 2)
         assert 0.6 < r < 1.6
+
+
+# --------------------------------------------------------------------------
+#  NPT (isotropic barostat)
+# --------------------------------------------------------------------------
+
+def _npt_cell():
+    from akitsushima_amd.structures import DIAMOND, cubic
+    lat, pos, t = cubic(DIAMOND, 5.40, 2, ["Si"] * 8, rattle=0.03, seed=1)
+    m = np.full(len(pos), 28.086)
+    return lat, pos, m, O.mb_velocities(600.0, m, 3)
+
+
+def _sw(l, p, pr):
+    return O.sw_flat(l, p, SW_SI, pairs=pr)
+
+
+def test_npt_with_a_frozen_barostat_is_nvt():
+    """tau_baro -> infinity: W -> infinity, the cell stays, and the step is
+    the Nose-Hoover NVT step operation for operation."""
+    lat, pos, m, vel = _npt_cell()
+    a = O.npt_run(_sw, lat, pos, vel, m, 1.0, 200, 3.77118, 0.5, 600.0, 50.0, 3, 0.0, 1e12)
+    ep, ek, ec, p1, v1, rb, _ = O.md_run(_sw, lat, pos, vel, m, 1.0, 200, 3.77118, 0.5,
+                                         nhc=(600.0, 50.0, 3))
+    assert np.array_equal(a["ep"], ep) and np.array_equal(a["positions"], p1)
+    assert a["rebuilds"] == rb
+
+
+def test_npt_conserves_h_and_relaxes_to_the_target_pressure():
+    lat, pos, m, vel = _npt_cell()
+    ranges = []
+    for dt in (1.0, 0.5):
+        r = O.npt_run(_sw, lat, pos, vel, m, dt, int(1500 / dt), 3.77118, 0.5, 600.0, 50.0, 3,
+                      0.0, 200.0)
+        et = r["ep"] + r["ek"]
+        ranges.append(r["h"].max() - r["h"].min())
+        assert ranges[-1] < 2e-3 * (et.max() - et.min())
+    assert 2.5 < ranges[0] / ranges[1] < 5.0
+    # compressed start (P_inst 2.3 GPa): the cell expands and P averages to 0
+    assert r["p"][0] > 2.0 and r["v"][-1] > 1.02 * r["v"][0]
+    assert abs(r["p"][len(r["p"]) // 2:].mean()) < 0.2
