@@ -227,10 +227,12 @@ def sw_md(n_steps=1000):
 def sw_gk(n_steps=1000):
     """The device MD variants on the SW Si 20^3 cell (64000 atoms, 1000 K
     start, dt 1 fs): NVE with the heat flux recorded every step (a Green-Kubo
-    production run) and NVT with a 3-link Nose-Hoover chain (900 K, tau 100
-    fs); ms per hipGraph step and the conserved quantity's range."""
+    production run), NVT with a 3-link Nose-Hoover chain (900 K, tau 100
+    fs), Langevin and CSVR (damping 100 fs) and isotropic NPT (0 GPa, tau_baro
+    1000 fs); ms per hipGraph step and the conserved quantities' ranges."""
     from akitsushima_amd import StillingerWeberPotential
-    from akitsushima_amd.md import masses_of, maxwell_boltzmann, md_run, nvt_run
+    from akitsushima_amd.md import (csvr_run, langevin_run, masses_of, maxwell_boltzmann, md_run,
+                                    npt_run, nvt_run)
     from akitsushima_amd.published import SW_SI
     from akitsushima_amd.structures import DIAMOND, cubic
 
@@ -244,6 +246,12 @@ def sw_gk(n_steps=1000):
                flux_every=1)
     b = nvt_run(pot, lat, pos, vel, types, 900.0, dt=1.0, n_steps=n_steps, tau_nhc=100.0,
                 nhc_chain=3, skin_width=0.3)
+    c = langevin_run(pot, lat, pos, vel, types, 900.0, dt=1.0, n_steps=n_steps, damping=100.0,
+                     skin_width=0.3)
+    d = csvr_run(pot, lat, pos, vel, types, 900.0, dt=1.0, n_steps=n_steps, damping=100.0,
+                 skin_width=0.3)
+    e = npt_run(pot, lat, pos, vel, types, 900.0, pressure=0.0, dt=1.0, n_steps=n_steps,
+                skin_width=0.3)
     return {"workload": f"SW Si diamond 20^3 ({n} atoms), dt 1 fs, {n_steps} steps, "
                         "skin 0.3 A, hipGraph-captured steps",
             "nve_flux_every_step_ms_per_step": a["seconds"] / n_steps * 1e3,
@@ -251,7 +259,12 @@ def sw_gk(n_steps=1000):
             "nve_flux_drift_per_atom_max_abs_eV": float(np.abs(a["drift"]).max() / n),
             "nvt_nhc_ms_per_step": b["seconds"] / n_steps * 1e3,
             "nvt_h_nhc_range_per_atom_eV": float((b["e_cons"].max() - b["e_cons"].min()) / n),
-            "nvt_T_final_K": float(b["T"][-1])}
+            "nvt_T_final_K": float(b["T"][-1]),
+            "langevin_ms_per_step": c["seconds"] / n_steps * 1e3,
+            "csvr_ms_per_step": d["seconds"] / n_steps * 1e3,
+            "npt_ms_per_step": e["seconds"] / n_steps * 1e3,
+            "npt_h_range_per_atom_eV": float((e["e_cons"].max() - e["e_cons"].min()) / n),
+            "npt_volume_change": float(e["volume"][-1] / e["volume"][0] - 1.0)}
 
 
 def eam_cuni(steps, warmup):
@@ -439,6 +452,9 @@ def main():
                          "(bond, partner), 0 automatic")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the SW MD, Zhou EAM and batched Tersoff lines")
+    ap.add_argument("--leg", default="",
+                    help="run only this extra leg (sw_md, eam_cuni, batched_tersoff, sw_elastic, "
+                         "sw_gesn, sw_gk) and print its JSON (for profiling)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="with --profile-only: also run a MiB-sized elementwise copy "
                          "kernel (counter calibration)")
@@ -449,6 +465,19 @@ def main():
         return 2
 
     import torch
+
+    if args.leg:
+        legs = {"sw_md": lambda: sw_md(1000), "eam_cuni": lambda: eam_cuni(20, 3),
+                "batched_tersoff": lambda: batched_tersoff(10), "sw_elastic": sw_elastic,
+                "sw_gesn": lambda: sw_gesn(50, 5, cpu=False), "sw_gk": lambda: sw_gk(1000)}
+        if args.leg not in legs:
+            print(f"bench.py: unknown leg {args.leg!r}", file=sys.stderr)
+            return 2
+        if not torch.cuda.is_available():
+            print("bench.py: no GPU visible", file=sys.stderr)
+            return 1
+        print(json.dumps({args.leg: legs[args.leg]()}))
+        return 0
 
     from akitsushima_amd import NeighbourList, TersoffPotential
     from akitsushima_amd import _lib
