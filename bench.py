@@ -327,6 +327,20 @@ def sw_gesn(steps, warmup, cpu=True):
                 oracle.sw_multi(lat, pos, types, recs, sp, nthreads=16)
             r["cpu_oracle_ms_per_call_16_threads"] = (time.perf_counter() - t0) / 5 * 1e3
             r["reference_ledger_call_ms_1_thread"] = 33.9447  # benchmarks/adr0047.jsonl:8
+        if cells == 20:
+            # NVT MD of the same cell on the device (the reference's ledger of
+            # GeSn SW NVT steps, docs/adr/0025-gesn-large-scale-relax-md-test.md:
+            # 209-213, is 6990 ms per step for 108864 atoms on one core)
+            from akitsushima_amd.md import maxwell_boltzmann, nvt_run
+
+            m = np.array([72.63 if x == "Ge" else 118.71 for x in types])
+            vel = maxwell_boltzmann(300.0, m, 0)
+            nvt_run(pot, lat, pos, vel, types, 300.0, dt=1.0, n_steps=10, masses=m)
+            md = nvt_run(pot, lat, pos, vel, types, 300.0, dt=1.0, n_steps=500, masses=m)
+            r["nvt_ms_per_step"] = md["seconds"] / 500 * 1e3
+            r["nvt_atom_steps_per_s"] = len(pos) * 500 / md["seconds"]
+            r["nvt_h_range_per_atom_eV"] = float((md["e_cons"].max() - md["e_cons"].min())
+                                                 / len(pos))
         out[f"cells_{cells}"] = r
     return out
 

@@ -3,7 +3,9 @@ relative in E, F, sigma) and against size-independent properties: zero net
 force, symmetric stress, and, for the batch, equality with single calls.
 
 Sizes: Tersoff Si 16^3 diamond cells (32768 atoms); SW Si 20^3 (64000);
-Zhou Cu-Ni fcc 40^3 (256000); 512 Tersoff structures of 64-2048 atoms."""
+Zhou Cu-Ni fcc 40^3 (256000); 512 Tersoff structures of 64-2048 atoms; the
+benchmark's multi-species SW Ge-Sn 20^3 cell (64000), setfl Cu-Ni at the
+EAM size, and the Green-Kubo flux of the 64000-atom SW cell."""
 import numpy as np
 import pytest
 
@@ -68,3 +70,45 @@ def test_batched_512_structures():
         _check(E[k], F[k], S[k], e0, f0, s0, f"batch {k}")
         e1, f1, s1 = single.compute(lat, p, t)
         assert abs(E[k] - e1) <= 1e-12 * abs(e1) and rel_err(F[k], f1) < 1e-12
+
+
+def test_sw_gesn_64000_atoms():
+    from _cases import GESN_SPECIES, SW_GESN
+
+    lat, pos, _ = cubic(DIAMOND, 5.762, 20, ["Ge"] * 8)
+    rng = np.random.default_rng(1)
+    t = [str(x) for x in rng.choice(GESN_SPECIES, len(pos), p=[0.875, 0.125])]
+    pot = StillingerWeberPotential(SW_GESN, species=GESN_SPECIES)
+    e, f, s = pot.compute(lat, pos, t)
+    e0, f0, s0 = O.sw_multi(lat, pos, t, SW_GESN, GESN_SPECIES, nthreads=8)
+    _check(e, f, s, e0, f0, s0, "gesn 64000")
+
+
+def test_setfl_cuni_256000_atoms(tmp_path):
+    from akitsushima_amd import read_setfl
+    from akitsushima_amd.published import ATOMIC_MASS
+
+    el = {"Cu": dict(ZHOU_2004["Cu"]), "Ni": dict(ZHOU_2004["Ni"])}
+    path = str(tmp_path / "CuNi.eam.alloy")
+    O.write_setfl_zhou(path, el, ["Cu", "Ni"], ATOMIC_MASS)
+    lat, pos, _ = cubic(FCC, 3.56, 40, ["X"] * 4, rattle=0.03, seed=1)
+    types = random_species(np.random.default_rng(0), ["Cu", "Ni"], len(pos))
+    pot = read_setfl(path)
+    e, f, s = pot.compute(lat, pos, types)
+    tab, names = O.read_setfl(path)
+    e0, f0, s0 = O.eam_table(lat, pos, types, tab, names, nthreads=8)
+    _check(e, f, s, e0, f0, s0, "setfl 256000")
+
+
+def test_heat_flux_64000_atoms():
+    import oracle.site_terms as ST
+    from akitsushima_amd.md import maxwell_boltzmann
+
+    lat, pos, t = cubic(DIAMOND, 5.431, 20, ["Si"] * 8, rattle=0.05, seed=2)
+    pot = StillingerWeberPotential(dict(SW_SI), species=["Si"])
+    m = np.full(len(pos), 28.086)
+    vel = maxwell_boltzmann(700.0, m, 5)
+    J, e = pot.heat_flux(lat, pos, vel, t, m)
+    ref = ST.sw(lat, pos, t, O.sw_flat_as_records(dict(SW_SI)), ["Si"])
+    J0 = ref.heat_flux(vel, m, abs(np.linalg.det(lat)))
+    assert rel_err(J, J0) < 1e-10 and rel_err(e, ref.e_site) < 1e-10
