@@ -810,7 +810,7 @@ def langevin_noise(n, step, k0, k1):
 
 
 def md_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, skin,
-           nhc=None, flux=None, every=0, langevin=None, csvr=None):
+           nhc=None, flux=None, every=0, langevin=None, csvr=None, state_out=None):
     """Velocity Verlet with PBC wrap and the Verlet-skin rebuild test of
     engine/neighborlist.py:613-673 (fixed cell: rebuild when r_list - 2 d_max
     <= r_cut); ``force(lattice, pos, (i, j))`` returns (E, F, S).  nhc =
@@ -822,7 +822,8 @@ def md_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, sk
     the device.  flux(pos, vel) -> J of the state after steps s with s %
     every == 0.  Returns E_pot, E_kin, conserved quantity (E_tot, H_NHC
     (:947-969) or E_tot - Q), final positions and velocities, rebuilds, J
-    frames."""
+    frames; a dict passed as state_out receives the final chain positions
+    "xi" and velocities "vxi"."""
     lattice = np.asarray(lattice, float)
     pos = np.array(positions, float)
     vel = np.array(velocities, float)
@@ -915,6 +916,8 @@ def md_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, sk
         econs.append(cons(e, ek(vel)))
         if flux is not None and every > 0 and step % every == 0:
             J.append(flux(pos, vel))
+    if state_out is not None and state is not None:
+        state_out.update(xi=list(state[1]), vxi=list(state[2]))
     return (np.array(epot), np.array(ekin), np.array(econs), pos, vel, rebuilds,
             np.array(J).reshape(-1, 3))
 
@@ -930,7 +933,8 @@ def npt_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, s
     check_and_rebuild with cell changes (engine/neighborlist.py:613-673).
     ``force(lattice, pos, pairs)`` -> (E, F, sigma).  Returns E_pot, E_kin,
     H_NPT (:2425-2454), V, P_inst [GPa] per step, final positions,
-    velocities and lattice, rebuilds."""
+    velocities and lattice, rebuilds, and the final particle chain ("xi",
+    "vxi") and barostat state ("p_eps", "xi_b", "vxi_b")."""
     lattice = np.array(lattice, float)
     pos = np.array(positions, float)
     vel = np.array(velocities, float)
@@ -1031,7 +1035,9 @@ def npt_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, s
         rec["v"].append(vol(lattice))
         rec["p"].append(Pn * 160.2176621)
     out = {k: np.array(v) for k, v in rec.items()}
-    out.update(positions=pos, velocities=vel, lattice=lattice, rebuilds=rebuilds)
+    out.update(positions=pos, velocities=vel, lattice=lattice, rebuilds=rebuilds,
+               xi=list(state[1]), vxi=list(state[2]), p_eps=b["p"], xi_b=[b["xi1"], b["xi2"]],
+               vxi_b=[b["v1"], b["v2"]])
     return out
 
 

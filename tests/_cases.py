@@ -6,7 +6,7 @@ import os
 
 import numpy as np
 
-from akitsushima_amd.structures import DIAMOND, FCC, cubic
+from akitsushima_amd.structures import DIAMOND, FCC, cubic, supercell
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -98,6 +98,17 @@ def sheared_si(n=2, shear=0.07, rattle=0.1, seed=3):
     lat2 = lat @ T
     pos2 = frac @ lat2
     return lat2, rattled(lat2, pos2, rattle, seed), t
+
+
+def si64_recorded_md():
+    """The 64-atom Si of the reference's recorded thermostat runs
+    (_perturbed_si_supercell(1.0, n=2, sigma=0.05, seed=1),
+    tests/test_solvers.py:223-241): 2x2x2 conventional diamond at
+    a = 5.430950 A, displaced by N(0, 0.05) draws of default_rng(1) with
+    their mean removed."""
+    lat, pos, t = supercell(DIAMOND, 5.430950, (2, 2, 2), ["Si"] * 8)
+    d = np.random.default_rng(1).normal(0.0, 0.05, pos.shape)
+    return lat, pos + (d - d.mean(axis=0)), t
 
 
 def rel_err(a, b):
