@@ -871,8 +871,8 @@ extern "C" {
 int ak_md_run(ak_potential* P, int32_t n, const double* lattice, const double* inv,
               double* positions, double* velocities, const double* masses,
               const int32_t* species, double dt, int32_t n_steps, double skin,
-              int32_t use_graph, const double* thermostat, int32_t flux_every, double* e_pot,
-              double* e_kin, double* e_cons, double* flux, double* lattice_out,
+              int32_t use_graph, const double* thermostat, double* state, int32_t flux_every,
+              double* e_pot, double* e_kin, double* e_cons, double* flux, double* lattice_out,
               double* volume, double* pressure, int32_t* n_rebuilds, double* seconds) {
   return guarded([&] {
     AK_REQUIRE(P && lattice && positions && velocities && masses && n > 0, AK_ERR_ARG,
@@ -904,13 +904,18 @@ int ak_md_run(ak_potential* P, int32_t n, const double* lattice, const double* i
         opt.pressure_gpa = thermostat[5];
         opt.tau_baro = thermostat[6];
         AK_REQUIRE(std::isfinite(opt.pressure_gpa), AK_ERR_ARG, "bad NPT pressure");
+        opt.cell_dof = (int)thermostat[7];
+        opt.baro_thermostat = (int)thermostat[8];
+        AK_REQUIRE(thermostat[7] == (double)opt.cell_dof &&
+                       thermostat[8] == (double)opt.baro_thermostat,
+                   AK_ERR_ARG, "bad NPT cell_dof or baro_thermostat code");
       }
     }
     opt.flux_every = flux_every;
     int nr = 0;
     md_run(P->pot, n, lattice, inv, positions, velocities, masses, species, dt, n_steps, skin,
-           use_graph != 0, opt, e_pot, e_kin, e_cons, flux, lattice_out, volume, pressure, &nr,
-           seconds, P->stream);
+           use_graph != 0, opt, state, e_pot, e_kin, e_cons, flux, lattice_out, volume,
+           pressure, &nr, seconds, P->stream);
     if (n_rebuilds) *n_rebuilds = nr;
   });
 }
@@ -935,8 +940,8 @@ int ak_md_nve(ak_potential* P, int32_t n, const double* lattice, const double* i
     }
     int nr = 0;
     md_run(P->pot, n, lattice, inv, positions, velocities, masses, species, dt, n_steps, skin,
-           use_graph != 0, MDOptions{}, e_pot, e_kin, nullptr, nullptr, nullptr, nullptr,
-           nullptr, &nr, seconds, P->stream);
+           use_graph != 0, MDOptions{}, nullptr, e_pot, e_kin, nullptr, nullptr, nullptr,
+           nullptr, nullptr, &nr, seconds, P->stream);
     if (n_rebuilds) *n_rebuilds = nr;
   });
 }

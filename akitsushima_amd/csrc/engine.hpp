@@ -227,8 +227,18 @@ struct MDOptions {
   int chain = 3;
   uint64_t seed = 0;  // random stream of the stochastic thermostats
   double pressure_gpa = 0.0, tau_baro = 0.0;  // NPT target pressure and barostat time
+  int cell_dof = 0;         // NPT cell degrees of freedom: 0 iso
+  int baro_thermostat = 0;  // NPT barostat thermostat: 0 Nose-Hoover chain
   int flux_every = 0;
 };
+
+// Thermostat / barostat state carried between runs (in and out of md_run):
+// the particle chain xi[16], vxi[16]; the isotropic barostat momentum p_eps,
+// its chain xi_b[2], vxi_b[2]; six strain momenta (reserved); the heat the
+// stochastic thermostat put in; the count of random draws already used.
+constexpr int kMdStateLen = 48;
+constexpr int kStXi = 0, kStVxi = 16, kStPeps = 32, kStXiB = 33, kStVxiB = 35, kStPg = 37,
+              kStHeat = 43, kStRng = 44;
 
 // eV/A^3 -> GPa (constants.py EV_ANG3_TO_GPA)
 constexpr double kEvAng3ToGpa = 160.2176621;
@@ -236,10 +246,12 @@ constexpr double kEvAng3ToGpa = 160.2176621;
 // md.hip: n_steps of velocity Verlet on one cell, positions and velocities
 // updated in place; e_pot, e_kin, e_cons (n_steps + 1) and flux
 // ((n_steps + flux_every - 1) / flux_every, 3) as recorded on the device.
+// state (kMdStateLen, may be null = start from rest) is read at the start
+// and holds the final thermostat state on return.
 void md_run(Potential& pot, int n, const double* lattice, const double* inv, double* pos,
             double* vel, const double* mass, const int32_t* spc, double dt, int n_steps,
-            double skin, bool use_graph, const MDOptions& opt, double* e_pot, double* e_kin,
-            double* e_cons, double* flux, double* lattice_out, double* volume,
+            double skin, bool use_graph, const MDOptions& opt, double* state, double* e_pot,
+            double* e_kin, double* e_cons, double* flux, double* lattice_out, double* volume,
             double* pressure, int* n_rebuilds, double* seconds, hipStream_t st);
 
 // Heat flux J = [sum_a E_a v_a - sum_(a,k) v_ak (G[a,k] . v_nbr)] / V of one

@@ -823,13 +823,19 @@ static void md_record(MDState& S, int n, hipStream_t st) {
 
 void md_run(Potential& pot, int n, const double* lattice, const double* inv, double* pos,
             double* vel, const double* mass, const int32_t* spc, double dt, int n_steps,
-            double skin, bool use_graph, const MDOptions& opt, double* e_pot, double* e_kin,
-            double* e_cons, double* flux, double* lattice_out, double* volume,
+            double skin, bool use_graph, const MDOptions& opt, double* state, double* e_pot,
+            double* e_kin, double* e_cons, double* flux, double* lattice_out, double* volume,
             double* pressure, int* n_rebuilds, double* seconds, hipStream_t st) {
   AK_REQUIRE(n > 0 && n_steps >= 0 && dt > 0.0, AK_ERR_ARG, "bad MD arguments");
   AK_REQUIRE(opt.thermostat >= 0 && opt.thermostat <= 4, AK_ERR_ARG,
              "thermostat must be 0 (NVE), 1 (Nose-Hoover chain), 2 (Langevin), 3 (CSVR) or "
              "4 (NPT)");
+  AK_REQUIRE(opt.cell_dof == 0 && opt.baro_thermostat == 0, AK_ERR_ARG,
+             "NPT: only cell_dof iso with the Nose-Hoover barostat chain is available");
+  if (state)
+    for (int k = 0; k < kMdStateLen; ++k)
+      AK_REQUIRE(std::isfinite(state[k]), AK_ERR_ARG, "thermostat state must be finite");
+  const auto st_at = [&](int k) { return state ? state[k] : 0.0; };
   AK_REQUIRE(opt.flux_every >= 0, AK_ERR_ARG, "flux_every must be >= 0");
   AK_REQUIRE(opt.thermostat != 4 || opt.flux_every == 0, AK_ERR_ARG,
              "the heat flux is recorded at fixed volume (not under NPT)");
@@ -880,6 +886,10 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
     h[0] = h[1] = 1.0;
     h[2] = S.ndof * kBoltzmannEv * opt.temperature * opt.tau * opt.tau;
     for (int k = 1; k < S.M; ++k) h[2 + k] = kBoltzmannEv * opt.temperature * opt.tau * opt.tau;
+    for (int k = 0; k < S.M; ++k) {
+      h[2 + S.M + k] = st_at(kStXi + k);
+      h[2 + 2 * S.M + k] = st_at(kStVxi + k);
+    }
     S.nhc.ensure(sizeof(double) * h.size());
     AK_HIP(hipMemcpyAsync(S.nhc.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
                           st));
@@ -901,6 +911,11 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
       S.npt.s_min = s_min;
       std::vector<double> q(16, 0.0);
       q[0] = q[1] = q[7] = q[8] = 1.0;
+      q[2] = st_at(kStPeps);
+      q[3] = st_at(kStXiB);
+      q[4] = st_at(kStXiB + 1);
+      q[5] = st_at(kStVxiB);
+      q[6] = st_at(kStVxiB + 1);
       S.np.ensure(sizeof(double) * q.size());
       AK_HIP(hipMemcpyAsync(S.np.p, q.data(), sizeof(double) * q.size(), hipMemcpyHostToDevice,
                             st));
@@ -924,6 +939,7 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
     S.inv_mass = 1.0 / M;
     std::vector<double> h(8, 0.0);
     h[0] = h[1] = 1.0;
+    h[2] = st_at(kStHeat);
     S.nhc.ensure(sizeof(double) * h.size());
     AK_HIP(hipMemcpyAsync(S.nhc.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
                           st));
@@ -967,8 +983,12 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
   md_step(pot, nl, set, S, n, 0.0, st);
   AK_HIP(hipStreamSynchronize(st));
   // restart the record, the rebuild count and the flux gate: step k records
-  // state k - 1 (NVE) or k (chain; state 0 recorded here)
-  int reset[4] = {0, 0, 0, 0};
+  // state k - 1 (NVE) or k (chain; state 0 recorded here); the random
+  // stream continues after the draws a previous run used
+  const double rng0 = st_at(kStRng);
+  AK_REQUIRE(rng0 >= 0.0 && rng0 == std::floor(rng0) && rng0 + 2.0 * n_steps + 4.0 < 2147483647.0,
+             AK_ERR_ARG, "thermostat state: bad random-draw count");
+  int reset[4] = {0, 0, 0, (int)rng0};
   AK_HIP(hipMemcpy(S.counters.as<int>(), reset, sizeof(reset), hipMemcpyHostToDevice));
   if (S.thermo) md_record(S, n, st);
 
@@ -1006,9 +1026,29 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
   AK_HIP(hipMemcpy(counters, S.counters.p, sizeof(counters), hipMemcpyDeviceToHost));
   AK_REQUIRE(counters[0] == n_steps + 1, AK_ERR_STATE, "MD step counter mismatch");
   AK_REQUIRE(S.every == 0 || counters[2] == n_steps, AK_ERR_STATE, "heat-flux gate mismatch");
+  if (state && (S.thermo == 1 || S.thermo == 4)) {
+    std::vector<double> h(2 + 3 * S.M);
+    AK_HIP(hipMemcpy(h.data(), S.nhc.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < S.M; ++k) {
+      state[kStXi + k] = h[2 + S.M + k];
+      state[kStVxi + k] = h[2 + 2 * S.M + k];
+    }
+  } else if (state && (S.thermo == 2 || S.thermo == 3)) {
+    double h[3];
+    AK_HIP(hipMemcpy(h, S.nhc.p, sizeof(h), hipMemcpyDeviceToHost));
+    state[kStHeat] = h[2];
+    state[kStRng] = (double)counters[3];
+  }
   if (S.thermo == 4) {
     double q[16];
     AK_HIP(hipMemcpy(q, S.np.p, sizeof(q), hipMemcpyDeviceToHost));
+    if (state) {
+      state[kStPeps] = q[2];
+      state[kStXiB] = q[3];
+      state[kStXiB + 1] = q[4];
+      state[kStVxiB] = q[5];
+      state[kStVxiB + 1] = q[6];
+    }
     AK_REQUIRE(q[9] == 0.0, AK_ERR_CAPACITY,
                "the NPT cell shrank beyond the range its neighbour grid and minimum image "
                "were set up for (5 % linear); restart from the current cell");

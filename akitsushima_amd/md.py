@@ -23,6 +23,42 @@ K_B = 1.380649e-23 * 6.241509074e18                                        # eV/
 EV_ANG3_TO_GPA = 160.2176621
 EV_ANG_TO_AMU_ANG_FS2 = (1.60217663e-19 / 1.0e-10) / (1.66053906660e-27) * 1.0e-20
 
+# thermostat state carried between device runs (AK_MD_STATE_LEN doubles)
+STATE_LEN = 48
+_ST = {"xi": (0, 16), "vxi": (16, 32), "p_eps": (32, 33), "xi_b": (33, 35), "vxi_b": (35, 37),
+       "heat": (43, 44), "rng_draws": (44, 45)}
+
+
+def thermostat_state(xi=(), vxi=(), p_eps=0.0, xi_b=(0.0, 0.0), vxi_b=(0.0, 0.0), heat=0.0,
+                     rng_draws=0):
+    """A state array for ``md_run(state=...)`` from the quantities a
+    reference checkpoint carries (NVTSimulation xi_nhc / vxi_nhc,
+    NPTSimulation p_eps, xi1_b, xi2_b, vxi1_b, vxi2_b;
+    solvers/molecular_dynamics.py:823-830, 2228): chain positions and
+    velocities (up to 16 links), the barostat momentum and its chain, the
+    heat a stochastic thermostat added and the random draws used."""
+    st = np.zeros(STATE_LEN)
+    for key, val in (("xi", xi), ("vxi", vxi), ("p_eps", p_eps), ("xi_b", xi_b),
+                     ("vxi_b", vxi_b), ("heat", heat), ("rng_draws", rng_draws)):
+        a, b = _ST[key]
+        v = np.atleast_1d(np.asarray(val, float))
+        if len(v) > b - a:
+            raise ValueError(f"thermostat_state: {key} has more than {b - a} entries")
+        st[a:a + len(v)] = v
+    return st
+
+
+def state_fields(state, chain=None):
+    """The named quantities of a state array (chain entries cut to
+    ``chain`` links when given)."""
+    st = np.asarray(state, float)
+    out = {k: st[a:b].copy() for k, (a, b) in _ST.items()}
+    if chain is not None:
+        out["xi"], out["vxi"] = out["xi"][:chain], out["vxi"][:chain]
+    out["p_eps"], out["heat"] = float(out["p_eps"][0]), float(out["heat"][0])
+    out["rng_draws"] = int(out["rng_draws"][0])
+    return out
+
 
 def maxwell_boltzmann(T, masses, seed):
     """Velocities [A/fs] at temperature T as _init_velocities_mb
@@ -47,7 +83,8 @@ def masses_of(atom_types):
 
 
 def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_steps=1000,
-           skin_width=0.3, masses=None, use_graph=True, thermostat=None, flux_every=0):
+           skin_width=0.3, masses=None, use_graph=True, thermostat=None, flux_every=0,
+           state=None):
     """n_steps of velocity Verlet on the device (``ak_md_run``).  thermostat:
     None (NVE), ("nhc", temperature K, tau fs, chain links), ("langevin",
     temperature, damping fs, seed), ("csvr", temperature, damping, seed) or
@@ -56,7 +93,10 @@ def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_step
     (E_tot, or H_NHC with a chain) per step (n_steps + 1, index 0 the
     initial state), the number of list rebuilds, the loop's wall time and,
     with flux_every > 0, the heat flux J (frames, 3) [eV/(A^2 fs)] of the
-    state after every step s with s % flux_every == 0."""
+    state after every step s with s % flux_every == 0.  ``state`` (the
+    "state" array a previous run returned, one of ``thermostat_state``, or
+    None to start from rest) continues the thermostat, barostat and random
+    stream where that run ended; the final one is returned as "state"."""
     lat = np.ascontiguousarray(lattice, float)
     pos = np.ascontiguousarray(positions, float).copy()
     vel = np.ascontiguousarray(velocities, float).copy()
@@ -75,20 +115,21 @@ def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_step
         kind = thermostat[0]
         if kind == "nhc":
             _, T, tau, chain = thermostat
-            th = np.array([1.0, float(T), float(tau), float(chain), 0.0, 0.0, 0.0])
+            th = np.array([1.0, float(T), float(tau), float(chain), 0.0, 0.0, 0.0, 0.0, 0.0])
         elif kind in ("langevin", "csvr"):
             _, T, damping, seed = thermostat
             th = np.array([2.0 if kind == "langevin" else 3.0, float(T), float(damping), 0.0,
-                           float(int(seed)), 0.0, 0.0])
+                           float(int(seed)), 0.0, 0.0, 0.0, 0.0])
         elif kind == "npt":
             _, T, tau, chain, p_gpa, tau_b = thermostat
             th = np.array([4.0, float(T), float(tau), float(chain), 0.0, float(p_gpa),
-                           float(tau_b)])
+                           float(tau_b), 0.0, 0.0])
         else:
             raise ValueError(f"thermostat: unknown kind {kind!r} (nhc, langevin, csvr)")
     lat_out = np.zeros((3, 3))
     vol = np.zeros(n_steps + 1)
     prs = np.zeros(n_steps + 1)
+    st = np.zeros(STATE_LEN) if state is None else np.array(state, float).reshape(STATE_LEN)
     nreb = ctypes.c_int32(0)
     secs = ctypes.c_double(0.0)
     P = _lib._pf64
@@ -96,14 +137,15 @@ def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_step
                           pos.ctypes.data_as(P), vel.ctypes.data_as(P), m.ctypes.data_as(P),
                           spc.ctypes.data_as(_lib._pi32), float(dt), int(n_steps),
                           float(skin_width), 1 if use_graph else 0,
-                          th.ctypes.data_as(P) if th is not None else None, every,
+                          th.ctypes.data_as(P) if th is not None else None,
+                          st.ctypes.data_as(P), every,
                           ep.ctypes.data_as(P), ek.ctypes.data_as(P), ec.ctypes.data_as(P),
                           J.ctypes.data_as(P), lat_out.ctypes.data_as(P), vol.ctypes.data_as(P),
                           prs.ctypes.data_as(P), ctypes.byref(nreb), ctypes.byref(secs)))
     et = ep + ek
     out = {"positions": pos, "velocities": vel, "e_pot": ep, "e_kin": ek, "e_tot": et,
            "e_cons": ec, "drift": et - et[0], "n_rebuilds": int(nreb.value),
-           "seconds": float(secs.value)}
+           "seconds": float(secs.value), "state": st}
     if every > 0:
         out["flux"] = J[:frames]
     if th is not None and th[0] == 4.0:
@@ -124,14 +166,14 @@ def nve_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_ste
 
 def nvt_run(potential, lattice, positions, velocities, atom_types, temperature, dt=1.0,
             n_steps=1000, tau_nhc=100.0, nhc_chain=3, skin_width=0.5, masses=None,
-            use_graph=True, flux_every=0):
+            use_graph=True, flux_every=0, state=None):
     """NVT with an M-link Nose-Hoover chain on the device (NVTSimulation.run,
     solvers/molecular_dynamics.py:1113-1160, _nhc_step :635-727; defaults of
     NVTSimulation: tau_nhc 100 fs, 3 links, skin 0.5 A).  e_cons is H_NHC
     (:947-969); T_inst = 2 E_kin / (n_dof k_B), n_dof = 3N - 3."""
     out = md_run(potential, lattice, positions, velocities, atom_types, dt, n_steps,
                  skin_width, masses, use_graph, ("nhc", temperature, tau_nhc, nhc_chain),
-                 flux_every)
+                 flux_every, state)
     n_dof = max(3 * len(positions) - 3, 1)
     out["T"] = 2.0 * out["e_kin"] / (n_dof * K_B)
     return out
@@ -139,7 +181,7 @@ def nvt_run(potential, lattice, positions, velocities, atom_types, temperature, 
 
 def langevin_run(potential, lattice, positions, velocities, atom_types, temperature, dt=1.0,
                  n_steps=1000, damping=100.0, seed=42, skin_width=0.5, masses=None,
-                 use_graph=True, flux_every=0):
+                 use_graph=True, flux_every=0, state=None):
     """NVT with a Langevin thermostat integrated by BAOAB on the device
     (LangevinSimulation, solvers/molecular_dynamics.py:1511-1587): e_cons is
     E_tot - Q, Q the heat the O steps and the centre-of-mass removal added.
@@ -147,26 +189,27 @@ def langevin_run(potential, lattice, positions, velocities, atom_types, temperat
     numpy's generator: same process, other realisations)."""
     out = md_run(potential, lattice, positions, velocities, atom_types, dt, n_steps,
                  skin_width, masses, use_graph, ("langevin", temperature, damping, seed),
-                 flux_every)
+                 flux_every, state)
     out["T"] = 2.0 * out["e_kin"] / (max(3 * len(positions) - 3, 1) * K_B)
     return out
 
 
 def csvr_run(potential, lattice, positions, velocities, atom_types, temperature, dt=1.0,
              n_steps=1000, damping=100.0, seed=42, skin_width=0.5, masses=None,
-             use_graph=True, flux_every=0):
+             use_graph=True, flux_every=0, state=None):
     """NVT with the Bussi-Donadio-Parrinello velocity rescaling on the device
     (CSVRSimulation, solvers/molecular_dynamics.py:1589-1687): R B A F B R,
     e_cons = E_tot - Q; random numbers as in :func:`langevin_run`."""
     out = md_run(potential, lattice, positions, velocities, atom_types, dt, n_steps,
-                 skin_width, masses, use_graph, ("csvr", temperature, damping, seed), flux_every)
+                 skin_width, masses, use_graph, ("csvr", temperature, damping, seed), flux_every,
+                 state)
     out["T"] = 2.0 * out["e_kin"] / (max(3 * len(positions) - 3, 1) * K_B)
     return out
 
 
 def npt_run(potential, lattice, positions, velocities, atom_types, temperature, pressure=0.0,
             dt=1.0, n_steps=1000, tau_nhc=100.0, nhc_chain=3, tau_baro=1000.0, skin_width=0.5,
-            masses=None, use_graph=True):
+            masses=None, use_graph=True, state=None):
     """NPT with the isotropic MTTK barostat (a 2-link chain on its momentum)
     and the particle Nose-Hoover chain, on the device (NPTSimulation with
     cell_dof "iso", baro_thermostat "nhc", solvers/molecular_dynamics.py:
@@ -176,6 +219,6 @@ def npt_run(potential, lattice, positions, velocities, atom_types, temperature, 
     H_NPT and T.  The cell may shrink by up to 5 % linear in one call."""
     out = md_run(potential, lattice, positions, velocities, atom_types, dt, n_steps,
                  skin_width, masses, use_graph,
-                 ("npt", temperature, tau_nhc, nhc_chain, pressure, tau_baro), 0)
+                 ("npt", temperature, tau_nhc, nhc_chain, pressure, tau_baro), 0, state)
     out["T"] = 2.0 * out["e_kin"] / (max(3 * len(positions) - 3, 1) * K_B)
     return out

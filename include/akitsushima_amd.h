@@ -219,10 +219,20 @@ int ak_md_nve(ak_potential *pot, int32_t n_atoms, const double *lattice,
  * :1511-1587), CSVRSimulation (:1589-1687) and NPTSimulation with the
  * isotropic barostat (:2141-2555, cell_dof "iso", baro_thermostat "nhc").
  * thermostat: NULL for NVE, or {kind, temperature K, tau fs, chain links,
- * seed, pressure GPa, tau_baro fs}: kind 1 = Nose-Hoover chain (tau_nhc,
- * 2-16 links), 2 = Langevin, 3 = CSVR (tau = damping; seed an integer
- * < 2^53 keys the Philox4x32-10 stream), 4 = NPT (the chain of kind 1 plus
- * the barostat; the cell may shrink by up to 5 % linear); n_dof = 3N - 3.
+ * seed, pressure GPa, tau_baro fs, cell_dof, baro_thermostat}: kind 1 =
+ * Nose-Hoover chain (tau_nhc, 2-16 links), 2 = Langevin, 3 = CSVR (tau =
+ * damping; seed an integer < 2^53 keys the Philox4x32-10 stream), 4 = NPT
+ * (the chain of kind 1 plus the barostat; the cell may shrink by up to 5 %
+ * linear; cell_dof 0 = iso, baro_thermostat 0 = Nose-Hoover chain);
+ * n_dof = 3N - 3.
+ * state (AK_MD_STATE_LEN doubles, may be NULL = start from rest) carries the
+ * thermostat between calls, as the reference's checkpoint scalars do
+ * (_ck_arrays / _CK_SCALARS, :431-445, 823-830, 2228, 2557-2564): read at
+ * the start, the final state on return.  Layout: [0, 16) chain positions xi, [16, 32) chain
+ * velocities vxi, 32 p_eps, [33, 35) barostat chain xi_b, [35, 37) vxi_b,
+ * [37, 43) reserved, 43 the heat Q a stochastic thermostat has added, 44
+ * the random draws used so far (a continued run draws new numbers), rest
+ * reserved (pass zeros).
  * e_cons (n_steps + 1): E_tot (NVE), H_NHC (:947-969), E_tot - Q, Q the heat
  * a stochastic thermostat added (the E-Q column), or H_NPT (:2425-2454).
  * lattice_out (9): the final cell; volume, pressure (n_steps + 1): V [A^3]
@@ -233,10 +243,11 @@ int ak_md_nve(ak_potential *pot, int32_t n_atoms, const double *lattice,
 int ak_md_run(ak_potential *pot, int32_t n_atoms, const double *lattice,
               const double *inv_lattice, double *positions, double *velocities,
               const double *masses, const int32_t *species, double dt, int32_t n_steps,
-              double skin_width, int32_t use_graph, const double *thermostat7,
-              int32_t flux_every, double *e_pot, double *e_kin, double *e_cons, double *flux,
-              double *lattice_out, double *volume, double *pressure, int32_t *n_rebuilds,
-              double *seconds);
+              double skin_width, int32_t use_graph, const double *thermostat9,
+              double *state, int32_t flux_every, double *e_pot, double *e_kin,
+              double *e_cons, double *flux, double *lattice_out, double *volume,
+              double *pressure, int32_t *n_rebuilds, double *seconds);
+#define AK_MD_STATE_LEN 48
 
 /* ---- Site terms and heat flux ------------------------------------------ */
 

@@ -46,6 +46,11 @@ What pins each function (none is left unpinned):
 * mb_velocities, nve_run, md_run: zero COM momentum and exact temperature;
   energy conservation and time reversibility; with the chain, conservation
   of H_NHC, time reversibility and the canonical mean temperature
+* md_run with the chain and npt_run: the reference's recorded 64-atom Si
+  NVT and NPT runs (tests/golden/nhc_legacy_si64.npz: energies, P, V of
+  every step, the chain and barostat state at step 150, the step-300
+  state), from the start and continued from its step-150 checkpoint
+  (tests/test_oracle_recorded_md.py)
 * site_terms: rebuilt forces/stress/energy = the force path; heat flux =
   d/dt sum_i r_i E_i on isolated clusters (tests/test_oracle_site_terms.py)
 """
@@ -810,7 +815,8 @@ def langevin_noise(n, step, k0, k1):
 
 
 def md_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, skin,
-           nhc=None, flux=None, every=0, langevin=None, csvr=None, state_out=None):
+           nhc=None, flux=None, every=0, langevin=None, csvr=None, state_out=None,
+           nhc_state=None):
     """Velocity Verlet with PBC wrap and the Verlet-skin rebuild test of
     engine/neighborlist.py:613-673 (fixed cell: rebuild when r_list - 2 d_max
     <= r_cut); ``force(lattice, pos, (i, j))`` returns (E, F, S).  nhc =
@@ -823,7 +829,8 @@ def md_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, sk
     every == 0.  Returns E_pot, E_kin, conserved quantity (E_tot, H_NHC
     (:947-969) or E_tot - Q), final positions and velocities, rebuilds, J
     frames; a dict passed as state_out receives the final chain positions
-    "xi" and velocities "vxi"."""
+    "xi" and velocities "vxi"; nhc_state = (xi, vxi) starts the chain there
+    (a checkpoint restart) instead of at rest."""
     lattice = np.asarray(lattice, float)
     pos = np.array(positions, float)
     vel = np.array(velocities, float)
@@ -842,6 +849,9 @@ def md_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, sk
         kT = K_B * T
         state = ([n_dof * K_B * T * tau ** 2] + [K_B * T * tau ** 2] * (M - 1), [0.0] * M,
                  [0.0] * M)
+        if nhc_state is not None:
+            state = (state[0], [float(x) for x in nhc_state[0]],
+                     [float(x) for x in nhc_state[1]])
     stoch = langevin if langevin is not None else csvr
     if stoch is not None:
         T, damping, seed = stoch
@@ -923,7 +933,7 @@ def md_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, sk
 
 
 def npt_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, skin, T, tau, M,
-            p_gpa, tau_baro):
+            p_gpa, tau_baro, start=None):
     """NPTSimulation.run with cell_dof "iso" and baro_thermostat "nhc"
     (solvers/molecular_dynamics.py:2871-2895): particle chain and barostat
     chain half-steps, barostat half-step at the current P_inst, half kick,
@@ -934,7 +944,9 @@ def npt_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, s
     ``force(lattice, pos, pairs)`` -> (E, F, sigma).  Returns E_pot, E_kin,
     H_NPT (:2425-2454), V, P_inst [GPa] per step, final positions,
     velocities and lattice, rebuilds, and the final particle chain ("xi",
-    "vxi") and barostat state ("p_eps", "xi_b", "vxi_b")."""
+    "vxi") and barostat state ("p_eps", "xi_b", "vxi_b"); a dict with those
+    keys passed as ``start`` begins the run from that state (a checkpoint
+    restart)."""
     lattice = np.array(lattice, float)
     pos = np.array(positions, float)
     vel = np.array(velocities, float)
@@ -947,6 +959,11 @@ def npt_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, s
     Qb = kT * tau_baro ** 2
     P_ext = p_gpa / 160.2176621
     b = {"p": 0.0, "xi1": 0.0, "xi2": 0.0, "v1": 0.0, "v2": 0.0}
+    if start is not None:
+        state = (state[0], [float(x) for x in start["xi"]], [float(x) for x in start["vxi"]])
+        b = {"p": float(start["p_eps"]), "xi1": float(start["xi_b"][0]),
+             "xi2": float(start["xi_b"][1]), "v1": float(start["vxi_b"][0]),
+             "v2": float(start["vxi_b"][1])}
     vol = lambda L: float(abs(np.linalg.det(L)))
 
     def p_inst(v, S, L):

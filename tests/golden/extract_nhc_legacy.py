@@ -5,10 +5,11 @@ final_300.npz of the reference checkout (only where it is present; numpy
 loads with allow_pickle=False) and keeps the recorded trajectory data:
 
 * nvt: E_pot, E_kin and T after steps 1..150, the thermo rows (written
-  after steps 1, 11, ..., 141 by that version) with H_NHC, and the chain
-  state after step 150;
-* npt: T, P [GPa] and V after steps 1..150, the chain and barostat state
-  after step 150 and H_NPT of the first row;
+  after steps 1, 11, ..., 141 by that version) with H_NHC, and the
+  checkpointed state after step 150 (positions, velocities, chain);
+* npt: T, P [GPa] and V after steps 1..150, the checkpointed state after
+  step 150 (positions, velocities, cell, chain and barostat) and H_NPT of
+  the first row;
 * the positions, velocities and lattice after step 300 of both runs.
 
 The runs are tests/test_nhc_chain.py:24-48,136-170 of the reference: 64-atom
@@ -41,6 +42,10 @@ def main():
         "nvt_e_pot": nvt["list_e_pot"], "nvt_e_kin": nvt["list_e_kin"], "nvt_T": nvt["list_T"],
         "nvt_row_step": nvt["tlog_step"], "nvt_row_H": nvt["tlog_H_nhc"],
         "nvt_chain150": np.array([mv["scalars"][k] for k in ("xi1", "xi2", "vxi1", "vxi2")]),
+        "nvt_pos150": nvt["positions"], "nvt_vel150": nvt["velocities"],
+        "nvt_lat150": nvt["lattice"],
+        "npt_pos150": npt["positions"], "npt_vel150": npt["velocities"],
+        "npt_lat150": npt["lattice"],
         "npt_T": npt["list_T"], "npt_P": npt["list_P"], "npt_V": npt["list_V"],
         "npt_state150": np.array([mp["scalars"][k] for k in
                                   ("xi1", "xi2", "vxi1", "vxi2", "p_eps", "xi1_b", "xi2_b",

@@ -116,3 +116,32 @@ def test_npt_state_after_300_steps(npt):
     assert _rel(r["positions"], REC["npt_pos300"]) < 1e-9
     assert _rel(r["velocities"], REC["npt_vel300"]) < 1e-9
     assert _rel(r["lattice"], REC["npt_lat300"]) < 1e-12
+
+
+# The reference's own restart test (tests/test_nhc_chain.py:143-170): the
+# step-150 checkpoint -- positions, velocities, cell, chain (and barostat)
+# scalars -- carried on for 150 steps lands on the uninterrupted run.
+
+def test_nvt_continued_from_the_checkpoint():
+    m = np.full(64, 28.086)
+    c = REC["nvt_chain150"]
+    *_, pos, vel, _, _ = O.md_run(_force, REC["nvt_lat150"], REC["nvt_pos150"],
+                                  REC["nvt_vel150"], m, 1.0, 150, R_CUT, 0.5,
+                                  nhc=(600.0, 100.0, 2), nhc_state=(c[0:2], c[2:4]))
+    assert _rel(pos, REC["nvt_pos300"]) < 1e-9
+    assert _rel(vel, REC["nvt_vel300"]) < 1e-9
+
+
+def test_npt_continued_from_the_checkpoint():
+    m = np.full(64, 28.086)
+    s = REC["npt_state150"]
+    start = {"xi": s[0:2], "vxi": s[2:4], "p_eps": s[4], "xi_b": s[5:7], "vxi_b": s[7:9]}
+    r = O.npt_run(_force, REC["npt_lat150"], REC["npt_pos150"], REC["npt_vel150"], m, 1.0, 150,
+                  R_CUT, 0.5, 600.0, 100.0, 2, 0.0, 500.0, start=start)
+    assert _rel(r["positions"], REC["npt_pos300"]) < 1e-9
+    assert _rel(r["velocities"], REC["npt_vel300"]) < 1e-9
+    assert _rel(r["lattice"], REC["npt_lat300"]) < 1e-12
+    # a restart that forgets the barostat momentum lands elsewhere
+    bad = O.npt_run(_force, REC["npt_lat150"], REC["npt_pos150"], REC["npt_vel150"], m, 1.0,
+                    150, R_CUT, 0.5, 600.0, 100.0, 2, 0.0, 500.0, start=dict(start, p_eps=0.0))
+    assert _rel(bad["lattice"], REC["npt_lat300"]) > 1e-6
