@@ -29,7 +29,8 @@ after the reference's Python glue:
 * ``tersoff_loss``: fitting/loss.py:51-248 (squared loss)
 * ``nve_run``, ``md_run``: NVESimulation.run, solvers/molecular_dynamics.py:1969-1996,
   and NVTSimulation.run with the Nose-Hoover chain (:635-727, 1113-1160)
-* ``npt_run``: NPTSimulation.run, isotropic barostat (:2141-2555)
+* ``npt_run``: NPTSimulation.run, every cell mode and both barostat
+  thermostats (:2141-2692, 2871-2895)
 * ``site_terms``: engine/site_terms.py (site energies, per-bond gradients,
   heat flux)
 
@@ -50,7 +51,10 @@ What pins each function (none is left unpinned):
   NVT and NPT runs (tests/golden/nhc_legacy_si64.npz: energies, P, V of
   every step, the chain and barostat state at step 150, the step-300
   state), from the start and continued from its step-150 checkpoint
-  (tests/test_oracle_recorded_md.py)
+  (tests/test_oracle_recorded_md.py); the flexible-cell modes: H_NPT
+  conserved to O(dt^2), the frozen-barostat limit = the NVT run, axes
+  without a generator left bit-identical, relaxation of an anisotropic
+  strain to the target pressure tensor (tests/test_oracle_npt_cell.py)
 * site_terms: rebuilt forces/stress/energy = the force path; heat flux =
   d/dt sum_i r_i E_i on isolated clusters (tests/test_oracle_site_terms.py)
 """
@@ -932,21 +936,63 @@ def md_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, sk
             np.array(J).reshape(-1, 3))
 
 
+def cell_generators(mode):
+    """Strain generators E_k of an NPT cell mode (_cell_generators,
+    solvers/molecular_dynamics.py:2116-2138): a diagonal unit lets one axis
+    breathe, e_ab + e_ba is a shear, the sum of two diagonal units couples
+    those axes."""
+    def unit(a, b):
+        e = np.zeros((3, 3))
+        e[a, b] = e[b, a] = 1.0
+        return e
+    xx, yy, zz = unit(0, 0), unit(1, 1), unit(2, 2)
+    table = {"aniso": [xx, yy, zz], "tri": [xx, yy, zz, unit(1, 2), unit(0, 2), unit(0, 1)],
+             "xy": [xx, yy], "xy_iso": [xx + yy], "z": [zz]}
+    if mode not in table:
+        raise ValueError(f"cell_dof must be one of iso | {' | '.join(table)}, got {mode!r}")
+    return np.array(table[mode])
+
+
+def baro_noise(n, draw, k0, k1):
+    """n standard normals of barostat draw ``draw``: Philox blocks
+    (k, draw, 5, 0), Box-Muller, first normal."""
+    k = np.arange(n, dtype=np.uint64)
+    z0, _ = normal2(philox4x32([k, np.full(n, draw), np.full(n, 5), np.zeros(n)], k0, k1))
+    return np.asarray(z0, float)
+
+
 def npt_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, skin, T, tau, M,
-            p_gpa, tau_baro, start=None):
-    """NPTSimulation.run with cell_dof "iso" and baro_thermostat "nhc"
-    (solvers/molecular_dynamics.py:2871-2895): particle chain and barostat
-    chain half-steps, barostat half-step at the current P_inst, half kick,
-    drift with the isotropic cell scaling and the (1 + 3/n_dof) velocity
-    factor (_scale_step :2517-2555), forces, half kick, barostat half-step at
-    the new P_inst, barostat chain, particle chain.  The list follows
-    check_and_rebuild with cell changes (engine/neighborlist.py:613-673).
+            p_gpa, tau_baro, start=None, cell_dof="iso", baro="nhc", seed=0, generators=None):
+    """NPTSimulation.run (solvers/molecular_dynamics.py:2871-2895): particle
+    chain and barostat thermostat half-steps, barostat half-step, half kick,
+    drift with the cell scaling, forces, half kick, barostat half-step at the
+    new pressure, barostat thermostat, particle chain.
+
+    cell_dof "iso": p_eps += dt/2 [3 V (P_inst - P) + (3/n_dof) 2 E_kin] with
+    P_inst of the velocities before the particle chain's half-step, the cell
+    and positions scaled by exp(eps_dot dt) and the velocities by
+    exp(-(1 + 3/n_dof) eps_dot dt) (_barostat_half_step, _scale_step
+    :2496-2555).  Other modes (ADR-0036, :2566-2692): one momentum p_k per
+    strain generator E_k (``cell_generators``, or ``generators`` given),
+    p_k += dt/2 [V Tr(E_k (P_int - P I)) + Tr(E_k) 2 E_kin / n_dof] with the
+    pressure tensor P_int = sum m v v^T / V + sym(sigma) of the current
+    velocities, G = sum_k (p_k / W) E_k, L -> L expm(G dt), r -> r expm(G dt),
+    v -> v expm(-(G + Tr(G)/n_dof) dt) (elementwise for a diagonal G,
+    through eigh otherwise).  baro "nhc": a 2-link chain on the barostat
+    momenta (Q1 = n_k kT tau_b^2, target n_k kT); "langevin": p <- c p +
+    sqrt((1 - c^2) W kT) xi, c = exp(-dt / (2 tau_b)), its heat subtracted
+    from H_NPT (_baro_langevin :2596-2617), xi from the Philox stream of the
+    device (``baro_noise``, keyed by seed).  The list follows
+    check_and_rebuild with cell changes (engine/neighborlist.py:613-677):
+    sigma_min(inv(L_b) L), 1 when L is allclose to L_b, and the non-affine
+    displacement from the build positions mapped by inv(L_b) L.
     ``force(lattice, pos, pairs)`` -> (E, F, sigma).  Returns E_pot, E_kin,
-    H_NPT (:2425-2454), V, P_inst [GPa] per step, final positions,
-    velocities and lattice, rebuilds, and the final particle chain ("xi",
-    "vxi") and barostat state ("p_eps", "xi_b", "vxi_b"); a dict with those
-    keys passed as ``start`` begins the run from that state (a checkpoint
-    restart)."""
+    H_NPT (:2425-2454, 2648-2657), V, P_inst [GPa], the cell ("lat") and the
+    pressure tensor [GPa] ("ptensor", _pressure_tensor :2570-2577) per step, final positions,
+    velocities and lattice, rebuilds, the final particle chain ("xi", "vxi")
+    and barostat state ("p_eps" (iso) or "p_g", "xi_b", "vxi_b", "heat"); a
+    dict with those keys passed as ``start`` begins the run from that state
+    (a checkpoint restart)."""
     lattice = np.array(lattice, float)
     pos = np.array(positions, float)
     vel = np.array(velocities, float)
@@ -956,37 +1002,89 @@ def npt_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, s
     ek = lambda v: 0.5 * float(np.sum(masses[:, None] * v ** 2)) / EV_ANG_TO_AMU_ANG_FS2
     state = ([n_dof * kT * tau ** 2] + [kT * tau ** 2] * (M - 1), [0.0] * M, [0.0] * M)
     W = (n_dof + 3) * kT * tau_baro ** 2
-    Qb = kT * tau_baro ** 2
+    iso = cell_dof == "iso" and generators is None
+    if iso:
+        gens, n_b = None, 1
+    else:
+        gens = np.asarray(generators, float) if generators is not None else \
+            cell_generators(cell_dof)
+        n_b = len(gens)
+        gtr = np.trace(gens, axis1=1, axis2=2)
+    Qb1, Qb2 = n_b * kT * tau_baro ** 2, kT * tau_baro ** 2
     P_ext = p_gpa / 160.2176621
+    if baro not in ("nhc", "langevin"):
+        raise ValueError("baro must be nhc | langevin")
     b = {"p": 0.0, "xi1": 0.0, "xi2": 0.0, "v1": 0.0, "v2": 0.0}
+    pg = np.zeros(n_b)
+    heat = [0.0]
+    draw = [0]
+    k0, k1 = int(seed) & 0xFFFFFFFF, int(seed) >> 32
     if start is not None:
         state = (state[0], [float(x) for x in start["xi"]], [float(x) for x in start["vxi"]])
-        b = {"p": float(start["p_eps"]), "xi1": float(start["xi_b"][0]),
+        b = {"p": float(start.get("p_eps", 0.0)), "xi1": float(start["xi_b"][0]),
              "xi2": float(start["xi_b"][1]), "v1": float(start["vxi_b"][0]),
              "v2": float(start["vxi_b"][1])}
+        if not iso:
+            pg = np.array(start["p_g"], float)
+        heat[0] = float(start.get("heat", 0.0))
+        draw[0] = int(start.get("draws", 0))
     vol = lambda L: float(abs(np.linalg.det(L)))
 
     def p_inst(v, S, L):
         return 2.0 * ek(v) / (3.0 * vol(L)) + np.trace(S) / 3.0
 
     def baro_nhc():
-        e1 = 0.5 * b["p"] ** 2 / W
-        G1 = (2.0 * e1 - kT) / Qb
-        G2 = (Qb * b["v1"] ** 2 - kT) / Qb
+        # _nhc_baro_step / _nhc_baro_step_g (:2456-2494, 2619-2646)
+        pb = np.array([b["p"]]) if iso else pg
+        e1 = 0.5 * float(np.sum(pb ** 2)) / W
+        G1 = (2.0 * e1 - n_b * kT) / Qb1
+        G2 = (Qb1 * b["v1"] ** 2 - kT) / Qb2
         b["v2"] += G2 * (dt * 0.25)
         sc = float(np.exp(np.clip(-b["v2"] * dt * 0.125, -350.0, 350.0)))
         b["v1"] = sc * (sc * b["v1"] + G1 * dt * 0.25)
-        b["p"] *= float(np.exp(np.clip(-b["v1"] * dt * 0.5, -350.0, 350.0)))
+        f = float(np.exp(np.clip(-b["v1"] * dt * 0.5, -350.0, 350.0)))
+        if iso:
+            b["p"] *= f
+            pb = np.array([b["p"]])
+        else:
+            pg[:] = pg * f
+            pb = pg
         b["xi1"] += b["v1"] * dt * 0.5
         b["xi2"] += b["v2"] * dt * 0.5
-        e1 = 0.5 * b["p"] ** 2 / W
-        G1 = (2.0 * e1 - kT) / Qb
+        e1 = 0.5 * float(np.sum(pb ** 2)) / W
+        G1 = (2.0 * e1 - n_b * kT) / Qb1
         b["v1"] = sc * (sc * b["v1"] + G1 * dt * 0.25)
-        G2 = (Qb * b["v1"] ** 2 - kT) / Qb
+        G2 = (Qb1 * b["v1"] ** 2 - kT) / Qb2
         b["v2"] += G2 * (dt * 0.25)
 
-    def baro_half(P, v, L):
+    def baro_lv():
+        c = math.exp(-0.5 * dt / tau_baro)
+        p = np.array([b["p"]]) if iso else pg.copy()
+        new = c * p + math.sqrt((1.0 - c * c) * W * kT) * baro_noise(len(p), draw[0], k0, k1)
+        draw[0] += 1
+        heat[0] += 0.5 * float(new @ new - p @ p) / W
+        if iso:
+            b["p"] = float(new[0])
+        else:
+            pg[:] = new
+
+    baro_thermo = baro_lv if baro == "langevin" else baro_nhc
+
+    def baro_half_iso(P, v, L):
         b["p"] += 0.5 * dt * (3.0 * vol(L) * (P - P_ext) + (3.0 / n_dof) * 2.0 * ek(v))
+
+    def baro_half_g(v, S, L):
+        V = vol(L)
+        kin = (v.T * masses) @ v / EV_ANG_TO_AMU_ANG_FS2
+        D = kin / V + 0.5 * (S + S.T) - P_ext * np.eye(3)
+        f = V * np.einsum("kab,ab->k", gens, D) + gtr * (2.0 * ek(v)) / n_dof
+        pg[:] = pg + 0.5 * dt * f
+
+    def wrap(p, L):
+        fr = p @ np.linalg.inv(L)
+        fr -= np.floor(fr)
+        fr[fr >= 1.0] = 0.0
+        return fr @ L
 
     def H(e, v, L):
         Q, xi, w = state
@@ -996,39 +1094,65 @@ def npt_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, s
         u = n_dof * kT * xi[0]
         for j in range(1, M):
             u = u + kT * xi[j]
-        ed = b["p"] / W
-        hb = 0.5 * Qb * b["v1"] ** 2 + 0.5 * Qb * b["v2"] ** 2 + kT * b["xi1"] + kT * b["xi2"]
-        return ek(v) + e + P_ext * vol(L) + 0.5 * W * ed ** 2 + t + u + hb
+        if iso:
+            ed = b["p"] / W
+            hb = 0.5 * Qb1 * b["v1"] ** 2 + 0.5 * Qb2 * b["v2"] ** 2 + kT * b["xi1"] + \
+                kT * b["xi2"]
+            return ek(v) + e + P_ext * vol(L) + 0.5 * W * ed ** 2 + t + u + hb - heat[0]
+        hb = 0.5 * Qb1 * b["v1"] ** 2 + 0.5 * Qb2 * b["v2"] ** 2 + n_b * kT * b["xi1"] + \
+            kT * b["xi2"]
+        return (ek(v) + e + P_ext * vol(L) + 0.5 * float(np.sum(pg ** 2)) / W + (t + u) + hb) \
+            - heat[0]
 
     ref = pos.copy()
     lat_b = lattice.copy()
     pairs = nl_build(lattice, pos, r_list)[:2]
     e, f, S = force(lattice, pos, pairs)
     acc = f * EV_ANG_TO_AMU_ANG_FS2 / masses[:, None]
+    def ptensor(v, S, L):
+        kin = (v.T * masses) @ v / EV_ANG_TO_AMU_ANG_FS2
+        return (kin / vol(L) + 0.5 * (S + S.T)) * 160.2176621
+
     rec = {"ep": [e], "ek": [ek(vel)], "h": [H(e, vel, lattice)], "v": [vol(lattice)],
-           "p": [p_inst(vel, S, lattice) * 160.2176621]}
+           "p": [p_inst(vel, S, lattice) * 160.2176621], "lat": [lattice.copy()],
+           "ptensor": [ptensor(vel, S, lattice)]}
     rebuilds = 0
     for _ in range(n_steps):
         P = p_inst(vel, S, lattice)
         vel = nhc_half(state, dt, kT, n_dof, vel, masses)
-        baro_nhc()
-        baro_half(P, vel, lattice)
-        vel = vel + 0.5 * dt * acc
-        ed = b["p"] / W
-        s = float(np.exp(ed * dt))
-        pos = pos + dt * vel
-        fr = pos @ np.linalg.inv(lattice)
-        fr -= np.floor(fr)
-        fr[fr >= 1.0] = 0.0
-        pos = fr @ lattice
-        fr = pos @ np.linalg.inv(lattice)
-        lattice = lattice * s
-        pos = fr @ lattice
-        vel = vel * float(np.exp(-(1.0 + 3.0 / n_dof) * ed * dt))
-        if np.allclose(lattice, lat_b, atol=1e-12):
-            Mm, sig = np.eye(3), 1.0
+        baro_thermo()
+        if iso:
+            baro_half_iso(P, vel, lattice)
         else:
-            Mm = np.linalg.inv(lat_b) @ lattice
+            baro_half_g(vel, S, lattice)
+        vel = vel + 0.5 * dt * acc
+        if iso:
+            ed = b["p"] / W
+            s = float(np.exp(ed * dt))
+            pos = wrap(pos + dt * vel, lattice)
+            fr = pos @ np.linalg.inv(lattice)
+            lattice = lattice * s
+            pos = fr @ lattice
+            vel = vel * float(np.exp(-(1.0 + 3.0 / n_dof) * ed * dt))
+        else:
+            G = np.einsum("k,kab->ab", pg / W, gens)
+            pos = wrap(pos + dt * vel, lattice)
+            a = np.trace(G) / n_dof
+            if not np.any(G - np.diag(np.diag(G))):
+                g = np.diag(G)
+                Mx = np.diag(np.exp(g * dt))
+                Mv = np.diag(np.exp(-(g + a) * dt))
+            else:
+                w, U = np.linalg.eigh(G)
+                Mx = (U * np.exp(w * dt)) @ U.T
+                Mv = (U * np.exp(-(w + a) * dt)) @ U.T
+            lattice = lattice @ Mx
+            pos = pos @ Mx
+            vel = vel @ Mv
+        Mm = np.linalg.inv(lat_b) @ lattice
+        if np.allclose(lattice, lat_b, atol=1e-12):
+            sig = 1.0
+        else:
             sig = float(np.linalg.svd(Mm, compute_uv=False)[-1])
         dr = pos - ref @ Mm
         fr = dr @ np.linalg.inv(lattice)
@@ -1043,18 +1167,24 @@ def npt_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, s
         acc = f * EV_ANG_TO_AMU_ANG_FS2 / masses[:, None]
         vel = vel + 0.5 * dt * acc
         Pn = p_inst(vel, S, lattice)
-        baro_half(Pn, vel, lattice)
-        baro_nhc()
+        if iso:
+            baro_half_iso(Pn, vel, lattice)
+        else:
+            baro_half_g(vel, S, lattice)
+        baro_thermo()
         vel = nhc_half(state, dt, kT, n_dof, vel, masses)
         rec["ep"].append(e)
         rec["ek"].append(ek(vel))
         rec["h"].append(H(e, vel, lattice))
         rec["v"].append(vol(lattice))
         rec["p"].append(Pn * 160.2176621)
+        rec["lat"].append(lattice.copy())
+        rec["ptensor"].append(ptensor(vel, S, lattice))
     out = {k: np.array(v) for k, v in rec.items()}
     out.update(positions=pos, velocities=vel, lattice=lattice, rebuilds=rebuilds,
-               xi=list(state[1]), vxi=list(state[2]), p_eps=b["p"], xi_b=[b["xi1"], b["xi2"]],
-               vxi_b=[b["v1"], b["v2"]])
+               xi=list(state[1]), vxi=list(state[2]), p_eps=b["p"], p_g=pg.copy(),
+               xi_b=[b["xi1"], b["xi2"]], vxi_b=[b["v1"], b["v2"]], heat=heat[0],
+               draws=draw[0], stress=S)
     return out
 
 
