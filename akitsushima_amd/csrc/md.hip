@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "sym3.hpp"
 
 namespace ak {
 
@@ -73,18 +74,23 @@ __global__ __launch_bounds__(256) void k_md_drift(int n, double* __restrict__ po
 }
 
 // a = F conv / m; v += dt/2 a (kick != 0); per-block kinetic-energy sums
-// 0.5 m v^2 / conv.
+// 0.5 m v^2 / conv (NT = 1), and with NT = 7 also the kinetic tensor
+// sum m v_a v_b / conv as xx, yy, zz, yz, xz, xy (ke_part[NT b + c]).
+template <int NT>
 __global__ __launch_bounds__(256) void k_md_kick(int n, double* __restrict__ vel,
                                                  double* __restrict__ acc,
                                                  const double* __restrict__ F,
                                                  const double* __restrict__ mass, double dt,
                                                  int kick, double* __restrict__ ke_part) {
-  __shared__ double smem[16];
+  __shared__ double smem[16 * NT];
   int a = blockIdx.x * blockDim.x + threadIdx.x;
-  double ke[1] = {0.0};
+  double ke[NT];
+#pragma unroll
+  for (int c = 0; c < NT; ++c) ke[c] = 0.0;
   if (a < n) {
     double m = mass[a];
     double s = 0.0;
+    double v3[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       double ad = F[3 * a + d] * kConv / m;
@@ -95,11 +101,24 @@ __global__ __launch_bounds__(256) void k_md_kick(int n, double* __restrict__ vel
         vel[3 * a + d] = v;
       }
       s += v * v;
+      v3[d] = v;
     }
     ke[0] = 0.5 * m * s / kConv;
+    if constexpr (NT == 7) {
+      const double mc = m / kConv;
+      ke[1] = mc * v3[0] * v3[0];
+      ke[2] = mc * v3[1] * v3[1];
+      ke[3] = mc * v3[2] * v3[2];
+      ke[4] = mc * v3[1] * v3[2];
+      ke[5] = mc * v3[0] * v3[2];
+      ke[6] = mc * v3[0] * v3[1];
+    }
   }
-  block_sum<1>(ke, smem);
-  if (threadIdx.x == 0) ke_part[blockIdx.x] = ke[0];
+  block_sum<NT>(ke, smem);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < NT; ++c) ke_part[NT * blockIdx.x + c] = ke[c];
+  }
 }
 
 // One block.  With e_pot != nullptr, records the state the previous kick
@@ -512,7 +531,47 @@ __global__ __launch_bounds__(256) void k_lang_rec(int nb, const double* __restri
 // ---------------------------------------------------------------------------
 struct NptParams {
   double kT, ndof, W, Qb, P_ext, s_min;
+  // flexible cell (cell_dof other than iso): nb strain generators E_k as
+  // symmetric xx, yy, zz, yz, xz, xy, their traces, the barostat chain's
+  // first mass nb kT tau_b^2 (Qb is the second)
+  int nb;
+  double gen[6][6], gtr[6], Qb1;
+  // Langevin barostat (baro_thermostat "langevin") with friction time tau_b
+  int lv;
+  double tau_b;
+  uint32_t k0, k1;
 };
+
+// np[] slots beyond the isotropic ones (0-9): 10 the heat the Langevin
+// barostat added; flexible cell: 16 p_k (6), 24 the step's cell map
+// expm(G dt) (9), 33 the velocity map expm(-(G + Tr G / n_dof) dt) (9), 48
+// the new cell (9), 57 its inverse (9), 66 its volume, 72 the cell at the
+// last list build (9), 81 its inverse (9), 96 the build-to-new cell map
+// inv(L_b) L (9).
+constexpr int kNpHeat = 10, kNpPg = 16, kNpMx = 24, kNpMv = 33, kNpLat = 48, kNpInv = 57,
+              kNpVol = 66, kNpLb = 72, kNpLbInv = 81, kNpMap = 96, kNpLen = 112;
+constexpr uint32_t kStreamBaro = 5;
+
+// _baro_langevin (:2596-2617): p_k <- c p_k + sqrt((1 - c^2) W kT) xi_k,
+// c = exp(-dt / (2 tau_b)), xi_k the first normal of Philox block
+// (k, draw, 5, 0), draw = counters[3]; the kinetic energy the noise adds,
+// (p'.p' - p.p) / 2W, goes to np[10].
+__device__ void npt_baro_lv(double* __restrict__ pb, int nb, double* __restrict__ np,
+                            const NptParams& p, double dt, int* __restrict__ counters) {
+  const uint32_t draw = (uint32_t)counters[3];
+  const double c = exp(-0.5 * dt / p.tau_b);
+  const double sig = sqrt((1.0 - c * c) * p.W * p.kT);
+  double old2 = 0.0, new2 = 0.0;
+  for (int k = 0; k < nb; ++k) old2 += pb[k] * pb[k];
+  for (int k = 0; k < nb; ++k) {
+    double z, unused;
+    normal2(philox4x32(U4{(uint32_t)k, draw, kStreamBaro, 0}, p.k0, p.k1), z, unused);
+    pb[k] = c * pb[k] + sig * z;
+    new2 += pb[k] * pb[k];
+  }
+  np[kNpHeat] += 0.5 * (new2 - old2) / p.W;
+  counters[3] = (int)draw + 1;
+}
 
 // _nhc_baro_step (:2456-2494): two links on the barostat momentum.
 __device__ void npt_baro_nhc(double* __restrict__ np, const NptParams& p, double dt) {
@@ -568,7 +627,10 @@ __global__ __launch_bounds__(256) void k_npt(int nb, const double* __restrict__ 
     const double P = 2.0 * ek / (3.0 * V) + trs / 3.0;
     const double s1 = nhc_half(nhc, M, dt, p.kT, p.ndof, ek);
     ek *= s1 * s1;
-    npt_baro_nhc(np, p, dt);
+    if (p.lv)
+      npt_baro_lv(np + 2, 1, np, p, dt, counters);
+    else
+      npt_baro_nhc(np, p, dt);
     npt_baro_half(np, p, dt, V, P, ek);
     const double ed = np[2] / p.W;
     np[7] = exp(ed * dt);
@@ -580,7 +642,10 @@ __global__ __launch_bounds__(256) void k_npt(int nb, const double* __restrict__ 
   const double P = 2.0 * ek / (3.0 * V) + trs / 3.0;
   if (mode == 2) {
     npt_baro_half(np, p, dt, V, P, ek);
-    npt_baro_nhc(np, p, dt);
+    if (p.lv)
+      npt_baro_lv(np + 2, 1, np, p, dt, counters);
+    else
+      npt_baro_nhc(np, p, dt);
     const double s2 = nhc_half(nhc, M, dt, p.kT, p.ndof, ek);
     nhc[0] = s2;
     ek *= s2 * s2;
@@ -598,7 +663,7 @@ __global__ __launch_bounds__(256) void k_npt(int nb, const double* __restrict__ 
   const int step = counters[0];
   e_pot[step] = es[0];
   e_kin[step] = ek;
-  e_cons[step] = ek + es[0] + p.P_ext * V + 0.5 * p.W * ed * ed + t + u + hb;
+  e_cons[step] = (ek + es[0] + p.P_ext * V + 0.5 * p.W * ed * ed + t + u + hb) - np[kNpHeat];
   h_vol[step] = V;
   h_press[step] = P;
   counters[0] = step + 1;
@@ -697,6 +762,296 @@ __global__ __launch_bounds__(256) void k_npt_flag(int nb, const double* __restri
   counters[1] += fl;
 }
 
+// ---------------------------------------------------------------------------
+// NPT with a flexible cell (ADR-0036: cell_dof aniso, tri, xy, xy_iso, z;
+// solvers/molecular_dynamics.py:2113-2138, 2566-2692): one momentum p_k per
+// strain generator E_k, the cell moving as L -> L expm(G dt) with
+// G = sum_k (p_k / W) E_k.  The kick kernels add the kinetic tensor to their
+// block sums, so the pressure tensor sum m v v^T / V + sym(sigma) of the
+// current velocities is formed without another pass over the atoms.
+// ---------------------------------------------------------------------------
+
+// _nhc_baro_step_g (:2619-2646): the 2-link chain over all p_k, target
+// n_k kT, every p_k scaled by the same factor.
+__device__ void nptg_baro_nhc(double* __restrict__ np, const NptParams& p, double dt) {
+  double* pg = np + kNpPg;
+  double e1 = 0.0;
+  for (int k = 0; k < p.nb; ++k) e1 += pg[k] * pg[k];
+  e1 = 0.5 * e1 / p.W;
+  const double nkT = p.nb * p.kT;
+  double G1 = (2.0 * e1 - nkT) / p.Qb1;
+  double G2 = (p.Qb1 * np[5] * np[5] - p.kT) / p.Qb;
+  np[6] += G2 * (dt * 0.25);
+  const double sc = exp(fmin(fmax(-np[6] * dt * 0.125, -350.0), 350.0));
+  np[5] = sc * (sc * np[5] + G1 * dt * 0.25);
+  const double f = exp(fmin(fmax(-np[5] * dt * 0.5, -350.0), 350.0));
+  for (int k = 0; k < p.nb; ++k) pg[k] *= f;
+  np[3] += np[5] * dt * 0.5;
+  np[4] += np[6] * dt * 0.5;
+  double e2 = 0.0;
+  for (int k = 0; k < p.nb; ++k) e2 += pg[k] * pg[k];
+  e2 = 0.5 * e2 / p.W;
+  G1 = (2.0 * e2 - nkT) / p.Qb1;
+  np[5] = sc * (sc * np[5] + G1 * dt * 0.25);
+  G2 = (p.Qb1 * np[5] * np[5] - p.kT) / p.Qb;
+  np[6] += G2 * (dt * 0.25);
+}
+
+// _barostat_half_step_g (:2648-2656): p_k += dt/2 [V Tr(E_k (P_int - P I))
+// + Tr(E_k) 2 E_kin / n_dof], P_int = K / V + sym(sigma), K the kinetic
+// tensor (xx, yy, zz, yz, xz, xy) and sigma = es[1..9] row-major.
+__device__ void nptg_baro_half(double* __restrict__ np, const NptParams& p, double dt, double V,
+                               const double* K, const double* __restrict__ es, double ek) {
+  double D[6];
+  D[0] = K[0] / V + es[1] - p.P_ext;
+  D[1] = K[1] / V + es[5] - p.P_ext;
+  D[2] = K[2] / V + es[9] - p.P_ext;
+  D[3] = K[3] / V + 0.5 * (es[6] + es[8]);
+  D[4] = K[4] / V + 0.5 * (es[3] + es[7]);
+  D[5] = K[5] / V + 0.5 * (es[2] + es[4]);
+  double* pg = np + kNpPg;
+  for (int k = 0; k < p.nb; ++k) {
+    const double* E = p.gen[k];
+    const double tr = E[0] * D[0] + E[1] * D[1] + E[2] * D[2] +
+                      2.0 * (E[3] * D[3] + E[4] * D[4] + E[5] * D[5]);
+    pg[k] = pg[k] + 0.5 * dt * (V * tr + p.gtr[k] * (2.0 * ek) / p.ndof);
+  }
+}
+
+// One block, as k_npt for the flexible cell.  mode 1: the particle chain,
+// the barostat thermostat and half-step, then G, the maps expm(G dt) and
+// expm(-(G + Tr G / n_dof) dt) (elementwise for a diagonal G, by Jacobi
+// eigenvectors otherwise; _scale_step_g :2658-2692), the new cell, its
+// inverse and volume, and the map inv(L_b) L_new.  mode 2: barostat
+// half-step and thermostat, particle chain, record.  mode 0: record.
+__global__ __launch_bounds__(256) void k_nptg(int nb, const double* __restrict__ part,
+                                              double* __restrict__ nhc, int M,
+                                              double* __restrict__ np, NptParams p, double dt,
+                                              int mode, const double* __restrict__ es,
+                                              const Geom* __restrict__ geom,
+                                              int* __restrict__ counters,
+                                              double* __restrict__ e_pot,
+                                              double* __restrict__ e_kin,
+                                              double* __restrict__ e_cons,
+                                              double* __restrict__ h_vol,
+                                              double* __restrict__ h_press) {
+  __shared__ double smem[16 * 7];
+  double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+#pragma unroll
+    for (int c = 0; c < 7; ++c) v[c] += part[7 * b + c];
+  block_sum<7>(v, smem);
+  if (threadIdx.x != 0) return;
+  const double V = geom->volume;
+  const double trs = (es[1] + es[5]) + es[9];
+  double ek = v[0];
+  double K[6] = {v[1], v[2], v[3], v[4], v[5], v[6]};
+  double* pg = np + kNpPg;
+  if (mode == 1) {
+    const double pend = nhc[0];
+    ek *= pend * pend;
+    for (int c = 0; c < 6; ++c) K[c] *= pend * pend;
+    const double s1 = nhc_half(nhc, M, dt, p.kT, p.ndof, ek);
+    ek *= s1 * s1;
+    for (int c = 0; c < 6; ++c) K[c] *= s1 * s1;
+    if (p.lv)
+      npt_baro_lv(pg, p.nb, np, p, dt, counters);
+    else
+      nptg_baro_nhc(np, p, dt);
+    nptg_baro_half(np, p, dt, V, K, es, ek);
+    // G (symmetric) from the momenta
+    double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < p.nb; ++k)
+      for (int c = 0; c < 6; ++c) g[c] += (pg[k] / p.W) * p.gen[k][c];
+    const double a = ((g[0] + g[1]) + g[2]) / p.ndof;
+    double* Mx = np + kNpMx;
+    double* Mv = np + kNpMv;
+    if (dt == 0.0) {
+      // a step of zero length leaves the cell exactly as it is
+      for (int c = 0; c < 9; ++c) Mx[c] = Mv[c] = (c % 4 == 0) ? 1.0 : 0.0;
+    } else if (g[3] == 0.0 && g[4] == 0.0 && g[5] == 0.0) {
+      for (int c = 0; c < 9; ++c) Mx[c] = Mv[c] = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        Mx[4 * d] = exp(g[d] * dt);
+        Mv[4 * d] = exp(-(g[d] + a) * dt);
+      }
+    } else {
+      const double G[9] = {g[0], g[5], g[4], g[5], g[1], g[3], g[4], g[3], g[2]};
+      double w[3], U[9], fx[3], fv[3];
+      sym3_eig(G, w, U);
+      for (int d = 0; d < 3; ++d) {
+        fx[d] = exp(w[d] * dt);
+        fv[d] = exp(-(w[d] + a) * dt);
+      }
+      sym3_compose(U, fx, Mx);
+      sym3_compose(U, fv, Mv);
+    }
+    double* L = np + kNpLat;
+    mat3_mul(geom->lat, Mx, L);
+    if (dt == 0.0) {
+      for (int c = 0; c < 9; ++c) np[kNpInv + c] = geom->inv[c];
+      np[kNpVol] = geom->volume;
+    } else {
+      np[kNpVol] = fabs(mat3_inv(L, np + kNpInv));
+    }
+    mat3_mul(np + kNpLbInv, L, np + kNpMap);
+    nhc[1] = pend * s1;
+    nhc[0] = 1.0;
+    return;
+  }
+  const double P = 2.0 * ek / (3.0 * V) + trs / 3.0;
+  if (mode == 2) {
+    nptg_baro_half(np, p, dt, V, K, es, ek);
+    if (p.lv)
+      npt_baro_lv(pg, p.nb, np, p, dt, counters);
+    else
+      nptg_baro_nhc(np, p, dt);
+    const double s2 = nhc_half(nhc, M, dt, p.kT, p.ndof, ek);
+    nhc[0] = s2;
+    ek *= s2 * s2;
+  }
+  const double* Q = nhc + 2;
+  const double* xi = Q + M;
+  const double* w = xi + M;
+  double t = 0.5 * Q[0] * w[0] * w[0];
+  for (int j = 1; j < M; ++j) t = t + 0.5 * Q[j] * w[j] * w[j];
+  double u = p.ndof * p.kT * xi[0];
+  for (int j = 1; j < M; ++j) u = u + p.kT * xi[j];
+  double pp = 0.0;
+  for (int k = 0; k < p.nb; ++k) pp += pg[k] * pg[k];
+  const double hb = 0.5 * p.Qb1 * np[5] * np[5] + 0.5 * p.Qb * np[6] * np[6] +
+                    p.nb * p.kT * np[3] + p.kT * np[4];
+  const int step = counters[0];
+  e_pot[step] = es[0];
+  e_kin[step] = ek;
+  e_cons[step] = (ek + es[0] + p.P_ext * V + 0.5 * pp / p.W + (t + u) + hb) - np[kNpHeat];
+  h_vol[step] = V;
+  h_press[step] = P;
+  counters[0] = step + 1;
+}
+
+// Drift of the flexible-cell step (_scale_step_g :2658-2692): v = v d +
+// dt/2 a; x += dt v, wrapped into the current cell; x @ expm(G dt);
+// v @ expm(-(G + Tr G / n_dof) dt); and the block's largest squared
+// non-affine displacement, x - x_build inv(L_b) L_new minimum-imaged in the
+// new cell (engine/neighborlist.py:646-668).
+__global__ __launch_bounds__(256) void k_nptg_drift(int n, double* __restrict__ pos,
+                                                    double* __restrict__ vel,
+                                                    const double* __restrict__ acc, double dt,
+                                                    const Geom* __restrict__ geom,
+                                                    const double* __restrict__ ref,
+                                                    const double* __restrict__ nhc,
+                                                    const double* __restrict__ np,
+                                                    double* __restrict__ part) {
+  __shared__ double m[256];
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  double d2 = 0.0;
+  if (a < n) {
+    const double d = nhc[1];
+    const double* Mx = np + kNpMx;
+    const double* Mv = np + kNpMv;
+    const double* Mp = np + kNpMap;
+    double x[3], v[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      v[q] = vel[3 * a + q] * d + 0.5 * dt * acc[3 * a + q];
+      x[q] = pos[3 * a + q] + dt * v[q];
+    }
+    double f[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      double sb = x[0] * geom->inv[b] + x[1] * geom->inv[3 + b] + x[2] * geom->inv[6 + b];
+      sb -= floor(sb);
+      f[b] = sb >= 1.0 ? 0.0 : sb;
+    }
+    double xw[3], y[3], r[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+      xw[b] = f[0] * geom->lat[b] + f[1] * geom->lat[3 + b] + f[2] * geom->lat[6 + b];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      y[b] = xw[0] * Mx[b] + xw[1] * Mx[3 + b] + xw[2] * Mx[6 + b];
+      pos[3 * a + b] = y[b];
+      vel[3 * a + b] = v[0] * Mv[b] + v[1] * Mv[3 + b] + v[2] * Mv[6 + b];
+      r[b] = ref[3 * a] * Mp[b] + ref[3 * a + 1] * Mp[3 + b] + ref[3 * a + 2] * Mp[6 + b];
+    }
+    double dx, dy, dz;
+    mic_exact(np + kNpLat, np + kNpInv, r[0], r[1], r[2], y[0], y[1], y[2], dx, dy, dz);
+    d2 = dx * dx + dy * dy + dz * dz;
+  }
+  m[threadIdx.x] = d2;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) m[threadIdx.x] = fmax(m[threadIdx.x], m[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = m[0];
+}
+
+// One block: the new cell becomes the set's cell; the list test of
+// check_and_rebuild with a cell change (engine/neighborlist.py:613-673):
+// sigma_min(inv(L_b) L) (1 when L is allclose to L_b: |L - L_b| <= 1e-12 +
+// 1e-5 |L_b| elementwise), rebuild when sigma_min r_list - 2 d_max <= r_cut,
+// a rebuild recording L_b = L.  A cell whose perpendicular widths leave the
+// cell grid or the minimum image is counted in np[9].
+__global__ __launch_bounds__(256) void k_nptg_flag(int nb, const double* __restrict__ disp_part,
+                                                   double r_list, double r_cut,
+                                                   Geom* __restrict__ geom,
+                                                   double* __restrict__ np,
+                                                   int* __restrict__ flag,
+                                                   int* __restrict__ counters) {
+  __shared__ double m[256];
+  double d = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) d = fmax(d, disp_part[b]);
+  m[threadIdx.x] = d;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) m[threadIdx.x] = fmax(m[threadIdx.x], m[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const double* L = np + kNpLat;
+  double* Lb = np + kNpLb;
+  for (int k = 0; k < 9; ++k) {
+    geom->lat[k] = L[k];
+    geom->inv[k] = np[kNpInv + k];
+  }
+  geom->volume = np[kNpVol];
+  bool close = true;
+  for (int k = 0; k < 9; ++k) close = close && fabs(L[k] - Lb[k]) <= 1e-12 + 1e-5 * fabs(Lb[k]);
+  double sig = 1.0;
+  if (!close) {
+    const double* Mp = np + kNpMap;
+    double Mt[9], B[9], w[3], U[9];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) Mt[3 * i + j] = Mp[3 * j + i];
+    mat3_mul(Mt, Mp, B);
+    sym3_eig(B, w, U);
+    sig = sqrt(fmax(fmin(fmin(w[0], w[1]), w[2]), 0.0));
+  }
+  const int fl = (sig * r_list - 2.0 * sqrt(m[0]) <= r_cut) ? 1 : 0;
+  if (fl) {
+    for (int k = 0; k < 9; ++k) {
+      Lb[k] = L[k];
+      np[kNpLbInv + k] = np[kNpInv + k];
+    }
+  }
+  // perpendicular widths V / |a_j x a_k| against the grid and the minimum image
+  double wmin = 1e300;
+  for (int ax = 0; ax < 3; ++ax) {
+    const double* u = L + 3 * ((ax + 1) % 3);
+    const double* q = L + 3 * ((ax + 2) % 3);
+    const double cx = u[1] * q[2] - u[2] * q[1], cy = u[2] * q[0] - u[0] * q[2],
+                 cz = u[0] * q[1] - u[1] * q[0];
+    const double wd = np[kNpVol] / sqrt(cx * cx + cy * cy + cz * cz);
+    wmin = fmin(wmin, wd);
+    if (wd / geom->nc[ax] < r_list) np[9] += 1.0;
+  }
+  if (r_list >= 0.5 * wmin) np[9] += 1.0;
+  *flag = fl;
+  counters[1] += fl;
+}
+
 // v *= scale[0] (the pending thermostat scale at the end of a run).
 __global__ __launch_bounds__(256) void k_scale_vel(int n, double* __restrict__ vel,
                                                    const double* __restrict__ scale) {
@@ -708,15 +1063,29 @@ struct MDState {
   DevBuf pos, vel, acc, F, mass, spc, es, ke, hist_ep, hist_ek, hist_ec, counters, flag;
   DevBuf nhc, esite, flux_part, hist_J, ke_b, ke_a, p_part, np, hist_V, hist_P;
   // thermostat: 0 none (NVE), 1 Nose-Hoover chain, 2 Langevin, 3 CSVR,
-  // 4 NPT (isotropic barostat + Nose-Hoover chain)
-  int thermo = 0, M = 0, every = 0;
+  // 4 NPT (barostat + Nose-Hoover chain); cell: 0 iso, else a flexible cell
+  // whose kicks also sum the kinetic tensor
+  int thermo = 0, M = 0, every = 0, cell = 0;
   double kT = 0.0, ndof = 0.0, damping = 0.0, inv_mass = 0.0;
   uint32_t k0 = 0, k1 = 0;
   NptParams npt{};
   Geom* geom = nullptr;  // the set's device cell
   // velocity-scaling thermostats keep a pending scale in nhc[0]
   bool scaled() const { return thermo == 1 || thermo == 3 || thermo == 4; }
+  bool flex() const { return thermo == 4 && cell != 0; }
 };
+
+static void kick(MDState& S, int n, double dt, int do_kick, hipStream_t st) {
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  if (S.flex())
+    hipLaunchKernelGGL(k_md_kick<7>, dim3(nb), dim3(256), 0, st, n, S.vel.as<double>(),
+                       S.acc.as<double>(), (const double*)S.F.as<double>(),
+                       (const double*)S.mass.as<double>(), dt, do_kick, S.ke.as<double>());
+  else
+    hipLaunchKernelGGL(k_md_kick<1>, dim3(nb), dim3(256), 0, st, n, S.vel.as<double>(),
+                       S.acc.as<double>(), (const double*)S.F.as<double>(),
+                       (const double*)S.mass.as<double>(), dt, do_kick, S.ke.as<double>());
+}
 
 static void thermo_kernel(MDState& S, int mode, double dt, hipStream_t st, unsigned nb) {
   if (S.thermo == 1)
@@ -730,6 +1099,13 @@ static void thermo_kernel(MDState& S, int mode, double dt, hipStream_t st, unsig
                        exp(-0.5 * dt / S.damping), S.ndof, S.kT, S.k0, S.k1, mode,
                        (const double*)S.es.as<double>(), S.counters.as<int>(),
                        S.hist_ep.as<double>(), S.hist_ek.as<double>(), S.hist_ec.as<double>());
+  else if (S.flex())
+    hipLaunchKernelGGL(k_nptg, dim3(1), dim3(256), 0, st, (int)nb,
+                       (const double*)S.ke.as<double>(), S.nhc.as<double>(), S.M,
+                       S.np.as<double>(), S.npt, dt, mode, (const double*)S.es.as<double>(),
+                       (const Geom*)S.geom, S.counters.as<int>(), S.hist_ep.as<double>(),
+                       S.hist_ek.as<double>(), S.hist_ec.as<double>(), S.hist_V.as<double>(),
+                       S.hist_P.as<double>());
   else if (S.thermo == 4)
     hipLaunchKernelGGL(k_npt, dim3(1), dim3(256), 0, st, (int)nb,
                        (const double*)S.ke.as<double>(), S.nhc.as<double>(), S.M,
@@ -768,6 +1144,12 @@ static void md_step(Potential& pot, NList& nl, SetDev& set, MDState& S, int n, d
                        S.vel.as<double>(), (const double*)S.mass.as<double>(), dt, set.geoms[0],
                        (const double*)nl.ref_pos.as<double>(), (const double*)S.nhc.as<double>(),
                        nl.disp_part.as<double>(), S.ke_a.as<double>());
+  } else if (S.flex()) {
+    hipLaunchKernelGGL(k_nptg_drift, dim3(nb), dim3(256), 0, st, n, S.pos.as<double>(),
+                       S.vel.as<double>(), (const double*)S.acc.as<double>(), dt,
+                       (const Geom*)S.geom, (const double*)nl.ref_pos.as<double>(),
+                       (const double*)S.nhc.as<double>(), (const double*)S.np.as<double>(),
+                       nl.disp_part.as<double>());
   } else if (S.thermo == 4) {
     hipLaunchKernelGGL(k_npt_drift, dim3(nb), dim3(256), 0, st, n, S.pos.as<double>(),
                        S.vel.as<double>(), (const double*)S.acc.as<double>(), dt,
@@ -780,7 +1162,11 @@ static void md_step(Potential& pot, NList& nl, SetDev& set, MDState& S, int n, d
                        (const double*)nl.ref_pos.as<double>(), nl.disp_part.as<double>(),
                        (const double*)(S.scaled() ? S.nhc.as<double>() + 1 : nullptr));
   }
-  if (S.thermo == 4)
+  if (S.flex())
+    hipLaunchKernelGGL(k_nptg_flag, dim3(1), dim3(256), 0, st, (int)nb,
+                       (const double*)nl.disp_part.as<double>(), nl.r_list, nl.r_cut, S.geom,
+                       S.np.as<double>(), S.flag.as<int>(), S.counters.as<int>());
+  else if (S.thermo == 4)
     hipLaunchKernelGGL(k_npt_flag, dim3(1), dim3(256), 0, st, (int)nb,
                        (const double*)nl.disp_part.as<double>(), nl.r_list, nl.r_cut,
                        S.npt.s_min, S.geom, S.np.as<double>(), S.flag.as<int>(),
@@ -799,9 +1185,7 @@ static void md_step(Potential& pot, NList& nl, SetDev& set, MDState& S, int n, d
   else
     compute_forces(pot, nl, set, S.pos.as<double>(), S.spc.as<int>(), S.F.as<double>(),
                    S.es.as<double>(), st);
-  hipLaunchKernelGGL(k_md_kick, dim3(nb), dim3(256), 0, st, n, S.vel.as<double>(),
-                     S.acc.as<double>(), (const double*)S.F.as<double>(),
-                     (const double*)S.mass.as<double>(), dt, 1, S.ke.as<double>());
+  kick(S, n, dt, 1, st);
   if (S.thermo) thermo_kernel(S, S.thermo == 2 ? 1 : 2, dt, st, nb);
   if (S.every > 0)
     heat_flux(pot, nl, set, S.pos.as<double>(), S.vel.as<double>(), S.mass.as<double>(),
@@ -830,8 +1214,10 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
   AK_REQUIRE(opt.thermostat >= 0 && opt.thermostat <= 4, AK_ERR_ARG,
              "thermostat must be 0 (NVE), 1 (Nose-Hoover chain), 2 (Langevin), 3 (CSVR) or "
              "4 (NPT)");
-  AK_REQUIRE(opt.cell_dof == 0 && opt.baro_thermostat == 0, AK_ERR_ARG,
-             "NPT: only cell_dof iso with the Nose-Hoover barostat chain is available");
+  AK_REQUIRE(opt.cell_dof >= 0 && opt.cell_dof <= 5, AK_ERR_ARG,
+             "NPT cell_dof must be 0 (iso), 1 (aniso), 2 (tri), 3 (xy), 4 (xy_iso) or 5 (z)");
+  AK_REQUIRE(opt.baro_thermostat == 0 || opt.baro_thermostat == 1, AK_ERR_ARG,
+             "NPT baro_thermostat must be 0 (Nose-Hoover chain) or 1 (Langevin)");
   if (state)
     for (int k = 0; k < kMdStateLen; ++k)
       AK_REQUIRE(std::isfinite(state[k]), AK_ERR_ARG, "thermostat state must be finite");
@@ -851,6 +1237,7 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
   MDState S;
   S.geom = set.d_geoms.as<Geom>();
   S.thermo = opt.thermostat;
+  S.cell = opt.thermostat == 4 ? opt.cell_dof : 0;
   S.every = opt.flux_every;
   S.pos.ensure(sizeof(double) * 3 * n);
   S.vel.ensure(sizeof(double) * 3 * n);
@@ -860,7 +1247,7 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
   S.spc.ensure(sizeof(int) * n);
   S.es.ensure(sizeof(double) * 10);
   const unsigned nb = (unsigned)((n + 255) / 256);
-  S.ke.ensure(sizeof(double) * nb);
+  S.ke.ensure(sizeof(double) * (S.flex() ? 7 : 1) * nb);
   S.hist_ep.ensure(sizeof(double) * (n_steps + 1));
   S.hist_ek.ensure(sizeof(double) * (n_steps + 1));
   S.hist_ec.ensure(sizeof(double) * (n_steps + 1));
@@ -909,16 +1296,48 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
       for (int ax = 0; ax < 3; ++ax)
         s_min = std::max(s_min, nl.r_list * g.nc[ax] / ci.perp[ax]);
       S.npt.s_min = s_min;
-      std::vector<double> q(16, 0.0);
+      // flexible cell: the strain generators of _cell_generators (:2113-2138)
+      // as xx, yy, zz, yz, xz, xy; iso keeps one momentum p_eps
+      static const double kGen[6][6] = {{1, 0, 0, 0, 0, 0}, {0, 1, 0, 0, 0, 0},
+                                        {0, 0, 1, 0, 0, 0}, {0, 0, 0, 1, 0, 0},
+                                        {0, 0, 0, 0, 1, 0}, {0, 0, 0, 0, 0, 1}};
+      static const double kXyIso[6] = {1, 1, 0, 0, 0, 0};
+      int ng = 1;
+      const double* gsrc[6] = {kGen[0], kGen[1], kGen[2], kGen[3], kGen[4], kGen[5]};
+      switch (opt.cell_dof) {
+        case 1: ng = 3; break;                                   // aniso: xx, yy, zz
+        case 2: ng = 6; break;                                   // tri: + yz, xz, xy
+        case 3: ng = 2; break;                                   // xy: xx, yy
+        case 4: ng = 1; gsrc[0] = kXyIso; break;                 // xy_iso: xx + yy
+        case 5: ng = 1; gsrc[0] = kGen[2]; break;                // z: zz
+        default: ng = 1; break;                                  // iso
+      }
+      S.npt.nb = ng;
+      for (int k = 0; k < 6; ++k)
+        for (int c = 0; c < 6; ++c) S.npt.gen[k][c] = k < ng ? gsrc[k][c] : 0.0;
+      for (int k = 0; k < 6; ++k)
+        S.npt.gtr[k] = k < ng ? S.npt.gen[k][0] + S.npt.gen[k][1] + S.npt.gen[k][2] : 0.0;
+      S.npt.Qb1 = (opt.cell_dof == 0 ? 1.0 : (double)ng) * S.npt.Qb;
+      S.npt.lv = opt.baro_thermostat == 1 ? 1 : 0;
+      S.npt.tau_b = opt.tau_baro;
+      S.npt.k0 = (uint32_t)(opt.seed & 0xffffffffu);
+      S.npt.k1 = (uint32_t)(opt.seed >> 32);
+      std::vector<double> q(kNpLen, 0.0);
       q[0] = q[1] = q[7] = q[8] = 1.0;
       q[2] = st_at(kStPeps);
       q[3] = st_at(kStXiB);
       q[4] = st_at(kStXiB + 1);
       q[5] = st_at(kStVxiB);
       q[6] = st_at(kStVxiB + 1);
+      q[kNpHeat] = st_at(kStHeat);
+      for (int k = 0; k < 6; ++k) q[kNpPg + k] = k < ng && opt.cell_dof != 0 ? st_at(kStPg + k)
+                                                                             : 0.0;
+      for (int k = 0; k < 9; ++k) {
+        q[kNpLb + k] = lattice[k];
+        q[kNpLbInv + k] = inv[k];
+      }
       S.np.ensure(sizeof(double) * q.size());
-      AK_HIP(hipMemcpyAsync(S.np.p, q.data(), sizeof(double) * q.size(), hipMemcpyHostToDevice,
-                            st));
+      AK_HIP(hipMemcpy(S.np.p, q.data(), sizeof(double) * q.size(), hipMemcpyHostToDevice));
       S.hist_V.ensure(sizeof(double) * (n_steps + 1));
       S.hist_P.ensure(sizeof(double) * (n_steps + 1));
     }
@@ -973,9 +1392,7 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
   // initial forces: acceleration and the kinetic-energy sums of state 0
   compute_forces(pot, nl, set, S.pos.as<double>(), S.spc.as<int>(), S.F.as<double>(),
                  S.es.as<double>(), st);
-  hipLaunchKernelGGL(k_md_kick, dim3(nb), dim3(256), 0, st, n, S.vel.as<double>(),
-                     S.acc.as<double>(), (const double*)S.F.as<double>(),
-                     (const double*)S.mass.as<double>(), dt, 0, S.ke.as<double>());
+  kick(S, n, dt, 0, st);
   AK_HIP(hipMemsetAsync(nl.stats.p, 0, sizeof(int) * 8, st));
   // make sure every scratch buffer a step touches exists before capture: a
   // step with dt = 0 leaves positions, velocities, the chain and the state-0
@@ -1040,7 +1457,7 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
     state[kStRng] = (double)counters[3];
   }
   if (S.thermo == 4) {
-    double q[16];
+    double q[kNpLen];
     AK_HIP(hipMemcpy(q, S.np.p, sizeof(q), hipMemcpyDeviceToHost));
     if (state) {
       state[kStPeps] = q[2];
@@ -1048,10 +1465,16 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
       state[kStXiB + 1] = q[4];
       state[kStVxiB] = q[5];
       state[kStVxiB + 1] = q[6];
+      for (int k = 0; k < 6; ++k) state[kStPg + k] = q[kNpPg + k];
+      state[kStHeat] = q[kNpHeat];
+      state[kStRng] = (double)counters[3];
     }
     AK_REQUIRE(q[9] == 0.0, AK_ERR_CAPACITY,
-               "the NPT cell shrank beyond the range its neighbour grid and minimum image "
-               "were set up for (5 % linear); restart from the current cell");
+               S.flex() ? "the NPT cell left the range its neighbour grid and minimum image were "
+                          "set up for (perpendicular widths 5 % below the start); restart from "
+                          "the current cell"
+                        : "the NPT cell shrank beyond the range its neighbour grid and minimum "
+                          "image were set up for (5 % linear); restart from the current cell");
     Geom g;
     AK_HIP(hipMemcpy(&g, S.geom, sizeof(Geom), hipMemcpyDeviceToHost));
     if (lattice_out) std::memcpy(lattice_out, g.lat, sizeof(g.lat));

@@ -23,23 +23,27 @@ K_B = 1.380649e-23 * 6.241509074e18                                        # eV/
 EV_ANG3_TO_GPA = 160.2176621
 EV_ANG_TO_AMU_ANG_FS2 = (1.60217663e-19 / 1.0e-10) / (1.66053906660e-27) * 1.0e-20
 
+# cell degrees of freedom of NPT (CELL_DOF_MODES, solvers/molecular_dynamics.py:2113)
+CELL_DOF_MODES = ("iso", "aniso", "tri", "xy", "xy_iso", "z")
+
 # thermostat state carried between device runs (AK_MD_STATE_LEN doubles)
 STATE_LEN = 48
 _ST = {"xi": (0, 16), "vxi": (16, 32), "p_eps": (32, 33), "xi_b": (33, 35), "vxi_b": (35, 37),
-       "heat": (43, 44), "rng_draws": (44, 45)}
+       "p_g": (37, 43), "heat": (43, 44), "rng_draws": (44, 45)}
 
 
-def thermostat_state(xi=(), vxi=(), p_eps=0.0, xi_b=(0.0, 0.0), vxi_b=(0.0, 0.0), heat=0.0,
-                     rng_draws=0):
+def thermostat_state(xi=(), vxi=(), p_eps=0.0, xi_b=(0.0, 0.0), vxi_b=(0.0, 0.0), p_g=(),
+                     heat=0.0, rng_draws=0):
     """A state array for ``md_run(state=...)`` from the quantities a
     reference checkpoint carries (NVTSimulation xi_nhc / vxi_nhc,
-    NPTSimulation p_eps, xi1_b, xi2_b, vxi1_b, vxi2_b;
-    solvers/molecular_dynamics.py:823-830, 2228): chain positions and
-    velocities (up to 16 links), the barostat momentum and its chain, the
-    heat a stochastic thermostat added and the random draws used."""
+    NPTSimulation p_eps, xi1_b, xi2_b, vxi1_b, vxi2_b, p_g, baro_heat;
+    solvers/molecular_dynamics.py:823-830, 2228, 2557-2564): chain positions
+    and velocities (up to 16 links), the barostat momentum (iso) or the
+    strain momenta of a flexible cell and the barostat chain, the heat a
+    stochastic thermostat or barostat added and the random draws used."""
     st = np.zeros(STATE_LEN)
     for key, val in (("xi", xi), ("vxi", vxi), ("p_eps", p_eps), ("xi_b", xi_b),
-                     ("vxi_b", vxi_b), ("heat", heat), ("rng_draws", rng_draws)):
+                     ("vxi_b", vxi_b), ("p_g", p_g), ("heat", heat), ("rng_draws", rng_draws)):
         a, b = _ST[key]
         v = np.atleast_1d(np.asarray(val, float))
         if len(v) > b - a:
@@ -88,7 +92,8 @@ def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_step
     """n_steps of velocity Verlet on the device (``ak_md_run``).  thermostat:
     None (NVE), ("nhc", temperature K, tau fs, chain links), ("langevin",
     temperature, damping fs, seed), ("csvr", temperature, damping, seed) or
-    ("npt", temperature, tau_nhc, chain links, pressure GPa, tau_baro fs).  Returns a
+    ("npt", temperature, tau_nhc, chain links, pressure GPa, tau_baro fs[,
+    cell_dof, baro_thermostat, seed]).  Returns a
     dict with the final positions and velocities, e_pot, e_kin and e_cons
     (E_tot, or H_NHC with a chain) per step (n_steps + 1, index 0 the
     initial state), the number of list rebuilds, the loop's wall time and,
@@ -121,11 +126,18 @@ def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_step
             th = np.array([2.0 if kind == "langevin" else 3.0, float(T), float(damping), 0.0,
                            float(int(seed)), 0.0, 0.0, 0.0, 0.0])
         elif kind == "npt":
-            _, T, tau, chain, p_gpa, tau_b = thermostat
-            th = np.array([4.0, float(T), float(tau), float(chain), 0.0, float(p_gpa),
-                           float(tau_b), 0.0, 0.0])
+            _, T, tau, chain, p_gpa, tau_b, *rest = thermostat
+            cell, baro, seed = (list(rest) + ["iso", "nhc", 0][len(rest):])[:3]
+            if cell not in CELL_DOF_MODES:
+                raise ValueError(f"NPT: cell_dof must be one of {' | '.join(CELL_DOF_MODES)}, "
+                                 f"got {cell!r}.")
+            if baro not in ("nhc", "langevin"):
+                raise ValueError(f"NPT: baro_thermostat must be nhc | langevin, got {baro!r}.")
+            th = np.array([4.0, float(T), float(tau), float(chain), float(int(seed)),
+                           float(p_gpa), float(tau_b), float(CELL_DOF_MODES.index(cell)),
+                           1.0 if baro == "langevin" else 0.0])
         else:
-            raise ValueError(f"thermostat: unknown kind {kind!r} (nhc, langevin, csvr)")
+            raise ValueError(f"thermostat: unknown kind {kind!r} (nhc, langevin, csvr, npt)")
     lat_out = np.zeros((3, 3))
     vol = np.zeros(n_steps + 1)
     prs = np.zeros(n_steps + 1)
@@ -209,16 +221,23 @@ def csvr_run(potential, lattice, positions, velocities, atom_types, temperature,
 
 def npt_run(potential, lattice, positions, velocities, atom_types, temperature, pressure=0.0,
             dt=1.0, n_steps=1000, tau_nhc=100.0, nhc_chain=3, tau_baro=1000.0, skin_width=0.5,
-            masses=None, use_graph=True, state=None):
-    """NPT with the isotropic MTTK barostat (a 2-link chain on its momentum)
-    and the particle Nose-Hoover chain, on the device (NPTSimulation with
-    cell_dof "iso", baro_thermostat "nhc", solvers/molecular_dynamics.py:
-    2141-2555; defaults tau_nhc 100 fs, 3 links, tau_baro 1000 fs, skin
-    0.5 A).  pressure in GPa.  Returns md_run's dict with the final
-    "lattice", "volume" [A^3] and "pressure_gpa" (P_inst) per step, e_cons =
-    H_NPT and T.  The cell may shrink by up to 5 % linear in one call."""
+            masses=None, use_graph=True, state=None, cell_dof="iso", baro_thermostat="nhc",
+            seed=42):
+    """NPT with the MTTK barostat and the particle Nose-Hoover chain, on the
+    device (NPTSimulation, solvers/molecular_dynamics.py:2141-2692; defaults
+    tau_nhc 100 fs, 3 links, tau_baro 1000 fs, cell_dof "iso",
+    baro_thermostat "nhc", skin 0.5 A).  cell_dof: "iso" (volume only),
+    "aniso", "tri", "xy", "xy_iso" or "z" (the strain generators of
+    ADR-0036); baro_thermostat: "nhc" (a 2-link chain on the barostat
+    momenta) or "langevin" (an Ornstein-Uhlenbeck step with friction time
+    tau_baro, noise from the Philox stream keyed by ``seed``; its heat is
+    subtracted from H_NPT).  pressure in GPa.  Returns md_run's dict with
+    the final "lattice", "volume" [A^3] and "pressure_gpa" (P_inst) per
+    step, e_cons = H_NPT and T.  A perpendicular cell width may shrink by up
+    to 5 % in one call."""
     out = md_run(potential, lattice, positions, velocities, atom_types, dt, n_steps,
                  skin_width, masses, use_graph,
-                 ("npt", temperature, tau_nhc, nhc_chain, pressure, tau_baro), 0, state)
+                 ("npt", temperature, tau_nhc, nhc_chain, pressure, tau_baro, cell_dof,
+                  baro_thermostat, seed), 0, state)
     out["T"] = 2.0 * out["e_kin"] / (max(3 * len(positions) - 3, 1) * K_B)
     return out

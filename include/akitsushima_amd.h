@@ -222,17 +222,22 @@ int ak_md_nve(ak_potential *pot, int32_t n_atoms, const double *lattice,
  * seed, pressure GPa, tau_baro fs, cell_dof, baro_thermostat}: kind 1 =
  * Nose-Hoover chain (tau_nhc, 2-16 links), 2 = Langevin, 3 = CSVR (tau =
  * damping; seed an integer < 2^53 keys the Philox4x32-10 stream), 4 = NPT
- * (the chain of kind 1 plus the barostat; the cell may shrink by up to 5 %
- * linear; cell_dof 0 = iso, baro_thermostat 0 = Nose-Hoover chain);
- * n_dof = 3N - 3.
+ * (the chain of kind 1 plus the barostat; a perpendicular cell width may
+ * shrink by up to 5 %); n_dof = 3N - 3.  cell_dof (CELL_DOF_MODES,
+ * :2113-2138): 0 iso, 1 aniso, 2 tri, 3 xy, 4 xy_iso, 5 z (the MTTK
+ * flexible cell restricted to those strain generators, :2566-2692);
+ * baro_thermostat: 0 the 2-link Nose-Hoover chain on the barostat momenta,
+ * 1 Langevin (:2596-2617; noise keyed by seed, its heat subtracted from
+ * H_NPT).
  * state (AK_MD_STATE_LEN doubles, may be NULL = start from rest) carries the
  * thermostat between calls, as the reference's checkpoint scalars do
  * (_ck_arrays / _CK_SCALARS, :431-445, 823-830, 2228, 2557-2564): read at
- * the start, the final state on return.  Layout: [0, 16) chain positions xi, [16, 32) chain
- * velocities vxi, 32 p_eps, [33, 35) barostat chain xi_b, [35, 37) vxi_b,
- * [37, 43) reserved, 43 the heat Q a stochastic thermostat has added, 44
- * the random draws used so far (a continued run draws new numbers), rest
- * reserved (pass zeros).
+ * the start, the final state on return.  Layout: [0, 16) chain positions
+ * xi, [16, 32) chain velocities vxi, 32 p_eps (iso), [33, 35) barostat
+ * chain xi_b, [35, 37) vxi_b, [37, 43) the strain momenta p_k of a flexible
+ * cell (in generator order), 43 the heat Q a stochastic thermostat or the
+ * Langevin barostat has added, 44 the random draws used so far (a
+ * continued run draws new numbers), rest reserved (pass zeros).
  * e_cons (n_steps + 1): E_tot (NVE), H_NHC (:947-969), E_tot - Q, Q the heat
  * a stochastic thermostat added (the E-Q column), or H_NPT (:2425-2454).
  * lattice_out (9): the final cell; volume, pressure (n_steps + 1): V [A^3]
