@@ -31,6 +31,7 @@ _vp = ctypes.c_void_p
 _pi32 = ctypes.POINTER(ctypes.c_int32)
 _pi64 = ctypes.POINTER(ctypes.c_int64)
 _pf64 = ctypes.POINTER(ctypes.c_double)
+_pu8 = ctypes.POINTER(ctypes.c_uint8)
 _handle = ctypes.c_void_p
 _phandle = ctypes.POINTER(ctypes.c_void_p)
 
@@ -68,8 +69,8 @@ SIGNATURES = {
     "ak_md_nve": (ctypes.c_int, [_handle, _i32, _pf64, _pf64, _pf64, _pf64, _pf64, _pi32, _f64,
                                  _i32, _f64, _i32, _pf64, _pf64, _pi32, _pf64]),
     "ak_md_run": (ctypes.c_int, [_handle, _i32, _pf64, _pf64, _pf64, _pf64, _pf64, _pi32, _f64,
-                                 _i32, _f64, _i32, _pf64, _pf64, _i32, _pf64, _pf64, _pf64, _pf64,
-                                 _pf64, _pf64, _pf64, _pi32, _pf64]),
+                                 _i32, _f64, _i32, _pf64, _pf64, _pu8, _i32, _pf64, _pf64, _pf64,
+                                 _pf64, _pf64, _pf64, _pf64, _pi32, _pf64]),
     "ak_site_terms": (ctypes.c_int, [_handle, _handle, _i32, _pf64, _pf64, _pf64, _pi32, _pf64,
                                      _i64, _pi32, _pi32, _pf64, _pf64,
                                      ctypes.POINTER(ctypes.c_int64)]),

@@ -871,9 +871,10 @@ extern "C" {
 int ak_md_run(ak_potential* P, int32_t n, const double* lattice, const double* inv,
               double* positions, double* velocities, const double* masses,
               const int32_t* species, double dt, int32_t n_steps, double skin,
-              int32_t use_graph, const double* thermostat, double* state, int32_t flux_every,
-              double* e_pot, double* e_kin, double* e_cons, double* flux, double* lattice_out,
-              double* volume, double* pressure, int32_t* n_rebuilds, double* seconds) {
+              int32_t use_graph, const double* thermostat, double* state,
+              const uint8_t* fix_mask, int32_t flux_every, double* e_pot, double* e_kin,
+              double* e_cons, double* flux, double* lattice_out, double* volume,
+              double* pressure, int32_t* n_rebuilds, double* seconds) {
   return guarded([&] {
     AK_REQUIRE(P && lattice && positions && velocities && masses && n > 0, AK_ERR_ARG,
                "bad arguments");
@@ -910,7 +911,12 @@ int ak_md_run(ak_potential* P, int32_t n, const double* lattice, const double* i
                        thermostat[8] == (double)opt.baro_thermostat,
                    AK_ERR_ARG, "bad NPT cell_dof or baro_thermostat code");
       }
+      AK_REQUIRE(thermostat[9] >= 0.0 && thermostat[9] < 2147483647.0 &&
+                     thermostat[9] == std::floor(thermostat[9]),
+                 AK_ERR_ARG, "velocity_rescale_steps must be an integer >= 0");
+      opt.rescale_steps = (int)thermostat[9];
     }
+    opt.fixed = fix_mask;
     opt.flux_every = flux_every;
     int nr = 0;
     md_run(P->pot, n, lattice, inv, positions, velocities, masses, species, dt, n_steps, skin,

@@ -230,15 +230,18 @@ struct MDOptions {
   int cell_dof = 0;         // NPT cell degrees of freedom: 0 iso
   int baro_thermostat = 0;  // NPT barostat thermostat: 0 Nose-Hoover chain
   int flux_every = 0;
+  int rescale_steps = 0;            // velocity_rescale_steps (chain and NPT)
+  const uint8_t* fixed = nullptr;   // fix_atoms: 3N flags, 1 = frozen component
 };
 
 // Thermostat / barostat state carried between runs (in and out of md_run):
 // the particle chain xi[16], vxi[16]; the isotropic barostat momentum p_eps,
 // its chain xi_b[2], vxi_b[2]; six strain momenta (reserved); the heat the
-// stochastic thermostat put in; the count of random draws already used.
+// stochastic thermostat put in; the count of random draws already used; the
+// steps already run (for velocity_rescale_steps).
 constexpr int kMdStateLen = 48;
 constexpr int kStXi = 0, kStVxi = 16, kStPeps = 32, kStXiB = 33, kStVxiB = 35, kStPg = 37,
-              kStHeat = 43, kStRng = 44;
+              kStHeat = 43, kStRng = 44, kStSteps = 45;
 
 // eV/A^3 -> GPa (constants.py EV_ANG3_TO_GPA)
 constexpr double kEvAng3ToGpa = 160.2176621;

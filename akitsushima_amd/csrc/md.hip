@@ -27,12 +27,15 @@ constexpr double kConv = kConvAmu;
 //   v += dt/2 a; x += dt v; frac = x @ inv; frac -= floor(frac);
 //   frac[frac >= 1] = 0; x = frac @ lat.
 // vscale (nullable): the thermostat's velocity scale, applied first.
+// keep (nullable, 3N of 0 or 1): frozen components (fix_atoms) are zeroed
+// after the kick, as `velocities *= ~fix_mask` does.
 __global__ __launch_bounds__(256) void k_md_drift(int n, double* __restrict__ pos,
                                                   double* __restrict__ vel,
                                                   const double* __restrict__ acc, double dt,
                                                   Geom g, const double* __restrict__ ref,
                                                   double* __restrict__ part,
-                                                  const double* __restrict__ vscale) {
+                                                  const double* __restrict__ vscale,
+                                                  const double* __restrict__ keep) {
   __shared__ double m[256];
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   double d2 = 0.0;
@@ -43,6 +46,7 @@ __global__ __launch_bounds__(256) void k_md_drift(int n, double* __restrict__ po
     for (int d = 0; d < 3; ++d) {
       const double v0 = vscale ? vel[3 * a + d] * sc : vel[3 * a + d];
       double v = v0 + 0.5 * dt * acc[3 * a + d];
+      if (keep) v *= keep[3 * a + d];
       vel[3 * a + d] = v;
       x[d] = pos[3 * a + d] + dt * v;
     }
@@ -81,7 +85,8 @@ __global__ __launch_bounds__(256) void k_md_kick(int n, double* __restrict__ vel
                                                  double* __restrict__ acc,
                                                  const double* __restrict__ F,
                                                  const double* __restrict__ mass, double dt,
-                                                 int kick, double* __restrict__ ke_part) {
+                                                 int kick, double* __restrict__ ke_part,
+                                                 const double* __restrict__ keep) {
   __shared__ double smem[16 * NT];
   int a = blockIdx.x * blockDim.x + threadIdx.x;
   double ke[NT];
@@ -98,6 +103,7 @@ __global__ __launch_bounds__(256) void k_md_kick(int n, double* __restrict__ vel
       double v = vel[3 * a + d];
       if (kick) {
         v += 0.5 * dt * ad;
+        if (keep) v *= keep[3 * a + d];
         vel[3 * a + d] = v;
       }
       s += v * v;
@@ -199,6 +205,25 @@ __device__ double nhc_half(double* __restrict__ nhc, int M, double dt, double kT
   return scale;
 }
 
+// velocity_rescale_steps (NVTSimulation.run :1154-1167, NPTSimulation.run
+// :2907-2920): for the steps s < vr_until of this call, after the end-of-step
+// chain half-step, v *= sqrt(T / T_now), T_now = 2 E_kin / (n_dof k_B), when
+// T_now > 1e-12 -- folded into the pending velocity scale nhc[0].  The step
+// is counters[0] - 1 (state 0 is record 0); the set-up step before the
+// record restarts runs at counters[0] = 0 and is never rescaled.
+__device__ __forceinline__ void vel_rescale(double* __restrict__ nhc, double& ek, double ndof,
+                                            int vr_until, double T,
+                                            const int* __restrict__ counters) {
+  const int s = counters[0] - 1;
+  if (s < 0 || s >= vr_until) return;
+  const double T_now = ek > 0.0 ? 2.0 * ek / (ndof * kBoltzmannEv) : 0.0;
+  if (T_now > 1e-12) {
+    const double f = sqrt(T / T_now);
+    nhc[0] *= f;
+    ek *= f * f;
+  }
+}
+
 // One block.  E_kin of the velocities the last kick left = the kick's block
 // sums in fixed order.  mode 1 (start of a step): half-step with E_kin of the
 // velocities after the pending scale; the drift applies pending x new scale.
@@ -208,7 +233,8 @@ __device__ double nhc_half(double* __restrict__ nhc, int M, double dt, double kT
 // records only (the initial state).
 __global__ __launch_bounds__(256) void k_nhc(int nb, const double* __restrict__ ke_part,
                                              double* __restrict__ nhc, int M, double dt,
-                                             double kT, double ndof, int mode,
+                                             double kT, double ndof, int vr_until, double T,
+                                             int mode,
                                              const double* __restrict__ es,
                                              int* __restrict__ counters,
                                              double* __restrict__ e_pot,
@@ -232,6 +258,7 @@ __global__ __launch_bounds__(256) void k_nhc(int nb, const double* __restrict__ 
     const double s = nhc_half(nhc, M, dt, kT, ndof, ek);
     nhc[0] = s;
     ek *= s * s;
+    vel_rescale(nhc, ek, ndof, vr_until, T, counters);
   }
   const double* Q = nhc + 2;
   const double* xi = Q + M;
@@ -380,7 +407,8 @@ __global__ __launch_bounds__(256) void k_lang_bao(int n, double* __restrict__ po
                                                   double c, double kTc, uint32_t k0, uint32_t k1,
                                                   const int* __restrict__ counters,
                                                   double* __restrict__ ke_part,
-                                                  double* __restrict__ p_part) {
+                                                  double* __restrict__ p_part,
+                                                  const double* __restrict__ keep) {
   __shared__ double smem[16 * 4];
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   double r[4] = {0.0, 0.0, 0.0, 0.0};
@@ -390,6 +418,7 @@ __global__ __launch_bounds__(256) void k_lang_bao(int n, double* __restrict__ po
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       v[d] = vel[3 * a + d] + 0.5 * dt * acc[3 * a + d];
+      if (keep) v[d] *= keep[3 * a + d];
       pos[3 * a + d] += 0.5 * dt * v[d];
     }
     r[0] = 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / kConv;
@@ -401,6 +430,7 @@ __global__ __launch_bounds__(256) void k_lang_bao(int n, double* __restrict__ po
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       v[d] = c * v[d] + sig * z[d];
+      if (keep) v[d] *= keep[3 * a + d];
       vel[3 * a + d] = v[d];
       r[1 + d] = m * v[d];
     }
@@ -540,6 +570,9 @@ struct NptParams {
   int lv;
   double tau_b;
   uint32_t k0, k1;
+  // velocity_rescale_steps of this call and the target temperature
+  int vr_until;
+  double T;
 };
 
 // np[] slots beyond the isotropic ones (0-9): 10 the heat the Langevin
@@ -649,6 +682,7 @@ __global__ __launch_bounds__(256) void k_npt(int nb, const double* __restrict__ 
     const double s2 = nhc_half(nhc, M, dt, p.kT, p.ndof, ek);
     nhc[0] = s2;
     ek *= s2 * s2;
+    vel_rescale(nhc, ek, p.ndof, p.vr_until, p.T, counters);
   }
   const double* Q = nhc + 2;
   const double* xi = Q + M;
@@ -680,7 +714,8 @@ __global__ __launch_bounds__(256) void k_npt_drift(int n, double* __restrict__ p
                                                    const double* __restrict__ ref,
                                                    const double* __restrict__ nhc,
                                                    const double* __restrict__ np,
-                                                   double* __restrict__ part) {
+                                                   double* __restrict__ part,
+                                                   const double* __restrict__ keep) {
   __shared__ double m[256];
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   double d2 = 0.0;
@@ -697,6 +732,7 @@ __global__ __launch_bounds__(256) void k_npt_drift(int n, double* __restrict__ p
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       v[q] = vel[3 * a + q] * d + 0.5 * dt * acc[3 * a + q];
+      if (keep) v[q] *= keep[3 * a + q];
       x[q] = pos[3 * a + q] + dt * v[q];
     }
     double f[3];
@@ -909,6 +945,7 @@ __global__ __launch_bounds__(256) void k_nptg(int nb, const double* __restrict__
     const double s2 = nhc_half(nhc, M, dt, p.kT, p.ndof, ek);
     nhc[0] = s2;
     ek *= s2 * s2;
+    vel_rescale(nhc, ek, p.ndof, p.vr_until, p.T, counters);
   }
   const double* Q = nhc + 2;
   const double* xi = Q + M;
@@ -942,7 +979,8 @@ __global__ __launch_bounds__(256) void k_nptg_drift(int n, double* __restrict__ 
                                                     const double* __restrict__ ref,
                                                     const double* __restrict__ nhc,
                                                     const double* __restrict__ np,
-                                                    double* __restrict__ part) {
+                                                    double* __restrict__ part,
+                                                    const double* __restrict__ keep) {
   __shared__ double m[256];
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   double d2 = 0.0;
@@ -955,6 +993,7 @@ __global__ __launch_bounds__(256) void k_nptg_drift(int n, double* __restrict__ 
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       v[q] = vel[3 * a + q] * d + 0.5 * dt * acc[3 * a + q];
+      if (keep) v[q] *= keep[3 * a + q];
       x[q] = pos[3 * a + q] + dt * v[q];
     }
     double f[3];
@@ -1061,13 +1100,19 @@ __global__ __launch_bounds__(256) void k_scale_vel(int n, double* __restrict__ v
 
 struct MDState {
   DevBuf pos, vel, acc, F, mass, spc, es, ke, hist_ep, hist_ek, hist_ec, counters, flag;
-  DevBuf nhc, esite, flux_part, hist_J, ke_b, ke_a, p_part, np, hist_V, hist_P;
+  DevBuf nhc, esite, flux_part, hist_J, ke_b, ke_a, p_part, np, hist_V, hist_P, keep;
   // thermostat: 0 none (NVE), 1 Nose-Hoover chain, 2 Langevin, 3 CSVR,
   // 4 NPT (barostat + Nose-Hoover chain); cell: 0 iso, else a flexible cell
   // whose kicks also sum the kinetic tensor
   int thermo = 0, M = 0, every = 0, cell = 0;
   double kT = 0.0, ndof = 0.0, damping = 0.0, inv_mass = 0.0;
   uint32_t k0 = 0, k1 = 0;
+  // fix_atoms: keep = 1 - frozen per component (null when nothing is frozen)
+  bool fixed = false;
+  // velocity_rescale_steps left for this call, target temperature
+  int vr_until = 0;
+  double T = 0.0;
+  const double* keep_ptr() const { return fixed ? keep.as<double>() : nullptr; }
   NptParams npt{};
   Geom* geom = nullptr;  // the set's device cell
   // velocity-scaling thermostats keep a pending scale in nhc[0]
@@ -1080,18 +1125,21 @@ static void kick(MDState& S, int n, double dt, int do_kick, hipStream_t st) {
   if (S.flex())
     hipLaunchKernelGGL(k_md_kick<7>, dim3(nb), dim3(256), 0, st, n, S.vel.as<double>(),
                        S.acc.as<double>(), (const double*)S.F.as<double>(),
-                       (const double*)S.mass.as<double>(), dt, do_kick, S.ke.as<double>());
+                       (const double*)S.mass.as<double>(), dt, do_kick, S.ke.as<double>(),
+                       S.keep_ptr());
   else
     hipLaunchKernelGGL(k_md_kick<1>, dim3(nb), dim3(256), 0, st, n, S.vel.as<double>(),
                        S.acc.as<double>(), (const double*)S.F.as<double>(),
-                       (const double*)S.mass.as<double>(), dt, do_kick, S.ke.as<double>());
+                       (const double*)S.mass.as<double>(), dt, do_kick, S.ke.as<double>(),
+                       S.keep_ptr());
 }
 
 static void thermo_kernel(MDState& S, int mode, double dt, hipStream_t st, unsigned nb) {
   if (S.thermo == 1)
     hipLaunchKernelGGL(k_nhc, dim3(1), dim3(256), 0, st, (int)nb,
                        (const double*)S.ke.as<double>(), S.nhc.as<double>(), S.M, dt, S.kT,
-                       S.ndof, mode, (const double*)S.es.as<double>(), S.counters.as<int>(),
+                       S.ndof, S.vr_until, S.T, mode, (const double*)S.es.as<double>(),
+                       S.counters.as<int>(),
                        S.hist_ep.as<double>(), S.hist_ek.as<double>(), S.hist_ec.as<double>());
   else if (S.thermo == 3)
     hipLaunchKernelGGL(k_csvr, dim3(1), dim3(256), 0, st, (int)nb,
@@ -1136,10 +1184,12 @@ static void md_step(Potential& pot, NList& nl, SetDev& set, MDState& S, int n, d
                        S.vel.as<double>(), (const double*)S.acc.as<double>(),
                        (const double*)S.mass.as<double>(), dt, exp(-dt / S.damping),
                        S.kT * kConv, S.k0, S.k1, (const int*)S.counters.as<int>(),
-                       S.ke_b.as<double>(), S.p_part.as<double>());
+                       S.ke_b.as<double>(), S.p_part.as<double>(), S.keep_ptr());
+    // with frozen components the centre-of-mass velocity is not removed
+    // (LangevinSimulation._o_step :1552-1565)
     hipLaunchKernelGGL(k_lang_com, dim3(1), dim3(256), 0, st, (int)nb,
-                       (const double*)S.p_part.as<double>(), S.inv_mass, dt > 0.0 ? 1 : 0,
-                       S.nhc.as<double>());
+                       (const double*)S.p_part.as<double>(), S.inv_mass,
+                       (dt > 0.0 && !S.fixed) ? 1 : 0, S.nhc.as<double>());
     hipLaunchKernelGGL(k_lang_a2, dim3(nb), dim3(256), 0, st, n, S.pos.as<double>(),
                        S.vel.as<double>(), (const double*)S.mass.as<double>(), dt, set.geoms[0],
                        (const double*)nl.ref_pos.as<double>(), (const double*)S.nhc.as<double>(),
@@ -1149,18 +1199,19 @@ static void md_step(Potential& pot, NList& nl, SetDev& set, MDState& S, int n, d
                        S.vel.as<double>(), (const double*)S.acc.as<double>(), dt,
                        (const Geom*)S.geom, (const double*)nl.ref_pos.as<double>(),
                        (const double*)S.nhc.as<double>(), (const double*)S.np.as<double>(),
-                       nl.disp_part.as<double>());
+                       nl.disp_part.as<double>(), S.keep_ptr());
   } else if (S.thermo == 4) {
     hipLaunchKernelGGL(k_npt_drift, dim3(nb), dim3(256), 0, st, n, S.pos.as<double>(),
                        S.vel.as<double>(), (const double*)S.acc.as<double>(), dt,
                        (const Geom*)S.geom, (const double*)nl.ref_pos.as<double>(),
                        (const double*)S.nhc.as<double>(), (const double*)S.np.as<double>(),
-                       nl.disp_part.as<double>());
+                       nl.disp_part.as<double>(), S.keep_ptr());
   } else {
     hipLaunchKernelGGL(k_md_drift, dim3(nb), dim3(256), 0, st, n, S.pos.as<double>(),
                        S.vel.as<double>(), (const double*)S.acc.as<double>(), dt, set.geoms[0],
                        (const double*)nl.ref_pos.as<double>(), nl.disp_part.as<double>(),
-                       (const double*)(S.scaled() ? S.nhc.as<double>() + 1 : nullptr));
+                       (const double*)(S.scaled() ? S.nhc.as<double>() + 1 : nullptr),
+                       S.keep_ptr());
   }
   if (S.flex())
     hipLaunchKernelGGL(k_nptg_flag, dim3(1), dim3(256), 0, st, (int)nb,
@@ -1238,6 +1289,23 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
   S.geom = set.d_geoms.as<Geom>();
   S.thermo = opt.thermostat;
   S.cell = opt.thermostat == 4 ? opt.cell_dof : 0;
+  // fix_atoms (_build_fix_mask :248-267): frozen components start at zero
+  // and stay there; _compute_n_dof (:341-361): 3N - 3 with nothing frozen,
+  // else 3N minus the frozen components
+  int n_fixed = 0;
+  if (opt.fixed)
+    for (int64_t k = 0; k < 3 * (int64_t)n; ++k) n_fixed += opt.fixed[k] ? 1 : 0;
+  S.fixed = n_fixed > 0;
+  const double ndof = (double)std::max(n_fixed ? 3 * n - n_fixed : 3 * n - 3, 1);
+  AK_REQUIRE(opt.rescale_steps >= 0, AK_ERR_ARG, "velocity_rescale_steps must be >= 0");
+  AK_REQUIRE(opt.rescale_steps == 0 || opt.thermostat == 1 || opt.thermostat == 4, AK_ERR_ARG,
+             "velocity_rescale_steps applies to the Nose-Hoover chain and NPT only");
+  const double steps_done = st_at(kStSteps);
+  AK_REQUIRE(steps_done >= 0.0 && steps_done == std::floor(steps_done), AK_ERR_ARG,
+             "thermostat state: bad step count");
+  S.vr_until = (int)std::max(0.0, std::min((double)opt.rescale_steps - steps_done,
+                                           (double)n_steps + 1.0));
+  S.T = opt.temperature;
   S.every = opt.flux_every;
   S.pos.ensure(sizeof(double) * 3 * n);
   S.vel.ensure(sizeof(double) * 3 * n);
@@ -1268,7 +1336,7 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
                "thermostat temperature and tau must be positive");
     S.M = opt.chain;
     S.kT = kBoltzmannEv * opt.temperature;
-    S.ndof = (double)std::max(3 * n - 3, 1);
+    S.ndof = ndof;
     std::vector<double> h(2 + 3 * S.M, 0.0);
     h[0] = h[1] = 1.0;
     h[2] = S.ndof * kBoltzmannEv * opt.temperature * opt.tau * opt.tau;
@@ -1322,6 +1390,8 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
       S.npt.tau_b = opt.tau_baro;
       S.npt.k0 = (uint32_t)(opt.seed & 0xffffffffu);
       S.npt.k1 = (uint32_t)(opt.seed >> 32);
+      S.npt.vr_until = S.vr_until;
+      S.npt.T = S.T;
       std::vector<double> q(kNpLen, 0.0);
       q[0] = q[1] = q[7] = q[8] = 1.0;
       q[2] = st_at(kStPeps);
@@ -1349,7 +1419,7 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
                "thermostat temperature and damping must be positive");
     AK_REQUIRE(n >= 2, AK_ERR_ARG, "a stochastic thermostat needs at least two atoms");
     S.kT = kBoltzmannEv * opt.temperature;
-    S.ndof = (double)std::max(3 * n - 3, 1);
+    S.ndof = ndof;
     S.damping = opt.tau;
     S.k0 = (uint32_t)(opt.seed & 0xffffffffu);
     S.k1 = (uint32_t)(opt.seed >> 32);
@@ -1368,6 +1438,15 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
     AK_HIP(hipStreamSynchronize(st));
   }
   AK_HIP(hipMemcpyAsync(S.pos.p, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
+  if (S.fixed) {
+    std::vector<double> kp(3 * (size_t)n);
+    for (size_t k = 0; k < kp.size(); ++k) {
+      kp[k] = opt.fixed[k] ? 0.0 : 1.0;
+      if (opt.fixed[k]) vel[k] = 0.0;
+    }
+    S.keep.ensure(sizeof(double) * kp.size());
+    AK_HIP(hipMemcpy(S.keep.p, kp.data(), sizeof(double) * kp.size(), hipMemcpyHostToDevice));
+  }
   AK_HIP(hipMemcpyAsync(S.vel.p, vel, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
   AK_HIP(hipMemcpyAsync(S.mass.p, mass, sizeof(double) * n, hipMemcpyHostToDevice, st));
   std::vector<int32_t> sp(n, 0);
@@ -1443,6 +1522,7 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
   AK_HIP(hipMemcpy(counters, S.counters.p, sizeof(counters), hipMemcpyDeviceToHost));
   AK_REQUIRE(counters[0] == n_steps + 1, AK_ERR_STATE, "MD step counter mismatch");
   AK_REQUIRE(S.every == 0 || counters[2] == n_steps, AK_ERR_STATE, "heat-flux gate mismatch");
+  if (state) state[kStSteps] = steps_done + n_steps;
   if (state && (S.thermo == 1 || S.thermo == 4)) {
     std::vector<double> h(2 + 3 * S.M);
     AK_HIP(hipMemcpy(h.data(), S.nhc.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost));

@@ -29,7 +29,7 @@ CELL_DOF_MODES = ("iso", "aniso", "tri", "xy", "xy_iso", "z")
 # thermostat state carried between device runs (AK_MD_STATE_LEN doubles)
 STATE_LEN = 48
 _ST = {"xi": (0, 16), "vxi": (16, 32), "p_eps": (32, 33), "xi_b": (33, 35), "vxi_b": (35, 37),
-       "p_g": (37, 43), "heat": (43, 44), "rng_draws": (44, 45)}
+       "p_g": (37, 43), "heat": (43, 44), "rng_draws": (44, 45), "steps": (45, 46)}
 
 
 def thermostat_state(xi=(), vxi=(), p_eps=0.0, xi_b=(0.0, 0.0), vxi_b=(0.0, 0.0), p_g=(),
@@ -61,6 +61,7 @@ def state_fields(state, chain=None):
         out["xi"], out["vxi"] = out["xi"][:chain], out["vxi"][:chain]
     out["p_eps"], out["heat"] = float(out["p_eps"][0]), float(out["heat"][0])
     out["rng_draws"] = int(out["rng_draws"][0])
+    out["steps"] = int(out["steps"][0])
     return out
 
 
@@ -88,7 +89,7 @@ def masses_of(atom_types):
 
 def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_steps=1000,
            skin_width=0.3, masses=None, use_graph=True, thermostat=None, flux_every=0,
-           state=None):
+           state=None, fix_atoms=None, velocity_rescale_steps=0):
     """n_steps of velocity Verlet on the device (``ak_md_run``).  thermostat:
     None (NVE), ("nhc", temperature K, tau fs, chain links), ("langevin",
     temperature, damping fs, seed), ("csvr", temperature, damping, seed) or
@@ -101,7 +102,11 @@ def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_step
     state after every step s with s % flux_every == 0.  ``state`` (the
     "state" array a previous run returned, one of ``thermostat_state``, or
     None to start from rest) continues the thermostat, barostat and random
-    stream where that run ended; the final one is returned as "state"."""
+    stream where that run ended; the final one is returned as "state".
+    fix_atoms: (N, 3) bool, True = frozen Cartesian component (every MD
+    class's fix_atoms, solvers/molecular_dynamics.py:248-267, 341-361);
+    velocity_rescale_steps: direct rescaling to the target temperature after
+    each of the first that many steps (chain and NPT only)."""
     lat = np.ascontiguousarray(lattice, float)
     pos = np.ascontiguousarray(positions, float).copy()
     vel = np.ascontiguousarray(velocities, float).copy()
@@ -120,11 +125,13 @@ def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_step
         kind = thermostat[0]
         if kind == "nhc":
             _, T, tau, chain = thermostat
-            th = np.array([1.0, float(T), float(tau), float(chain), 0.0, 0.0, 0.0, 0.0, 0.0])
+            th = np.array([1.0, float(T), float(tau), float(chain), 0.0, 0.0, 0.0, 0.0, 0.0,
+                           float(int(velocity_rescale_steps))])
         elif kind in ("langevin", "csvr"):
             _, T, damping, seed = thermostat
             th = np.array([2.0 if kind == "langevin" else 3.0, float(T), float(damping), 0.0,
-                           float(int(seed)), 0.0, 0.0, 0.0, 0.0])
+                           float(int(seed)), 0.0, 0.0, 0.0, 0.0,
+                           float(int(velocity_rescale_steps))])
         elif kind == "npt":
             _, T, tau, chain, p_gpa, tau_b, *rest = thermostat
             cell, baro, seed = (list(rest) + ["iso", "nhc", 0][len(rest):])[:3]
@@ -135,13 +142,23 @@ def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_step
                 raise ValueError(f"NPT: baro_thermostat must be nhc | langevin, got {baro!r}.")
             th = np.array([4.0, float(T), float(tau), float(chain), float(int(seed)),
                            float(p_gpa), float(tau_b), float(CELL_DOF_MODES.index(cell)),
-                           1.0 if baro == "langevin" else 0.0])
+                           1.0 if baro == "langevin" else 0.0,
+                           float(int(velocity_rescale_steps))])
         else:
             raise ValueError(f"thermostat: unknown kind {kind!r} (nhc, langevin, csvr, npt)")
     lat_out = np.zeros((3, 3))
     vol = np.zeros(n_steps + 1)
     prs = np.zeros(n_steps + 1)
     st = np.zeros(STATE_LEN) if state is None else np.array(state, float).reshape(STATE_LEN)
+    if th is None and velocity_rescale_steps:
+        raise ValueError("velocity_rescale_steps needs a thermostat (nhc or npt)")
+    fix = None
+    if fix_atoms is not None:
+        fa = np.asarray(fix_atoms, dtype=bool)
+        if fa.shape != (n, 3):
+            raise ValueError(f"fix_atoms must have shape ({n}, 3), got {fa.shape}.")
+        if fa.any():
+            fix = np.ascontiguousarray(fa, dtype=np.uint8)
     nreb = ctypes.c_int32(0)
     secs = ctypes.c_double(0.0)
     P = _lib._pf64
@@ -150,7 +167,8 @@ def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_step
                           spc.ctypes.data_as(_lib._pi32), float(dt), int(n_steps),
                           float(skin_width), 1 if use_graph else 0,
                           th.ctypes.data_as(P) if th is not None else None,
-                          st.ctypes.data_as(P), every,
+                          st.ctypes.data_as(P),
+                          fix.ctypes.data_as(_lib._pu8) if fix is not None else None, every,
                           ep.ctypes.data_as(P), ek.ctypes.data_as(P), ec.ctypes.data_as(P),
                           J.ctypes.data_as(P), lat_out.ctypes.data_as(P), vol.ctypes.data_as(P),
                           prs.ctypes.data_as(P), ctypes.byref(nreb), ctypes.byref(secs)))
@@ -167,33 +185,40 @@ def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_step
     return out
 
 
+def n_dof_of(n_atoms, fix_atoms=None):
+    """Degrees of freedom of the temperature (_compute_n_dof,
+    solvers/molecular_dynamics.py:341-361): 3N - 3, or 3N minus the frozen
+    components when any is frozen."""
+    n_fixed = int(np.sum(np.asarray(fix_atoms, dtype=bool))) if fix_atoms is not None else 0
+    return max(3 * n_atoms - 3, 1) if n_fixed == 0 else max(3 * n_atoms - n_fixed, 1)
+
+
 def nve_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_steps=1000,
-            skin_width=0.3, masses=None, use_graph=True, flux_every=0):
+            skin_width=0.3, masses=None, use_graph=True, flux_every=0, fix_atoms=None):
     """Run n_steps of NVE on the device (NVESimulation.run,
     solvers/molecular_dynamics.py:1969-1996).  Returns md_run's dict;
     drift = e_tot - e_tot[0]."""
     return md_run(potential, lattice, positions, velocities, atom_types, dt, n_steps,
-                  skin_width, masses, use_graph, None, flux_every)
+                  skin_width, masses, use_graph, None, flux_every, fix_atoms=fix_atoms)
 
 
 def nvt_run(potential, lattice, positions, velocities, atom_types, temperature, dt=1.0,
             n_steps=1000, tau_nhc=100.0, nhc_chain=3, skin_width=0.5, masses=None,
-            use_graph=True, flux_every=0, state=None):
+            use_graph=True, flux_every=0, state=None, fix_atoms=None, velocity_rescale_steps=0):
     """NVT with an M-link Nose-Hoover chain on the device (NVTSimulation.run,
-    solvers/molecular_dynamics.py:1113-1160, _nhc_step :635-727; defaults of
+    solvers/molecular_dynamics.py:1113-1167, _nhc_step :635-727; defaults of
     NVTSimulation: tau_nhc 100 fs, 3 links, skin 0.5 A).  e_cons is H_NHC
-    (:947-969); T_inst = 2 E_kin / (n_dof k_B), n_dof = 3N - 3."""
+    (:947-969); T_inst = 2 E_kin / (n_dof k_B) with ``n_dof_of``."""
     out = md_run(potential, lattice, positions, velocities, atom_types, dt, n_steps,
                  skin_width, masses, use_graph, ("nhc", temperature, tau_nhc, nhc_chain),
-                 flux_every, state)
-    n_dof = max(3 * len(positions) - 3, 1)
-    out["T"] = 2.0 * out["e_kin"] / (n_dof * K_B)
+                 flux_every, state, fix_atoms, velocity_rescale_steps)
+    out["T"] = 2.0 * out["e_kin"] / (n_dof_of(len(positions), fix_atoms) * K_B)
     return out
 
 
 def langevin_run(potential, lattice, positions, velocities, atom_types, temperature, dt=1.0,
                  n_steps=1000, damping=100.0, seed=42, skin_width=0.5, masses=None,
-                 use_graph=True, flux_every=0, state=None):
+                 use_graph=True, flux_every=0, state=None, fix_atoms=None):
     """NVT with a Langevin thermostat integrated by BAOAB on the device
     (LangevinSimulation, solvers/molecular_dynamics.py:1511-1587): e_cons is
     E_tot - Q, Q the heat the O steps and the centre-of-mass removal added.
@@ -201,28 +226,28 @@ def langevin_run(potential, lattice, positions, velocities, atom_types, temperat
     numpy's generator: same process, other realisations)."""
     out = md_run(potential, lattice, positions, velocities, atom_types, dt, n_steps,
                  skin_width, masses, use_graph, ("langevin", temperature, damping, seed),
-                 flux_every, state)
-    out["T"] = 2.0 * out["e_kin"] / (max(3 * len(positions) - 3, 1) * K_B)
+                 flux_every, state, fix_atoms)
+    out["T"] = 2.0 * out["e_kin"] / (n_dof_of(len(positions), fix_atoms) * K_B)
     return out
 
 
 def csvr_run(potential, lattice, positions, velocities, atom_types, temperature, dt=1.0,
              n_steps=1000, damping=100.0, seed=42, skin_width=0.5, masses=None,
-             use_graph=True, flux_every=0, state=None):
+             use_graph=True, flux_every=0, state=None, fix_atoms=None):
     """NVT with the Bussi-Donadio-Parrinello velocity rescaling on the device
     (CSVRSimulation, solvers/molecular_dynamics.py:1589-1687): R B A F B R,
     e_cons = E_tot - Q; random numbers as in :func:`langevin_run`."""
     out = md_run(potential, lattice, positions, velocities, atom_types, dt, n_steps,
                  skin_width, masses, use_graph, ("csvr", temperature, damping, seed), flux_every,
-                 state)
-    out["T"] = 2.0 * out["e_kin"] / (max(3 * len(positions) - 3, 1) * K_B)
+                 state, fix_atoms)
+    out["T"] = 2.0 * out["e_kin"] / (n_dof_of(len(positions), fix_atoms) * K_B)
     return out
 
 
 def npt_run(potential, lattice, positions, velocities, atom_types, temperature, pressure=0.0,
             dt=1.0, n_steps=1000, tau_nhc=100.0, nhc_chain=3, tau_baro=1000.0, skin_width=0.5,
             masses=None, use_graph=True, state=None, cell_dof="iso", baro_thermostat="nhc",
-            seed=42):
+            seed=42, fix_atoms=None, velocity_rescale_steps=0):
     """NPT with the MTTK barostat and the particle Nose-Hoover chain, on the
     device (NPTSimulation, solvers/molecular_dynamics.py:2141-2692; defaults
     tau_nhc 100 fs, 3 links, tau_baro 1000 fs, cell_dof "iso",
@@ -238,6 +263,6 @@ def npt_run(potential, lattice, positions, velocities, atom_types, temperature, 
     out = md_run(potential, lattice, positions, velocities, atom_types, dt, n_steps,
                  skin_width, masses, use_graph,
                  ("npt", temperature, tau_nhc, nhc_chain, pressure, tau_baro, cell_dof,
-                  baro_thermostat, seed), 0, state)
-    out["T"] = 2.0 * out["e_kin"] / (max(3 * len(positions) - 3, 1) * K_B)
+                  baro_thermostat, seed), 0, state, fix_atoms, velocity_rescale_steps)
+    out["T"] = 2.0 * out["e_kin"] / (n_dof_of(len(positions), fix_atoms) * K_B)
     return out

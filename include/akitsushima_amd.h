@@ -219,7 +219,8 @@ int ak_md_nve(ak_potential *pot, int32_t n_atoms, const double *lattice,
  * :1511-1587), CSVRSimulation (:1589-1687) and NPTSimulation with the
  * isotropic barostat (:2141-2555, cell_dof "iso", baro_thermostat "nhc").
  * thermostat: NULL for NVE, or {kind, temperature K, tau fs, chain links,
- * seed, pressure GPa, tau_baro fs, cell_dof, baro_thermostat}: kind 1 =
+ * seed, pressure GPa, tau_baro fs, cell_dof, baro_thermostat,
+ * velocity_rescale_steps}: kind 1 =
  * Nose-Hoover chain (tau_nhc, 2-16 links), 2 = Langevin, 3 = CSVR (tau =
  * damping; seed an integer < 2^53 keys the Philox4x32-10 stream), 4 = NPT
  * (the chain of kind 1 plus the barostat; a perpendicular cell width may
@@ -228,7 +229,14 @@ int ak_md_nve(ak_potential *pot, int32_t n_atoms, const double *lattice,
  * flexible cell restricted to those strain generators, :2566-2692);
  * baro_thermostat: 0 the 2-link Nose-Hoover chain on the barostat momenta,
  * 1 Langevin (:2596-2617; noise keyed by seed, its heat subtracted from
- * H_NPT).
+ * H_NPT).  velocity_rescale_steps (kinds 1 and 4, :1154-1167, 2907-2920):
+ * v *= sqrt(T / T_inst) after the end-of-step chain half-step of the first
+ * that many steps (counted over restarts through state[45]).
+ * fix_mask (3N bytes, 1 = frozen Cartesian component; NULL = none; fix_atoms
+ * of every MD class, :248-267): frozen components start at zero and are
+ * zeroed after every kick; n_dof = 3N - (number frozen) when any is frozen
+ * (_compute_n_dof :341-361); a Langevin run with frozen components keeps
+ * its centre-of-mass velocity (:1552-1565).
  * state (AK_MD_STATE_LEN doubles, may be NULL = start from rest) carries the
  * thermostat between calls, as the reference's checkpoint scalars do
  * (_ck_arrays / _CK_SCALARS, :431-445, 823-830, 2228, 2557-2564): read at
@@ -237,7 +245,8 @@ int ak_md_nve(ak_potential *pot, int32_t n_atoms, const double *lattice,
  * chain xi_b, [35, 37) vxi_b, [37, 43) the strain momenta p_k of a flexible
  * cell (in generator order), 43 the heat Q a stochastic thermostat or the
  * Langevin barostat has added, 44 the random draws used so far (a
- * continued run draws new numbers), rest reserved (pass zeros).
+ * continued run draws new numbers), 45 the steps run so far, rest reserved
+ * (pass zeros).
  * e_cons (n_steps + 1): E_tot (NVE), H_NHC (:947-969), E_tot - Q, Q the heat
  * a stochastic thermostat added (the E-Q column), or H_NPT (:2425-2454).
  * lattice_out (9): the final cell; volume, pressure (n_steps + 1): V [A^3]
@@ -248,10 +257,10 @@ int ak_md_nve(ak_potential *pot, int32_t n_atoms, const double *lattice,
 int ak_md_run(ak_potential *pot, int32_t n_atoms, const double *lattice,
               const double *inv_lattice, double *positions, double *velocities,
               const double *masses, const int32_t *species, double dt, int32_t n_steps,
-              double skin_width, int32_t use_graph, const double *thermostat9,
-              double *state, int32_t flux_every, double *e_pot, double *e_kin,
-              double *e_cons, double *flux, double *lattice_out, double *volume,
-              double *pressure, int32_t *n_rebuilds, double *seconds);
+              double skin_width, int32_t use_graph, const double *thermostat10,
+              double *state, const uint8_t *fix_mask, int32_t flux_every, double *e_pot,
+              double *e_kin, double *e_cons, double *flux, double *lattice_out,
+              double *volume, double *pressure, int32_t *n_rebuilds, double *seconds);
 #define AK_MD_STATE_LEN 48
 
 /* ---- Site terms and heat flux ------------------------------------------ */

@@ -820,7 +820,7 @@ def langevin_noise(n, step, k0, k1):
 
 def md_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, skin,
            nhc=None, flux=None, every=0, langevin=None, csvr=None, state_out=None,
-           nhc_state=None):
+           nhc_state=None, fix=None, rescale_steps=0):
     """Velocity Verlet with PBC wrap and the Verlet-skin rebuild test of
     engine/neighborlist.py:613-673 (fixed cell: rebuild when r_list - 2 d_max
     <= r_cut); ``force(lattice, pos, (i, j))`` returns (E, F, S).  nhc =
@@ -834,7 +834,12 @@ def md_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, sk
     (:947-969) or E_tot - Q), final positions and velocities, rebuilds, J
     frames; a dict passed as state_out receives the final chain positions
     "xi" and velocities "vxi"; nhc_state = (xi, vxi) starts the chain there
-    (a checkpoint restart) instead of at rest."""
+    (a checkpoint restart) instead of at rest.  fix (N, 3) bool: frozen
+    components, zeroed at the start and after every kick, n_dof = 3N minus
+    their number, no centre-of-mass removal in the Langevin O step
+    (_build_fix_mask, _compute_n_dof :248-267, 341-361, _o_step :1552-1565);
+    rescale_steps: with the chain, v *= sqrt(T / T_inst) after the
+    end-of-step half-step of the first that many steps (:1154-1167)."""
     lattice = np.asarray(lattice, float)
     pos = np.array(positions, float)
     vel = np.array(velocities, float)
@@ -845,7 +850,13 @@ def md_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, sk
     e, f, _ = force(lattice, pos, pairs)
     acc = f * EV_ANG_TO_AMU_ANG_FS2 / masses[:, None]
     ek = lambda v: 0.5 * float(np.sum(masses[:, None] * v ** 2)) / EV_ANG_TO_AMU_ANG_FS2
-    n_dof = max(3 * len(pos) - 3, 1)
+    keep = None
+    n_fixed = int(np.sum(fix)) if fix is not None else 0
+    if n_fixed:
+        keep = ~np.asarray(fix, dtype=bool)
+        vel = vel * keep
+    n_dof = max(3 * len(pos) - 3, 1) if n_fixed == 0 else max(3 * len(pos) - n_fixed, 1)
+    mask = (lambda v: v * keep) if keep is not None else (lambda v: v)
     state = None
     heat = [0.0]
     if nhc is not None:
@@ -876,6 +887,13 @@ def md_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, sk
         f[f >= 1.0] = 0.0
         return f @ lattice
 
+    def vel_rescale(v):
+        e_now = ek(v)
+        T_now = 2.0 * e_now / (n_dof * K_B) if e_now > 0 else 0.0
+        if T_now > 1e-12:
+            v = mask(v * np.sqrt(T / T_now))
+        return v
+
     def cons(e_pot, e_kin):
         if stoch is not None:
             return e_kin + e_pot - heat[0]
@@ -899,16 +917,19 @@ def md_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, sk
             vel = rescale(vel)
         if langevin is not None:
             c = math.exp(-dt / damping)
-            vel = vel + 0.5 * dt * acc
+            vel = mask(vel + 0.5 * dt * acc)
             pos = pos + 0.5 * dt * vel
             before = ek(vel)
             sig = np.sqrt((1.0 - c * c) * kT * EV_ANG_TO_AMU_ANG_FS2 / masses)
             vel = c * vel + sig[:, None] * langevin_noise(len(pos), step, k0, k1)
-            vel = vel - np.sum(masses[:, None] * vel, axis=0) / M_tot
+            if keep is not None:
+                vel = vel * keep
+            else:
+                vel = vel - np.sum(masses[:, None] * vel, axis=0) / M_tot
             heat[0] += ek(vel) - before
             pos = wrap(pos + 0.5 * dt * vel)
         else:
-            vel += 0.5 * dt * acc
+            vel = mask(vel + 0.5 * dt * acc)
             pos = wrap(pos + dt * vel)
         dr = pos - ref
         fr = dr @ inv
@@ -920,9 +941,11 @@ def md_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, sk
             rebuilds += 1
         e, f, _ = force(lattice, pos, pairs)
         acc = f * EV_ANG_TO_AMU_ANG_FS2 / masses[:, None]
-        vel += 0.5 * dt * acc
+        vel = mask(vel + 0.5 * dt * acc)
         if state is not None:
             vel = nhc_half(state, dt, kT, n_dof, vel, masses)
+            if step < rescale_steps:
+                vel = vel_rescale(vel)
         if csvr is not None:
             vel = rescale(vel)
         epot.append(e)
@@ -962,7 +985,8 @@ def baro_noise(n, draw, k0, k1):
 
 
 def npt_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, skin, T, tau, M,
-            p_gpa, tau_baro, start=None, cell_dof="iso", baro="nhc", seed=0, generators=None):
+            p_gpa, tau_baro, start=None, cell_dof="iso", baro="nhc", seed=0, generators=None,
+            fix=None, rescale_steps=0):
     """NPTSimulation.run (solvers/molecular_dynamics.py:2871-2895): particle
     chain and barostat thermostat half-steps, barostat half-step, half kick,
     drift with the cell scaling, forces, half kick, barostat half-step at the
@@ -992,12 +1016,17 @@ def npt_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, s
     velocities and lattice, rebuilds, the final particle chain ("xi", "vxi")
     and barostat state ("p_eps" (iso) or "p_g", "xi_b", "vxi_b", "heat"); a
     dict with those keys passed as ``start`` begins the run from that state
-    (a checkpoint restart)."""
+    (a checkpoint restart).  fix and rescale_steps as in ``md_run``
+    (:2360-2373, 2886-2920)."""
     lattice = np.array(lattice, float)
     pos = np.array(positions, float)
     vel = np.array(velocities, float)
     r_list = r_cut + skin
-    n_dof = max(3 * len(pos) - 3, 1)
+    n_fixed = int(np.sum(fix)) if fix is not None else 0
+    keep = ~np.asarray(fix, dtype=bool) if n_fixed else None
+    mask = (lambda v: v * keep) if keep is not None else (lambda v: v)
+    vel = mask(vel)
+    n_dof = max(3 * len(pos) - 3, 1) if n_fixed == 0 else max(3 * len(pos) - n_fixed, 1)
     kT = K_B * T
     ek = lambda v: 0.5 * float(np.sum(masses[:, None] * v ** 2)) / EV_ANG_TO_AMU_ANG_FS2
     state = ([n_dof * kT * tau ** 2] + [kT * tau ** 2] * (M - 1), [0.0] * M, [0.0] * M)
@@ -1117,7 +1146,7 @@ def npt_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, s
            "p": [p_inst(vel, S, lattice) * 160.2176621], "lat": [lattice.copy()],
            "ptensor": [ptensor(vel, S, lattice)]}
     rebuilds = 0
-    for _ in range(n_steps):
+    for step in range(n_steps):
         P = p_inst(vel, S, lattice)
         vel = nhc_half(state, dt, kT, n_dof, vel, masses)
         baro_thermo()
@@ -1125,7 +1154,7 @@ def npt_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, s
             baro_half_iso(P, vel, lattice)
         else:
             baro_half_g(vel, S, lattice)
-        vel = vel + 0.5 * dt * acc
+        vel = mask(vel + 0.5 * dt * acc)
         if iso:
             ed = b["p"] / W
             s = float(np.exp(ed * dt))
@@ -1165,7 +1194,7 @@ def npt_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, s
             rebuilds += 1
         e, f, S = force(lattice, pos, pairs)
         acc = f * EV_ANG_TO_AMU_ANG_FS2 / masses[:, None]
-        vel = vel + 0.5 * dt * acc
+        vel = mask(vel + 0.5 * dt * acc)
         Pn = p_inst(vel, S, lattice)
         if iso:
             baro_half_iso(Pn, vel, lattice)
@@ -1173,6 +1202,11 @@ def npt_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, s
             baro_half_g(vel, S, lattice)
         baro_thermo()
         vel = nhc_half(state, dt, kT, n_dof, vel, masses)
+        if step < rescale_steps:
+            e_now = ek(vel)
+            T_now = 2.0 * e_now / (n_dof * K_B) if e_now > 0 else 0.0
+            if T_now > 1e-12:
+                vel = mask(vel * np.sqrt(T / T_now))
         rec["ep"].append(e)
         rec["ek"].append(ek(vel))
         rec["h"].append(H(e, vel, lattice))
