@@ -4,11 +4,12 @@ Hand-written HIP kernels for gfx950 behind the reference's force-function
 API: Tersoff (one or many species, Tersoff 1989 mixing), Stillinger-Weber
 (flat or multi-species records) and alloy EAM (Zhou forms or setfl tables)
 energies, forces and virial stress, a device cell-list neighbour list,
-batched evaluation of many structures, device-resident NVE and Nose-Hoover
-NVT integrators, site energies and the exact many-body heat flux (Green-Kubo
-thermal conductivity), a Tersoff parameter gradient, and the batched callers
-of the force call: force constants and Gamma-point phonons, and elastic
-constants.  The compute
+batched evaluation of many structures, device-resident NVE, NVT
+(Nose-Hoover chain, Langevin, CSVR) and NPT (isotropic or flexible cell)
+integrators behind the reference's simulation classes, site energies and the
+exact many-body heat flux (Green-Kubo thermal conductivity), a Tersoff
+parameter gradient, and the batched callers of the force call: force
+constants and Gamma-point phonons, and elastic constants.  The compute
 path is the HIP library ``libakitsushima_amd.so`` only; importing works
 without a GPU, computing does not.
 """
@@ -26,6 +27,8 @@ from .potentials import (
 from .setfl import read_setfl, write_setfl
 from .site_terms import SiteTerms
 from .thermal import ThermalConductivityCalculator
+from .simulations import (CSVRSimulation, LangevinSimulation, NPTSimulation, NVESimulation,
+                          NVTSimulation)
 from .elastic import elastic_constants
 from .phonon import force_constants, gamma_frequencies
 from .zhou import zhou_eam
@@ -45,6 +48,11 @@ __all__ = [
     "read_setfl",
     "SiteTerms",
     "ThermalConductivityCalculator",
+    "NVESimulation",
+    "NVTSimulation",
+    "LangevinSimulation",
+    "CSVRSimulation",
+    "NPTSimulation",
     "write_setfl",
     "tersoff1989_to_triplets",
     "elastic_constants",

@@ -872,9 +872,10 @@ int ak_md_run(ak_potential* P, int32_t n, const double* lattice, const double* i
               double* positions, double* velocities, const double* masses,
               const int32_t* species, double dt, int32_t n_steps, double skin,
               int32_t use_graph, const double* thermostat, double* state,
-              const uint8_t* fix_mask, int32_t flux_every, double* e_pot, double* e_kin,
-              double* e_cons, double* flux, double* lattice_out, double* volume,
-              double* pressure, int32_t* n_rebuilds, double* seconds) {
+              const uint8_t* fix_mask, int32_t flux_every, int32_t frame_every, double* frames,
+              double* frame_lattices, double* e_pot, double* e_kin, double* e_cons,
+              double* flux, double* lattice_out, double* volume, double* pressure,
+              int32_t* rebuild_counts, int32_t* n_rebuilds, double* seconds) {
   return guarded([&] {
     AK_REQUIRE(P && lattice && positions && velocities && masses && n > 0, AK_ERR_ARG,
                "bad arguments");
@@ -917,6 +918,11 @@ int ak_md_run(ak_potential* P, int32_t n, const double* lattice, const double* i
       opt.rescale_steps = (int)thermostat[9];
     }
     opt.fixed = fix_mask;
+    AK_REQUIRE(frame_every >= 0, AK_ERR_ARG, "frame_every must be >= 0");
+    opt.frame_every = frames ? frame_every : 0;
+    opt.frames = frames;
+    opt.frame_lattices = frame_lattices;
+    opt.rebuild_counts = rebuild_counts;
     opt.flux_every = flux_every;
     int nr = 0;
     md_run(P->pot, n, lattice, inv, positions, velocities, masses, species, dt, n_steps, skin,

@@ -232,6 +232,13 @@ struct MDOptions {
   int flux_every = 0;
   int rescale_steps = 0;            // velocity_rescale_steps (chain and NPT)
   const uint8_t* fixed = nullptr;   // fix_atoms: 3N flags, 1 = frozen component
+  // trajectory: positions (n_steps / frame_every, N, 3) after every
+  // frame_every steps, and under NPT the cell of each frame (host outputs)
+  int frame_every = 0;
+  double* frames = nullptr;
+  double* frame_lattices = nullptr;
+  // list rebuilds of the run up to each recorded state (n_steps + 1)
+  int32_t* rebuild_counts = nullptr;
 };
 
 // Thermostat / barostat state carried between runs (in and out of md_run):

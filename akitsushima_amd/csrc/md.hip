@@ -141,7 +141,8 @@ __global__ __launch_bounds__(256) void k_md_flag(int nb, const double* __restric
                                                  int* __restrict__ counters,
                                                  double* __restrict__ e_pot,
                                                  double* __restrict__ e_kin,
-                                                 double* __restrict__ e_cons) {
+                                                 double* __restrict__ e_cons, double V,
+    double* __restrict__ h_press, int* __restrict__ h_nb) {
   __shared__ double smem[16];
   __shared__ double m[256];
   double v[1] = {0.0};
@@ -163,6 +164,8 @@ __global__ __launch_bounds__(256) void k_md_flag(int nb, const double* __restric
       e_pot[step] = es[0];
       e_kin[step] = v[0];
       e_cons[step] = es[0] + v[0];
+      h_press[step] = 2.0 * v[0] / (3.0 * V) + ((es[1] + es[5]) + es[9]) / 3.0;
+      h_nb[step] = counters[1];
       counters[0] = step + 1;
     }
     if (disp_part) {
@@ -239,7 +242,8 @@ __global__ __launch_bounds__(256) void k_nhc(int nb, const double* __restrict__ 
                                              int* __restrict__ counters,
                                              double* __restrict__ e_pot,
                                              double* __restrict__ e_kin,
-                                             double* __restrict__ e_cons) {
+                                             double* __restrict__ e_cons, double V,
+    double* __restrict__ h_press, int* __restrict__ h_nb) {
   __shared__ double smem[16];
   double v[1] = {0.0};
   for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += ke_part[b];
@@ -271,6 +275,8 @@ __global__ __launch_bounds__(256) void k_nhc(int nb, const double* __restrict__ 
   e_pot[step] = es[0];
   e_kin[step] = ek;
   e_cons[step] = ek + es[0] + t + u;
+  h_press[step] = 2.0 * ek / (3.0 * V) + ((es[1] + es[5]) + es[9]) / 3.0;
+  h_nb[step] = counters[1];
   counters[0] = step + 1;
 }
 
@@ -367,7 +373,8 @@ __global__ __launch_bounds__(256) void k_csvr(int nb, const double* __restrict__
                                               int* __restrict__ counters,
                                               double* __restrict__ e_pot,
                                               double* __restrict__ e_kin,
-                                              double* __restrict__ e_cons) {
+                                              double* __restrict__ e_cons, double V,
+    double* __restrict__ h_press, int* __restrict__ h_nb) {
   __shared__ double smem[16];
   double v[1] = {0.0};
   for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += ke_part[b];
@@ -393,6 +400,8 @@ __global__ __launch_bounds__(256) void k_csvr(int nb, const double* __restrict__
   e_pot[step] = es[0];
   e_kin[step] = ek;
   e_cons[step] = ek + es[0] - th[2];
+  h_press[step] = 2.0 * ek / (3.0 * V) + ((es[1] + es[5]) + es[9]) / 3.0;
+  h_nb[step] = counters[1];
   counters[0] = step + 1;
 }
 
@@ -525,7 +534,8 @@ __global__ __launch_bounds__(256) void k_lang_rec(int nb, const double* __restri
                                                   int* __restrict__ counters,
                                                   double* __restrict__ e_pot,
                                                   double* __restrict__ e_kin,
-                                                  double* __restrict__ e_cons) {
+                                                  double* __restrict__ e_cons, double V,
+    double* __restrict__ h_press, int* __restrict__ h_nb) {
   __shared__ double smem[16 * 3];
   double v[3] = {0.0, 0.0, 0.0};
   for (int b = threadIdx.x; b < nb; b += blockDim.x) {
@@ -545,6 +555,8 @@ __global__ __launch_bounds__(256) void k_lang_rec(int nb, const double* __restri
   e_pot[step] = es[0];
   e_kin[step] = v[0];
   e_cons[step] = v[0] + es[0] - th[2];
+  h_press[step] = 2.0 * v[0] / (3.0 * V) + ((es[1] + es[5]) + es[9]) / 3.0;
+  h_nb[step] = counters[1];
   counters[0] = step + 1;
 }
 
@@ -645,7 +657,8 @@ __global__ __launch_bounds__(256) void k_npt(int nb, const double* __restrict__ 
                                              double* __restrict__ e_kin,
                                              double* __restrict__ e_cons,
                                              double* __restrict__ h_vol,
-                                             double* __restrict__ h_press) {
+                                             double* __restrict__ h_press,
+    int* __restrict__ h_nb) {
   __shared__ double smem[16];
   double v[1] = {0.0};
   for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += ke_part[b];
@@ -700,6 +713,7 @@ __global__ __launch_bounds__(256) void k_npt(int nb, const double* __restrict__ 
   e_cons[step] = (ek + es[0] + p.P_ext * V + 0.5 * p.W * ed * ed + t + u + hb) - np[kNpHeat];
   h_vol[step] = V;
   h_press[step] = P;
+  h_nb[step] = counters[1];
   counters[0] = step + 1;
 }
 
@@ -870,7 +884,8 @@ __global__ __launch_bounds__(256) void k_nptg(int nb, const double* __restrict__
                                               double* __restrict__ e_kin,
                                               double* __restrict__ e_cons,
                                               double* __restrict__ h_vol,
-                                              double* __restrict__ h_press) {
+                                              double* __restrict__ h_press,
+    int* __restrict__ h_nb) {
   __shared__ double smem[16 * 7];
   double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int b = threadIdx.x; b < nb; b += blockDim.x)
@@ -964,6 +979,7 @@ __global__ __launch_bounds__(256) void k_nptg(int nb, const double* __restrict__
   e_cons[step] = (ek + es[0] + p.P_ext * V + 0.5 * pp / p.W + (t + u) + hb) - np[kNpHeat];
   h_vol[step] = V;
   h_press[step] = P;
+  h_nb[step] = counters[1];
   counters[0] = step + 1;
 }
 
@@ -1091,6 +1107,26 @@ __global__ __launch_bounds__(256) void k_nptg_flag(int nb, const double* __restr
   counters[1] += fl;
 }
 
+// Trajectory frame of the state after every `every` steps: positions (and
+// the cell) copied into frame f = completed / every - 1, completed = the
+// record index counters[0] less rec_off (1 when state 0 is recorded before
+// the loop).  Steps outside [1, n_frames * every] (the set-up step) write
+// nothing.
+__global__ __launch_bounds__(256) void k_frame(int n, const double* __restrict__ pos,
+                                               const Geom* __restrict__ geom,
+                                               const int* __restrict__ counters, int rec_off,
+                                               int every, int n_frames,
+                                               double* __restrict__ frames,
+                                               double* __restrict__ frame_lat) {
+  const int done = counters[0] - rec_off;
+  if (done < 1 || done % every != 0) return;
+  const int f = done / every - 1;
+  if (f >= n_frames) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < 3 * (int64_t)n) frames[(int64_t)f * 3 * n + t] = pos[t];
+  if (frame_lat && t < 9) frame_lat[(int64_t)f * 9 + t] = geom->lat[t];
+}
+
 // v *= scale[0] (the pending thermostat scale at the end of a run).
 __global__ __launch_bounds__(256) void k_scale_vel(int n, double* __restrict__ vel,
                                                    const double* __restrict__ scale) {
@@ -1100,7 +1136,7 @@ __global__ __launch_bounds__(256) void k_scale_vel(int n, double* __restrict__ v
 
 struct MDState {
   DevBuf pos, vel, acc, F, mass, spc, es, ke, hist_ep, hist_ek, hist_ec, counters, flag;
-  DevBuf nhc, esite, flux_part, hist_J, ke_b, ke_a, p_part, np, hist_V, hist_P, keep;
+  DevBuf nhc, esite, flux_part, hist_J, ke_b, ke_a, p_part, np, hist_V, hist_P, keep, hist_nb;
   // thermostat: 0 none (NVE), 1 Nose-Hoover chain, 2 Langevin, 3 CSVR,
   // 4 NPT (barostat + Nose-Hoover chain); cell: 0 iso, else a flexible cell
   // whose kicks also sum the kinetic tensor
@@ -1112,6 +1148,10 @@ struct MDState {
   // velocity_rescale_steps left for this call, target temperature
   int vr_until = 0;
   double T = 0.0;
+  double V = 0.0;  // volume of a fixed cell (the pressure record)
+  // trajectory frames every frame_every steps (0 = none)
+  int frame_every = 0, n_frames = 0;
+  DevBuf frames, frame_lat;
   const double* keep_ptr() const { return fixed ? keep.as<double>() : nullptr; }
   NptParams npt{};
   Geom* geom = nullptr;  // the set's device cell
@@ -1139,34 +1179,36 @@ static void thermo_kernel(MDState& S, int mode, double dt, hipStream_t st, unsig
     hipLaunchKernelGGL(k_nhc, dim3(1), dim3(256), 0, st, (int)nb,
                        (const double*)S.ke.as<double>(), S.nhc.as<double>(), S.M, dt, S.kT,
                        S.ndof, S.vr_until, S.T, mode, (const double*)S.es.as<double>(),
-                       S.counters.as<int>(),
-                       S.hist_ep.as<double>(), S.hist_ek.as<double>(), S.hist_ec.as<double>());
+                       S.counters.as<int>(), S.hist_ep.as<double>(), S.hist_ek.as<double>(),
+                       S.hist_ec.as<double>(), S.V, S.hist_P.as<double>(), S.hist_nb.as<int>());
   else if (S.thermo == 3)
     hipLaunchKernelGGL(k_csvr, dim3(1), dim3(256), 0, st, (int)nb,
                        (const double*)S.ke.as<double>(), S.nhc.as<double>(),
                        exp(-0.5 * dt / S.damping), S.ndof, S.kT, S.k0, S.k1, mode,
                        (const double*)S.es.as<double>(), S.counters.as<int>(),
-                       S.hist_ep.as<double>(), S.hist_ek.as<double>(), S.hist_ec.as<double>());
+                       S.hist_ep.as<double>(), S.hist_ek.as<double>(), S.hist_ec.as<double>(),
+                       S.V, S.hist_P.as<double>(), S.hist_nb.as<int>());
   else if (S.flex())
     hipLaunchKernelGGL(k_nptg, dim3(1), dim3(256), 0, st, (int)nb,
                        (const double*)S.ke.as<double>(), S.nhc.as<double>(), S.M,
                        S.np.as<double>(), S.npt, dt, mode, (const double*)S.es.as<double>(),
                        (const Geom*)S.geom, S.counters.as<int>(), S.hist_ep.as<double>(),
                        S.hist_ek.as<double>(), S.hist_ec.as<double>(), S.hist_V.as<double>(),
-                       S.hist_P.as<double>());
+                       S.hist_P.as<double>(), S.hist_nb.as<int>());
   else if (S.thermo == 4)
     hipLaunchKernelGGL(k_npt, dim3(1), dim3(256), 0, st, (int)nb,
                        (const double*)S.ke.as<double>(), S.nhc.as<double>(), S.M,
                        S.np.as<double>(), S.npt, dt, mode, (const double*)S.es.as<double>(),
                        (const Geom*)S.geom, S.counters.as<int>(), S.hist_ep.as<double>(),
                        S.hist_ek.as<double>(), S.hist_ec.as<double>(), S.hist_V.as<double>(),
-                       S.hist_P.as<double>());
+                       S.hist_P.as<double>(), S.hist_nb.as<int>());
   else if (S.thermo == 2)
     hipLaunchKernelGGL(k_lang_rec, dim3(1), dim3(256), 0, st, (int)nb,
                        (const double*)S.ke.as<double>(), (const double*)S.ke_b.as<double>(),
                        (const double*)S.ke_a.as<double>(), mode, S.nhc.as<double>(),
                        (const double*)S.es.as<double>(), S.counters.as<int>(),
-                       S.hist_ep.as<double>(), S.hist_ek.as<double>(), S.hist_ec.as<double>());
+                       S.hist_ep.as<double>(), S.hist_ek.as<double>(), S.hist_ec.as<double>(),
+                       S.V, S.hist_P.as<double>(), S.hist_nb.as<int>());
 }
 
 // One step: [thermostat half-step;] drift (+ block displacement maxima) --
@@ -1228,7 +1270,8 @@ static void md_step(Potential& pot, NList& nl, SetDev& set, MDState& S, int n, d
                        (const double*)nl.disp_part.as<double>(), nl.r_list, nl.r_cut,
                        S.flag.as<int>(), S.counters.as<int>(),
                        S.thermo ? (double*)nullptr : S.hist_ep.as<double>(),
-                       S.hist_ek.as<double>(), S.hist_ec.as<double>());
+                       S.hist_ek.as<double>(), S.hist_ec.as<double>(), S.V,
+                       S.hist_P.as<double>(), S.hist_nb.as<int>());
   nl_build_gated(nl, set, S.pos.as<double>(), S.flag.as<int>(), st);
   if (S.every > 0)
     site_grads(pot, nl, set, S.pos.as<double>(), S.spc.as<int>(), S.F.as<double>(),
@@ -1242,6 +1285,12 @@ static void md_step(Potential& pot, NList& nl, SetDev& set, MDState& S, int n, d
     heat_flux(pot, nl, set, S.pos.as<double>(), S.vel.as<double>(), S.mass.as<double>(),
               S.esite.as<double>(), S.scaled() ? S.nhc.as<double>() : nullptr, kConv,
               S.counters.as<int>(), S.every, S.flux_part.as<double>(), S.hist_J.as<double>(), st);
+  if (S.frame_every > 0)
+    hipLaunchKernelGGL(k_frame, dim3((unsigned)((3 * (int64_t)n + 255) / 256)), dim3(256), 0,
+                       st, n, (const double*)S.pos.as<double>(), (const Geom*)S.geom,
+                       (const int*)S.counters.as<int>(), S.thermo ? 1 : 0, S.frame_every,
+                       S.n_frames, S.frames.as<double>(),
+                       S.thermo == 4 ? S.frame_lat.as<double>() : (double*)nullptr);
 }
 
 static void md_record(MDState& S, int n, hipStream_t st) {
@@ -1253,7 +1302,8 @@ static void md_record(MDState& S, int n, hipStream_t st) {
   hipLaunchKernelGGL(k_md_flag, dim3(1), dim3(256), 0, st, (int)nb,
                      (const double*)S.ke.as<double>(), (const double*)S.es.as<double>(),
                      (const double*)nullptr, 0.0, 0.0, (int*)nullptr, S.counters.as<int>(),
-                     S.hist_ep.as<double>(), S.hist_ek.as<double>(), S.hist_ec.as<double>());
+                     S.hist_ep.as<double>(), S.hist_ek.as<double>(), S.hist_ec.as<double>(), S.V,
+                     S.hist_P.as<double>(), S.hist_nb.as<int>());
 }
 
 void md_run(Potential& pot, int n, const double* lattice, const double* inv, double* pos,
@@ -1319,6 +1369,16 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
   S.hist_ep.ensure(sizeof(double) * (n_steps + 1));
   S.hist_ek.ensure(sizeof(double) * (n_steps + 1));
   S.hist_ec.ensure(sizeof(double) * (n_steps + 1));
+  S.hist_P.ensure(sizeof(double) * (n_steps + 1));
+  S.hist_nb.ensure(sizeof(int) * (n_steps + 1));
+  S.V = set.geoms[0].volume;
+  AK_REQUIRE(opt.frame_every >= 0, AK_ERR_ARG, "frame_every must be >= 0");
+  if (opt.frame_every > 0 && opt.frames) {
+    S.frame_every = opt.frame_every;
+    S.n_frames = n_steps / opt.frame_every;
+    S.frames.ensure(sizeof(double) * 3 * (size_t)n * std::max(S.n_frames, 1));
+    S.frame_lat.ensure(sizeof(double) * 9 * std::max(S.n_frames, 1));
+  }
   S.counters.ensure(sizeof(int) * 4);
   S.flag.ensure(sizeof(int));
   const int frames = S.every > 0 ? (n_steps + S.every - 1) / S.every : 0;
@@ -1409,7 +1469,6 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
       S.np.ensure(sizeof(double) * q.size());
       AK_HIP(hipMemcpy(S.np.p, q.data(), sizeof(double) * q.size(), hipMemcpyHostToDevice));
       S.hist_V.ensure(sizeof(double) * (n_steps + 1));
-      S.hist_P.ensure(sizeof(double) * (n_steps + 1));
     }
     AK_HIP(hipStreamSynchronize(st));
   } else if (S.thermo == 2 || S.thermo == 3) {
@@ -1564,8 +1623,13 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
     if (pressure)
       AK_HIP(hipMemcpy(pressure, S.hist_P.p, sizeof(double) * (n_steps + 1),
                        hipMemcpyDeviceToHost));
-  } else if (lattice_out) {
-    std::memcpy(lattice_out, lattice, 9 * sizeof(double));
+  } else {
+    if (lattice_out) std::memcpy(lattice_out, lattice, 9 * sizeof(double));
+    if (volume)
+      for (int k = 0; k <= n_steps; ++k) volume[k] = S.V;
+    if (pressure)
+      AK_HIP(hipMemcpy(pressure, S.hist_P.p, sizeof(double) * (n_steps + 1),
+                       hipMemcpyDeviceToHost));
   }
   AK_HIP(hipMemcpy(pos, S.pos.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
   AK_HIP(hipMemcpy(vel, S.vel.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
@@ -1580,7 +1644,22 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
                      hipMemcpyDeviceToHost));
   if (flux && frames > 0)
     AK_HIP(hipMemcpy(flux, S.hist_J.p, sizeof(double) * 3 * frames, hipMemcpyDeviceToHost));
+  if (S.frame_every > 0 && S.n_frames > 0) {
+    AK_HIP(hipMemcpy(opt.frames, S.frames.p, sizeof(double) * 3 * (size_t)n * S.n_frames,
+                     hipMemcpyDeviceToHost));
+    if (opt.frame_lattices) {
+      if (S.thermo == 4)
+        AK_HIP(hipMemcpy(opt.frame_lattices, S.frame_lat.p, sizeof(double) * 9 * S.n_frames,
+                         hipMemcpyDeviceToHost));
+      else
+        for (int f = 0; f < S.n_frames; ++f)
+          std::memcpy(opt.frame_lattices + 9 * (size_t)f, lattice, 9 * sizeof(double));
+    }
+  }
   if (n_rebuilds) *n_rebuilds = counters[1];
+  if (opt.rebuild_counts)
+    AK_HIP(hipMemcpy(opt.rebuild_counts, S.hist_nb.p, sizeof(int) * (n_steps + 1),
+                     hipMemcpyDeviceToHost));
 }
 
 }  // namespace ak

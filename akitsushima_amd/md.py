@@ -89,15 +89,15 @@ def masses_of(atom_types):
 
 def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_steps=1000,
            skin_width=0.3, masses=None, use_graph=True, thermostat=None, flux_every=0,
-           state=None, fix_atoms=None, velocity_rescale_steps=0):
+           state=None, fix_atoms=None, velocity_rescale_steps=0, frame_every=0):
     """n_steps of velocity Verlet on the device (``ak_md_run``).  thermostat:
     None (NVE), ("nhc", temperature K, tau fs, chain links), ("langevin",
     temperature, damping fs, seed), ("csvr", temperature, damping, seed) or
     ("npt", temperature, tau_nhc, chain links, pressure GPa, tau_baro fs[,
     cell_dof, baro_thermostat, seed]).  Returns a
-    dict with the final positions and velocities, e_pot, e_kin and e_cons
-    (E_tot, or H_NHC with a chain) per step (n_steps + 1, index 0 the
-    initial state), the number of list rebuilds, the loop's wall time and,
+    dict with the final positions and velocities, e_pot, e_kin, e_cons
+    (E_tot, H_NHC, E_tot - Q or H_NPT) and pressure_gpa (P_inst = 2 E_kin /
+    3V + Tr(sigma) / 3) per step (n_steps + 1, index 0 the initial state), the number of list rebuilds, the loop's wall time and,
     with flux_every > 0, the heat flux J (frames, 3) [eV/(A^2 fs)] of the
     state after every step s with s % flux_every == 0.  ``state`` (the
     "state" array a previous run returned, one of ``thermostat_state``, or
@@ -106,7 +106,10 @@ def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_step
     fix_atoms: (N, 3) bool, True = frozen Cartesian component (every MD
     class's fix_atoms, solvers/molecular_dynamics.py:248-267, 341-361);
     velocity_rescale_steps: direct rescaling to the target temperature after
-    each of the first that many steps (chain and NPT only)."""
+    each of the first that many steps (chain and NPT only).  frame_every > 0
+    returns "frames" (n_steps // frame_every, N, 3) and "frame_lattices", the
+    positions and cell after every frame_every steps, recorded on the
+    device."""
     lat = np.ascontiguousarray(lattice, float)
     pos = np.ascontiguousarray(positions, float).copy()
     vel = np.ascontiguousarray(velocities, float).copy()
@@ -159,6 +162,13 @@ def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_step
             raise ValueError(f"fix_atoms must have shape ({n}, 3), got {fa.shape}.")
         if fa.any():
             fix = np.ascontiguousarray(fa, dtype=np.uint8)
+    fe = int(frame_every)
+    if fe < 0:
+        raise ValueError("frame_every must be >= 0")
+    n_fr = n_steps // fe if fe else 0
+    fr = np.zeros((max(n_fr, 1), n, 3))
+    frl = np.zeros((max(n_fr, 1), 3, 3))
+    nbc = np.zeros(n_steps + 1, np.int32)
     nreb = ctypes.c_int32(0)
     secs = ctypes.c_double(0.0)
     P = _lib._pf64
@@ -169,19 +179,25 @@ def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_step
                           th.ctypes.data_as(P) if th is not None else None,
                           st.ctypes.data_as(P),
                           fix.ctypes.data_as(_lib._pu8) if fix is not None else None, every,
+                          fe, fr.ctypes.data_as(P) if fe else None,
+                          frl.ctypes.data_as(P) if fe else None,
                           ep.ctypes.data_as(P), ek.ctypes.data_as(P), ec.ctypes.data_as(P),
                           J.ctypes.data_as(P), lat_out.ctypes.data_as(P), vol.ctypes.data_as(P),
-                          prs.ctypes.data_as(P), ctypes.byref(nreb), ctypes.byref(secs)))
+                          prs.ctypes.data_as(P), nbc.ctypes.data_as(_lib._pi32),
+                          ctypes.byref(nreb), ctypes.byref(secs)))
     et = ep + ek
     out = {"positions": pos, "velocities": vel, "e_pot": ep, "e_kin": ek, "e_tot": et,
            "e_cons": ec, "drift": et - et[0], "n_rebuilds": int(nreb.value),
-           "seconds": float(secs.value), "state": st}
+           "seconds": float(secs.value), "state": st, "rebuild_counts": nbc}
     if every > 0:
         out["flux"] = J[:frames]
+    if fe:
+        out["frames"] = fr[:n_fr]
+        out["frame_lattices"] = frl[:n_fr]
+    out["pressure_gpa"] = prs * EV_ANG3_TO_GPA
     if th is not None and th[0] == 4.0:
         out["lattice"] = lat_out
         out["volume"] = vol
-        out["pressure_gpa"] = prs * EV_ANG3_TO_GPA
     return out
 
 

@@ -250,7 +250,15 @@ int ak_md_nve(ak_potential *pot, int32_t n_atoms, const double *lattice,
  * e_cons (n_steps + 1): E_tot (NVE), H_NHC (:947-969), E_tot - Q, Q the heat
  * a stochastic thermostat added (the E-Q column), or H_NPT (:2425-2454).
  * lattice_out (9): the final cell; volume, pressure (n_steps + 1): V [A^3]
- * and P_inst [eV/A^3] per step under NPT.
+ * and P_inst = 2 E_kin / 3V + Tr(sigma) / 3 [eV/A^3] per step (every kind;
+ * V is constant without a barostat).
+ * rebuild_counts (n_steps + 1, may be NULL): list rebuilds of this call up to
+ * each recorded state (the thermo log's NL_builds, less the initial build).
+ * frame_every > 0 with frames != NULL: the positions after every
+ * frame_every steps, (n_steps / frame_every, N, 3), recorded inside the
+ * captured step into device memory (the trajectory frames of traj_every,
+ * :128-129, 452-463), and in frame_lattices (n_steps / frame_every, 3, 3;
+ * may be NULL) the cell of each frame.
  * flux_every > 0: J [eV/(A^2 fs)] of the state after every step s with
  * s % flux_every == 0 into flux ((n_steps + flux_every - 1) / flux_every, 3),
  * the exact site-term heat flux of ak_heat_flux.  Any output may be NULL. */
@@ -258,9 +266,11 @@ int ak_md_run(ak_potential *pot, int32_t n_atoms, const double *lattice,
               const double *inv_lattice, double *positions, double *velocities,
               const double *masses, const int32_t *species, double dt, int32_t n_steps,
               double skin_width, int32_t use_graph, const double *thermostat10,
-              double *state, const uint8_t *fix_mask, int32_t flux_every, double *e_pot,
+              double *state, const uint8_t *fix_mask, int32_t flux_every,
+              int32_t frame_every, double *frames, double *frame_lattices, double *e_pot,
               double *e_kin, double *e_cons, double *flux, double *lattice_out,
-              double *volume, double *pressure, int32_t *n_rebuilds, double *seconds);
+              double *volume, double *pressure, int32_t *rebuild_counts,
+              int32_t *n_rebuilds, double *seconds);
 #define AK_MD_STATE_LEN 48
 
 /* ---- Site terms and heat flux ------------------------------------------ */
