@@ -826,7 +826,7 @@ __global__ __launch_bounds__(256) void k_gather(int n, const int* __restrict__ c
                                                 const int* __restrict__ first, int n_chunks,
                                                 const double* __restrict__ partial,
                                                 double* __restrict__ out,
-                                                const uint64_t* __restrict__ amask) {
+                                                const uint64_t* __restrict__ amask, int sym) {
   if ((int)blockIdx.x >= atom_blocks) {
     struct_reduce_block((int)blockIdx.x - atom_blocks, geoms, first, n_chunks, partial, out);
     return;
@@ -850,12 +850,15 @@ __global__ __launch_bounds__(256) void k_gather(int n, const int* __restrict__ c
       } else {
         // rows of stride L: slot r is entry r % L of atom r / L; only the
         // bonds inside r_c were written (the others are zero)
-        if ((amask[a] >> k) & 1) {
+        const bool own = (amask[a] >> k) & 1;
+        if (own) {
           fx = G[s];
           fy = G[plane + s];
           fz = G[2 * plane + s];
         }
-        if ((amask[r / L] >> (r & (L - 1))) & 1) {
+        // sym: the cutoff test is symmetric in the bond, so the reverse bond
+        // is active exactly when this one is and its mask need not be read
+        if (sym ? own : ((amask[r / L] >> (r & (L - 1))) & 1)) {
           fx -= G[rs];
           fy -= G[plane + rs];
           fz -= G[2 * plane + rs];
@@ -1025,7 +1028,7 @@ __global__ __launch_bounds__(256) void k_gather_c(int n, const int* __restrict__
                                                   const int* __restrict__ first, int n_chunks,
                                                   const double* __restrict__ partial,
                                                   double* __restrict__ out,
-                                                  const uint64_t* __restrict__ amask) {
+                                                  const uint64_t* __restrict__ amask, int sym) {
   if ((int)blockIdx.x >= atom_blocks) {
     struct_reduce_block((int)blockIdx.x - atom_blocks, geoms, first, n_chunks, partial, out);
     return;
@@ -1047,12 +1050,15 @@ __global__ __launch_bounds__(256) void k_gather_c(int n, const int* __restrict__
 #pragma unroll
     for (int i = 0; i < R; ++i) {
       const int k = q + LA * i;
-      rv[i] = k < c ? rev[(size_t)a * stride + k] : -1;
       own[i] = k < c && ((ma >> k) & 1);
+      // sym (cutoff symmetric in the bond): only active bonds have an active
+      // reverse, so only their reverse slots are read
+      rv[i] = (sym ? own[i] : k < c) ? rev[(size_t)a * stride + k] : -1;
     }
 #pragma unroll
     for (int i = 0; i < R; ++i)
-      oth[i] = rv[i] >= 0 && ((amask[rv[i] / L] >> (rv[i] & (L - 1))) & 1);
+      oth[i] = rv[i] >= 0 &&
+               (sym ? own[i] : (bool)((amask[rv[i] / L] >> (rv[i] & (L - 1))) & 1));
 #pragma unroll
     for (int i = 0; i < R; ++i) {
       if (rv[i] < 0) continue;
@@ -1729,22 +1735,26 @@ static void launch_bond_kernels(Potential& pot, NList& nl, SetDev& set, const do
   }
   const int* rev = nl.rev.as<int>();
   const int* first = ch.first.as<int>();
+  // Tersoff (r < r_c of the potential) and flat SW (r < a sigma) test a bond
+  // and its reverse with the same bitwise distance; multi-species SW legs
+  // have per-species cutoffs
+  const int sym = pot.sw_multi ? 0 : 1;
   if (pot.sw_multi) {
     constexpr int LA = L < kRowLanes ? L : kRowLanes;
     const int atom_blocks = (int)(((int64_t)n * LA + 255) / 256);
     hipLaunchKernelGGL((k_gather_c<L, LA>), dim3(atom_blocks + set.n_struct), dim3(256), 0, st,
                        n, cnt, rev, nl.stride, (const double*)G, d_F, atom_blocks, geoms, first,
-                       ch.n, (const double*)part, d_out, (const uint64_t*)amask);
+                       ch.n, (const double*)part, d_out, (const uint64_t*)amask, sym);
   } else if constexpr (L >= kLongRow) {
     const int atom_blocks = (int)(((int64_t)n * kRowLanes + 255) / 256);
     hipLaunchKernelGGL((k_gather_c<L, kRowLanes>), dim3(atom_blocks + set.n_struct), dim3(256),
                        0, st, n, cnt, rev, nl.stride, (const double*)G, d_F, atom_blocks, geoms,
-                       first, ch.n, (const double*)part, d_out, (const uint64_t*)amask);
+                       first, ch.n, (const double*)part, d_out, (const uint64_t*)amask, sym);
   } else {
     const int atom_blocks = (int)(((int64_t)n * L + 255) / 256);
     hipLaunchKernelGGL(k_gather<L>, dim3(atom_blocks + set.n_struct), dim3(256), 0, st, n, cnt,
                        rev, nl.stride, (const double*)G, d_F, atom_blocks, geoms, first, ch.n,
-                       (const double*)part, d_out, (const uint64_t*)amask);
+                       (const double*)part, d_out, (const uint64_t*)amask, sym);
   }
 }
 
