@@ -875,7 +875,8 @@ int ak_md_run(ak_potential* P, int32_t n, const double* lattice, const double* i
               const uint8_t* fix_mask, int32_t flux_every, int32_t frame_every, double* frames,
               double* frame_lattices, double* e_pot, double* e_kin, double* e_cons,
               double* flux, double* lattice_out, double* volume, double* pressure,
-              int32_t* rebuild_counts, int32_t* n_rebuilds, double* seconds) {
+              int32_t* rebuild_counts, double* cell_history, double* ptensor_history,
+              int32_t* n_rebuilds, double* seconds) {
   return guarded([&] {
     AK_REQUIRE(P && lattice && positions && velocities && masses && n > 0, AK_ERR_ARG,
                "bad arguments");
@@ -923,6 +924,8 @@ int ak_md_run(ak_potential* P, int32_t n, const double* lattice, const double* i
     opt.frames = frames;
     opt.frame_lattices = frame_lattices;
     opt.rebuild_counts = rebuild_counts;
+    opt.cell_history = cell_history;
+    opt.ptensor_history = ptensor_history;
     opt.flux_every = flux_every;
     int nr = 0;
     md_run(P->pot, n, lattice, inv, positions, velocities, masses, species, dt, n_steps, skin,

@@ -239,6 +239,10 @@ struct MDOptions {
   double* frame_lattices = nullptr;
   // list rebuilds of the run up to each recorded state (n_steps + 1)
   int32_t* rebuild_counts = nullptr;
+  // the cell (n_steps + 1, 9) and, for flexible NPT cells, the pressure
+  // tensor (n_steps + 1, 6: xx, yy, zz, yz, xz, xy; NaN otherwise) of each record
+  double* cell_history = nullptr;
+  double* ptensor_history = nullptr;
 };
 
 // Thermostat / barostat state carried between runs (in and out of md_run):

@@ -658,7 +658,7 @@ __global__ __launch_bounds__(256) void k_npt(int nb, const double* __restrict__ 
                                              double* __restrict__ e_cons,
                                              double* __restrict__ h_vol,
                                              double* __restrict__ h_press,
-    int* __restrict__ h_nb) {
+    int* __restrict__ h_nb, double* __restrict__ h_lat) {
   __shared__ double smem[16];
   double v[1] = {0.0};
   for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += ke_part[b];
@@ -713,6 +713,8 @@ __global__ __launch_bounds__(256) void k_npt(int nb, const double* __restrict__ 
   e_cons[step] = (ek + es[0] + p.P_ext * V + 0.5 * p.W * ed * ed + t + u + hb) - np[kNpHeat];
   h_vol[step] = V;
   h_press[step] = P;
+  if (h_lat)
+    for (int c = 0; c < 9; ++c) h_lat[9 * (size_t)step + c] = geom->lat[c];
   h_nb[step] = counters[1];
   counters[0] = step + 1;
 }
@@ -885,7 +887,7 @@ __global__ __launch_bounds__(256) void k_nptg(int nb, const double* __restrict__
                                               double* __restrict__ e_cons,
                                               double* __restrict__ h_vol,
                                               double* __restrict__ h_press,
-    int* __restrict__ h_nb) {
+    int* __restrict__ h_nb, double* __restrict__ h_lat, double* __restrict__ h_pt) {
   __shared__ double smem[16 * 7];
   double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int b = threadIdx.x; b < nb; b += blockDim.x)
@@ -979,6 +981,20 @@ __global__ __launch_bounds__(256) void k_nptg(int nb, const double* __restrict__
   e_cons[step] = (ek + es[0] + p.P_ext * V + 0.5 * pp / p.W + (t + u) + hb) - np[kNpHeat];
   h_vol[step] = V;
   h_press[step] = P;
+  if (h_lat)
+    for (int c = 0; c < 9; ++c) h_lat[9 * (size_t)step + c] = geom->lat[c];
+  if (h_pt) {
+    // pressure tensor of the recorded velocities (the pending scale applied):
+    // sum m v v^T / V + sym(sigma) as xx, yy, zz, yz, xz, xy (_pressure_tensor)
+    const double f2 = nhc[0] * nhc[0];
+    double* q = h_pt + 6 * (size_t)step;
+    q[0] = K[0] * f2 / V + es[1];
+    q[1] = K[1] * f2 / V + es[5];
+    q[2] = K[2] * f2 / V + es[9];
+    q[3] = K[3] * f2 / V + 0.5 * (es[6] + es[8]);
+    q[4] = K[4] * f2 / V + 0.5 * (es[3] + es[7]);
+    q[5] = K[5] * f2 / V + 0.5 * (es[2] + es[4]);
+  }
   h_nb[step] = counters[1];
   counters[0] = step + 1;
 }
@@ -1152,6 +1168,9 @@ struct MDState {
   // trajectory frames every frame_every steps (0 = none)
   int frame_every = 0, n_frames = 0;
   DevBuf frames, frame_lat;
+  // flexible-cell NPT: the cell and the pressure tensor of every record
+  bool cell_hist = false;
+  DevBuf hist_L, hist_Pt;
   const double* keep_ptr() const { return fixed ? keep.as<double>() : nullptr; }
   NptParams npt{};
   Geom* geom = nullptr;  // the set's device cell
@@ -1194,14 +1213,17 @@ static void thermo_kernel(MDState& S, int mode, double dt, hipStream_t st, unsig
                        S.np.as<double>(), S.npt, dt, mode, (const double*)S.es.as<double>(),
                        (const Geom*)S.geom, S.counters.as<int>(), S.hist_ep.as<double>(),
                        S.hist_ek.as<double>(), S.hist_ec.as<double>(), S.hist_V.as<double>(),
-                       S.hist_P.as<double>(), S.hist_nb.as<int>());
+                       S.hist_P.as<double>(), S.hist_nb.as<int>(),
+                       S.cell_hist ? S.hist_L.as<double>() : (double*)nullptr,
+                       S.cell_hist ? S.hist_Pt.as<double>() : (double*)nullptr);
   else if (S.thermo == 4)
     hipLaunchKernelGGL(k_npt, dim3(1), dim3(256), 0, st, (int)nb,
                        (const double*)S.ke.as<double>(), S.nhc.as<double>(), S.M,
                        S.np.as<double>(), S.npt, dt, mode, (const double*)S.es.as<double>(),
                        (const Geom*)S.geom, S.counters.as<int>(), S.hist_ep.as<double>(),
                        S.hist_ek.as<double>(), S.hist_ec.as<double>(), S.hist_V.as<double>(),
-                       S.hist_P.as<double>(), S.hist_nb.as<int>());
+                       S.hist_P.as<double>(), S.hist_nb.as<int>(),
+                       S.cell_hist ? S.hist_L.as<double>() : (double*)nullptr);
   else if (S.thermo == 2)
     hipLaunchKernelGGL(k_lang_rec, dim3(1), dim3(256), 0, st, (int)nb,
                        (const double*)S.ke.as<double>(), (const double*)S.ke_b.as<double>(),
@@ -1372,6 +1394,11 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
   S.hist_P.ensure(sizeof(double) * (n_steps + 1));
   S.hist_nb.ensure(sizeof(int) * (n_steps + 1));
   S.V = set.geoms[0].volume;
+  if (S.thermo == 4 && (opt.cell_history || opt.ptensor_history)) {
+    S.cell_hist = true;
+    S.hist_L.ensure(sizeof(double) * 9 * (size_t)(n_steps + 1));
+    S.hist_Pt.ensure(sizeof(double) * 6 * (size_t)(n_steps + 1));
+  }
   AK_REQUIRE(opt.frame_every >= 0, AK_ERR_ARG, "frame_every must be >= 0");
   if (opt.frame_every > 0 && opt.frames) {
     S.frame_every = opt.frame_every;
@@ -1659,6 +1686,19 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
   if (n_rebuilds) *n_rebuilds = counters[1];
   if (opt.rebuild_counts)
     AK_HIP(hipMemcpy(opt.rebuild_counts, S.hist_nb.p, sizeof(int) * (n_steps + 1),
+                     hipMemcpyDeviceToHost));
+  for (int k = 0; k <= n_steps; ++k) {
+    // a fixed cell on the host; the pressure tensor only for flexible cells
+    if (opt.cell_history && !S.cell_hist)
+      std::memcpy(opt.cell_history + 9 * (size_t)k, lattice, 9 * sizeof(double));
+    if (opt.ptensor_history && !S.flex())
+      for (int c = 0; c < 6; ++c) opt.ptensor_history[6 * (size_t)k + c] = NAN;
+  }
+  if (S.cell_hist && opt.cell_history)
+    AK_HIP(hipMemcpy(opt.cell_history, S.hist_L.p, sizeof(double) * 9 * (n_steps + 1),
+                     hipMemcpyDeviceToHost));
+  if (S.flex() && S.cell_hist && opt.ptensor_history)
+    AK_HIP(hipMemcpy(opt.ptensor_history, S.hist_Pt.p, sizeof(double) * 6 * (n_steps + 1),
                      hipMemcpyDeviceToHost));
 }
 

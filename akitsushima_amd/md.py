@@ -169,6 +169,8 @@ def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_step
     fr = np.zeros((max(n_fr, 1), n, 3))
     frl = np.zeros((max(n_fr, 1), 3, 3))
     nbc = np.zeros(n_steps + 1, np.int32)
+    lath = np.zeros((n_steps + 1, 3, 3))
+    pth = np.zeros((n_steps + 1, 6))
     nreb = ctypes.c_int32(0)
     secs = ctypes.c_double(0.0)
     P = _lib._pf64
@@ -184,6 +186,7 @@ def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_step
                           ep.ctypes.data_as(P), ek.ctypes.data_as(P), ec.ctypes.data_as(P),
                           J.ctypes.data_as(P), lat_out.ctypes.data_as(P), vol.ctypes.data_as(P),
                           prs.ctypes.data_as(P), nbc.ctypes.data_as(_lib._pi32),
+                          lath.ctypes.data_as(P), pth.ctypes.data_as(P),
                           ctypes.byref(nreb), ctypes.byref(secs)))
     et = ep + ek
     out = {"positions": pos, "velocities": vel, "e_pot": ep, "e_kin": ek, "e_tot": et,
@@ -198,6 +201,9 @@ def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_step
     if th is not None and th[0] == 4.0:
         out["lattice"] = lat_out
         out["volume"] = vol
+        out["lattices"] = lath
+        if th[7] != 0.0:
+            out["pressure_tensor_gpa"] = pth * EV_ANG3_TO_GPA
     return out
 
 

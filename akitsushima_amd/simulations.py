@@ -543,9 +543,8 @@ class NVESimulation(_DeviceMD):
 
 class NPTSimulation(_DeviceMD):
     """NPT with the MTTK barostat (:2141-3060) on the device: every cell_dof
-    and baro_thermostat of the reference.  thermo.log carries the isotropic
-    columns; the extra per-axis columns of the flexible cells are not
-    written."""
+    and baro_thermostat of the reference, with the per-axis thermo columns and
+    summary of the flexible cells."""
     _TASK = "npt"
     _ENSEMBLE = "npt"
     _POSCAR = "POSCAR_final_NPT"
@@ -582,9 +581,35 @@ class NPTSimulation(_DeviceMD):
     def _variable_cell(self):
         return True
 
+    # per-axis output of the flexible cells (_extra_cols, _extra_values,
+    # _extra_summary :3015-3054)
+    _P_KEYS = ("Pxx", "Pyy", "Pzz", "Pyz", "Pxz", "Pxy")
+
+    def _extra_cols(self):
+        if self.cell_dof == "tri":
+            return (*self._P_KEYS, "a", "b", "c", "alpha", "beta", "gamma")
+        return ("Pxx", "Pyy", "Pzz", "a", "b", "c")
+
+    def _extra_values(self, r, k):
+        L = r["lattices"][k]
+        out = dict(zip(self._P_KEYS, map(float, r["pressure_tensor_gpa"][k])))
+        for key, x in zip("abc", np.linalg.norm(L, axis=1)):
+            out[key] = float(x)
+
+        def ang(u, w):
+            return float(np.degrees(np.arccos(np.clip(
+                u @ w / (np.linalg.norm(u) * np.linalg.norm(w)), -1.0, 1.0))))
+        out.update(alpha=ang(L[1], L[2]), beta=ang(L[0], L[2]), gamma=ang(L[0], L[1]))
+        for i, x in enumerate(L.ravel()):
+            out[f"L{i}"] = float(x)
+        return out
+
     def _header(self):
-        return (f"{'Step':>8}  {'Time(ps)':>10}  {'E_pot(eV)':>14}  {'E_kin(eV)':>12}  "
-                f"{'T_inst(K)':>10}  {'P_inst(GPa)':>12}  {'V(Ang3)':>12}  {'H_NPT(eV)':>14}")
+        h = (f"{'Step':>8}  {'Time(ps)':>10}  {'E_pot(eV)':>14}  {'E_kin(eV)':>12}  "
+             f"{'T_inst(K)':>10}  {'P_inst(GPa)':>12}  {'V(Ang3)':>12}  {'H_NPT(eV)':>14}")
+        if self.cell_dof != "iso":
+            h += "".join(f"  {c:>10}" for c in self._extra_cols())
+        return h
 
     def _row(self, c, r, k, nb):
         t_ps = c * self.dt * 1e-3
@@ -593,17 +618,27 @@ class NPTSimulation(_DeviceMD):
         P, V, H = r["pressure_gpa"][k], r["volume"][k], r["e_cons"][k]
         text = (f"{c:8d}  {t_ps:10.4f}  {ep:14.6f}  {ek:12.6f}  {T:10.2f}  {P:12.4f}  "
                 f"{V:12.4f}  {H:14.6f}")
+        if self.cell_dof != "iso":
+            xv = self._extra_values(r, k)
+            text += "".join(f"  {xv[key]:10.4f}" for key in self._extra_cols())
         return text, {"step": c, "time_ps": t_ps, "e_pot": ep, "e_kin": ek, "T_inst": T,
                       "P_gpa": P, "V": V, "H_npt": H}
 
     def _list_keys(self):
-        return ("T", "P", "V", "e_pot", "e_kin", "H")
+        keys = ("T", "P", "V", "e_pot", "e_kin", "H")
+        if self.cell_dof != "iso":
+            keys += (*self._P_KEYS, "a", "b", "c", *(f"L{i}" for i in range(9)))
+        return keys
 
     def _collect(self, lists, r, k):
         super()._collect(lists, r, k)
         lists["P"].append(r["pressure_gpa"][k])
         lists["V"].append(r["volume"][k])
         lists["H"].append(r["e_cons"][k])
+        if self.cell_dof != "iso":
+            xv = self._extra_values(r, k)
+            for key in (*self._P_KEYS, "a", "b", "c", *(f"L{i}" for i in range(9))):
+                lists[key].append(xv[key])
 
     def _ck_scalars(self):
         f = state_fields(self._state)
@@ -623,11 +658,23 @@ class NPTSimulation(_DeviceMD):
         T, P, V, H = (np.array(lists[k]) for k in ("T", "P", "V", "H"))
         r = self._last
         h_ref = H[0] if len(H) else r["e_cons"][0]
-        return {"positions": self.positions.copy(), "velocities": self.velocities.copy(),
-                "lattice": self.lattice.copy(),
-                "T_mean": float(T.mean()) if len(T) else 0.0,
-                "T_std": float(T.std()) if len(T) else 0.0,
-                "P_mean_gpa": float(P.mean()) if len(P) else 0.0,
-                "P_std_gpa": float(P.std()) if len(P) else 0.0,
-                "V_mean": float(V.mean()) if len(V) else 0.0,
-                "H_npt_drift": float(r["e_cons"][-1] - h_ref)}
+        out = {"positions": self.positions.copy(), "velocities": self.velocities.copy(),
+               "lattice": self.lattice.copy(),
+               "T_mean": float(T.mean()) if len(T) else 0.0,
+               "T_std": float(T.std()) if len(T) else 0.0,
+               "P_mean_gpa": float(P.mean()) if len(P) else 0.0,
+               "P_std_gpa": float(P.std()) if len(P) else 0.0,
+               "V_mean": float(V.mean()) if len(V) else 0.0,
+               "H_npt_drift": float(r["e_cons"][-1] - h_ref)}
+        if self.cell_dof != "iso" and len(lists["a"]):
+            Pm = np.zeros((3, 3))
+            for key, (i, j) in zip(self._P_KEYS, ((0, 0), (1, 1), (2, 2), (1, 2), (0, 2),
+                                                   (0, 1))):
+                Pm[i, j] = Pm[j, i] = float(np.mean(lists[key]))
+            abc = np.array([lists["a"], lists["b"], lists["c"]])
+            out.update({"cell_dof": self.cell_dof,
+                        "lattice_mean": np.array([np.mean(lists[f"L{i}"])
+                                                  for i in range(9)]).reshape(3, 3),
+                        "P_tensor_mean_gpa": Pm, "abc_mean": abc.mean(axis=1),
+                        "abc_std": abc.std(axis=1)})
+        return out

@@ -254,6 +254,11 @@ int ak_md_nve(ak_potential *pot, int32_t n_atoms, const double *lattice,
  * V is constant without a barostat).
  * rebuild_counts (n_steps + 1, may be NULL): list rebuilds of this call up to
  * each recorded state (the thermo log's NL_builds, less the initial build).
+ * cell_history (n_steps + 1, 3, 3) and ptensor_history (n_steps + 1, 6 =
+ * xx, yy, zz, yz, xz, xy [eV/A^3]; may be NULL): the cell of each record and,
+ * under a flexible-cell NPT, the pressure tensor sum m v v^T / V + sym(sigma)
+ * of the recorded velocities (_pressure_tensor, _extra_values :2570-2577,
+ * 3027-3037; NaN for other kinds).
  * frame_every > 0 with frames != NULL: the positions after every
  * frame_every steps, (n_steps / frame_every, N, 3), recorded inside the
  * captured step into device memory (the trajectory frames of traj_every,
@@ -270,7 +275,8 @@ int ak_md_run(ak_potential *pot, int32_t n_atoms, const double *lattice,
               int32_t frame_every, double *frames, double *frame_lattices, double *e_pot,
               double *e_kin, double *e_cons, double *flux, double *lattice_out,
               double *volume, double *pressure, int32_t *rebuild_counts,
-              int32_t *n_rebuilds, double *seconds);
+              double *cell_history, double *ptensor_history, int32_t *n_rebuilds,
+              double *seconds);
 #define AK_MD_STATE_LEN 48
 
 /* ---- Site terms and heat flux ------------------------------------------ */

@@ -88,6 +88,11 @@ def test_npt_cell_modes_match_the_oracle(mode, baro, name):
     r = O.npt_run(force, lat, pos, vel, m, 1.0, n_steps, pot.r_cut, skin, 500.0, 40.0, 3, 0.5,
                   150.0, cell_dof=mode, baro=baro, seed=17)
     _compare(res, r, n_steps)
+    assert np.abs(res["lattices"] - r["lat"]).max() < 1e-10 * np.abs(r["lat"]).max()
+    if mode != "iso":
+        idx = ((0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1))
+        want = np.stack([r["ptensor"][:, i, j] for i, j in idx], axis=1)
+        assert np.abs(res["pressure_tensor_gpa"] - want).max() < 1e-8 * np.abs(want).max()
     L0, L = lat, res["lattice"]
     assert abs(res["volume"][-1] - res["volume"][0]) > 1e-4 * res["volume"][0]
     if mode == "z":

@@ -123,7 +123,12 @@ def test_nve_and_npt_runs_write_their_columns(tmp_path):
                         cell_dof="aniso", **_kw(tmp_path / "npt", traj_every=20))
     out = npt.run()
     rows = _rows(tmp_path / "npt" / "thermo.log")
+    hdr = open(tmp_path / "npt" / "thermo.log").readline().split()
+    assert hdr[-6:] == ["Pxx", "Pyy", "Pzz", "a", "b", "c"] and len(rows[-1]) == len(hdr)
     assert len(rows) == 7 and float(rows[-1][6]) == pytest.approx(out["V_mean"], rel=0.02)
+    assert float(rows[-1][-3]) == pytest.approx(np.linalg.norm(out["lattice"][0]), abs=1e-4)
+    assert out["cell_dof"] == "aniso" and out["P_tensor_mean_gpa"].shape == (3, 3)
+    assert np.allclose(out["abc_mean"], np.diag(out["lattice_mean"]), rtol=1e-12)
     xd = open(tmp_path / "npt" / "XDATCAR_npt").read()
     assert xd.count("Direct configuration") == 4 and xd.count("akitsushima.py NPT-MD") == 4
     assert not np.array_equal(out["lattice"], lat)
