@@ -32,6 +32,7 @@ _pi32 = ctypes.POINTER(ctypes.c_int32)
 _pi64 = ctypes.POINTER(ctypes.c_int64)
 _pf64 = ctypes.POINTER(ctypes.c_double)
 _pu8 = ctypes.POINTER(ctypes.c_uint8)
+_pf32 = ctypes.POINTER(ctypes.c_float)
 _handle = ctypes.c_void_p
 _phandle = ctypes.POINTER(ctypes.c_void_p)
 
@@ -71,7 +72,8 @@ SIGNATURES = {
     "ak_md_run": (ctypes.c_int, [_handle, _i32, _pf64, _pf64, _pf64, _pf64, _pf64, _pi32, _f64,
                                  _i32, _f64, _i32, _pf64, _pf64, _pu8, _i32, _i32, _pf64, _pf64,
                                  _pf64, _pf64, _pf64, _pf64, _pf64, _pf64, _pf64, _pi32, _pf64,
-                                 _pf64, _pi32, _pf64]),
+                                 _pf64, _pi32, _i32, _i32, _pf64, _i32, _pf64, _pf32, _pi32,
+                                 _pf64]),
     "ak_site_terms": (ctypes.c_int, [_handle, _handle, _i32, _pf64, _pf64, _pf64, _pi32, _pf64,
                                      _i64, _pi32, _pi32, _pf64, _pf64,
                                      ctypes.POINTER(ctypes.c_int64)]),

@@ -876,7 +876,9 @@ int ak_md_run(ak_potential* P, int32_t n, const double* lattice, const double* i
               double* frame_lattices, double* e_pot, double* e_kin, double* e_cons,
               double* flux, double* lattice_out, double* volume, double* pressure,
               int32_t* rebuild_counts, double* cell_history, double* ptensor_history,
-              int32_t* n_rebuilds, double* seconds) {
+              const int32_t* flux_labels, int32_t n_labels, int32_t region_axis,
+              const double* region_bounds, int32_t n_regions, double* flux_groups,
+              float* atom_flux, int32_t* n_rebuilds, double* seconds) {
   return guarded([&] {
     AK_REQUIRE(P && lattice && positions && velocities && masses && n > 0, AK_ERR_ARG,
                "bad arguments");
@@ -926,6 +928,13 @@ int ak_md_run(ak_potential* P, int32_t n, const double* lattice, const double* i
     opt.rebuild_counts = rebuild_counts;
     opt.cell_history = cell_history;
     opt.ptensor_history = ptensor_history;
+    opt.flux_labels = flux_labels;
+    opt.n_labels = n_labels;
+    opt.region_axis = region_axis;
+    opt.region_bounds = region_bounds;
+    opt.n_regions = n_regions;
+    opt.flux_groups = flux_groups;
+    opt.atom_flux = atom_flux;
     opt.flux_every = flux_every;
     int nr = 0;
     md_run(P->pot, n, lattice, inv, positions, velocities, masses, species, dt, n_steps, skin,

@@ -243,6 +243,18 @@ struct MDOptions {
   // tensor (n_steps + 1, 6: xx, yy, zz, yz, xz, xy; NaN otherwise) of each record
   double* cell_history = nullptr;
   double* ptensor_history = nullptr;
+  // heat flux per group of atoms (with flux_every): static labels (N, values
+  // below n_labels) and/or regions along region_axis between region_bounds
+  // (n_regions + 1); out (frames, n_labels * n_regions, 3)
+  const int32_t* flux_labels = nullptr;
+  int n_labels = 1;
+  int region_axis = -1;
+  const double* region_bounds = nullptr;
+  int n_regions = 1;
+  double* flux_groups = nullptr;
+  // per-atom heat currents J_a / V of every recorded frame, float32
+  // (frames, N, 3); may be given without flux_groups
+  float* atom_flux = nullptr;
 };
 
 // Thermostat / barostat state carried between runs (in and out of md_run):
@@ -268,13 +280,30 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
             double* e_kin, double* e_cons, double* flux, double* lattice_out, double* volume,
             double* pressure, int* n_rebuilds, double* seconds, hipStream_t st);
 
+// Per-group heat fluxes (device pointers): group of atom a = labels[a] (or
+// 0) * n_reg + its region along `axis` (-1: none, n_reg = 1) between
+// bounds[0..n_reg]; ng groups; part (grid * ng * 3) scratch; out
+// (frames, ng, 3); atoms (frames, n, 3, float32, may be null) the per-atom
+// currents / V.
+struct FluxGroups {
+  const int* labels = nullptr;
+  int axis = -1;
+  const double* bounds = nullptr;
+  int n_reg = 1;
+  int ng = 0;
+  double* part = nullptr;
+  double* out = nullptr;
+  float* atoms = nullptr;
+};
+
 // Heat flux J = [sum_a E_a v_a - sum_(a,k) v_ak (G[a,k] . v_nbr)] / V of one
 // cell after site_grads (d_part: one triple per 32 atoms).  With d_gate, only
-// on steps counters[2] % every == 0, into d_out[3 * (step / every)].
+// on steps counters[2] % every == 0, into d_out[3 * (step / every)]; with
+// groups, also the per-group fluxes of the per-atom heat currents.
 void heat_flux(Potential& pot, NList& nl, SetDev& set, const double* d_pos, const double* d_vel,
                const double* d_mass, const double* d_esite, const double* d_vscale,
                double conv, int* d_gate, int every, double* d_part, double* d_out,
-               hipStream_t st);
+               hipStream_t st, const FluxGroups* groups = nullptr);
 
 inline int pow2ceil(int x) {
   int p = 1;

@@ -2024,6 +2024,105 @@ __global__ __launch_bounds__(256) void k_heat_flux_final(int nb, const double* _
   }
 }
 
+// Per-atom heat currents summed per group (ThermalConductivityCalculator.
+// _per_atom_heat_current and _group_labels, solvers/thermal_conductivity.py:
+// 305-363).  Atom i carries E_i v_i and, for each neighbour entry (j -> i),
+// -vec_ji (dU_j/dvec_ji . v_i) = vec_ij (G[rev(i, k)] . v_i): the term goes to
+// the atom whose velocity it holds.  The group of i is labels[i] (or 0) times
+// n_reg plus its region: the fractional coordinate along `axis`, wrapped,
+// placed between bounds[0..n_reg] (searchsorted right - 1, clipped).  Per
+// block, one sum of J per group: part[(block * ng + g) * 3 + c].
+template <int LE>
+__global__ __launch_bounds__(256) void k_heat_flux_groups(
+    int n, const double* __restrict__ pos, const double* __restrict__ vel,
+    const double* __restrict__ mass, const double* __restrict__ esite,
+    const double* __restrict__ vscale, const int* __restrict__ sid,
+    const Geom* __restrict__ geoms, int single, double rc, const int* __restrict__ cnt,
+    const int* __restrict__ nbr, const int* __restrict__ rev, int stride,
+    const double* __restrict__ G, const uint64_t* __restrict__ amask, int masked, double conv,
+    const int* __restrict__ gate, int every, const int* __restrict__ labels, int axis,
+    const double* __restrict__ bounds, int n_reg, int ng, double* __restrict__ part,
+    float* __restrict__ atoms, double inv_volume) {
+  if (gate && gate[2] % every != 0) return;
+  __shared__ double smem[16 * 3];
+  const int a = (int)(blockIdx.x * blockDim.x + threadIdx.x) / LE;
+  const int l = threadIdx.x & (LE - 1);
+  double J[3] = {0.0, 0.0, 0.0};
+  int grp = -1;
+  if (a < n) {
+    const double sc = vscale ? vscale[0] : 1.0;
+    const Geom& g = geoms[single ? 0 : sid[a]];
+    const int nk = cnt[a];
+    const size_t plane = (size_t)n * stride;
+    const double vx = vel[3 * (size_t)a] * sc, vy = vel[3 * (size_t)a + 1] * sc,
+                 vz = vel[3 * (size_t)a + 2] * sc;
+    for (int k = l; k < nk; k += LE) {
+      const size_t slot = (size_t)a * stride + k;
+      const int r = rev[slot];
+      if (r < 0) continue;
+      if (masked && !((amask[r / stride] >> (r % stride)) & 1)) continue;
+      double ux, uy, uz;
+      mic_fast(g, pos, a, nbr[slot], ux, uy, uz);
+      const double r2 = ux * ux + uy * uy + uz * uz;
+      if (!(r2 < rc * rc) || !(r2 > 1e-24)) continue;
+      const double gv = G[r] * vx + G[plane + r] * vy + G[2 * plane + r] * vz;
+      J[0] += ux * gv;
+      J[1] += uy * gv;
+      J[2] += uz * gv;
+    }
+    if (l == 0) {
+      const double E = 0.5 * mass[a] * (vx * vx + vy * vy + vz * vz) / conv + esite[a];
+      J[0] += E * vx;
+      J[1] += E * vy;
+      J[2] += E * vz;
+    }
+    int rg = 0;
+    if (axis >= 0) {
+      double f = pos[3 * (size_t)a] * g.inv[axis] + pos[3 * (size_t)a + 1] * g.inv[3 + axis] +
+                 pos[3 * (size_t)a + 2] * g.inv[6 + axis];
+      f -= floor(f);
+      for (int k = 1; k < n_reg; ++k)
+        if (bounds[k] <= f) rg = k;
+    }
+    grp = (labels ? labels[a] : 0) * n_reg + rg;
+  }
+  for (int c = 0; c < 3; ++c) J[c] = seg_sum<LE>(J[c]);
+  if (atoms && a < n && l == 0) {
+    const size_t f = gate ? (size_t)(gate[2] / every) : 0;
+    for (int c = 0; c < 3; ++c) atoms[(f * n + a) * 3 + c] = (float)(J[c] * inv_volume);
+  }
+  for (int q = 0; q < ng; ++q) {
+    double w[3] = {0.0, 0.0, 0.0};
+    if (grp == q && l == 0) {
+      w[0] = J[0];
+      w[1] = J[1];
+      w[2] = J[2];
+    }
+    block_sum<3>(w, smem);
+    if (threadIdx.x == 0)
+      for (int c = 0; c < 3; ++c) part[((size_t)blockIdx.x * ng + q) * 3 + c] = w[c];
+  }
+}
+
+// Group fluxes of a record step: (sum of the block sums, fixed order) / V
+// into out[(frame * ng + g) * 3 + c]; the gate is read, not advanced.
+__global__ __launch_bounds__(256) void k_heat_flux_groups_final(
+    int nb, int ng, const double* __restrict__ part, double inv_volume,
+    const int* __restrict__ gate, int every, double* __restrict__ out) {
+  __shared__ double smem[16 * 3];
+  const int step = gate ? gate[2] : 0;
+  if (gate && step % every != 0) return;
+  const size_t f = gate ? (size_t)(step / every) : 0;
+  for (int q = 0; q < ng; ++q) {
+    double J[3] = {0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+      for (int c = 0; c < 3; ++c) J[c] += part[((size_t)b * ng + q) * 3 + c];
+    block_sum<3>(J, smem);
+    if (threadIdx.x == 0)
+      for (int c = 0; c < 3; ++c) out[(f * ng + q) * 3 + c] = J[c] * inv_volume;
+  }
+}
+
 bool site_masked(const Potential& pot, const NList& nl) {
   return pot.kind != KIND_EAM && (nl.stride >= 8 || pot.sw_multi);
 }
@@ -2047,10 +2146,25 @@ void site_grads(Potential& pot, NList& nl, SetDev& set, const double* d_pos, con
 void heat_flux(Potential& pot, NList& nl, SetDev& set, const double* d_pos, const double* d_vel,
                const double* d_mass, const double* d_esite, const double* d_vscale,
                double conv, int* d_gate, int every, double* d_part, double* d_out,
-               hipStream_t st) {
+               hipStream_t st, const FluxGroups* groups) {
   const int n = set.n_atoms;
   constexpr int LE = 8;
   const unsigned grid = (unsigned)(((int64_t)n * LE + 255) / 256);
+  if (groups && groups->ng > 0) {
+    hipLaunchKernelGGL(k_heat_flux_groups<LE>, dim3(grid), dim3(256), 0, st, n, d_pos, d_vel,
+                       d_mass, d_esite, d_vscale, (const int*)set.d_sid.as<int>(),
+                       (const Geom*)set.d_geoms.as<Geom>(), set.n_struct == 1 ? 1 : 0,
+                       pot.r_cut, (const int*)nl.cnt.as<int>(), (const int*)nl.nbr.as<int>(),
+                       (const int*)nl.rev.as<int>(), nl.stride,
+                       (const double*)pot.G.as<double>(),
+                       (const uint64_t*)pot.amask.as<uint64_t>(), site_masked(pot, nl) ? 1 : 0,
+                       conv, (const int*)d_gate, every, groups->labels, groups->axis,
+                       groups->bounds, groups->n_reg, groups->ng, groups->part, groups->atoms,
+                       1.0 / set.geoms[0].volume);
+    hipLaunchKernelGGL(k_heat_flux_groups_final, dim3(1), dim3(256), 0, st, (int)grid,
+                       groups->ng, (const double*)groups->part, 1.0 / set.geoms[0].volume,
+                       (const int*)d_gate, every, groups->out);
+  }
   hipLaunchKernelGGL(k_heat_flux<LE>, dim3(grid), dim3(256), 0, st, n, d_pos, d_vel, d_mass,
                      d_esite, d_vscale, (const int*)set.d_sid.as<int>(),
                      (const Geom*)set.d_geoms.as<Geom>(), set.n_struct == 1 ? 1 : 0, pot.r_cut,

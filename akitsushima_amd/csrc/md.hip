@@ -1171,6 +1171,9 @@ struct MDState {
   // flexible-cell NPT: the cell and the pressure tensor of every record
   bool cell_hist = false;
   DevBuf hist_L, hist_Pt;
+  // heat flux per group of atoms
+  FluxGroups fg;
+  DevBuf fg_labels, fg_bounds, fg_part, hist_Jg, hist_Ja;
   const double* keep_ptr() const { return fixed ? keep.as<double>() : nullptr; }
   NptParams npt{};
   Geom* geom = nullptr;  // the set's device cell
@@ -1306,7 +1309,8 @@ static void md_step(Potential& pot, NList& nl, SetDev& set, MDState& S, int n, d
   if (S.every > 0)
     heat_flux(pot, nl, set, S.pos.as<double>(), S.vel.as<double>(), S.mass.as<double>(),
               S.esite.as<double>(), S.scaled() ? S.nhc.as<double>() : nullptr, kConv,
-              S.counters.as<int>(), S.every, S.flux_part.as<double>(), S.hist_J.as<double>(), st);
+              S.counters.as<int>(), S.every, S.flux_part.as<double>(), S.hist_J.as<double>(), st,
+              S.fg.ng > 0 ? &S.fg : nullptr);
   if (S.frame_every > 0)
     hipLaunchKernelGGL(k_frame, dim3((unsigned)((3 * (int64_t)n + 255) / 256)), dim3(256), 0,
                        st, n, (const double*)S.pos.as<double>(), (const Geom*)S.geom,
@@ -1409,6 +1413,52 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
   S.counters.ensure(sizeof(int) * 4);
   S.flag.ensure(sizeof(int));
   const int frames = S.every > 0 ? (n_steps + S.every - 1) / S.every : 0;
+  if (S.every > 0 && (opt.flux_groups || opt.atom_flux)) {
+    // groups of the per-atom heat currents (ThermalConductivityCalculator.run_md
+    // decompose, solvers/thermal_conductivity.py:339-363)
+    AK_REQUIRE(opt.n_labels >= 1 && opt.n_regions >= 1 &&
+                   (long long)opt.n_labels * opt.n_regions <= 1024,
+               AK_ERR_ARG, "heat-flux groups: 1 to 1024 groups (labels x regions)");
+    AK_REQUIRE(opt.region_axis >= -1 && opt.region_axis <= 2, AK_ERR_ARG,
+               "heat-flux regions: axis must be 0, 1, 2 (or -1 for none)");
+    AK_REQUIRE(opt.region_axis >= 0 || opt.n_regions == 1, AK_ERR_ARG,
+               "heat-flux regions need an axis");
+    S.fg.ng = opt.n_labels * opt.n_regions;
+    S.fg.n_reg = opt.n_regions;
+    S.fg.axis = opt.region_axis;
+    if (opt.flux_labels) {
+      for (int a = 0; a < n; ++a)
+        AK_REQUIRE(opt.flux_labels[a] >= 0 && opt.flux_labels[a] < opt.n_labels, AK_ERR_ARG,
+                   "heat-flux group labels must lie in [0, n_labels)");
+      S.fg_labels.ensure(sizeof(int) * n);
+      AK_HIP(hipMemcpy(S.fg_labels.p, opt.flux_labels, sizeof(int) * n, hipMemcpyHostToDevice));
+      S.fg.labels = S.fg_labels.as<int>();
+    } else {
+      AK_REQUIRE(opt.n_labels == 1, AK_ERR_ARG, "heat-flux groups: n_labels > 1 needs labels");
+    }
+    if (opt.region_axis >= 0) {
+      AK_REQUIRE(opt.region_bounds != nullptr, AK_ERR_ARG, "heat-flux regions need bounds");
+      for (int k = 0; k <= opt.n_regions; ++k)
+        AK_REQUIRE(std::isfinite(opt.region_bounds[k]) &&
+                       (k == 0 || opt.region_bounds[k] >= opt.region_bounds[k - 1]),
+                   AK_ERR_ARG, "heat-flux region bounds must be finite and ascending");
+      S.fg_bounds.ensure(sizeof(double) * (opt.n_regions + 1));
+      AK_HIP(hipMemcpy(S.fg_bounds.p, opt.region_bounds, sizeof(double) * (opt.n_regions + 1),
+                       hipMemcpyHostToDevice));
+      S.fg.bounds = S.fg_bounds.as<double>();
+    }
+    const size_t grid = ((size_t)n * 8 + 255) / 256;
+    S.fg_part.ensure(sizeof(double) * 3 * grid * S.fg.ng);
+    S.fg.part = S.fg_part.as<double>();
+    const int fr = (n_steps + S.every - 1) / S.every;
+    S.hist_Jg.ensure(sizeof(double) * 3 * (size_t)std::max(fr, 1) * S.fg.ng);
+    S.fg.out = S.hist_Jg.as<double>();
+    S.fg.atoms = nullptr;
+    if (opt.atom_flux) {
+      S.hist_Ja.ensure(sizeof(float) * 3 * (size_t)n * std::max(fr, 1));
+      S.fg.atoms = S.hist_Ja.as<float>();
+    }
+  }
   if (S.every > 0) {
     S.esite.ensure(sizeof(double) * n);
     S.flux_part.ensure(sizeof(double) * 3 * ((size_t)n / 32 + 1));
@@ -1671,6 +1721,12 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
                      hipMemcpyDeviceToHost));
   if (flux && frames > 0)
     AK_HIP(hipMemcpy(flux, S.hist_J.p, sizeof(double) * 3 * frames, hipMemcpyDeviceToHost));
+  if (opt.flux_groups && S.fg.ng > 0 && frames > 0)
+    AK_HIP(hipMemcpy(opt.flux_groups, S.hist_Jg.p, sizeof(double) * 3 * frames * S.fg.ng,
+                     hipMemcpyDeviceToHost));
+  if (opt.atom_flux && S.fg.atoms && frames > 0)
+    AK_HIP(hipMemcpy(opt.atom_flux, S.hist_Ja.p, sizeof(float) * 3 * (size_t)n * frames,
+                     hipMemcpyDeviceToHost));
   if (S.frame_every > 0 && S.n_frames > 0) {
     AK_HIP(hipMemcpy(opt.frames, S.frames.p, sizeof(double) * 3 * (size_t)n * S.n_frames,
                      hipMemcpyDeviceToHost));

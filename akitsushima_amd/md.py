@@ -89,7 +89,8 @@ def masses_of(atom_types):
 
 def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_steps=1000,
            skin_width=0.3, masses=None, use_graph=True, thermostat=None, flux_every=0,
-           state=None, fix_atoms=None, velocity_rescale_steps=0, frame_every=0):
+           state=None, fix_atoms=None, velocity_rescale_steps=0, frame_every=0,
+           flux_groups=None, atom_flux=False):
     """n_steps of velocity Verlet on the device (``ak_md_run``).  thermostat:
     None (NVE), ("nhc", temperature K, tau fs, chain links), ("langevin",
     temperature, damping fs, seed), ("csvr", temperature, damping, seed) or
@@ -109,7 +110,12 @@ def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_step
     each of the first that many steps (chain and NPT only).  frame_every > 0
     returns "frames" (n_steps // frame_every, N, 3) and "frame_lattices", the
     positions and cell after every frame_every steps, recorded on the
-    device."""
+    device.  flux_groups = (labels (N,) or None, n_labels, (axis, bounds) or
+    None) also records "flux_groups" (frames, n_labels, n_regions, 3), the
+    heat flux of each group of atoms (per-atom heat currents; labels times
+    regions along a cell axis between fractional bounds).  atom_flux=True
+    also returns "atom_flux" (frames, N, 3) float32, the per-atom heat
+    currents J_i / V of every frame."""
     lat = np.ascontiguousarray(lattice, float)
     pos = np.ascontiguousarray(positions, float).copy()
     vel = np.ascontiguousarray(velocities, float).copy()
@@ -168,6 +174,26 @@ def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_step
     n_fr = n_steps // fe if fe else 0
     fr = np.zeros((max(n_fr, 1), n, 3))
     frl = np.zeros((max(n_fr, 1), 3, 3))
+    glab, n_lab, g_axis, gb, n_reg, gJ = None, 1, -1, None, 1, None
+    if flux_groups is not None:
+        if every <= 0:
+            raise ValueError("flux_groups needs flux_every > 0")
+        labels, n_lab, region = flux_groups
+        n_lab = int(n_lab)
+        if labels is not None:
+            glab = np.ascontiguousarray(labels, np.int32)
+            if glab.shape != (n,):
+                raise ValueError("flux_groups labels must have one entry per atom")
+        if region is not None:
+            g_axis = int(region[0])
+            gb = np.ascontiguousarray(region[1], float)
+            n_reg = len(gb) - 1
+        gJ = np.zeros((max(frames, 1), n_lab * n_reg, 3))
+    aJ = None
+    if atom_flux:
+        if every <= 0:
+            raise ValueError("atom_flux needs flux_every > 0")
+        aJ = np.zeros((max(frames, 1), n, 3), np.float32)
     nbc = np.zeros(n_steps + 1, np.int32)
     lath = np.zeros((n_steps + 1, 3, 3))
     pth = np.zeros((n_steps + 1, 6))
@@ -187,6 +213,10 @@ def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_step
                           J.ctypes.data_as(P), lat_out.ctypes.data_as(P), vol.ctypes.data_as(P),
                           prs.ctypes.data_as(P), nbc.ctypes.data_as(_lib._pi32),
                           lath.ctypes.data_as(P), pth.ctypes.data_as(P),
+                          glab.ctypes.data_as(_lib._pi32) if glab is not None else None,
+                          n_lab, g_axis, gb.ctypes.data_as(P) if gb is not None else None, n_reg,
+                          gJ.ctypes.data_as(P) if gJ is not None else None,
+                          aJ.ctypes.data_as(_lib._pf32) if aJ is not None else None,
                           ctypes.byref(nreb), ctypes.byref(secs)))
     et = ep + ek
     out = {"positions": pos, "velocities": vel, "e_pot": ep, "e_kin": ek, "e_tot": et,
@@ -194,6 +224,10 @@ def md_run(potential, lattice, positions, velocities, atom_types, dt=1.0, n_step
            "seconds": float(secs.value), "state": st, "rebuild_counts": nbc}
     if every > 0:
         out["flux"] = J[:frames]
+        if gJ is not None:
+            out["flux_groups"] = gJ[:frames].reshape(frames, n_lab, n_reg, 3)
+        if aJ is not None:
+            out["atom_flux"] = aJ[:frames]
     if fe:
         out["frames"] = fr[:n_fr]
         out["frame_lattices"] = frl[:n_fr]

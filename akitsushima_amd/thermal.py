@@ -87,15 +87,27 @@ class ThermalConductivityCalculator:
         self._hcacf = self._tau_axis = self._kappa = self._kappa_run = self._kappa_std = None
 
     def run_md(self, n_steps=100_000, timestep=0.5, equil_steps=10_000, velocities=None,
-               heat_flux_every=5, output_dir=None, seed=42, tau_nhc=100.0, nhc_chain=3,
-               skin_width=0.5, verbose=False):
+               heat_flux_every=5, traj_every=100, thermo_every=1000, output_dir=None, seed=42,
+               verbose=False, decompose=None, *, tau_nhc=100.0, nhc_chain=3, skin_width=0.5):
         """NVT (Nose-Hoover chain) equilibration for equil_steps, then NVE
         for n_steps recording J after every step s with s %
         heat_flux_every == 0.  Without ``velocities`` the start is a
-        Maxwell-Boltzmann draw at the temperature from ``seed``.  Returns
-        {"J_series" (frames, 3) [eV/(A^2 fs)], "dt_hf" fs, "positions",
-        "velocities", "e_tot" (NVE), "equil" (the NVT run's record or
-        None)}."""
+        Maxwell-Boltzmann draw at the temperature from ``seed``.
+        ``decompose`` = {"species": bool, "regions": {"axis": "a"|"b"|"c",
+        "bounds": [fractional edges]}, "save_atom_flux": bool} also records
+        the heat flux of each group of atoms (per-atom heat currents,
+        regions by each frame's positions) for compute_kappa_decomposition;
+        save_atom_flux writes the per-atom currents J_i / V of every frame,
+        float32, to output_dir/atom_heat_flux.npy (frames, N, 3).  traj_every and
+        thermo_every are accepted for the reference's signature; the device
+        run writes heat_flux.dat only.  Returns {"J_series" (frames, 3)
+        [eV/(A^2 fs)], "dt_hf" fs, "positions", "velocities", "e_tot"
+        (NVE), "equil" (the NVT run's record or None)}."""
+        decompose = dict(decompose or {})
+        save_atoms = bool(decompose.get("save_atom_flux"))
+        if save_atoms and not output_dir:
+            raise ValueError("run_md: save_atom_flux writes atom_heat_flux.npy and needs "
+                             "output_dir")
         pos = self.positions.copy()
         vel = (np.asarray(velocities, float).copy() if velocities is not None
                else maxwell_boltzmann(self.temperature, self.masses, seed))
@@ -105,11 +117,35 @@ class ThermalConductivityCalculator:
                          self.temperature, dt=timestep, n_steps=equil_steps, tau_nhc=tau_nhc,
                          nhc_chain=nhc_chain, skin_width=skin_width, masses=self.masses)
             pos, vel = eq["positions"], eq["velocities"]
+        groups = None
+        species_names = region_names = None
+        if decompose.get("species") or decompose.get("regions"):
+            labels, n_lab, region = None, 1, None
+            if decompose.get("species"):
+                species_names = sorted(set(self.atom_types), key=self.atom_types.index)
+                labels = np.array([species_names.index(t) for t in self.atom_types])
+                n_lab = len(species_names)
+            reg = decompose.get("regions")
+            if reg:
+                ax = {"a": 0, "b": 1, "c": 2}.get(str(reg.get("axis", "c")), reg.get("axis"))
+                bounds = np.asarray(reg["bounds"], dtype=float)
+                region = (int(ax), bounds)
+                region_names = [f"[{bounds[k]:.3f},{bounds[k + 1]:.3f})"
+                                for k in range(len(bounds) - 1)]
+            groups = (labels, n_lab, region)
         prod = md_run(self.potential, self.lattice, pos, vel, self.atom_types, dt=timestep,
                       n_steps=n_steps, skin_width=skin_width, masses=self.masses,
-                      flux_every=int(heat_flux_every))
+                      flux_every=int(heat_flux_every), flux_groups=groups,
+                      atom_flux=save_atoms)
         J = prod["flux"]
         self._J_series = J
+        self._J_groups = {}
+        if groups is not None:
+            Jg = prod["flux_groups"]                  # (frames, labels, regions, 3)
+            if species_names is not None:
+                self._J_groups["species"] = (species_names, Jg.sum(axis=2))
+            if region_names is not None:
+                self._J_groups["regions"] = (region_names, Jg.sum(axis=1))
         self._dt_hf = float(timestep * heat_flux_every)
         if output_dir:
             os.makedirs(output_dir, exist_ok=True)
@@ -117,6 +153,8 @@ class ThermalConductivityCalculator:
                 fh.write("step  Jx_eV_Ang2_fs  Jy  Jz\n")
                 for k, j in enumerate(J):
                     fh.write(f"{k * heat_flux_every + 1}  {j[0]:.8e}  {j[1]:.8e}  {j[2]:.8e}\n")
+            if save_atoms:
+                np.save(os.path.join(output_dir, "atom_heat_flux.npy"), prod["atom_flux"])
         if verbose:
             print(f"  NVE production: {n_steps} steps, {len(J)} heat-flux frames, "
                   f"{prod['seconds']:.2f} s on the device")
@@ -166,3 +204,43 @@ class ThermalConductivityCalculator:
             print(f"  Green-Kubo (T={T:.0f} K, tau_max={tau_max:.1f} ps): k = {d} W/(m K), "
                   f"mean {d.mean():.3f}")
         return kappa
+
+    def compute_kappa_decomposition(self, tau_max=20.0, verbose=True):
+        """Group contributions to the Green-Kubo tensor
+        (compute_kappa_decomposition, solvers/thermal_conductivity.py:
+        832-883): kappa_g,ab = V / (k_B T^2) int <J_g,a(0) J_b(tau)> dt with
+        J_g the group's flux; linear in J_g, so the groups of one
+        decomposition add up to the total.  Needs run_md(decompose=...).
+        Returns {decomposition: {group: (3, 3) W/(m K)}, "total": (3, 3),
+        "sum_rule_residual": {decomposition: max |sum_g kappa_g - kappa|}}."""
+        if not getattr(self, "_J_groups", None):
+            raise RuntimeError("compute_kappa_decomposition: run run_md(decompose=...) first.")
+        J = self._J_series
+        dt = float(self._dt_hf)
+        V = abs(np.linalg.det(self.lattice))
+        n_lag = min(int(tau_max * 1e3 / dt), len(J) // 2)
+        pref = V / (K_B * self.temperature ** 2) * EV_ANG_K_FS_TO_W_MK
+
+        def integral(A, B):
+            k = np.zeros((3, 3))
+            for a in range(3):
+                for b in range(3):
+                    c = autocorr_fft(A[:, a], B[:, b], n_lag)
+                    k[a, b] = 0.5 * dt * np.sum(c[1:] + c[:-1])
+            return k * pref
+
+        total = integral(J, J)
+        out, resid = {}, {}
+        for name, (names, G) in self._J_groups.items():
+            out[name] = {g: integral(G[:, k], J) for k, g in enumerate(names)}
+            resid[name] = float(np.max(np.abs(sum(out[name].values()) - total)))
+        out["sum_rule_residual"] = resid
+        out["total"] = total
+        if verbose:
+            for name, parts in out.items():
+                if name in ("sum_rule_residual", "total"):
+                    continue
+                print(f"\n  kappa by {name} (W/m/K; xx yy zz):")
+                for g, k in parts.items():
+                    print(f"    {g:>18s} : {k[0, 0]:9.4f} {k[1, 1]:9.4f} {k[2, 2]:9.4f}")
+        return out

@@ -259,6 +259,18 @@ int ak_md_nve(ak_potential *pot, int32_t n_atoms, const double *lattice,
  * under a flexible-cell NPT, the pressure tensor sum m v v^T / V + sym(sigma)
  * of the recorded velocities (_pressure_tensor, _extra_values :2570-2577,
  * 3027-3037; NaN for other kinds).
+ * flux_groups (frames, n_labels * n_regions, 3; NULL = none; needs
+ * flux_every > 0): the heat flux of groups of atoms, from the per-atom heat
+ * currents (each entry's virial term to the atom whose velocity it carries;
+ * ThermalConductivityCalculator._per_atom_heat_current and _group_labels,
+ * solvers/thermal_conductivity.py:305-363); atom a belongs to group
+ * flux_labels[a] * n_regions + its region (flux_labels may be NULL with
+ * n_labels = 1): region_axis 0/1/2 (-1 none, n_regions = 1) and the
+ * fractional coordinate, wrapped, between region_bounds[0..n_regions]
+ * (searchsorted right - 1, clipped).  The groups sum to flux.
+ * atom_flux (frames, N, 3, float32; NULL = none; needs flux_every > 0): the
+ * per-atom heat currents J_a / V of every recorded frame, rounded to float32
+ * (run_md's save_atom_flux, atom_heat_flux.npy, :551-558, 637-645).
  * frame_every > 0 with frames != NULL: the positions after every
  * frame_every steps, (n_steps / frame_every, N, 3), recorded inside the
  * captured step into device memory (the trajectory frames of traj_every,
@@ -275,7 +287,9 @@ int ak_md_run(ak_potential *pot, int32_t n_atoms, const double *lattice,
               int32_t frame_every, double *frames, double *frame_lattices, double *e_pot,
               double *e_kin, double *e_cons, double *flux, double *lattice_out,
               double *volume, double *pressure, int32_t *rebuild_counts,
-              double *cell_history, double *ptensor_history, int32_t *n_rebuilds,
+              double *cell_history, double *ptensor_history, const int32_t *flux_labels,
+              int32_t n_labels, int32_t region_axis, const double *region_bounds,
+              int32_t n_regions, double *flux_groups, float *atom_flux, int32_t *n_rebuilds,
               double *seconds);
 #define AK_MD_STATE_LEN 48
 

@@ -51,6 +51,39 @@ class Terms:
         vir = -(c[:, None] * self.vec).sum(axis=0)
         return (conv + vir) / volume
 
+    def atom_heat_current(self, velocities, masses):
+        """Per-atom heat current J_i [eV A/fs] with sum_i J_i = V J
+        (ThermalConductivityCalculator._per_atom_heat_current,
+        solvers/thermal_conductivity.py:305-337): E_i v_i, and each entry's
+        term -vec_m (dU/dv_m . v_nbr) to the atom whose velocity it carries."""
+        v = np.asarray(velocities, float)
+        E = 0.5 * masses * np.sum(v * v, axis=1) / EV_ANG_TO_AMU_ANG_FS2 + self.e_site
+        Ji = E[:, None] * v
+        c = np.sum(self.dUdv * v[self.nbr], axis=1)
+        np.add.at(Ji, self.nbr, -c[:, None] * self.vec)
+        return Ji
+
+
+def group_labels(lattice, positions, atom_types, decompose):
+    """{name: (labels (N,), names)} of a decomposition (_group_labels,
+    solvers/thermal_conductivity.py:339-363): "species" in order of first
+    appearance; "regions" by the fractional coordinate along axis a/b/c
+    between the bounds (searchsorted right - 1, clipped)."""
+    out = {}
+    if decompose.get("species"):
+        names = sorted(set(atom_types), key=list(atom_types).index)
+        out["species"] = (np.array([names.index(t) for t in atom_types]), names)
+    reg = decompose.get("regions")
+    if reg:
+        ax = {"a": 0, "b": 1, "c": 2}.get(str(reg.get("axis", "c")), reg.get("axis"))
+        bounds = np.asarray(reg["bounds"], dtype=float)
+        f = (np.asarray(positions, float) @ np.linalg.inv(lattice))[:, int(ax)]
+        f -= np.floor(f)
+        lab = np.clip(np.searchsorted(bounds, f, side="right") - 1, 0, len(bounds) - 2)
+        out["regions"] = (lab, [f"[{bounds[k]:.3f},{bounds[k + 1]:.3f})"
+                                for k in range(len(bounds) - 1)])
+    return out
+
 
 def entries(lattice, positions, rc):
     """Bonds within rc under the minimum image, both directions, grouped by

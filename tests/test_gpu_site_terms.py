@@ -319,3 +319,75 @@ def test_npt_relaxes_to_the_target_pressure():
     assert h.max() - h.min() < 3e-3 * (et.max() - et.min())
     assert r["pressure_gpa"][0] > 2.0 and r["volume"][-1] > 1.02 * r["volume"][0]
     assert abs(r["pressure_gpa"][2000:].mean()) < 0.2
+
+
+# --------------------------------------------------------------------------
+#  Heat flux per group of atoms (run_md decompose)
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("name", ["sw_gesn", "tersoff_xy", "eam_cuni"])
+def test_group_heat_fluxes_match_the_oracle(name):
+    make, terms, cell = CASES[name]
+    pot = make()
+    lat, pos, t = cell()
+    m = np.array([ATOMIC_MASS.get(x, 28.086) for x in t])
+    vel = maxwell_boltzmann(500.0, m, 5)
+    species = sorted(set(t), key=list(t).index)
+    labels = np.array([species.index(x) for x in t])
+    bounds = [0.0, 0.3, 0.55, 1.0]
+    n_steps, every, skin = 12, 4, 0.3
+    res = md_run(pot, lat, pos, vel, t, dt=1.0, n_steps=n_steps, skin_width=skin, masses=m,
+                 flux_every=every, flux_groups=(labels, len(species), (1, bounds)),
+                 atom_flux=True)
+    Jg = res["flux_groups"]
+    assert Jg.shape == (3, len(species), 3, 3)
+    # the groups add up to the total flux
+    assert rel_err(Jg.sum(axis=(1, 2)), res["flux"]) < 1e-12
+    # each group against the oracle's per-atom currents on the same states
+    if name == "sw_gesn":
+        force = lambda l, p, pr: O.sw_multi(l, p, t, SW_GESN, GESN_SPECIES, pairs=pr)
+    elif name == "tersoff_xy":
+        force = lambda l, p, pr: O.tersoff_multi(l, p, t, TERSOFF_XY, ["X", "Y"], pairs=pr)
+    else:
+        force = lambda l, p, pr: O.eam_zhou(l, p, t, CUNI, ["Cu", "Ni"], pairs=pr)
+    V = abs(np.linalg.det(lat))
+    frames, atoms = [], []
+
+    def flux(p, v):
+        Ji = terms(lat, p, t).atom_heat_current(v, m)
+        atoms.append(Ji / V)
+        reg = ST.group_labels(lat, p, t, {"regions": {"axis": "b", "bounds": bounds}})
+        rl = reg["regions"][0]
+        g = np.zeros((len(species), 3, 3))
+        np.add.at(g, (labels, rl), Ji)
+        frames.append(g / V)
+        return Ji.sum(axis=0) / V
+
+    O.md_run(force, lat, pos, vel, m, 1.0, n_steps, pot.r_cut, skin, flux=flux, every=every)
+    want = np.array(frames)
+    assert rel_err(Jg, want) < 1e-8
+    assert np.count_nonzero(np.abs(want).max(axis=(0, 3)) > 0) >= 4
+    # the per-atom currents of every frame, rounded to float32
+    Ja, wa = res["atom_flux"], np.array(atoms)
+    assert Ja.dtype == np.float32 and Ja.shape == (3, len(t), 3)
+    assert np.abs(Ja - wa).max() < 1e-6 * np.abs(wa).max()
+
+
+def test_green_kubo_decomposition_sums_to_the_total(tmp_path):
+    lat, pos, t = _gesn(2)
+    pot = StillingerWeberPotential(SW_GESN, species=GESN_SPECIES)
+    tc = ThermalConductivityCalculator(lat, pos, t, pot, temperature=500.0)
+    tc.run_md(n_steps=400, timestep=1.0, equil_steps=50, heat_flux_every=2, seed=3,
+              output_dir=str(tmp_path),
+              decompose={"species": True, "save_atom_flux": True,
+                         "regions": {"axis": "c", "bounds": [0, 0.5, 1]}})
+    flux = np.load(tmp_path / "atom_heat_flux.npy")
+    assert flux.shape == (200, len(t), 3) and flux.dtype == np.float32
+    assert np.allclose(flux.sum(axis=1), tc._J_series, rtol=1e-5,
+                       atol=1e-6 * np.abs(tc._J_series).max())
+    tc.compute_kappa(tau_max=0.1, n_blocks=2)
+    out = tc.compute_kappa_decomposition(tau_max=0.1, verbose=False)
+    assert set(out["species"]) == set(GESN_SPECIES) and len(out["regions"]) == 2
+    scale = np.abs(out["total"]).max()
+    assert max(out["sum_rule_residual"].values()) < 1e-10 * scale
+    assert np.allclose(np.diag(out["total"]), np.diag(tc._kappa_run[-1]), rtol=1e-9)

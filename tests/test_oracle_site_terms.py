@@ -118,3 +118,95 @@ def test_heat_flux_is_the_time_derivative_of_the_energy_moment(name):
     J, J_fd = flux_vs_moment_derivative(force, terms, lat, pos, t, m, rng)
     scale = np.abs(J_fd).max()
     assert np.abs(J - J_fd).max() < 1e-5 * scale, (J, J_fd)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_per_atom_heat_currents_add_up_to_the_heat_flux(name):
+    """sum_i J_i = V J (the reference's
+    test_per_atom_heat_currents_add_up_to_the_heat_flux, tests/test_nemd.py:
+    41-67), J itself pinned by the moment derivative above; and with one
+    atom moving the whole current sits on that atom (each entry's term goes
+    to the atom whose velocity it carries), which an assignment to the
+    centre atom fails."""
+    _, terms, cell = CASES[name]
+    lat, pos, t = cell()
+    rng = np.random.default_rng(7)
+    m = rng.uniform(20.0, 80.0, len(pos))
+    vel = rng.normal(0.0, 0.01, pos.shape)
+    st = terms(lat, pos, t)
+    V = abs(np.linalg.det(lat))
+    Ji = st.atom_heat_current(vel, m)
+    assert np.allclose(Ji.sum(axis=0) / V, st.heat_flux(vel, m, V), rtol=1e-12, atol=1e-16)
+    one = np.zeros_like(vel)
+    one[5] = vel[5]
+    Ji = st.atom_heat_current(one, m)
+    assert np.abs(np.delete(Ji, 5, axis=0)).max() == 0.0
+    assert np.allclose(Ji[5], st.heat_flux(one, m, V) * V, rtol=1e-12, atol=0.0)
+    assert np.abs(Ji[5] - (0.5 * m[5] * one[5] @ one[5] / C + st.e_site[5]) * one[5]).max() > \
+        1e-6 * np.abs(Ji[5]).max()
+
+
+def test_group_labels_follow_the_reference_rules():
+    """species in order of first appearance; regions by the wrapped
+    fractional coordinate, searchsorted right - 1 (a point on an edge joins
+    the region above), clipped into range (_group_labels,
+    solvers/thermal_conductivity.py:339-363)."""
+    # a sheared cell whose inverse is exact in binary, so the point on the
+    # edge 0.25 lands on it exactly
+    lat = np.array([[4.0, 2.0, 0.0], [0.0, 8.0, 0.0], [0.0, 0.0, 2.0]])
+    fc = np.array([0.1, 0.35, 0.6, 0.95, 1.2, -0.1, 0.25])
+    frac = np.stack([np.full(7, 0.25), fc, np.full(7, 0.5)], axis=1)
+    pos = frac @ lat
+    t = ["B", "A", "B", "C", "A", "A", "C"]
+    g = ST.group_labels(lat, pos, t, {"species": True,
+                                     "regions": {"axis": "b", "bounds": [0, 0.25, 0.7, 1.0]}})
+    assert g["species"][1] == ["B", "A", "C"]
+    assert g["species"][0].tolist() == [0, 1, 0, 2, 1, 1, 2]
+    assert g["regions"][0].tolist() == [0, 1, 1, 2, 0, 2, 1]
+    assert g["regions"][1] == ["[0.000,0.250)", "[0.250,0.700)", "[0.700,1.000)"]
+    g = ST.group_labels(lat, pos, t, {"regions": {"axis": 1, "bounds": [0.2, 0.5, 0.8]}})
+    assert set(g) == {"regions"}
+    assert g["regions"][0].tolist() == [0, 0, 1, 1, 0, 1, 0]
+
+
+def test_kappa_decomposition_is_the_brute_force_correlation_integral():
+    """compute_kappa_decomposition on synthetic group series: each group's
+    tensor is V / (k_B T^2) times the trapezoid integral of the
+    mean-subtracted <G_a(t + k) J_b(t)> summed directly over t, and the
+    groups add up to the total (the integral is linear in the group
+    flux)."""
+    from akitsushima_amd import StillingerWeberPotential
+    from akitsushima_amd.thermal import EV_ANG_K_FS_TO_W_MK, K_B, ThermalConductivityCalculator
+    lat, pos, _ = sheared_si(1, shear=0.0, rattle=0.0, seed=0)
+    tc = ThermalConductivityCalculator(lat, pos, ["Si"] * len(pos),
+                                       StillingerWeberPotential(SW_SI, species=["Si"]),
+                                       temperature=400.0)
+    with pytest.raises(RuntimeError):
+        tc.compute_kappa_decomposition(verbose=False)
+    rng = np.random.default_rng(11)
+    n, dt = 96, 2.0
+    G = np.cumsum(rng.normal(size=(n, 3, 3)), axis=0) * 1e-4 + rng.normal(size=(n, 3, 3)) * 1e-4
+    J = G.sum(axis=1)
+    tc._J_series, tc._dt_hf = J, dt
+    tc._J_groups = {"species": (np.array(["A", "B", "C"]), G)}
+    out = tc.compute_kappa_decomposition(tau_max=0.05, verbose=False)
+    n_lag = min(int(0.05 * 1e3 / dt), n // 2)
+    V = abs(np.linalg.det(lat))
+    pref = V / (K_B * 400.0 ** 2) * EV_ANG_K_FS_TO_W_MK
+
+    def brute(A, B):
+        A = A - A.mean(0)
+        B = B - B.mean(0)
+        k = np.zeros((3, 3))
+        for a in range(3):
+            for b in range(3):
+                c = [np.dot(A[j:, a], B[:n - j, b]) / (n - j) for j in range(n_lag)]
+                k[a, b] = sum(0.5 * dt * (c[j - 1] + c[j]) for j in range(1, n_lag))
+        return k * pref
+
+    total = brute(J, J)
+    assert np.allclose(out["total"], total, rtol=1e-10, atol=1e-12 * np.abs(total).max())
+    for q, g in enumerate("ABC"):
+        want = brute(G[:, q], J)
+        assert np.allclose(out["species"][g], want, rtol=1e-9, atol=1e-12 * np.abs(want).max())
+    assert out["sum_rule_residual"]["species"] < 1e-12 * np.abs(total).max()
