@@ -29,6 +29,7 @@ from .site_terms import SiteTerms
 from .thermal import ThermalConductivityCalculator
 from .simulations import (CSVRSimulation, LangevinSimulation, NPTSimulation, NVESimulation,
                           NVTSimulation)
+from .nemd import NEMDSimulation
 from .elastic import elastic_constants
 from .phonon import force_constants, gamma_frequencies
 from .zhou import zhou_eam
@@ -48,6 +49,7 @@ __all__ = [
     "read_setfl",
     "SiteTerms",
     "ThermalConductivityCalculator",
+    "NEMDSimulation",
     "NVESimulation",
     "NVTSimulation",
     "LangevinSimulation",

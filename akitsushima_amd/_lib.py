@@ -74,6 +74,10 @@ SIGNATURES = {
                                  _pf64, _pf64, _pf64, _pf64, _pf64, _pf64, _pf64, _pi32, _pf64,
                                  _pf64, _pi32, _i32, _i32, _pf64, _i32, _pf64, _pf32, _pi32,
                                  _pf64]),
+    "ak_nemd_run": (ctypes.c_int, [_handle, _i32, _pf64, _pf64, _pf64, _pf64, _pf64, _pi32, _f64,
+                                   _i32, _f64, _i32, _pu8, _f64, _f64, _f64, _f64, _pf64, _i32,
+                                   _i32, _i32, _i32, _pf64, _pf64, _pf64, _pf64, _pf64, _pi32,
+                                   _pf64]),
     "ak_site_terms": (ctypes.c_int, [_handle, _handle, _i32, _pf64, _pf64, _pf64, _pi32, _pf64,
                                      _i64, _pi32, _pi32, _pf64, _pf64,
                                      ctypes.POINTER(ctypes.c_int64)]),

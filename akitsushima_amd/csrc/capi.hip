@@ -944,6 +944,57 @@ int ak_md_run(ak_potential* P, int32_t n, const double* lattice, const double* i
   });
 }
 
+int ak_nemd_run(ak_potential* P, int32_t n, const double* lattice, const double* inv,
+                double* positions, double* velocities, const double* masses,
+                const int32_t* species, double dt, int32_t n_steps, double skin,
+                int32_t use_graph, const uint8_t* bath, double T_hot, double T_cold,
+                double damping, double seed, double* state, int32_t sample_every,
+                int32_t sample_phase, int32_t axis, int32_t n_bins, double* e_pot,
+                double* e_kin, double* e_cons, double* bath_history, double* bins,
+                int32_t* n_rebuilds, double* seconds) {
+  return guarded([&] {
+    AK_REQUIRE(P && lattice && positions && velocities && masses && bath && n > 0, AK_ERR_ARG,
+               "bad arguments");
+    AK_REQUIRE(skin >= 0.0, AK_ERR_ARG, "skin_width must be >= 0");
+    AK_REQUIRE(sample_every >= 0, AK_ERR_ARG, "sample_every must be >= 0");
+    AK_REQUIRE(sample_every == 0 || (axis >= 0 && axis <= 2 && n_bins >= 1 && n_bins <= 1024),
+               AK_ERR_ARG, "NEMD bins: axis 0, 1 or 2 and 1 to 1024 bins");
+    AK_REQUIRE(seed >= 0.0 && seed < 9007199254740992.0 && seed == std::floor(seed), AK_ERR_ARG,
+               "seed must be an integer in [0, 2^53)");
+    check_finite(lattice, 9, "lattice");
+    check_finite(positions, 3 * (size_t)n, "positions");
+    check_finite(velocities, 3 * (size_t)n, "velocities");
+    for (int a = 0; a < n; ++a) {
+      AK_REQUIRE(masses[a] > 0.0, AK_ERR_ARG, "masses must be positive");
+      if (species)
+        AK_REQUIRE(species[a] >= 0 && species[a] < P->pot.ns, AK_ERR_ARG,
+                   "species index out of range for this potential");
+    }
+    MDOptions opt;
+    opt.thermostat = 5;
+    opt.temperature = T_hot;
+    opt.T_cold = T_cold;
+    opt.tau = damping;
+    opt.seed = (uint64_t)seed;
+    opt.bath = bath;
+    opt.bath_history = bath_history;
+    opt.flux_every = sample_every;
+    opt.flux_phase = sample_phase;
+    if (sample_every > 0) {
+      opt.region_axis = axis;
+      opt.n_regions = n_bins;
+      opt.uniform_bins = true;
+      opt.group_kinetic = true;
+      opt.flux_groups = bins;
+    }
+    int nr = 0;
+    md_run(P->pot, n, lattice, inv, positions, velocities, masses, species, dt, n_steps, skin,
+           use_graph != 0, opt, state, e_pot, e_kin, e_cons, nullptr, nullptr, nullptr, nullptr,
+           &nr, seconds, P->stream);
+    if (n_rebuilds) *n_rebuilds = nr;
+  });
+}
+
 int ak_md_nve(ak_potential* P, int32_t n, const double* lattice, const double* inv,
               double* positions, double* velocities, const double* masses,
               const int32_t* species, double dt, int32_t n_steps, double skin,

@@ -222,7 +222,9 @@ bool site_masked(const Potential& pot, const NList& nl);
 // pressure_gpa with time constant tau_baro, and the chain of 1); heat flux
 // of the state after every step s with s % flux_every == 0 (0 = none).
 struct MDOptions {
-  int thermostat = 0;  // 0 NVE, 1 Nose-Hoover chain, 2 Langevin (BAOAB), 3 CSVR, 4 NPT
+  // 0 NVE, 1 Nose-Hoover chain, 2 Langevin (BAOAB), 3 CSVR, 4 NPT, 5 NEMD
+  // (velocity Verlet with Ornstein-Uhlenbeck heat baths)
+  int thermostat = 0;
   double temperature = 0.0, tau = 0.0;  // tau: chain time constant or damping, fs
   int chain = 3;
   uint64_t seed = 0;  // random stream of the stochastic thermostats
@@ -255,6 +257,18 @@ struct MDOptions {
   // per-atom heat currents J_a / V of every recorded frame, float32
   // (frames, N, 3); may be given without flux_groups
   float* atom_flux = nullptr;
+  // record the flux on the steps s with (s + flux_phase) % flux_every == 0;
+  // regions as region_bounds == nullptr equal bins; group sums of five
+  // components (J, the kinetic energy, the atom count)
+  int flux_phase = 0;
+  bool uniform_bins = false;
+  bool group_kinetic = false;
+  // NEMD (thermostat 5): bath per atom (0 none, 1 hot at `temperature`, 2
+  // cold at T_cold), damping `tau`; per record (dQ_hot, dQ_cold, K_hot,
+  // K_cold) into bath_history (n_steps + 1, 4)
+  const uint8_t* bath = nullptr;
+  double T_cold = 0.0;
+  double* bath_history = nullptr;
 };
 
 // Thermostat / barostat state carried between runs (in and out of md_run):
@@ -283,8 +297,9 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
 // Per-group heat fluxes (device pointers): group of atom a = labels[a] (or
 // 0) * n_reg + its region along `axis` (-1: none, n_reg = 1) between
 // bounds[0..n_reg]; ng groups; part (grid * ng * 3) scratch; out
-// (frames, ng, 3); atoms (frames, n, 3, float32, may be null) the per-atom
-// currents / V.
+// (frames, ng, nc); atoms (frames, n, 3, float32, may be null) the per-atom
+// currents / V.  bounds == nullptr with axis >= 0: n_reg equal bins.  nc = 3
+// (the group fluxes) or 5 (also the group's kinetic energy and atom count).
 struct FluxGroups {
   const int* labels = nullptr;
   int axis = -1;
@@ -294,16 +309,18 @@ struct FluxGroups {
   double* part = nullptr;
   double* out = nullptr;
   float* atoms = nullptr;
+  int nc = 3;
 };
 
 // Heat flux J = [sum_a E_a v_a - sum_(a,k) v_ak (G[a,k] . v_nbr)] / V of one
 // cell after site_grads (d_part: one triple per 32 atoms).  With d_gate, only
-// on steps counters[2] % every == 0, into d_out[3 * (step / every)]; with
-// groups, also the per-group fluxes of the per-atom heat currents.
+// on the steps s = counters[2] with (s + phase) % every == 0, into
+// d_out[3 * ((s + phase) / every - (phase > 0))], and counters[2] advances;
+// with groups, also the per-group sums of the per-atom heat currents.
 void heat_flux(Potential& pot, NList& nl, SetDev& set, const double* d_pos, const double* d_vel,
                const double* d_mass, const double* d_esite, const double* d_vscale,
                double conv, int* d_gate, int every, double* d_part, double* d_out,
-               hipStream_t st, const FluxGroups* groups = nullptr);
+               hipStream_t st, const FluxGroups* groups = nullptr, int phase = 0);
 
 inline int pow2ceil(int x) {
   int p = 1;

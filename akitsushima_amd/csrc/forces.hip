@@ -19,6 +19,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstring>
 
@@ -1939,6 +1940,17 @@ __global__ __launch_bounds__(256) void k_eam_site_g(
   }
 }
 
+// Record frame of the heat-flux gate: with gate, the step counters[2] of
+// this call records when (step + phase) % every == 0 (phase in [0, every)),
+// into frame (step + phase) / every - (phase > 0); -1 = no record.  Without
+// gate, frame 0.
+__device__ __forceinline__ int flux_frame(const int* __restrict__ gate, int every, int phase) {
+  if (!gate) return 0;
+  const int t = gate[2] + phase;
+  if (t % every != 0) return -1;
+  return t / every - (phase > 0 ? 1 : 0);
+}
+
 // Per-block sums of V J = sum_a E_a v_a - sum_(a,k) v_ak (G[a, k] . vel_nbr),
 // E_a = m_a |v_a|^2 / (2 conv) + U_a (Fan et al., PRB 92, 094301 (2015),
 // eq. 21, per neighbour entry).  vscale (nullable) multiplies every
@@ -1952,8 +1964,8 @@ __global__ __launch_bounds__(256) void k_heat_flux(
     const Geom* __restrict__ geoms, int single, double rc, const int* __restrict__ cnt,
     const int* __restrict__ nbr, int stride, const double* __restrict__ G,
     const uint64_t* __restrict__ amask, int masked, double conv,
-    const int* __restrict__ gate, int every, double* __restrict__ part) {
-  if (gate && gate[2] % every != 0) return;
+    const int* __restrict__ gate, int every, int phase, double* __restrict__ part) {
+  if (flux_frame(gate, every, phase) < 0) return;
   __shared__ double smem[16 * 3];
   const int a = (int)(blockIdx.x * blockDim.x + threadIdx.x) / LE;
   const int l = threadIdx.x & (LE - 1);
@@ -2001,12 +2013,12 @@ __global__ __launch_bounds__(256) void k_heat_flux(
 // advances every call.
 __global__ __launch_bounds__(256) void k_heat_flux_final(int nb, const double* __restrict__ part,
                                                          double inv_volume, int* __restrict__ gate,
-                                                         int every, double* __restrict__ out) {
+                                                         int every, int phase,
+                                                         double* __restrict__ out) {
   __shared__ double smem[16 * 3];
-  const int step = gate ? gate[2] : 0;
-  const bool rec = !gate || step % every == 0;
+  const int f = flux_frame(gate, every, phase);
   double J[3] = {0.0, 0.0, 0.0};
-  if (rec)
+  if (f >= 0)
     for (int b = threadIdx.x; b < nb; b += blockDim.x) {
       J[0] += part[3 * b];
       J[1] += part[3 * b + 1];
@@ -2014,13 +2026,13 @@ __global__ __launch_bounds__(256) void k_heat_flux_final(int nb, const double* _
     }
   block_sum<3>(J, smem);
   if (threadIdx.x == 0) {
-    if (rec) {
-      double* o = out + (gate ? 3 * (size_t)(step / every) : 0);
+    if (f >= 0) {
+      double* o = out + 3 * (size_t)f;
       o[0] = J[0] * inv_volume;
       o[1] = J[1] * inv_volume;
       o[2] = J[2] * inv_volume;
     }
-    if (gate) gate[2] = step + 1;
+    if (gate) gate[2] += 1;
   }
 }
 
@@ -2030,9 +2042,11 @@ __global__ __launch_bounds__(256) void k_heat_flux_final(int nb, const double* _
 // -vec_ji (dU_j/dvec_ji . v_i) = vec_ij (G[rev(i, k)] . v_i): the term goes to
 // the atom whose velocity it holds.  The group of i is labels[i] (or 0) times
 // n_reg plus its region: the fractional coordinate along `axis`, wrapped,
-// placed between bounds[0..n_reg] (searchsorted right - 1, clipped).  Per
-// block, one sum of J per group: part[(block * ng + g) * 3 + c].
-template <int LE>
+// placed between bounds[0..n_reg] (searchsorted right - 1, clipped), or
+// with bounds == nullptr in n_reg equal bins, min(floor(f n_reg), n_reg - 1).
+// Per block, one sum per group: part[(block * ng + g) * NC + c], NC = 3 (J)
+// or 5 (J, the kinetic energy, the atom count).
+template <int LE, int NC>
 __global__ __launch_bounds__(256) void k_heat_flux_groups(
     int n, const double* __restrict__ pos, const double* __restrict__ vel,
     const double* __restrict__ mass, const double* __restrict__ esite,
@@ -2040,14 +2054,16 @@ __global__ __launch_bounds__(256) void k_heat_flux_groups(
     const Geom* __restrict__ geoms, int single, double rc, const int* __restrict__ cnt,
     const int* __restrict__ nbr, const int* __restrict__ rev, int stride,
     const double* __restrict__ G, const uint64_t* __restrict__ amask, int masked, double conv,
-    const int* __restrict__ gate, int every, const int* __restrict__ labels, int axis,
-    const double* __restrict__ bounds, int n_reg, int ng, double* __restrict__ part,
+    const int* __restrict__ gate, int every, int phase, const int* __restrict__ labels,
+    int axis, const double* __restrict__ bounds, int n_reg, int ng, double* __restrict__ part,
     float* __restrict__ atoms, double inv_volume) {
-  if (gate && gate[2] % every != 0) return;
-  __shared__ double smem[16 * 3];
+  const int frame = flux_frame(gate, every, phase);
+  if (frame < 0) return;
+  __shared__ double smem[16 * NC];
   const int a = (int)(blockIdx.x * blockDim.x + threadIdx.x) / LE;
   const int l = threadIdx.x & (LE - 1);
   double J[3] = {0.0, 0.0, 0.0};
+  double ke = 0.0;
   int grp = -1;
   if (a < n) {
     const double sc = vscale ? vscale[0] : 1.0;
@@ -2071,7 +2087,8 @@ __global__ __launch_bounds__(256) void k_heat_flux_groups(
       J[2] += uz * gv;
     }
     if (l == 0) {
-      const double E = 0.5 * mass[a] * (vx * vx + vy * vy + vz * vz) / conv + esite[a];
+      ke = 0.5 * mass[a] * (vx * vx + vy * vy + vz * vz) / conv;
+      const double E = ke + esite[a];
       J[0] += E * vx;
       J[1] += E * vy;
       J[2] += E * vz;
@@ -2081,45 +2098,60 @@ __global__ __launch_bounds__(256) void k_heat_flux_groups(
       double f = pos[3 * (size_t)a] * g.inv[axis] + pos[3 * (size_t)a + 1] * g.inv[3 + axis] +
                  pos[3 * (size_t)a + 2] * g.inv[6 + axis];
       f -= floor(f);
-      for (int k = 1; k < n_reg; ++k)
-        if (bounds[k] <= f) rg = k;
+      if (bounds) {
+        for (int k = 1; k < n_reg; ++k)
+          if (bounds[k] <= f) rg = k;
+      } else {
+        rg = min((int)(f * n_reg), n_reg - 1);
+      }
     }
     grp = (labels ? labels[a] : 0) * n_reg + rg;
   }
   for (int c = 0; c < 3; ++c) J[c] = seg_sum<LE>(J[c]);
-  if (atoms && a < n && l == 0) {
-    const size_t f = gate ? (size_t)(gate[2] / every) : 0;
-    for (int c = 0; c < 3; ++c) atoms[(f * n + a) * 3 + c] = (float)(J[c] * inv_volume);
-  }
+  if (atoms && a < n && l == 0)
+    for (int c = 0; c < 3; ++c)
+      atoms[((size_t)frame * n + a) * 3 + c] = (float)(J[c] * inv_volume);
   for (int q = 0; q < ng; ++q) {
-    double w[3] = {0.0, 0.0, 0.0};
+    double w[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) w[c] = 0.0;
     if (grp == q && l == 0) {
       w[0] = J[0];
       w[1] = J[1];
       w[2] = J[2];
+      if constexpr (NC == 5) {
+        w[3] = ke;
+        w[4] = 1.0;
+      }
     }
-    block_sum<3>(w, smem);
+    block_sum<NC>(w, smem);
     if (threadIdx.x == 0)
-      for (int c = 0; c < 3; ++c) part[((size_t)blockIdx.x * ng + q) * 3 + c] = w[c];
+      for (int c = 0; c < NC; ++c) part[((size_t)blockIdx.x * ng + q) * NC + c] = w[c];
   }
 }
 
-// Group fluxes of a record step: (sum of the block sums, fixed order) / V
-// into out[(frame * ng + g) * 3 + c]; the gate is read, not advanced.
+// Group sums of a record step: (sum of the block sums, fixed order) into
+// out[(frame * ng + g) * NC + c], the flux components divided by V; the gate
+// is read, not advanced.
+template <int NC>
 __global__ __launch_bounds__(256) void k_heat_flux_groups_final(
     int nb, int ng, const double* __restrict__ part, double inv_volume,
-    const int* __restrict__ gate, int every, double* __restrict__ out) {
-  __shared__ double smem[16 * 3];
-  const int step = gate ? gate[2] : 0;
-  if (gate && step % every != 0) return;
-  const size_t f = gate ? (size_t)(step / every) : 0;
+    const int* __restrict__ gate, int every, int phase, double* __restrict__ out) {
+  __shared__ double smem[16 * NC];
+  const int frame = flux_frame(gate, every, phase);
+  if (frame < 0) return;
+  const size_t f = (size_t)frame;
   for (int q = 0; q < ng; ++q) {
-    double J[3] = {0.0, 0.0, 0.0};
+    double J[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) J[c] = 0.0;
     for (int b = threadIdx.x; b < nb; b += blockDim.x)
-      for (int c = 0; c < 3; ++c) J[c] += part[((size_t)b * ng + q) * 3 + c];
-    block_sum<3>(J, smem);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) J[c] += part[((size_t)b * ng + q) * NC + c];
+    block_sum<NC>(J, smem);
     if (threadIdx.x == 0)
-      for (int c = 0; c < 3; ++c) out[(f * ng + q) * 3 + c] = J[c] * inv_volume;
+      for (int c = 0; c < NC; ++c)
+        out[(f * ng + q) * NC + c] = c < 3 ? J[c] * inv_volume : J[c];
   }
 }
 
@@ -2146,33 +2178,41 @@ void site_grads(Potential& pot, NList& nl, SetDev& set, const double* d_pos, con
 void heat_flux(Potential& pot, NList& nl, SetDev& set, const double* d_pos, const double* d_vel,
                const double* d_mass, const double* d_esite, const double* d_vscale,
                double conv, int* d_gate, int every, double* d_part, double* d_out,
-               hipStream_t st, const FluxGroups* groups) {
+               hipStream_t st, const FluxGroups* groups, int phase) {
   const int n = set.n_atoms;
   constexpr int LE = 8;
   const unsigned grid = (unsigned)(((int64_t)n * LE + 255) / 256);
+  const double inv_v = 1.0 / set.geoms[0].volume;
   if (groups && groups->ng > 0) {
-    hipLaunchKernelGGL(k_heat_flux_groups<LE>, dim3(grid), dim3(256), 0, st, n, d_pos, d_vel,
-                       d_mass, d_esite, d_vscale, (const int*)set.d_sid.as<int>(),
-                       (const Geom*)set.d_geoms.as<Geom>(), set.n_struct == 1 ? 1 : 0,
-                       pot.r_cut, (const int*)nl.cnt.as<int>(), (const int*)nl.nbr.as<int>(),
-                       (const int*)nl.rev.as<int>(), nl.stride,
-                       (const double*)pot.G.as<double>(),
-                       (const uint64_t*)pot.amask.as<uint64_t>(), site_masked(pot, nl) ? 1 : 0,
-                       conv, (const int*)d_gate, every, groups->labels, groups->axis,
-                       groups->bounds, groups->n_reg, groups->ng, groups->part, groups->atoms,
-                       1.0 / set.geoms[0].volume);
-    hipLaunchKernelGGL(k_heat_flux_groups_final, dim3(1), dim3(256), 0, st, (int)grid,
-                       groups->ng, (const double*)groups->part, 1.0 / set.geoms[0].volume,
-                       (const int*)d_gate, every, groups->out);
+    auto launch = [&](auto nc) {
+      constexpr int NC = decltype(nc)::value;
+      hipLaunchKernelGGL((k_heat_flux_groups<LE, NC>), dim3(grid), dim3(256), 0, st, n, d_pos,
+                         d_vel, d_mass, d_esite, d_vscale, (const int*)set.d_sid.as<int>(),
+                         (const Geom*)set.d_geoms.as<Geom>(), set.n_struct == 1 ? 1 : 0,
+                         pot.r_cut, (const int*)nl.cnt.as<int>(), (const int*)nl.nbr.as<int>(),
+                         (const int*)nl.rev.as<int>(), nl.stride,
+                         (const double*)pot.G.as<double>(),
+                         (const uint64_t*)pot.amask.as<uint64_t>(),
+                         site_masked(pot, nl) ? 1 : 0, conv, (const int*)d_gate, every, phase,
+                         groups->labels, groups->axis, groups->bounds, groups->n_reg,
+                         groups->ng, groups->part, groups->atoms, inv_v);
+      hipLaunchKernelGGL((k_heat_flux_groups_final<NC>), dim3(1), dim3(256), 0, st, (int)grid,
+                         groups->ng, (const double*)groups->part, inv_v, (const int*)d_gate,
+                         every, phase, groups->out);
+    };
+    if (groups->nc == 5)
+      launch(std::integral_constant<int, 5>{});
+    else
+      launch(std::integral_constant<int, 3>{});
   }
   hipLaunchKernelGGL(k_heat_flux<LE>, dim3(grid), dim3(256), 0, st, n, d_pos, d_vel, d_mass,
                      d_esite, d_vscale, (const int*)set.d_sid.as<int>(),
                      (const Geom*)set.d_geoms.as<Geom>(), set.n_struct == 1 ? 1 : 0, pot.r_cut,
                      (const int*)nl.cnt.as<int>(), (const int*)nl.nbr.as<int>(), nl.stride,
                      (const double*)pot.G.as<double>(), (const uint64_t*)pot.amask.as<uint64_t>(),
-                     site_masked(pot, nl) ? 1 : 0, conv, (const int*)d_gate, every, d_part);
+                     site_masked(pot, nl) ? 1 : 0, conv, (const int*)d_gate, every, phase, d_part);
   hipLaunchKernelGGL(k_heat_flux_final, dim3(1), dim3(256), 0, st, (int)grid, (const double*)d_part,
-                     1.0 / set.geoms[0].volume, d_gate, every, d_out);
+                     inv_v, d_gate, every, phase, d_out);
 }
 
 }  // namespace ak

@@ -561,6 +561,96 @@ __global__ __launch_bounds__(256) void k_lang_rec(int nb, const double* __restri
 }
 
 // ---------------------------------------------------------------------------
+// NEMD with two Langevin heat baths (NEMDSimulation.run, solvers/nemd.py:
+// 150-232): after each velocity Verlet step the bath atoms take an exact
+// Ornstein-Uhlenbeck update v <- c v + sqrt((1 - c^2) kT_bath / m) xi,
+// c = exp(-dt / damping) (_o_step :113-117), xi of Philox blocks
+// (a, draw, 7, 0/1), draw = counters[3]; the kinetic energy each bath gains
+// is its heat.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kStreamNemd = 7;
+
+// Per block: the kinetic energy of all atoms after the update and of each
+// bath before and after it, part[5 * block + (0: all, 1, 2: hot before,
+// after, 3, 4: cold before, after)].  apply = 0 measures only.
+__global__ __launch_bounds__(256) void k_nemd_o(int n, double* __restrict__ vel,
+                                                const double* __restrict__ mass,
+                                                const uint8_t* __restrict__ bath, double c,
+                                                double kTc_hot, double kTc_cold, uint32_t k0,
+                                                uint32_t k1, const int* __restrict__ counters,
+                                                int apply, double* __restrict__ part) {
+  __shared__ double smem[16 * 5];
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (a < n) {
+    const double m = mass[a];
+    double v[3] = {vel[3 * a], vel[3 * a + 1], vel[3 * a + 2]};
+    const int b = bath[a];
+    if (b == 1 || b == 2) {
+      const double before = 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / kConv;
+      if (apply) {
+        const uint32_t s = (uint32_t)counters[3];
+        double z[4];
+        normal2(philox4x32(U4{(uint32_t)a, s, kStreamNemd, 0}, k0, k1), z[0], z[1]);
+        normal2(philox4x32(U4{(uint32_t)a, s, kStreamNemd, 1}, k0, k1), z[2], z[3]);
+        const double sig = sqrt((1.0 - c * c) * (b == 1 ? kTc_hot : kTc_cold) / m);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          v[d] = c * v[d] + sig * z[d];
+          vel[3 * a + d] = v[d];
+        }
+      }
+      const double after = 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / kConv;
+      r[b == 1 ? 1 : 3] = before;
+      r[b == 1 ? 2 : 4] = after;
+    }
+    r[0] = 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / kConv;
+  }
+  block_sum<5>(r, smem);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 5; ++k) part[5 * blockIdx.x + k] = r[k];
+}
+
+// One block: the bath heats of the step (mode 1: Q += dQ_hot + dQ_cold in
+// th[2], the random draw advances) and the record of E_pot, E_kin, E_tot - Q
+// (the E_minus_Q drift), P_inst and (dQ_hot, dQ_cold, K_hot, K_cold).  mode
+// 0: record only.
+__global__ __launch_bounds__(256) void k_nemd_rec(int nb, const double* __restrict__ part,
+                                                  int mode, double* __restrict__ th,
+                                                  const double* __restrict__ es,
+                                                  int* __restrict__ counters,
+                                                  double* __restrict__ e_pot,
+                                                  double* __restrict__ e_kin,
+                                                  double* __restrict__ e_cons, double V,
+                                                  double* __restrict__ h_press,
+                                                  int* __restrict__ h_nb,
+                                                  double* __restrict__ h_bath) {
+  __shared__ double smem[16 * 5];
+  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < 5; ++k) v[k] += part[5 * b + k];
+  block_sum<5>(v, smem);
+  if (threadIdx.x != 0) return;
+  const double dh = mode ? v[2] - v[1] : 0.0, dc = mode ? v[4] - v[3] : 0.0;
+  if (mode) {
+    th[2] += dh + dc;
+    counters[3] += 1;
+  }
+  const int step = counters[0];
+  e_pot[step] = es[0];
+  e_kin[step] = v[0];
+  e_cons[step] = v[0] + es[0] - th[2];
+  h_press[step] = 2.0 * v[0] / (3.0 * V) + ((es[1] + es[5]) + es[9]) / 3.0;
+  h_nb[step] = counters[1];
+  h_bath[4 * (size_t)step] = dh;
+  h_bath[4 * (size_t)step + 1] = dc;
+  h_bath[4 * (size_t)step + 2] = v[2];
+  h_bath[4 * (size_t)step + 3] = v[4];
+  counters[0] = step + 1;
+}
+
+// ---------------------------------------------------------------------------
 // NPT: isotropic MTTK barostat with a 2-link chain on its momentum and the
 // particle Nose-Hoover chain (NPTSimulation, solvers/molecular_dynamics.py:
 // 2141-2555, cell_dof "iso", baro_thermostat "nhc"; step :2871-2895).  The
@@ -1154,9 +1244,10 @@ struct MDState {
   DevBuf pos, vel, acc, F, mass, spc, es, ke, hist_ep, hist_ek, hist_ec, counters, flag;
   DevBuf nhc, esite, flux_part, hist_J, ke_b, ke_a, p_part, np, hist_V, hist_P, keep, hist_nb;
   // thermostat: 0 none (NVE), 1 Nose-Hoover chain, 2 Langevin, 3 CSVR,
-  // 4 NPT (barostat + Nose-Hoover chain); cell: 0 iso, else a flexible cell
+  // 4 NPT (barostat + Nose-Hoover chain), 5 NEMD (two heat baths); cell: 0
+  // iso, else a flexible cell
   // whose kicks also sum the kinetic tensor
-  int thermo = 0, M = 0, every = 0, cell = 0;
+  int thermo = 0, M = 0, every = 0, cell = 0, n = 0;
   double kT = 0.0, ndof = 0.0, damping = 0.0, inv_mass = 0.0;
   uint32_t k0 = 0, k1 = 0;
   // fix_atoms: keep = 1 - frozen per component (null when nothing is frozen)
@@ -1174,6 +1265,10 @@ struct MDState {
   // heat flux per group of atoms
   FluxGroups fg;
   DevBuf fg_labels, fg_bounds, fg_part, hist_Jg, hist_Ja;
+  int flux_phase = 0;
+  // NEMD: bath per atom, the cold bath's kT, the per-record bath heats
+  DevBuf bath, hist_bath;
+  double kT_cold = 0.0;
   const double* keep_ptr() const { return fixed ? keep.as<double>() : nullptr; }
   NptParams npt{};
   Geom* geom = nullptr;  // the set's device cell
@@ -1227,7 +1322,18 @@ static void thermo_kernel(MDState& S, int mode, double dt, hipStream_t st, unsig
                        S.hist_ek.as<double>(), S.hist_ec.as<double>(), S.hist_V.as<double>(),
                        S.hist_P.as<double>(), S.hist_nb.as<int>(),
                        S.cell_hist ? S.hist_L.as<double>() : (double*)nullptr);
-  else if (S.thermo == 2)
+  else if (S.thermo == 5) {
+    hipLaunchKernelGGL(k_nemd_o, dim3(nb), dim3(256), 0, st, S.n, S.vel.as<double>(),
+                       (const double*)S.mass.as<double>(), (const uint8_t*)S.bath.as<uint8_t>(),
+                       exp(-dt / S.damping), S.kT * kConv, S.kT_cold * kConv, S.k0, S.k1,
+                       (const int*)S.counters.as<int>(), mode != 0 ? 1 : 0, S.ke_b.as<double>());
+    hipLaunchKernelGGL(k_nemd_rec, dim3(1), dim3(256), 0, st, (int)nb,
+                       (const double*)S.ke_b.as<double>(), mode != 0 ? 1 : 0, S.nhc.as<double>(),
+                       (const double*)S.es.as<double>(), S.counters.as<int>(),
+                       S.hist_ep.as<double>(), S.hist_ek.as<double>(), S.hist_ec.as<double>(),
+                       S.V, S.hist_P.as<double>(), S.hist_nb.as<int>(),
+                       S.hist_bath.as<double>());
+  } else if (S.thermo == 2)
     hipLaunchKernelGGL(k_lang_rec, dim3(1), dim3(256), 0, st, (int)nb,
                        (const double*)S.ke.as<double>(), (const double*)S.ke_b.as<double>(),
                        (const double*)S.ke_a.as<double>(), mode, S.nhc.as<double>(),
@@ -1310,7 +1416,7 @@ static void md_step(Potential& pot, NList& nl, SetDev& set, MDState& S, int n, d
     heat_flux(pot, nl, set, S.pos.as<double>(), S.vel.as<double>(), S.mass.as<double>(),
               S.esite.as<double>(), S.scaled() ? S.nhc.as<double>() : nullptr, kConv,
               S.counters.as<int>(), S.every, S.flux_part.as<double>(), S.hist_J.as<double>(), st,
-              S.fg.ng > 0 ? &S.fg : nullptr);
+              S.fg.ng > 0 ? &S.fg : nullptr, S.flux_phase);
   if (S.frame_every > 0)
     hipLaunchKernelGGL(k_frame, dim3((unsigned)((3 * (int64_t)n + 255) / 256)), dim3(256), 0,
                        st, n, (const double*)S.pos.as<double>(), (const Geom*)S.geom,
@@ -1338,9 +1444,9 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
             double* e_kin, double* e_cons, double* flux, double* lattice_out, double* volume,
             double* pressure, int* n_rebuilds, double* seconds, hipStream_t st) {
   AK_REQUIRE(n > 0 && n_steps >= 0 && dt > 0.0, AK_ERR_ARG, "bad MD arguments");
-  AK_REQUIRE(opt.thermostat >= 0 && opt.thermostat <= 4, AK_ERR_ARG,
-             "thermostat must be 0 (NVE), 1 (Nose-Hoover chain), 2 (Langevin), 3 (CSVR) or "
-             "4 (NPT)");
+  AK_REQUIRE(opt.thermostat >= 0 && opt.thermostat <= 5, AK_ERR_ARG,
+             "thermostat must be 0 (NVE), 1 (Nose-Hoover chain), 2 (Langevin), 3 (CSVR), "
+             "4 (NPT) or 5 (NEMD)");
   AK_REQUIRE(opt.cell_dof >= 0 && opt.cell_dof <= 5, AK_ERR_ARG,
              "NPT cell_dof must be 0 (iso), 1 (aniso), 2 (tri), 3 (xy), 4 (xy_iso) or 5 (z)");
   AK_REQUIRE(opt.baro_thermostat == 0 || opt.baro_thermostat == 1, AK_ERR_ARG,
@@ -1362,6 +1468,7 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
   set.setup(1, std::vector<int64_t>{0, (int64_t)n}.data(), lattice, inv, nl.r_list, st, true,
             opt.thermostat == 4 ? nl.r_list / kNptShrink : 0.0);
   MDState S;
+  S.n = n;
   S.geom = set.d_geoms.as<Geom>();
   S.thermo = opt.thermostat;
   S.cell = opt.thermostat == 4 ? opt.cell_dof : 0;
@@ -1383,6 +1490,9 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
                                            (double)n_steps + 1.0));
   S.T = opt.temperature;
   S.every = opt.flux_every;
+  AK_REQUIRE(opt.flux_phase >= 0 && (S.every == 0 ? opt.flux_phase == 0 : opt.flux_phase < S.every),
+             AK_ERR_ARG, "flux_phase must lie in [0, flux_every)");
+  S.flux_phase = opt.flux_phase;
   S.pos.ensure(sizeof(double) * 3 * n);
   S.vel.ensure(sizeof(double) * 3 * n);
   S.acc.ensure(sizeof(double) * 3 * n);
@@ -1412,7 +1522,10 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
   }
   S.counters.ensure(sizeof(int) * 4);
   S.flag.ensure(sizeof(int));
-  const int frames = S.every > 0 ? (n_steps + S.every - 1) / S.every : 0;
+  // records at the steps s < n_steps with (s + phase) % every == 0
+  const int frames = (S.every > 0 && n_steps > 0)
+                         ? (n_steps - 1 + S.flux_phase) / S.every - (S.flux_phase > 0 ? 1 : 0) + 1
+                         : 0;
   if (S.every > 0 && (opt.flux_groups || opt.atom_flux)) {
     // groups of the per-atom heat currents (ThermalConductivityCalculator.run_md
     // decompose, solvers/thermal_conductivity.py:339-363)
@@ -1423,6 +1536,7 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
                "heat-flux regions: axis must be 0, 1, 2 (or -1 for none)");
     AK_REQUIRE(opt.region_axis >= 0 || opt.n_regions == 1, AK_ERR_ARG,
                "heat-flux regions need an axis");
+    S.fg.nc = opt.group_kinetic ? 5 : 3;
     S.fg.ng = opt.n_labels * opt.n_regions;
     S.fg.n_reg = opt.n_regions;
     S.fg.axis = opt.region_axis;
@@ -1436,7 +1550,8 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
     } else {
       AK_REQUIRE(opt.n_labels == 1, AK_ERR_ARG, "heat-flux groups: n_labels > 1 needs labels");
     }
-    if (opt.region_axis >= 0) {
+    S.fg.bounds = nullptr;
+    if (opt.region_axis >= 0 && !opt.uniform_bins) {
       AK_REQUIRE(opt.region_bounds != nullptr, AK_ERR_ARG, "heat-flux regions need bounds");
       for (int k = 0; k <= opt.n_regions; ++k)
         AK_REQUIRE(std::isfinite(opt.region_bounds[k]) &&
@@ -1448,10 +1563,10 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
       S.fg.bounds = S.fg_bounds.as<double>();
     }
     const size_t grid = ((size_t)n * 8 + 255) / 256;
-    S.fg_part.ensure(sizeof(double) * 3 * grid * S.fg.ng);
+    S.fg_part.ensure(sizeof(double) * S.fg.nc * grid * S.fg.ng);
     S.fg.part = S.fg_part.as<double>();
-    const int fr = (n_steps + S.every - 1) / S.every;
-    S.hist_Jg.ensure(sizeof(double) * 3 * (size_t)std::max(fr, 1) * S.fg.ng);
+    const int fr = frames;
+    S.hist_Jg.ensure(sizeof(double) * S.fg.nc * (size_t)std::max(fr, 1) * S.fg.ng);
     S.fg.out = S.hist_Jg.as<double>();
     S.fg.atoms = nullptr;
     if (opt.atom_flux) {
@@ -1572,6 +1687,35 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
     S.ke_a.ensure(sizeof(double) * nb);
     S.p_part.ensure(sizeof(double) * 3 * nb);
     AK_HIP(hipStreamSynchronize(st));
+  } else if (S.thermo == 5) {
+    // NEMD: hot bath at `temperature`, cold at T_cold, damping tau; state
+    // [-, -, heat Q = Q_hot + Q_cold]
+    AK_REQUIRE(opt.bath != nullptr, AK_ERR_ARG, "NEMD needs the bath of every atom");
+    AK_REQUIRE(opt.temperature >= 0.0 && opt.T_cold >= 0.0 && opt.tau > 0.0, AK_ERR_ARG,
+               "NEMD bath temperatures must be >= 0 and the damping positive");
+    AK_REQUIRE(!S.fixed, AK_ERR_ARG, "NEMD does not take frozen atoms");
+    int n_hot = 0, n_cold = 0;
+    for (int a = 0; a < n; ++a) {
+      AK_REQUIRE(opt.bath[a] <= 2, AK_ERR_ARG, "NEMD bath codes are 0 (none), 1 hot, 2 cold");
+      n_hot += opt.bath[a] == 1;
+      n_cold += opt.bath[a] == 2;
+    }
+    AK_REQUIRE(n_hot > 0 && n_cold > 0, AK_ERR_ARG, "NEMD: a heat bath has no atoms");
+    S.kT = kBoltzmannEv * opt.temperature;
+    S.kT_cold = kBoltzmannEv * opt.T_cold;
+    S.ndof = ndof;
+    S.damping = opt.tau;
+    S.k0 = (uint32_t)(opt.seed & 0xffffffffu);
+    S.k1 = (uint32_t)(opt.seed >> 32);
+    std::vector<double> h(8, 0.0);
+    h[0] = h[1] = 1.0;
+    h[2] = st_at(kStHeat);
+    S.nhc.ensure(sizeof(double) * h.size());
+    AK_HIP(hipMemcpy(S.nhc.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    S.bath.ensure((size_t)n);
+    AK_HIP(hipMemcpy(S.bath.p, opt.bath, (size_t)n, hipMemcpyHostToDevice));
+    S.ke_b.ensure(sizeof(double) * 5 * nb);
+    S.hist_bath.ensure(sizeof(double) * 4 * (size_t)(n_steps + 1));
   }
   AK_HIP(hipMemcpyAsync(S.pos.p, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
   if (S.fixed) {
@@ -1666,7 +1810,7 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
       state[kStXi + k] = h[2 + S.M + k];
       state[kStVxi + k] = h[2 + 2 * S.M + k];
     }
-  } else if (state && (S.thermo == 2 || S.thermo == 3)) {
+  } else if (state && (S.thermo == 2 || S.thermo == 3 || S.thermo == 5)) {
     double h[3];
     AK_HIP(hipMemcpy(h, S.nhc.p, sizeof(h), hipMemcpyDeviceToHost));
     state[kStHeat] = h[2];
@@ -1722,7 +1866,10 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
   if (flux && frames > 0)
     AK_HIP(hipMemcpy(flux, S.hist_J.p, sizeof(double) * 3 * frames, hipMemcpyDeviceToHost));
   if (opt.flux_groups && S.fg.ng > 0 && frames > 0)
-    AK_HIP(hipMemcpy(opt.flux_groups, S.hist_Jg.p, sizeof(double) * 3 * frames * S.fg.ng,
+    AK_HIP(hipMemcpy(opt.flux_groups, S.hist_Jg.p,
+                     sizeof(double) * S.fg.nc * (size_t)frames * S.fg.ng, hipMemcpyDeviceToHost));
+  if (S.thermo == 5 && opt.bath_history)
+    AK_HIP(hipMemcpy(opt.bath_history, S.hist_bath.p, sizeof(double) * 4 * (n_steps + 1),
                      hipMemcpyDeviceToHost));
   if (opt.atom_flux && S.fg.atoms && frames > 0)
     AK_HIP(hipMemcpy(opt.atom_flux, S.hist_Ja.p, sizeof(float) * 3 * (size_t)n * frames,

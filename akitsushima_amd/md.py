@@ -322,3 +322,65 @@ def npt_run(potential, lattice, positions, velocities, atom_types, temperature, 
                   baro_thermostat, seed), 0, state, fix_atoms, velocity_rescale_steps)
     out["T"] = 2.0 * out["e_kin"] / (n_dof_of(len(positions), fix_atoms) * K_B)
     return out
+
+
+def n_samples(n_steps, every, phase=0):
+    """Records of the steps s < n_steps with (s + phase) % every == 0."""
+    if every <= 0 or n_steps <= 0:
+        return 0
+    return (n_steps - 1 + phase) // every - (1 if phase > 0 else 0) + 1
+
+
+def nemd_run(potential, lattice, positions, velocities, atom_types, bath, T_hot, T_cold,
+             damping, dt=1.0, n_steps=1000, seed=42, skin_width=0.5, masses=None,
+             use_graph=True, state=None, sample_every=0, sample_phase=0, axis=2, n_bins=20):
+    """n_steps of NEMD on the device (``ak_nemd_run``; NEMDSimulation.run's
+    step, solvers/nemd.py:184-193): velocity Verlet, then the exact
+    Ornstein-Uhlenbeck update of the bath atoms (bath (N,): 0 none, 1 hot at
+    T_hot, 2 cold at T_cold; c = exp(-dt / damping)), the noise Philox4x32-10
+    keyed by ``seed``.  Returns the final positions and velocities, e_pot,
+    e_kin, e_cons (E_tot - Q), "bath" (n_steps + 1, 4: dQ_hot, dQ_cold of the
+    step, K_hot, K_cold after it), the rebuild count, the wall time, the
+    final "state" and, with sample_every > 0, "bins" (samples, n_bins, 5):
+    per equal bin of the fractional coordinate along ``axis``, sum_i J_i / V
+    (3), the kinetic energy and the atom count after the steps s with
+    (s + sample_phase) % sample_every == 0."""
+    lat = np.ascontiguousarray(lattice, float)
+    pos = np.ascontiguousarray(positions, float).copy()
+    vel = np.ascontiguousarray(velocities, float).copy()
+    n = len(pos)
+    m = np.ascontiguousarray(masses if masses is not None else masses_of(atom_types), float)
+    inv = np.ascontiguousarray(np.linalg.inv(lat))
+    spc = np.ascontiguousarray(potential.species_ids(atom_types), np.int32)
+    b = np.ascontiguousarray(bath, dtype=np.uint8)
+    if b.shape != (n,):
+        raise ValueError(f"bath must have one entry per atom ({n}), got shape {b.shape}")
+    se, ph = int(sample_every), int(sample_phase)
+    if se < 0 or (se > 0 and not 0 <= ph < se):
+        raise ValueError("sample_every must be >= 0 and sample_phase in [0, sample_every)")
+    ns = n_samples(n_steps, se, ph)
+    bins = np.zeros((max(ns, 1), int(n_bins), 5))
+    ep = np.zeros(n_steps + 1)
+    ek = np.zeros(n_steps + 1)
+    ec = np.zeros(n_steps + 1)
+    rows = np.zeros((n_steps + 1, 4))
+    st = np.zeros(STATE_LEN) if state is None else np.array(state, float).reshape(STATE_LEN)
+    nreb = ctypes.c_int32(0)
+    secs = ctypes.c_double(0.0)
+    P = _lib._pf64
+    check(lib().ak_nemd_run(potential.handle(), n, lat.ctypes.data_as(P), inv.ctypes.data_as(P),
+                            pos.ctypes.data_as(P), vel.ctypes.data_as(P), m.ctypes.data_as(P),
+                            spc.ctypes.data_as(_lib._pi32), float(dt), int(n_steps),
+                            float(skin_width), 1 if use_graph else 0,
+                            b.ctypes.data_as(_lib._pu8), float(T_hot), float(T_cold),
+                            float(damping), float(int(seed)), st.ctypes.data_as(P), se, ph,
+                            int(axis), int(n_bins), ep.ctypes.data_as(P), ek.ctypes.data_as(P),
+                            ec.ctypes.data_as(P), rows.ctypes.data_as(P),
+                            bins.ctypes.data_as(P) if se > 0 else None, ctypes.byref(nreb),
+                            ctypes.byref(secs)))
+    out = {"positions": pos, "velocities": vel, "e_pot": ep, "e_kin": ek, "e_cons": ec,
+           "bath": rows, "n_rebuilds": int(nreb.value), "seconds": float(secs.value),
+           "state": st}
+    if se > 0:
+        out["bins"] = bins[:ns]
+    return out

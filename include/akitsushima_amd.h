@@ -293,6 +293,33 @@ int ak_md_run(ak_potential *pot, int32_t n_atoms, const double *lattice,
               double *seconds);
 #define AK_MD_STATE_LEN 48
 
+/* Non-equilibrium MD with two Langevin heat baths (NEMDSimulation.run,
+ * solvers/nemd.py:150-232, replacing its velocity Verlet loop, _o_step
+ * :113-117 and _sample :239-256): n_steps of velocity Verlet on the device,
+ * after each step an exact Ornstein-Uhlenbeck update of the bath atoms,
+ * v <- c v + sqrt((1 - c^2) k_B T / m) xi, c = exp(-dt / damping), xi from
+ * the Philox4x32-10 stream keyed by seed (blocks (atom, draw, 7, 0/1); the
+ * reference draws from numpy's generator instead).  bath (N bytes): 0 none,
+ * 1 hot (T_hot), 2 cold (T_cold).  state (AK_MD_STATE_LEN, may be NULL) as
+ * for ak_md_run: [43] the heat Q_hot + Q_cold so far, [44] the draws used.
+ * e_pot, e_kin, e_cons = E_tot - Q (n_steps + 1, record 0 the initial
+ * state); bath_history (n_steps + 1, 4): the heat each bath added in the
+ * step (dQ_hot, dQ_cold) and the kinetic energy of each bath after it.
+ * sample_every > 0: after the steps s (0-based in this call) with
+ * (s + sample_phase) % sample_every == 0, per bin of n_bins equal bins of
+ * the wrapped fractional coordinate along axis (min(floor(f n_bins),
+ * n_bins - 1)) the sums (sum_i J_i / V (3), kinetic energy, atom count)
+ * into bins (samples, n_bins, 5), J_i the per-atom heat currents of
+ * ak_md_run's flux_groups.  Any output may be NULL. */
+int ak_nemd_run(ak_potential *pot, int32_t n_atoms, const double *lattice,
+                const double *inv_lattice, double *positions, double *velocities,
+                const double *masses, const int32_t *species, double dt, int32_t n_steps,
+                double skin_width, int32_t use_graph, const uint8_t *bath, double T_hot,
+                double T_cold, double damping, double seed, double *state,
+                int32_t sample_every, int32_t sample_phase, int32_t axis, int32_t n_bins,
+                double *e_pot, double *e_kin, double *e_cons, double *bath_history,
+                double *bins, int32_t *n_rebuilds, double *seconds);
+
 /* ---- Site terms and heat flux ------------------------------------------ */
 
 /* Site-energy decomposition E = sum_a U_a (potential.site_terms,

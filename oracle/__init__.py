@@ -33,6 +33,8 @@ after the reference's Python glue:
   thermostats (:2141-2692, 2871-2895)
 * ``site_terms``: engine/site_terms.py (site energies, per-bond gradients,
   heat flux)
+* ``nemd_run``: NEMDSimulation.run's step with the two heat baths,
+  solvers/nemd.py:113-117, 184-193
 
 What pins each function (none is left unpinned):
 
@@ -57,6 +59,11 @@ What pins each function (none is left unpinned):
   strain to the target pressure tensor (tests/test_oracle_npt_cell.py)
 * site_terms: rebuilt forces/stress/energy = the force path; heat flux =
   d/dt sum_i r_i E_i on isolated clusters (tests/test_oracle_site_terms.py)
+* nemd_run: E_tot - Q conserved to O(dt^2) (the reference's own check,
+  tests/test_nemd.py:113-121); with the baths off (damping -> infinity)
+  the NVE run bit for bit; free bath atoms relax to the exact
+  Ornstein-Uhlenbeck variance k_B T / m per component
+  (tests/test_oracle_nemd.py)
 """
 from __future__ import annotations
 
@@ -1227,3 +1234,82 @@ def nve_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, s
     ep, ek, _, pos, vel, rb, _ = md_run(force, lattice, positions, velocities, masses, dt,
                                         n_steps, r_cut, skin)
     return ep, ek, pos, vel, rb
+
+def bath_noise(n, draw, k0, k1):
+    """(n, 3) standard normals of the NEMD bath update at random draw
+    `draw`: Philox blocks (a, draw, 7, 0) and (a, draw, 7, 1), Box-Muller."""
+    a = np.arange(n, dtype=np.uint64)
+    z0, z1 = normal2(philox4x32([a, np.full(n, draw), np.full(n, 7), np.zeros(n)], k0, k1))
+    z2, _ = normal2(philox4x32([a, np.full(n, draw), np.full(n, 7), np.ones(n)], k0, k1))
+    return np.stack([z0, z1, z2], axis=1)
+
+
+def nemd_run(force, lattice, positions, velocities, masses, dt, n_steps, r_cut, skin, bath,
+             T_hot, T_cold, damping, seed, draw0=0, heat0=0.0, sample=None, every=0, phase=0):
+    """NEMDSimulation.run's step (solvers/nemd.py:184-193): velocity Verlet
+    (with the wrap and skin-test rebuild of md_run), then the exact
+    Ornstein-Uhlenbeck update of the hot bath's atoms and then the cold
+    bath's, v <- c v + sqrt((1 - c^2) k_B T EV_ANG_TO_AMU_ANG_FS2 / m) xi,
+    c = exp(-dt / damping) (_o_step :113-117), the heat of each bath its
+    kinetic-energy change; xi from the Philox stream of the device (draw0 +
+    step).  bath (N,): 0 none, 1 hot, 2 cold.  sample(pos, vel) after the
+    steps s with (s + phase) % every == 0.  Returns a dict: ep, ek, econs
+    (E_tot - Q) and rows (dQ_hot, dQ_cold, K_hot, K_cold) per record
+    (n_steps + 1, record 0 the initial state), positions, velocities,
+    rebuilds, samples."""
+    lattice = np.asarray(lattice, float)
+    pos = np.array(positions, float)
+    vel = np.array(velocities, float)
+    bath = np.asarray(bath)
+    inv = np.linalg.inv(lattice)
+    r_list = r_cut + skin
+    ref = pos.copy()
+    pairs = nl_build(lattice, pos, r_list)[:2]
+    e, f, _ = force(lattice, pos, pairs)
+    conv = EV_ANG_TO_AMU_ANG_FS2
+    acc = f * conv / masses[:, None]
+    ek = lambda v, idx=slice(None): 0.5 * float(np.sum(masses[idx, None] * v[idx] ** 2)) / conv
+    hot, cold = np.flatnonzero(bath == 1), np.flatnonzero(bath == 2)
+    k0, k1 = int(seed) & 0xFFFFFFFF, int(seed) >> 32
+    c = math.exp(-dt / damping)
+    Q = float(heat0)
+    ep, ekin, ec = [e], [ek(vel)], [e + ek(vel) - Q]
+    rows = [(0.0, 0.0, ek(vel, hot), ek(vel, cold))]
+    samples = []
+    rebuilds = 0
+
+    def wrap(p):
+        fr = p @ inv
+        fr -= np.floor(fr)
+        fr[fr >= 1.0] = 0.0
+        return fr @ lattice
+
+    for step in range(n_steps):
+        vel = vel + 0.5 * dt * acc
+        pos = wrap(pos + dt * vel)
+        dr = (pos - ref) @ inv
+        dr -= np.round(dr)
+        if r_list - 2.0 * float(np.max(np.linalg.norm(dr @ lattice, axis=1))) <= r_cut:
+            pairs = nl_build(lattice, pos, r_list)[:2]
+            ref = pos.copy()
+            rebuilds += 1
+        e, f, _ = force(lattice, pos, pairs)
+        acc = f * conv / masses[:, None]
+        vel = vel + 0.5 * dt * acc
+        z = bath_noise(len(pos), draw0 + step, k0, k1)
+        dq = []
+        for idx, T in ((hot, T_hot), (cold, T_cold)):
+            before = ek(vel, idx)
+            sig = np.sqrt((1.0 - c * c) * K_B * T * conv / masses[idx])
+            vel[idx] = c * vel[idx] + sig[:, None] * z[idx]
+            dq.append(ek(vel, idx) - before)
+        Q += dq[0] + dq[1]
+        ep.append(e)
+        ekin.append(ek(vel))
+        ec.append(e + ek(vel) - Q)
+        rows.append((dq[0], dq[1], ek(vel, hot), ek(vel, cold)))
+        if sample is not None and every > 0 and (step + phase) % every == 0:
+            samples.append(sample(pos, vel))
+    return {"ep": np.array(ep), "ek": np.array(ekin), "econs": np.array(ec),
+            "rows": np.array(rows), "positions": pos, "velocities": vel, "rebuilds": rebuilds,
+            "samples": samples, "heat": Q}
