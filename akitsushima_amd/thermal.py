@@ -20,7 +20,7 @@ import warnings
 
 import numpy as np
 
-from .md import K_B, maxwell_boltzmann, masses_of, md_run, nvt_run
+from .md import EV_ANG3_TO_GPA, K_B, maxwell_boltzmann, masses_of, md_run, nvt_run
 
 EV_ANG_K_FS_TO_W_MK = (1.60217663e-19) / (1e-10 * 1e-15)
 
@@ -98,9 +98,12 @@ class ThermalConductivityCalculator:
         the heat flux of each group of atoms (per-atom heat currents,
         regions by each frame's positions) for compute_kappa_decomposition;
         save_atom_flux writes the per-atom currents J_i / V of every frame,
-        float32, to output_dir/atom_heat_flux.npy (frames, N, 3).  traj_every and
-        thermo_every are accepted for the reference's signature; the device
-        run writes heat_flux.dat only.  Returns {"J_series" (frames, 3)
+        float32, to output_dir/atom_heat_flux.npy (frames, N, 3).  With
+        output_dir, the NVE run's thermo_nve.log (every thermo_every steps:
+        T = 2 E_kin / (3 N k_B), E_pot, E_tot, P = Tr(sigma) / 3),
+        trajectory_tc.xyz (every traj_every steps, frames recorded on the
+        device) and heat_flux.dat are written in the reference's formats
+        (:560-620).  Returns {"J_series" (frames, 3)
         [eV/(A^2 fs)], "dt_hf" fs, "positions", "velocities", "e_tot"
         (NVE), "equil" (the NVT run's record or None)}."""
         decompose = dict(decompose or {})
@@ -133,10 +136,12 @@ class ThermalConductivityCalculator:
                 region_names = [f"[{bounds[k]:.3f},{bounds[k + 1]:.3f})"
                                 for k in range(len(bounds) - 1)]
             groups = (labels, n_lab, region)
+        traj = int(traj_every) if output_dir and 0 < int(traj_every) <= n_steps else 0
+        pos0 = pos.copy()
         prod = md_run(self.potential, self.lattice, pos, vel, self.atom_types, dt=timestep,
                       n_steps=n_steps, skin_width=skin_width, masses=self.masses,
                       flux_every=int(heat_flux_every), flux_groups=groups,
-                      atom_flux=save_atoms)
+                      atom_flux=save_atoms, frame_every=traj)
         J = prod["flux"]
         self._J_series = J
         self._J_groups = {}
@@ -155,11 +160,79 @@ class ThermalConductivityCalculator:
                     fh.write(f"{k * heat_flux_every + 1}  {j[0]:.8e}  {j[1]:.8e}  {j[2]:.8e}\n")
             if save_atoms:
                 np.save(os.path.join(output_dir, "atom_heat_flux.npy"), prod["atom_flux"])
+            self._write_nve_logs(output_dir, prod, pos0, n_steps, int(thermo_every), traj)
         if verbose:
             print(f"  NVE production: {n_steps} steps, {len(J)} heat-flux frames, "
                   f"{prod['seconds']:.2f} s on the device")
         return {"J_series": J, "dt_hf": self._dt_hf, "positions": prod["positions"],
                 "velocities": prod["velocities"], "e_tot": prod["e_tot"], "equil": eq}
+
+    def _write_nve_logs(self, output_dir, prod, pos0, n_steps, thermo_every, traj_every):
+        """thermo_nve.log and trajectory_tc.xyz of the NVE run from the
+        device records (row c = the state after c steps, c = 0 the start)."""
+        V = abs(np.linalg.det(self.lattice))
+        N = self.N
+        ek, ep = prod["e_kin"], prod["e_pot"]
+        # the log's P is the virial part Tr(sigma) / 3 of the recorded P_inst
+        p_vir = prod["pressure_gpa"] - 2.0 * ek / (3.0 * V) * EV_ANG3_TO_GPA
+        with open(os.path.join(output_dir, "thermo_nve.log"), "w") as fh:
+            fh.write(f"{'Step':>8}  {'T(K)':>9}  {'E_pot(eV)':>14}  {'E_tot(eV)':>14}  "
+                     f"{'P(GPa)':>9}\n")
+            for c in range(0, n_steps + 1, max(thermo_every, 1)):
+                T = ek[c] * 2.0 / (3.0 * N * K_B)
+                fh.write(f"{c:8d}  {T:9.2f}  {ep[c]:14.6f}  {ep[c] + ek[c]:14.6f}  "
+                         f"{p_vir[c]:9.4f}\n")
+        if traj_every <= 0:
+            return
+        a, b, cc = self.lattice
+        head = (f"Lattice=\"{a[0]:.6f} {a[1]:.6f} {a[2]:.6f} {b[0]:.6f} {b[1]:.6f} "
+                f"{b[2]:.6f} {cc[0]:.6f} {cc[1]:.6f} {cc[2]:.6f}\" "
+                f"Properties=species:S:1:pos:R:3 ")
+        frames = [(0, pos0)] + [((k + 1) * traj_every, f)
+                                for k, f in enumerate(prod["frames"])]
+        with open(os.path.join(output_dir, "trajectory_tc.xyz"), "w") as fh:
+            for c, p in frames:
+                fh.write(f"{N}\n{head}step={c}\n")
+                for sp, x in zip(self.atom_types, p):
+                    fh.write(f"{sp}  {x[0]:.8f}  {x[1]:.8f}  {x[2]:.8f}\n")
+
+    def save_results(self, output_dir=".", verbose=True):
+        """hcacf.csv, kappa_running.csv and kappa_tensor.csv of the last
+        compute_kappa in the reference's layouts (save_results,
+        solvers/thermal_conductivity.py:1017-1069; no plots)."""
+        os.makedirs(output_dir, exist_ok=True)
+        if self._hcacf is not None and self._tau_axis is not None:
+            c = self._hcacf
+            path = os.path.join(output_dir, "hcacf.csv")
+            with open(path, "w") as fh:
+                fh.write("tau_ps,Cxx,Cyy,Czz,Cxy,Cxz,Cyz\n")
+                for i, tau in enumerate(self._tau_axis):
+                    fh.write(f"{tau:.6f},{c[i, 0, 0]:.8e},{c[i, 1, 1]:.8e},{c[i, 2, 2]:.8e},"
+                             f"{c[i, 0, 1]:.8e},{c[i, 0, 2]:.8e},{c[i, 1, 2]:.8e}\n")
+            if verbose:
+                print(f"  HCACF      -> {path}")
+        if self._kappa_run is not None:
+            path = os.path.join(output_dir, "kappa_running.csv")
+            with open(path, "w") as fh:
+                fh.write("tau_ps,kappa_xx,kappa_yy,kappa_zz,kappa_avg  # W/(m.K)\n")
+                for i, tau in enumerate(self._tau_axis):
+                    k = self._kappa_run[i]
+                    avg = (k[0, 0] + k[1, 1] + k[2, 2]) / 3
+                    fh.write(f"{tau:.6f},{k[0, 0]:.4f},{k[1, 1]:.4f},{k[2, 2]:.4f},"
+                             f"{avg:.4f}\n")
+            if verbose:
+                print(f"  Running kappa -> {path}")
+        if self._kappa is not None:
+            path = os.path.join(output_dir, "kappa_tensor.csv")
+            std = self._kappa_std if self._kappa_std is not None else np.zeros((3, 3))
+            with open(path, "w") as fh:
+                fh.write("# Thermal conductivity tensor [W/(m.K)]\n")
+                fh.write("alpha,beta,kappa_WmK,kappa_std_WmK\n")
+                for a in range(3):
+                    for b in range(3):
+                        fh.write(f"{a},{b},{self._kappa[a, b]:.4f},{std[a, b]:.4f}\n")
+            if verbose:
+                print(f"  kappa tensor -> {path}")
 
     def compute_kappa(self, J_series=None, dt_hf=None, tau_max=20.0, n_blocks=5, volume=None,
                       verbose=False):

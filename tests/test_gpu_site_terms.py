@@ -377,10 +377,23 @@ def test_green_kubo_decomposition_sums_to_the_total(tmp_path):
     lat, pos, t = _gesn(2)
     pot = StillingerWeberPotential(SW_GESN, species=GESN_SPECIES)
     tc = ThermalConductivityCalculator(lat, pos, t, pot, temperature=500.0)
-    tc.run_md(n_steps=400, timestep=1.0, equil_steps=50, heat_flux_every=2, seed=3,
-              output_dir=str(tmp_path),
+    out = tc.run_md(n_steps=400, timestep=1.0, equil_steps=50, heat_flux_every=2, seed=3,
+              output_dir=str(tmp_path), thermo_every=50, traj_every=100,
               decompose={"species": True, "save_atom_flux": True,
                          "regions": {"axis": "c", "bounds": [0, 0.5, 1]}})
+    # the NVE logs in the reference's formats, from the device records
+    rows = [ln.split() for ln in open(tmp_path / "thermo_nve.log")][1:]
+    assert [int(r[0]) for r in rows] == list(range(0, 401, 50))
+    assert np.allclose([float(r[3]) for r in rows], out["e_tot"][::50], atol=2e-6)
+    p0 = out["equil"]["positions"]
+    _, _, S0 = pot.compute(lat, p0, t)
+    assert abs(float(rows[0][4]) - np.trace(S0) / 3 * 160.2176621) < 2e-4
+    xyz = open(tmp_path / "trajectory_tc.xyz").read().split("\n")
+    n = len(t)
+    steps = [int(ln.rsplit("step=", 1)[1]) for ln in xyz if "step=" in ln]
+    assert steps == [0, 100, 200, 300, 400]
+    last = np.array([[float(x) for x in ln.split()[1:]] for ln in xyz[-n - 1:-1]])
+    assert np.abs(last - out["positions"]).max() < 1e-7
     flux = np.load(tmp_path / "atom_heat_flux.npy")
     assert flux.shape == (200, len(t), 3) and flux.dtype == np.float32
     assert np.allclose(flux.sum(axis=1), tc._J_series, rtol=1e-5,

@@ -210,3 +210,27 @@ def test_kappa_decomposition_is_the_brute_force_correlation_integral():
         want = brute(G[:, q], J)
         assert np.allclose(out["species"][g], want, rtol=1e-9, atol=1e-12 * np.abs(want).max())
     assert out["sum_rule_residual"]["species"] < 1e-12 * np.abs(total).max()
+
+
+def test_save_results_writes_the_reference_csv_layouts(tmp_path):
+    """compute_kappa on a synthetic flux series, then save_results: the
+    three CSV files with the reference's headers, one row per lag, and the
+    tensor rows equal to the returned kappa (save_results,
+    solvers/thermal_conductivity.py:1017-1069)."""
+    from akitsushima_amd import StillingerWeberPotential
+    from akitsushima_amd.thermal import ThermalConductivityCalculator
+    lat, pos, _ = sheared_si(1, shear=0.0, rattle=0.0, seed=0)
+    tc = ThermalConductivityCalculator(lat, pos, ["Si"] * len(pos),
+                                       StillingerWeberPotential(SW_SI, species=["Si"]),
+                                       temperature=300.0)
+    J = np.random.default_rng(1).normal(size=(400, 3)) * 1e-6
+    kappa = tc.compute_kappa(J_series=J, dt_hf=2.0, tau_max=0.05, n_blocks=2)
+    tc.save_results(str(tmp_path), verbose=False)
+    h = open(tmp_path / "hcacf.csv").read().splitlines()
+    assert h[0] == "tau_ps,Cxx,Cyy,Czz,Cxy,Cxz,Cyz" and len(h) == 1 + 25
+    r = open(tmp_path / "kappa_running.csv").read().splitlines()
+    assert r[0].startswith("tau_ps,kappa_xx,kappa_yy,kappa_zz,kappa_avg") and len(r) == 26
+    t = [ln.split(",") for ln in open(tmp_path / "kappa_tensor.csv").read().splitlines()[2:]]
+    assert len(t) == 9
+    got = np.array([float(x[2]) for x in t]).reshape(3, 3)
+    assert np.allclose(got, kappa, atol=6e-5)
