@@ -149,6 +149,9 @@ struct Potential {
   bool sw_multi = false;
   // per-call scratch; amask (n) = bonds inside r_c per atom (rows >= 8)
   DevBuf G, aux, out, amask;
+  // EAM: per structure, 1 when any bond lies inside r_c (set by the density
+  // kernel, read and cleared by the structure reduction)
+  DevBuf pair_flag;
   // launch shape chosen by the autotuner (0 = default)
   int eam_lanes = 0;
   // Tersoff site kernel form: 2 = one lane per (bond, partner) pair (list

@@ -784,7 +784,8 @@ __device__ __forceinline__ void struct_reduce_block(int s, const Geom* __restric
                                                     const int* __restrict__ first,
                                                     int n_chunks,
                                                     const double* __restrict__ partial,
-                                                    double* __restrict__ out) {
+                                                    double* __restrict__ out,
+                                                    int* __restrict__ pair_flag = nullptr) {
   __shared__ double sm[10][4];
   const int c0 = first[s], c1 = first[s + 1];
   double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -800,19 +801,26 @@ __device__ __forceinline__ void struct_reduce_block(int s, const Geom* __restric
     for (int q = 0; q < 10; ++q) sm[q][wid] = v[q];
   }
   __syncthreads();
+  // EAM: a structure without any bond inside r_c gives zeros, its isolated
+  // atoms' embedding energy F(0) included (EAMPotential.compute,
+  // engine/eam.py:236-237); the flag is cleared for the next call
+  const bool none = pair_flag && pair_flag[s] == 0;
   if (threadIdx.x < 10) {
     const int q = threadIdx.x;
     double t = ((sm[q][0] + sm[q][1]) + sm[q][2]) + sm[q][3];
-    out[10 * (size_t)s + q] = q == 0 ? t : t / geoms[s].volume;
+    out[10 * (size_t)s + q] = none ? 0.0 : (q == 0 ? t : t / geoms[s].volume);
   }
+  __syncthreads();
+  if (pair_flag && threadIdx.x == 0) pair_flag[s] = 0;
 }
 
 __global__ __launch_bounds__(256) void k_struct_final(const Geom* __restrict__ geoms,
                                                       const int* __restrict__ first,
                                                       int n_chunks,
                                                       const double* __restrict__ partial,
-                                                      double* __restrict__ out) {
-  struct_reduce_block(blockIdx.x, geoms, first, n_chunks, partial, out);
+                                                      double* __restrict__ out,
+                                                      int* __restrict__ pair_flag) {
+  struct_reduce_block(blockIdx.x, geoms, first, n_chunks, partial, out, pair_flag);
 }
 
 // F_a = sum_k G[a, k] - G[rev(a, k)].  Blocks past the atom blocks reduce
@@ -1521,12 +1529,13 @@ __global__ __launch_bounds__(256) void k_eam_rho(
     const Geom* __restrict__ geoms, int single, const int* __restrict__ spc, int ns,
     const double* __restrict__ fn, const int* __restrict__ map, double rc,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
-    double* __restrict__ dF, double* __restrict__ Fe) {
+    double* __restrict__ dF, double* __restrict__ Fe, int* __restrict__ pair_flag) {
   const int gt = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int a = gt / LE;
   const int l = gt & (LE - 1);
   const bool live = a < n;
   double rho = 0.0;
+  double n_in = 0.0;  // bonds inside r_c
   if (live) {
     const Geom& g = geoms[single ? 0 : sid[a]];
     const int nk = cnt[a];
@@ -1540,10 +1549,17 @@ __global__ __launch_bounds__(256) void k_eam_rho(
         double v, dv;
         z_feval(fn, fn + (size_t)ZW * map[sj], r, v, dv);
         rho += v;
+        n_in += 1.0;
       }
     }
   }
   rho = seg_sum<LE>(rho);
+  n_in = seg_sum<LE>(n_in);
+  if (live && l == 0 && n_in > 0.0) {
+    // every writer stores the same value; the read only spares stores
+    int* fl = pair_flag + (single ? 0 : sid[a]);
+    if (*fl == 0) *fl = 1;
+  }
   if (live && l == 0) {
     const int sa = ns > 1 ? spc[a] : 0;
     double F, d;
@@ -1772,7 +1788,8 @@ static void launch_eam(Potential& pot, NList& nl, SetDev& set, const double* d_p
   hipLaunchKernelGGL(k_eam_rho<LE>, dim3(grid), dim3(256), 0, st, n, d_pos,
                      set.d_sid.as<int>(), set.d_geoms.as<Geom>(), single, d_spc, pot.ns,
                      pot.d_params.as<double>(), pot.d_map.as<int>(), pot.r_cut,
-                     nl.cnt.as<int>(), nl.nbr.as<int>(), nl.stride, dF, Fe);
+                     nl.cnt.as<int>(), nl.nbr.as<int>(), nl.stride, dF, Fe,
+                     pot.pair_flag.as<int>());
   launch_timed(pot, k_eam_force<LE>, chunk_grid(pot, ch), dim3(pot.block), st, (size_t)0, ch.n,
                d_pos,
                (const int*)set.d_sid.as<int>(), (const Geom*)set.d_geoms.as<Geom>(), single, d_spc,
@@ -1865,6 +1882,11 @@ void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
   while (swm && blk > 64 && sw_m_lds(blk, lanes, atom_lanes, pot.ns) > 48 * 1024) blk >>= 1;
   AK_REQUIRE(atom_lanes <= blk, AK_ERR_STATE, "block smaller than one atom's lanes");
   SetDev::Chunks& ch = set.chunks(blk / atom_lanes, st);
+  if (pot.kind == KIND_EAM && pot.pair_flag.bytes < sizeof(int) * (size_t)set.n_struct) {
+    // cleared once here, then by every structure reduction after reading it
+    pot.pair_flag.ensure(sizeof(int) * (size_t)set.n_struct);
+    AK_HIP(hipMemsetAsync(pot.pair_flag.p, 0, pot.pair_flag.bytes, st));
+  }
   if (n > 0) {
     if (pot.kind == KIND_EAM) {
       pot.aux.ensure(sizeof(double) * 2 * (size_t)n);
@@ -1891,7 +1913,8 @@ void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
   if (pot.kind == KIND_EAM || n == 0)
     hipLaunchKernelGGL(k_struct_final, dim3(set.n_struct), dim3(256), 0, st,
                        set.d_geoms.as<Geom>(), (const int*)ch.first.as<int>(), ch.n,
-                       (const double*)ch.partial.as<double>(), d_out);
+                       (const double*)ch.partial.as<double>(), d_out,
+                       pot.kind == KIND_EAM ? pot.pair_flag.as<int>() : (int*)nullptr);
 }
 
 // ---------------------------------------------------------------------------
