@@ -149,8 +149,7 @@ struct Potential {
   bool sw_multi = false;
   // per-call scratch; amask (n) = bonds inside r_c per atom (rows >= 8)
   DevBuf G, aux, out, amask;
-  // EAM: per structure, 1 when any bond lies inside r_c (set by the density
-  // kernel, read and cleared by the structure reduction)
+  // EAM: per launch chunk, 1 when any of its bonds lies inside r_c
   DevBuf pair_flag;
   // launch shape chosen by the autotuner (0 = default)
   int eam_lanes = 0;

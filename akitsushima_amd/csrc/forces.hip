@@ -785,7 +785,7 @@ __device__ __forceinline__ void struct_reduce_block(int s, const Geom* __restric
                                                     int n_chunks,
                                                     const double* __restrict__ partial,
                                                     double* __restrict__ out,
-                                                    int* __restrict__ pair_flag = nullptr) {
+                                                    const int* __restrict__ pair_flag = nullptr) {
   __shared__ double sm[10][4];
   const int c0 = first[s], c1 = first[s + 1];
   double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -803,15 +803,16 @@ __device__ __forceinline__ void struct_reduce_block(int s, const Geom* __restric
   __syncthreads();
   // EAM: a structure without any bond inside r_c gives zeros, its isolated
   // atoms' embedding energy F(0) included (EAMPotential.compute,
-  // engine/eam.py:236-237); the flag is cleared for the next call
-  const bool none = pair_flag && pair_flag[s] == 0;
+  // engine/eam.py:236-237); pair_flag holds one flag per chunk
+  int any = 0;
+  if (pair_flag)
+    for (int c = c0 + threadIdx.x; c < c1; c += 256) any |= pair_flag[c];
+  const bool none = pair_flag && !__syncthreads_or(any);
   if (threadIdx.x < 10) {
     const int q = threadIdx.x;
     double t = ((sm[q][0] + sm[q][1]) + sm[q][2]) + sm[q][3];
     out[10 * (size_t)s + q] = none ? 0.0 : (q == 0 ? t : t / geoms[s].volume);
   }
-  __syncthreads();
-  if (pair_flag && threadIdx.x == 0) pair_flag[s] = 0;
 }
 
 __global__ __launch_bounds__(256) void k_struct_final(const Geom* __restrict__ geoms,
@@ -819,7 +820,7 @@ __global__ __launch_bounds__(256) void k_struct_final(const Geom* __restrict__ g
                                                       int n_chunks,
                                                       const double* __restrict__ partial,
                                                       double* __restrict__ out,
-                                                      int* __restrict__ pair_flag) {
+                                                      const int* __restrict__ pair_flag) {
   struct_reduce_block(blockIdx.x, geoms, first, n_chunks, partial, out, pair_flag);
 }
 
@@ -1529,13 +1530,12 @@ __global__ __launch_bounds__(256) void k_eam_rho(
     const Geom* __restrict__ geoms, int single, const int* __restrict__ spc, int ns,
     const double* __restrict__ fn, const int* __restrict__ map, double rc,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
-    double* __restrict__ dF, double* __restrict__ Fe, int* __restrict__ pair_flag) {
+    double* __restrict__ dF, double* __restrict__ Fe) {
   const int gt = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int a = gt / LE;
   const int l = gt & (LE - 1);
   const bool live = a < n;
   double rho = 0.0;
-  double n_in = 0.0;  // bonds inside r_c
   if (live) {
     const Geom& g = geoms[single ? 0 : sid[a]];
     const int nk = cnt[a];
@@ -1549,17 +1549,10 @@ __global__ __launch_bounds__(256) void k_eam_rho(
         double v, dv;
         z_feval(fn, fn + (size_t)ZW * map[sj], r, v, dv);
         rho += v;
-        n_in += 1.0;
       }
     }
   }
   rho = seg_sum<LE>(rho);
-  n_in = seg_sum<LE>(n_in);
-  if (live && l == 0 && n_in > 0.0) {
-    // every writer stores the same value; the read only spares stores
-    int* fl = pair_flag + (single ? 0 : sid[a]);
-    if (*fl == 0) *fl = 1;
-  }
   if (live && l == 0) {
     const int sa = ns > 1 ? spc[a] : 0;
     double F, d;
@@ -1577,10 +1570,11 @@ __device__ __forceinline__ void k_eam_force_chunk(int c, int n_chunks,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const double* __restrict__ dF, const double* __restrict__ Fe,
     const int* __restrict__ chunk_rng, double* __restrict__ F, double* __restrict__ partial,
-    const double* __restrict__ fast, double* __restrict__ esite) {
+    const double* __restrict__ fast, double* __restrict__ esite, int* __restrict__ pair_flag) {
   const int a = chunk_rng[2 * c] + (int)threadIdx.x / LE;
   const int l = threadIdx.x & (LE - 1);
   const bool live = a < chunk_rng[2 * c + 1];
+  int bonded = 0;  // a bond inside r_c seen by this lane
   double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
   double w[6] = {0, 0, 0, 0, 0, 0};  // xx, xy, xz, yy, yz, zz
   if (live) {
@@ -1608,6 +1602,7 @@ __device__ __forceinline__ void k_eam_force_chunk(int c, int n_chunks,
             z_feval(fn, fa_rho, r, v, dfa);
           z_feval(fn, fn + (size_t)ZW * map[2 * ns + sa * ns + sj], r, phi, dphi);
         }
+        bonded = 1;
         double c = dphi + dFa * dfj + dF[j] * dfa;
         double s = c / r;
         fx += s * ux;
@@ -1638,6 +1633,8 @@ __device__ __forceinline__ void k_eam_force_chunk(int c, int n_chunks,
   }
   double v[10] = {e, w[0], w[1], w[2], w[1], w[3], w[4], w[2], w[4], w[5]};
   wave_partial(v, partial, c, n_chunks);
+  const int any = __syncthreads_or(bonded);
+  if (threadIdx.x == 0) pair_flag[c] = any;
 }
 
 template <int LE>
@@ -1647,9 +1644,9 @@ __global__ __launch_bounds__(256) void k_eam_force(int n_chunks, const double* _
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const double* __restrict__ dF, const double* __restrict__ Fe,
     const int* __restrict__ chunk_rng, double* __restrict__ F, double* __restrict__ partial,
-    const double* __restrict__ fast, double* __restrict__ esite) {
+    const double* __restrict__ fast, double* __restrict__ esite, int* __restrict__ pair_flag) {
   for (int cc = blockIdx.x; cc < n_chunks; cc += gridDim.x)
-    k_eam_force_chunk<LE>(xcd_remap(cc, n_chunks), n_chunks, pos, sid, geoms, single, spc, ns, fn, map, rc, cnt, nbr, stride, dF, Fe, chunk_rng, F, partial, fast, esite);
+    k_eam_force_chunk<LE>(xcd_remap(cc, n_chunks), n_chunks, pos, sid, geoms, single, spc, ns, fn, map, rc, cnt, nbr, stride, dF, Fe, chunk_rng, F, partial, fast, esite, pair_flag);
 }
 
 // Function f of the table at m arguments (tabulation of a potential).
@@ -1788,8 +1785,7 @@ static void launch_eam(Potential& pot, NList& nl, SetDev& set, const double* d_p
   hipLaunchKernelGGL(k_eam_rho<LE>, dim3(grid), dim3(256), 0, st, n, d_pos,
                      set.d_sid.as<int>(), set.d_geoms.as<Geom>(), single, d_spc, pot.ns,
                      pot.d_params.as<double>(), pot.d_map.as<int>(), pot.r_cut,
-                     nl.cnt.as<int>(), nl.nbr.as<int>(), nl.stride, dF, Fe,
-                     pot.pair_flag.as<int>());
+                     nl.cnt.as<int>(), nl.nbr.as<int>(), nl.stride, dF, Fe);
   launch_timed(pot, k_eam_force<LE>, chunk_grid(pot, ch), dim3(pot.block), st, (size_t)0, ch.n,
                d_pos,
                (const int*)set.d_sid.as<int>(), (const Geom*)set.d_geoms.as<Geom>(), single, d_spc,
@@ -1797,7 +1793,8 @@ static void launch_eam(Potential& pot, NList& nl, SetDev& set, const double* d_p
                (const int*)pot.d_map.as<int>(), pot.r_cut, (const int*)nl.cnt.as<int>(),
                (const int*)nl.nbr.as<int>(), nl.stride, (const double*)dF, (const double*)Fe,
                (const int*)ch.rng.as<int>(), d_F, ch.partial.as<double>(),
-               (const double*)(pot.eam_fast ? pot.d_eam_fast.as<double>() : nullptr), esite);
+               (const double*)(pot.eam_fast ? pot.d_eam_fast.as<double>() : nullptr), esite,
+               pot.pair_flag.as<int>());
 }
 
 bool eam_fast_table(int ns, const std::vector<double>& rows, const std::vector<int>& map,
@@ -1882,11 +1879,8 @@ void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
   while (swm && blk > 64 && sw_m_lds(blk, lanes, atom_lanes, pot.ns) > 48 * 1024) blk >>= 1;
   AK_REQUIRE(atom_lanes <= blk, AK_ERR_STATE, "block smaller than one atom's lanes");
   SetDev::Chunks& ch = set.chunks(blk / atom_lanes, st);
-  if (pot.kind == KIND_EAM && pot.pair_flag.bytes < sizeof(int) * (size_t)set.n_struct) {
-    // cleared once here, then by every structure reduction after reading it
-    pot.pair_flag.ensure(sizeof(int) * (size_t)set.n_struct);
-    AK_HIP(hipMemsetAsync(pot.pair_flag.p, 0, pot.pair_flag.bytes, st));
-  }
+  // every chunk writes its flag in each call (no clearing needed)
+  if (pot.kind == KIND_EAM) pot.pair_flag.ensure(sizeof(int) * (size_t)std::max(1, ch.n));
   if (n > 0) {
     if (pot.kind == KIND_EAM) {
       pot.aux.ensure(sizeof(double) * 2 * (size_t)n);
@@ -1914,7 +1908,8 @@ void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
     hipLaunchKernelGGL(k_struct_final, dim3(set.n_struct), dim3(256), 0, st,
                        set.d_geoms.as<Geom>(), (const int*)ch.first.as<int>(), ch.n,
                        (const double*)ch.partial.as<double>(), d_out,
-                       pot.kind == KIND_EAM ? pot.pair_flag.as<int>() : (int*)nullptr);
+                       pot.kind == KIND_EAM && n > 0 ? (const int*)pot.pair_flag.as<int>()
+                                                     : (const int*)nullptr);
 }
 
 // ---------------------------------------------------------------------------
