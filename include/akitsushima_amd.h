@@ -133,7 +133,10 @@ int ak_nlist_counters(ak_nlist *nl, int64_t *n_builds, int64_t *n_steps);
  * (engine/tersoff.py:1208-1228, engine/stillinger_weber.py:900-922,
  * engine/eam.py:174-256).  nl == NULL: the potential keeps a list of its
  * own with skin 0.3 A (owned_neighbour_list, engine/neighborlist.py:1103-1163).
- * Host buffers; stress may be NULL when need_stress == 0. */
+ * Host buffers; stress may be NULL when need_stress == 0.  A structure with
+ * no bond inside r_cut gives zero energy, forces and stress for every kind,
+ * EAM included (engine/tersoff.py:1293-1294, engine/eam.py:236-237); n_atoms
+ * may be 0. */
 int ak_compute(ak_potential *pot, ak_nlist *nl, int32_t n_atoms, const double *lattice,
                const double *inv_lattice, const double *positions, const int32_t *species,
                int32_t need_stress, double *energy, double *forces, double *stress);
