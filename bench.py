@@ -228,11 +228,13 @@ def sw_gk(n_steps=1000):
     """The device MD variants on the SW Si 20^3 cell (64000 atoms, 1000 K
     start, dt 1 fs): NVE with the heat flux recorded every step (a Green-Kubo
     production run), NVT with a 3-link Nose-Hoover chain (900 K, tau 100
-    fs), Langevin and CSVR (damping 100 fs) and isotropic NPT (0 GPa, tau_baro
-    1000 fs); ms per hipGraph step and the conserved quantities' ranges."""
+    fs), Langevin and CSVR (damping 100 fs), isotropic NPT (0 GPa, tau_baro
+    1000 fs) and NEMD (hot slab 1100 K, cold slab 900 K, damping 100 fs, 20
+    bins sampled every 10 steps); ms per hipGraph step and the conserved
+    quantities' ranges."""
     from akitsushima_amd import StillingerWeberPotential
     from akitsushima_amd.md import (csvr_run, langevin_run, masses_of, maxwell_boltzmann, md_run,
-                                    npt_run, nvt_run)
+                                    nemd_run, npt_run, nvt_run)
     from akitsushima_amd.published import SW_SI
     from akitsushima_amd.structures import DIAMOND, cubic
 
@@ -252,6 +254,10 @@ def sw_gk(n_steps=1000):
                  skin_width=0.3)
     e = npt_run(pot, lat, pos, vel, types, 900.0, pressure=0.0, dt=1.0, n_steps=n_steps,
                 skin_width=0.3)
+    fz = (pos @ np.linalg.inv(lat))[:, 2] % 1.0
+    bath = np.where(fz < 0.05, 1, np.where((fz >= 0.5) & (fz < 0.55), 2, 0))
+    g = nemd_run(pot, lat, pos, vel, types, bath, 1100.0, 900.0, 100.0, dt=1.0, n_steps=n_steps,
+                 skin_width=0.3, sample_every=10, axis=2, n_bins=20)
     return {"workload": f"SW Si diamond 20^3 ({n} atoms), dt 1 fs, {n_steps} steps, "
                         "skin 0.3 A, hipGraph-captured steps",
             "nve_flux_every_step_ms_per_step": a["seconds"] / n_steps * 1e3,
@@ -264,7 +270,12 @@ def sw_gk(n_steps=1000):
             "csvr_ms_per_step": d["seconds"] / n_steps * 1e3,
             "npt_ms_per_step": e["seconds"] / n_steps * 1e3,
             "npt_h_range_per_atom_eV": float((e["e_cons"].max() - e["e_cons"].min()) / n),
-            "npt_volume_change": float(e["volume"][-1] / e["volume"][0] - 1.0)}
+            "npt_volume_change": float(e["volume"][-1] / e["volume"][0] - 1.0),
+            "nemd_ms_per_step": g["seconds"] / n_steps * 1e3,
+            "nemd_samples": int(len(g["bins"])),
+            "nemd_e_minus_q_range_per_atom_eV": float((g["e_cons"].max() - g["e_cons"].min()) / n),
+            "nemd_q_hot_eV": float(g["bath"][1:, 0].sum()),
+            "nemd_q_cold_eV": float(g["bath"][1:, 1].sum())}
 
 
 def eam_cuni(steps, warmup):
