@@ -1348,7 +1348,7 @@ static void thermo_kernel(MDState& S, int mode, double dt, hipStream_t st, unsig
 // half-step and record;] [heat flux].  For NVE the state after the last step
 // is recorded by md_record after the loop.
 static void md_step(Potential& pot, NList& nl, SetDev& set, MDState& S, int n, double dt,
-                    hipStream_t st) {
+                    hipStream_t st, bool flux_now = true) {
   const unsigned nb = (unsigned)((n + 255) / 256);
   nl.disp_part.ensure(sizeof(double) * nb);
   if (S.scaled()) thermo_kernel(S, 1, dt, st, nb);
@@ -1404,7 +1404,8 @@ static void md_step(Potential& pot, NList& nl, SetDev& set, MDState& S, int n, d
                        S.hist_ek.as<double>(), S.hist_ec.as<double>(), S.V,
                        S.hist_P.as<double>(), S.hist_nb.as<int>());
   nl_build_gated(nl, set, S.pos.as<double>(), S.flag.as<int>(), st);
-  if (S.every > 0)
+  // site energies and gradients only on the steps that record the flux
+  if (S.every > 0 && flux_now)
     site_grads(pot, nl, set, S.pos.as<double>(), S.spc.as<int>(), S.F.as<double>(),
                S.es.as<double>(), S.esite.as<double>(), st);
   else
@@ -1412,11 +1413,12 @@ static void md_step(Potential& pot, NList& nl, SetDev& set, MDState& S, int n, d
                    S.es.as<double>(), st);
   kick(S, n, dt, 1, st);
   if (S.thermo) thermo_kernel(S, S.thermo == 2 ? 1 : 2, dt, st, nb);
-  if (S.every > 0)
+  // a recording step: the gate counts the records, so frame = counters[2]
+  if (S.every > 0 && flux_now)
     heat_flux(pot, nl, set, S.pos.as<double>(), S.vel.as<double>(), S.mass.as<double>(),
               S.esite.as<double>(), S.scaled() ? S.nhc.as<double>() : nullptr, kConv,
-              S.counters.as<int>(), S.every, S.flux_part.as<double>(), S.hist_J.as<double>(), st,
-              S.fg.ng > 0 ? &S.fg : nullptr, S.flux_phase);
+              S.counters.as<int>(), 1, S.flux_part.as<double>(), S.hist_J.as<double>(), st,
+              S.fg.ng > 0 ? &S.fg : nullptr, 0);
   if (S.frame_every > 0)
     hipLaunchKernelGGL(k_frame, dim3((unsigned)((3 * (int64_t)n + 255) / 256)), dim3(256), 0,
                        st, n, (const double*)S.pos.as<double>(), (const Geom*)S.geom,
@@ -1757,6 +1759,9 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
   // step with dt = 0 leaves positions, velocities, the chain and the state-0
   // sums as they were
   md_step(pot, nl, set, S, n, 0.0, st);
+  // with a sampled flux, the steps between records take the plain force path
+  // (its launch shape and chunk table may differ): set that up too
+  if (S.every > 0) md_step(pot, nl, set, S, n, 0.0, st, false);
   AK_HIP(hipStreamSynchronize(st));
   // restart the record, the rebuild count and the flux gate: step k records
   // state k - 1 (NVE) or k (chain; state 0 recorded here); the random
@@ -1768,25 +1773,38 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
   AK_HIP(hipMemcpy(S.counters.as<int>(), reset, sizeof(reset), hipMemcpyHostToDevice));
   if (S.thermo) md_record(S, n, st);
 
+  // the steps s with (s + phase) % every == 0 record the heat flux; the
+  // others run the plain force path and launch no flux kernel
+  const auto records = [&](int s) {
+    return S.every > 0 && (s + S.flux_phase) % S.every == 0;
+  };
   auto t0 = std::chrono::steady_clock::now();
   if (use_graph && n_steps > 0) {
-    hipGraph_t graph;
-    hipGraphExec_t exec;
-    AK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    md_step(pot, nl, set, S, n, dt, st);
-    AK_HIP(hipStreamEndCapture(st, &graph));
-    AK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    // one captured step of each kind that occurs
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    bool need[2] = {false, false};
+    for (int s = 0; s < n_steps && !(need[0] && need[1]); ++s) need[records(s) ? 1 : 0] = true;
+    for (int k = 0; k < 2; ++k) {
+      if (!need[k]) continue;
+      AK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      md_step(pot, nl, set, S, n, dt, st, k == 1);
+      AK_HIP(hipStreamEndCapture(st, &graph[k]));
+      AK_HIP(hipGraphInstantiate(&exec[k], graph[k], nullptr, nullptr, 0));
+    }
     AK_HIP(hipStreamSynchronize(st));
     t0 = std::chrono::steady_clock::now();
-    for (int s = 0; s < n_steps; ++s) AK_HIP(hipGraphLaunch(exec, st));
+    for (int s = 0; s < n_steps; ++s) AK_HIP(hipGraphLaunch(exec[records(s) ? 1 : 0], st));
     if (S.thermo == 0) md_record(S, n, st);
     AK_HIP(hipStreamSynchronize(st));
     auto t1 = std::chrono::steady_clock::now();
     if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
+    for (int k = 0; k < 2; ++k) {
+      if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
+      if (graph[k]) (void)hipGraphDestroy(graph[k]);
+    }
   } else {
-    for (int s = 0; s < n_steps; ++s) md_step(pot, nl, set, S, n, dt, st);
+    for (int s = 0; s < n_steps; ++s) md_step(pot, nl, set, S, n, dt, st, records(s));
     if (S.thermo == 0) md_record(S, n, st);
     AK_HIP(hipStreamSynchronize(st));
     auto t1 = std::chrono::steady_clock::now();
@@ -1801,7 +1819,7 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
   int counters[4];
   AK_HIP(hipMemcpy(counters, S.counters.p, sizeof(counters), hipMemcpyDeviceToHost));
   AK_REQUIRE(counters[0] == n_steps + 1, AK_ERR_STATE, "MD step counter mismatch");
-  AK_REQUIRE(S.every == 0 || counters[2] == n_steps, AK_ERR_STATE, "heat-flux gate mismatch");
+  AK_REQUIRE(S.every == 0 || counters[2] == frames, AK_ERR_STATE, "heat-flux gate mismatch");
   if (state) state[kStSteps] = steps_done + n_steps;
   if (state && (S.thermo == 1 || S.thermo == 4)) {
     std::vector<double> h(2 + 3 * S.M);
