@@ -470,6 +470,195 @@ __global__ __launch_bounds__(256) void k_tersoff_site(int n_chunks, int n, const
 }
 
 // ---------------------------------------------------------------------------
+// Tersoff site kernel for the flat one-species form with rows of four slots
+// (diamond-like solids, list skin below the second shell): m = 3, gamma = 1
+// as the reference's flat path hard-codes them (engine/tersoff.py:86, 245,
+// 494, 1431).  One DPP quad per atom, lane k = bond k.  The partner legs
+// arrive by quad rotations (lane k reads lane k+P mod 4, P = 1, 2, 3), so
+// every lane evaluates the same three partner terms with no self slot and
+// no out-of-line calls.  g(theta), g'(theta) and the two exponential
+// factors of each (bond, partner) pair are formed once: the factor of the
+// partner's zeta in which this bond is the third leg is the partner lane's
+// own factor for this bond, brought over by the inverse rotation.  Block
+// sums of energy and virial are reduced through LDS.  Terms and gating are
+// those of k_tersoff_site; only the summation order of the zeta sums and of
+// the bond gradients differs.
+// ---------------------------------------------------------------------------
+
+// Lane k of every quad receives lane (k + P) mod 4 of its quad.
+template <int P>
+__device__ __forceinline__ double qrot(double x) {
+  static_assert(P >= 1 && P <= 3, "rotation of a quad");
+  constexpr int ctrl = P == 1 ? 0x39 : (P == 2 ? 0x4E : 0x93);
+  return dpp_quad<ctrl>(x);
+}
+
+// Block sums of NV values per thread (blockDim a multiple of 64, <= 256):
+// thread t stores its values to LDS; one row of 16 lanes serves a value,
+// each lane adding blockDim/16 consecutive entries in thread order, and the
+// row's 16 partial sums are combined by row and quad butterflies.  Fixed
+// order, bitwise reproducible.  Lane 0 of the row of value q writes it to
+// partial[q * n_chunks + c].
+template <int NV>
+__device__ __forceinline__ void block_partial_lds(const double (&v)[NV],
+                                                  double* __restrict__ partial, int c,
+                                                  int n_chunks) {
+  __shared__ double red[NV][256];
+  const int t = threadIdx.x, nt = blockDim.x, per = nt >> 4;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) red[q][t] = v[q];
+  __syncthreads();
+  for (int q = t >> 4; q < NV; q += nt >> 4) {  // row-uniform: whole rows take part
+    const int w = t & 15;
+    double s = 0.0;
+    for (int e = 0; e < per; ++e) s += red[q][w * per + e];
+    s += dpp_row<0x128>(s);  // row_ror:8
+    s += dpp_row<0x124>(s);  // row_ror:4
+    s += dpp_quad<0x4E>(s);
+    s += dpp_quad<0xB1>(s);
+    if (w == 0) partial[(size_t)q * n_chunks + c] = s;
+  }
+  __syncthreads();  // red is written again by the block's next chunk
+}
+
+template <bool L3Z>
+__device__ __forceinline__ void k_tersoff_q4_chunk(
+    int c, int n_chunks, int n, const double* __restrict__ pos, const int* __restrict__ sid,
+    const Geom* __restrict__ geoms, int single, const double* __restrict__ tb,
+    const double* __restrict__ tri, double rc, const int* __restrict__ cnt,
+    const int* __restrict__ nbr, const int* __restrict__ chunk_rng, double* __restrict__ G,
+    double* __restrict__ partial, double* __restrict__ esite) {
+  const int a = chunk_rng[2 * c] + (int)threadIdx.x / 4;
+  const int k = threadIdx.x & 3;
+  const bool live = a < chunk_rng[2 * c + 1];
+  const int ac = live ? a : 0;
+  const int nk = live ? cnt[ac] : 0;
+  const bool has = live && k < nk;
+  const size_t slot = (size_t)ac * 4 + k;
+  const int jr = live ? nbr[slot] : 0;
+  const int j = has ? jr : ac;
+  // one record: tb = [valid, A, B, lambda1, lambda2, R, D, beta, n],
+  // tri = [valid, c, d, h, lambda3, m, gamma]
+  const double A = tb[1], B = tb[2], l1 = tb[3], l2 = tb[4], R = tb[5], D = tb[6];
+  const double beta = tb[7], nn = tb[8];
+  const double cc = tri[1], dd = tri[2], hh = tri[3], l3 = tri[4];
+  const double c2 = cc * cc, d2 = dd * dd, g0 = 1.0 + c2 / d2, l3c = l3 * l3 * l3;
+
+  double ux = 0.0, uy = 0.0, uz = 0.0, r = 0.0, rinv = 0.0;
+  bool act = false;
+  if (has) {
+    const Geom& g = geoms[single ? 0 : sid[ac]];
+    mic_fast(g, pos, ac, j, ux, uy, uz);
+    r = sqrt(ux * ux + uy * uy + uz * uz);
+    act = r < rc && r > 1e-12;
+    rinv = act ? 1.0 / r : 0.0;
+  }
+  // unit vector (zero for slots outside r_c or past the row)
+  const double ex = ux * rinv, ey = uy * rinv, ez = uz * rinv;
+  double fc = 0.0, dfc = 0.0;
+  if (act) te_fc(r, R, D, fc, dfc);
+
+  // zeta of this bond over the three partner legs; per partner P: cos,
+  // g, g' and exp(l3^3 (r - r_P)^3) with its derivative
+  double cs[4], gv[4], dg[4], xe[4], xd[4];
+  double zeta = 0.0;
+#define AK_Q4_PARTNER(P)                                                  \
+  {                                                                       \
+    const double wx = qrot<P>(ex), wy = qrot<P>(ey), wz = qrot<P>(ez);    \
+    const double wfc = qrot<P>(fc);                                       \
+    cs[P] = ex * wx + ey * wy + ez * wz;                                  \
+    const double hc = hh - cs[P];                                         \
+    const double iq = 1.0 / (d2 + hc * hc);                               \
+    gv[P] = g0 - c2 * iq;                                                 \
+    dg[P] = -2.0 * c2 * hc * iq * iq;                                     \
+    if constexpr (L3Z) {                                                  \
+      xe[P] = 1.0;                                                        \
+      xd[P] = 0.0;                                                        \
+      zeta += wfc * gv[P];                                                \
+    } else {                                                              \
+      const double du = r - qrot<P>(r), u2 = du * du;                     \
+      xe[P] = nb_exp(l3c * u2 * du);                                      \
+      xd[P] = xe[P] * 3.0 * l3c * u2;                                     \
+      zeta += wfc * gv[P] * xe[P];                                        \
+    }                                                                     \
+  }
+  AK_Q4_PARTNER(1)
+  AK_Q4_PARTNER(2)
+  AK_Q4_PARTNER(3)
+#undef AK_Q4_PARTNER
+
+  // bond order and the two-body term (engine/tersoff.py:1380-1414)
+  double b = 1.0, db = 0.0;
+  if (act && zeta > 0.0) {
+    const double bz = exp(nn * log(beta * zeta));
+    b = exp((-0.5 / nn) * log1p(bz));
+    db = -0.5 * b * bz / (zeta * (1.0 + bz));
+  }
+  double e = 0.0, dVdr = 0.0, cown = 0.0;
+  if (act) {
+    const double fr = A * nb_exp(-l1 * r);
+    const double fa = -B * nb_exp(-l2 * r);
+    e = 0.5 * fc * (fr + b * fa);
+    dVdr = 0.5 * (dfc * (fr + b * fa) + fc * (-l1 * fr + b * (-l2 * fa)));
+    if (fabs(db) > 1e-30) cown = 0.5 * fc * fa * db;
+  }
+
+  // gradient of E_a in this bond vector: the own bond order (angles to the
+  // partners) and each partner's bond order, in which this bond is the
+  // third leg (its cutoff, angle and exponential factor)
+  double sr = dVdr, swx = 0.0, swy = 0.0, swz = 0.0;
+#define AK_Q4_GRAD(P)                                                                  \
+  {                                                                                    \
+    const double wx = qrot<P>(ex), wy = qrot<P>(ey), wz = qrot<P>(ez);                 \
+    const double wfc = qrot<P>(fc), wc = qrot<P>(cown);                                \
+    double alpha, bet;                                                                 \
+    if constexpr (L3Z) {                                                               \
+      alpha = (cown * wfc + wc * fc) * dg[P];                                          \
+      bet = wc * gv[P] * dfc;                                                          \
+    } else {                                                                           \
+      const double xr = qrot<P>(xe[4 - P]), xrd = qrot<P>(xd[4 - P]);                  \
+      alpha = (cown * wfc * xe[P] + wc * fc * xr) * dg[P];                             \
+      bet = cown * wfc * gv[P] * xd[P] + wc * gv[P] * (dfc * xr - fc * xrd);           \
+    }                                                                                  \
+    sr += bet - alpha * cs[P] * rinv;                                                  \
+    const double ar = alpha * rinv;                                                    \
+    swx += ar * wx;                                                                    \
+    swy += ar * wy;                                                                    \
+    swz += ar * wz;                                                                    \
+  }
+  AK_Q4_GRAD(1)
+  AK_Q4_GRAD(2)
+  AK_Q4_GRAD(3)
+#undef AK_Q4_GRAD
+  const double Gx = sr * ex + swx, Gy = sr * ey + swy, Gz = sr * ez + swz;
+  if (has) {
+    const size_t plane = (size_t)n * 4;
+    G[slot] = Gx;
+    G[plane + slot] = Gy;
+    G[2 * plane + slot] = Gz;
+  }
+  if (esite) {  // per-atom site energy U_a (uniform branch)
+    const double ea = seg_sum<4>(e);
+    if (live && k == 0) esite[a] = ea;
+  }
+  const double v[10] = {e,        -Gx * ux, -Gx * uy, -Gx * uz, -Gy * ux,
+                        -Gy * uy, -Gy * uz, -Gz * ux, -Gz * uy, -Gz * uz};
+  block_partial_lds<10>(v, partial, c, n_chunks);
+}
+
+template <bool L3Z>
+__global__ __launch_bounds__(256) void k_tersoff_q4(
+    int n_chunks, int n, const double* __restrict__ pos, const int* __restrict__ sid,
+    const Geom* __restrict__ geoms, int single, const double* __restrict__ tb,
+    const double* __restrict__ tri, double rc, const int* __restrict__ cnt,
+    const int* __restrict__ nbr, const int* __restrict__ chunk_rng, double* __restrict__ G,
+    double* __restrict__ partial, double* __restrict__ esite) {
+  for (int cc = blockIdx.x; cc < n_chunks; cc += gridDim.x)
+    k_tersoff_q4_chunk<L3Z>(xcd_remap(cc, n_chunks), n_chunks, n, pos, sid, geoms, single, tb,
+                            tri, rc, cnt, nbr, chunk_rng, G, partial, esite);
+}
+
+// ---------------------------------------------------------------------------
 // Tersoff site kernel, one lane per (bond, partner) pair.
 // L*L lanes serve one atom (L = list stride, L*L <= 64): lane (k, p) of the
 // atom handles bond k and its partner leg s = (k + p) mod L.  Lane p = 0 of
@@ -824,9 +1013,10 @@ __global__ __launch_bounds__(256) void k_struct_final(const Geom* __restrict__ g
   struct_reduce_block(blockIdx.x, geoms, first, n_chunks, partial, out, pair_flag);
 }
 
-// F_a = sum_k G[a, k] - G[rev(a, k)].  Blocks past the atom blocks reduce
-// the site kernel's partials to per-structure energy and stress (the site
-// kernel has completed, so any block may read them).
+// F_a = sum_k G[a, k] - G[rev(a, k)].  The first gridDim - atom_blocks
+// blocks (one per structure) reduce the site kernel's partials to
+// per-structure energy and stress (the site kernel has completed, so any
+// block may read them).
 template <int L>
 __global__ __launch_bounds__(256) void k_gather(int n, const int* __restrict__ cnt,
                                                 const int* __restrict__ rev, int stride,
@@ -837,11 +1027,14 @@ __global__ __launch_bounds__(256) void k_gather(int n, const int* __restrict__ c
                                                 const double* __restrict__ partial,
                                                 double* __restrict__ out,
                                                 const uint64_t* __restrict__ amask, int sym) {
-  if ((int)blockIdx.x >= atom_blocks) {
-    struct_reduce_block((int)blockIdx.x - atom_blocks, geoms, first, n_chunks, partial, out);
+  // the structure blocks come first, so that they are dispatched before
+  // the atom blocks instead of forming the launch's tail
+  const int n_struct = (int)gridDim.x - atom_blocks;
+  if ((int)blockIdx.x < n_struct) {
+    struct_reduce_block((int)blockIdx.x, geoms, first, n_chunks, partial, out);
     return;
   }
-  const int gt = xcd_remap(blockIdx.x, atom_blocks) * blockDim.x + threadIdx.x;
+  const int gt = xcd_remap((int)blockIdx.x - n_struct, atom_blocks) * blockDim.x + threadIdx.x;
   const int a = gt / L;
   const int k = gt & (L - 1);
   const bool live = a < n;
@@ -1039,11 +1232,14 @@ __global__ __launch_bounds__(256) void k_gather_c(int n, const int* __restrict__
                                                   const double* __restrict__ partial,
                                                   double* __restrict__ out,
                                                   const uint64_t* __restrict__ amask, int sym) {
-  if ((int)blockIdx.x >= atom_blocks) {
-    struct_reduce_block((int)blockIdx.x - atom_blocks, geoms, first, n_chunks, partial, out);
+  // the structure blocks come first, so that they are dispatched before
+  // the atom blocks instead of forming the launch's tail
+  const int n_struct = (int)gridDim.x - atom_blocks;
+  if ((int)blockIdx.x < n_struct) {
+    struct_reduce_block((int)blockIdx.x, geoms, first, n_chunks, partial, out);
     return;
   }
-  const int gt = xcd_remap(blockIdx.x, atom_blocks) * blockDim.x + threadIdx.x;
+  const int gt = xcd_remap((int)blockIdx.x - n_struct, atom_blocks) * blockDim.x + threadIdx.x;
   const int a = gt / LA;
   const int q = gt & (LA - 1);
   const bool live = a < n;
@@ -1724,6 +1920,16 @@ static void launch_bond_kernels(Potential& pot, NList& nl, SetDev& set, const do
         launch_timed(pot, sp1 ? k_tersoff_trip<L, true> : k_tersoff_trip<L, false>, grid, block,
                      st, (size_t)0, ch.n, n, d_pos, sid, geoms, single, d_spc, pot.ns, tb, tri, pot.r_cut,
                      cnt, nbr, nl.stride, rng, G, part, amask);
+        done = true;
+      }
+    }
+    if constexpr (L == 4) {
+      // the flat form (one valid record, m = 3, gamma = 1) on rows of four
+      const double* hp = pot.h_params.data();
+      if (!done && sp1 && hp[0] != 0.0 && hp[9] != 0.0 && hp[14] == 3.0 && hp[15] == 1.0) {
+        launch_timed(pot, hp[13] == 0.0 ? k_tersoff_q4<true> : k_tersoff_q4<false>, grid, block,
+                     st, (size_t)0, ch.n, n, d_pos, sid, geoms, single, tb, tri, pot.r_cut, cnt,
+                     nbr, rng, G, part, esite);
         done = true;
       }
     }
