@@ -15,9 +15,17 @@ The line also carries:
                 start/stop events of each of its dispatches on its launch
                 stream, over a second pass of the K timed steps (the events
                 cost host time, so the throughput pass runs without them);
-                ``traffic`` from the committed rocprofv3 counter summary;
+                the kernel is bound by FP64 VALU issue (no matrix work);
+                ``traffic`` and the hardware FP64 operation count from the
+                committed rocprofv3 counter summary of the same kernel
+                (profiles/tersoff_site_32768_counters.json);
   cpu_baseline  the oracle (C/OpenMP restatement of the reference path,
-                kind "port") on this host's cores, same workload;
+                kind "port") on all physical cores this process may use
+                (capped by the host's OMP_NUM_THREADS) and on one core, same
+                workload, with the CPU model and core counts;
+  sw_md, eam_cuni, batched_tersoff
+                the other configurations, each with a stated flop count, its
+                fraction of the FP64 peak and the oracle's host timing;
   fit           batched Tersoff fit-gradient throughput (256 structures of
                 64-512 atoms) against a per-structure GPU loop.
 Run ``python bench.py --gpus 1 --steps K --warmup W``.
@@ -61,6 +69,60 @@ def tersoff_flops(n_pairs, n_triplets, lambda3):
     return n_pairs * per_pair + n_triplets * per_trip
 
 
+def sw_flops(n_pairs, n_triplets):
+    """Algorithmic FP64 work of one Stillinger-Weber force call, counted on
+    the reference's algorithm (engine/stillinger_weber.py:23-52, 96-154,
+    258-395), flops as in tersoff_flops:
+      per pair within a sigma: minimum image 45 + 1 sqrt; two-body energy and
+        radial derivative 20 + 3 T (x^-p, x^-q, exp(1/(x - a))); force and
+        virial 15;
+      per unordered triplet (i; j, k) with both legs inside a sigma: the two
+        leg factors exp(gamma/(x - a)) 2 T, cos and (cos - cos0)^2 12, the
+        analytic gradient of _sw_h3_grad 50 + 2 sqrt, forces and virial 36.
+    """
+    per_pair = 45 + 1 + 20 + 15 + 3 * W_TRANSCENDENTAL
+    per_trip = 12 + 50 + 2 + 36 + 2 * W_TRANSCENDENTAL
+    return n_pairs * per_pair + n_triplets * per_trip
+
+
+def eam_flops(n_pairs, n_atoms):
+    """Algorithmic FP64 work of one Zhou EAM force call, counted on the
+    reference's kernel (engine/_eam_kernels.py:129-307), flops as in
+    tersoff_flops:
+      per pair within r_c: minimum image 45 + 1 sqrt; the density of each
+        end's species with its derivative (exp, the (r/re - lambda)^20
+        denominator) 2 x (2 T + 30); the pair function with derivative for
+        both species (a cross pair is built from both) 4 T + 80; the quintic
+        switch 20; the force and virial 33;
+      per atom: the embedding function and its derivative (power and log
+        branches) 2 T + 20.
+    """
+    per_pair = 45 + 1 + 2 * (2 * W_TRANSCENDENTAL + 30) + 4 * W_TRANSCENDENTAL + 80 + 20 + 33
+    per_atom = 2 * W_TRANSCENDENTAL + 20
+    return n_pairs * per_pair + n_atoms * per_atom
+
+
+def pair_counts(lat, pos, r_cut, host_pairs=None):
+    """Pairs i < j within r_cut and unordered centred triplets
+    sum_i n_i (n_i - 1) / 2 (oracle pair list)."""
+    import oracle as O
+
+    i, j, _, r = O.nl_build(lat, pos, r_cut)
+    m = r < r_cut
+    cnt = np.bincount(np.concatenate([i[m], j[m]]), minlength=len(pos))
+    return int(m.sum()), int(np.sum(cnt * (cnt - 1)) // 2)
+
+
+def site_kernel_name(params, stride, tile):
+    """The Tersoff site kernel the engine launches for a flat (one-species,
+    m = 3, gamma = 1) parameter set on rows of `stride` slots."""
+    if tile == 2 and stride <= 8:
+        return f"k_tersoff_trip<{stride}, true>"
+    if stride == 4:
+        return f"k_tersoff_q4<{'true' if params['lambda3'] == 0.0 else 'false'}>"
+    return f"k_tersoff_site<{stride}, true>"
+
+
 def geometry_counts(lat, pos, r_cut):
     """Pairs within r_cut and ordered triplets sum_i n_i (n_i - 1)."""
     from akitsushima_amd import NeighbourList
@@ -78,29 +140,73 @@ def geometry_counts(lat, pos, r_cut):
     return int(m.sum()), int(np.sum(cnt * (cnt - 1)))
 
 
-def read_traffic():
-    """HBM bytes per launch of the site kernel from the committed rocprofv3
-    counter summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel)."""
-    path = os.path.join(HERE, "profiles", "tersoff_site_32768_traffic.json")
+COUNTERS = os.path.join("profiles", "tersoff_site_32768_counters.json")
+
+
+def read_counters(kernel):
+    """The committed rocprofv3 counter summary of the headline site kernel
+    (per-dispatch medians of the timed launch shape): HBM bytes per launch
+    (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel) and the hardware
+    FP64 operation count.  None when the summary is missing or names another
+    kernel."""
     try:
-        with open(path) as fh:
-            return json.load(fh).get("bytes_per_launch")
+        with open(os.path.join(HERE, COUNTERS)) as fh:
+            c = json.load(fh)
     except (OSError, ValueError):
         return None
+    return c if c.get("kernel") == kernel else None
+
+
+def host_cpu():
+    """CPU model, logical CPUs, the CPUs this process may run on and the
+    physical cores among them (from /proc/cpuinfo), and the thread count of
+    an all-cores run: the physical cores, capped by OMP_NUM_THREADS when the
+    host sets it (a shared host's CPU share)."""
+    model, core_of = "unknown", {}
+    try:
+        cpu = phys = None
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "processor":
+                    cpu, phys = int(v), "0"
+                elif k == "model name":
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id" and cpu is not None:
+                    core_of[cpu] = (phys, v)
+    except OSError:
+        pass
+    allowed = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else []
+    cores = len({core_of[c] for c in allowed if c in core_of}) or len(allowed) or 1
+    threads = cores
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        threads = min(threads, max(1, int(os.environ["OMP_NUM_THREADS"])))
+    return {"cpu_model": model, "nproc": os.cpu_count(), "cpus_allowed": len(allowed),
+            "physical_cores_allowed": cores, "threads_all_cores": threads}
+
+
+def time_best(fn, budget_s, max_calls=5000, min_calls=2):
+    """Best of repeated warm calls within budget_s seconds (after one warm-up
+    call): (best seconds, calls, seconds spent)."""
+    fn()
+    times = []
+    t_end = time.perf_counter() + budget_s
+    while len(times) < max_calls and (len(times) < min_calls or time.perf_counter() < t_end):
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    return min(times), len(times), float(np.sum(times))
 
 
 def cpu_baseline(lat, pos, params, budget_s, threads):
     import oracle as O
 
     ci, cj, _, _ = O.nl_build(lat, pos, params["R"] + params["D"] + 0.3)
-    O.tersoff_flat(lat, pos, params, pairs=(ci, cj), nthreads=threads)  # warm
-    times = []
-    t_end = time.perf_counter() + budget_s
-    while time.perf_counter() < t_end and len(times) < 5000:
-        t0 = time.perf_counter()
-        O.tersoff_flat(lat, pos, params, pairs=(ci, cj), nthreads=threads)
-        times.append(time.perf_counter() - t0)
-    return min(times), len(times), float(np.sum(times))
+    return time_best(lambda: O.tersoff_flat(lat, pos, params, pairs=(ci, cj), nthreads=threads),
+                     budget_s)
 
 
 def fit_throughput(n_struct, reps, seed=17):
@@ -195,10 +301,12 @@ def device_force_rate(pot, lat, pos, types, steps, warmup, skin=0.3):
             "energy_per_atom": float(es[0] / n)}
 
 
-def sw_md(n_steps=1000):
+def sw_md(n_steps=1000, cpu_threads=0):
     """Stillinger-Weber Si NVE: diamond 20^3 cells (64000 atoms), Maxwell-
     Boltzmann velocities at 1000 K (seed 0), dt 1 fs, skin 0.3 A, one step
-    captured in a hipGraph and replayed."""
+    captured in a hipGraph and replayed.  With cpu_threads, the oracle's
+    velocity Verlet (same rebuild rule) timed over 1 and 4 steps on the host:
+    ms per step at the device run's rebuild rate."""
     from akitsushima_amd import StillingerWeberPotential
     from akitsushima_amd.md import masses_of, maxwell_boltzmann, nve_run
     from akitsushima_amd.published import SW_SI
@@ -213,15 +321,42 @@ def sw_md(n_steps=1000):
                   use_graph=True)
     n = len(pos)
     drift = res["drift"] / n
-    return {"workload": f"SW Si diamond 20^3 ({n} atoms), 1000 K, dt 1 fs, {n_steps} steps, "
-                        "skin 0.3 A, hipGraph-captured step",
-            "atoms": n, "steps": n_steps, "seconds": res["seconds"],
-            "ms_per_step": res["seconds"] / n_steps * 1e3,
-            "atom_steps_per_s": n * n_steps / res["seconds"],
-            "n_rebuilds": res["n_rebuilds"],
-            "drift_per_atom_final_eV": float(drift[-1]),
-            "drift_per_atom_max_abs_eV": float(np.abs(drift).max()),
-            "e_tot_per_atom_start_eV": float(res["e_tot"][0] / n)}
+    ms = res["seconds"] / n_steps * 1e3
+    n_pairs, n_trip = pair_counts(lat, pos, pot.r_cut)
+    fl = sw_flops(n_pairs, n_trip)
+    out = {"workload": f"SW Si diamond 20^3 ({n} atoms), 1000 K, dt 1 fs, {n_steps} steps, "
+                       "skin 0.3 A, hipGraph-captured step",
+           "atoms": n, "steps": n_steps, "seconds": res["seconds"],
+           "ms_per_step": ms, "atom_steps_per_s": n * n_steps / res["seconds"],
+           "n_rebuilds": res["n_rebuilds"],
+           "drift_per_atom_final_eV": float(drift[-1]),
+           "drift_per_atom_max_abs_eV": float(np.abs(drift).max()),
+           "e_tot_per_atom_start_eV": float(res["e_tot"][0] / n),
+           "roofline": {"bound": "fp64_valu", "flops_per_step": fl,
+                        "flops_note": "force call counted by sw_flops on the start geometry "
+                                      f"({n_pairs} pairs, {n_trip} triplets); integration "
+                                      "not counted",
+                        "achieved": fl / (ms * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
+                        "unit": "TFLOP/s", "frac": fl / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}}
+    if cpu_threads:
+        import oracle as O
+
+        force = lambda l, p, pr: O.sw_flat(l, p, SW_SI, pairs=pr, nthreads=cpu_threads)
+        t = []
+        for k in (1, 4):
+            t0 = time.perf_counter()
+            O.nve_run(force, lat, pos, vel, masses, 1.0, k, pot.r_cut, 0.3)
+            t.append(time.perf_counter() - t0)
+        step = (t[1] - t[0]) / 3
+        build = max(t[0] - step, 0.0)
+        cpu_ms = (step + build * res["n_rebuilds"] / n_steps) * 1e3
+        out["cpu_baseline"] = {"ms_per_step": cpu_ms, "cores": cpu_threads, "kind": "port",
+                               "sample": "oracle velocity Verlet (oracle.nve_run, SW force on "
+                                         f"{cpu_threads} threads) over 1 and 4 steps: "
+                                         f"{step * 1e3:.1f} ms per step + {build * 1e3:.1f} ms "
+                                         "per list build at the device run's rebuild rate"}
+        out["gpu_over_cpu"] = cpu_ms / ms
+    return out
 
 
 def sw_gk(n_steps=1000):
@@ -278,9 +413,11 @@ def sw_gk(n_steps=1000):
             "nemd_q_cold_eV": float(g["bath"][1:, 1].sum())}
 
 
-def eam_cuni(steps, warmup):
+def eam_cuni(steps, warmup, cpu_threads=0):
     """Zhou 2004 Cu-Ni random alloy: fcc 40^3 (256000 atoms), a = 3.56 A,
-    species drawn 50/50 from seed 0, rattle 0.03 A (seed 1)."""
+    species drawn 50/50 from seed 0, rattle 0.03 A (seed 1).  With
+    cpu_threads, one warm oracle force call on the host with a prebuilt
+    pair list."""
     from akitsushima_amd import zhou_eam
     from akitsushima_amd.published import ZHOU_2004
     from akitsushima_amd.structures import FCC, cubic, random_species
@@ -292,16 +429,37 @@ def eam_cuni(steps, warmup):
     out = {"workload": f"Zhou EAM Cu-Ni fcc 40^3 ({len(pos)} atoms), a 3.56 A, rattle 0.03 A",
            "atoms": len(pos), "r_cut": pot.r_cut, "autotuned": tune["best"]}
     out.update(device_force_rate(pot, lat, pos, types, steps, warmup))
+    import oracle as O
+
+    pi, pj, _, pr = O.nl_build(lat, pos, pot.r_cut)
+    n_pairs = int(np.sum(pr < pot.r_cut))
+    fl = eam_flops(n_pairs, len(pos))
+    call_s = out["ms_per_call"] * 1e-3
+    out["roofline"] = {"bound": "fp64_valu", "flops_per_call": fl,
+                       "flops_note": f"eam_flops over {n_pairs} pairs within r_c",
+                       "achieved": fl / call_s / 1e12, "peak": FP64_PEAK_TFLOPS,
+                       "unit": "TFLOP/s", "frac": fl / call_s / 1e12 / FP64_PEAK_TFLOPS}
+    if cpu_threads:
+        elements = {"Cu": dict(ZHOU_2004["Cu"]), "Ni": dict(ZHOU_2004["Ni"])}
+        tab = O.zhou_table(elements, ["Cu", "Ni"])
+        best, nc, spent = time_best(
+            lambda: O.eam_table(lat, pos, types, tab, ["Cu", "Ni"], pairs=(pi, pj),
+                                nthreads=cpu_threads), 0.0, max_calls=1, min_calls=1)
+        out["cpu_baseline"] = {"value": len(pos) / best, "unit": "atom-force-evals/s",
+                               "cores": cpu_threads, "kind": "port",
+                               "sample": f"one warm oracle force call ({best:.2f} s) on "
+                                         f"{cpu_threads} threads with a prebuilt pair list"}
+        out["gpu_over_cpu"] = out["atom_evals_per_s"] / (len(pos) / best)
     return out
 
 
-def sw_gesn(steps, warmup, cpu=True):
+def sw_gesn(steps, warmup, cpu_threads=0):
     """Multi-species SW Ge0.875Sn0.125 (Tomita et al. 2018 records): the
     reference's ledger cell (diamond 8^3, 4096 atoms, a = 5.762 A, rattle
     0.05 A seed 0, species from default_rng(1) after the 512-atom cell's
     draw, benchmarks/bench_force.py:223-229) and a 20^3 cell (64000 atoms,
     species from default_rng(1)), autotuned; the ledger cell also on the
-    host oracle (16 threads)."""
+    host oracle (cpu_threads threads)."""
     from akitsushima_amd import StillingerWeberPotential
     from akitsushima_amd.structures import DIAMOND, cubic
 
@@ -329,14 +487,15 @@ def sw_gesn(steps, warmup, cpu=True):
         tune = pot.autotune([(lat, pos, types)], reps=3)
         r = {"atoms": len(pos), "autotuned": tune["best"]}
         r.update(device_force_rate(pot, lat, pos, types, steps, warmup))
-        if cells == 8 and cpu:
+        if cells == 8 and cpu_threads:
             import oracle
 
-            oracle.sw_multi(lat, pos, types, recs, sp, nthreads=16)
+            oracle.sw_multi(lat, pos, types, recs, sp, nthreads=cpu_threads)
             t0 = time.perf_counter()
             for _ in range(5):
-                oracle.sw_multi(lat, pos, types, recs, sp, nthreads=16)
-            r["cpu_oracle_ms_per_call_16_threads"] = (time.perf_counter() - t0) / 5 * 1e3
+                oracle.sw_multi(lat, pos, types, recs, sp, nthreads=cpu_threads)
+            r["cpu_oracle_ms_per_call"] = (time.perf_counter() - t0) / 5 * 1e3
+            r["cpu_oracle_threads"] = cpu_threads
             r["reference_ledger_call_ms_1_thread"] = 33.9447  # benchmarks/adr0047.jsonl:8
         if cells == 20:
             # NVT MD of the same cell on the device (the reference's ledger of
@@ -356,10 +515,11 @@ def sw_gesn(steps, warmup, cpu=True):
     return out
 
 
-def batched_tersoff(reps):
+def batched_tersoff(reps, cpu_threads=0):
     """512 diamond Si structures of 64-2048 atoms (seeded supercell repeats,
     +-2 % symmetric strain, rattle 0.02 A) in one batched call, autotuned,
-    against a loop of single-structure device calls."""
+    against a loop of single-structure device calls; with cpu_threads, the
+    oracle over the first 64 structures (prebuilt pair lists) on the host."""
     import torch
 
     from akitsushima_amd import NeighbourList, TersoffPotential
@@ -415,12 +575,46 @@ def batched_tersoff(reps):
         _lib.check(L.ak_compute_device(*c[3]))
     torch.cuda.synchronize()
     loop = time.perf_counter() - t0
-    return {"workload": f"{S} diamond Si structures, 64-2048 atoms ({n} atoms), strain +-2 %, "
-                        "rattle 0.02 A, Tersoff T3",
-            "structures": S, "atoms": n, "autotuned": tune["best"],
-            "candidates_tried": len(tune["tried"]),
-            "batched_ms_per_call": batched * 1e3, "atom_evals_per_s": n / batched,
-            "per_structure_loop_ms": loop * 1e3, "speedup_vs_loop": loop / batched}
+    import oracle as O
+
+    params = dict(TERSOFF_SI_T3)
+    rc = pot.r_cut
+    lists = [O.candidate_pairs(lat, p, rc) for lat, p, _ in structs]
+    n_pairs = n_trip = 0
+    for (lat, p, _), (ci, cj) in zip(structs, lists):
+        d = p[cj] - p[ci]
+        fr = d @ np.linalg.inv(lat)
+        r = np.linalg.norm((fr - np.round(fr)) @ lat, axis=1)
+        m = r < rc
+        cnt = np.bincount(np.concatenate([ci[m], cj[m]]), minlength=len(p))
+        n_pairs += int(m.sum())
+        n_trip += int(np.sum(cnt * (cnt - 1)))
+    fl = tersoff_flops(n_pairs, n_trip, params["lambda3"])
+    out = {"workload": f"{S} diamond Si structures, 64-2048 atoms ({n} atoms), strain +-2 %, "
+                       "rattle 0.02 A, Tersoff T3",
+           "structures": S, "atoms": n, "autotuned": tune["best"],
+           "candidates_tried": len(tune["tried"]),
+           "batched_ms_per_call": batched * 1e3, "atom_evals_per_s": n / batched,
+           "per_structure_loop_ms": loop * 1e3, "speedup_vs_loop": loop / batched,
+           "roofline": {"bound": "fp64_valu", "flops_per_call": fl,
+                        "flops_note": f"tersoff_flops over {n_pairs} pairs and {n_trip} "
+                                      "ordered triplets; the whole batched call (site kernel "
+                                      "and gather) as the time",
+                        "achieved": fl / batched / 1e12, "peak": FP64_PEAK_TFLOPS,
+                        "unit": "TFLOP/s", "frac": fl / batched / 1e12 / FP64_PEAK_TFLOPS}}
+    if cpu_threads:
+        sample = list(zip(structs, lists))[:64]
+        atoms = sum(len(p) for (_, p, _), _ in sample)
+        run = lambda: [O.tersoff_flat(lat, p, params, pairs=pr, nthreads=cpu_threads)
+                       for (lat, p, _), pr in sample]
+        best, nc, spent = time_best(run, 2.0)
+        out["cpu_baseline"] = {"value": atoms / best, "unit": "atom-force-evals/s",
+                               "cores": cpu_threads, "kind": "port",
+                               "sample": f"the first 64 structures ({atoms} atoms) one after "
+                                         f"another, best of {nc} passes, {cpu_threads} threads, "
+                                         "prebuilt pair lists"}
+        out["gpu_over_cpu"] = (n / batched) / (atoms / best)
+    return out
 
 
 def sw_elastic():
@@ -494,7 +688,7 @@ def main():
     if args.leg:
         legs = {"sw_md": lambda: sw_md(1000), "eam_cuni": lambda: eam_cuni(20, 3),
                 "batched_tersoff": lambda: batched_tersoff(10), "sw_elastic": sw_elastic,
-                "sw_gesn": lambda: sw_gesn(50, 5, cpu=False), "sw_gk": lambda: sw_gk(1000)}
+                "sw_gesn": lambda: sw_gesn(50, 5), "sw_gk": lambda: sw_gk(1000)}
         if args.leg not in legs:
             print(f"bench.py: unknown leg {args.leg!r}", file=sys.stderr)
             return 2
@@ -613,15 +807,15 @@ def main():
     while stride < int(np.bincount(np.concatenate([ci, cj]), minlength=n).max()):
         stride *= 2
     tile = args.bond_tile or (tuned or {}).get("bond_tile", 1)
-    site_kernel = (f"k_tersoff_trip<{stride}>" if tile == 2 and stride <= 8
-                   else f"k_tersoff_site<{stride}>")
+    site_kernel = site_kernel_name(params, stride, tile)
     n_pairs, n_trip = geometry_counts(lat, pos, pot.r_cut)
     flops = tersoff_flops(n_pairs, n_trip, params["lambda3"])
     avg_site_s = tot_ms.value / max(cnt.value, 1) * 1e-3
     achieved = flops / avg_site_s / 1e12
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+    ctr = read_counters(site_kernel)
+    roofline = {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                "traffic": read_traffic(),
+                "traffic": ctr["bytes_per_launch"] if ctr else None,
                 "kernel": site_kernel, "launch_shape": tuned,
                 "kernel_avg_us": avg_site_s * 1e6,
                 "kernel_min_us": min_ms.value * 1e3, "launches_timed": int(cnt.value),
@@ -631,6 +825,21 @@ def main():
                 "flops_per_launch": flops,
                 "flops_per_atom": flops / n,
                 "compute_unit": "FP64 VALU (MI355X FP64 vector peak = 78.6 TFLOP/s)"}
+    if ctr:
+        # hardware side from the committed counter summary: FP64 operations the
+        # kernel issued (SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes) over this run's
+        # event-timed average, and the HBM rate of the counted traffic
+        hw = ctr["fp64_flops_per_launch"]
+        roofline["counters"] = {
+            "file": COUNTERS, "fp64_flops_per_launch": hw,
+            "fp64_tflops_hw": hw / avg_site_s / 1e12,
+            "fp64_frac_hw": hw / avg_site_s / 1e12 / FP64_PEAK_TFLOPS,
+            "hbm_gbs": ctr["bytes_per_launch"] / avg_site_s / 1e9,
+            "hbm_frac": ctr["bytes_per_launch"] / avg_site_s / 1e9 / HBM_PEAK_GBS,
+            "rocprof_avg_us": ctr.get("rocprof_avg_us"),
+            **{k: ctr[k] for k in ("waves", "valu_insts_per_wave", "fp64_insts_per_wave",
+                                   "vgprs", "waves_per_simd_max", "wave_time_waiting")
+               if k in ctr}}
 
     out = {"metric": METRIC, "value": value, "unit": "atom-force-evals/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup,
@@ -647,23 +856,32 @@ def main():
            "host_roundtrip_atom_evals_per_s": host_rate,
            "roofline": roofline}
 
+    host = host_cpu()
     if not args.no_cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads = host["threads_all_cores"]
         best, ncalls, spent = cpu_baseline(lat, pos, params, args.cpu_seconds, threads)
+        best1, ncalls1, spent1 = cpu_baseline(lat, pos, params, args.cpu_seconds, 1)
         out["cpu_baseline"] = {
             "value": n / best, "unit": "atom-force-evals/s", "cores": threads, "kind": "port",
             "sample": f"best of {ncalls} warm Tersoff force calls on the same {n}-atom "
-                      f"workload with a prebuilt list ({spent:.1f} s of CPU time), C/OpenMP "
-                      "restatement of the reference Numba path (oracle/refcpu.c)"}
+                      f"workload with a prebuilt list ({spent:.1f} s), C/OpenMP "
+                      "restatement of the reference Numba path (oracle/refcpu.c), on "
+                      f"{threads} threads (all physical cores allowed, capped by the host's "
+                      "OMP_NUM_THREADS)",
+            "single_core": {"value": n / best1, "cores": 1,
+                            "sample": f"best of {ncalls1} calls ({spent1:.1f} s), one thread"},
+            **host}
         out["gpu_over_cpu"] = value / (n / best)
+        out["gpu_over_cpu_single_core"] = value / (n / best1)
     if not args.no_fit:
         out["fit"] = fit_throughput(256, args.fit_reps)
     if not args.no_extras:
-        out["sw_md"] = sw_md(1000)
-        out["eam_cuni"] = eam_cuni(20, 3)
-        out["batched_tersoff"] = batched_tersoff(10)
+        ct = 0 if args.no_cpu else host["threads_all_cores"]
+        out["sw_md"] = sw_md(1000, cpu_threads=ct)
+        out["eam_cuni"] = eam_cuni(20, 3, cpu_threads=ct)
+        out["batched_tersoff"] = batched_tersoff(10, cpu_threads=ct)
         out["sw_elastic"] = sw_elastic()
-        out["sw_gesn"] = sw_gesn(50, 5, cpu=not args.no_cpu)
+        out["sw_gesn"] = sw_gesn(50, 5, cpu_threads=ct)
         out["sw_gk"] = sw_gk(1000)
     print(json.dumps(out))
     return 0
