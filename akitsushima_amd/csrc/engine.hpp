@@ -211,6 +211,37 @@ void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
                     const int* d_spc, double* d_F, double* d_out, hipStream_t st,
                     double* d_esite = nullptr);
 
+// Launch shape of a force evaluation: lanes per atom of the site (or EAM)
+// kernels, whether the Tersoff lane-per-(bond, partner) form runs, threads
+// per block and atoms per launch chunk.
+struct ForcePlan {
+  int lanes = 0, blk = 256, chunk = 0;
+  bool trip = false;
+};
+ForcePlan force_plan(const Potential& pot, const NList& nl, bool esite);
+
+// Device memory of one force evaluation besides its inputs and outputs:
+// structure geometry and atom -> structure map, the launch chunks (ranges
+// (n_chunks, 2), first chunk of each structure (n_struct + 1), block
+// partials (10, n_chunks)), and the kernels' scratch: bond gradients G
+// (3 n stride) and active-bond masks (n) for Tersoff / SW, the embedding
+// derivatives (2 n) and per-chunk pair flags for EAM.
+struct ForceWork {
+  int n = 0, n_struct = 1, n_chunks = 0;
+  const int* sid = nullptr;
+  const Geom* geoms = nullptr;
+  const int* rng = nullptr;
+  const int* first = nullptr;
+  double* partial = nullptr;
+  double* G = nullptr;
+  uint64_t* amask = nullptr;
+  double* aux = nullptr;
+  int* pair_flag = nullptr;
+};
+void compute_forces_work(Potential& pot, NList& nl, const ForcePlan& plan, const ForceWork& w,
+                         const double* d_pos, const int* d_spc, double* d_F, double* d_out,
+                         hipStream_t st, double* d_esite = nullptr);
+
 // Forces, per-atom site energies and the site gradients G[a, k] = dU_a/dv_ak
 // in pot.G for every kind (forces.hip); site_masked: whether only the slots
 // whose amask bit is set hold G (else every slot below cnt does).

@@ -1888,28 +1888,29 @@ static inline size_t sw_c_lds(int blk, int L) {
 
 // Site-kernel grid: one block per chunk, or at most grid_cap blocks that
 // stride over the chunks (persistent form).
-static inline dim3 chunk_grid(const Potential& pot, const SetDev::Chunks& ch) {
-  int g = ch.n;
+static inline dim3 chunk_grid(const Potential& pot, int n_chunks) {
+  int g = n_chunks;
   if (pot.grid_cap > 0 && g > pot.grid_cap) g = pot.grid_cap;
   return dim3(g > 0 ? g : 1);
 }
 
 template <int L>
-static void launch_bond_kernels(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
-                                const int* d_spc, double* d_F, double* d_out,
-                                SetDev::Chunks& ch, bool trip, int blk, double* esite,
+static void launch_bond_kernels(Potential& pot, NList& nl, const ForceWork& w,
+                                const double* d_pos, const int* d_spc, double* d_F,
+                                double* d_out, bool trip, int blk, double* esite,
                                 hipStream_t st) {
-  const int n = set.n_atoms;
-  const int single = set.n_struct == 1 ? 1 : 0;
-  double* G = pot.G.as<double>();
-  const dim3 grid = chunk_grid(pot, ch), block(blk);
-  const int* sid = set.d_sid.as<int>();
-  const Geom* geoms = set.d_geoms.as<Geom>();
+  const int n = w.n;
+  const int single = w.n_struct == 1 ? 1 : 0;
+  double* G = w.G;
+  const dim3 grid = chunk_grid(pot, w.n_chunks), block(blk);
+  const int* sid = w.sid;
+  const Geom* geoms = w.geoms;
   const int* cnt = nl.cnt.as<int>();
   const int* nbr = nl.nbr.as<int>();
-  const int* rng = ch.rng.as<int>();
-  double* part = ch.partial.as<double>();
-  uint64_t* amask = pot.amask.as<uint64_t>();
+  const int* rng = w.rng;
+  double* part = w.partial;
+  uint64_t* amask = w.amask;
+  const int n_chunks = w.n_chunks, n_struct = w.n_struct;
   if (pot.kind == KIND_TERSOFF) {
     const double* tb = pot.d_params.as<double>();
     const double* tri = tb + (size_t)pot.ns * pot.ns * 9;
@@ -1918,7 +1919,7 @@ static void launch_bond_kernels(Potential& pot, NList& nl, SetDev& set, const do
     if constexpr (L * L <= 64) {
       if (trip) {
         launch_timed(pot, sp1 ? k_tersoff_trip<L, true> : k_tersoff_trip<L, false>, grid, block,
-                     st, (size_t)0, ch.n, n, d_pos, sid, geoms, single, d_spc, pot.ns, tb, tri, pot.r_cut,
+                     st, (size_t)0, n_chunks, n, d_pos, sid, geoms, single, d_spc, pot.ns, tb, tri, pot.r_cut,
                      cnt, nbr, nl.stride, rng, G, part, amask);
         done = true;
       }
@@ -1928,33 +1929,33 @@ static void launch_bond_kernels(Potential& pot, NList& nl, SetDev& set, const do
       const double* hp = pot.h_params.data();
       if (!done && sp1 && hp[0] != 0.0 && hp[9] != 0.0 && hp[14] == 3.0 && hp[15] == 1.0) {
         launch_timed(pot, hp[13] == 0.0 ? k_tersoff_q4<true> : k_tersoff_q4<false>, grid, block,
-                     st, (size_t)0, ch.n, n, d_pos, sid, geoms, single, tb, tri, pot.r_cut, cnt,
+                     st, (size_t)0, n_chunks, n, d_pos, sid, geoms, single, tb, tri, pot.r_cut, cnt,
                      nbr, rng, G, part, esite);
         done = true;
       }
     }
     if (!done)
       launch_timed(pot, sp1 ? k_tersoff_site<L, true> : k_tersoff_site<L, false>, grid, block, st,
-                   (size_t)0, ch.n, n, d_pos, sid, geoms, single, d_spc, pot.ns, tb, tri, pot.r_cut, cnt,
+                   (size_t)0, n_chunks, n, d_pos, sid, geoms, single, d_spc, pot.ns, tb, tri, pot.r_cut, cnt,
                    nbr, nl.stride, rng, G, part, amask, esite);
   } else if (pot.sw_multi) {
     constexpr int LA = L < kRowLanes ? L : kRowLanes;
-    launch_timed(pot, k_sw_multi<L, LA>, grid, block, st, sw_m_lds(blk, L, LA, pot.ns), ch.n, n,
+    launch_timed(pot, k_sw_multi<L, LA>, grid, block, st, sw_m_lds(blk, L, LA, pot.ns), n_chunks, n,
                  d_pos, sid, geoms, single, d_spc, pot.ns,
                  (const double*)pot.d_params.as<double>(), pot.r_cut, cnt, nbr, nl.stride, rng, G,
                  part, amask, esite);
   } else if constexpr (L >= kLongRow) {
     const size_t lds = sw_c_lds(blk, L);
-    launch_timed(pot, k_sw_site_c<L, kRowLanes>, grid, block, st, lds, ch.n, n, d_pos, sid,
+    launch_timed(pot, k_sw_site_c<L, kRowLanes>, grid, block, st, lds, n_chunks, n, d_pos, sid,
                  geoms, single, (const double*)pot.d_params.as<double>(), cnt, nbr, nl.stride,
                  rng, G, part, amask, esite);
   } else {
-    launch_timed(pot, k_sw_site<L>, grid, block, st, (size_t)0, ch.n, n, d_pos, sid, geoms,
+    launch_timed(pot, k_sw_site<L>, grid, block, st, (size_t)0, n_chunks, n, d_pos, sid, geoms,
                  single, (const double*)pot.d_params.as<double>(), cnt, nbr, nl.stride, rng, G,
                  part, amask, esite);
   }
   const int* rev = nl.rev.as<int>();
-  const int* first = ch.first.as<int>();
+  const int* first = w.first;
   // Tersoff (r < r_c of the potential) and flat SW (r < a sigma) test a bond
   // and its reverse with the same bitwise distance; multi-species SW legs
   // have per-species cutoffs
@@ -1962,45 +1963,41 @@ static void launch_bond_kernels(Potential& pot, NList& nl, SetDev& set, const do
   if (pot.sw_multi) {
     constexpr int LA = L < kRowLanes ? L : kRowLanes;
     const int atom_blocks = (int)(((int64_t)n * LA + 255) / 256);
-    hipLaunchKernelGGL((k_gather_c<L, LA>), dim3(atom_blocks + set.n_struct), dim3(256), 0, st,
+    hipLaunchKernelGGL((k_gather_c<L, LA>), dim3(atom_blocks + n_struct), dim3(256), 0, st,
                        n, cnt, rev, nl.stride, (const double*)G, d_F, atom_blocks, geoms, first,
-                       ch.n, (const double*)part, d_out, (const uint64_t*)amask, sym);
+                       n_chunks, (const double*)part, d_out, (const uint64_t*)amask, sym);
   } else if constexpr (L >= kLongRow) {
     const int atom_blocks = (int)(((int64_t)n * kRowLanes + 255) / 256);
-    hipLaunchKernelGGL((k_gather_c<L, kRowLanes>), dim3(atom_blocks + set.n_struct), dim3(256),
+    hipLaunchKernelGGL((k_gather_c<L, kRowLanes>), dim3(atom_blocks + n_struct), dim3(256),
                        0, st, n, cnt, rev, nl.stride, (const double*)G, d_F, atom_blocks, geoms,
-                       first, ch.n, (const double*)part, d_out, (const uint64_t*)amask, sym);
+                       first, n_chunks, (const double*)part, d_out, (const uint64_t*)amask, sym);
   } else {
     const int atom_blocks = (int)(((int64_t)n * L + 255) / 256);
-    hipLaunchKernelGGL(k_gather<L>, dim3(atom_blocks + set.n_struct), dim3(256), 0, st, n, cnt,
-                       rev, nl.stride, (const double*)G, d_F, atom_blocks, geoms, first, ch.n,
+    hipLaunchKernelGGL(k_gather<L>, dim3(atom_blocks + n_struct), dim3(256), 0, st, n, cnt,
+                       rev, nl.stride, (const double*)G, d_F, atom_blocks, geoms, first, n_chunks,
                        (const double*)part, d_out, (const uint64_t*)amask, sym);
   }
 }
 
 template <int LE>
-static void launch_eam(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
-                       const int* d_spc, double* d_F, SetDev::Chunks& ch, double* esite,
-                       hipStream_t st) {
-  const int n = set.n_atoms;
+static void launch_eam(Potential& pot, NList& nl, const ForceWork& w, const double* d_pos,
+                       const int* d_spc, double* d_F, double* esite, hipStream_t st) {
+  const int n = w.n;
   const int64_t threads = (int64_t)n * LE;
   const unsigned grid = (unsigned)((threads + 255) / 256);
-  double* dF = pot.aux.as<double>();
+  double* dF = w.aux;
   double* Fe = dF + n;
-  const int single = set.n_struct == 1 ? 1 : 0;
-  hipLaunchKernelGGL(k_eam_rho<LE>, dim3(grid), dim3(256), 0, st, n, d_pos,
-                     set.d_sid.as<int>(), set.d_geoms.as<Geom>(), single, d_spc, pot.ns,
-                     pot.d_params.as<double>(), pot.d_map.as<int>(), pot.r_cut,
-                     nl.cnt.as<int>(), nl.nbr.as<int>(), nl.stride, dF, Fe);
-  launch_timed(pot, k_eam_force<LE>, chunk_grid(pot, ch), dim3(pot.block), st, (size_t)0, ch.n,
-               d_pos,
-               (const int*)set.d_sid.as<int>(), (const Geom*)set.d_geoms.as<Geom>(), single, d_spc,
-               pot.ns, (const double*)pot.d_params.as<double>(),
-               (const int*)pot.d_map.as<int>(), pot.r_cut, (const int*)nl.cnt.as<int>(),
-               (const int*)nl.nbr.as<int>(), nl.stride, (const double*)dF, (const double*)Fe,
-               (const int*)ch.rng.as<int>(), d_F, ch.partial.as<double>(),
+  const int single = w.n_struct == 1 ? 1 : 0;
+  hipLaunchKernelGGL(k_eam_rho<LE>, dim3(grid), dim3(256), 0, st, n, d_pos, w.sid, w.geoms,
+                     single, d_spc, pot.ns, pot.d_params.as<double>(), pot.d_map.as<int>(),
+                     pot.r_cut, nl.cnt.as<int>(), nl.nbr.as<int>(), nl.stride, dF, Fe);
+  launch_timed(pot, k_eam_force<LE>, chunk_grid(pot, w.n_chunks), dim3(pot.block), st,
+               (size_t)0, w.n_chunks, d_pos, w.sid, w.geoms, single, d_spc, pot.ns,
+               (const double*)pot.d_params.as<double>(), (const int*)pot.d_map.as<int>(),
+               pot.r_cut, (const int*)nl.cnt.as<int>(), (const int*)nl.nbr.as<int>(), nl.stride,
+               (const double*)dF, (const double*)Fe, w.rng, d_F, w.partial,
                (const double*)(pot.eam_fast ? pot.d_eam_fast.as<double>() : nullptr), esite,
-               pot.pair_flag.as<int>());
+               w.pair_flag);
 }
 
 bool eam_fast_table(int ns, const std::vector<double>& rows, const std::vector<int>& map,
@@ -2053,69 +2050,101 @@ bool eam_fast_table(int ns, const std::vector<double>& rows, const std::vector<i
   return true;
 }
 
-void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
-                    const int* d_spc, double* d_F, double* d_out, hipStream_t st,
-                    double* esite) {
-  const int n = set.n_atoms;
+ForcePlan force_plan(const Potential& pot, const NList& nl, bool esite) {
   AK_REQUIRE(nl.valid, AK_ERR_STATE, "neighbour list has not been built");
-  AK_REQUIRE(nl.n_atoms == n, AK_ERR_STATE, "neighbour list was built for another size");
   AK_REQUIRE(pot.block == 64 || pot.block == 128 || pot.block == 256, AK_ERR_STATE,
              "block size must be 64, 128 or 256 threads");
-  int lanes;
+  ForcePlan pl;
   if (pot.kind == KIND_EAM) {
-    lanes = pot.eam_lanes > 0 ? pot.eam_lanes : 16;
+    pl.lanes = pot.eam_lanes > 0 ? pot.eam_lanes : 16;
   } else {
     AK_REQUIRE(nl.stride <= 64, AK_ERR_CAPACITY,
                "more than 64 neighbours within r_cut + skin for a bond-order kernel");
-    lanes = nl.stride;
+    pl.lanes = nl.stride;
   }
+  const int lanes = pl.lanes;
   // lanes per atom of the site kernel: L, or L * L for the Tersoff form with
   // one lane per (bond, partner) pair
   // per-atom site energies come from the lane-per-bond Tersoff form
-  const bool trip =
-      pot.kind == KIND_TERSOFF && lanes * lanes <= 64 && pot.bond_tile == 2 && !esite;
+  pl.trip = pot.kind == KIND_TERSOFF && lanes * lanes <= 64 && pot.bond_tile == 2 && !esite;
   const bool swm = pot.kind == KIND_SW && pot.sw_multi;
   const bool sw_long = pot.kind == KIND_SW && !swm && lanes >= kLongRow;
-  const int atom_lanes = trip ? lanes * lanes
-                              : (sw_long || (swm && lanes > kRowLanes) ? kRowLanes : lanes);
+  const int atom_lanes = pl.trip ? lanes * lanes
+                                 : (sw_long || (swm && lanes > kRowLanes) ? kRowLanes : lanes);
   int blk = pot.block;
   // the long-row and multi-species SW kernels stage every leg of their atoms
   // in LDS: keep it within 48 KiB per block
   while (sw_long && blk > 64 && sw_c_lds(blk, lanes) > 48 * 1024) blk >>= 1;
   while (swm && blk > 64 && sw_m_lds(blk, lanes, atom_lanes, pot.ns) > 48 * 1024) blk >>= 1;
   AK_REQUIRE(atom_lanes <= blk, AK_ERR_STATE, "block smaller than one atom's lanes");
-  SetDev::Chunks& ch = set.chunks(blk / atom_lanes, st);
-  // every chunk writes its flag in each call (no clearing needed)
-  if (pot.kind == KIND_EAM) pot.pair_flag.ensure(sizeof(int) * (size_t)std::max(1, ch.n));
+  pl.blk = blk;
+  pl.chunk = blk / atom_lanes;
+  return pl;
+}
+
+void compute_forces_work(Potential& pot, NList& nl, const ForcePlan& pl, const ForceWork& w,
+                         const double* d_pos, const int* d_spc, double* d_F, double* d_out,
+                         hipStream_t st, double* esite) {
+  const int n = w.n;
+  AK_REQUIRE(nl.n_atoms == n, AK_ERR_STATE, "neighbour list was built for another size");
   if (n > 0) {
     if (pot.kind == KIND_EAM) {
-      pot.aux.ensure(sizeof(double) * 2 * (size_t)n);
-      switch (lanes) {
-        case 4: launch_eam<4>(pot, nl, set, d_pos, d_spc, d_F, ch, esite, st); break;
-        case 8: launch_eam<8>(pot, nl, set, d_pos, d_spc, d_F, ch, esite, st); break;
-        case 32: launch_eam<32>(pot, nl, set, d_pos, d_spc, d_F, ch, esite, st); break;
-        case 64: launch_eam<64>(pot, nl, set, d_pos, d_spc, d_F, ch, esite, st); break;
-        default: launch_eam<16>(pot, nl, set, d_pos, d_spc, d_F, ch, esite, st); break;
+      switch (pl.lanes) {
+        case 4: launch_eam<4>(pot, nl, w, d_pos, d_spc, d_F, esite, st); break;
+        case 8: launch_eam<8>(pot, nl, w, d_pos, d_spc, d_F, esite, st); break;
+        case 32: launch_eam<32>(pot, nl, w, d_pos, d_spc, d_F, esite, st); break;
+        case 64: launch_eam<64>(pot, nl, w, d_pos, d_spc, d_F, esite, st); break;
+        default: launch_eam<16>(pot, nl, w, d_pos, d_spc, d_F, esite, st); break;
       }
     } else {
-      pot.G.ensure(sizeof(double) * 3 * (size_t)n * nl.stride);
-      pot.amask.ensure(sizeof(uint64_t) * (size_t)n);
-      switch (lanes) {
-        case 4: launch_bond_kernels<4>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, blk, esite, st); break;
-        case 8: launch_bond_kernels<8>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, blk, esite, st); break;
-        case 16: launch_bond_kernels<16>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, blk, esite, st); break;
-        case 32: launch_bond_kernels<32>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, blk, esite, st); break;
-        case 64: launch_bond_kernels<64>(pot, nl, set, d_pos, d_spc, d_F, d_out, ch, trip, blk, esite, st); break;
+      switch (pl.lanes) {
+        case 4: launch_bond_kernels<4>(pot, nl, w, d_pos, d_spc, d_F, d_out, pl.trip, pl.blk, esite, st); break;
+        case 8: launch_bond_kernels<8>(pot, nl, w, d_pos, d_spc, d_F, d_out, pl.trip, pl.blk, esite, st); break;
+        case 16: launch_bond_kernels<16>(pot, nl, w, d_pos, d_spc, d_F, d_out, pl.trip, pl.blk, esite, st); break;
+        case 32: launch_bond_kernels<32>(pot, nl, w, d_pos, d_spc, d_F, d_out, pl.trip, pl.blk, esite, st); break;
+        case 64: launch_bond_kernels<64>(pot, nl, w, d_pos, d_spc, d_F, d_out, pl.trip, pl.blk, esite, st); break;
         default: throw Error(AK_ERR_STATE, "unsupported neighbour stride");
       }
     }
   }
   if (pot.kind == KIND_EAM || n == 0)
-    hipLaunchKernelGGL(k_struct_final, dim3(set.n_struct), dim3(256), 0, st,
-                       set.d_geoms.as<Geom>(), (const int*)ch.first.as<int>(), ch.n,
-                       (const double*)ch.partial.as<double>(), d_out,
-                       pot.kind == KIND_EAM && n > 0 ? (const int*)pot.pair_flag.as<int>()
+    hipLaunchKernelGGL(k_struct_final, dim3(w.n_struct), dim3(256), 0, st, w.geoms, w.first,
+                       w.n_chunks, (const double*)w.partial, d_out,
+                       pot.kind == KIND_EAM && n > 0 ? (const int*)w.pair_flag
                                                      : (const int*)nullptr);
+}
+
+void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
+                    const int* d_spc, double* d_F, double* d_out, hipStream_t st,
+                    double* esite) {
+  const int n = set.n_atoms;
+  const ForcePlan pl = force_plan(pot, nl, esite != nullptr);
+  AK_REQUIRE(nl.n_atoms == n, AK_ERR_STATE, "neighbour list was built for another size");
+  SetDev::Chunks& ch = set.chunks(pl.chunk, st);
+  ForceWork w;
+  w.n = n;
+  w.n_struct = set.n_struct;
+  w.sid = set.d_sid.as<int>();
+  w.geoms = set.d_geoms.as<Geom>();
+  w.rng = ch.rng.as<int>();
+  w.first = ch.first.as<int>();
+  w.partial = ch.partial.as<double>();
+  w.n_chunks = ch.n;
+  // every chunk writes its flag in each call (no clearing needed)
+  if (pot.kind == KIND_EAM) {
+    pot.pair_flag.ensure(sizeof(int) * (size_t)std::max(1, ch.n));
+    w.pair_flag = pot.pair_flag.as<int>();
+    if (n > 0) {
+      pot.aux.ensure(sizeof(double) * 2 * (size_t)n);
+      w.aux = pot.aux.as<double>();
+    }
+  } else if (n > 0) {
+    pot.G.ensure(sizeof(double) * 3 * (size_t)n * nl.stride);
+    pot.amask.ensure(sizeof(uint64_t) * (size_t)n);
+    w.G = pot.G.as<double>();
+    w.amask = pot.amask.as<uint64_t>();
+  }
+  compute_forces_work(pot, nl, pl, w, d_pos, d_spc, d_F, d_out, st, esite);
 }
 
 // ---------------------------------------------------------------------------
