@@ -63,6 +63,8 @@ SIGNATURES = {
                                   _pf64, _pf64, _pf64]),
     "ak_compute_device": (ctypes.c_int, [_handle, _handle, _i32, _pf64, _pf64, _vp, _vp, _vp,
                                          _vp, _vp]),
+    "ak_compute_device_noalloc": (ctypes.c_int, [_handle, _handle, _i32, _pf64, _pf64, _vp,
+                                                 _vp, _vp, _vp, _vp]),
     "ak_compute_batched": (ctypes.c_int, [_handle, _i32, _pi64, _pf64, _pf64, _pf64, _pi32,
                                           _pf64, _pf64, _pf64]),
     "ak_compute_batched_device": (ctypes.c_int, [_handle, _i32, _pi64, _pf64, _vp, _vp, _vp,

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -672,6 +673,139 @@ int ak_compute_device(ak_potential* P, ak_nlist* L, int32_t n, const double* lat
       spc = P->d_spc.as<int>();
     }
     compute_forces(P->pot, L->nl, L->set, d_pos, spc, d_F, d_es, st);
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Force call without device allocation.  Every buffer of the force kernels
+// besides the caller's inputs and outputs (geometry, launch chunks, block
+// partials, bond gradients, masks, EAM derivatives) is carved out of one
+// file-scope device array, `scratch`, of kScratchBytes.  Host code never
+// passes the array's name to a kernel: the address of each device's copy
+// comes from hipGetSymbolAddress on the first call on that device (with the
+// device current) and is kept per device.  All calls on one device share
+// its copy, so calls that overlap on one device race.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kScratchDevices] = {};
+
+unsigned char* device_scratch() {
+  int dev = 0;
+  AK_HIP(hipGetDevice(&dev));
+  AK_REQUIRE(dev >= 0 && dev < kScratchDevices, AK_ERR_STATE, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    AK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    AK_REQUIRE(p != nullptr, AK_ERR_HIP, "no device address for the scratch array");
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
+// Geometry, atom -> structure map (all 0) and launch chunks of one structure
+// of n atoms, written by a kernel (no memset or host copy).
+__global__ void k_work_setup(Geom g, int n, int chunk, int n_chunks, Geom* __restrict__ geoms,
+                             int* __restrict__ sid, int* __restrict__ rng,
+                             int* __restrict__ first) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) {
+    *geoms = g;
+    first[0] = 0;
+    first[1] = n_chunks;
+  }
+  if (t < n) sid[t] = 0;
+  if (t < n_chunks) {
+    rng[2 * t] = t * chunk;
+    rng[2 * t + 1] = min((t + 1) * chunk, n);
+  }
+}
+}  // namespace
+
+int ak_compute_device_noalloc(ak_potential* P, ak_nlist* L, int32_t n, const double* lattice,
+                              const double* inv, const double* d_pos, const int32_t* d_spc,
+                              double* d_F, double* d_es, void* stream) {
+  return guarded([&] {
+    AK_REQUIRE(P && L && lattice && d_pos && d_spc && d_F && d_es && n >= 0, AK_ERR_ARG,
+               "bad arguments (device positions, species, forces and energy/stress are "
+               "required)");
+    AK_REQUIRE(L->nl.r_cut + 1e-12 >= P->pot.r_cut, AK_ERR_ARG,
+               "neighbour list r_cut is smaller than the potential's cutoff");
+    AK_REQUIRE(L->nl.valid && L->nl.n_atoms == n, AK_ERR_STATE,
+               "the neighbour list was not built for this structure");
+    hipStream_t st = stream ? (hipStream_t)stream : P->stream;
+    unsigned char* base = device_scratch();
+    // geometry of the cell on the host (no allocation)
+    Geom g{};
+    std::memcpy(g.lat, lattice, sizeof g.lat);
+    if (inv)
+      std::memcpy(g.inv, inv, sizeof g.inv);
+    else
+      invert3(g.lat, g.inv);
+    const CellInfo ci = cell_info(g.lat);
+    g.volume = ci.volume;
+    AK_REQUIRE(L->nl.r_list < ci.min_width / 2.0, AK_ERR_MIN_IMAGE,
+               "r_cut + skin_width must be < half the smallest perpendicular cell width for "
+               "the minimum-image convention to be valid");
+    for (int ax = 0; ax < 3; ++ax) g.nc[ax] = std::max(1, (int)std::floor(ci.perp[ax] / L->nl.r_list));
+    g.natoms = n;
+    const ForcePlan pl = force_plan(P->pot, L->nl, false);
+    const int n_chunks = n > 0 ? (n + pl.chunk - 1) / pl.chunk : 0;
+    // scratch layout, 256-byte aligned segments
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+      const size_t at = off;
+      off = (off + bytes + 255) & ~size_t(255);
+      return at;
+    };
+    const bool eam = P->pot.kind == KIND_EAM;
+    const size_t o_geom = take(sizeof(Geom));
+    const size_t o_sid = take(sizeof(int) * (size_t)std::max(1, n));
+    const size_t o_rng = take(sizeof(int) * 2 * (size_t)std::max(1, n_chunks));
+    const size_t o_first = take(sizeof(int) * 2);
+    const size_t o_part = take(sizeof(double) * 10 * (size_t)std::max(1, n_chunks));
+    const size_t o_G = eam ? 0 : take(sizeof(double) * 3 * (size_t)n * L->nl.stride);
+    const size_t o_mask = eam ? 0 : take(sizeof(uint64_t) * (size_t)n);
+    const size_t o_aux = eam ? take(sizeof(double) * 2 * (size_t)n) : 0;
+    const size_t o_flag = eam ? take(sizeof(int) * (size_t)std::max(1, n_chunks)) : 0;
+    if (off > kScratchBytes) {
+      char buf[256];
+      snprintf(buf, sizeof buf,
+               "%d atoms with %d neighbour slots per row need %zu bytes of scratch; the "
+               "no-allocation force call holds at most %zu",
+               n, L->nl.stride, off, kScratchBytes);
+      throw Error(AK_ERR_CAPACITY, buf);
+    }
+    ForceWork w;
+    w.n = n;
+    w.n_struct = 1;
+    w.n_chunks = n_chunks;
+    Geom* geoms = reinterpret_cast<Geom*>(base + o_geom);
+    int* sid = reinterpret_cast<int*>(base + o_sid);
+    int* rng = reinterpret_cast<int*>(base + o_rng);
+    int* first = reinterpret_cast<int*>(base + o_first);
+    w.geoms = geoms;
+    w.sid = sid;
+    w.rng = rng;
+    w.first = first;
+    w.partial = reinterpret_cast<double*>(base + o_part);
+    if (eam) {
+      w.aux = reinterpret_cast<double*>(base + o_aux);
+      w.pair_flag = reinterpret_cast<int*>(base + o_flag);
+    } else {
+      w.G = reinterpret_cast<double*>(base + o_G);
+      w.amask = reinterpret_cast<uint64_t*>(base + o_mask);
+    }
+    const int items = std::max(1, std::max(n, n_chunks));
+    hipLaunchKernelGGL(k_work_setup, dim3((items + 255) / 256), dim3(256), 0, st, g, n, pl.chunk,
+                       n_chunks, geoms, sid, rng, first);
+    compute_forces_work(P->pot, L->nl, pl, w, d_pos, d_spc, d_F, d_es, st);
+    AK_HIP(hipGetLastError());
   });
 }
 

@@ -151,6 +151,21 @@ int ak_compute_device(ak_potential *pot, ak_nlist *nl, int32_t n_atoms,
                       const double *d_positions, const int32_t *d_species, double *d_forces,
                       double *d_energy_stress, void *stream);
 
+/* ak_compute_device without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as ak_compute_device, but
+ * d_species is required, and the kernels' scratch (geometry, launch chunks,
+ * block partials, bond gradients, EAM embedding derivatives) lives in one
+ * 64 MiB device array of the library, per GPU, held until the process
+ * ends; inputs that need more (roughly 24 * N * list stride bytes) fail
+ * with AK_ERR_CAPACITY.  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (other streams or threads) race. */
+int ak_compute_device_noalloc(ak_potential *pot, ak_nlist *nl, int32_t n_atoms,
+                              const double *lattice, const double *inv_lattice,
+                              const double *d_positions, const int32_t *d_species,
+                              double *d_forces, double *d_energy_stress, void *stream);
+
 /* Many independent structures in one launch sequence (the phonon / elastic /
  * fitting callers of the force function, solvers/phonon.py:412-482,
  * solvers/elastic.py:245-291).  atom_offsets (n_struct + 1); lattices
