@@ -1504,16 +1504,19 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
   S.es.ensure(sizeof(double) * 10);
   const unsigned nb = (unsigned)((n + 255) / 256);
   S.ke.ensure(sizeof(double) * (S.flex() ? 7 : 1) * nb);
-  S.hist_ep.ensure(sizeof(double) * (n_steps + 1));
-  S.hist_ek.ensure(sizeof(double) * (n_steps + 1));
-  S.hist_ec.ensure(sizeof(double) * (n_steps + 1));
-  S.hist_P.ensure(sizeof(double) * (n_steps + 1));
-  S.hist_nb.ensure(sizeof(int) * (n_steps + 1));
+  // one record slot per step, plus the two set-up steps (dt = 0) before
+  // the counters are reset, which record at slots 0 and 1 even when
+  // n_steps is 0
+  S.hist_ep.ensure(sizeof(double) * (n_steps + 2));
+  S.hist_ek.ensure(sizeof(double) * (n_steps + 2));
+  S.hist_ec.ensure(sizeof(double) * (n_steps + 2));
+  S.hist_P.ensure(sizeof(double) * (n_steps + 2));
+  S.hist_nb.ensure(sizeof(int) * (n_steps + 2));
   S.V = set.geoms[0].volume;
   if (S.thermo == 4 && (opt.cell_history || opt.ptensor_history)) {
     S.cell_hist = true;
-    S.hist_L.ensure(sizeof(double) * 9 * (size_t)(n_steps + 1));
-    S.hist_Pt.ensure(sizeof(double) * 6 * (size_t)(n_steps + 1));
+    S.hist_L.ensure(sizeof(double) * 9 * (size_t)(n_steps + 2));
+    S.hist_Pt.ensure(sizeof(double) * 6 * (size_t)(n_steps + 2));
   }
   AK_REQUIRE(opt.frame_every >= 0, AK_ERR_ARG, "frame_every must be >= 0");
   if (opt.frame_every > 0 && opt.frames) {
@@ -1662,7 +1665,7 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
       }
       S.np.ensure(sizeof(double) * q.size());
       AK_HIP(hipMemcpy(S.np.p, q.data(), sizeof(double) * q.size(), hipMemcpyHostToDevice));
-      S.hist_V.ensure(sizeof(double) * (n_steps + 1));
+      S.hist_V.ensure(sizeof(double) * (n_steps + 2));
     }
     AK_HIP(hipStreamSynchronize(st));
   } else if (S.thermo == 2 || S.thermo == 3) {
@@ -1717,7 +1720,7 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
     S.bath.ensure((size_t)n);
     AK_HIP(hipMemcpy(S.bath.p, opt.bath, (size_t)n, hipMemcpyHostToDevice));
     S.ke_b.ensure(sizeof(double) * 5 * nb);
-    S.hist_bath.ensure(sizeof(double) * 4 * (size_t)(n_steps + 1));
+    S.hist_bath.ensure(sizeof(double) * 4 * (size_t)(n_steps + 2));
   }
   AK_HIP(hipMemcpyAsync(S.pos.p, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
   if (S.fixed) {

@@ -26,6 +26,11 @@ EV_ANG_TO_AMU_ANG_FS2 = (1.60217663e-19 / 1.0e-10) / (1.66053906660e-27) * 1.0e-
 
 @dataclass
 class SiteTerms:
+    """Per-atom site energies and the per-entry data of the reference's
+    SiteTerms (engine/site_terms.py:34-70), with the quantities derived from
+    them: total energy, forces F_c += dUdv, F_nbr -= dUdv, the symmetrised
+    stress -(1/V) sum_m dUdv_m (x) v_m, and the virial part of the flux."""
+
     e_site: np.ndarray
     centre: np.ndarray
     nbr: np.ndarray
@@ -34,34 +39,37 @@ class SiteTerms:
 
     @property
     def energy(self) -> float:
-        return float(self.e_site.sum())
+        return float(np.sum(self.e_site))
 
     def forces(self) -> np.ndarray:
-        F = np.zeros((len(self.e_site), 3))
-        np.add.at(F, self.centre, self.dUdv)
-        np.add.at(F, self.nbr, -self.dUdv)
-        return F
+        n = len(self.e_site)
+        return np.stack([np.bincount(self.centre, self.dUdv[:, a], minlength=n)
+                         - np.bincount(self.nbr, self.dUdv[:, a], minlength=n)
+                         for a in range(3)], axis=1)
 
     def stress(self, volume: float) -> np.ndarray:
-        S = -(self.dUdv.T @ self.vec) / volume
-        return 0.5 * (S + S.T)
+        W = np.tensordot(self.dUdv, self.vec, axes=(0, 0))
+        return -(W + W.T) / (2.0 * volume)
 
     def virial_flux(self, velocities: np.ndarray) -> np.ndarray:
         """-sum_m v_m (dUdv_m . vel_nbr) [eV A/fs], not divided by V."""
-        c = np.einsum("ma,ma->m", self.dUdv, np.asarray(velocities, float)[self.nbr])
-        return -np.einsum("m,ma->a", c, self.vec)
+        proj = np.sum(self.dUdv * np.asarray(velocities, float)[self.nbr], axis=1)
+        return -(self.vec.T @ proj)
 
     @staticmethod
     def combine(parts, n_atoms):
-        parts = [p for p in parts if p is not None]
-        if not parts:
-            z = np.zeros(0, dtype=np.int64)
-            return SiteTerms(np.zeros(n_atoms), z, z, np.zeros((0, 3)), np.zeros((0, 3)))
-        return SiteTerms(np.sum([p.e_site for p in parts], axis=0),
-                         np.concatenate([p.centre for p in parts]),
-                         np.concatenate([p.nbr for p in parts]),
-                         np.concatenate([p.vec for p in parts]),
-                         np.concatenate([p.dUdv for p in parts]))
+        """Site terms of a sum of potentials: energies added per atom, the
+        entries of every part listed one after another."""
+        present = [p for p in parts if p is not None]
+        if not present:
+            none = np.zeros(0, dtype=np.int64)
+            return SiteTerms(np.zeros(n_atoms), none, none, np.zeros((0, 3)), np.zeros((0, 3)))
+        joined = {f: np.concatenate([getattr(p, f) for p in present])
+                  for f in ("centre", "nbr", "vec", "dUdv")}
+        e = np.zeros(n_atoms)
+        for p in present:
+            e = e + p.e_site
+        return SiteTerms(e, **joined)
 
 
 def _inputs(potential, lattice, positions, atom_types):

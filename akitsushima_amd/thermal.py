@@ -50,12 +50,18 @@ def _running_integral(h, dt):
     return run
 
 
-def _hcacf(J, n_lag):
+def _cross_cf(A, B, n_lag):
+    """(n_lag, 3, 3) correlation tensor c[k, a, b] = <A_a(t + k) B_b(t)> of
+    two (n, 3) series."""
     h = np.zeros((n_lag, 3, 3))
     for a in range(3):
         for b in range(3):
-            h[:, a, b] = autocorr_fft(J[:, a], J[:, b], n_lag)
+            h[:, a, b] = autocorr_fft(A[:, a], B[:, b], n_lag)
     return h
+
+
+def _hcacf(J, n_lag):
+    return _cross_cf(J, J, n_lag)
 
 
 class ThermalConductivityCalculator:
@@ -286,34 +292,32 @@ class ThermalConductivityCalculator:
         decomposition add up to the total.  Needs run_md(decompose=...).
         Returns {decomposition: {group: (3, 3) W/(m K)}, "total": (3, 3),
         "sum_rule_residual": {decomposition: max |sum_g kappa_g - kappa|}}."""
-        if not getattr(self, "_J_groups", None):
+        groups = getattr(self, "_J_groups", None)
+        if not groups:
             raise RuntimeError("compute_kappa_decomposition: run run_md(decompose=...) first.")
         J = self._J_series
         dt = float(self._dt_hf)
-        V = abs(np.linalg.det(self.lattice))
         n_lag = min(int(tau_max * 1e3 / dt), len(J) // 2)
-        pref = V / (K_B * self.temperature ** 2) * EV_ANG_K_FS_TO_W_MK
-
-        def integral(A, B):
-            k = np.zeros((3, 3))
-            for a in range(3):
-                for b in range(3):
-                    c = autocorr_fft(A[:, a], B[:, b], n_lag)
-                    k[a, b] = 0.5 * dt * np.sum(c[1:] + c[:-1])
-            return k * pref
-
-        total = integral(J, J)
-        out, resid = {}, {}
-        for name, (names, G) in self._J_groups.items():
-            out[name] = {g: integral(G[:, k], J) for k, g in enumerate(names)}
-            resid[name] = float(np.max(np.abs(sum(out[name].values()) - total)))
-        out["sum_rule_residual"] = resid
-        out["total"] = total
+        scale = (abs(np.linalg.det(self.lattice)) / (K_B * self.temperature ** 2)
+                 * EV_ANG_K_FS_TO_W_MK)
+        # the Green-Kubo integral up to the last lag, trapezoid rule over the
+        # lags as in compute_kappa, of <flux(t + tau) J(t)>
+        gk = lambda flux: _running_integral(_cross_cf(flux, J, n_lag), dt)[-1] * scale
+        total = gk(J)
+        out = {}
+        residual = {}
+        for decomposition, (labels, series) in groups.items():
+            parts = {label: gk(series[:, k]) for k, label in enumerate(labels)}
+            out[decomposition] = parts
+            residual[decomposition] = float(np.abs(np.sum(list(parts.values()), axis=0)
+                                                   - total).max())
         if verbose:
-            for name, parts in out.items():
-                if name in ("sum_rule_residual", "total"):
-                    continue
-                print(f"\n  kappa by {name} (W/m/K; xx yy zz):")
-                for g, k in parts.items():
-                    print(f"    {g:>18s} : {k[0, 0]:9.4f} {k[1, 1]:9.4f} {k[2, 2]:9.4f}")
+            for decomposition, parts in out.items():
+                print(f"\n  Green-Kubo kappa by {decomposition} [W/(m K)], diagonal:")
+                width = max(len(str(g)) for g in parts)
+                for label, k in parts.items():
+                    print("    " + str(label).rjust(width) + "  "
+                          + "  ".join(f"{v:10.4f}" for v in np.diag(k)))
+        out["sum_rule_residual"] = residual
+        out["total"] = total
         return out

@@ -130,3 +130,34 @@ def test_restart_continues_bit_for_bit(tmp_path, monkeypatch):
         assert np.array_equal(st[k], ref.state[k]), k
     for k in ("Q_hot", "Q_cold", "E0", "n_samples", "E_minus_Q_drift_eV"):
         assert st[k] == ref.state[k], k
+
+
+def test_zero_step_run_with_sampling_records_the_start_only():
+    # the two set-up steps record at slots 0 and 1 of the history buffers;
+    # with n_steps = 0 the run still returns just the start state
+    lat, pos, vel, m, bath = _slab()
+    pot = StillingerWeberPotential(SW_SI, species=["Si"])
+    t = ["Si"] * len(pos)
+    print("nemd_run n_steps=0 sample_every=5", flush=True)
+    res = nemd_run(pot, lat, pos, vel, t, bath, 900.0, 300.0, 20.0, dt=1.0, n_steps=0, seed=3,
+                   masses=m, sample_every=5, axis=2, n_bins=8)
+    assert res["e_pot"].shape == (1,) and res["bath"].shape == (1, 4)
+    assert np.array_equal(res["velocities"], vel)
+    assert len(res["bins"]) == 0
+
+
+def test_finished_checkpoint_returns_without_a_device_call(tmp_path, monkeypatch):
+    ck = str(tmp_path / "done.npz")
+    first = _sim().run(20, 40, sample_every=5, checkpoint=ck, checkpoint_every=30,
+                       verbose=False)
+
+    def forbidden(*a, **k):
+        raise AssertionError("a finished run called the device")
+
+    monkeypatch.setattr(nemd_mod, "nemd_run", forbidden)
+    again = _sim().run(20, 40, sample_every=5, checkpoint=ck, checkpoint_every=30,
+                       verbose=False)
+    for k in ("pos", "vel", "ke_bin", "n_bin", "j_bin", "q_block"):
+        assert np.array_equal(again[k], first[k]), k
+    for k in ("Q_hot", "Q_cold", "E0", "n_samples", "n_prod", "E_minus_Q_drift_eV"):
+        assert again[k] == first[k], k

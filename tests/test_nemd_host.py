@@ -74,3 +74,37 @@ def test_results_from_linear_segments_and_an_interface_jump():
     k = sim.results(interfaces=[0.3])["kapitza"][0]
     assert np.isclose(k["delta_T_K"], 25.0)
     assert np.isclose(k["R_m2K_per_W"], 25.0 / (J * EV_ANG2_FS_TO_W_M2))
+
+
+def test_bath_interval_rule():
+    from akitsushima_amd.nemd import _bath_members
+
+    f = np.array([0.0, 0.3, 0.55, 0.999999, -1e-12, 0.7])
+    # [lo, hi), a wrapping interval, and lo == hi (a full period, as (0, 1))
+    assert _bath_members(f, (0.3, 0.55)).tolist() == [1]
+    assert _bath_members(f, (0.6, 0.31)).tolist() == [0, 1, 3, 4, 5]
+    assert _bath_members(f, (0.0, 1.0)).tolist() == list(range(6))
+    # a layer a rounding error below 0 belongs to a slab starting at 0
+    assert 4 in _bath_members(f, (0.0, 0.05)).tolist()
+
+
+def test_production_sums_round_trip_through_a_checkpoint(tmp_path):
+    from akitsushima_amd.nemd import _Production
+
+    p = _Production(6)
+    for k in range(-3, 9):
+        p.add_step(0.1 * k, -0.05 * k, k, 8, 1.0)
+    p.add_sample(2, np.arange(6.0), np.ones(6), np.full(6, 0.5))
+    p.add_sample(3, np.arange(6.0), np.ones(6))
+    p.step, p.E0, p.E_last = 11, -5.0, -4.5
+    assert p.n_prod == 8 and p.blocks[:, 2].tolist() == [2.0, 2.0, 2.0, 2.0]
+    assert np.isclose(p.blocks[0, 0], 0.1 + 0.2) and p.n_jsamp.tolist() == [0, 0, 1, 0]
+    np.savez(tmp_path / "p.npz", **p.arrays())
+    q = _Production.from_arrays(np.load(tmp_path / "p.npz"))
+    a, b = p.as_dict(), q.as_dict()
+    for k in a:
+        if isinstance(a[k], np.ndarray):
+            assert np.array_equal(a[k], b[k]), k
+        else:
+            assert a[k] == b[k], k
+    assert _Production.from_arrays(np.load(tmp_path / "p.npz")).E0 == -5.0
