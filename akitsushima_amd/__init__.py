@@ -8,7 +8,8 @@ batched evaluation of many structures, device-resident NVE, NVT
 (Nose-Hoover chain, Langevin, CSVR) and NPT (isotropic or flexible cell)
 integrators behind the reference's simulation classes, site energies and the
 exact many-body heat flux (Green-Kubo thermal conductivity), a Tersoff
-parameter gradient, and the batched callers of the force call: force
+parameter gradient with the reference's fitting interface (parameter set,
+loss, residuals), and the batched callers of the force call: force
 constants and Gamma-point phonons, and elastic constants.  The compute
 path is the HIP library ``libakitsushima_amd.so`` only; importing works
 without a GPU, computing does not.
@@ -33,6 +34,8 @@ from .nemd import NEMDSimulation
 from .elastic import elastic_constants
 from .phonon import force_constants, gamma_frequencies
 from .zhou import zhou_eam
+from .fitting import (TersoffParameterSet, generic_loss_and_grad, generic_loss_function,
+                      generic_residuals, residual_jacobian, residual_length)
 
 __version__ = "0.1.0"
 
@@ -60,4 +63,10 @@ __all__ = [
     "elastic_constants",
     "force_constants",
     "gamma_frequencies",
+    "TersoffParameterSet",
+    "generic_loss_function",
+    "generic_loss_and_grad",
+    "generic_residuals",
+    "residual_length",
+    "residual_jacobian",
 ]

@@ -92,6 +92,7 @@ SIGNATURES = {
                                             _pf64]),
     "ak_tersoff_fit_load": (ctypes.c_int, [_handle, _i32, _pi64, _pf64, _pf64, _pf64, _pf64,
                                            _pf64]),
+    "ak_tersoff_fit_terms": (ctypes.c_int, [_handle, _i32, _pf64, _i32, _pf64]),
     "ak_tersoff_fit_eval": (ctypes.c_int, [_handle, _pf64, _pf64, _pf64, _pf64, _pf64, _pf64,
                                            _pf64]),
 }

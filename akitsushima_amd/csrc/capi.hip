@@ -998,6 +998,8 @@ void tersoff_fit_set(Potential& pot, int S, const int64_t* off, const double* la
 void tersoff_fit_eval(Potential& pot, const double* theta, const double* weights, double* loss,
                       double* grad, double* energies, double* forces, double* stresses,
                       hipStream_t st);
+void tersoff_fit_terms(Potential& pot, int S, const double* terms, int huber,
+                       const double* deltas);
 }  // namespace ak
 
 extern "C" {
@@ -1198,6 +1200,15 @@ int ak_tersoff_fit_eval(ak_potential* P, const double* theta, const double* weig
     AK_REQUIRE(P->pot.kind == KIND_TERSOFF && P->pot.ns == 1, AK_ERR_ARG,
                "the parameter gradient is defined for a single-species Tersoff potential");
     tersoff_fit_eval(P->pot, theta, weights, loss, grad, energies, forces, stresses, P->stream);
+  });
+}
+
+int ak_tersoff_fit_terms(ak_potential* P, int32_t S, const double* terms, int32_t loss_type,
+                         const double* huber_deltas) {
+  return guarded([&] {
+    AK_REQUIRE(P && terms && S >= 1 && (loss_type == 0 || huber_deltas), AK_ERR_ARG,
+               "bad arguments");
+    tersoff_fit_terms(P->pot, S, terms, loss_type, huber_deltas);
   });
 }
 

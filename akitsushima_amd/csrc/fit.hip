@@ -279,15 +279,24 @@ __global__ __launch_bounds__(256) void k_tersoff_grad(
   }
 }
 
-// Per atom: lambda_a = 2 w_F r0^2 / (3 N_s) (F_a - F_a^ref) and the atom's
-// share of the force term of the loss.
+// Per-structure loss terms (S, 4): trust weight times scale squared, 1 when
+// the structure has reference forces, the volume of its stress scale, 1 when
+// it has a reference stress (fitting/loss.py:212-242).
+constexpr int kTerms = 4;
+
+// Per atom: lambda_a = 2 r0^2 / (3 N_s) (F_a - F_a^ref) and the atom's share
+// r0^2 / (3 N_s) |F_a - F_a^ref|^2 of the structure's mean squared force
+// residual (zero for a structure without reference forces); the structure's
+// weights multiply lambda later (k_fit_lam_scale).
 __global__ void k_fit_atoms(int n, const int* __restrict__ sid, const Geom* __restrict__ geoms,
                             const double* __restrict__ F, const double* __restrict__ Fref,
-                            double wF, double* __restrict__ lam, double* __restrict__ lf) {
+                            double r0sq, const double* __restrict__ terms,
+                            double* __restrict__ lam, double* __restrict__ lf) {
   int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= n) return;
-  const Geom& g = geoms[sid[a]];
-  double scale = wF / (3.0 * g.natoms);
+  const int s = sid[a];
+  const Geom& g = geoms[s];
+  double scale = terms[kTerms * s + 1] != 0.0 ? r0sq / (3.0 * g.natoms) : 0.0;
   double s2 = 0.0;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -298,14 +307,31 @@ __global__ void k_fit_atoms(int n, const int* __restrict__ sid, const Geom* __re
   lf[a] = scale * s2;
 }
 
-// Per structure (one block): loss, kappa_s, mu_s / V_s.
+// Huber transform of a squared residual q (fitting/loss.py:27-48): q while
+// sqrt(q) <= delta, else 2 delta sqrt(q) - delta^2; and its derivative in q.
+__device__ __forceinline__ void huber_sq(double q, double delta, double& rho, double& drho) {
+  const double r = sqrt(fmax(q, 0.0));
+  if (r <= delta) {
+    rho = q;
+    drho = 1.0;
+  } else {
+    rho = 2.0 * delta * r - delta * delta;
+    drho = delta / r;
+  }
+}
+
+// Per structure (one block): loss, kappa_s = dL/dE_s, the force factor
+// fmul_s (dL/dF = fmul_s lambda) and mu_s / V_s (fitting/loss.py:212-242):
+//   L_s = c_s [w_E rho(dE^2) + w_F rho(r0^2 <dF^2>) + w_S rho((V eps0)^2 <dS^2>)]
+// with rho the identity, or the Huber transform with deltas (dE, dF, dS).
 //   out_es (S, 10) from the forward pass; ref (S, 10) = [e per atom, sigma];
-//   loss_s (S), kappa (S), muv (S, 9).
+//   loss_s (S), kappa (S), fmul (S), muv (S, 9).
 __global__ __launch_bounds__(256) void k_fit_structs(
     const Geom* __restrict__ geoms, const double* __restrict__ es,
-    const double* __restrict__ ref, const double* __restrict__ lf, double wE, double wS,
-    double eps0, double* __restrict__ loss_s, double* __restrict__ kappa,
-    double* __restrict__ muv) {
+    const double* __restrict__ ref, const double* __restrict__ lf, double wE, double wF,
+    double wS, double eps0, const double* __restrict__ terms, int huber, double hE, double hF,
+    double hS, double* __restrict__ loss_s, double* __restrict__ kappa,
+    double* __restrict__ fmul, double* __restrict__ muv) {
   __shared__ double smem[16];
   const int s = blockIdx.x;
   const Geom& g = geoms[s];
@@ -316,18 +342,40 @@ __global__ __launch_bounds__(256) void k_fit_structs(
   if (threadIdx.x == 0) {
     const double* E = es + 10 * (size_t)s;
     const double* R = ref + 10 * (size_t)s;
-    double N = g.natoms;
-    double dEs = E[0] - N * R[0];
-    double sc = wS * (g.volume * eps0) * (g.volume * eps0) / 9.0;
+    const double* T = terms + kTerms * (size_t)s;
+    const double c = T[0];
+    const double N = g.natoms;
+    const double dEs = E[0] - N * R[0];
+    const double vs = T[2] * eps0;
+    const double sc = T[3] != 0.0 ? vs * vs / 9.0 : 0.0;
     double ls = 0.0;
     for (int q = 0; q < 9; ++q) {
-      double d = E[1 + q] - R[1 + q];
+      const double d = E[1 + q] - R[1 + q];
       ls += d * d;
-      muv[9 * (size_t)s + q] = 2.0 * sc * d / g.volume;
     }
-    loss_s[s] = wE * dEs * dEs + v[0] + sc * ls;
-    kappa[s] = 2.0 * wE * dEs;
+    double rE = dEs * dEs, dE = 1.0, rF = v[0], dF = 1.0, rS = sc * ls, dS = 1.0;
+    if (huber) {
+      huber_sq(dEs * dEs, hE, rE, dE);
+      huber_sq(v[0], hF, rF, dF);
+      huber_sq(sc * ls, hS, rS, dS);
+    }
+    for (int q = 0; q < 9; ++q)
+      muv[9 * (size_t)s + q] = c * wS * dS * 2.0 * sc * (E[1 + q] - R[1 + q]) / g.volume;
+    loss_s[s] = c * (wE * rE + wF * rF + wS * rS);
+    kappa[s] = c * wE * dE * 2.0 * dEs;
+    fmul[s] = c * wF * dF;
   }
+}
+
+// lambda_a *= fmul of the atom's structure.
+__global__ void k_fit_lam_scale(int n, const int* __restrict__ sid,
+                                const double* __restrict__ fmul, double* __restrict__ lam) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  const double f = fmul[sid[a]];
+  lam[3 * (size_t)a] *= f;
+  lam[3 * (size_t)a + 1] *= f;
+  lam[3 * (size_t)a + 2] *= f;
 }
 
 // grad_s[p] = kappa_s sum_a dE[p, a] + sum_a dD[p, a] (one block per (s, p)).
@@ -387,6 +435,11 @@ struct FitState {
   SetDev set;
   NList nl;
   DevBuf pos, spc, fref, ref, F, es, lam, lf, loss_s, kappa, muv, dE, dD, grad_s, cbuf, tot;
+  // per-structure loss terms (kTerms each) and the Huber option
+  std::vector<double> h_terms;
+  DevBuf terms, fmul;
+  int huber = 0;
+  double hdelta[3] = {0.0, 0.0, 0.0};
   // pinned staging: [params 16 | theta 14 | loss, grad 15 | es 10 S]
   double* pin = nullptr;
   size_t pin_n = 0;
@@ -454,8 +507,47 @@ void tersoff_fit_set(Potential& pot, int S, const int64_t* off, const double* la
                         st));
   AK_HIP(hipMemsetAsync(F.spc.p, 0, sizeof(int) * n, st));
   AK_HIP(hipStreamSynchronize(st));
+  // default terms: weight 1, scale 1, forces and stress present, the stress
+  // scaled by the cell volume; squared loss
+  F.h_terms.assign(kTerms * (size_t)S, 1.0);
+  for (int s = 0; s < S; ++s) F.h_terms[kTerms * s + 2] = cell_info(lattices + 9 * s).volume;
+  F.terms.ensure(sizeof(double) * kTerms * S);
+  F.fmul.ensure(sizeof(double) * S);
+  AK_HIP(hipMemcpy(F.terms.p, F.h_terms.data(), sizeof(double) * kTerms * S,
+                   hipMemcpyHostToDevice));
+  F.huber = 0;
   F.nl.valid = false;
   F.loaded = true;
+}
+
+// Per-structure loss terms of the loaded batch and the loss type
+// (fitting/loss.py:51-248): terms (S, 4) = [trust weight w_s, scale c_s,
+// stress volume V_s, flags (1: reference forces, 2: reference stress)];
+// huber: 0 squared, 1 Huber with deltas (energy, forces, stress) in eV.
+void tersoff_fit_terms(Potential& pot, int S, const double* terms, int huber,
+                       const double* deltas) {
+  FitState& F = fit_state(pot);
+  AK_REQUIRE(F.loaded && F.S == S, AK_ERR_STATE, "terms for another batch than the loaded one");
+  AK_REQUIRE(huber == 0 || huber == 1, AK_ERR_ARG, "loss type must be 0 (squared) or 1 (huber)");
+  std::vector<double> h(kTerms * (size_t)S);
+  for (int s = 0; s < S; ++s) {
+    const double w = terms[4 * s], c = terms[4 * s + 1], v = terms[4 * s + 2];
+    const int flags = (int)terms[4 * s + 3];
+    AK_REQUIRE(std::isfinite(w) && std::isfinite(c) && std::isfinite(v) && flags >= 0 &&
+                   flags <= 3 && terms[4 * s + 3] == flags,
+               AK_ERR_ARG, "bad per-structure loss terms");
+    h[kTerms * s] = w * c * c;
+    h[kTerms * s + 1] = (flags & 1) ? 1.0 : 0.0;
+    h[kTerms * s + 2] = v;
+    h[kTerms * s + 3] = (flags & 2) ? 1.0 : 0.0;
+  }
+  if (huber)
+    for (int q = 0; q < 3; ++q)
+      AK_REQUIRE(std::isfinite(deltas[q]) && deltas[q] >= 0.0, AK_ERR_ARG, "bad Huber delta");
+  F.h_terms = h;
+  AK_HIP(hipMemcpy(F.terms.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+  F.huber = huber;
+  for (int q = 0; q < 3; ++q) F.hdelta[q] = huber ? deltas[q] : 0.0;
 }
 
 // Loss and gradient at theta for the loaded batch.
@@ -517,8 +609,8 @@ void tersoff_fit_eval(Potential& pot, const double* theta, const double* weights
   compute_forces(pot, F.nl, F.set, F.pos.as<double>(), F.spc.as<int>(), F.F.as<double>(),
                  F.es.as<double>(), st);
   // loss and adjoints
-  const double wE = weights[0], wF = weights[1] * weights[3] * weights[3], wS = weights[2],
-               eps0 = weights[4];
+  const double wE = weights[0], wF = weights[1], r0sq = weights[3] * weights[3],
+               wS = weights[2], eps0 = weights[4];
   F.lam.ensure(sizeof(double) * 3 * n);
   F.lf.ensure(sizeof(double) * n);
   F.loss_s.ensure(sizeof(double) * S);
@@ -526,13 +618,18 @@ void tersoff_fit_eval(Potential& pot, const double* theta, const double* weights
   F.muv.ensure(sizeof(double) * 9 * S);
   hipLaunchKernelGGL(k_fit_atoms, dim3((n + 255) / 256), dim3(256), 0, st, n,
                      F.set.d_sid.as<int>(), F.set.d_geoms.as<Geom>(),
-                     (const double*)F.F.as<double>(), (const double*)F.fref.as<double>(), wF,
-                     F.lam.as<double>(), F.lf.as<double>());
+                     (const double*)F.F.as<double>(), (const double*)F.fref.as<double>(), r0sq,
+                     (const double*)F.terms.as<double>(), F.lam.as<double>(), F.lf.as<double>());
   hipLaunchKernelGGL(k_fit_structs, dim3(S), dim3(256), 0, st, F.set.d_geoms.as<Geom>(),
                      (const double*)F.es.as<double>(), (const double*)F.ref.as<double>(),
-                     (const double*)F.lf.as<double>(), wE, wS, eps0, F.loss_s.as<double>(),
-                     F.kappa.as<double>(), F.muv.as<double>());
+                     (const double*)F.lf.as<double>(), wE, wF, wS, eps0,
+                     (const double*)F.terms.as<double>(), F.huber, F.hdelta[0], F.hdelta[1],
+                     F.hdelta[2], F.loss_s.as<double>(), F.kappa.as<double>(),
+                     F.fmul.as<double>(), F.muv.as<double>());
   if (grad) {
+    hipLaunchKernelGGL(k_fit_lam_scale, dim3((n + 255) / 256), dim3(256), 0, st, n,
+                       F.set.d_sid.as<int>(), (const double*)F.fmul.as<double>(),
+                       F.lam.as<double>());
     F.dE.ensure(sizeof(double) * 14 * (size_t)n);
     F.dD.ensure(sizeof(double) * 14 * (size_t)n);
     F.grad_s.ensure(sizeof(double) * 14 * (size_t)S);

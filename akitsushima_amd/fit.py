@@ -42,12 +42,31 @@ def params_from_theta(theta):
 
 class TersoffFit:
     """structures: list of dicts with lattice, positions, ref_energy (eV per
-    atom), ref_forces (N, 3), ref_stress (3, 3)."""
+    atom), ref_forces (N, 3) and ref_stress (3, 3) (either may be None or
+    absent: its term is left out), and optionally the per-structure keys of
+    fitting/loss.py:212-242: "weight" (trust, linear), "scale" (unit bridge,
+    squared) and "volume" (of the stress scale; default |det lattice|).
+    weights: w_E ("energy"), w_F ("forces"), w_S ("stress"), "r0", "eps0";
+    with loss_type "huber" also "huber_delta_energy", "huber_delta_forces"
+    and "huber_delta_stress" (fitting/loss.py:27-48, 166-175)."""
 
-    def __init__(self, structures, weights=None, autotune=True):
+    def __init__(self, structures, weights=None, autotune=True, loss_type="squared"):
         w = dict(DEFAULT_WEIGHTS)
         w.update(weights or {})
-        self.weights = np.array([w["energy"], w["forces"], w["stress"], w["r0"], w["eps0"]])
+        self.weights = np.array([w["energy"], w["forces"], w["stress"], w["r0"], w["eps0"]],
+                                float)
+        if loss_type not in ("squared", "huber"):
+            raise ValueError(f"TersoffFit: unknown loss_type '{loss_type}'. Choose 'squared' "
+                             "or 'huber'.")
+        self.loss_type = loss_type
+        self.huber = None
+        if loss_type == "huber":
+            keys = ("huber_delta_energy", "huber_delta_forces", "huber_delta_stress")
+            for k in keys:
+                if k not in w:
+                    raise ValueError(f"TersoffFit: loss_type='huber' requires weights['{k}'] "
+                                     "(eV) -- no silent default is guessed.")
+            self.huber = np.array([float(w[k]) for k in keys])
         S = len(structures)
         self.S = S
         self.off = np.zeros(S + 1, np.int64)
@@ -58,10 +77,19 @@ class TersoffFit:
         self.pos = np.ascontiguousarray(np.concatenate([np.asarray(st["positions"], float)
                                                         for st in structures]))
         self.ref_e = np.ascontiguousarray([float(st["ref_energy"]) for st in structures])
-        self.ref_f = np.ascontiguousarray(np.concatenate([np.asarray(st["ref_forces"], float)
-                                                          for st in structures]))
-        self.ref_s = np.ascontiguousarray(np.stack([np.asarray(st["ref_stress"], float)
-                                                    for st in structures]))
+        has_f = [st.get("ref_forces") is not None for st in structures]
+        has_s = [st.get("ref_stress") is not None for st in structures]
+        self.ref_f = np.ascontiguousarray(np.concatenate(
+            [np.asarray(st["ref_forces"], float) if h else np.zeros((len(st["positions"]), 3))
+             for st, h in zip(structures, has_f)]))
+        self.ref_s = np.ascontiguousarray(np.stack(
+            [np.asarray(st["ref_stress"], float) if h else np.zeros((3, 3))
+             for st, h in zip(structures, has_s)]))
+        self.terms = np.ascontiguousarray(np.array(
+            [[float(st.get("weight", 1.0)), float(st.get("scale", 1.0)),
+              float(st.get("volume", abs(np.linalg.det(np.asarray(st["lattice"], float))))),
+              float((1 if hf else 0) | (2 if hs else 0))]
+             for st, hf, hs in zip(structures, has_f, has_s)], float).reshape(S, 4))
         self._pot = TersoffPotential({"A": 1.0, "B": 1.0, "lambda1": 1.0, "lambda2": 1.0,
                                       "lambda3": 0.0, "beta": 1.0, "n": 1.0, "c": 1.0,
                                       "d": 1.0, "h": 0.0, "R": 2.85, "D": 0.15})
@@ -78,6 +106,10 @@ class TersoffFit:
                 h, self.S, self.off.ctypes.data_as(_lib._pi64), self.lat.ctypes.data_as(P),
                 self.pos.ctypes.data_as(P), self.ref_e.ctypes.data_as(P),
                 self.ref_f.ctypes.data_as(P), self.ref_s.ctypes.data_as(P)))
+            deltas = self.huber if self.huber is not None else np.zeros(3)
+            check(lib().ak_tersoff_fit_terms(h, self.S, self.terms.ctypes.data_as(P),
+                                             1 if self.huber is not None else 0,
+                                             deltas.ctypes.data_as(P)))
             self._tls.loaded = h
             if self._autotune and self.launch_shape is None:
                 # launch shape of the forward force pass, timed on this batch

@@ -391,6 +391,18 @@ int ak_tersoff_fit_eval(ak_potential *pot, const double *theta14, const double *
                         double *loss, double *grad14, double *energies, double *forces,
                         double *stresses);
 
+/* Per-structure terms and loss type of the loaded batch, the options of
+ * generic_loss_function (fitting/loss.py:51-248): terms (n_struct, 4) =
+ * [trust weight w_s ("weight", linear), unit scale c_s ("scale",
+ * squared), stress volume V_s ("volume", default det(lattice)), flags (1:
+ * the structure has reference forces, 2: reference stress)];
+ * loss_type 0 squared, 1 Huber with huber_deltas = (energy, forces,
+ * stress) in eV applied per structure and term (fitting/loss.py:27-48,
+ * 166-175).  ak_tersoff_fit_load resets them to weight 1, scale 1, the cell
+ * volume, both references present, squared loss. */
+int ak_tersoff_fit_terms(ak_potential *pot, int32_t n_struct, const double *terms,
+                         int32_t loss_type, const double *huber_deltas);
+
 /* ---- Measurement -------------------------------------------------------- */
 
 /* No counterpart in the reference: timing support for benchmarks.  While

@@ -696,6 +696,64 @@ def tersoff_loss(structures, records, species, weights=None, nthreads=1):
     return total
 
 
+def _huber_of_squared(q, delta):
+    """fitting/loss.py:27-48: q if sqrt(q) <= delta else 2 delta sqrt(q) -
+    delta^2 (q a squared residual, or a mean of squared residuals)."""
+    r = np.sqrt(max(q, 0.0))
+    return q if r <= delta else 2.0 * delta * r - delta ** 2
+
+
+def generic_loss(params, structures, weights, loss_type="squared", nthreads=1):
+    """generic_loss_function (fitting/loss.py:166-248) for a flat Tersoff
+    parameter dict: per structure c_s^2 w_s [w_E rho((N dE)^2) + w_F rho(r0^2
+    mean dF^2) + w_S rho((V eps0)^2 mean dsigma^2)], rho the identity or the
+    Huber transform of each term, terms without a reference left out, V the
+    structure's "volume" when given; an evaluation that raises gives 1e30."""
+    w_E, w_F = weights.get("energy", 1.0), weights.get("forces", 1.0)
+    w_S = weights.get("stress", 0.1)
+    r0, eps0 = float(weights.get("r0", 1.0)), float(weights.get("eps0", 1.0))
+    rho = ((lambda q, key: _huber_of_squared(q, weights[key])) if loss_type == "huber"
+           else (lambda q, key: q))
+    total = 0.0
+    for s in structures:
+        N = s["natoms"]
+        vol = float(s.get("volume", abs(np.linalg.det(s["lattice"]))))
+        try:
+            e, f, st = tersoff_flat(s["lattice"], s["positions"], params, nthreads=nthreads)
+        except Exception:
+            return 1e30
+        part = w_E * rho((N * (e / N - s["ref_energy"])) ** 2, "huber_delta_energy")
+        if s.get("ref_forces") is not None:
+            part += w_F * rho(r0 ** 2 * float(np.mean((f - s["ref_forces"]) ** 2)),
+                              "huber_delta_forces")
+        if s.get("ref_stress") is not None:
+            part += w_S * rho((vol * eps0) ** 2 * float(np.mean((st - s["ref_stress"]) ** 2)),
+                              "huber_delta_stress")
+        total += float(s.get("weight", 1.0)) * float(s.get("scale", 1.0)) ** 2 * part
+    return total
+
+
+def generic_residuals(params, structures, weights, nthreads=1):
+    """generic_residuals (fitting/loss.py:263-319): per structure
+    sqrt(w_s) c_s [sqrt(w_E) N dE/N, sqrt(w_F / 3N) r0 dF (3N), sqrt(w_S / 9)
+    V eps0 dsigma (9)], the force and stress parts only with a reference."""
+    w_E, w_F = weights.get("energy", 1.0), weights.get("forces", 1.0)
+    w_S = weights.get("stress", 0.1)
+    r0, eps0 = float(weights.get("r0", 1.0)), float(weights.get("eps0", 1.0))
+    out = []
+    for s in structures:
+        N = s["natoms"]
+        vol = float(s.get("volume", abs(np.linalg.det(s["lattice"]))))
+        e, f, st = tersoff_flat(s["lattice"], s["positions"], params, nthreads=nthreads)
+        c = np.sqrt(float(s.get("weight", 1.0))) * float(s.get("scale", 1.0))
+        out.append([c * np.sqrt(w_E) * N * (e / N - s["ref_energy"])])
+        if s.get("ref_forces") is not None:
+            out.append(c * np.sqrt(w_F / (3.0 * N)) * r0 * (f - s["ref_forces"]).ravel())
+        if s.get("ref_stress") is not None:
+            out.append(c * np.sqrt(w_S / 9.0) * vol * eps0 * (st - s["ref_stress"]).ravel())
+    return np.concatenate(out) if out else np.zeros(0)
+
+
 # ---------------------------------------------------------------------------
 #  NVE velocity Verlet (solvers/molecular_dynamics.py:195-221, 1969-1996)
 # ---------------------------------------------------------------------------
