@@ -112,3 +112,71 @@ def test_heat_flux_64000_atoms():
     ref = ST.sw(lat, pos, t, O.sw_flat_as_records(dict(SW_SI)), ["Si"])
     J0 = ref.heat_flux(vel, m, abs(np.linalg.det(lat)))
     assert rel_err(J, J0) < 1e-10 and rel_err(e, ref.e_site) < 1e-10
+
+
+def test_tersoff_literal_cutoff_r27_d03_4096_atoms():
+    """The configuration's literal cutoff, R = 2.7, D = 0.3 (switch over
+    2.4-3.0 A, so rattled bonds fall inside it), against the oracle."""
+    lat, pos, t = cubic(DIAMOND, 5.431, 8, ["Si"] * 8, rattle=0.05, seed=0)
+    params = dict(TERSOFF_SI_T3, R=2.7, D=0.3)
+    pot = TersoffPotential(params, species=["Si"])
+    e, f, s = pot.compute(lat, pos, t)
+    e0, f0, s0 = O.tersoff_flat(lat, pos, params, nthreads=8)
+    _check(e, f, s, e0, f0, s0, "tersoff R 2.7 D 0.3")
+    ref = TersoffPotential(dict(TERSOFF_SI_T3), species=["Si"]).compute(lat, pos, t)[0]
+    assert abs(e - ref) > 1e-3 * abs(ref)  # a different potential from R 2.85, D 0.15
+
+
+def test_sw_nve_64000_atoms_captured_steps_match_the_oracle():
+    """The SW NVE configuration (diamond 20^3, 1000 K from seed 0, dt 1 fs,
+    skin 0.3 A) for 30 hipGraph steps, which include list rebuilds, against
+    the oracle's velocity Verlet with the reference's rebuild rule
+    (solvers/molecular_dynamics.py:1969-1996)."""
+    from akitsushima_amd.md import masses_of, maxwell_boltzmann, nve_run
+
+    lat, pos, t = cubic(DIAMOND, 5.431, 20, ["Si"] * 8, rattle=0.0, seed=0)
+    pot = StillingerWeberPotential(dict(SW_SI), species=["Si"])
+    m = masses_of(t)
+    vel = maxwell_boltzmann(1000.0, m, 0)
+    n_steps = 30
+    res = nve_run(pot, lat, pos, vel, t, dt=1.0, n_steps=n_steps, skin_width=0.3, masses=m,
+                  use_graph=True)
+    force = lambda l, p, pr: O.sw_flat(l, p, dict(SW_SI), pairs=pr, nthreads=8)
+    ep, ek, p1, v1, nreb = O.nve_run(force, lat, pos, vel, m, 1.0, n_steps, pot.r_cut, 0.3)
+    assert nreb >= 1 and res["n_rebuilds"] == nreb
+    assert np.abs(res["e_pot"] - ep).max() < 1e-9 * np.abs(ep).max()
+    assert np.abs(res["e_kin"] - ek).max() < 1e-9 * np.abs(ek).max()
+    d = res["positions"] - p1
+    d -= np.round(d @ np.linalg.inv(lat)) @ lat
+    assert np.abs(d).max() < 1e-8
+
+
+def test_fit_gradient_on_16_structures_of_256_to_512_atoms():
+    """The fit configuration's structure sizes (rattle 0.15 A, strain +-3 %,
+    targets from parameters perturbed by +-5 %) on 16 structures: loss
+    against the oracle, all 14 components of dL/dtheta against the torch
+    autograd reference (fitting/loss.py:212-242)."""
+    import torch_tersoff as TT
+    from akitsushima_amd.fit import TersoffFit, theta_from_params
+
+    structs = diamond_batch(16, 256, 512, 0.03, 0.15, seed=31, rep_range=(2, 4))
+    theta = theta_from_params(dict(TERSOFF_SI_T3, lambda3=1.3), gamma=1.0)
+    pert = theta * (1 + np.random.default_rng(32).uniform(-0.05, 0.05, 14))
+    pert[12] = 3.0
+    recp = {("Si", "Si", "Si"): dict(zip(TT.KEYS, pert))}
+    data = []
+    for lat, pos, t in structs:
+        assert 256 <= len(pos) <= 512
+        e, f, s = O.tersoff_multi(lat, pos, t, recp, ["Si"], nthreads=8)
+        data.append({"lattice": lat, "positions": pos, "atom_types": t,
+                     "ref_energy": e / len(pos), "ref_forces": f, "ref_stress": s,
+                     "_topo": TT.topology(lat, pos, theta[10] + theta[11] + 0.3)})
+    L, g = TersoffFit(data).loss_and_grad(theta)
+    L0 = O.tersoff_loss(data, {("Si", "Si", "Si"): dict(zip(TT.KEYS, theta))}, ["Si"],
+                        nthreads=8)
+    assert abs(L - L0) <= 1e-10 * abs(L0)
+    Lt, gt = TT.loss_and_grad(theta, data)
+    scale = np.abs(gt).max()
+    for p in range(14):
+        assert abs(g[p] - gt[p]) <= 1e-9 * max(abs(gt[p]), 1e-6 * scale), (TT.KEYS[p], g[p],
+                                                                           gt[p])
