@@ -404,3 +404,33 @@ def test_green_kubo_decomposition_sums_to_the_total(tmp_path):
     scale = np.abs(out["total"]).max()
     assert max(out["sum_rule_residual"].values()) < 1e-10 * scale
     assert np.allclose(np.diag(out["total"]), np.diag(tc._kappa_run[-1]), rtol=1e-9)
+
+
+@pytest.mark.parametrize("every", [3, 0])
+def test_md_with_the_bond_pair_site_form_matches_the_oracle(every):
+    """Tersoff with the lane-per-(bond, partner) site form: a captured MD run
+    that records the flux every 3 steps alternates it (plain steps) with the
+    lane-per-bond form (recording steps, which need site energies); and the
+    same run with the flux off."""
+    make, terms, cell = CASES["tersoff_xy"]
+    pot = make()
+    pot.set_bond_tile(2)
+    lat, pos, t = cell()
+    m = np.full(len(pos), 28.086)
+    vel = maxwell_boltzmann(500.0, m, 6)
+    n_steps, skin = 40, 0.3
+    res = md_run(pot, lat, pos, vel, t, dt=1.0, n_steps=n_steps, skin_width=skin, masses=m,
+                 flux_every=every)
+    force = lambda l, p, pr: O.tersoff_multi(l, p, t, TERSOFF_XY, ["X", "Y"], pairs=pr)
+    V = abs(np.linalg.det(lat))
+    flux = (lambda p, v: terms(lat, p, t).heat_flux(v, m, V)) if every else None
+    ep, ek, ec, p1, v1, nreb, J = O.md_run(force, lat, pos, vel, m, 1.0, n_steps, pot.r_cut,
+                                           skin, flux=flux, every=every)
+    assert np.abs(res["e_pot"] - ep).max() < 1e-9 * np.abs(ep).max()
+    assert np.abs(res["e_kin"] - ek).max() < 1e-9 * np.abs(ek).max()
+    assert res["n_rebuilds"] == nreb
+    d = res["positions"] - p1
+    d -= np.round(d @ np.linalg.inv(lat)) @ lat
+    assert np.abs(d).max() < 1e-8
+    if every:
+        assert rel_err(res["flux"], J) < 1e-8
