@@ -92,8 +92,9 @@ bool lattice_allclose(const double* a, const double* b) {
 }
 
 void check_finite(const double* x, size_t n, const char* what) {
-  for (size_t k = 0; k < n; ++k)
-    AK_REQUIRE(std::isfinite(x[k]), AK_ERR_ARG, std::string(what) + " contains non-finite values");
+  bool ok = true;  // a branch-free pass (vectorised)
+  for (size_t k = 0; k < n; ++k) ok &= std::isfinite(x[k]);
+  AK_REQUIRE(ok, AK_ERR_ARG, std::string(what) + " contains non-finite values");
 }
 
 }  // namespace
@@ -171,12 +172,17 @@ static bool check_and_rebuild(ak_nlist& L, int n, const double* lattice, const d
 }
 
 static void stage_species(ak_potential* P, int n, const int32_t* species) {
+  // unchanged species (the common repeated call) need no copy at all
+  if (P->spc_valid && (int)P->last_spc.size() == n &&
+      (species ? std::equal(species, species + n, P->last_spc.begin())
+               : std::all_of(P->last_spc.begin(), P->last_spc.end(),
+                             [](int32_t v) { return v == 0; })))
+    return;
   std::vector<int32_t> spc(species ? species : nullptr, species ? species + n : nullptr);
   if (!species) spc.assign(n, 0);
   for (int a = 0; a < n; ++a)
     AK_REQUIRE(spc[a] >= 0 && spc[a] < P->pot.ns, AK_ERR_ARG,
                "species index out of range for this potential");
-  if (P->spc_valid && spc == P->last_spc) return;
   P->d_spc.ensure(sizeof(int32_t) * std::max(1, n));
   if (n > 0)
     AK_HIP(hipMemcpyAsync(P->d_spc.p, spc.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice,
@@ -213,6 +219,43 @@ static ak_nlist* select_list(ak_potential* P, ak_nlist* L, int n, const double* 
   }
   check_and_rebuild(*L, n, lattice, inv, P->d_pos.as<double>(), st);
   return L;
+}
+
+// The owned list's rebuild test without a separate round trip: the force
+// call runs on the current list right behind the displacement check, and
+// the largest displacement comes back with the results (P->d_out[10],
+// copied with the energy and stress).  Only when it shows
+// that the list needed a rebuild (then the results are discarded) does the
+// caller rebuild and run again; returns false in that case and when the list
+// must be built first (not yet built, new size, new skin), true when `run`
+// has produced the results (the stream synchronised).
+template <class Run>
+static bool owned_list_speculative(ak_potential* P, int n, const double* lattice,
+                                   const double* inv, hipStream_t st, Run&& run) {
+  ak_nlist& L = P->owned;
+  const CellInfo ci = cell_info(lattice);
+  const double skin = (P->pot.r_cut + 0.3 < ci.min_width / 2.0) ? 0.3 : 0.0;
+  if (skin != P->owned_skin || !L.nl.valid || L.nl.n_atoms != n) return false;
+  L.set.setup(1, std::vector<int64_t>{0, (int64_t)n}.data(), lattice, inv, L.nl.r_list, st);
+  const double* Lb = L.nl.built_geoms[0].lat;
+  const double* Ib = L.nl.built_geoms[0].inv;
+  double sig = 1.0;
+  Mat3 M;
+  matmul3(Ib, lattice, M.m);
+  if (!lattice_allclose(lattice, Lb)) sig = sigma_min3(M.m);
+  nl_check_displacement(L.nl, L.set, P->d_pos.as<double>(), M, sig, nullptr,
+                        P->d_out.as<double>() + 10, st);
+  const double& dmax_out = run(&L);  // filled by the asynchronous copy
+  AK_HIP(hipStreamSynchronize(st));
+  const double dmax = dmax_out;
+  L.n_steps++;
+  L.nl.n_checks++;
+  if (sig * L.nl.r_list - 2.0 * dmax <= L.nl.r_cut) {
+    nl_build(L.nl, L.set, P->d_pos.as<double>(), st);  // stale list: rebuild, run again
+    run(&L);
+    AK_HIP(hipStreamSynchronize(st));
+  }
+  return true;
 }
 
 // Minimum-image bond vectors of every slot below cnt (3 planes n x stride).
@@ -526,19 +569,26 @@ int ak_compute(ak_potential* P, ak_nlist* nl, int32_t n, const double* lattice,
     stage_species(P, n, species);
     P->d_pos.ensure(sizeof(double) * 3 * std::max(1, n));
     P->d_F.ensure(sizeof(double) * 3 * std::max(1, n));
-    P->d_out.ensure(sizeof(double) * 10);
+    P->d_out.ensure(sizeof(double) * 11);
     if (n > 0)
       AK_HIP(hipMemcpyAsync(P->d_pos.p, positions, sizeof(double) * 3 * n,
                             hipMemcpyHostToDevice, st));
-    ak_nlist* L = select_list(P, nl, n, lattice, inv, st);
-    compute_forces(P->pot, L->nl, L->set, P->d_pos.as<double>(), P->d_spc.as<int>(),
-                   P->d_F.as<double>(), P->d_out.as<double>(), st);
-    double out[10];
-    AK_HIP(hipMemcpyAsync(out, P->d_out.p, sizeof(out), hipMemcpyDeviceToHost, st));
-    if (n > 0)
-      AK_HIP(hipMemcpyAsync(forces, P->d_F.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
-                            st));
-    AK_HIP(hipStreamSynchronize(st));
+    // out: energy, stress, and the owned list's largest displacement (valid
+    // once the stream has been synchronised; run returns a reference to it)
+    double out[11] = {};
+    auto run = [&](ak_nlist* Lst) -> const double& {
+      compute_forces(P->pot, Lst->nl, Lst->set, P->d_pos.as<double>(), P->d_spc.as<int>(),
+                     P->d_F.as<double>(), P->d_out.as<double>(), st);
+      AK_HIP(hipMemcpyAsync(out, P->d_out.p, sizeof(out), hipMemcpyDeviceToHost, st));
+      if (n > 0)
+        AK_HIP(hipMemcpyAsync(forces, P->d_F.p, sizeof(double) * 3 * n,
+                              hipMemcpyDeviceToHost, st));
+      return out[10];
+    };
+    if (nl || !owned_list_speculative(P, n, lattice, inv, st, run)) {
+      run(select_list(P, nl, n, lattice, inv, st));
+      AK_HIP(hipStreamSynchronize(st));
+    }
     *energy = out[0];
     if (stress && need_stress) std::memcpy(stress, out + 1, 9 * sizeof(double));
   });

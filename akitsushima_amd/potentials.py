@@ -129,9 +129,25 @@ class _GpuPotential:
         spc = np.concatenate([self.species_ids(t) for _, _, t in structures]).astype(np.int32)
         return off, lats, invs, pos, spc
 
+    def _inverse(self, lat):
+        """numpy's inverse of the lattice (the minimum image uses it), kept
+        for the last lattice seen: repeated calls on one cell skip it."""
+        key = lat.tobytes()
+        cached = getattr(self, "_inv_cache", None)
+        if cached is None or cached[0] != key:
+            inv = np.ascontiguousarray(np.linalg.inv(lat))
+            cached = (key, inv, inv.ctypes.data_as(_lib._pf64))
+            self._inv_cache = cached
+        return cached[1], cached[2]
+
     def species_ids(self, atom_types):
         if len(self.species) == 1:
-            return np.zeros(len(atom_types), np.int32)
+            z = getattr(self, "_zeros", None)
+            if z is None or len(z) != len(atom_types):
+                z = np.zeros(len(atom_types), np.int32)
+                z.setflags(write=False)
+                self._zeros = z
+            return z
         idx = {s: k for k, s in enumerate(self.species)}
         try:
             return np.array([idx[t] for t in atom_types], np.int32)
@@ -151,9 +167,9 @@ class _GpuPotential:
             raise ValueError(f"positions must have shape (N, 3), got {pos.shape}")
         if len(atom_types) != n:
             raise ValueError("atom_types and positions differ in length")
-        inv, pinv = f64(np.linalg.inv(lat))
+        inv, pinv = self._inverse(lat)
         spc, pspc = i32(self.species_ids(atom_types))
-        forces = np.zeros((n, 3))
+        forces = np.empty((n, 3))  # every row is written
         stress = np.zeros((3, 3))
         e = ctypes.c_double(0.0)
         nl = neigh_list.handle() if isinstance(neigh_list, NeighbourList) else None
