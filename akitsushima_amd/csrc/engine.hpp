@@ -162,6 +162,9 @@ struct Potential {
   int grid_cap = 0;
   // fitting session (fit.hip)
   std::shared_ptr<void> fit;
+  // list of the pairs inside r_cut alone, for force calls whose list rows
+  // are longer than a wavefront (compute_forces)
+  std::unique_ptr<NList> tight;
   // optional HIP-event timing of the dominant kernel (site kernel for
   // Tersoff/SW, force pass for EAM), recorded on the launch stream
   struct EventPair {

@@ -2118,6 +2118,25 @@ void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
                     const int* d_spc, double* d_F, double* d_out, hipStream_t st,
                     double* esite) {
   const int n = set.n_atoms;
+  if (pot.kind != KIND_EAM && nl.valid && nl.stride > kWave) {
+    // rows longer than a wavefront (a wide skin, a compressed cell): the bond
+    // kernels run on a list of the pairs inside r_cut alone, built from the
+    // same positions (the result depends only on those pairs).  Site
+    // gradients are defined on the caller's list, so they are refused.
+    AK_REQUIRE(!esite, AK_ERR_CAPACITY,
+               "site terms and heat fluxes need at most 64 neighbours within r_cut + skin");
+    AK_REQUIRE(nl.n_atoms == n, AK_ERR_STATE, "neighbour list was built for another size");
+    if (!pot.tight) pot.tight = std::make_unique<NList>();
+    NList& t = *pot.tight;
+    t.r_cut = t.r_list = pot.r_cut;
+    t.skin = 0.0;
+    t.valid = false;
+    nl_build(t, set, d_pos, st);
+    AK_REQUIRE(t.stride <= kWave, AK_ERR_CAPACITY,
+               "more than 64 neighbours within r_cut for a bond-order kernel");
+    compute_forces(pot, t, set, d_pos, d_spc, d_F, d_out, st, nullptr);
+    return;
+  }
   const ForcePlan pl = force_plan(pot, nl, esite != nullptr);
   AK_REQUIRE(nl.n_atoms == n, AK_ERR_STATE, "neighbour list was built for another size");
   SetDev::Chunks& ch = set.chunks(pl.chunk, st);

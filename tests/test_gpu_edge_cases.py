@@ -96,3 +96,50 @@ def test_bad_input_raises_the_reference_exception_types():
     flat = np.array([[5.0, 0.0, 0.0], [0.0, 5.0, 0.0], [5.0, 5.0, 0.0]])
     with pytest.raises(ValueError):
         pot.compute(flat, SPARSE, t)
+
+
+def _compressed(n=5, factor=0.85, rattle=0.03, seed=11):
+    from akitsushima_amd.structures import DIAMOND, cubic
+
+    return cubic(DIAMOND, 5.431 * factor, n, ["Si"] * 8, rattle=rattle, seed=seed)
+
+
+@pytest.mark.parametrize("name", ["tersoff", "sw"])
+@pytest.mark.parametrize("skin", [1.0, 3.0])
+def test_compressed_cell_with_a_wide_skin_matches_the_oracle(name, skin):
+    """15 % compressed diamond Si with a 1.0 A skin (rows of up to 64 slots)
+    and a 3.0 A skin (rows of 128 slots, longer than a wavefront, which the
+    bond kernels evaluate on a list of the pairs inside r_cut)."""
+    from akitsushima_amd import NeighbourList
+
+    from _cases import rel_err
+
+    lat, pos, t = _compressed()
+    pot = (TersoffPotential(TERSOFF_SI, species=["Si"]) if name == "tersoff"
+           else StillingerWeberPotential(SW_SI, species=["Si"]))
+    nl = NeighbourList(pot.r_cut, skin_width=skin)
+    nl.build(lat, pos)
+    print(f"compressed {name} skin {skin}: {len(pos)} atoms", flush=True)
+    e, f, s = pot.compute(lat, pos, t, neigh_list=nl)
+    e0, f0, s0 = (O.tersoff_flat(lat, pos, TERSOFF_SI) if name == "tersoff"
+                  else O.sw_flat(lat, pos, SW_SI))
+    assert abs(e - e0) <= 1e-10 * abs(e0)
+    assert rel_err(f, f0) < 1e-10 and rel_err(s, s0) < 1e-10
+    if skin > 2.0:
+        with pytest.raises(Exception, match="64 neighbours"):
+            pot.site_terms(lat, pos, t, neigh_list=nl)
+
+
+def test_compressed_cell_md_with_a_wide_skin_matches_the_oracle():
+    from akitsushima_amd.md import maxwell_boltzmann, nve_run
+
+    lat, pos, t = _compressed()
+    m = np.full(len(pos), 28.086)
+    vel = maxwell_boltzmann(600.0, m, 2)
+    pot = StillingerWeberPotential(SW_SI, species=["Si"])
+    res = nve_run(pot, lat, pos, vel, t, dt=1.0, n_steps=20, skin_width=1.0, masses=m)
+    force = lambda l, p, pr: O.sw_flat(l, p, SW_SI, pairs=pr)
+    ep, ek, p1, v1, nreb = O.nve_run(force, lat, pos, vel, m, 1.0, 20, pot.r_cut, 1.0)
+    assert np.abs(res["e_pot"] - ep).max() < 1e-9 * np.abs(ep).max()
+    assert np.abs(res["e_kin"] - ek).max() < 1e-9 * np.abs(ek).max()
+    assert res["n_rebuilds"] == nreb
