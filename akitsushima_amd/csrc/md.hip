@@ -1783,26 +1783,33 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
   };
   auto t0 = std::chrono::steady_clock::now();
   if (use_graph && n_steps > 0) {
-    // one captured step of each kind that occurs
-    hipGraph_t graph[2] = {nullptr, nullptr};
-    hipGraphExec_t exec[2] = {nullptr, nullptr};
-    bool need[2] = {false, false};
-    for (int s = 0; s < n_steps && !(need[0] && need[1]); ++s) need[records(s) ? 1 : 0] = true;
-    for (int k = 0; k < 2; ++k) {
+    // one captured step of each kind that occurs; without flux records, a
+    // graph of kBurst steps carries the run and single steps the remainder
+    constexpr int kBurst = 8;
+    const bool burst = S.every == 0 && n_steps >= kBurst;
+    hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
+    bool need[3] = {false, false, burst};
+    const int singles_from = burst ? n_steps - n_steps % kBurst : 0;
+    for (int s = singles_from; s < n_steps && !(need[0] && need[1]); ++s)
+      need[records(s) ? 1 : 0] = true;
+    for (int k = 0; k < 3; ++k) {
       if (!need[k]) continue;
       AK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      md_step(pot, nl, set, S, n, dt, st, k == 1);
+      for (int r = 0; r < (k == 2 ? kBurst : 1); ++r) md_step(pot, nl, set, S, n, dt, st, k == 1);
       AK_HIP(hipStreamEndCapture(st, &graph[k]));
       AK_HIP(hipGraphInstantiate(&exec[k], graph[k], nullptr, nullptr, 0));
     }
     AK_HIP(hipStreamSynchronize(st));
     t0 = std::chrono::steady_clock::now();
-    for (int s = 0; s < n_steps; ++s) AK_HIP(hipGraphLaunch(exec[records(s) ? 1 : 0], st));
+    for (int s = 0; s < singles_from; s += kBurst) AK_HIP(hipGraphLaunch(exec[2], st));
+    for (int s = singles_from; s < n_steps; ++s)
+      AK_HIP(hipGraphLaunch(exec[records(s) ? 1 : 0], st));
     if (S.thermo == 0) md_record(S, n, st);
     AK_HIP(hipStreamSynchronize(st));
     auto t1 = std::chrono::steady_clock::now();
     if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < 3; ++k) {
       if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
       if (graph[k]) (void)hipGraphDestroy(graph[k]);
     }

@@ -229,6 +229,10 @@ __device__ __forceinline__ uint64_t match_digit(unsigned d, bool valid) {
 }
 
 constexpr int RS_T = 256;
+// Blocks of the gated (captured MD) cell search and reverse-slot kernels,
+// which loop over their work: a step without a rebuild dispatches this many
+// blocks that return at once instead of one per cell or per 256 entries.
+constexpr int kGatedBlocks = 4096;
 
 // Digit histogram of one radix pass.  With pos != nullptr (first pass) the
 // keys are computed here -- the cell of each atom -- and stored with the atom
@@ -415,16 +419,12 @@ __device__ __forceinline__ int find_struct(const int* __restrict__ offs, int S, 
 }
 
 // One wavefront (one 64-thread block) per cell.
-__global__ __launch_bounds__(64) void k_nl_search(
-    int C, int S, const int* __restrict__ cell_offs, const Geom* __restrict__ geoms,
+__device__ __forceinline__ void search_cell(
+    int c, int S, const int* __restrict__ cell_offs, const Geom* __restrict__ geoms,
     const int* __restrict__ cell_start, const unsigned* __restrict__ cell_atoms,
     const double* __restrict__ pos, double rl2, int cap, int cmax, int* __restrict__ cnt,
-    int* __restrict__ nbr, int* __restrict__ stats, const int* gate) {
-  if (gated_off(gate)) return;
-  extern __shared__ unsigned cand[];  // cmax entries (power of two)
-  __shared__ int spre[28];
-  __shared__ int scell[27];
-  const int c = xcd_remap(blockIdx.x, gridDim.x);
+    int* __restrict__ nbr, int* __restrict__ stats, unsigned* __restrict__ cand,
+    int* __restrict__ spre, int* __restrict__ scell) {
   const int lane = threadIdx.x;
   const int s = find_struct(cell_offs, S, c);
   const Geom& g = geoms[s];
@@ -528,21 +528,37 @@ __global__ __launch_bounds__(64) void k_nl_search(
   }
 }
 
+// One wavefront per cell; a grid smaller than the cell count loops over the
+// cells (the captured MD step launches few blocks, so that a step without a
+// rebuild, whose blocks return at once, dispatches few of them).
+__global__ __launch_bounds__(64) void k_nl_search(
+    int C, int S, const int* __restrict__ cell_offs, const Geom* __restrict__ geoms,
+    const int* __restrict__ cell_start, const unsigned* __restrict__ cell_atoms,
+    const double* __restrict__ pos, double rl2, int cap, int cmax, int* __restrict__ cnt,
+    int* __restrict__ nbr, int* __restrict__ stats, const int* gate) {
+  if (gated_off(gate)) return;
+  extern __shared__ unsigned cand[];  // cmax entries (power of two)
+  __shared__ int spre[28];
+  __shared__ int scell[27];
+  for (int cc = blockIdx.x; cc < C; cc += gridDim.x) {
+    search_cell(xcd_remap(cc, C), S, cell_offs, geoms, cell_start, cell_atoms, pos, rl2, cap,
+                cmax, cnt, nbr, stats, cand, spre, scell);
+    __syncthreads();  // the shared arrays are filled again for the next cell
+  }
+}
+
 // Register-resident variant of k_nl_search for stencils of at most 64 * EPL
 // candidates: lane l holds candidates l, l + 64, ...; the bitonic network
 // runs on registers (shuffles across lanes, swaps within a lane), and each
 // candidate's position is loaded once per cell.  Same pairs, same order.
 template <int EPL>
-__global__ __launch_bounds__(64) void k_nl_search_reg(
-    int C, int S, const int* __restrict__ cell_offs, const Geom* __restrict__ geoms,
+__device__ __forceinline__ void search_cell_reg(
+    int c, int S, const int* __restrict__ cell_offs, const Geom* __restrict__ geoms,
     const int* __restrict__ cell_start, const unsigned* __restrict__ cell_atoms,
     const double* __restrict__ pos, double rl2, int cap, int* __restrict__ cnt,
-    int* __restrict__ nbr, int* __restrict__ stats, const int* gate) {
-  if (gated_off(gate)) return;
+    int* __restrict__ nbr, int* __restrict__ stats, int* __restrict__ spre,
+    int* __restrict__ scell) {
   constexpr int NP = 64 * EPL;
-  __shared__ int spre[28];
-  __shared__ int scell[27];
-  const int c = xcd_remap(blockIdx.x, gridDim.x);  // neighbouring cells on one XCD
   const int lane = threadIdx.x;
   const int s = find_struct(cell_offs, S, c);
   const Geom& g = geoms[s];
@@ -673,6 +689,23 @@ __global__ __launch_bounds__(64) void k_nl_search_reg(
   }
 }
 
+template <int EPL>
+__global__ __launch_bounds__(64) void k_nl_search_reg(
+    int C, int S, const int* __restrict__ cell_offs, const Geom* __restrict__ geoms,
+    const int* __restrict__ cell_start, const unsigned* __restrict__ cell_atoms,
+    const double* __restrict__ pos, double rl2, int cap, int* __restrict__ cnt,
+    int* __restrict__ nbr, int* __restrict__ stats, const int* gate) {
+  if (gated_off(gate)) return;
+  __shared__ int spre[28];
+  __shared__ int scell[27];
+  // neighbouring cells on one XCD; a smaller grid loops over the cells
+  for (int cc = blockIdx.x; cc < C; cc += gridDim.x) {
+    search_cell_reg<EPL>(xcd_remap(cc, C), S, cell_offs, geoms, cell_start, cell_atoms, pos, rl2,
+                         cap, cnt, nbr, stats, spre, scell);
+    __syncthreads();  // the shared arrays are filled again for the next cell
+  }
+}
+
 // stats[0] = max row length over atoms (one block).
 __global__ __launch_bounds__(1024) void k_nl_maxcnt(int n, const int* __restrict__ cnt,
                                                     int* __restrict__ stats, const int* gate) {
@@ -699,13 +732,9 @@ __global__ void k_nl_repack(int n, const int* __restrict__ cnt, const int* __res
 
 // Also copies the build positions (copy_src -> copy_dst, 3n doubles) when
 // given: the reference positions of the next displacement test.
-__global__ void k_nl_rev(int n, const int* __restrict__ cnt, const int* __restrict__ nbr,
-                         int stride, int* __restrict__ rev, int* stats, const int* gate,
-                         const double* __restrict__ copy_src, double* __restrict__ copy_dst) {
-  if (gated_off(gate)) return;
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (copy_src && t < 3 * (int64_t)n) copy_dst[t] = copy_src[t];
-  if (t >= (int64_t)n * stride) return;
+__device__ __forceinline__ void rev_entry(int64_t t, const int* __restrict__ cnt,
+                                          const int* __restrict__ nbr, int stride,
+                                          int* __restrict__ rev, int* stats) {
   int a = (int)(t / stride), k = (int)(t % stride);
   if (k >= cnt[a]) {
     rev[t] = -1;
@@ -726,6 +755,21 @@ __global__ void k_nl_rev(int n, const int* __restrict__ cnt, const int* __restri
   } else {
     rev[t] = -1;
     stats[3] = 1;  // asymmetric list: should never happen
+  }
+}
+
+// Entries t of the grid-stride loop (a grid smaller than n * stride / 256
+// loops); with copy_src, also copy_dst[t] = copy_src[t] for t < 3 n.
+__global__ void k_nl_rev(int n, const int* __restrict__ cnt, const int* __restrict__ nbr,
+                         int stride, int* __restrict__ rev, int* stats, const int* gate,
+                         const double* __restrict__ copy_src, double* __restrict__ copy_dst) {
+  if (gated_off(gate)) return;
+  const int64_t total = (int64_t)n * stride;
+  const int64_t end = copy_src ? max(total, 3 * (int64_t)n) : total;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < end;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    if (copy_src && t < 3 * (int64_t)n) copy_dst[t] = copy_src[t];
+    if (t < total) rev_entry(t, cnt, nbr, stride, rev, stats);
   }
 }
 
@@ -862,26 +906,28 @@ static void launch_search(NList& nl, SetDev& set, const double* d_pos, int* d_cn
   const unsigned* ca = nl.vals_a.as<unsigned>();
   int* stt = nl.stats.as<int>();
   const int S = set.n_struct;
+  // the captured (gated) search runs on a capped grid that loops over cells
+  const unsigned G = gate ? (unsigned)std::min(C, kGatedBlocks) : (unsigned)C;
   switch (nl.cmax) {
     case 64:
-      hipLaunchKernelGGL(k_nl_search_reg<1>, dim3(C), dim3(64), 0, st, C, S, co, ge, cs, ca,
+      hipLaunchKernelGGL(k_nl_search_reg<1>, dim3(G), dim3(64), 0, st, C, S, co, ge, cs, ca,
                          d_pos, rl2, cap, d_cnt, d_nbr, stt, gate);
       return;
     case 128:
-      hipLaunchKernelGGL(k_nl_search_reg<2>, dim3(C), dim3(64), 0, st, C, S, co, ge, cs, ca,
+      hipLaunchKernelGGL(k_nl_search_reg<2>, dim3(G), dim3(64), 0, st, C, S, co, ge, cs, ca,
                          d_pos, rl2, cap, d_cnt, d_nbr, stt, gate);
       return;
     case 256:
-      hipLaunchKernelGGL(k_nl_search_reg<4>, dim3(C), dim3(64), 0, st, C, S, co, ge, cs, ca,
+      hipLaunchKernelGGL(k_nl_search_reg<4>, dim3(G), dim3(64), 0, st, C, S, co, ge, cs, ca,
                          d_pos, rl2, cap, d_cnt, d_nbr, stt, gate);
       return;
     case 512:
-      hipLaunchKernelGGL(k_nl_search_reg<8>, dim3(C), dim3(64), 0, st, C, S, co, ge, cs, ca,
+      hipLaunchKernelGGL(k_nl_search_reg<8>, dim3(G), dim3(64), 0, st, C, S, co, ge, cs, ca,
                          d_pos, rl2, cap, d_cnt, d_nbr, stt, gate);
       return;
     default: {
       size_t lds = sizeof(unsigned) * (size_t)nl.cmax;
-      hipLaunchKernelGGL(k_nl_search, dim3(C), dim3(64), lds, st, C, S, co, ge, cs, ca, d_pos,
+      hipLaunchKernelGGL(k_nl_search, dim3(G), dim3(64), lds, st, C, S, co, ge, cs, ca, d_pos,
                          rl2, cap, nl.cmax, d_cnt, d_nbr, stt, gate);
     }
   }
@@ -978,8 +1024,9 @@ void nl_build_gated(NList& nl, SetDev& set, const double* d_pos, const int* d_fl
   sort_cells(nl, set, d_pos, st, d_flag);
   // search straight into the final layout (cap = stride)
   launch_search(nl, set, d_pos, nl.cnt.as<int>(), nl.nbr.as<int>(), nl.stride, st, d_flag);
-  int64_t tot = (int64_t)n * nl.stride;
-  hipLaunchKernelGGL(k_nl_rev, dim3(nblk(tot, 256)), dim3(256), 0, st, n, nl.cnt.as<int>(),
+  int64_t tot = std::max((int64_t)n * nl.stride, 3 * (int64_t)n);
+  const unsigned grid = (unsigned)std::min<int64_t>(nblk(tot, 256), kGatedBlocks);
+  hipLaunchKernelGGL(k_nl_rev, dim3(grid), dim3(256), 0, st, n, nl.cnt.as<int>(),
                      nl.nbr.as<int>(), nl.stride, nl.rev.as<int>(), nl.stats.as<int>(), d_flag,
                      d_pos, nl.ref_pos.as<double>());
 }
