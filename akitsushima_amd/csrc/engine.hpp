@@ -106,13 +106,18 @@ struct NList {
   int maxcnt = 0;   // largest row
   int cap = 0;      // search capacity per atom
   int cmax = 0;     // candidate capacity per cell
+  int max_cand = 0; // most candidates around one cell at the last full build
   int n_atoms = 0;
   int n_builds = 0, n_checks = 0;
   bool valid = false;
   bool fixed_stride = false;  // graph mode: stride never changes
   std::vector<Geom> built_geoms;
   DevBuf cnt, nbr, rev, ref_pos;
-  DevBuf keys_a, vals_a, keys_b, vals_b, hist, cell_start, tmp_nbr, stats, half_off;
+  // cell sort: cell and rank per atom, counts / block prefixes / block sums
+  // per cell, the atoms grouped by cell and each cell's first position
+  DevBuf cell_key, cell_rank, cell_count, cell_pre, cell_blk, cell_atoms, cell_start;
+  DevBuf cell_cand;  // candidates around each cell (full builds)
+  DevBuf tmp_nbr, stats, half_off;
   DevBuf disp_part, flag;
 };
 

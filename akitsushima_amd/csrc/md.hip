@@ -1740,10 +1740,11 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
   AK_HIP(hipMemsetAsync(S.counters.p, 0, sizeof(int) * 4, st));
   AK_HIP(hipMemsetAsync(S.flag.p, 0, sizeof(int), st));
 
-  // initial list with head room for the captured rebuilds: rows up to twice
-  // the current longest, candidate buffers twice the current need
+  // initial list with head room for the captured rebuilds: rows of one and a
+  // half times the current longest (at least four more), candidate buffers of
+  // at least twice the densest stencil
   nl_build(nl, set, S.pos.as<double>(), st);
-  nl.cmax = std::max(nl.cmax, 2 * nl.cmax);
+  nl.cmax = std::max(nl.cmax, pow2ceil(std::max(64, 2 * nl.max_cand)));
   int stride = pow2ceil(std::max(4, nl.maxcnt + std::max(4, nl.maxcnt / 2)));
   AK_REQUIRE(stride <= 64 || pot.kind == KIND_EAM, AK_ERR_CAPACITY,
              "too many neighbours within r_cut + skin for a captured MD step");

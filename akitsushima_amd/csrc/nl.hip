@@ -216,62 +216,117 @@ __device__ __forceinline__ unsigned cell_of(int a, const double* __restrict__ po
   return (unsigned)(g.cell_off + (c[0] * g.nc[1] + c[1]) * g.nc[2] + c[2]);
 }
 
-// Lanes of this wavefront holding the same 8-bit digit as the calling lane.
-__device__ __forceinline__ uint64_t match_digit(unsigned d, bool valid) {
-  uint64_t m = __ballot(valid);
-#pragma unroll
-  for (int b = 0; b < 8; ++b) {
-    int bit = (d >> b) & 1;
-    uint64_t bb = __ballot(bit);
-    m &= bit ? bb : ~bb;
-  }
-  return m;
-}
-
-constexpr int RS_T = 256;
 // Blocks of the gated (captured MD) cell search and reverse-slot kernels,
 // which loop over their work: a step without a rebuild dispatches this many
 // blocks that return at once instead of one per cell or per 256 entries.
 constexpr int kGatedBlocks = 4096;
 
-// Digit histogram of one radix pass.  With pos != nullptr (first pass) the
-// keys are computed here -- the cell of each atom -- and stored with the atom
-// index as the value.
-__global__ __launch_bounds__(RS_T) void k_rs_hist(int n, unsigned* __restrict__ keys,
-                                                  int shift, int chunk,
-                                                  unsigned* __restrict__ hist, int B,
-                                                  const int* gate, const double* __restrict__ pos,
-                                                  const int* __restrict__ sid,
-                                                  const Geom* __restrict__ geoms,
-                                                  unsigned* __restrict__ vals) {
+// Counting sort of the atoms by cell, in three kernels: k_cell_count (cell
+// and arrival rank of every atom, counts by atomic add), k_cell_scan
+// (exclusive prefix of the counts inside blocks of CS_SCAN cells) and
+// k_cell_place (block offsets, cell_start and the atom lists).  Atoms inside
+// a cell end in arrival order, which varies from run to run; the search
+// sorts each stencil's candidates by atom index and writes every atom's row
+// on its own, so the list does not depend on that order.
+constexpr int CS_T = 256;      // threads per block of the atom kernels
+constexpr int CS_SCAN = 1024;  // cells per scan block
+// block sums that k_cell_place scans in LDS itself; more take a k_scan_excl
+constexpr int CS_PLACE_LDS = 4096;
+
+// key[a] = cell of atom a, rank[a] = its slot among the atoms of that cell;
+// count[] is zero on entry (k_cell_scan clears it again).
+__global__ __launch_bounds__(CS_T) void k_cell_count(int n, const double* __restrict__ pos,
+                                                     const int* __restrict__ sid,
+                                                     const Geom* __restrict__ geoms,
+                                                     unsigned* __restrict__ key,
+                                                     unsigned* __restrict__ rank,
+                                                     unsigned* __restrict__ count,
+                                                     const int* gate) {
   if (gated_off(gate)) return;
-  __shared__ unsigned cnt[RS_T / 64][256];
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  for (int i = tid; i < (RS_T / 64) * 256; i += RS_T) (&cnt[0][0])[i] = 0;
-  __syncthreads();
-  const int base = blockIdx.x * chunk;
-  for (int t = 0; t < chunk; t += RS_T) {
-    int idx = base + t + tid;
-    bool v = idx < n;
-    unsigned key = 0u;
-    if (v) {
-      if (pos) {
-        key = cell_of(idx, pos, sid, geoms);
-        keys[idx] = key;
-        vals[idx] = (unsigned)idx;
-      } else {
-        key = keys[idx];
-      }
-    }
-    unsigned d = (key >> shift) & 255u;
-    uint64_t m = match_digit(d, v);
-    if (v && lane == __ffsll((unsigned long long)m) - 1) cnt[wave][d] += __popcll(m);
+  for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < n; a += gridDim.x * blockDim.x) {
+    const unsigned c = cell_of(a, pos, sid, geoms);
+    key[a] = c;
+    rank[a] = atomicAdd(&count[c], 1u);
   }
+}
+
+// pre[c] = sum of count[c'] for c' in [CS_SCAN * b, c), b = c / CS_SCAN, for
+// c in [0, C] (count[C] taken as 0); blk[b] = the block's total.  Clears
+// count[] for the next sort.
+__global__ __launch_bounds__(CS_SCAN) void k_cell_scan(int C, unsigned* __restrict__ count,
+                                                       unsigned* __restrict__ pre,
+                                                       unsigned* __restrict__ blk,
+                                                       const int* gate) {
+  if (gated_off(gate)) return;
+  __shared__ unsigned wsum[CS_SCAN / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int c = blockIdx.x * CS_SCAN + tid;
+  unsigned v = 0u;
+  if (c < C) {
+    v = count[c];
+    count[c] = 0u;
+  }
+  unsigned inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += y;
+  }
+  if (lane == 63) wsum[wid] = inc;
   __syncthreads();
-  for (int d = tid; d < 256; d += RS_T) {
-    unsigned s = 0;
-    for (int w = 0; w < RS_T / 64; ++w) s += cnt[w][d];
-    hist[(size_t)d * B + blockIdx.x] = s;
+  unsigned before = 0u;
+  for (int w = 0; w < wid; ++w) before += wsum[w];
+  if (c <= C) pre[c] = before + inc - v;
+  if (tid == CS_SCAN - 1) blk[blockIdx.x] = before + inc;
+}
+
+// cell_start[c] = pre[c] + (offset of block c / CS_SCAN) for c in [0, C],
+// and cell_atoms[cell_start[key[a]] + rank[a]] = a.  The nb block sums are
+// scanned here in LDS (dynamic, nb entries), or, with scanned != 0, blk
+// already holds their exclusive prefix.
+__global__ __launch_bounds__(CS_T) void k_cell_place(int n, int C, const unsigned* __restrict__ key,
+                                                     const unsigned* __restrict__ rank,
+                                                     const unsigned* __restrict__ pre,
+                                                     const unsigned* __restrict__ blk, int nb,
+                                                     int scanned, int* __restrict__ cell_start,
+                                                     unsigned* __restrict__ cell_atoms,
+                                                     const int* gate) {
+  if (gated_off(gate)) return;
+  extern __shared__ unsigned boff[];
+  __shared__ unsigned wsum[CS_T / 64];
+  __shared__ unsigned carry;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  if (!scanned) {
+    if (tid == 0) carry = 0u;
+    __syncthreads();
+    for (int b0 = 0; b0 < nb; b0 += CS_T) {
+      const int b = b0 + tid;
+      const unsigned v = b < nb ? blk[b] : 0u;
+      unsigned inc = v;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const unsigned y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+      }
+      if (lane == 63) wsum[wid] = inc;
+      __syncthreads();
+      unsigned before = carry;
+      for (int w = 0; w < wid; ++w) before += wsum[w];
+      if (b < nb) boff[b] = before + inc - v;
+      __syncthreads();
+      if (tid == CS_T - 1) carry = before + inc;
+      __syncthreads();
+    }
+  }
+  const unsigned* off = scanned ? blk : boff;
+  const int64_t end = max((int64_t)n, (int64_t)C + 1);
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + tid; t < end;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    if (t <= C) cell_start[t] = (int)(pre[t] + off[t / CS_SCAN]);
+    if (t < n) {
+      const unsigned c = key[t];
+      cell_atoms[pre[c] + off[c / CS_SCAN] + rank[t]] = (unsigned)t;
+    }
   }
 }
 
@@ -319,93 +374,6 @@ __global__ __launch_bounds__(1024) void k_scan_excl(unsigned* __restrict__ data,
   if (total && tid == 1023) *total = before + inc;
 }
 
-// Stable scatter of one radix pass.  Each block derives its own output
-// offsets from the digit histograms (hist[d * B + b]): thread d sums digit
-// d over all blocks and over the blocks before this one, and an exclusive
-// scan of the digit totals over the 256 threads gives the digit bases -- no
-// separate scan kernel.
-__global__ __launch_bounds__(RS_T) void k_rs_scatter(int n, const unsigned* __restrict__ kin,
-                                                     const unsigned* __restrict__ vin,
-                                                     unsigned* __restrict__ kout,
-                                                     unsigned* __restrict__ vout, int shift,
-                                                     int chunk, const unsigned* __restrict__ hist,
-                                                     int B, const int* gate) {
-  static_assert(RS_T == 256, "one thread per digit");
-  if (gated_off(gate)) return;
-  __shared__ unsigned run[256];
-  __shared__ unsigned wcnt[RS_T / 64][256];
-  __shared__ unsigned wtot[RS_T / 64];
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  {
-    const unsigned* row = hist + (size_t)tid * B;
-    unsigned tot = 0, pre = 0;
-    for (int b0 = 0; b0 < B; b0 += 4) {
-      unsigned h[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) h[q] = b0 + q < B ? row[b0 + q] : 0u;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        tot += h[q];
-        if (b0 + q < (int)blockIdx.x) pre += h[q];
-      }
-    }
-    unsigned inc = tot;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      unsigned y = __shfl_up(inc, o, 64);
-      if (lane >= o) inc += y;
-    }
-    if (lane == 63) wtot[wave] = inc;
-    __syncthreads();
-    unsigned before = 0;
-    for (int w = 0; w < wave; ++w) before += wtot[w];
-    run[tid] = before + inc - tot + pre;  // first output position of digit tid
-  }
-  const int base = blockIdx.x * chunk;
-  for (int t = 0; t < chunk; t += RS_T) {
-    for (int i = tid; i < (RS_T / 64) * 256; i += RS_T) (&wcnt[0][0])[i] = 0;
-    __syncthreads();
-    int idx = base + t + tid;
-    bool v = idx < n;
-    unsigned key = v ? kin[idx] : 0u;
-    unsigned d = (key >> shift) & 255u;
-    uint64_t m = match_digit(d, v);
-    int rank = __popcll(m & lanemask_lt());
-    if (v && lane == __ffsll((unsigned long long)m) - 1) wcnt[wave][d] = __popcll(m);
-    __syncthreads();
-    if (v) {
-      unsigned p = run[d] + rank;
-      for (int w = 0; w < wave; ++w) p += wcnt[w][d];
-      kout[p] = key;
-      vout[p] = vin[idx];
-    }
-    __syncthreads();
-    for (int dd = tid; dd < 256; dd += RS_T) {
-      unsigned s = 0;
-      for (int w = 0; w < RS_T / 64; ++w) s += wcnt[w][dd];
-      run[dd] += s;
-    }
-    __syncthreads();
-  }
-}
-
-// cell_start[c] = first sorted position with key >= c, for c in [0, C].
-__global__ void k_cell_start(int n, int C, const unsigned* __restrict__ skey,
-                             int* __restrict__ cell_start, const int* gate) {
-  if (gated_off(gate)) return;
-  int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n == 0) {
-    for (int c = idx; c <= C; c += gridDim.x * blockDim.x) cell_start[c] = 0;
-    return;
-  }
-  if (idx >= n) return;
-  int k = (int)skey[idx];
-  int prev = idx > 0 ? (int)skey[idx - 1] : -1;
-  for (int c = prev + 1; c <= k; ++c) cell_start[c] = idx;
-  if (idx == n - 1)
-    for (int c = k + 1; c <= C; ++c) cell_start[c] = n;
-}
-
 __device__ __forceinline__ int find_struct(const int* __restrict__ offs, int S, int c) {
   int lo = 0, hi = S;  // offs[lo] <= c < offs[hi]
   while (hi - lo > 1) {
@@ -423,8 +391,8 @@ __device__ __forceinline__ void search_cell(
     int c, int S, const int* __restrict__ cell_offs, const Geom* __restrict__ geoms,
     const int* __restrict__ cell_start, const unsigned* __restrict__ cell_atoms,
     const double* __restrict__ pos, double rl2, int cap, int cmax, int* __restrict__ cnt,
-    int* __restrict__ nbr, int* __restrict__ stats, unsigned* __restrict__ cand,
-    int* __restrict__ spre, int* __restrict__ scell) {
+    int* __restrict__ nbr, int* __restrict__ stats, int* __restrict__ cand_n,
+    unsigned* __restrict__ cand, int* __restrict__ spre, int* __restrict__ scell) {
   const int lane = threadIdx.x;
   const int s = find_struct(cell_offs, S, c);
   const Geom& g = geoms[s];
@@ -457,6 +425,7 @@ __device__ __forceinline__ void search_cell(
   }
   if (lane == 0) spre[0] = 0;
   const int first = cell_start[c], last = cell_start[c + 1];
+  if (cand_n && lane == 0) cand_n[c] = ncand;
   if (ncand > cmax) {
     if (lane == 0) stats[1] = max(stats[1], ncand);  // candidate overflow
     for (int t = first + lane; t < last; t += 64) cnt[cell_atoms[t]] = 0;
@@ -535,14 +504,14 @@ __global__ __launch_bounds__(64) void k_nl_search(
     int C, int S, const int* __restrict__ cell_offs, const Geom* __restrict__ geoms,
     const int* __restrict__ cell_start, const unsigned* __restrict__ cell_atoms,
     const double* __restrict__ pos, double rl2, int cap, int cmax, int* __restrict__ cnt,
-    int* __restrict__ nbr, int* __restrict__ stats, const int* gate) {
+    int* __restrict__ nbr, int* __restrict__ stats, int* __restrict__ cand_n, const int* gate) {
   if (gated_off(gate)) return;
   extern __shared__ unsigned cand[];  // cmax entries (power of two)
   __shared__ int spre[28];
   __shared__ int scell[27];
   for (int cc = blockIdx.x; cc < C; cc += gridDim.x) {
     search_cell(xcd_remap(cc, C), S, cell_offs, geoms, cell_start, cell_atoms, pos, rl2, cap,
-                cmax, cnt, nbr, stats, cand, spre, scell);
+                cmax, cnt, nbr, stats, cand_n, cand, spre, scell);
     __syncthreads();  // the shared arrays are filled again for the next cell
   }
 }
@@ -556,8 +525,8 @@ __device__ __forceinline__ void search_cell_reg(
     int c, int S, const int* __restrict__ cell_offs, const Geom* __restrict__ geoms,
     const int* __restrict__ cell_start, const unsigned* __restrict__ cell_atoms,
     const double* __restrict__ pos, double rl2, int cap, int* __restrict__ cnt,
-    int* __restrict__ nbr, int* __restrict__ stats, int* __restrict__ spre,
-    int* __restrict__ scell) {
+    int* __restrict__ nbr, int* __restrict__ stats, int* __restrict__ cand_n,
+    int* __restrict__ spre, int* __restrict__ scell) {
   constexpr int NP = 64 * EPL;
   const int lane = threadIdx.x;
   const int s = find_struct(cell_offs, S, c);
@@ -590,6 +559,7 @@ __device__ __forceinline__ void search_cell_reg(
   }
   if (lane == 0) spre[0] = 0;
   const int first = cell_start[c], last = cell_start[c + 1];
+  if (cand_n && lane == 0) cand_n[c] = ncand;
   if (ncand > NP) {
     if (lane == 0) stats[1] = max(stats[1], ncand);  // candidate overflow
     for (int t = first + lane; t < last; t += 64) cnt[cell_atoms[t]] = 0;
@@ -694,32 +664,49 @@ __global__ __launch_bounds__(64) void k_nl_search_reg(
     int C, int S, const int* __restrict__ cell_offs, const Geom* __restrict__ geoms,
     const int* __restrict__ cell_start, const unsigned* __restrict__ cell_atoms,
     const double* __restrict__ pos, double rl2, int cap, int* __restrict__ cnt,
-    int* __restrict__ nbr, int* __restrict__ stats, const int* gate) {
+    int* __restrict__ nbr, int* __restrict__ stats, int* __restrict__ cand_n, const int* gate) {
   if (gated_off(gate)) return;
   __shared__ int spre[28];
   __shared__ int scell[27];
   // neighbouring cells on one XCD; a smaller grid loops over the cells
   for (int cc = blockIdx.x; cc < C; cc += gridDim.x) {
     search_cell_reg<EPL>(xcd_remap(cc, C), S, cell_offs, geoms, cell_start, cell_atoms, pos, rl2,
-                         cap, cnt, nbr, stats, spre, scell);
+                         cap, cnt, nbr, stats, cand_n, spre, scell);
     __syncthreads();  // the shared arrays are filled again for the next cell
   }
 }
 
-// stats[0] = max row length over atoms (one block).
-__global__ __launch_bounds__(1024) void k_nl_maxcnt(int n, const int* __restrict__ cnt,
+// stats[0] = max row length over the n atoms and, with cand_n, stats[4] =
+// the most candidates around one of the C cells: block maxima, one atomic
+// per block.
+__global__ __launch_bounds__(1024) void k_nl_maxcnt(int n, const int* __restrict__ cnt, int C,
+                                                    const int* __restrict__ cand_n,
                                                     int* __restrict__ stats, const int* gate) {
   if (gated_off(gate)) return;
-  __shared__ int m[1024];
-  int v = 0;
-  for (int a = threadIdx.x; a < n; a += 1024) v = max(v, cnt[a]);
-  m[threadIdx.x] = v;
-  __syncthreads();
-  for (int o = 512; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) m[threadIdx.x] = max(m[threadIdx.x], m[threadIdx.x + o]);
-    __syncthreads();
+  __shared__ int m[2][16];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  int v = 0, w = 0;
+  for (int a = blockIdx.x * 1024 + tid; a < n; a += gridDim.x * 1024) v = max(v, cnt[a]);
+  if (cand_n)
+    for (int c = blockIdx.x * 1024 + tid; c < C; c += gridDim.x * 1024) w = max(w, cand_n[c]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    v = max(v, __shfl_xor(v, o, 64));
+    w = max(w, __shfl_xor(w, o, 64));
   }
-  if (threadIdx.x == 0) stats[0] = max(stats[0], m[0]);
+  if (lane == 0) {
+    m[0][wid] = v;
+    m[1][wid] = w;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int k = 1; k < 16; ++k) {
+      v = max(v, m[0][k]);
+      w = max(w, m[1][k]);
+    }
+    atomicMax(&stats[0], v);
+    if (cand_n) atomicMax(&stats[4], w);
+  }
 }
 
 __global__ void k_nl_repack(int n, const int* __restrict__ cnt, const int* __restrict__ src,
@@ -732,26 +719,37 @@ __global__ void k_nl_repack(int n, const int* __restrict__ cnt, const int* __res
 
 // Also copies the build positions (copy_src -> copy_dst, 3n doubles) when
 // given: the reference positions of the next displacement test.
+// Reverse slot of entry t = (a, k): the position of a in the row of its
+// neighbour j.  Rows are sorted ascending, so that position is the number of
+// entries of row j below a; the row is read 16 bytes at a time (the stride
+// is a power of two of at least four).  A row whose search overflowed its
+// capacity is read up to the stride only.
 __device__ __forceinline__ void rev_entry(int64_t t, const int* __restrict__ cnt,
                                           const int* __restrict__ nbr, int stride,
                                           int* __restrict__ rev, int* stats) {
-  int a = (int)(t / stride), k = (int)(t % stride);
+  const int a = (int)(t / stride), k = (int)(t % stride);
   if (k >= cnt[a]) {
     rev[t] = -1;
     return;
   }
-  int j = nbr[t];
-  const int* row = nbr + (size_t)j * stride;
-  int lo = 0, hi = cnt[j];
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (row[mid] < a)
-      lo = mid + 1;
-    else
-      hi = mid;
+  const int j = nbr[t];
+  const int cj = min(cnt[j], stride);
+  const int4* row = reinterpret_cast<const int4*>(nbr + (size_t)j * stride);
+  int below = 0;
+  bool found = false;
+#pragma unroll 4
+  for (int m = 0; m < cj; m += 4) {
+    const int4 q = row[m >> 2];
+    const int v[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const bool in = m + e < cj;
+      below += (in && v[e] < a) ? 1 : 0;
+      found |= in && v[e] == a;
+    }
   }
-  if (lo < cnt[j] && row[lo] == a) {
-    rev[t] = j * stride + lo;
+  if (found) {
+    rev[t] = j * stride + below;
   } else {
     rev[t] = -1;
     stats[3] = 1;  // asymmetric list: should never happen
@@ -859,41 +857,41 @@ static void sort_cells(NList& nl, SetDev& set, const double* d_pos, hipStream_t 
                        const int* gate) {
   const int n = set.n_atoms;
   const int C = set.n_cells;
-  nl.keys_a.ensure(sizeof(unsigned) * (n + 1));
-  nl.vals_a.ensure(sizeof(unsigned) * (n + 1));
-  nl.keys_b.ensure(sizeof(unsigned) * (n + 1));
-  nl.vals_b.ensure(sizeof(unsigned) * (n + 1));
+  const int nb = (int)nblk((int64_t)C + 1, CS_SCAN);
+  nl.cell_key.ensure(sizeof(unsigned) * (n + 1));
+  nl.cell_rank.ensure(sizeof(unsigned) * (n + 1));
+  nl.cell_atoms.ensure(sizeof(unsigned) * (n + 1));
+  nl.cell_pre.ensure(sizeof(unsigned) * (C + 2));
+  nl.cell_blk.ensure(sizeof(unsigned) * (nb + 1));
   nl.cell_start.ensure(sizeof(int) * (C + 2));
-  int bits = 0;
-  while ((1LL << bits) < (long long)C) ++bits;
-  int passes = std::max(1, (bits + 7) / 8);
-  // about 64 blocks of whole 256-atom tiles: few enough that every scatter
-  // block can sum the digit histograms itself
-  int tpb = std::max(1, (n + 256 * 64 - 1) / (256 * 64));
-  int chunk = 256 * tpb;
-  int B = std::max(1, (n + chunk - 1) / chunk);
-  nl.hist.ensure(sizeof(unsigned) * 256 * (size_t)B);
-  unsigned *ka = nl.keys_a.as<unsigned>(), *va = nl.vals_a.as<unsigned>();
-  unsigned *kb = nl.keys_b.as<unsigned>(), *vb = nl.vals_b.as<unsigned>();
-  for (int p = 0; p < passes && n > 0; ++p) {
-    int shift = 8 * p;
-    hipLaunchKernelGGL(k_rs_hist, dim3(B), dim3(RS_T), 0, st, n, ka, shift, chunk,
-                       nl.hist.as<unsigned>(), B, gate, p == 0 ? d_pos : nullptr,
-                       (const int*)set.d_sid.as<int>(), (const Geom*)set.d_geoms.as<Geom>(), va);
-    hipLaunchKernelGGL(k_rs_scatter, dim3(B), dim3(RS_T), 0, st, n, ka, va, kb, vb, shift,
-                       chunk, nl.hist.as<unsigned>(), B, gate);
-    std::swap(ka, kb);
-    std::swap(va, vb);
+  // the counts are zero between sorts (k_cell_scan clears them); a sort
+  // outside a captured step clears them here as well
+  const bool fresh = nl.cell_count.bytes < sizeof(unsigned) * (C + 1);
+  nl.cell_count.ensure(sizeof(unsigned) * (C + 1));
+  if (!gate || fresh) {
+    AK_REQUIRE(!gate || !fresh, AK_ERR_STATE, "captured cell sort without prepared counts");
+    AK_HIP(hipMemsetAsync(nl.cell_count.p, 0, sizeof(unsigned) * (C + 1), st));
   }
-  // sorted keys/values end in ka/va; remember which buffer holds them
-  if (ka != nl.keys_a.as<unsigned>()) {
-    std::swap(nl.keys_a.p, nl.keys_b.p);
-    std::swap(nl.keys_a.bytes, nl.keys_b.bytes);
-    std::swap(nl.vals_a.p, nl.vals_b.p);
-    std::swap(nl.vals_a.bytes, nl.vals_b.bytes);
-  }
-  hipLaunchKernelGGL(k_cell_start, dim3(std::max(1u, nblk(n, 256))), dim3(256), 0, st, n, C,
-                     nl.keys_a.as<unsigned>(), nl.cell_start.as<int>(), gate);
+  unsigned* key = nl.cell_key.as<unsigned>();
+  unsigned* rank = nl.cell_rank.as<unsigned>();
+  unsigned* blk = nl.cell_blk.as<unsigned>();
+  const unsigned cap = gate ? (unsigned)kGatedBlocks : 65535u;
+  if (n > 0)
+    hipLaunchKernelGGL(k_cell_count, dim3(std::min(nblk(n, CS_T), cap)), dim3(CS_T), 0, st, n,
+                       d_pos, (const int*)set.d_sid.as<int>(), (const Geom*)set.d_geoms.as<Geom>(),
+                       key, rank, nl.cell_count.as<unsigned>(), gate);
+  hipLaunchKernelGGL(k_cell_scan, dim3(nb), dim3(CS_SCAN), 0, st, C,
+                     nl.cell_count.as<unsigned>(), nl.cell_pre.as<unsigned>(), blk, gate);
+  const int scanned = nb > CS_PLACE_LDS ? 1 : 0;
+  if (scanned)
+    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, blk, nb, (unsigned*)nullptr,
+                       gate);
+  const size_t lds = scanned ? 0 : sizeof(unsigned) * (size_t)nb;
+  const int64_t items = std::max((int64_t)n, (int64_t)C + 1);
+  hipLaunchKernelGGL(k_cell_place, dim3(std::min(nblk(items, CS_T), cap)), dim3(CS_T), lds, st,
+                     n, C, (const unsigned*)key, (const unsigned*)rank,
+                     (const unsigned*)nl.cell_pre.as<unsigned>(), (const unsigned*)blk, nb,
+                     scanned, nl.cell_start.as<int>(), nl.cell_atoms.as<unsigned>(), gate);
 }
 
 static void launch_search(NList& nl, SetDev& set, const double* d_pos, int* d_cnt, int* d_nbr,
@@ -903,32 +901,38 @@ static void launch_search(NList& nl, SetDev& set, const double* d_pos, int* d_cn
   const int* co = set.d_cell_offs.as<int>();
   const Geom* ge = set.d_geoms.as<Geom>();
   const int* cs = nl.cell_start.as<int>();
-  const unsigned* ca = nl.vals_a.as<unsigned>();
+  const unsigned* ca = nl.cell_atoms.as<unsigned>();
   int* stt = nl.stats.as<int>();
+  // candidates per cell, kept by full builds (a captured step reads none)
+  int* cn = nullptr;
+  if (!gate) {
+    nl.cell_cand.ensure(sizeof(int) * (size_t)std::max(1, C));
+    cn = nl.cell_cand.as<int>();
+  }
   const int S = set.n_struct;
   // the captured (gated) search runs on a capped grid that loops over cells
   const unsigned G = gate ? (unsigned)std::min(C, kGatedBlocks) : (unsigned)C;
   switch (nl.cmax) {
     case 64:
       hipLaunchKernelGGL(k_nl_search_reg<1>, dim3(G), dim3(64), 0, st, C, S, co, ge, cs, ca,
-                         d_pos, rl2, cap, d_cnt, d_nbr, stt, gate);
+                         d_pos, rl2, cap, d_cnt, d_nbr, stt, cn, gate);
       return;
     case 128:
       hipLaunchKernelGGL(k_nl_search_reg<2>, dim3(G), dim3(64), 0, st, C, S, co, ge, cs, ca,
-                         d_pos, rl2, cap, d_cnt, d_nbr, stt, gate);
+                         d_pos, rl2, cap, d_cnt, d_nbr, stt, cn, gate);
       return;
     case 256:
       hipLaunchKernelGGL(k_nl_search_reg<4>, dim3(G), dim3(64), 0, st, C, S, co, ge, cs, ca,
-                         d_pos, rl2, cap, d_cnt, d_nbr, stt, gate);
+                         d_pos, rl2, cap, d_cnt, d_nbr, stt, cn, gate);
       return;
     case 512:
       hipLaunchKernelGGL(k_nl_search_reg<8>, dim3(G), dim3(64), 0, st, C, S, co, ge, cs, ca,
-                         d_pos, rl2, cap, d_cnt, d_nbr, stt, gate);
+                         d_pos, rl2, cap, d_cnt, d_nbr, stt, cn, gate);
       return;
     default: {
       size_t lds = sizeof(unsigned) * (size_t)nl.cmax;
       hipLaunchKernelGGL(k_nl_search, dim3(G), dim3(64), lds, st, C, S, co, ge, cs, ca, d_pos,
-                         rl2, cap, nl.cmax, d_cnt, d_nbr, stt, gate);
+                         rl2, cap, nl.cmax, d_cnt, d_nbr, stt, cn, gate);
     }
   }
 }
@@ -969,8 +973,11 @@ void nl_build(NList& nl, SetDev& set, const double* d_pos, hipStream_t st) {
     if (n > 0) {
       launch_search(nl, set, d_pos, nl.cnt.as<int>(), nl.tmp_nbr.as<int>(), nl.cap, st,
                     nullptr);
-      hipLaunchKernelGGL(k_nl_maxcnt, dim3(1), dim3(1024), 0, st, n, nl.cnt.as<int>(),
-                         nl.stats.as<int>(), (const int*)nullptr);
+      const int C = set.n_cells;
+      const unsigned g = std::min(64u, nblk(std::max(n, C), 1024));
+      hipLaunchKernelGGL(k_nl_maxcnt, dim3(g), dim3(1024), 0, st, n, nl.cnt.as<int>(), C,
+                         (const int*)nl.cell_cand.as<int>(), nl.stats.as<int>(),
+                         (const int*)nullptr);
     }
     AK_HIP(hipMemcpyAsync(h_stats, nl.stats.p, sizeof(int) * 8, hipMemcpyDeviceToHost, st));
     AK_HIP(hipStreamSynchronize(st));
@@ -990,6 +997,7 @@ void nl_build(NList& nl, SetDev& set, const double* d_pos, hipStream_t st) {
     AK_REQUIRE(attempt < 7, AK_ERR_CAPACITY, "neighbour list capacity search did not converge");
   }
   nl.maxcnt = h_stats[0];
+  nl.max_cand = h_stats[4];
   int stride = std::max(4, pow2ceil(std::max(1, nl.maxcnt)));
   if (nl.fixed_stride && nl.stride > 0) {
     AK_REQUIRE(stride <= nl.stride, AK_ERR_CAPACITY,
