@@ -22,10 +22,45 @@ namespace ak {
 
 constexpr double kConv = kConvAmu;
 
+// x wrapped into the cell (frac = x @ inv; frac -= floor(frac);
+// frac[frac >= 1] = 0; x = frac @ lat) and stored as atom a's position;
+// returns the squared minimum-image displacement from ref.
+__device__ __forceinline__ double wrap_store(int a, const double (&x)[3], const Geom& g,
+                                             const double* __restrict__ ref,
+                                             double* __restrict__ pos) {
+  double f[3];
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    double s = x[0] * g.inv[b] + x[1] * g.inv[3 + b] + x[2] * g.inv[6 + b];
+    s -= floor(s);
+    f[b] = s >= 1.0 ? 0.0 : s;
+  }
+  double y[3];
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    y[b] = f[0] * g.lat[b] + f[1] * g.lat[3 + b] + f[2] * g.lat[6 + b];
+    pos[3 * a + b] = y[b];
+  }
+  double dx, dy, dz;
+  mic_exact(g.lat, g.inv, ref[3 * a], ref[3 * a + 1], ref[3 * a + 2], y[0], y[1], y[2], dx, dy,
+            dz);
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// part[blockIdx.x] = the block's largest d2 (m: 256 doubles of LDS).
+__device__ __forceinline__ void block_max_store(double d2, double* m, double* __restrict__ part) {
+  m[threadIdx.x] = d2;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) m[threadIdx.x] = fmax(m[threadIdx.x], m[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = m[0];
+}
+
 // Half kick, drift and wrap into the cell, and this block's largest squared
 // displacement since the list was built (minimum image of x - x_build):
-//   v += dt/2 a; x += dt v; frac = x @ inv; frac -= floor(frac);
-//   frac[frac >= 1] = 0; x = frac @ lat.
+//   v += dt/2 a; x += dt v; wrap (wrap_store).
 // vscale (nullable): the thermostat's velocity scale, applied first.
 // keep (nullable, 3N of 0 or 1): frozen components (fix_atoms) are zeroed
 // after the kick, as `velocities *= ~fix_mask` does.
@@ -50,31 +85,56 @@ __global__ __launch_bounds__(256) void k_md_drift(int n, double* __restrict__ po
       vel[3 * a + d] = v;
       x[d] = pos[3 * a + d] + dt * v;
     }
-    double f[3];
-#pragma unroll
-    for (int b = 0; b < 3; ++b) {
-      double s = x[0] * g.inv[b] + x[1] * g.inv[3 + b] + x[2] * g.inv[6 + b];
-      s -= floor(s);
-      f[b] = s >= 1.0 ? 0.0 : s;
-    }
-    double y[3];
-#pragma unroll
-    for (int b = 0; b < 3; ++b) {
-      y[b] = f[0] * g.lat[b] + f[1] * g.lat[3 + b] + f[2] * g.lat[6 + b];
-      pos[3 * a + b] = y[b];
-    }
-    double dx, dy, dz;
-    mic_exact(g.lat, g.inv, ref[3 * a], ref[3 * a + 1], ref[3 * a + 2], y[0], y[1], y[2], dx,
-              dy, dz);
-    d2 = dx * dx + dy * dy + dz * dz;
+    d2 = wrap_store(a, x, g, ref, pos);
   }
-  m[threadIdx.x] = d2;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) m[threadIdx.x] = fmax(m[threadIdx.x], m[threadIdx.x + o]);
-    __syncthreads();
+  block_max_store(d2, m, part);
+}
+
+// Plain NVE with the closing half kick of a step deferred to the next
+// step's first pass: with counters[0] > 0 (a step has run since the records
+// restarted) v += dt/2 a first, a = F conv / m; the kinetic-energy sums of
+// that state go to ke_part (one per block, as k_md_kick<1> writes them);
+// then the half kick and the drift of k_md_drift.  The same operations in
+// the same order as k_md_kick<1> followed by k_md_drift, so the trajectory
+// is bitwise the same; the run's last closing kick is a k_md_kick.
+__global__ __launch_bounds__(256) void k_md_kick_drift(
+    int n, double* __restrict__ pos, double* __restrict__ vel, const double* __restrict__ F,
+    const double* __restrict__ mass, double dt, Geom g, const double* __restrict__ ref,
+    double* __restrict__ part, double* __restrict__ ke_part, const int* __restrict__ counters,
+    const double* __restrict__ keep) {
+  __shared__ double m[256];
+  __shared__ double smem[16];
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool pending = counters[0] > 0;
+  double d2 = 0.0;
+  double ke[1] = {0.0};
+  if (a < n) {
+    const double ms = mass[a];
+    double ad[3], v[3], s = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      ad[d] = F[3 * a + d] * kConv / ms;
+      v[d] = vel[3 * a + d];
+      if (pending) {
+        v[d] += 0.5 * dt * ad[d];
+        if (keep) v[d] *= keep[3 * a + d];
+      }
+      s += v[d] * v[d];
+    }
+    ke[0] = 0.5 * ms * s / kConv;
+    double x[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double w = v[d] + 0.5 * dt * ad[d];
+      if (keep) w *= keep[3 * a + d];
+      vel[3 * a + d] = w;
+      x[d] = pos[3 * a + d] + dt * w;
+    }
+    d2 = wrap_store(a, x, g, ref, pos);
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = m[0];
+  block_sum<1>(ke, smem);
+  if (threadIdx.x == 0) ke_part[blockIdx.x] = ke[0];
+  block_max_store(d2, m, part);
 }
 
 // a = F conv / m; v += dt/2 a (kick != 0); per-block kinetic-energy sums
@@ -1274,6 +1334,9 @@ struct MDState {
   Geom* geom = nullptr;  // the set's device cell
   // velocity-scaling thermostats keep a pending scale in nhc[0]
   bool scaled() const { return thermo == 1 || thermo == 3 || thermo == 4; }
+  // plain NVE without flux or frame records: each step's closing half kick
+  // runs in the next step's drift (k_md_kick_drift)
+  bool lazy = false;
   bool flex() const { return thermo == 4 && cell != 0; }
 };
 
@@ -1379,6 +1442,12 @@ static void md_step(Potential& pot, NList& nl, SetDev& set, MDState& S, int n, d
                        (const Geom*)S.geom, (const double*)nl.ref_pos.as<double>(),
                        (const double*)S.nhc.as<double>(), (const double*)S.np.as<double>(),
                        nl.disp_part.as<double>(), S.keep_ptr());
+  } else if (S.lazy) {
+    hipLaunchKernelGGL(k_md_kick_drift, dim3(nb), dim3(256), 0, st, n, S.pos.as<double>(),
+                       S.vel.as<double>(), (const double*)S.F.as<double>(),
+                       (const double*)S.mass.as<double>(), dt, set.geoms[0],
+                       (const double*)nl.ref_pos.as<double>(), nl.disp_part.as<double>(),
+                       S.ke.as<double>(), (const int*)S.counters.as<int>(), S.keep_ptr());
   } else {
     hipLaunchKernelGGL(k_md_drift, dim3(nb), dim3(256), 0, st, n, S.pos.as<double>(),
                        S.vel.as<double>(), (const double*)S.acc.as<double>(), dt, set.geoms[0],
@@ -1411,7 +1480,7 @@ static void md_step(Potential& pot, NList& nl, SetDev& set, MDState& S, int n, d
   else
     compute_forces(pot, nl, set, S.pos.as<double>(), S.spc.as<int>(), S.F.as<double>(),
                    S.es.as<double>(), st);
-  kick(S, n, dt, 1, st);
+  if (!S.lazy) kick(S, n, dt, 1, st);
   if (S.thermo) thermo_kernel(S, S.thermo == 2 ? 1 : 2, dt, st, nb);
   // a recording step: the gate counts the records, so frame = counters[2]
   if (S.every > 0 && flux_now)
@@ -1438,6 +1507,13 @@ static void md_record(MDState& S, int n, hipStream_t st) {
                      (const double*)nullptr, 0.0, 0.0, (int*)nullptr, S.counters.as<int>(),
                      S.hist_ep.as<double>(), S.hist_ek.as<double>(), S.hist_ec.as<double>(), S.V,
                      S.hist_P.as<double>(), S.hist_nb.as<int>());
+}
+
+// NVE: the closing half kick still pending after the last step (deferred
+// kicks), then the record of the final state.
+static void md_finish(MDState& S, int n, double dt, int n_steps, hipStream_t st) {
+  if (S.lazy && n_steps > 0) kick(S, n, dt, 1, st);
+  md_record(S, n, st);
 }
 
 void md_run(Potential& pot, int n, const double* lattice, const double* inv, double* pos,
@@ -1525,6 +1601,7 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
     S.frames.ensure(sizeof(double) * 3 * (size_t)n * std::max(S.n_frames, 1));
     S.frame_lat.ensure(sizeof(double) * 9 * std::max(S.n_frames, 1));
   }
+  S.lazy = S.thermo == 0 && S.every == 0 && S.frame_every == 0;
   S.counters.ensure(sizeof(int) * 4);
   S.flag.ensure(sizeof(int));
   // records at the steps s < n_steps with (s + phase) % every == 0
@@ -1806,7 +1883,7 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
     for (int s = 0; s < singles_from; s += kBurst) AK_HIP(hipGraphLaunch(exec[2], st));
     for (int s = singles_from; s < n_steps; ++s)
       AK_HIP(hipGraphLaunch(exec[records(s) ? 1 : 0], st));
-    if (S.thermo == 0) md_record(S, n, st);
+    if (S.thermo == 0) md_finish(S, n, dt, n_steps, st);
     AK_HIP(hipStreamSynchronize(st));
     auto t1 = std::chrono::steady_clock::now();
     if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
@@ -1816,7 +1893,7 @@ void md_run(Potential& pot, int n, const double* lattice, const double* inv, dou
     }
   } else {
     for (int s = 0; s < n_steps; ++s) md_step(pot, nl, set, S, n, dt, st, records(s));
-    if (S.thermo == 0) md_record(S, n, st);
+    if (S.thermo == 0) md_finish(S, n, dt, n_steps, st);
     AK_HIP(hipStreamSynchronize(st));
     auto t1 = std::chrono::steady_clock::now();
     if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
