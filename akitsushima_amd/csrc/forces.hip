@@ -96,6 +96,49 @@ __device__ __forceinline__ double xpow(double x, double e) {
   return pow(x, e);
 }
 
+// x^e for the exponents of the Stillinger-Weber pair term, given xi = 1 / x:
+// the negative integer powers are products of xi, other exponents pow().
+__device__ __forceinline__ double xpow_r(double x, double xi, double e) {
+  if (e == 0.0) return 1.0;
+  if (e == -4.0) {
+    const double q = xi * xi;
+    return q * q;
+  }
+  if (e == -5.0) {
+    const double q = xi * xi;
+    return q * q * xi;
+  }
+  if (e == -1.0) return xi;
+  if (e == 4.0) {
+    const double q = x * x;
+    return q * q;
+  }
+  if (e == 1.0) return x;
+  return pow(x, e);
+}
+
+// Stillinger-Weber two-body term of one leg (x = r / sigma < a), with
+// rinv = 1 / r: e2 = V2 / 2, su = (dV2/dr) / (2 r), dlnE = d ln E / dr for
+// E = exp(gamma / (x - a)), where
+//   V2 = eps A (B x^-p - x^-q) exp(1 / (x - a)).
+// Two divisions (x and 1 / (x - a)); the powers of x use 1 / x = sigma / r.
+__device__ __forceinline__ void sw_leg(double r, double rinv, double eps, double sig,
+                                       double inv_sig, double Asw, double Bsw, double pp,
+                                       double qq, double asw, double gam, double& e2, double& su,
+                                       double& dlnE) {
+  const double x = r / sig;
+  const double xi = sig * rinv;
+  const double t = 1.0 / (x - asw);
+  dlnE = -gam * inv_sig * (t * t);
+  const double ex = nb_exp(t);
+  const double bx = Bsw * xpow_r(x, xi, -pp) - xpow_r(x, xi, -qq);
+  const double dV2 = eps * Asw * ex *
+                     ((-pp * Bsw * xpow_r(x, xi, -pp - 1.0) + qq * xpow_r(x, xi, -qq - 1.0)) +
+                      bx * (-t * t));
+  e2 = 0.5 * (eps * Asw * bx * ex);
+  su = 0.5 * (dV2 * inv_sig) * rinv;
+}
+
 // Block sums of 10 values (energy, virial) of launch chunk c: DPP wavefront
 // sums, one LDS exchange of the wavefront totals, combined in fixed order;
 // thread q < 10 writes value q to partial[q * n_chunks + c].  A chunk is the
@@ -882,17 +925,8 @@ __device__ __forceinline__ void k_sw_site_chunk(int c, int n_chunks,
   double E = 0.0, dlnE = 0.0;  // exp(gamma/(x - a)) and d ln / dr
   double e = 0.0, su = 0.0;
   if (leg) {
-    double dx_ = x - asw;
-    E = nb_exp(gam / dx_);
-    dlnE = -gam / (sig * dx_ * dx_);
-    double t = 1.0 / dx_;
-    double ex = nb_exp(t);
-    double bx = Bsw * xpow(x, -pp) - xpow(x, -qq);
-    double V2 = eps * Asw * bx * ex;
-    double dV2 = eps * Asw * ex *
-                 ((-pp * Bsw * xpow(x, -pp - 1.0) + qq * xpow(x, -qq - 1.0)) + bx * (-t * t));
-    e = 0.5 * V2;
-    su = 0.5 * (dV2 / sig) * rinv;
+    E = nb_exp(gam / (x - asw));
+    sw_leg(r, rinv, eps, sig, 1.0 / sig, Asw, Bsw, pp, qq, asw, gam, e, su, dlnE);
   }
   const double c3 = lam * eps;
   double swx = 0.0, swy = 0.0, swz = 0.0;
@@ -1104,8 +1138,8 @@ __device__ __forceinline__ void k_sw_site_c_chunk(
   double* lx = lds;
   double* ly = lx + apb * L;
   double* lz = ly + apb * L;
-  double* lr = lz + apb * L;
-  double* lE = lr + apb * L;
+  double* lri = lz + apb * L;  // 1 / r
+  double* lE = lri + apb * L;
   int* lslot = (int*)(lE + apb * L);
   const int reg = al * L;
   const int a = chunk_rng[2 * c] + al;
@@ -1115,6 +1149,7 @@ __device__ __forceinline__ void k_sw_site_c_chunk(
   const double eps = prm[0], sig = prm[1], Asw = prm[2], Bsw = prm[3], pp = prm[4],
                qq = prm[5], asw = prm[6], lam = prm[7], gam = prm[8], cth0 = prm[9];
   const double rc = asw * sig;
+  const double inv_sig = 1.0 / sig;
   const Geom& g = geoms[single ? 0 : sid[ac]];
 
   // phase 1: active legs, compacted in slot order
@@ -1139,7 +1174,7 @@ __device__ __forceinline__ void k_sw_site_c_chunk(
       lx[t] = ux;
       ly[t] = uy;
       lz[t] = uz;
-      lr[t] = r;
+      lri[t] = 1.0 / r;
       lE[t] = nb_exp(gam / (r / sig - asw));
       lslot[t] = k;
     }
@@ -1153,25 +1188,17 @@ __device__ __forceinline__ void k_sw_site_c_chunk(
   const double c3 = lam * eps;
   double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int t = q; t < count; t += LA) {
-    const double ux = lx[reg + t], uy = ly[reg + t], uz = lz[reg + t], r = lr[reg + t];
+    const double ux = lx[reg + t], uy = ly[reg + t], uz = lz[reg + t], rinv = lri[reg + t];
     const double E = lE[reg + t];
-    const double x = r / sig;
-    const double rinv = 1.0 / r;
-    const double dx_ = x - asw;
-    const double dlnE = -gam / (sig * dx_ * dx_);
-    const double tt = 1.0 / dx_;
-    const double ex = nb_exp(tt);
-    const double bx = Bsw * xpow(x, -pp) - xpow(x, -qq);
-    const double V2 = eps * Asw * bx * ex;
-    const double dV2 = eps * Asw * ex *
-                       ((-pp * Bsw * xpow(x, -pp - 1.0) + qq * xpow(x, -qq - 1.0)) + bx * (-tt * tt));
-    double e = 0.5 * V2;
-    double su = 0.5 * (dV2 / sig) * rinv;
+    // the distance of phase 1, recomputed with the same operations
+    const double r = sqrt(ux * ux + uy * uy + uz * uz);
+    double e, su, dlnE;
+    sw_leg(r, rinv, eps, sig, inv_sig, Asw, Bsw, pp, qq, asw, gam, e, su, dlnE);
     double swx = 0.0, swy = 0.0, swz = 0.0;
     for (int s2 = 0; s2 < count; ++s2) {
       if (s2 == t) continue;
       const double wx = lx[reg + s2], wy = ly[reg + s2], wz = lz[reg + s2];
-      const double wri = 1.0 / lr[reg + s2], wE = lE[reg + s2];
+      const double wri = lri[reg + s2], wE = lE[reg + s2];
       double cs = (ux * wx + uy * wy + uz * wz) * rinv * wri;
       double dc = cs - cth0;
       double pre = c3 * E * wE * dc;

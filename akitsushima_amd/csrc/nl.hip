@@ -221,10 +221,11 @@ __device__ __forceinline__ unsigned cell_of(int a, const double* __restrict__ po
 // blocks that return at once instead of one per cell or per 256 entries.
 constexpr int kGatedBlocks = 4096;
 
-// Counting sort of the atoms by cell, in three kernels: k_cell_count (cell
-// and arrival rank of every atom, counts by atomic add), k_cell_scan
-// (exclusive prefix of the counts inside blocks of CS_SCAN cells) and
-// k_cell_place (block offsets, cell_start and the atom lists).  Atoms inside
+// Counting sort of the atoms by cell: k_cell_count (cell and arrival rank of
+// every atom, counts by atomic add), then k_cell_scan_place when the counts
+// fit in one block's LDS, else k_cell_scan (exclusive prefix of the counts
+// inside blocks of CS_SCAN cells) and k_cell_place (block offsets,
+// cell_start and the atom lists).  Atoms inside
 // a cell end in arrival order, which varies from run to run; the search
 // sorts each stencil's candidates by atom index and writes every atom's row
 // on its own, so the list does not depend on that order.
@@ -327,6 +328,53 @@ __global__ __launch_bounds__(CS_T) void k_cell_place(int n, int C, const unsigne
       const unsigned c = key[t];
       cell_atoms[pre[c] + off[c / CS_SCAN] + rank[t]] = (unsigned)t;
     }
+  }
+}
+
+// k_cell_scan and k_cell_place in one pass, for cell counts that fit in LDS
+// (C + 1 entries, dynamic): every block copies the counts into LDS, scans
+// them, writes its share of cell_start (grid-stride) and places its atoms.
+// The counts stay as they are; k_nl_rev, which follows every sort, clears
+// them.
+__global__ __launch_bounds__(CS_T) void k_cell_scan_place(int n, int C,
+                                                          const unsigned* __restrict__ key,
+                                                          const unsigned* __restrict__ rank,
+                                                          const unsigned* __restrict__ count,
+                                                          int* __restrict__ cell_start,
+                                                          unsigned* __restrict__ cell_atoms,
+                                                          const int* gate) {
+  if (gated_off(gate)) return;
+  extern __shared__ unsigned pre[];
+  __shared__ unsigned wsum[CS_T / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int m = C + 1;
+  for (int i = tid; i < m; i += CS_T) pre[i] = i < C ? count[i] : 0u;
+  __syncthreads();
+  // thread tid owns the run [lo, hi); exclusive scan of the run sums
+  const int per = (m + CS_T - 1) / CS_T;
+  const int lo = min(m, tid * per), hi = min(m, lo + per);
+  unsigned s = 0u;
+  for (int i = lo; i < hi; ++i) s += pre[i];
+  unsigned inc = s;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += y;
+  }
+  if (lane == 63) wsum[wid] = inc;
+  __syncthreads();
+  unsigned run = inc - s;
+  for (int w = 0; w < wid; ++w) run += wsum[w];
+  for (int i = lo; i < hi; ++i) {
+    const unsigned v = pre[i];
+    pre[i] = run;
+    run += v;
+  }
+  __syncthreads();
+  const int64_t end = max((int64_t)n, (int64_t)m);
+  for (int64_t t = (int64_t)blockIdx.x * CS_T + tid; t < end; t += (int64_t)gridDim.x * CS_T) {
+    if (t < m) cell_start[t] = (int)pre[t];
+    if (t < n) cell_atoms[pre[key[t]] + rank[t]] = (unsigned)t;
   }
 }
 
@@ -757,17 +805,21 @@ __device__ __forceinline__ void rev_entry(int64_t t, const int* __restrict__ cnt
 }
 
 // Entries t of the grid-stride loop (a grid smaller than n * stride / 256
-// loops); with copy_src, also copy_dst[t] = copy_src[t] for t < 3 n.
+// loops); with copy_src, also copy_dst[t] = copy_src[t] for t < 3 n; and
+// clear[t] = 0 for t < n_clear (the cell counts of the sort before).
 __global__ void k_nl_rev(int n, const int* __restrict__ cnt, const int* __restrict__ nbr,
                          int stride, int* __restrict__ rev, int* stats, const int* gate,
-                         const double* __restrict__ copy_src, double* __restrict__ copy_dst) {
+                         const double* __restrict__ copy_src, double* __restrict__ copy_dst,
+                         unsigned* __restrict__ clear, int n_clear) {
   if (gated_off(gate)) return;
   const int64_t total = (int64_t)n * stride;
-  const int64_t end = copy_src ? max(total, 3 * (int64_t)n) : total;
+  int64_t end = copy_src ? max(total, 3 * (int64_t)n) : total;
+  end = max(end, (int64_t)n_clear);
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < end;
        t += (int64_t)gridDim.x * blockDim.x) {
     if (copy_src && t < 3 * (int64_t)n) copy_dst[t] = copy_src[t];
     if (t < total) rev_entry(t, cnt, nbr, stride, rev, stats);
+    if (t < n_clear) clear[t] = 0u;
   }
 }
 
@@ -853,6 +905,20 @@ __global__ void k_half_write(int n, const int* __restrict__ cnt, const int* __re
 
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// LDS a block may allocate on the current device (cached per device).
+static size_t lds_per_block() {
+  static int cached[64] = {0};
+  int dev = 0;
+  AK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return 64 * 1024;
+  if (cached[dev] == 0) {
+    int v = 0;
+    AK_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    cached[dev] = v > 0 ? v : 64 * 1024;
+  }
+  return (size_t)cached[dev];
+}
+
 static void sort_cells(NList& nl, SetDev& set, const double* d_pos, hipStream_t st,
                        const int* gate) {
   const int n = set.n_atoms;
@@ -864,8 +930,9 @@ static void sort_cells(NList& nl, SetDev& set, const double* d_pos, hipStream_t 
   nl.cell_pre.ensure(sizeof(unsigned) * (C + 2));
   nl.cell_blk.ensure(sizeof(unsigned) * (nb + 1));
   nl.cell_start.ensure(sizeof(int) * (C + 2));
-  // the counts are zero between sorts (k_cell_scan clears them); a sort
-  // outside a captured step clears them here as well
+  // the counts are zero between sorts (k_cell_scan clears them, or k_nl_rev
+  // after k_cell_scan_place); a sort outside a captured step clears them here
+  // as well
   const bool fresh = nl.cell_count.bytes < sizeof(unsigned) * (C + 1);
   nl.cell_count.ensure(sizeof(unsigned) * (C + 1));
   if (!gate || fresh) {
@@ -880,6 +947,16 @@ static void sort_cells(NList& nl, SetDev& set, const double* d_pos, hipStream_t 
     hipLaunchKernelGGL(k_cell_count, dim3(std::min(nblk(n, CS_T), cap)), dim3(CS_T), 0, st, n,
                        d_pos, (const int*)set.d_sid.as<int>(), (const Geom*)set.d_geoms.as<Geom>(),
                        key, rank, nl.cell_count.as<unsigned>(), gate);
+  const int64_t items = std::max((int64_t)n, (int64_t)C + 1);
+  const size_t lds_all = sizeof(unsigned) * ((size_t)C + 1);
+  if (lds_all + 1024 <= lds_per_block()) {
+    // one pass: every block scans the counts in LDS; k_nl_rev clears them
+    hipLaunchKernelGGL(k_cell_scan_place, dim3(std::min(nblk(items, CS_T), cap)), dim3(CS_T),
+                       lds_all, st, n, C, (const unsigned*)key, (const unsigned*)rank,
+                       (const unsigned*)nl.cell_count.as<unsigned>(), nl.cell_start.as<int>(),
+                       nl.cell_atoms.as<unsigned>(), gate);
+    return;
+  }
   hipLaunchKernelGGL(k_cell_scan, dim3(nb), dim3(CS_SCAN), 0, st, C,
                      nl.cell_count.as<unsigned>(), nl.cell_pre.as<unsigned>(), blk, gate);
   const int scanned = nb > CS_PLACE_LDS ? 1 : 0;
@@ -887,7 +964,6 @@ static void sort_cells(NList& nl, SetDev& set, const double* d_pos, hipStream_t 
     hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, blk, nb, (unsigned*)nullptr,
                        gate);
   const size_t lds = scanned ? 0 : sizeof(unsigned) * (size_t)nb;
-  const int64_t items = std::max((int64_t)n, (int64_t)C + 1);
   hipLaunchKernelGGL(k_cell_place, dim3(std::min(nblk(items, CS_T), cap)), dim3(CS_T), lds, st,
                      n, C, (const unsigned*)key, (const unsigned*)rank,
                      (const unsigned*)nl.cell_pre.as<unsigned>(), (const unsigned*)blk, nb,
@@ -1013,7 +1089,8 @@ void nl_build(NList& nl, SetDev& set, const double* d_pos, hipStream_t st) {
                        nl.tmp_nbr.as<int>(), nl.cap, nl.nbr.as<int>(), stride);
     hipLaunchKernelGGL(k_nl_rev, dim3(nblk(tot, 256)), dim3(256), 0, st, n, nl.cnt.as<int>(),
                        nl.nbr.as<int>(), stride, nl.rev.as<int>(), nl.stats.as<int>(),
-                       (const int*)nullptr, (const double*)nullptr, (double*)nullptr);
+                       (const int*)nullptr, (const double*)nullptr, (double*)nullptr,
+                       nl.cell_count.as<unsigned>(), set.n_cells + 1);
   }
   nl.ref_pos.ensure(sizeof(double) * 3 * (size_t)std::max(1, n));
   if (n > 0)
@@ -1032,11 +1109,12 @@ void nl_build_gated(NList& nl, SetDev& set, const double* d_pos, const int* d_fl
   sort_cells(nl, set, d_pos, st, d_flag);
   // search straight into the final layout (cap = stride)
   launch_search(nl, set, d_pos, nl.cnt.as<int>(), nl.nbr.as<int>(), nl.stride, st, d_flag);
-  int64_t tot = std::max((int64_t)n * nl.stride, 3 * (int64_t)n);
+  const int n_clear = set.n_cells + 1;
+  int64_t tot = std::max(std::max((int64_t)n * nl.stride, 3 * (int64_t)n), (int64_t)n_clear);
   const unsigned grid = (unsigned)std::min<int64_t>(nblk(tot, 256), kGatedBlocks);
   hipLaunchKernelGGL(k_nl_rev, dim3(grid), dim3(256), 0, st, n, nl.cnt.as<int>(),
                      nl.nbr.as<int>(), nl.stride, nl.rev.as<int>(), nl.stats.as<int>(), d_flag,
-                     d_pos, nl.ref_pos.as<double>());
+                     d_pos, nl.ref_pos.as<double>(), nl.cell_count.as<unsigned>(), n_clear);
 }
 
 void nl_check_displacement(NList& nl, SetDev& set, const double* d_pos, const Mat3& M,
