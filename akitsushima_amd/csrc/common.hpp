@@ -137,6 +137,52 @@ __device__ __forceinline__ double dpp_row(double x) {
   return __hiloint2double(hi, lo);
 }
 
+// Exchange of one recursive-halving step: the value of the partner lane
+// (row_half_mirror: lane i <-> 7 - i within 8; quad xor 1, xor 2; row_ror 8:
+// xor 8), all within a row of 16 lanes.
+template <int STEP>
+__device__ __forceinline__ double halving_partner(double x) {
+  if constexpr (STEP == 0) return dpp_row<0x141>(x);
+  else if constexpr (STEP == 1) return dpp_quad<0xB1>(x);
+  else if constexpr (STEP == 2) return dpp_quad<0x4E>(x);
+  else return dpp_row<0x128>(x);
+}
+
+// Keep half of the H + H values w (the upper half where `upper`) and add
+// the partner lane's copy of that half.
+template <int STEP, int H>
+__device__ __forceinline__ void halving_step(double (&w)[16], bool upper) {
+#pragma unroll
+  for (int i = 0; i < H; ++i) {
+    const double lo = halving_partner<STEP>(w[i]), hi = halving_partner<STEP>(w[H + i]);
+    w[i] = upper ? w[H + i] + hi : w[i] + lo;
+  }
+}
+
+// Totals over the 64 lanes of a wavefront of NV <= 16 values per lane, by
+// recursive halving: four exchange steps inside each row of 16 lanes (lane
+// bit 2, 0, 1, 3 selects the half a lane keeps) leave each lane one value,
+// two cross-row shuffles sum it over the rows.  Lane l then holds the total
+// of value idx = 8 b2 + 4 b0 + 2 b1 + b3 (b the bits of l); lanes 0-15 hold
+// the 16 indices once each.  Fixed order, so bitwise reproducible.
+template <int NV>
+__device__ __forceinline__ double wave_sums(const double (&v)[NV], int& idx) {
+  static_assert(NV >= 1 && NV <= 16, "at most 16 values");
+  const int lane = __lane_id();
+  double w[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) w[i] = i < NV ? v[i] : 0.0;
+  halving_step<0, 8>(w, (lane & 4) != 0);
+  halving_step<1, 4>(w, (lane & 1) != 0);
+  halving_step<2, 2>(w, (lane & 2) != 0);
+  halving_step<3, 1>(w, (lane & 8) != 0);
+  double t = w[0];
+  t += __shfl_xor(t, 16, kWave);
+  t += __shfl_xor(t, 32, kWave);
+  idx = ((lane & 4) << 1) | ((lane & 1) << 2) | (lane & 2) | ((lane >> 3) & 1);
+  return t;
+}
+
 // Sum over the 64 lanes of a wavefront, every lane receives it: DPP quad
 // butterflies, row rotations by 4 and 8, then two cross-row shuffles.
 // Fixed order, so bitwise reproducible.

@@ -139,20 +139,18 @@ __device__ __forceinline__ void sw_leg(double r, double rinv, double eps, double
   su = 0.5 * (dV2 * inv_sig) * rinv;
 }
 
-// Block sums of 10 values (energy, virial) of launch chunk c: DPP wavefront
-// sums, one LDS exchange of the wavefront totals, combined in fixed order;
+// Block sums of 10 values (energy, virial) of launch chunk c: wavefront
+// totals by recursive halving (wave_sums), one LDS exchange of them,
+// combined in fixed order;
 // thread q < 10 writes value q to partial[q * n_chunks + c].  A chunk is the
 // atoms of one block (of one structure).
 __device__ __forceinline__ void wave_partial(double (&v)[10], double* __restrict__ partial, int c,
                                              int n_chunks) {
   __shared__ double sm[10][4];
-#pragma unroll
-  for (int q = 0; q < 10; ++q) v[q] = wave_sum(v[q]);
+  int idx;
+  const double t = wave_sums<10>(v, idx);
   const int wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-    for (int q = 0; q < 10; ++q) sm[q][wid] = v[q];
-  }
+  if ((threadIdx.x & 63) < 16 && idx < 10) sm[idx][wid] = t;
   __syncthreads();
   if (threadIdx.x < 10) {
     const int q = threadIdx.x;
