@@ -122,10 +122,79 @@ struct NList {
 };
 
 void nl_build(NList& nl, SetDev& set, const double* d_pos, hipStream_t st);
+// The Verlet-skin test and the record of a captured MD step, for one block
+// of 256 threads (all of them call it).  With disp_part, the test
+// (engine/neighborlist.py:653-673, fixed cell): rebuild when
+// r_list - 2 sqrt(max disp_part) <= r_cut.  With `owner`, *flag receives the
+// result and counters[1] counts it, and with e_pot the state the previous
+// kick left is recorded at index counters[0], which advances: E_pot =
+// es[0], E_kin = the sum of the nb kinetic partials in fixed order, their
+// sum, the pressure from E_kin and the virial diagonal (es[1], es[5],
+// es[9]) and the rebuild count.  Returns the test's result (0 without
+// disp_part).
+struct SkinGate {
+  int nb = 0;
+  const double* ke_part = nullptr;
+  const double* es = nullptr;
+  const double* disp_part = nullptr;
+  double r_list = 0.0, r_cut = 0.0;
+  int* flag = nullptr;
+  int* counters = nullptr;
+  double* e_pot = nullptr;
+  double* e_kin = nullptr;
+  double* e_cons = nullptr;
+  double* h_press = nullptr;
+  int* h_nb = nullptr;
+  double V = 0.0;
+};
+
+__device__ __forceinline__ int skin_gate_eval(const SkinGate& g, bool owner) {
+  __shared__ double smem[16];
+  __shared__ double m[256];
+  __shared__ int s_fl;
+  double v[1] = {0.0};
+  double d = 0.0;
+  for (int b = threadIdx.x; b < g.nb; b += blockDim.x) {
+    if (owner) v[0] += g.ke_part[b];
+    if (g.disp_part) d = fmax(d, g.disp_part[b]);
+  }
+  block_sum<1>(v, smem);
+  m[threadIdx.x] = d;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) m[threadIdx.x] = fmax(m[threadIdx.x], m[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const int fl = (g.disp_part && g.r_list - 2.0 * sqrt(m[0]) <= g.r_cut) ? 1 : 0;
+    s_fl = fl;
+    if (owner) {
+      if (g.e_pot) {
+        const int step = g.counters[0];
+        g.e_pot[step] = g.es[0];
+        g.e_kin[step] = v[0];
+        g.e_cons[step] = g.es[0] + v[0];
+        g.h_press[step] = 2.0 * v[0] / (3.0 * g.V) + ((g.es[1] + g.es[5]) + g.es[9]) / 3.0;
+        g.h_nb[step] = g.counters[1];
+        g.counters[0] = step + 1;
+      }
+      if (g.disp_part) {
+        *g.flag = fl;
+        g.counters[1] += fl;
+      }
+    }
+  }
+  __syncthreads();
+  return s_fl;
+}
+
 // Device-gated rebuild for captured MD steps: runs the cell sort and search
-// only when *d_flag is nonzero; rows keep nl.stride.
+// only when *d_flag is nonzero; rows keep nl.stride.  With sg (and n > 0),
+// the skin test is evaluated by the sort's first kernel in every block
+// (block 0 owns the flag and the record) instead of read from *d_flag,
+// which the later kernels read.
 void nl_build_gated(NList& nl, SetDev& set, const double* d_pos, const int* d_flag,
-                    hipStream_t st);
+                    hipStream_t st, const SkinGate* sg = nullptr);
 // Largest minimum-image displacement since the last build -> *d_flag = 1 when
 // sigma_min * r_list - 2 d_max <= r_cut (fixed cell: sigma_min = 1, M = I);
 // d_flag and d_dmax may be null.

@@ -187,54 +187,8 @@ __global__ __launch_bounds__(256) void k_md_kick(int n, double* __restrict__ vel
   }
 }
 
-// One block.  With e_pot != nullptr, records the state the previous kick
-// left -- E_pot = es[0] and E_kin = sum of the kick's block sums in fixed
-// order -- at index counters[0] and advances it; with disp_part != nullptr,
-// also the Verlet-skin test (engine/neighborlist.py:653-673, fixed cell:
-// rebuild when r_list - 2 max|MIC(x - x_build)| <= r_cut) -> *flag, counted
-// in counters[1].
-__global__ __launch_bounds__(256) void k_md_flag(int nb, const double* __restrict__ ke_part,
-                                                 const double* __restrict__ es,
-                                                 const double* __restrict__ disp_part,
-                                                 double r_list, double r_cut,
-                                                 int* __restrict__ flag,
-                                                 int* __restrict__ counters,
-                                                 double* __restrict__ e_pot,
-                                                 double* __restrict__ e_kin,
-                                                 double* __restrict__ e_cons, double V,
-    double* __restrict__ h_press, int* __restrict__ h_nb) {
-  __shared__ double smem[16];
-  __shared__ double m[256];
-  double v[1] = {0.0};
-  double d = 0.0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-    v[0] += ke_part[b];
-    if (disp_part) d = fmax(d, disp_part[b]);
-  }
-  block_sum<1>(v, smem);
-  m[threadIdx.x] = d;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) m[threadIdx.x] = fmax(m[threadIdx.x], m[threadIdx.x + o]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    if (e_pot) {
-      int step = counters[0];
-      e_pot[step] = es[0];
-      e_kin[step] = v[0];
-      e_cons[step] = es[0] + v[0];
-      h_press[step] = 2.0 * v[0] / (3.0 * V) + ((es[1] + es[5]) + es[9]) / 3.0;
-      h_nb[step] = counters[1];
-      counters[0] = step + 1;
-    }
-    if (disp_part) {
-      const int fl = (r_list - 2.0 * sqrt(m[0]) <= r_cut) ? 1 : 0;
-      *flag = fl;
-      counters[1] += fl;
-    }
-  }
-}
+// One block of 256 threads: the skin test and the record (skin_gate_eval).
+__global__ __launch_bounds__(256) void k_md_flag(SkinGate g) { (void)skin_gate_eval(g, true); }
 
 // Nose-Hoover chain state on the device: nhc[0] = velocity scale of the last
 // end-of-step half-step not yet applied to the velocities, nhc[1] = the
@@ -1405,6 +1359,31 @@ static void thermo_kernel(MDState& S, int mode, double dt, hipStream_t st, unsig
                        S.V, S.hist_P.as<double>(), S.hist_nb.as<int>());
 }
 
+// The skin test of list nl (nullable: none) and, for NVE, the record of a
+// step, over the nb kinetic and displacement partials of the step's passes.
+static SkinGate md_gate(MDState& S, const NList* nl, int nb) {
+  SkinGate g;
+  g.nb = nb;
+  g.ke_part = S.ke.as<double>();
+  g.es = S.es.as<double>();
+  g.counters = S.counters.as<int>();
+  if (nl) {
+    g.disp_part = nl->disp_part.as<double>();
+    g.r_list = nl->r_list;
+    g.r_cut = nl->r_cut;
+    g.flag = S.flag.as<int>();
+  }
+  if (S.thermo == 0) {
+    g.e_pot = S.hist_ep.as<double>();
+    g.e_kin = S.hist_ek.as<double>();
+    g.e_cons = S.hist_ec.as<double>();
+    g.h_press = S.hist_P.as<double>();
+    g.h_nb = S.hist_nb.as<int>();
+  }
+  g.V = S.V;
+  return g;
+}
+
 // One step: [thermostat half-step;] drift (+ block displacement maxima) --
 // for Langevin the B A O A sequence; the skin test (NVE: also the record of
 // the previous state); the gated list rebuild; forces; kick; [thermostat
@@ -1464,15 +1443,11 @@ static void md_step(Potential& pot, NList& nl, SetDev& set, MDState& S, int n, d
                        (const double*)nl.disp_part.as<double>(), nl.r_list, nl.r_cut,
                        S.npt.s_min, S.geom, S.np.as<double>(), S.flag.as<int>(),
                        S.counters.as<int>());
-  else
-    hipLaunchKernelGGL(k_md_flag, dim3(1), dim3(256), 0, st, (int)nb,
-                       (const double*)S.ke.as<double>(), (const double*)S.es.as<double>(),
-                       (const double*)nl.disp_part.as<double>(), nl.r_list, nl.r_cut,
-                       S.flag.as<int>(), S.counters.as<int>(),
-                       S.thermo ? (double*)nullptr : S.hist_ep.as<double>(),
-                       S.hist_ek.as<double>(), S.hist_ec.as<double>(), S.V,
-                       S.hist_P.as<double>(), S.hist_nb.as<int>());
-  nl_build_gated(nl, set, S.pos.as<double>(), S.flag.as<int>(), st);
+  // fixed cell: the skin test and the record run in the list's first kernel
+  const bool fixed_cell = !S.flex() && S.thermo != 4;
+  const SkinGate sg = md_gate(S, &nl, (int)nb);
+  nl_build_gated(nl, set, S.pos.as<double>(), S.flag.as<int>(), st,
+                 fixed_cell ? &sg : nullptr);
   // site energies and gradients only on the steps that record the flux
   if (S.every > 0 && flux_now)
     site_grads(pot, nl, set, S.pos.as<double>(), S.spc.as<int>(), S.F.as<double>(),
@@ -1502,11 +1477,7 @@ static void md_record(MDState& S, int n, hipStream_t st) {
     thermo_kernel(S, 0, 0.0, st, nb);
     return;
   }
-  hipLaunchKernelGGL(k_md_flag, dim3(1), dim3(256), 0, st, (int)nb,
-                     (const double*)S.ke.as<double>(), (const double*)S.es.as<double>(),
-                     (const double*)nullptr, 0.0, 0.0, (int*)nullptr, S.counters.as<int>(),
-                     S.hist_ep.as<double>(), S.hist_ek.as<double>(), S.hist_ec.as<double>(), S.V,
-                     S.hist_P.as<double>(), S.hist_nb.as<int>());
+  hipLaunchKernelGGL(k_md_flag, dim3(1), dim3(256), 0, st, md_gate(S, nullptr, (int)nb));
 }
 
 // NVE: the closing half kick still pending after the last step (deferred

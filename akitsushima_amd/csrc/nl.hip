@@ -235,15 +235,21 @@ constexpr int CS_SCAN = 1024;  // cells per scan block
 constexpr int CS_PLACE_LDS = 4096;
 
 // key[a] = cell of atom a, rank[a] = its slot among the atoms of that cell;
-// count[] is zero on entry (k_cell_scan clears it again).
+// count[] is zero on entry (k_cell_scan clears it again).  With sg.flag
+// (a captured MD step), every block evaluates the skin test itself and
+// block 0 writes the flag the later kernels read; else *gate decides.
 __global__ __launch_bounds__(CS_T) void k_cell_count(int n, const double* __restrict__ pos,
                                                      const int* __restrict__ sid,
                                                      const Geom* __restrict__ geoms,
                                                      unsigned* __restrict__ key,
                                                      unsigned* __restrict__ rank,
                                                      unsigned* __restrict__ count,
-                                                     const int* gate) {
-  if (gated_off(gate)) return;
+                                                     const int* gate, SkinGate sg) {
+  if (sg.flag) {
+    if (!skin_gate_eval(sg, blockIdx.x == 0)) return;
+  } else if (gated_off(gate)) {
+    return;
+  }
   for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < n; a += gridDim.x * blockDim.x) {
     const unsigned c = cell_of(a, pos, sid, geoms);
     key[a] = c;
@@ -920,7 +926,7 @@ static size_t lds_per_block() {
 }
 
 static void sort_cells(NList& nl, SetDev& set, const double* d_pos, hipStream_t st,
-                       const int* gate) {
+                       const int* gate, const SkinGate* sg = nullptr) {
   const int n = set.n_atoms;
   const int C = set.n_cells;
   const int nb = (int)nblk((int64_t)C + 1, CS_SCAN);
@@ -946,7 +952,7 @@ static void sort_cells(NList& nl, SetDev& set, const double* d_pos, hipStream_t 
   if (n > 0)
     hipLaunchKernelGGL(k_cell_count, dim3(std::min(nblk(n, CS_T), cap)), dim3(CS_T), 0, st, n,
                        d_pos, (const int*)set.d_sid.as<int>(), (const Geom*)set.d_geoms.as<Geom>(),
-                       key, rank, nl.cell_count.as<unsigned>(), gate);
+                       key, rank, nl.cell_count.as<unsigned>(), gate, sg ? *sg : SkinGate{});
   const int64_t items = std::max((int64_t)n, (int64_t)C + 1);
   const size_t lds_all = sizeof(unsigned) * ((size_t)C + 1);
   if (lds_all + 1024 <= lds_per_block()) {
@@ -1102,11 +1108,13 @@ void nl_build(NList& nl, SetDev& set, const double* d_pos, hipStream_t st) {
 }
 
 void nl_build_gated(NList& nl, SetDev& set, const double* d_pos, const int* d_flag,
-                    hipStream_t st) {
+                    hipStream_t st, const SkinGate* sg) {
   const int n = set.n_atoms;
   AK_REQUIRE(nl.valid && nl.fixed_stride && nl.stride > 0, AK_ERR_STATE,
              "gated rebuild needs a list built with a fixed stride");
-  sort_cells(nl, set, d_pos, st, d_flag);
+  AK_REQUIRE(!sg || (n > 0 && sg->flag == d_flag), AK_ERR_STATE,
+             "a skin gate needs atoms and writes the list's own flag");
+  sort_cells(nl, set, d_pos, st, d_flag, sg);
   // search straight into the final layout (cap = stride)
   launch_search(nl, set, d_pos, nl.cnt.as<int>(), nl.nbr.as<int>(), nl.stride, st, d_flag);
   const int n_clear = set.n_cells + 1;
