@@ -220,6 +220,9 @@ __device__ __forceinline__ unsigned cell_of(int a, const double* __restrict__ po
 // which loop over their work: a step without a rebuild dispatches this many
 // blocks that return at once instead of one per cell or per 256 entries.
 constexpr int kGatedBlocks = 4096;
+// The cell search's cap (one wavefront per block): a rebuild step gains more
+// from the wider grid than the other steps lose.
+constexpr int kGatedSearchBlocks = 8192;
 
 // Counting sort of the atoms by cell: k_cell_count (cell and arrival rank of
 // every atom, counts by atomic add), then k_cell_scan_place when the counts
@@ -354,7 +357,26 @@ __global__ __launch_bounds__(CS_T) void k_cell_scan_place(int n, int C,
   __shared__ unsigned wsum[CS_T / 64];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int m = C + 1;
-  for (int i = tid; i < m; i += CS_T) pre[i] = i < C ? count[i] : 0u;
+  // counts -> LDS, 16 bytes per load and eight loads in flight per thread
+  // (count[] holds a multiple of four entries; those from C on read as 0)
+  const int m4 = (m + 3) >> 2;
+  const uint4* c4 = reinterpret_cast<const uint4*>(count);
+  for (int i0 = tid; i0 < m4; i0 += 8 * CS_T) {
+    uint4 q[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 + u * CS_T;
+      q[u] = i < m4 ? c4[i] : make_uint4(0u, 0u, 0u, 0u);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = 4 * (i0 + u * CS_T);
+      if (i < m) pre[i] = i < C ? q[u].x : 0u;
+      if (i + 1 < m) pre[i + 1] = i + 1 < C ? q[u].y : 0u;
+      if (i + 2 < m) pre[i + 2] = i + 2 < C ? q[u].z : 0u;
+      if (i + 3 < m) pre[i + 3] = i + 3 < C ? q[u].w : 0u;
+    }
+  }
   __syncthreads();
   // thread tid owns the run [lo, hi); exclusive scan of the run sums
   const int per = (m + CS_T - 1) / CS_T;
@@ -782,18 +804,23 @@ __device__ __forceinline__ void rev_entry(int64_t t, const int* __restrict__ cnt
                                           const int* __restrict__ nbr, int stride,
                                           int* __restrict__ rev, int* stats) {
   const int a = (int)(t / stride), k = (int)(t % stride);
-  if (k >= cnt[a]) {
+  const int ca = cnt[a];
+  const int j = nbr[t];  // read with cnt[a] (in the row's storage either way)
+  if (k >= ca) {
     rev[t] = -1;
     return;
   }
-  const int j = nbr[t];
-  const int cj = min(cnt[j], stride);
   const int4* row = reinterpret_cast<const int4*>(nbr + (size_t)j * stride);
+  // the first (up to) 16 entries are read with cnt[j], not after it
+  constexpr int kHead = 4;
+  const int head = min(kHead, stride >> 2);
+  int4 h[kHead];
+#pragma unroll
+  for (int u = 0; u < kHead; ++u) h[u] = u < head ? row[u] : make_int4(0, 0, 0, 0);
+  const int cj = min(cnt[j], stride);
   int below = 0;
   bool found = false;
-#pragma unroll 4
-  for (int m = 0; m < cj; m += 4) {
-    const int4 q = row[m >> 2];
+  auto scan4 = [&](const int4& q, int m) {
     const int v[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
@@ -801,7 +828,11 @@ __device__ __forceinline__ void rev_entry(int64_t t, const int* __restrict__ cnt
       below += (in && v[e] < a) ? 1 : 0;
       found |= in && v[e] == a;
     }
-  }
+  };
+#pragma unroll
+  for (int u = 0; u < kHead; ++u) scan4(h[u], 4 * u);
+#pragma unroll 4
+  for (int m = 4 * kHead; m < cj; m += 4) scan4(row[m >> 2], m);
   if (found) {
     rev[t] = j * stride + below;
   } else {
@@ -939,11 +970,14 @@ static void sort_cells(NList& nl, SetDev& set, const double* d_pos, hipStream_t 
   // the counts are zero between sorts (k_cell_scan clears them, or k_nl_rev
   // after k_cell_scan_place); a sort outside a captured step clears them here
   // as well
-  const bool fresh = nl.cell_count.bytes < sizeof(unsigned) * (C + 1);
-  nl.cell_count.ensure(sizeof(unsigned) * (C + 1));
+  // (C + 1 rounded up to whole 16-byte groups, read as such by
+  // k_cell_scan_place)
+  const size_t count_bytes = sizeof(unsigned) * (((size_t)C + 4) & ~(size_t)3);
+  const bool fresh = nl.cell_count.bytes < count_bytes;
+  nl.cell_count.ensure(count_bytes);
   if (!gate || fresh) {
     AK_REQUIRE(!gate || !fresh, AK_ERR_STATE, "captured cell sort without prepared counts");
-    AK_HIP(hipMemsetAsync(nl.cell_count.p, 0, sizeof(unsigned) * (C + 1), st));
+    AK_HIP(hipMemsetAsync(nl.cell_count.p, 0, count_bytes, st));
   }
   unsigned* key = nl.cell_key.as<unsigned>();
   unsigned* rank = nl.cell_rank.as<unsigned>();
@@ -993,7 +1027,7 @@ static void launch_search(NList& nl, SetDev& set, const double* d_pos, int* d_cn
   }
   const int S = set.n_struct;
   // the captured (gated) search runs on a capped grid that loops over cells
-  const unsigned G = gate ? (unsigned)std::min(C, kGatedBlocks) : (unsigned)C;
+  const unsigned G = gate ? (unsigned)std::min(C, kGatedSearchBlocks) : (unsigned)C;
   switch (nl.cmax) {
     case 64:
       hipLaunchKernelGGL(k_nl_search_reg<1>, dim3(G), dim3(64), 0, st, C, S, co, ge, cs, ca,
