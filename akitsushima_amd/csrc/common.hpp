@@ -137,6 +137,13 @@ __device__ __forceinline__ double dpp_row(double x) {
   return __hiloint2double(hi, lo);
 }
 
+// Maximum over the 64 lanes of a wavefront, every lane receives it.
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+
 // Exchange of one recursive-halving step: the value of the partner lane
 // (row_half_mirror: lane i <-> 7 - i within 8; quad xor 1, xor 2; row_ror 8:
 // xor 8), all within a row of 16 lanes.

@@ -150,7 +150,7 @@ struct SkinGate {
 
 __device__ __forceinline__ int skin_gate_eval(const SkinGate& g, bool owner) {
   __shared__ double smem[16];
-  __shared__ double m[256];
+  __shared__ double wm[16];
   __shared__ int s_fl;
   double v[1] = {0.0};
   double d = 0.0;
@@ -158,15 +158,13 @@ __device__ __forceinline__ int skin_gate_eval(const SkinGate& g, bool owner) {
     if (owner) v[0] += g.ke_part[b];
     if (g.disp_part) d = fmax(d, g.disp_part[b]);
   }
-  block_sum<1>(v, smem);
-  m[threadIdx.x] = d;
+  if (owner) block_sum<1>(v, smem);  // block-uniform branch
+  d = wave_max(d);
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = d;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) m[threadIdx.x] = fmax(m[threadIdx.x], m[threadIdx.x + o]);
-    __syncthreads();
-  }
   if (threadIdx.x == 0) {
-    const int fl = (g.disp_part && g.r_list - 2.0 * sqrt(m[0]) <= g.r_cut) ? 1 : 0;
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) d = fmax(d, wm[w]);
+    const int fl = (g.disp_part && g.r_list - 2.0 * sqrt(d) <= g.r_cut) ? 1 : 0;
     s_fl = fl;
     if (owner) {
       if (g.e_pot) {

@@ -47,15 +47,18 @@ __device__ __forceinline__ double wrap_store(int a, const double (&x)[3], const 
   return dx * dx + dy * dy + dz * dz;
 }
 
-// part[blockIdx.x] = the block's largest d2 (m: 256 doubles of LDS).
-__device__ __forceinline__ void block_max_store(double d2, double* m, double* __restrict__ part) {
-  m[threadIdx.x] = d2;
+// part[blockIdx.x] = the block's largest d2 (wavefront maxima, then one
+// LDS exchange).
+__device__ __forceinline__ void block_max_store(double d2, double* __restrict__ part) {
+  __shared__ double wm[16];
+  d2 = wave_max(d2);
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = d2;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) m[threadIdx.x] = fmax(m[threadIdx.x], m[threadIdx.x + o]);
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = wm[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) x = fmax(x, wm[w]);
+    part[blockIdx.x] = x;
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = m[0];
 }
 
 // Half kick, drift and wrap into the cell, and this block's largest squared
@@ -71,7 +74,6 @@ __global__ __launch_bounds__(256) void k_md_drift(int n, double* __restrict__ po
                                                   double* __restrict__ part,
                                                   const double* __restrict__ vscale,
                                                   const double* __restrict__ keep) {
-  __shared__ double m[256];
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   double d2 = 0.0;
   if (a < n) {
@@ -87,7 +89,7 @@ __global__ __launch_bounds__(256) void k_md_drift(int n, double* __restrict__ po
     }
     d2 = wrap_store(a, x, g, ref, pos);
   }
-  block_max_store(d2, m, part);
+  block_max_store(d2, part);
 }
 
 // Plain NVE with the closing half kick of a step deferred to the next
@@ -102,7 +104,6 @@ __global__ __launch_bounds__(256) void k_md_kick_drift(
     const double* __restrict__ mass, double dt, Geom g, const double* __restrict__ ref,
     double* __restrict__ part, double* __restrict__ ke_part, const int* __restrict__ counters,
     const double* __restrict__ keep) {
-  __shared__ double m[256];
   __shared__ double smem[16];
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   const bool pending = counters[0] > 0;
@@ -134,7 +135,7 @@ __global__ __launch_bounds__(256) void k_md_kick_drift(
   }
   block_sum<1>(ke, smem);
   if (threadIdx.x == 0) ke_part[blockIdx.x] = ke[0];
-  block_max_store(d2, m, part);
+  block_max_store(d2, part);
 }
 
 // a = F conv / m; v += dt/2 a (kick != 0); per-block kinetic-energy sums
