@@ -137,6 +137,44 @@ __device__ __forceinline__ double dpp_row(double x) {
   return __hiloint2double(hi, lo);
 }
 
+// Position of atom a in row j of a neighbour list (rows ascending, stride a
+// power of two of at least four, cnt[j] clamped to the stride), or -1 when
+// a is not in the row: the number of entries below a, counted over 16-byte
+// loads, the first 16 entries read with cnt[j] rather than after it.
+__device__ __forceinline__ int row_rank(int a, int j, const int* __restrict__ cnt,
+                                        const int* __restrict__ nbr, int stride) {
+  const int4* row = reinterpret_cast<const int4*>(nbr + (size_t)j * stride);
+  constexpr int kHead = 4;
+  const int head = min(kHead, stride >> 2);
+  int4 h[kHead];
+#pragma unroll
+  for (int u = 0; u < kHead; ++u) h[u] = u < head ? row[u] : make_int4(0, 0, 0, 0);
+  const int cj = min(cnt[j], stride);
+  int below = 0;
+  bool found = false;
+  auto scan4 = [&](const int4& q, int m) {
+    const int v[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const bool in = m + e < cj;
+      below += (in && v[e] < a) ? 1 : 0;
+      found |= in && v[e] == a;
+    }
+  };
+#pragma unroll
+  for (int u = 0; u < kHead; ++u) scan4(h[u], 4 * u);
+#pragma unroll 4
+  for (int m = 4 * kHead; m < cj; m += 4) scan4(row[m >> 2], m);
+  return found ? below : -1;
+}
+
+// Lane l's value of x (l wave-uniform).
+__device__ __forceinline__ double readlane_d(double x, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
+  return __hiloint2double(hi, lo);
+}
+
 // Maximum over the 64 lanes of a wavefront, every lane receives it.
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll

@@ -468,7 +468,7 @@ __device__ __forceinline__ void search_cell(
     const int* __restrict__ cell_start, const unsigned* __restrict__ cell_atoms,
     const double* __restrict__ pos, double rl2, int cap, int cmax, int* __restrict__ cnt,
     int* __restrict__ nbr, int* __restrict__ stats, int* __restrict__ cand_n,
-    unsigned* __restrict__ cand, int* __restrict__ spre, int* __restrict__ scell) {
+    unsigned* __restrict__ cand, int* __restrict__ spre, int* __restrict__ sstart) {
   const int lane = threadIdx.x;
   const int s = find_struct(cell_offs, S, c);
   const Geom& g = geoms[s];
@@ -479,13 +479,14 @@ __device__ __forceinline__ void search_cell(
   const int n1 = nc1 == 1 ? 1 : (nc1 == 2 ? 2 : 3);
   const int n2 = nc2 == 1 ? 1 : (nc2 == 2 ? 2 : 3);
   const int nst = n0 * n1 * n2;
-  int size = 0, cid = 0;
+  int size = 0, cid = 0, cst = 0;
   if (lane < nst) {
     int a0 = lane / (n1 * n2), a1 = (lane / n2) % n1, a2 = lane % n2;
     int o0 = n0 == 3 ? a0 - 1 : a0, o1 = n1 == 3 ? a1 - 1 : a1, o2 = n2 == 3 ? a2 - 1 : a2;
     int j0 = (c0 + o0 + nc0) % nc0, j1 = (c1 + o1 + nc1) % nc1, j2 = (c2 + o2 + nc2) % nc2;
     cid = g.cell_off + (j0 * nc1 + j1) * nc2 + j2;
-    size = cell_start[cid + 1] - cell_start[cid];
+    cst = cell_start[cid];
+    size = cell_start[cid + 1] - cst;
   }
   // inclusive wave prefix of the stencil cell sizes
   int inc = size;
@@ -497,11 +498,21 @@ __device__ __forceinline__ void search_cell(
   const int ncand = __shfl(inc, 31, 64);
   if (lane < nst) {
     spre[lane + 1] = inc;
-    scell[lane] = cid;
+    sstart[lane] = cst;  // first sorted position of the stencil cell
   }
   if (lane == 0) spre[0] = 0;
   const int first = cell_start[c], last = cell_start[c + 1];
   if (cand_n && lane == 0) cand_n[c] = ncand;
+  // the cell's own atoms, one per lane (the first 64), read alongside the
+  // candidates rather than one after another in the row loop
+  int h_i = 0;
+  double h_x = 0.0, h_y = 0.0, h_z = 0.0;
+  if (lane < last - first) {
+    h_i = (int)cell_atoms[first + lane];
+    h_x = pos[3 * h_i];
+    h_y = pos[3 * h_i + 1];
+    h_z = pos[3 * h_i + 2];
+  }
   if (ncand > cmax) {
     if (lane == 0) stats[1] = max(stats[1], ncand);  // candidate overflow
     for (int t = first + lane; t < last; t += 64) cnt[cell_atoms[t]] = 0;
@@ -521,7 +532,7 @@ __device__ __forceinline__ void search_cell(
         else
           hi = mid;
       }
-      v = cell_atoms[cell_start[scell[lo]] + (q - spre[lo])];
+      v = cell_atoms[sstart[lo] + (q - spre[lo])];
     }
     cand[q] = v;
   }
@@ -544,8 +555,20 @@ __device__ __forceinline__ void search_cell(
     }
   }
   for (int t = first; t < last; ++t) {
-    const int i = (int)cell_atoms[t];
-    const double pix = pos[3 * i], piy = pos[3 * i + 1], piz = pos[3 * i + 2];
+    const int h = t - first;  // wave-uniform
+    int i;
+    double pix, piy, piz;
+    if (h < 64) {
+      i = __builtin_amdgcn_readlane(h_i, h);
+      pix = readlane_d(h_x, h);
+      piy = readlane_d(h_y, h);
+      piz = readlane_d(h_z, h);
+    } else {
+      i = (int)cell_atoms[t];
+      pix = pos[3 * i];
+      piy = pos[3 * i + 1];
+      piz = pos[3 * i + 2];
+    }
     int count = 0;
     for (int q0 = 0; q0 < ncand; q0 += 64) {
       int q = q0 + lane;
@@ -584,10 +607,10 @@ __global__ __launch_bounds__(64) void k_nl_search(
   if (gated_off(gate)) return;
   extern __shared__ unsigned cand[];  // cmax entries (power of two)
   __shared__ int spre[28];
-  __shared__ int scell[27];
+  __shared__ int sstart[27];
   for (int cc = blockIdx.x; cc < C; cc += gridDim.x) {
     search_cell(xcd_remap(cc, C), S, cell_offs, geoms, cell_start, cell_atoms, pos, rl2, cap,
-                cmax, cnt, nbr, stats, cand_n, cand, spre, scell);
+                cmax, cnt, nbr, stats, cand_n, cand, spre, sstart);
     __syncthreads();  // the shared arrays are filled again for the next cell
   }
 }
@@ -602,7 +625,7 @@ __device__ __forceinline__ void search_cell_reg(
     const int* __restrict__ cell_start, const unsigned* __restrict__ cell_atoms,
     const double* __restrict__ pos, double rl2, int cap, int* __restrict__ cnt,
     int* __restrict__ nbr, int* __restrict__ stats, int* __restrict__ cand_n,
-    int* __restrict__ spre, int* __restrict__ scell) {
+    int* __restrict__ spre, int* __restrict__ sstart) {
   constexpr int NP = 64 * EPL;
   const int lane = threadIdx.x;
   const int s = find_struct(cell_offs, S, c);
@@ -614,13 +637,14 @@ __device__ __forceinline__ void search_cell_reg(
   const int n1 = nc1 == 1 ? 1 : (nc1 == 2 ? 2 : 3);
   const int n2 = nc2 == 1 ? 1 : (nc2 == 2 ? 2 : 3);
   const int nst = n0 * n1 * n2;
-  int size = 0, cid = 0;
+  int size = 0, cid = 0, cst = 0;
   if (lane < nst) {
     int a0 = lane / (n1 * n2), a1 = (lane / n2) % n1, a2 = lane % n2;
     int o0 = n0 == 3 ? a0 - 1 : a0, o1 = n1 == 3 ? a1 - 1 : a1, o2 = n2 == 3 ? a2 - 1 : a2;
     int j0 = (c0 + o0 + nc0) % nc0, j1 = (c1 + o1 + nc1) % nc1, j2 = (c2 + o2 + nc2) % nc2;
     cid = g.cell_off + (j0 * nc1 + j1) * nc2 + j2;
-    size = cell_start[cid + 1] - cell_start[cid];
+    cst = cell_start[cid];
+    size = cell_start[cid + 1] - cst;
   }
   int inc = size;
 #pragma unroll
@@ -631,11 +655,21 @@ __device__ __forceinline__ void search_cell_reg(
   const int ncand = __shfl(inc, 31, 64);
   if (lane < nst) {
     spre[lane + 1] = inc;
-    scell[lane] = cid;
+    sstart[lane] = cst;  // first sorted position of the stencil cell
   }
   if (lane == 0) spre[0] = 0;
   const int first = cell_start[c], last = cell_start[c + 1];
   if (cand_n && lane == 0) cand_n[c] = ncand;
+  // the cell's own atoms, one per lane (the first 64), read alongside the
+  // candidates rather than one after another in the row loop
+  int h_i = 0;
+  double h_x = 0.0, h_y = 0.0, h_z = 0.0;
+  if (lane < last - first) {
+    h_i = (int)cell_atoms[first + lane];
+    h_x = pos[3 * h_i];
+    h_y = pos[3 * h_i + 1];
+    h_z = pos[3 * h_i + 2];
+  }
   if (ncand > NP) {
     if (lane == 0) stats[1] = max(stats[1], ncand);  // candidate overflow
     for (int t = first + lane; t < last; t += 64) cnt[cell_atoms[t]] = 0;
@@ -657,7 +691,7 @@ __device__ __forceinline__ void search_cell_reg(
         else
           hi = mid;
       }
-      v[r] = cell_atoms[cell_start[scell[lo]] + (q - spre[lo])];
+      v[r] = cell_atoms[sstart[lo] + (q - spre[lo])];
     }
   }
   // bitonic sort, ascending, of the elements e = lane + 64 r below P, the
@@ -708,8 +742,20 @@ __device__ __forceinline__ void search_cell_reg(
     pz[r] = ok ? pos[3 * j + 2] : 0.0;
   }
   for (int t = first; t < last; ++t) {
-    const int i = (int)cell_atoms[t];
-    const double pix = pos[3 * i], piy = pos[3 * i + 1], piz = pos[3 * i + 2];
+    const int h = t - first;  // wave-uniform
+    int i;
+    double pix, piy, piz;
+    if (h < 64) {
+      i = __builtin_amdgcn_readlane(h_i, h);
+      pix = readlane_d(h_x, h);
+      piy = readlane_d(h_y, h);
+      piz = readlane_d(h_z, h);
+    } else {
+      i = (int)cell_atoms[t];
+      pix = pos[3 * i];
+      piy = pos[3 * i + 1];
+      piz = pos[3 * i + 2];
+    }
     int count = 0;
 #pragma unroll
     for (int r = 0; r < EPL; ++r) {
@@ -743,11 +789,11 @@ __global__ __launch_bounds__(64) void k_nl_search_reg(
     int* __restrict__ nbr, int* __restrict__ stats, int* __restrict__ cand_n, const int* gate) {
   if (gated_off(gate)) return;
   __shared__ int spre[28];
-  __shared__ int scell[27];
+  __shared__ int sstart[27];
   // neighbouring cells on one XCD; a smaller grid loops over the cells
   for (int cc = blockIdx.x; cc < C; cc += gridDim.x) {
     search_cell_reg<EPL>(xcd_remap(cc, C), S, cell_offs, geoms, cell_start, cell_atoms, pos, rl2,
-                         cap, cnt, nbr, stats, cand_n, spre, scell);
+                         cap, cnt, nbr, stats, cand_n, spre, sstart);
     __syncthreads();  // the shared arrays are filled again for the next cell
   }
 }
@@ -789,12 +835,11 @@ __global__ void k_nl_repack(int n, const int* __restrict__ cnt, const int* __res
                             int cap, int* __restrict__ dst, int stride) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)n * stride) return;
-  int a = (int)(t / stride), k = (int)(t % stride);
+  const int sh = 31 - __clz(stride);  // the stride is a power of two
+  const int a = (int)(t >> sh), k = (int)(t & (stride - 1));
   dst[t] = k < cnt[a] ? src[(size_t)a * cap + k] : -1;
 }
 
-// Also copies the build positions (copy_src -> copy_dst, 3n doubles) when
-// given: the reference positions of the next displacement test.
 // Reverse slot of entry t = (a, k): the position of a in the row of its
 // neighbour j.  Rows are sorted ascending, so that position is the number of
 // entries of row j below a; the row is read 16 bytes at a time (the stride
@@ -803,37 +848,16 @@ __global__ void k_nl_repack(int n, const int* __restrict__ cnt, const int* __res
 __device__ __forceinline__ void rev_entry(int64_t t, const int* __restrict__ cnt,
                                           const int* __restrict__ nbr, int stride,
                                           int* __restrict__ rev, int* stats) {
-  const int a = (int)(t / stride), k = (int)(t % stride);
+  const int sh = 31 - __clz(stride);  // the stride is a power of two
+  const int a = (int)(t >> sh), k = (int)(t & (stride - 1));
   const int ca = cnt[a];
   const int j = nbr[t];  // read with cnt[a] (in the row's storage either way)
   if (k >= ca) {
     rev[t] = -1;
     return;
   }
-  const int4* row = reinterpret_cast<const int4*>(nbr + (size_t)j * stride);
-  // the first (up to) 16 entries are read with cnt[j], not after it
-  constexpr int kHead = 4;
-  const int head = min(kHead, stride >> 2);
-  int4 h[kHead];
-#pragma unroll
-  for (int u = 0; u < kHead; ++u) h[u] = u < head ? row[u] : make_int4(0, 0, 0, 0);
-  const int cj = min(cnt[j], stride);
-  int below = 0;
-  bool found = false;
-  auto scan4 = [&](const int4& q, int m) {
-    const int v[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const bool in = m + e < cj;
-      below += (in && v[e] < a) ? 1 : 0;
-      found |= in && v[e] == a;
-    }
-  };
-#pragma unroll
-  for (int u = 0; u < kHead; ++u) scan4(h[u], 4 * u);
-#pragma unroll 4
-  for (int m = 4 * kHead; m < cj; m += 4) scan4(row[m >> 2], m);
-  if (found) {
+  const int below = row_rank(a, j, cnt, nbr, stride);
+  if (below >= 0) {
     rev[t] = j * stride + below;
   } else {
     rev[t] = -1;
@@ -1120,6 +1144,8 @@ void nl_build(NList& nl, SetDev& set, const double* d_pos, hipStream_t st) {
                "neighbour row exceeds the fixed stride of a captured list");
     stride = nl.stride;
   }
+  AK_REQUIRE(stride >= 4 && (stride & (stride - 1)) == 0, AK_ERR_STATE,
+             "neighbour rows need a power-of-two stride");
   nl.stride = stride;
   nl.nbr.ensure(sizeof(int) * (size_t)std::max(1, n) * stride);
   nl.rev.ensure(sizeof(int) * (size_t)std::max(1, n) * stride);
