@@ -112,3 +112,31 @@ def test_oracle_loss_matches_the_references_loss_tests():
     r = O.generic_residuals(params, mixed, w)
     assert len(r) == residual_length(mixed)
     assert np.isclose(r @ r, O.generic_loss(params, mixed, w), rtol=1e-12)
+
+
+def test_oracle_huber_terms_against_closed_forms():
+    """Energy term (N dE/N)^2 and force term r0^2 mean(dF^2) with known
+    offsets, through both branches of rho(q) = q (sqrt q <= delta) and
+    2 delta sqrt(q) - delta^2 (fitting/loss.py:27-48, 212-227)."""
+    params, structs = _oracle_set(n=1)
+    s = structs[0]
+    N = s["natoms"]
+    e, f, _ = O.tersoff_flat(s["lattice"], s["positions"], params)
+    d_e = 0.25 / N  # so that N dE = 0.25 eV
+    s_e = dict(s, ref_energy=e / N + d_e, ref_forces=None, ref_stress=None)
+    w = {"energy": 1.0, "forces": 0.0, "stress": 0.0}
+    assert O.generic_loss(params, [s_e], w) == pytest.approx(0.0625, rel=1e-9)
+    hub = dict(w, huber_delta_energy=0.1, huber_delta_forces=1.0, huber_delta_stress=1.0)
+    # r = 0.25 > delta = 0.1: 2 (0.1)(0.25) - 0.01 = 0.04
+    assert O.generic_loss(params, [s_e], hub, "huber") == pytest.approx(0.04, rel=1e-9)
+    hub_in = dict(hub, huber_delta_energy=0.3)  # r <= delta: the square
+    assert O.generic_loss(params, [s_e], hub_in, "huber") == pytest.approx(0.0625, rel=1e-9)
+    # forces offset by 0.2 eV/A in every component: q = r0^2 0.04, r0 = 1.5
+    s_f = dict(s, ref_energy=e / N, ref_forces=f + 0.2, ref_stress=None)
+    wf = {"energy": 0.0, "forces": 2.0, "stress": 0.0, "r0": 1.5}
+    q = 1.5 ** 2 * 0.04
+    assert O.generic_loss(params, [s_f], wf) == pytest.approx(2.0 * q, rel=1e-9)
+    hf = dict(wf, huber_delta_energy=1.0, huber_delta_forces=0.1, huber_delta_stress=1.0)
+    r = np.sqrt(q)
+    assert O.generic_loss(params, [s_f], hf, "huber") == pytest.approx(
+        2.0 * (2 * 0.1 * r - 0.01), rel=1e-9)
