@@ -223,6 +223,8 @@ constexpr int kGatedBlocks = 4096;
 // The cell search's cap (one wavefront per block): a rebuild step gains more
 // from the wider grid than the other steps lose.
 constexpr int kGatedSearchBlocks = 8192;
+// The reverse-slot pass's cap (256 threads per block).
+constexpr int kGatedRevBlocks = 2048;
 
 // Counting sort of the atoms by cell: k_cell_count (cell and arrival rank of
 // every atom, counts by atomic add), then k_cell_scan_place when the counts
@@ -1179,7 +1181,7 @@ void nl_build_gated(NList& nl, SetDev& set, const double* d_pos, const int* d_fl
   launch_search(nl, set, d_pos, nl.cnt.as<int>(), nl.nbr.as<int>(), nl.stride, st, d_flag);
   const int n_clear = set.n_cells + 1;
   int64_t tot = std::max(std::max((int64_t)n * nl.stride, 3 * (int64_t)n), (int64_t)n_clear);
-  const unsigned grid = (unsigned)std::min<int64_t>(nblk(tot, 256), kGatedBlocks);
+  const unsigned grid = (unsigned)std::min<int64_t>(nblk(tot, 256), kGatedRevBlocks);
   hipLaunchKernelGGL(k_nl_rev, dim3(grid), dim3(256), 0, st, n, nl.cnt.as<int>(),
                      nl.nbr.as<int>(), nl.stride, nl.rev.as<int>(), nl.stats.as<int>(), d_flag,
                      d_pos, nl.ref_pos.as<double>(), nl.cell_count.as<unsigned>(), n_clear);
