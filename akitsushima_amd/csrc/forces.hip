@@ -521,7 +521,8 @@ __global__ __launch_bounds__(256) void k_tersoff_site(int n_chunks, int n, const
 // factors of each (bond, partner) pair are formed once: the factor of the
 // partner's zeta in which this bond is the third leg is the partner lane's
 // own factor for this bond, brought over by the inverse rotation.  Block
-// sums of energy and virial are reduced through LDS.  Terms and gating are
+// sums of energy and virial as in the other bond kernels (wave_partial).
+// Terms and gating are
 // those of k_tersoff_site; only the summation order of the zeta sums and of
 // the bond gradients differs.
 // ---------------------------------------------------------------------------
@@ -532,34 +533,6 @@ __device__ __forceinline__ double qrot(double x) {
   static_assert(P >= 1 && P <= 3, "rotation of a quad");
   constexpr int ctrl = P == 1 ? 0x39 : (P == 2 ? 0x4E : 0x93);
   return dpp_quad<ctrl>(x);
-}
-
-// Block sums of NV values per thread (blockDim a multiple of 64, <= 256):
-// thread t stores its values to LDS; one row of 16 lanes serves a value,
-// each lane adding blockDim/16 consecutive entries in thread order, and the
-// row's 16 partial sums are combined by row and quad butterflies.  Fixed
-// order, bitwise reproducible.  Lane 0 of the row of value q writes it to
-// partial[q * n_chunks + c].
-template <int NV>
-__device__ __forceinline__ void block_partial_lds(const double (&v)[NV],
-                                                  double* __restrict__ partial, int c,
-                                                  int n_chunks) {
-  __shared__ double red[NV][256];
-  const int t = threadIdx.x, nt = blockDim.x, per = nt >> 4;
-#pragma unroll
-  for (int q = 0; q < NV; ++q) red[q][t] = v[q];
-  __syncthreads();
-  for (int q = t >> 4; q < NV; q += nt >> 4) {  // row-uniform: whole rows take part
-    const int w = t & 15;
-    double s = 0.0;
-    for (int e = 0; e < per; ++e) s += red[q][w * per + e];
-    s += dpp_row<0x128>(s);  // row_ror:8
-    s += dpp_row<0x124>(s);  // row_ror:4
-    s += dpp_quad<0x4E>(s);
-    s += dpp_quad<0xB1>(s);
-    if (w == 0) partial[(size_t)q * n_chunks + c] = s;
-  }
-  __syncthreads();  // red is written again by the block's next chunk
 }
 
 template <bool L3Z>
@@ -682,9 +655,9 @@ __device__ __forceinline__ void k_tersoff_q4_chunk(
     const double ea = seg_sum<4>(e);
     if (live && k == 0) esite[a] = ea;
   }
-  const double v[10] = {e,        -Gx * ux, -Gx * uy, -Gx * uz, -Gy * ux,
-                        -Gy * uy, -Gy * uz, -Gz * ux, -Gz * uy, -Gz * uz};
-  block_partial_lds<10>(v, partial, c, n_chunks);
+  double v[10] = {e,        -Gx * ux, -Gx * uy, -Gx * uz, -Gy * ux,
+                  -Gy * uy, -Gy * uz, -Gz * ux, -Gz * uy, -Gz * uz};
+  wave_partial(v, partial, c, n_chunks);
 }
 
 template <bool L3Z>
