@@ -228,19 +228,6 @@ __device__ __forceinline__ double wave_sums(const double (&v)[NV], int& idx) {
   return t;
 }
 
-// Sum over the 64 lanes of a wavefront, every lane receives it: DPP quad
-// butterflies, row rotations by 4 and 8, then two cross-row shuffles.
-// Fixed order, so bitwise reproducible.
-__device__ __forceinline__ double wave_sum(double v) {
-  v += dpp_quad<0xB1>(v);
-  v += dpp_quad<0x4E>(v);
-  v += dpp_row<0x124>(v);
-  v += dpp_row<0x128>(v);
-  v += __shfl_xor(v, 16, kWave);
-  v += __shfl_xor(v, 32, kWave);
-  return v;
-}
-
 // The dispatcher hands consecutive workgroups to the 8 XCDs in turn, and each
 // XCD has its own L2.  Chunks (and atom blocks) are numbered so that XCD x
 // gets the contiguous x-th eighth of them: neighbouring atoms, whose

@@ -987,13 +987,10 @@ __device__ __forceinline__ void struct_reduce_block(int s, const Geom* __restric
 #pragma unroll
     for (int q = 0; q < 10; ++q) v[q] += partial[(size_t)q * n_chunks + c];
   }
-#pragma unroll
-  for (int q = 0; q < 10; ++q) v[q] = wave_sum(v[q]);
+  int idx;
+  const double t = wave_sums<10>(v, idx);
   const int wid = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-    for (int q = 0; q < 10; ++q) sm[q][wid] = v[q];
-  }
+  if ((threadIdx.x & 63) < 16 && idx < 10) sm[idx][wid] = t;
   __syncthreads();
   // EAM: a structure without any bond inside r_c gives zeros, its isolated
   // atoms' embedding energy F(0) included (EAMPotential.compute,
@@ -1027,19 +1024,22 @@ __global__ __launch_bounds__(256) void k_gather(int n, const int* __restrict__ c
                                                 const int* __restrict__ rev, int stride,
                                                 const double* __restrict__ G,
                                                 double* __restrict__ F, int atom_blocks,
-                                                const Geom* __restrict__ geoms,
+                                                int n_struct, const Geom* __restrict__ geoms,
                                                 const int* __restrict__ first, int n_chunks,
                                                 const double* __restrict__ partial,
                                                 double* __restrict__ out,
                                                 const uint64_t* __restrict__ amask, int sym) {
   // the structure blocks come first, so that they are dispatched before
-  // the atom blocks instead of forming the launch's tail
-  const int n_struct = (int)gridDim.x - atom_blocks;
-  if ((int)blockIdx.x < n_struct) {
-    struct_reduce_block((int)blockIdx.x, geoms, first, n_chunks, partial, out);
+  // the atom blocks instead of forming the launch's tail; gather_lead pads
+  // them to a whole number of XCD rounds, so that atom tile b runs on the
+  // XCD whose L2 holds the site gradients of site-kernel chunk b
+  const int lead = (int)gridDim.x - atom_blocks;
+  if ((int)blockIdx.x < lead) {
+    if ((int)blockIdx.x < n_struct)
+      struct_reduce_block((int)blockIdx.x, geoms, first, n_chunks, partial, out);
     return;
   }
-  const int gt = xcd_remap((int)blockIdx.x - n_struct, atom_blocks) * blockDim.x + threadIdx.x;
+  const int gt = xcd_remap((int)blockIdx.x - lead, atom_blocks) * blockDim.x + threadIdx.x;
   const int a = gt / L;
   const int k = gt & (L - 1);
   const bool live = a < n;
@@ -1225,19 +1225,22 @@ __global__ __launch_bounds__(256) void k_gather_c(int n, const int* __restrict__
                                                   const int* __restrict__ rev, int stride,
                                                   const double* __restrict__ G,
                                                   double* __restrict__ F, int atom_blocks,
-                                                  const Geom* __restrict__ geoms,
+                                                  int n_struct, const Geom* __restrict__ geoms,
                                                   const int* __restrict__ first, int n_chunks,
                                                   const double* __restrict__ partial,
                                                   double* __restrict__ out,
                                                   const uint64_t* __restrict__ amask, int sym) {
   // the structure blocks come first, so that they are dispatched before
-  // the atom blocks instead of forming the launch's tail
-  const int n_struct = (int)gridDim.x - atom_blocks;
-  if ((int)blockIdx.x < n_struct) {
-    struct_reduce_block((int)blockIdx.x, geoms, first, n_chunks, partial, out);
+  // the atom blocks instead of forming the launch's tail; gather_lead pads
+  // them to a whole number of XCD rounds, so that atom tile b runs on the
+  // XCD whose L2 holds the site gradients of site-kernel chunk b
+  const int lead = (int)gridDim.x - atom_blocks;
+  if ((int)blockIdx.x < lead) {
+    if ((int)blockIdx.x < n_struct)
+      struct_reduce_block((int)blockIdx.x, geoms, first, n_chunks, partial, out);
     return;
   }
-  const int gt = xcd_remap((int)blockIdx.x - n_struct, atom_blocks) * blockDim.x + threadIdx.x;
+  const int gt = xcd_remap((int)blockIdx.x - lead, atom_blocks) * blockDim.x + threadIdx.x;
   const int a = gt / LA;
   const int q = gt & (LA - 1);
   const bool live = a < n;
@@ -1892,6 +1895,10 @@ static inline dim3 chunk_grid(const Potential& pot, int n_chunks) {
   return dim3(g > 0 ? g : 1);
 }
 
+// Leading blocks of a gather launch: one per structure, rounded up to a
+// multiple of the XCD count (blocks are dispatched to the XCDs round-robin).
+static inline int gather_lead(int n_struct) { return (n_struct + kXcds - 1) / kXcds * kXcds; }
+
 template <int L>
 static void launch_bond_kernels(Potential& pot, NList& nl, const ForceWork& w,
                                 const double* d_pos, const int* d_spc, double* d_F,
@@ -1958,22 +1965,24 @@ static void launch_bond_kernels(Potential& pot, NList& nl, const ForceWork& w,
   // and its reverse with the same bitwise distance; multi-species SW legs
   // have per-species cutoffs
   const int sym = pot.sw_multi ? 0 : 1;
+  const int lead = gather_lead(n_struct);
   if (pot.sw_multi) {
     constexpr int LA = L < kRowLanes ? L : kRowLanes;
     const int atom_blocks = (int)(((int64_t)n * LA + 255) / 256);
-    hipLaunchKernelGGL((k_gather_c<L, LA>), dim3(atom_blocks + n_struct), dim3(256), 0, st,
-                       n, cnt, rev, nl.stride, (const double*)G, d_F, atom_blocks, geoms, first,
-                       n_chunks, (const double*)part, d_out, (const uint64_t*)amask, sym);
+    hipLaunchKernelGGL((k_gather_c<L, LA>), dim3(atom_blocks + lead), dim3(256), 0, st, n, cnt,
+                       rev, nl.stride, (const double*)G, d_F, atom_blocks, n_struct, geoms,
+                       first, n_chunks, (const double*)part, d_out, (const uint64_t*)amask, sym);
   } else if constexpr (L >= kLongRow) {
     const int atom_blocks = (int)(((int64_t)n * kRowLanes + 255) / 256);
-    hipLaunchKernelGGL((k_gather_c<L, kRowLanes>), dim3(atom_blocks + n_struct), dim3(256),
-                       0, st, n, cnt, rev, nl.stride, (const double*)G, d_F, atom_blocks, geoms,
-                       first, n_chunks, (const double*)part, d_out, (const uint64_t*)amask, sym);
+    hipLaunchKernelGGL((k_gather_c<L, kRowLanes>), dim3(atom_blocks + lead), dim3(256), 0, st,
+                       n, cnt, rev, nl.stride, (const double*)G, d_F, atom_blocks, n_struct,
+                       geoms, first, n_chunks, (const double*)part, d_out,
+                       (const uint64_t*)amask, sym);
   } else {
     const int atom_blocks = (int)(((int64_t)n * L + 255) / 256);
-    hipLaunchKernelGGL(k_gather<L>, dim3(atom_blocks + n_struct), dim3(256), 0, st, n, cnt,
-                       rev, nl.stride, (const double*)G, d_F, atom_blocks, geoms, first, n_chunks,
-                       (const double*)part, d_out, (const uint64_t*)amask, sym);
+    hipLaunchKernelGGL(k_gather<L>, dim3(atom_blocks + lead), dim3(256), 0, st, n, cnt, rev,
+                       nl.stride, (const double*)G, d_F, atom_blocks, n_struct, geoms, first,
+                       n_chunks, (const double*)part, d_out, (const uint64_t*)amask, sym);
   }
 }
 
