@@ -625,13 +625,13 @@ int ak_site_terms(ak_potential* P, ak_nlist* nl, int32_t n, const double* lattic
                        d_vec.as<double>());
     AK_HIP(hipGetLastError());
     std::vector<int> cnt(n), nbr(ns);
-    std::vector<double> G(3 * ns), V(3 * ns);
+    std::vector<double> G(kGRec * ns), V(3 * ns);
     std::vector<uint64_t> mask(n, ~uint64_t(0));
     const bool masked = site_masked(P->pot, L->nl);
     AK_HIP(hipMemcpyAsync(cnt.data(), L->nl.cnt.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
     AK_HIP(hipMemcpyAsync(nbr.data(), L->nl.nbr.p, sizeof(int) * ns, hipMemcpyDeviceToHost, st));
-    AK_HIP(hipMemcpyAsync(G.data(), P->pot.G.p, sizeof(double) * 3 * ns, hipMemcpyDeviceToHost,
-                          st));
+    AK_HIP(hipMemcpyAsync(G.data(), P->pot.G.p, sizeof(double) * kGRec * ns,
+                          hipMemcpyDeviceToHost, st));
     AK_HIP(hipMemcpyAsync(V.data(), d_vec.p, sizeof(double) * 3 * ns, hipMemcpyDeviceToHost, st));
     if (masked)
       AK_HIP(hipMemcpyAsync(mask.data(), P->pot.amask.p, sizeof(uint64_t) * n,
@@ -656,9 +656,9 @@ int ak_site_terms(ak_potential* P, ak_nlist* nl, int32_t n, const double* lattic
           vec[3 * m] = x;
           vec[3 * m + 1] = y;
           vec[3 * m + 2] = z;
-          dudv[3 * m] = has_g ? G[t] : 0.0;
-          dudv[3 * m + 1] = has_g ? G[ns + t] : 0.0;
-          dudv[3 * m + 2] = has_g ? G[2 * ns + t] : 0.0;
+          dudv[3 * m] = has_g ? G[kGRec * t] : 0.0;
+          dudv[3 * m + 1] = has_g ? G[kGRec * t + 1] : 0.0;
+          dudv[3 * m + 2] = has_g ? G[kGRec * t + 2] : 0.0;
         }
         ++m;
       }
@@ -819,7 +819,7 @@ int ak_compute_device_noalloc(ak_potential* P, ak_nlist* L, int32_t n, const dou
     const size_t o_rng = take(sizeof(int) * 2 * (size_t)std::max(1, n_chunks));
     const size_t o_first = take(sizeof(int) * 2);
     const size_t o_part = take(sizeof(double) * 10 * (size_t)std::max(1, n_chunks));
-    const size_t o_G = eam ? 0 : take(sizeof(double) * 3 * (size_t)n * L->nl.stride);
+    const size_t o_G = eam ? 0 : take(sizeof(double) * kGRec * (size_t)n * L->nl.stride);
     const size_t o_mask = eam ? 0 : take(sizeof(uint64_t) * (size_t)n);
     const size_t o_aux = eam ? take(sizeof(double) * 2 * (size_t)n) : 0;
     const size_t o_flag = eam ? take(sizeof(int) * (size_t)std::max(1, n_chunks)) : 0;

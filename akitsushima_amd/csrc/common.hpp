@@ -168,6 +168,27 @@ __device__ __forceinline__ int row_rank(int a, int j, const int* __restrict__ cn
   return found ? below : -1;
 }
 
+// Site gradients in HBM: slot s = a * stride + k of a neighbour list holds
+// G[a, k] = dU_a / dv_ak as the record (x, y, z, 0) at G + kGRec * s.  A row
+// holds every candidate within r_cut + skin and often only a few bonds
+// inside r_cut, so the bond kernels write and the gathers read scattered
+// slots: one 32-byte record costs one memory sector per slot where three
+// component planes cost three.
+constexpr int kGRec = 4;
+__device__ __forceinline__ void g_put(double* __restrict__ G, size_t s, double x, double y,
+                                      double z) {
+  double2* p = reinterpret_cast<double2*>(G + kGRec * s);
+  p[0] = make_double2(x, y);
+  p[1] = make_double2(z, 0.0);
+}
+__device__ __forceinline__ void g_get(const double* __restrict__ G, size_t s, double& x,
+                                      double& y, double& z) {
+  const double2 u = reinterpret_cast<const double2*>(G + kGRec * s)[0];
+  x = u.x;
+  y = u.y;
+  z = G[kGRec * s + 2];
+}
+
 // Lane l's value of x (l wave-uniform).
 __device__ __forceinline__ double readlane_d(double x, int l) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);

@@ -486,10 +486,7 @@ __device__ __forceinline__ void k_tersoff_site_chunk(int c, int n_chunks,
     if (live && k == 0) amask[ac] = al.mask;
   }
   if (has && (L == 4 || act)) {
-    const size_t plane = (size_t)n * stride;
-    G[slot] = Gx;
-    G[plane + slot] = Gy;
-    G[2 * plane + slot] = Gz;
+    g_put(G, slot, Gx, Gy, Gz);
   }
   if (esite) {  // per-atom site energy U_a (uniform branch: every lane joins the sum)
     const double ea = seg_sum<L>(e);
@@ -646,10 +643,7 @@ __device__ __forceinline__ void k_tersoff_q4_chunk(
 #undef AK_Q4_GRAD
   const double Gx = sr * ex + swx, Gy = sr * ey + swy, Gz = sr * ez + swz;
   if (has) {
-    const size_t plane = (size_t)n * 4;
-    G[slot] = Gx;
-    G[plane + slot] = Gy;
-    G[2 * plane + slot] = Gz;
+    g_put(G, slot, Gx, Gy, Gz);
   }
   if (esite) {  // per-atom site energy U_a (uniform branch)
     const double ea = seg_sum<4>(e);
@@ -834,10 +828,7 @@ __device__ __forceinline__ void k_tersoff_trip_chunk(int c, int n_chunks,
     Gy = su * u.y + swy;
     Gz = su * u.z + swz;
     if (has && (L == 4 || u.act)) {
-      const size_t plane = (size_t)n * stride;
-      G[slot] = Gx;
-      G[plane + slot] = Gy;
-      G[2 * plane + slot] = Gz;
+      g_put(G, slot, Gx, Gy, Gz);
     }
   }
   block_partial(e, Gx, Gy, Gz, u.x, u.y, u.z, partial, c, n_chunks);
@@ -948,10 +939,7 @@ __device__ __forceinline__ void k_sw_site_chunk(int c, int n_chunks,
   const double Gx = su * ux + swx, Gy = su * uy + swy, Gz = su * uz + swz;
   // inactive legs have G = 0 exactly; with long rows the gather skips them
   if (has && (L == 4 || leg)) {
-    const size_t plane = (size_t)n * stride;
-    G[slot] = Gx;
-    G[plane + slot] = Gy;
-    G[2 * plane + slot] = Gz;
+    g_put(G, slot, Gx, Gy, Gz);
   }
   if (esite) {  // per-atom site energy U_a (uniform branch: every lane joins the sum)
     const double ea = seg_sum<L>(e);
@@ -1049,27 +1037,27 @@ __global__ __launch_bounds__(256) void k_gather(int n, const int* __restrict__ c
     const int c = cnt[a];
     const int r = rev[s];  // issued alongside cnt[]; used when k < c
     if (k < c && r >= 0) {
-      const size_t plane = (size_t)n * stride;
       const size_t rs = (size_t)r;
       if constexpr (L == 4) {
-        fx = G[s] - G[rs];
-        fy = G[plane + s] - G[plane + rs];
-        fz = G[2 * plane + s] - G[2 * plane + rs];
+        double ax, ay, az, bx, by, bz;
+        g_get(G, s, ax, ay, az);
+        g_get(G, rs, bx, by, bz);
+        fx = ax - bx;
+        fy = ay - by;
+        fz = az - bz;
       } else {
         // rows of stride L: slot r is entry r % L of atom r / L; only the
         // bonds inside r_c were written (the others are zero)
         const bool own = (amask[a] >> k) & 1;
-        if (own) {
-          fx = G[s];
-          fy = G[plane + s];
-          fz = G[2 * plane + s];
-        }
+        if (own) g_get(G, s, fx, fy, fz);
         // sym: the cutoff test is symmetric in the bond, so the reverse bond
         // is active exactly when this one is and its mask need not be read
         if (sym ? own : ((amask[r / L] >> (r & (L - 1))) & 1)) {
-          fx -= G[rs];
-          fy -= G[plane + rs];
-          fz -= G[2 * plane + rs];
+          double bx, by, bz;
+          g_get(G, rs, bx, by, bz);
+          fx -= bx;
+          fy -= by;
+          fz -= bz;
         }
       }
     }
@@ -1183,10 +1171,7 @@ __device__ __forceinline__ void k_sw_site_c_chunk(
     }
     const double Gx = su * ux + swx, Gy = su * uy + swy, Gz = su * uz + swz;
     const size_t slot = (size_t)ac * stride + lslot[reg + t];
-    const size_t plane = (size_t)n * stride;
-    G[slot] = Gx;
-    G[plane + slot] = Gy;
-    G[2 * plane + slot] = Gz;
+    g_put(G, slot, Gx, Gy, Gz);
     v[0] += e;
     v[1] -= Gx * ux;
     v[2] -= Gx * uy;
@@ -1248,7 +1233,6 @@ __global__ __launch_bounds__(256) void k_gather_c(int n, const int* __restrict__
   if (live) {
     const int c = cnt[a];
     const uint64_t ma = amask[a];
-    const size_t plane = (size_t)n * stride;
     // all rounds unrolled so that the reverse-slot, mask and G loads of the
     // rounds are in flight together
     constexpr int R = L / LA;
@@ -1271,15 +1255,18 @@ __global__ __launch_bounds__(256) void k_gather_c(int n, const int* __restrict__
       if (rv[i] < 0) continue;
       const size_t s = (size_t)a * stride + q + LA * i;
       if (own[i]) {
-        fx += G[s];
-        fy += G[plane + s];
-        fz += G[2 * plane + s];
+        double gx, gy, gz;
+        g_get(G, s, gx, gy, gz);
+        fx += gx;
+        fy += gy;
+        fz += gz;
       }
       if (oth[i]) {
-        const size_t r = (size_t)rv[i];
-        fx -= G[r];
-        fy -= G[plane + r];
-        fz -= G[2 * plane + r];
+        double gx, gy, gz;
+        g_get(G, (size_t)rv[i], gx, gy, gz);
+        fx -= gx;
+        fy -= gy;
+        fz -= gz;
       }
     }
   }
@@ -1448,10 +1435,7 @@ __device__ __forceinline__ void k_sw_multi_chunk(
     }
     const double Gx = su * ux + swx, Gy = su * uy + swy, Gz = su * uz + swz;
     const size_t slot = (size_t)ac * stride + lslot[reg + t];
-    const size_t plane = (size_t)n * stride;
-    G[slot] = Gx;
-    G[plane + slot] = Gy;
-    G[2 * plane + slot] = Gz;
+    g_put(G, slot, Gx, Gy, Gz);
     v[0] += e;
     v[1] -= Gx * ux;
     v[2] -= Gx * uy;
@@ -2165,7 +2149,7 @@ void compute_forces(Potential& pot, NList& nl, SetDev& set, const double* d_pos,
       w.aux = pot.aux.as<double>();
     }
   } else if (n > 0) {
-    pot.G.ensure(sizeof(double) * 3 * (size_t)n * nl.stride);
+    pot.G.ensure(sizeof(double) * kGRec * (size_t)n * nl.stride);
     pot.amask.ensure(sizeof(uint64_t) * (size_t)n);
     w.G = pot.G.as<double>();
     w.amask = pot.amask.as<uint64_t>();
@@ -2195,7 +2179,6 @@ __global__ __launch_bounds__(256) void k_eam_site_g(
   const int nk = cnt[a];
   const int sa = ns > 1 ? spc[a] : 0;
   const double dFa = dF[a];
-  const size_t plane = (size_t)n * stride;
   for (int k = l; k < nk; k += LE) {
     const size_t slot = (size_t)a * stride + k;
     const int j = nbr[slot];
@@ -2213,9 +2196,7 @@ __global__ __launch_bounds__(256) void k_eam_site_g(
       gy = s * uy;
       gz = s * uz;
     }
-    G[slot] = gx;
-    G[plane + slot] = gy;
-    G[2 * plane + slot] = gz;
+    g_put(G, slot, gx, gy, gz);
   }
 }
 
@@ -2253,7 +2234,6 @@ __global__ __launch_bounds__(256) void k_heat_flux(
     const double sc = vscale ? vscale[0] : 1.0;
     const Geom& g = geoms[single ? 0 : sid[a]];
     const int nk = cnt[a];
-    const size_t plane = (size_t)n * stride;
     const uint64_t ma = masked ? amask[a] : ~uint64_t(0);
     for (int k = l; k < nk; k += LE) {
       if (masked && !((ma >> k) & 1)) continue;
@@ -2263,8 +2243,10 @@ __global__ __launch_bounds__(256) void k_heat_flux(
       mic_fast(g, pos, a, j, ux, uy, uz);
       const double r2 = ux * ux + uy * uy + uz * uz;
       if (!(r2 < rc * rc) || !(r2 > 1e-24)) continue;
-      const double gv = (G[slot] * vel[3 * (size_t)j] + G[plane + slot] * vel[3 * (size_t)j + 1] +
-                         G[2 * plane + slot] * vel[3 * (size_t)j + 2]) *
+      double gx, gy, gz;
+      g_get(G, slot, gx, gy, gz);
+      const double gv = (gx * vel[3 * (size_t)j] + gy * vel[3 * (size_t)j + 1] +
+                         gz * vel[3 * (size_t)j + 2]) *
                         sc;
       J[0] -= ux * gv;
       J[1] -= uy * gv;
@@ -2348,7 +2330,6 @@ __global__ __launch_bounds__(256) void k_heat_flux_groups(
     const double sc = vscale ? vscale[0] : 1.0;
     const Geom& g = geoms[single ? 0 : sid[a]];
     const int nk = cnt[a];
-    const size_t plane = (size_t)n * stride;
     const double vx = vel[3 * (size_t)a] * sc, vy = vel[3 * (size_t)a + 1] * sc,
                  vz = vel[3 * (size_t)a + 2] * sc;
     for (int k = l; k < nk; k += LE) {
@@ -2360,7 +2341,9 @@ __global__ __launch_bounds__(256) void k_heat_flux_groups(
       mic_fast(g, pos, a, nbr[slot], ux, uy, uz);
       const double r2 = ux * ux + uy * uy + uz * uz;
       if (!(r2 < rc * rc) || !(r2 > 1e-24)) continue;
-      const double gv = G[r] * vx + G[plane + r] * vy + G[2 * plane + r] * vz;
+      double gx, gy, gz;
+      g_get(G, (size_t)r, gx, gy, gz);
+      const double gv = gx * vx + gy * vy + gz * vz;
       J[0] += ux * gv;
       J[1] += uy * gv;
       J[2] += uz * gv;
@@ -2443,7 +2426,7 @@ void site_grads(Potential& pot, NList& nl, SetDev& set, const double* d_pos, con
   compute_forces(pot, nl, set, d_pos, d_spc, d_F, d_out, st, d_esite);
   if (pot.kind != KIND_EAM || set.n_atoms == 0) return;
   const int n = set.n_atoms;
-  pot.G.ensure(sizeof(double) * 3 * (size_t)n * nl.stride);
+  pot.G.ensure(sizeof(double) * kGRec * (size_t)n * nl.stride);
   constexpr int LE = 8;
   const unsigned grid = (unsigned)(((int64_t)n * LE + 255) / 256);
   hipLaunchKernelGGL(k_eam_site_g<LE>, dim3(grid), dim3(256), 0, st, n, d_pos,

@@ -158,7 +158,7 @@ int ak_compute_device(ak_potential *pot, ak_nlist *nl, int32_t n_atoms,
  * d_species is required, and the kernels' scratch (geometry, launch chunks,
  * block partials, bond gradients, EAM embedding derivatives) lives in one
  * 64 MiB device array of the library, per GPU, held until the process
- * ends; inputs that need more (roughly 24 * N * list stride bytes) fail
+ * ends; inputs that need more (roughly 32 * N * list stride bytes) fail
  * with AK_ERR_CAPACITY.  Every call on a GPU shares that GPU's array:
  * calls that overlap on one GPU (other streams or threads) race. */
 int ak_compute_device_noalloc(ak_potential *pot, ak_nlist *nl, int32_t n_atoms,

@@ -129,7 +129,7 @@ def test_noalloc_tersoff_matches_oracle():
 
 def test_noalloc_refuses_inputs_beyond_its_scratch():
     # 64000 atoms with a 2 A skin: 34 neighbours within 5.77 A, rows of 64
-    # slots, about 100 MB of bond gradients
+    # slots, about 130 MB of bond gradients
     lat, pos, t = si(20)
     pot = StillingerWeberPotential(SW_SI, species=["Si"])
     nl = NeighbourList(pot.r_cut, skin_width=2.0)
