@@ -508,20 +508,78 @@ __global__ __launch_bounds__(256) void k_tersoff_site(int n_chunks, int n, const
 }
 
 // ---------------------------------------------------------------------------
+// Reciprocal and reciprocal square root from the hardware estimates with two
+// Newton steps each (a few ulp; the correctly rounded division and square
+// root cost about three times the instructions).  x > 0 and finite.
+__device__ __forceinline__ double fast_rcp(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+__device__ __forceinline__ double fast_rsq(double x) {
+  const double hx = 0.5 * x;
+  double y = __builtin_amdgcn_rsq(x);
+  y *= fma(-hx, y * y, 1.5);
+  y *= fma(-hx, y * y, 1.5);
+  return y;
+}
+
+// Wavefront totals of 8 values per lane by recursive halving: three exchange
+// steps inside each group of 8 lanes (row half-mirror, quad xor 1, quad
+// xor 2), each keeping half of a lane's values and adding the partner
+// lane's copy of that half, then the groups summed (row rotation by 8, two
+// cross-row shuffles).  Lane l < 8 holds the total of value
+// idx = 4 b2 + 2 b0 + b1 (b the bits of l).  Fixed order.
+__device__ __forceinline__ double wave_sums8(const double (&v)[8], int& idx) {
+  const int lane = __lane_id();
+  double w[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) w[i] = v[i];
+  auto step = [&](auto partner, int H, bool upper) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i < H) {
+        const double keep = upper ? w[H + i] : w[i];
+        const double send = upper ? w[i] : w[H + i];
+        w[i] = keep + partner(send);
+      }
+    }
+  };
+  step([](double x) { return dpp_row<0x141>(x); }, 4, (lane & 4) != 0);
+  step([](double x) { return dpp_quad<0xB1>(x); }, 2, (lane & 1) != 0);
+  step([](double x) { return dpp_quad<0x4E>(x); }, 1, (lane & 2) != 0);
+  double t = w[0];
+  t += dpp_row<0x128>(t);
+  t += __shfl_xor(t, 16, kWave);
+  t += __shfl_xor(t, 32, kWave);
+  idx = ((lane & 4)) | ((lane & 1) << 1) | ((lane >> 1) & 1);
+  return t;
+}
+
 // Tersoff site kernel for the flat one-species form with rows of four slots
 // (diamond-like solids, list skin below the second shell): m = 3, gamma = 1
 // as the reference's flat path hard-codes them (engine/tersoff.py:86, 245,
-// 494, 1431).  One DPP quad per atom, lane k = bond k.  The partner legs
-// arrive by quad rotations (lane k reads lane k+P mod 4, P = 1, 2, 3), so
-// every lane evaluates the same three partner terms with no self slot and
-// no out-of-line calls.  g(theta), g'(theta) and the two exponential
-// factors of each (bond, partner) pair are formed once: the factor of the
-// partner's zeta in which this bond is the third leg is the partner lane's
-// own factor for this bond, brought over by the inverse rotation.  Block
-// sums of energy and virial as in the other bond kernels (wave_partial).
-// Terms and gating are
-// those of k_tersoff_site; only the summation order of the zeta sums and of
-// the bond gradients differs.
+// 494, 1431).  One DPP quad per atom, lane k = bond k, one chunk of
+// blockDim / 4 atoms per block.  The partner legs arrive by quad rotations
+// (lane k reads lane k+P mod 4, P = 1, 2, 3), so every lane evaluates the
+// same three partner terms with no self slot and no out-of-line calls.
+// g(theta), g'(theta) and the two exponential factors of each (bond,
+// partner) pair are formed once: the factor of the partner's zeta in which
+// this bond is the third leg is the partner lane's own factor for this bond,
+// brought over by the inverse rotation.  Terms and gating are those of
+// k_tersoff_site; the summation order of the zeta sums and the bond
+// gradients differs, and 1 / r, |r| and the three 1 / (d^2 + (h - cos)^2)
+// come from the hardware estimates refined by Newton steps.
+//
+// Chunk c is atoms [chunk_rng[2c], chunk_rng[2c + 1]) of one structure, so
+// the cell geometry is uniform over the block (scalar loads); with a single
+// structure the chunk is c * blockDim / 4 onwards and chunk_rng is not read.
+// Row length and neighbour index are loaded together.  Block partials: the
+// energy and the six components of the symmetrised virial (the site energy
+// of each atom is rotation invariant, so sum_k u_k (x) G_k is symmetric per
+// atom), written as the ten values the structure reduction reads.
 // ---------------------------------------------------------------------------
 
 // Lane k of every quad receives lane (k + P) mod 4 of its quad.
@@ -533,37 +591,39 @@ __device__ __forceinline__ double qrot(double x) {
 }
 
 template <bool L3Z>
-__device__ __forceinline__ void k_tersoff_q4_chunk(
-    int c, int n_chunks, int n, const double* __restrict__ pos, const int* __restrict__ sid,
+__global__ __launch_bounds__(256) void k_tersoff_q4(
+    int n_chunks, int n, const double* __restrict__ pos, const int* __restrict__ sid,
     const Geom* __restrict__ geoms, int single, const double* __restrict__ tb,
     const double* __restrict__ tri, double rc, const int* __restrict__ cnt,
     const int* __restrict__ nbr, const int* __restrict__ chunk_rng, double* __restrict__ G,
     double* __restrict__ partial, double* __restrict__ esite) {
-  const int a = chunk_rng[2 * c] + (int)threadIdx.x / 4;
+  const int c = xcd_remap((int)blockIdx.x, n_chunks);
+  const int apb = (int)blockDim.x >> 2;
+  const int a0 = single ? c * apb : chunk_rng[2 * c];
+  const int a1 = single ? min(a0 + apb, n) : chunk_rng[2 * c + 1];
+  const Geom& g = geoms[single ? 0 : sid[a0]];
+  const int a = a0 + (int)threadIdx.x / 4;
   const int k = threadIdx.x & 3;
-  const bool live = a < chunk_rng[2 * c + 1];
-  const int ac = live ? a : 0;
-  const int nk = live ? cnt[ac] : 0;
-  const bool has = live && k < nk;
+  const bool live = a < a1;
+  const int ac = live ? a : a0;
   const size_t slot = (size_t)ac * 4 + k;
-  const int jr = live ? nbr[slot] : 0;
+  const int nk = cnt[ac];
+  const int jr = nbr[slot];
+  const bool has = live && k < nk;
   const int j = has ? jr : ac;
   // one record: tb = [valid, A, B, lambda1, lambda2, R, D, beta, n],
   // tri = [valid, c, d, h, lambda3, m, gamma]
   const double A = tb[1], B = tb[2], l1 = tb[3], l2 = tb[4], R = tb[5], D = tb[6];
   const double beta = tb[7], nn = tb[8];
   const double cc = tri[1], dd = tri[2], hh = tri[3], l3 = tri[4];
-  const double c2 = cc * cc, d2 = dd * dd, g0 = 1.0 + c2 / d2, l3c = l3 * l3 * l3;
+  const double c2 = cc * cc, d2 = dd * dd, g0 = 1.0 + c2 * fast_rcp(d2), l3c = l3 * l3 * l3;
 
-  double ux = 0.0, uy = 0.0, uz = 0.0, r = 0.0, rinv = 0.0;
-  bool act = false;
-  if (has) {
-    const Geom& g = geoms[single ? 0 : sid[ac]];
-    mic_fast(g, pos, ac, j, ux, uy, uz);
-    r = sqrt(ux * ux + uy * uy + uz * uz);
-    act = r < rc && r > 1e-12;
-    rinv = act ? 1.0 / r : 0.0;
-  }
+  double ux, uy, uz;
+  mic_fast(g, pos, ac, j, ux, uy, uz);
+  const double r2 = ux * ux + uy * uy + uz * uz;
+  const bool act = has && r2 > 1e-24 && r2 < rc * rc;
+  const double rinv = act ? fast_rsq(r2) : 0.0;
+  const double r = r2 * rinv;
   // unit vector (zero for slots outside r_c or past the row)
   const double ex = ux * rinv, ey = uy * rinv, ez = uz * rinv;
   double fc = 0.0, dfc = 0.0;
@@ -579,7 +639,7 @@ __device__ __forceinline__ void k_tersoff_q4_chunk(
     const double wfc = qrot<P>(fc);                                       \
     cs[P] = ex * wx + ey * wy + ez * wz;                                  \
     const double hc = hh - cs[P];                                         \
-    const double iq = 1.0 / (d2 + hc * hc);                               \
+    const double iq = fast_rcp(d2 + hc * hc);                             \
     gv[P] = g0 - c2 * iq;                                                 \
     dg[P] = -2.0 * c2 * hc * iq * iq;                                     \
     if constexpr (L3Z) {                                                  \
@@ -598,12 +658,14 @@ __device__ __forceinline__ void k_tersoff_q4_chunk(
   AK_Q4_PARTNER(3)
 #undef AK_Q4_PARTNER
 
-  // bond order and the two-body term (engine/tersoff.py:1380-1414)
+  // bond order and the two-body term (engine/tersoff.py:1380-1414); with
+  // bz = (beta zeta)^n, b = (1 + bz)^(-1 / 2n): log(1 + bz) in place of
+  // log1p(bz) changes b by less than an ulp of 1
   double b = 1.0, db = 0.0;
   if (act && zeta > 0.0) {
     const double bz = exp(nn * log(beta * zeta));
-    b = exp((-0.5 / nn) * log1p(bz));
-    db = -0.5 * b * bz / (zeta * (1.0 + bz));
+    b = exp(-0.5 * fast_rcp(nn) * log(1.0 + bz));
+    db = -0.5 * b * bz * fast_rcp(zeta * (1.0 + bz));
   }
   double e = 0.0, dVdr = 0.0, cown = 0.0;
   if (act) {
@@ -649,21 +711,25 @@ __device__ __forceinline__ void k_tersoff_q4_chunk(
     const double ea = seg_sum<4>(e);
     if (live && k == 0) esite[a] = ea;
   }
-  double v[10] = {e,        -Gx * ux, -Gx * uy, -Gx * uz, -Gy * ux,
-                  -Gy * uy, -Gy * uz, -Gz * ux, -Gz * uy, -Gz * uz};
-  wave_partial(v, partial, c, n_chunks);
-}
-
-template <bool L3Z>
-__global__ __launch_bounds__(256) void k_tersoff_q4(
-    int n_chunks, int n, const double* __restrict__ pos, const int* __restrict__ sid,
-    const Geom* __restrict__ geoms, int single, const double* __restrict__ tb,
-    const double* __restrict__ tri, double rc, const int* __restrict__ cnt,
-    const int* __restrict__ nbr, const int* __restrict__ chunk_rng, double* __restrict__ G,
-    double* __restrict__ partial, double* __restrict__ esite) {
-  for (int cc = blockIdx.x; cc < n_chunks; cc += gridDim.x)
-    k_tersoff_q4_chunk<L3Z>(xcd_remap(cc, n_chunks), n_chunks, n, pos, sid, geoms, single, tb,
-                            tri, rc, cnt, nbr, chunk_rng, G, partial, esite);
+  // energy and -sum G (x) u, symmetrised: xy = -(Gx uy + Gy ux) / 2
+  double v[8] = {e,        -Gx * ux,           -0.5 * (Gx * uy + Gy * ux),
+                 -0.5 * (Gx * uz + Gz * ux), -Gy * uy, -0.5 * (Gy * uz + Gz * uy),
+                 -Gz * uz, 0.0};
+  __shared__ double sm[8][4];
+  int idx;
+  const double t = wave_sums8(v, idx);
+  const int wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) < 8) sm[idx][wid] = t;
+  __syncthreads();
+  if (threadIdx.x < 10) {
+    // value q of the ten (e, xx, xy, xz, yx, yy, yz, zx, zy, zz) from the
+    // seven sums (e, xx, xy, xz, yy, yz, zz)
+    // (4-bit fields of 0x6535423210, field q = source of value q)
+    const int q = (int)((0x6535423210ull >> (4 * threadIdx.x)) & 0xF);
+    double s = sm[q][0];
+    for (int w = 1; w < nw; ++w) s += sm[q][w];
+    partial[(size_t)threadIdx.x * n_chunks + c] = s;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1914,10 +1980,12 @@ static void launch_bond_kernels(Potential& pot, NList& nl, const ForceWork& w,
       }
     }
     if constexpr (L == 4) {
-      // the flat form (one valid record, m = 3, gamma = 1) on rows of four
+      // the flat form (one valid record, m = 3, gamma = 1) on rows of four:
+      // one block per chunk (the grid cap does not apply)
       const double* hp = pot.h_params.data();
       if (!done && sp1 && hp[0] != 0.0 && hp[9] != 0.0 && hp[14] == 3.0 && hp[15] == 1.0) {
-        launch_timed(pot, hp[13] == 0.0 ? k_tersoff_q4<true> : k_tersoff_q4<false>, grid, block,
+        launch_timed(pot, hp[13] == 0.0 ? k_tersoff_q4<true> : k_tersoff_q4<false>,
+                     dim3(n_chunks > 0 ? n_chunks : 1), block,
                      st, (size_t)0, n_chunks, n, d_pos, sid, geoms, single, tb, tri, pot.r_cut, cnt,
                      nbr, rng, G, part, esite);
         done = true;
