@@ -558,6 +558,29 @@ __device__ __forceinline__ double wave_sums8(const double (&v)[8], int& idx) {
   return t;
 }
 
+// Block sums of the energy and a symmetric virial (e, xx, xy, xz, yy, yz, zz;
+// v[7] unused) of launch chunk c, written as the ten values (e, xx, xy, xz,
+// yx, yy, yz, zx, zy, zz) the structure reduction reads: wavefront totals
+// by wave_sums8, one LDS exchange, combined in fixed order.  Every thread of
+// the block calls it once (one chunk per block).
+__device__ __forceinline__ void block_partial7(const double (&v)[8], double* __restrict__ partial,
+                                               int c, int n_chunks) {
+  __shared__ double sm[8][4];
+  int idx;
+  const double t = wave_sums8(v, idx);
+  const int wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) < 8) sm[idx][wid] = t;
+  __syncthreads();
+  if (threadIdx.x < 10) {
+    // value q of the ten from the seven sums: 4-bit fields of 0x6535423210,
+    // field q = source of value q
+    const int q = (int)((0x6535423210ull >> (4 * threadIdx.x)) & 0xF);
+    double s = sm[q][0];
+    for (int w = 1; w < nw; ++w) s += sm[q][w];
+    partial[(size_t)threadIdx.x * n_chunks + c] = s;
+  }
+}
+
 // Tersoff site kernel for the flat one-species form with rows of four slots
 // (diamond-like solids, list skin below the second shell): m = 3, gamma = 1
 // as the reference's flat path hard-codes them (engine/tersoff.py:86, 245,
@@ -715,21 +738,7 @@ __global__ __launch_bounds__(256) void k_tersoff_q4(
   double v[8] = {e,        -Gx * ux,           -0.5 * (Gx * uy + Gy * ux),
                  -0.5 * (Gx * uz + Gz * ux), -Gy * uy, -0.5 * (Gy * uz + Gz * uy),
                  -Gz * uz, 0.0};
-  __shared__ double sm[8][4];
-  int idx;
-  const double t = wave_sums8(v, idx);
-  const int wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  if ((threadIdx.x & 63) < 8) sm[idx][wid] = t;
-  __syncthreads();
-  if (threadIdx.x < 10) {
-    // value q of the ten (e, xx, xy, xz, yx, yy, yz, zx, zy, zz) from the
-    // seven sums (e, xx, xy, xz, yy, yz, zz)
-    // (4-bit fields of 0x6535423210, field q = source of value q)
-    const int q = (int)((0x6535423210ull >> (4 * threadIdx.x)) & 0xF);
-    double s = sm[q][0];
-    for (int w = 1; w < nw; ++w) s += sm[q][w];
-    partial[(size_t)threadIdx.x * n_chunks + c] = s;
-  }
+  block_partial7(v, partial, c, n_chunks);
 }
 
 // ---------------------------------------------------------------------------
@@ -1682,59 +1691,74 @@ __device__ __forceinline__ void z_feval(const double* __restrict__ fn,
   dv = dval * S + val * dS;
 }
 
-// Shared-exponential evaluation of the Zhou functions of one pair.  For
-// species s at distance r (x = r / r_e):
+// Zhou functions of one pair from a per-species table (the alloy form
+// checked by eam_fast_table).  For species s at distance r (x = r / r_e):
 //   density  f   = f_e e^{-beta (x-1)} / (1 + (x - lambda)^20)
 //   pair     phi = A e^{-alpha (x-1)} / (1 + (x - kappa)^20)
 //                - B e^{-beta (x-1)} / (1 + (x - lambda)^20)
-// share e^{-beta (x-1)} and the lambda denominator: two exponentials per
-// species instead of one per function term (a cross pair needs both species'
-// densities and pair terms: 4 instead of 8).  Every value is formed with the
-// operations, in the order, of z_decay / z_feval.
-// Fast table: per species (ZF_SP doubles) r_e, f_e, beta, lambda, A, alpha,
-// kappa, B, density-row r_c and width; per ordered pair (sa, sj) (ZF_PR
-// doubles) code (4 pair, 5 cross), r_c, width, chi, swap (the cross row's
-// first species is sj).
-constexpr int ZF_SP = 10, ZF_PR = 5;
+// e^{-beta (x-1)} and the lambda denominator are shared: two exponentials
+// per species instead of one per function term (a cross pair needs both
+// species' densities and pair terms: 4 instead of 8).  Divisions by r_e and
+// by the switch widths are products with reciprocals kept in the table; the
+// two denominators and the cross pair's density ratio use the hardware
+// reciprocal with Newton steps.  The values agree with z_decay / z_feval to
+// a few ulp.
+// Table: per species (ZF_SP doubles) r_e, f_e, beta, lambda, A, alpha,
+// kappa, B, density-row r_c and switch width, 1 / r_e, 1 / width; per
+// ordered pair (sa, sj) (ZF_PR doubles) code (4 pair, 5 cross), r_c, width,
+// chi, swap (the cross row's first species is sj), 1 / width.
+constexpr int ZF_SP = 12, ZF_PR = 6;
 
 struct ZBase {
   double fv, fd, pv, pd;  // unswitched density and pair value, derivative
 };
 
+// (x - c)^19 and (x - c)^20
+__device__ __forceinline__ void z_pow20(double u, double& u19, double& u20) {
+  const double u2 = u * u, u4 = u2 * u2, u8 = u4 * u4, u16 = u8 * u8;
+  u19 = u16 * u2 * u;
+  u20 = u19 * u;
+}
+
 __device__ __forceinline__ ZBase z_base(const double* __restrict__ sp, double r) {
-  const double re = sp[0], fe = sp[1], beta = sp[2], lam = sp[3], A = sp[4], alpha = sp[5],
-               kap = sp[6], B = sp[7];
-  const double x = r / re;
-  const double Eb = exp(-beta * (x - 1.0));
-  const double Ea = exp(-alpha * (x - 1.0));
-  double ul = x - lam, uk = x - kap;
-  double ul2 = ul * ul, ul4 = ul2 * ul2, ul8 = ul4 * ul4, ul16 = ul8 * ul8;
-  double ul19 = ul16 * ul2 * ul;
-  double idl = 1.0 / (1.0 + ul19 * ul);
-  double uk2 = uk * uk, uk4 = uk2 * uk2, uk8 = uk4 * uk4, uk16 = uk8 * uk8;
-  double uk19 = uk16 * uk2 * uk;
-  double idk = 1.0 / (1.0 + uk19 * uk);
+  const double fe = sp[1], beta = sp[2], lam = sp[3], A = sp[4], alpha = sp[5], kap = sp[6],
+               B = sp[7], ire = sp[10];
+  const double x = r * ire, xm = x - 1.0;
+  const double Eb = exp(-beta * xm);
+  const double Ea = exp(-alpha * xm);
+  double ul19, ul20, uk19, uk20;
+  z_pow20(x - lam, ul19, ul20);
+  z_pow20(x - kap, uk19, uk20);
+  const double idl = fast_rcp(1.0 + ul20), idk = fast_rcp(1.0 + uk20);
+  const double tl = 20.0 * ul19 * idl, tk = 20.0 * uk19 * idk;  // -d ln(den) / dx
   ZBase b;
   const double ef = fe * Eb;
   b.fv = ef * idl;
-  b.fd = (-beta * b.fv - ef * 20.0 * ul19 * idl * idl) / re;
-  const double eA = A * Ea;
-  const double vA = eA * idk;
-  const double dA = (-alpha * vA - eA * 20.0 * uk19 * idk * idk) / re;
-  const double eB = B * Eb;
-  const double vB = eB * idl;
-  const double dB = (-beta * vB - eB * 20.0 * ul19 * idl * idl) / re;
+  b.fd = -(beta + tl) * b.fv * ire;
+  const double vA = A * Ea * idk, vB = B * Eb * idl;
   b.pv = vA - vB;
-  b.pd = dA - dB;
+  b.pd = (-(alpha + tk) * vA + (beta + tl) * vB) * ire;
   return b;
 }
 
-__device__ __forceinline__ void z_switch(double r, double rc, double w, double val, double dval,
-                                         double& v, double& dv) {
-  double s = (r - (rc - w)) / w;
+// Switched density of species s at r (value only).
+__device__ __forceinline__ double z_dens(const double* __restrict__ sp, double r) {
+  const double fe = sp[1], beta = sp[2], lam = sp[3], ire = sp[10];
+  const double x = r * ire;
+  double ul19, ul20;
+  z_pow20(x - lam, ul19, ul20);
+  const double v = fe * exp(-beta * (x - 1.0)) * fast_rcp(1.0 + ul20);
+  double t = (r - (sp[8] - sp[9])) * sp[11];
+  t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+  return v * (1.0 - t * t * t * (10.0 - 15.0 * t + 6.0 * t * t));
+}
+
+__device__ __forceinline__ void z_switch(double r, double rc, double w, double iw, double val,
+                                         double dval, double& v, double& dv) {
+  double s = (r - (rc - w)) * iw;
   s = s < 0.0 ? 0.0 : (s > 1.0 ? 1.0 : s);
-  double S = 1.0 - s * s * s * (10.0 - 15.0 * s + 6.0 * s * s);
-  double dS = -30.0 * s * s * (1.0 - s) * (1.0 - s) / w;
+  const double S = 1.0 - s * s * s * (10.0 - 15.0 * s + 6.0 * s * s);
+  const double dS = -30.0 * s * s * (1.0 - s) * (1.0 - s) * iw;
   v = val * S;
   dv = dval * S + val * dS;
 }
@@ -1747,27 +1771,26 @@ __device__ __forceinline__ void z_pair_fast(const double* __restrict__ fast, int
   const double* spa = fast + (size_t)ZF_SP * sa;
   const double* pr = fast + (size_t)ZF_SP * ns + (size_t)ZF_PR * (sa * ns + sj);
   const ZBase bj = z_base(spj, r);
-  const ZBase ba = sa == sj ? bj : z_base(spa, r);
   double v;
-  z_switch(r, spj[8], spj[9], bj.fv, bj.fd, v, dfj);
-  if (sa == sj)
-    dfa = dfj;
-  else
-    z_switch(r, spa[8], spa[9], ba.fv, ba.fd, v, dfa);
+  z_switch(r, spj[8], spj[9], spj[11], bj.fv, bj.fd, v, dfj);
   double val, dval;
-  if (pr[0] == 4.0) {
+  if (sa == sj) {
+    dfa = dfj;
     val = bj.pv;
     dval = bj.pd;
   } else {
+    const ZBase ba = z_base(spa, r);
+    z_switch(r, spa[8], spa[9], spa[11], ba.fv, ba.fd, v, dfa);
     const ZBase& X = pr[4] != 0.0 ? bj : ba;  // species of the row's first block
     const ZBase& Y = pr[4] != 0.0 ? ba : bj;
     const double kc = pr[3];
-    double q = Y.fv / X.fv;
-    double dq = (Y.fd * X.fv - Y.fv * X.fd) / (X.fv * X.fv);
-    val = 0.5 * kc * (q * X.pv + Y.pv / q);
-    dval = 0.5 * kc * (dq * X.pv + q * X.pd + Y.pd / q - Y.pv * dq / (q * q));
+    const double iX = fast_rcp(X.fv), iY = fast_rcp(Y.fv);
+    const double q = Y.fv * iX, iq = X.fv * iY;
+    const double dq = (Y.fd * X.fv - Y.fv * X.fd) * iX * iX;
+    val = 0.5 * kc * (q * X.pv + Y.pv * iq);
+    dval = 0.5 * kc * (dq * X.pv + q * X.pd + Y.pd * iq - Y.pv * dq * iq * iq);
   }
-  z_switch(r, pr[1], pr[2], val, dval, phi, dphi);
+  z_switch(r, pr[1], pr[2], pr[5], val, dval, phi, dphi);
 }
 
 // map: [rho_f[ns], emb_f[ns], phi_f[ns*ns]]
@@ -1817,7 +1840,7 @@ __device__ __forceinline__ void k_eam_force_chunk(int c, int n_chunks,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const double* __restrict__ dF, const double* __restrict__ Fe,
     const int* __restrict__ chunk_rng, double* __restrict__ F, double* __restrict__ partial,
-    const double* __restrict__ fast, double* __restrict__ esite, int* __restrict__ pair_flag) {
+    double* __restrict__ esite, int* __restrict__ pair_flag) {
   const int a = chunk_rng[2 * c] + (int)threadIdx.x / LE;
   const int l = threadIdx.x & (LE - 1);
   const bool live = a < chunk_rng[2 * c + 1];
@@ -1839,16 +1862,12 @@ __device__ __forceinline__ void k_eam_force_chunk(int c, int n_chunks,
       if (r < rc && r > 1e-12) {
         const int sj = ns > 1 ? spc[j] : 0;
         double v, dfj, dfa, phi, dphi;
-        if (fast) {
-          z_pair_fast(fast, ns, sa, sj, r, dfj, dfa, phi, dphi);
-        } else {
-          z_feval(fn, fn + (size_t)ZW * map[sj], r, v, dfj);
-          if (sj == sa)
-            dfa = dfj;
-          else
-            z_feval(fn, fa_rho, r, v, dfa);
-          z_feval(fn, fn + (size_t)ZW * map[2 * ns + sa * ns + sj], r, phi, dphi);
-        }
+        z_feval(fn, fn + (size_t)ZW * map[sj], r, v, dfj);
+        if (sj == sa)
+          dfa = dfj;
+        else
+          z_feval(fn, fa_rho, r, v, dfa);
+        z_feval(fn, fn + (size_t)ZW * map[2 * ns + sa * ns + sj], r, phi, dphi);
         bonded = 1;
         double c = dphi + dFa * dfj + dF[j] * dfa;
         double s = c / r;
@@ -1891,9 +1910,128 @@ __global__ __launch_bounds__(256) void k_eam_force(int n_chunks, const double* _
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const double* __restrict__ dF, const double* __restrict__ Fe,
     const int* __restrict__ chunk_rng, double* __restrict__ F, double* __restrict__ partial,
-    const double* __restrict__ fast, double* __restrict__ esite, int* __restrict__ pair_flag) {
+    double* __restrict__ esite, int* __restrict__ pair_flag) {
   for (int cc = blockIdx.x; cc < n_chunks; cc += gridDim.x)
-    k_eam_force_chunk<LE>(xcd_remap(cc, n_chunks), n_chunks, pos, sid, geoms, single, spc, ns, fn, map, rc, cnt, nbr, stride, dF, Fe, chunk_rng, F, partial, fast, esite, pair_flag);
+    k_eam_force_chunk<LE>(xcd_remap(cc, n_chunks), n_chunks, pos, sid, geoms, single, spc, ns, fn, map, rc, cnt, nbr, stride, dF, Fe, chunk_rng, F, partial, esite, pair_flag);
+}
+
+// ---------------------------------------------------------------------------
+// Zhou alloy EAM on the per-species table (z_base, z_dens): the density pass
+// and the force pass of EAMPotential.compute (engine/eam.py:174-256,
+// engine/_eam_kernels.py:231-307).  One block per structure-aligned chunk of
+// blockDim / LE atoms (the cell geometry is uniform over the block and
+// comes through scalar loads; with a single structure the chunk is
+// c * blockDim / LE onwards), LE lanes per atom striding over its row.
+// Same terms as k_eam_rho / k_eam_force; the virial partials are the six
+// components of the symmetric sum -1/2 sum s u (x) u.
+// ---------------------------------------------------------------------------
+template <int LE>
+__global__ __launch_bounds__(256) void k_eam_rho_z(
+    int n_chunks, int n, const double* __restrict__ pos, const int* __restrict__ sid,
+    const Geom* __restrict__ geoms, int single, const int* __restrict__ spc, int ns,
+    const double* __restrict__ fn, const int* __restrict__ map,
+    const double* __restrict__ fast, double rc, const int* __restrict__ cnt,
+    const int* __restrict__ nbr, int stride, const int* __restrict__ chunk_rng,
+    double* __restrict__ dF, double* __restrict__ Fe) {
+  const int c = xcd_remap((int)blockIdx.x, n_chunks);
+  const int apb = (int)blockDim.x / LE;
+  const int a0 = single ? c * apb : chunk_rng[2 * c];
+  const int a1 = single ? min(a0 + apb, n) : chunk_rng[2 * c + 1];
+  const Geom& g = geoms[single ? 0 : sid[a0]];
+  const int a = a0 + (int)threadIdx.x / LE;
+  const int l = threadIdx.x & (LE - 1);
+  const bool live = a < a1;
+  const int ac = live ? a : a0;
+  const int nk = live ? cnt[ac] : 0;
+  const double rc2 = rc * rc;
+  double rho = 0.0;
+  for (int k = l; k < nk; k += LE) {
+    const int j = nbr[(size_t)ac * stride + k];
+    double ux, uy, uz;
+    mic_fast(g, pos, ac, j, ux, uy, uz);
+    const double r2 = ux * ux + uy * uy + uz * uz;
+    if (r2 < rc2 && r2 > 1e-24) {
+      const int sj = ns > 1 ? spc[j] : 0;
+      rho += z_dens(fast + (size_t)ZF_SP * sj, r2 * fast_rsq(r2));
+    }
+  }
+  rho = seg_sum<LE>(rho);
+  if (live && l == 0) {
+    const int sa = ns > 1 ? spc[a] : 0;
+    double F, d;
+    z_feval(fn, fn + (size_t)ZW * map[ns + sa], rho > 0.0 ? rho : 0.0, F, d);
+    Fe[a] = F;
+    dF[a] = d;
+  }
+}
+
+template <int LE>
+__global__ __launch_bounds__(256) void k_eam_force_z(
+    int n_chunks, int n, const double* __restrict__ pos, const int* __restrict__ sid,
+    const Geom* __restrict__ geoms, int single, const int* __restrict__ spc, int ns,
+    const double* __restrict__ fast, double rc, const int* __restrict__ cnt,
+    const int* __restrict__ nbr, int stride, const double* __restrict__ dF,
+    const double* __restrict__ Fe, const int* __restrict__ chunk_rng, double* __restrict__ F,
+    double* __restrict__ partial, double* __restrict__ esite, int* __restrict__ pair_flag) {
+  const int c = xcd_remap((int)blockIdx.x, n_chunks);
+  const int apb = (int)blockDim.x / LE;
+  const int a0 = single ? c * apb : chunk_rng[2 * c];
+  const int a1 = single ? min(a0 + apb, n) : chunk_rng[2 * c + 1];
+  const Geom& g = geoms[single ? 0 : sid[a0]];
+  const int a = a0 + (int)threadIdx.x / LE;
+  const int l = threadIdx.x & (LE - 1);
+  const bool live = a < a1;
+  const int ac = live ? a : a0;
+  const int nk = live ? cnt[ac] : 0;
+  const int sa = ns > 1 ? spc[ac] : 0;
+  const double dFa = dF[ac];
+  const double rc2 = rc * rc;
+  int bonded = 0;  // a bond inside r_c seen by this lane
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  double e = live && l == 0 ? Fe[ac] : 0.0;
+  double w[6] = {0, 0, 0, 0, 0, 0};  // xx, xy, xz, yy, yz, zz
+  for (int k = l; k < nk; k += LE) {
+    const int j = nbr[(size_t)ac * stride + k];
+    double ux, uy, uz;
+    mic_fast(g, pos, ac, j, ux, uy, uz);
+    const double r2 = ux * ux + uy * uy + uz * uz;
+    if (r2 < rc2 && r2 > 1e-24) {
+      const int sj = ns > 1 ? spc[j] : 0;
+      const double dFj = dF[j];
+      const double rinv = fast_rsq(r2), r = r2 * rinv;
+      double dfj, dfa, phi, dphi;
+      z_pair_fast(fast, ns, sa, sj, r, dfj, dfa, phi, dphi);
+      bonded = 1;
+      const double s = (dphi + dFa * dfj + dFj * dfa) * rinv;
+      fx += s * ux;
+      fy += s * uy;
+      fz += s * uz;
+      e += 0.5 * phi;
+      const double hs = -0.5 * s;
+      w[0] += hs * ux * ux;
+      w[1] += hs * ux * uy;
+      w[2] += hs * ux * uz;
+      w[3] += hs * uy * uy;
+      w[4] += hs * uy * uz;
+      w[5] += hs * uz * uz;
+    }
+  }
+  fx = seg_sum<LE>(fx);
+  fy = seg_sum<LE>(fy);
+  fz = seg_sum<LE>(fz);
+  if (live && l == 0) {
+    F[3 * (size_t)a] = fx;
+    F[3 * (size_t)a + 1] = fy;
+    F[3 * (size_t)a + 2] = fz;
+  }
+  if (esite) {  // per-atom site energy U_a = F_a + 1/2 sum_j phi_aj
+    const double ea = seg_sum<LE>(e);
+    if (live && l == 0) esite[a] = ea;
+  }
+  const double v[8] = {e, w[0], w[1], w[2], w[3], w[4], w[5], 0.0};
+  block_partial7(v, partial, c, n_chunks);
+  const int any = __syncthreads_or(bonded);
+  if (threadIdx.x == 0) pair_flag[c] = any;
 }
 
 // Function f of the table at m arguments (tabulation of a potential).
@@ -2047,6 +2185,22 @@ static void launch_eam(Potential& pot, NList& nl, const ForceWork& w, const doub
   double* dF = w.aux;
   double* Fe = dF + n;
   const int single = w.n_struct == 1 ? 1 : 0;
+  if (pot.eam_fast) {
+    // Zhou alloy table: one block per chunk for both passes (the grid cap
+    // does not apply)
+    const dim3 cg(w.n_chunks > 0 ? w.n_chunks : 1), blk(pot.block);
+    const double* fast = pot.d_eam_fast.as<double>();
+    hipLaunchKernelGGL(k_eam_rho_z<LE>, cg, blk, 0, st, w.n_chunks, n, d_pos, w.sid, w.geoms,
+                       single, d_spc, pot.ns, (const double*)pot.d_params.as<double>(),
+                       (const int*)pot.d_map.as<int>(), fast, pot.r_cut,
+                       (const int*)nl.cnt.as<int>(), (const int*)nl.nbr.as<int>(), nl.stride,
+                       w.rng, dF, Fe);
+    launch_timed(pot, k_eam_force_z<LE>, cg, blk, st, (size_t)0, w.n_chunks, n, d_pos, w.sid,
+                 w.geoms, single, d_spc, pot.ns, fast, pot.r_cut, (const int*)nl.cnt.as<int>(),
+                 (const int*)nl.nbr.as<int>(), nl.stride, (const double*)dF, (const double*)Fe,
+                 w.rng, d_F, w.partial, esite, w.pair_flag);
+    return;
+  }
   hipLaunchKernelGGL(k_eam_rho<LE>, dim3(grid), dim3(256), 0, st, n, d_pos, w.sid, w.geoms,
                      single, d_spc, pot.ns, pot.d_params.as<double>(), pot.d_map.as<int>(),
                      pot.r_cut, nl.cnt.as<int>(), nl.nbr.as<int>(), nl.stride, dF, Fe);
@@ -2054,9 +2208,7 @@ static void launch_eam(Potential& pot, NList& nl, const ForceWork& w, const doub
                (size_t)0, w.n_chunks, d_pos, w.sid, w.geoms, single, d_spc, pot.ns,
                (const double*)pot.d_params.as<double>(), (const int*)pot.d_map.as<int>(),
                pot.r_cut, (const int*)nl.cnt.as<int>(), (const int*)nl.nbr.as<int>(), nl.stride,
-               (const double*)dF, (const double*)Fe, w.rng, d_F, w.partial,
-               (const double*)(pot.eam_fast ? pot.d_eam_fast.as<double>() : nullptr), esite,
-               w.pair_flag);
+               (const double*)dF, (const double*)Fe, w.rng, d_F, w.partial, esite, w.pair_flag);
 }
 
 bool eam_fast_table(int ns, const std::vector<double>& rows, const std::vector<int>& map,
@@ -2081,6 +2233,8 @@ bool eam_fast_table(int ns, const std::vector<double>& rows, const std::vector<i
     o[7] = pa[7];  // B
     o[8] = d[1];
     o[9] = d[2];
+    o[10] = 1.0 / pa[0];
+    o[11] = 1.0 / d[2];
   }
   for (int a = 0; a < ns; ++a)
     for (int b = 0; b < ns; ++b) {
@@ -2092,6 +2246,7 @@ bool eam_fast_table(int ns, const std::vector<double>& rows, const std::vector<i
       o[1] = f[1];
       o[2] = f[2];
       o[3] = f[3];
+      o[5] = 1.0 / f[2];
       if (f[0] == 4.0) {
         if (a != b || !same(f + 4, ea)) return false;
       } else if (f[0] == 5.0) {
