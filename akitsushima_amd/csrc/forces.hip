@@ -1037,18 +1037,35 @@ __global__ __launch_bounds__(256) void k_sw_site(
 
 // Per-structure energy and stress of structure s by one block of 256
 // threads: the chunk partials of s in fixed order; out[s] = [E, W/V (9)].
+// Per-structure energy and stress from the block partials of chunks
+// [first[s], first[s + 1]): each thread sums its chunks in chunk order, U
+// chunks' loads in flight at a time (a structure of many chunks is reduced
+// by one block, and one load round trip per chunk was the cost), then the
+// wavefront sums and the waves in order.  Any multiple of 64 threads up to
+// 1024.
+template <int U>
 __device__ __forceinline__ void struct_reduce_block(int s, const Geom* __restrict__ geoms,
                                                     const int* __restrict__ first,
                                                     int n_chunks,
                                                     const double* __restrict__ partial,
                                                     double* __restrict__ out,
                                                     const int* __restrict__ pair_flag = nullptr) {
-  __shared__ double sm[10][4];
+  __shared__ double sm[10][16];
   const int c0 = first[s], c1 = first[s + 1];
+  const int nt = (int)blockDim.x, nw = nt >> 6;
   double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int c = c0 + threadIdx.x; c < c1; c += 256) {
+  for (int c = c0 + (int)threadIdx.x; c < c1; c += U * nt) {
+    double x[U][10];
 #pragma unroll
-    for (int q = 0; q < 10; ++q) v[q] += partial[(size_t)q * n_chunks + c];
+    for (int u = 0; u < U; ++u) {
+      const int cu = c + u * nt;
+#pragma unroll
+      for (int q = 0; q < 10; ++q) x[u][q] = cu < c1 ? partial[(size_t)q * n_chunks + cu] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int q = 0; q < 10; ++q) v[q] += x[u][q];
   }
   int idx;
   const double t = wave_sums<10>(v, idx);
@@ -1060,22 +1077,23 @@ __device__ __forceinline__ void struct_reduce_block(int s, const Geom* __restric
   // engine/eam.py:236-237); pair_flag holds one flag per chunk
   int any = 0;
   if (pair_flag)
-    for (int c = c0 + threadIdx.x; c < c1; c += 256) any |= pair_flag[c];
+    for (int c = c0 + (int)threadIdx.x; c < c1; c += nt) any |= pair_flag[c];
   const bool none = pair_flag && !__syncthreads_or(any);
   if (threadIdx.x < 10) {
     const int q = threadIdx.x;
-    double t = ((sm[q][0] + sm[q][1]) + sm[q][2]) + sm[q][3];
+    double t = sm[q][0];
+    for (int w = 1; w < nw; ++w) t += sm[q][w];
     out[10 * (size_t)s + q] = none ? 0.0 : (q == 0 ? t : t / geoms[s].volume);
   }
 }
 
-__global__ __launch_bounds__(256) void k_struct_final(const Geom* __restrict__ geoms,
+__global__ __launch_bounds__(1024) void k_struct_final(const Geom* __restrict__ geoms,
                                                       const int* __restrict__ first,
                                                       int n_chunks,
                                                       const double* __restrict__ partial,
                                                       double* __restrict__ out,
                                                       const int* __restrict__ pair_flag) {
-  struct_reduce_block(blockIdx.x, geoms, first, n_chunks, partial, out, pair_flag);
+  struct_reduce_block<4>(blockIdx.x, geoms, first, n_chunks, partial, out, pair_flag);
 }
 
 // F_a = sum_k G[a, k] - G[rev(a, k)].  The first gridDim - atom_blocks
@@ -1099,7 +1117,7 @@ __global__ __launch_bounds__(256) void k_gather(int n, const int* __restrict__ c
   const int lead = (int)gridDim.x - atom_blocks;
   if ((int)blockIdx.x < lead) {
     if ((int)blockIdx.x < n_struct)
-      struct_reduce_block((int)blockIdx.x, geoms, first, n_chunks, partial, out);
+      struct_reduce_block<2>((int)blockIdx.x, geoms, first, n_chunks, partial, out);
     return;
   }
   const int gt = xcd_remap((int)blockIdx.x - lead, atom_blocks) * blockDim.x + threadIdx.x;
@@ -1297,7 +1315,7 @@ __global__ __launch_bounds__(256) void k_gather_c(int n, const int* __restrict__
   const int lead = (int)gridDim.x - atom_blocks;
   if ((int)blockIdx.x < lead) {
     if ((int)blockIdx.x < n_struct)
-      struct_reduce_block((int)blockIdx.x, geoms, first, n_chunks, partial, out);
+      struct_reduce_block<2>((int)blockIdx.x, geoms, first, n_chunks, partial, out);
     return;
   }
   const int gt = xcd_remap((int)blockIdx.x - lead, atom_blocks) * blockDim.x + threadIdx.x;
@@ -2322,7 +2340,7 @@ void compute_forces_work(Potential& pot, NList& nl, const ForcePlan& pl, const F
     }
   }
   if (pot.kind == KIND_EAM || n == 0)
-    hipLaunchKernelGGL(k_struct_final, dim3(w.n_struct), dim3(256), 0, st, w.geoms, w.first,
+    hipLaunchKernelGGL(k_struct_final, dim3(w.n_struct), dim3(1024), 0, st, w.geoms, w.first,
                        w.n_chunks, (const double*)w.partial, d_out,
                        pot.kind == KIND_EAM && n > 0 ? (const int*)w.pair_flag
                                                      : (const int*)nullptr);
