@@ -126,9 +126,9 @@ __device__ __forceinline__ void sw_leg(double r, double rinv, double eps, double
                                        double inv_sig, double Asw, double Bsw, double pp,
                                        double qq, double asw, double gam, double& e2, double& su,
                                        double& dlnE) {
-  const double x = r / sig;
+  const double x = r * inv_sig;
   const double xi = sig * rinv;
-  const double t = 1.0 / (x - asw);
+  const double t = fast_rcp(x - asw);
   dlnE = -gam * inv_sig * (t * t);
   const double ex = nb_exp(t);
   const double bx = Bsw * xpow_r(x, xi, -pp) - xpow_r(x, xi, -qq);
@@ -508,24 +508,6 @@ __global__ __launch_bounds__(256) void k_tersoff_site(int n_chunks, int n, const
 }
 
 // ---------------------------------------------------------------------------
-// Reciprocal and reciprocal square root from the hardware estimates with two
-// Newton steps each (a few ulp; the correctly rounded division and square
-// root cost about three times the instructions).  x > 0 and finite.
-__device__ __forceinline__ double fast_rcp(double x) {
-  double y = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, y, 1.0);
-  y = fma(y, e, y);
-  e = fma(-x, y, 1.0);
-  return fma(y, e, y);
-}
-__device__ __forceinline__ double fast_rsq(double x) {
-  const double hx = 0.5 * x;
-  double y = __builtin_amdgcn_rsq(x);
-  y *= fma(-hx, y * y, 1.5);
-  y *= fma(-hx, y * y, 1.5);
-  return y;
-}
-
 // Wavefront totals of 8 values per lane by recursive halving: three exchange
 // steps inside each group of 8 lanes (row half-mirror, quad xor 1, quad
 // xor 2), each keeping half of a lane's values and adding the partner
@@ -1176,13 +1158,15 @@ __global__ __launch_bounds__(256) void k_gather(int n, const int* __restrict__ c
 // the leg's slot.  Dynamic LDS: (block / LA) * L legs of 5 doubles + 1 int.
 // ---------------------------------------------------------------------------
 template <int L, int LA>
-__device__ __forceinline__ void k_sw_site_c_chunk(
-    int c, int n_chunks, int n, const double* __restrict__ pos, const int* __restrict__ sid,
+__global__ __launch_bounds__(256) void k_sw_site_c(
+    int n_chunks, int n, const double* __restrict__ pos, const int* __restrict__ sid,
     const Geom* __restrict__ geoms, int single, const double* __restrict__ prm,
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
-    uint64_t* __restrict__ amask, double* __restrict__ lds, double* __restrict__ esite) {
+    uint64_t* __restrict__ amask, double* __restrict__ esite) {
   static_assert(L >= LA && L % LA == 0 && LA <= 64, "LA lanes cover the row in L / LA rounds");
+  extern __shared__ double lds[];
+  const int c = xcd_remap((int)blockIdx.x, n_chunks);
   const int apb = blockDim.x / LA;          // atoms per block
   const int al = (int)threadIdx.x / LA;     // atom of this lane within the block
   const int q = threadIdx.x & (LA - 1);
@@ -1194,31 +1178,35 @@ __device__ __forceinline__ void k_sw_site_c_chunk(
   double* lE = lri + apb * L;
   int* lslot = (int*)(lE + apb * L);
   const int reg = al * L;
-  const int a = chunk_rng[2 * c] + al;
-  const bool live = a < chunk_rng[2 * c + 1];
-  const int ac = live ? a : 0;
+  const int a0 = single ? c * apb : chunk_rng[2 * c];
+  const int a1 = single ? min(a0 + apb, n) : chunk_rng[2 * c + 1];
+  const Geom& g = geoms[single ? 0 : sid[a0]];
+  const int a = a0 + al;
+  const bool live = a < a1;
+  const int ac = live ? a : a0;
   const int nk = live ? cnt[ac] : 0;
   const double eps = prm[0], sig = prm[1], Asw = prm[2], Bsw = prm[3], pp = prm[4],
                qq = prm[5], asw = prm[6], lam = prm[7], gam = prm[8], cth0 = prm[9];
   const double rc = asw * sig;
-  const double inv_sig = 1.0 / sig;
-  const Geom& g = geoms[single ? 0 : sid[ac]];
+  const double inv_sig = fast_rcp(sig);
 
-  // phase 1: active legs, compacted in slot order
+  // phase 1: active legs, compacted in slot order (the row's indices are
+  // loaded for every round at once)
+  int jr[L / LA];
+#pragma unroll
+  for (int i = 0; i < L / LA; ++i) jr[i] = nbr[(size_t)ac * stride + q + LA * i];
   int count = 0;
   uint64_t mask = 0;
 #pragma unroll
   for (int i = 0; i < L / LA; ++i) {
     const int k = q + LA * i;
     const bool has = live && k < nk;
-    double ux = 0.0, uy = 0.0, uz = 0.0, r = 0.0;
-    bool leg = false;
-    if (has) {
-      const int j = nbr[(size_t)ac * stride + k];
-      mic_fast(g, pos, ac, j, ux, uy, uz);
-      r = sqrt(ux * ux + uy * uy + uz * uz);
-      leg = r < rc && r > 1e-12 && r / sig < asw;
-    }
+    double ux, uy, uz;
+    mic_fast(g, pos, ac, has ? jr[i] : ac, ux, uy, uz);
+    const double r2 = ux * ux + uy * uy + uz * uz;
+    const double rinv = r2 > 1e-24 ? fast_rsq(r2) : 0.0;
+    const double r = r2 * rinv, x = r * inv_sig;
+    const bool leg = has && r < rc && r > 1e-12 && x < asw;
     const uint64_t bal = __ballot(leg);
     const uint64_t sm = LA == 64 ? bal : (bal >> seg) & ((uint64_t(1) << (LA & 63)) - 1);
     if (leg) {
@@ -1226,8 +1214,8 @@ __device__ __forceinline__ void k_sw_site_c_chunk(
       lx[t] = ux;
       ly[t] = uy;
       lz[t] = uz;
-      lri[t] = 1.0 / r;
-      lE[t] = nb_exp(gam / (r / sig - asw));
+      lri[t] = rinv;
+      lE[t] = nb_exp(gam * fast_rcp(x - asw));
       lslot[t] = k;
     }
     mask |= sm << (LA * i);
@@ -1238,12 +1226,12 @@ __device__ __forceinline__ void k_sw_site_c_chunk(
 
   // phase 2: legs q, q + LA, ... of this atom
   const double c3 = lam * eps;
-  double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // e, -sum G (x) u symmetrised
   for (int t = q; t < count; t += LA) {
     const double ux = lx[reg + t], uy = ly[reg + t], uz = lz[reg + t], rinv = lri[reg + t];
     const double E = lE[reg + t];
-    // the distance of phase 1, recomputed with the same operations
-    const double r = sqrt(ux * ux + uy * uy + uz * uz);
+    // the distance of phase 1, from the same operations
+    const double r = (ux * ux + uy * uy + uz * uz) * rinv;
     double e, su, dlnE;
     sw_leg(r, rinv, eps, sig, inv_sig, Asw, Bsw, pp, qq, asw, gam, e, su, dlnE);
     double swx = 0.0, swy = 0.0, swz = 0.0;
@@ -1267,33 +1255,17 @@ __device__ __forceinline__ void k_sw_site_c_chunk(
     g_put(G, slot, Gx, Gy, Gz);
     v[0] += e;
     v[1] -= Gx * ux;
-    v[2] -= Gx * uy;
-    v[3] -= Gx * uz;
-    v[4] -= Gy * ux;
-    v[5] -= Gy * uy;
-    v[6] -= Gy * uz;
-    v[7] -= Gz * ux;
-    v[8] -= Gz * uy;
-    v[9] -= Gz * uz;
+    v[2] -= 0.5 * (Gx * uy + Gy * ux);
+    v[3] -= 0.5 * (Gx * uz + Gz * ux);
+    v[4] -= Gy * uy;
+    v[5] -= 0.5 * (Gy * uz + Gz * uy);
+    v[6] -= Gz * uz;
   }
   if (esite) {  // per-atom site energy U_a
     const double ea = seg_sum<LA>(v[0]);
     if (live && q == 0) esite[a] = ea;
   }
-  wave_partial(v, partial, c, n_chunks);
-}
-
-template <int L, int LA>
-__global__ __launch_bounds__(256) void k_sw_site_c(
-    int n_chunks, int n, const double* __restrict__ pos, const int* __restrict__ sid,
-    const Geom* __restrict__ geoms, int single, const double* __restrict__ prm,
-    const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
-    const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
-    uint64_t* __restrict__ amask, double* __restrict__ esite) {
-  extern __shared__ double lds_sw[];
-  for (int cc = blockIdx.x; cc < n_chunks; cc += gridDim.x)
-    k_sw_site_c_chunk<L, LA>(xcd_remap(cc, n_chunks), n_chunks, n, pos, sid, geoms, single, prm, cnt, nbr, stride,
-                             chunk_rng, G, partial, amask, lds_sw, esite);
+  block_partial7(v, partial, c, n_chunks);
 }
 
 // Gather for long rows (stride L >= 16): LA lanes per atom, lane q summing
@@ -2159,7 +2131,8 @@ static void launch_bond_kernels(Potential& pot, NList& nl, const ForceWork& w,
                  part, amask, esite);
   } else if constexpr (L >= kLongRow) {
     const size_t lds = sw_c_lds(blk, L);
-    launch_timed(pot, k_sw_site_c<L, kRowLanes>, grid, block, st, lds, n_chunks, n, d_pos, sid,
+    launch_timed(pot, k_sw_site_c<L, kRowLanes>, dim3(n_chunks > 0 ? n_chunks : 1), block, st,
+                 lds, n_chunks, n, d_pos, sid,
                  geoms, single, (const double*)pot.d_params.as<double>(), cnt, nbr, nl.stride,
                  rng, G, part, amask, esite);
   } else {
