@@ -45,6 +45,31 @@ __device__ __forceinline__ void mic_fast(const Geom& g, const double* __restrict
   dz = sx * g.lat[2] + sy * g.lat[5] + sz * g.lat[8];
 }
 
+// A cell whose lattice and inverse have exactly zero off-diagonal entries
+// (uniform over a block: evaluate once).
+__device__ __forceinline__ bool geom_diagonal(const Geom& g) {
+  return g.lat[1] == 0.0 && g.lat[2] == 0.0 && g.lat[3] == 0.0 && g.lat[5] == 0.0 &&
+         g.lat[6] == 0.0 && g.lat[7] == 0.0 && g.inv[1] == 0.0 && g.inv[2] == 0.0 &&
+         g.inv[3] == 0.0 && g.inv[5] == 0.0 && g.inv[6] == 0.0 && g.inv[7] == 0.0;
+}
+
+// mic_fast, with the products by the zero off-diagonal entries of a
+// diagonal cell left out (the same values up to the sign of zero).
+__device__ __forceinline__ void mic_cell(const Geom& g, bool diagonal,
+                                         const double* __restrict__ pos, int i, int j,
+                                         double& dx, double& dy, double& dz) {
+  if (!diagonal) {
+    mic_fast(g, pos, i, j, dx, dy, dz);
+    return;
+  }
+  double sx = (pos[3 * j] - pos[3 * i]) * g.inv[0];
+  double sy = (pos[3 * j + 1] - pos[3 * i + 1]) * g.inv[4];
+  double sz = (pos[3 * j + 2] - pos[3 * i + 2]) * g.inv[8];
+  dx = (sx - rint(sx)) * g.lat[0];
+  dy = (sy - rint(sy)) * g.lat[4];
+  dz = (sz - rint(sz)) * g.lat[8];
+}
+
 // Minimum-image displacement pj - pi in the structure's cell, with the
 // reference's operation order (engine/neighborlist.py:350-361): fractional
 // components rounded half-to-even (rint), no fused multiply-add, so the

@@ -1753,14 +1753,43 @@ __device__ __forceinline__ void z_switch(double r, double rc, double w, double i
   dv = dval * S + val * dS;
 }
 
-// d f_sj / dr, d f_sa / dr and phi_{sa sj}, phi' at r.
+// d f_sj / dr, d f_sa / dr and phi_{sa sj}, phi' at r.  one_sw: every row
+// has the switch of species 0's density row (the table's trailing flag),
+// formed once and applied to all three functions.
 __device__ __forceinline__ void z_pair_fast(const double* __restrict__ fast, int ns, int sa,
-                                            int sj, double r, double& dfj, double& dfa,
-                                            double& phi, double& dphi) {
+                                            int sj, double r, bool one_sw, double& dfj,
+                                            double& dfa, double& phi, double& dphi) {
   const double* spj = fast + (size_t)ZF_SP * sj;
   const double* spa = fast + (size_t)ZF_SP * sa;
   const double* pr = fast + (size_t)ZF_SP * ns + (size_t)ZF_PR * (sa * ns + sj);
   const ZBase bj = z_base(spj, r);
+  if (one_sw) {
+    double t = (r - (fast[8] - fast[9])) * fast[11];
+    t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+    const double S = 1.0 - t * t * t * (10.0 - 15.0 * t + 6.0 * t * t);
+    const double dS = -30.0 * t * t * (1.0 - t) * (1.0 - t) * fast[11];
+    dfj = bj.fd * S + bj.fv * dS;
+    double val, dval;
+    if (sa == sj) {
+      dfa = dfj;
+      val = bj.pv;
+      dval = bj.pd;
+    } else {
+      const ZBase ba = z_base(spa, r);
+      dfa = ba.fd * S + ba.fv * dS;
+      const ZBase& X = pr[4] != 0.0 ? bj : ba;
+      const ZBase& Y = pr[4] != 0.0 ? ba : bj;
+      const double kc = pr[3];
+      const double iX = fast_rcp(X.fv), iY = fast_rcp(Y.fv);
+      const double q = Y.fv * iX, iq = X.fv * iY;
+      const double dq = (Y.fd * X.fv - Y.fv * X.fd) * iX * iX;
+      val = 0.5 * kc * (q * X.pv + Y.pv * iq);
+      dval = 0.5 * kc * (dq * X.pv + q * X.pd + Y.pd * iq - Y.pv * dq * iq * iq);
+    }
+    phi = val * S;
+    dphi = dval * S + val * dS;
+    return;
+  }
   double v;
   z_switch(r, spj[8], spj[9], spj[11], bj.fv, bj.fd, v, dfj);
   double val, dval;
@@ -1934,11 +1963,12 @@ __global__ __launch_bounds__(256) void k_eam_rho_z(
   const int ac = live ? a : a0;
   const int nk = live ? cnt[ac] : 0;
   const double rc2 = rc * rc;
+  const bool ortho = geom_diagonal(g);
   double rho = 0.0;
   for (int k = l; k < nk; k += LE) {
     const int j = nbr[(size_t)ac * stride + k];
     double ux, uy, uz;
-    mic_fast(g, pos, ac, j, ux, uy, uz);
+    mic_cell(g, ortho, pos, ac, j, ux, uy, uz);
     const double r2 = ux * ux + uy * uy + uz * uz;
     if (r2 < rc2 && r2 > 1e-24) {
       const int sj = ns > 1 ? spc[j] : 0;
@@ -1976,6 +2006,8 @@ __global__ __launch_bounds__(256) void k_eam_force_z(
   const int sa = ns > 1 ? spc[ac] : 0;
   const double dFa = dF[ac];
   const double rc2 = rc * rc;
+  const bool one_sw = fast[(size_t)ZF_SP * ns + (size_t)ZF_PR * ns * ns] != 0.0;
+  const bool ortho = geom_diagonal(g);
   int bonded = 0;  // a bond inside r_c seen by this lane
   double fx = 0.0, fy = 0.0, fz = 0.0;
   double e = live && l == 0 ? Fe[ac] : 0.0;
@@ -1983,14 +2015,14 @@ __global__ __launch_bounds__(256) void k_eam_force_z(
   for (int k = l; k < nk; k += LE) {
     const int j = nbr[(size_t)ac * stride + k];
     double ux, uy, uz;
-    mic_fast(g, pos, ac, j, ux, uy, uz);
+    mic_cell(g, ortho, pos, ac, j, ux, uy, uz);
     const double r2 = ux * ux + uy * uy + uz * uz;
     if (r2 < rc2 && r2 > 1e-24) {
       const int sj = ns > 1 ? spc[j] : 0;
       const double dFj = dF[j];
       const double rinv = fast_rsq(r2), r = r2 * rinv;
       double dfj, dfa, phi, dphi;
-      z_pair_fast(fast, ns, sa, sj, r, dfj, dfa, phi, dphi);
+      z_pair_fast(fast, ns, sa, sj, r, one_sw, dfj, dfa, phi, dphi);
       bonded = 1;
       const double s = (dphi + dFa * dfj + dFj * dfa) * rinv;
       fx += s * ux;
@@ -2208,7 +2240,7 @@ bool eam_fast_table(int ns, const std::vector<double>& rows, const std::vector<i
   auto same = [](const double* x, const double* y) {
     return std::memcmp(x, y, 20 * sizeof(double)) == 0;
   };
-  out.assign((size_t)ZF_SP * ns + (size_t)ZF_PR * ns * ns, 0.0);
+  out.assign((size_t)ZF_SP * ns + (size_t)ZF_PR * ns * ns + 1, 0.0);
   for (int s = 0; s < ns; ++s) {
     const double* d = row(map[s]);
     if (d[0] != 3.0) return false;
@@ -2252,6 +2284,18 @@ bool eam_fast_table(int ns, const std::vector<double>& rows, const std::vector<i
         return false;
       }
     }
+  // one switch: every density and pair row has the same r_c and width
+  bool one = true;
+  const double rc0 = out[8], w0 = out[9];
+  for (int a = 0; a < ns; ++a) {
+    const double* o = out.data() + (size_t)ZF_SP * a;
+    one = one && o[8] == rc0 && o[9] == w0;
+    for (int b = 0; b < ns; ++b) {
+      const double* q = out.data() + (size_t)ZF_SP * ns + (size_t)ZF_PR * (a * ns + b);
+      one = one && q[1] == rc0 && q[2] == w0;
+    }
+  }
+  out.back() = one ? 1.0 : 0.0;
   return true;
 }
 
