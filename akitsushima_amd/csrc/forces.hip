@@ -217,10 +217,11 @@ __device__ __forceinline__ void k_tersoff_site_chunk(int c, int n_chunks,
   const int seg = (threadIdx.x & 63) & ~(L - 1);
   const bool live = a < chunk_rng[2 * c + 1];
   const int ac = live ? a : 0;
-  const int nk = live ? cnt[ac] : 0;
+  const int nk_row = cnt[ac];  // loaded with the neighbour index
+  const int nk = live ? nk_row : 0;
   const bool has = live && k < nk;
   const size_t slot = (size_t)ac * stride + k;
-  const int jr = live ? nbr[slot] : 0;  // issued alongside cnt[]; used when has
+  const int jr = nbr[slot];  // used when has
   const int j = has ? jr : ac;
   const int sa = ns > 1 ? spc[ac] : 0;
   const int sj = ns > 1 ? spc[j] : 0;
@@ -228,7 +229,7 @@ __device__ __forceinline__ void k_tersoff_site_chunk(int c, int n_chunks,
   double ux = 0.0, uy = 0.0, uz = 0.0, r = 0.0, rinv = 0.0;
   bool act = false;
   if (has) {
-    const Geom& g = geoms[single ? 0 : sid[ac]];
+    const Geom& g = geoms[single ? 0 : sid[chunk_rng[2 * c]]];
     mic_fast(g, pos, ac, j, ux, uy, uz);
     r = sqrt(ux * ux + uy * uy + uz * uz);
     act = r < rc && r > 1e-12;
@@ -502,8 +503,7 @@ __global__ __launch_bounds__(256) void k_tersoff_site(int n_chunks, int n, const
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
     uint64_t* __restrict__ amask, double* __restrict__ esite) {
-  for (int cc = blockIdx.x; cc < n_chunks; cc += gridDim.x)
-    k_tersoff_site_chunk<L, SP1>(xcd_remap(cc, n_chunks), n_chunks, n, pos, sid, geoms, single, spc, ns_arg, tb, tri, rc, cnt,
+  k_tersoff_site_chunk<L, SP1>(xcd_remap((int)blockIdx.x, n_chunks), n_chunks, n, pos, sid, geoms, single, spc, ns_arg, tb, tri, rc, cnt,
                                 nbr, stride, chunk_rng, G, partial, amask, esite);
 }
 
@@ -780,14 +780,15 @@ __device__ __forceinline__ void k_tersoff_trip_chunk(int c, int n_chunks,
   const int aseg = wl & ~(LL - 1);   // first lane of this atom
   const bool live = a < chunk_rng[2 * c + 1];
   const int ac = live ? a : 0;
-  const int nk = live ? cnt[ac] : 0;
+  const int nk_row = cnt[ac];  // loaded with the neighbour index
+  const int nk = live ? nk_row : 0;
   const bool has = live && k < nk;
   const bool has_s = live && p != 0 && s < nk;
   const size_t slot = (size_t)ac * stride + k;
-  const int jr = live ? nbr[slot] : 0;
-  const int jsr = live ? nbr[(size_t)ac * stride + s] : 0;
+  const int jr = nbr[slot];
+  const int jsr = nbr[(size_t)ac * stride + s];
   const int sa = ns > 1 ? spc[ac] : 0;
-  const Geom& g = geoms[single ? 0 : sid[ac]];
+  const Geom& g = geoms[single ? 0 : sid[chunk_rng[2 * c]]];
   const Bond u = te_bond(g, pos, spc, ns, tb, rc, ac, sa, jr, has);
   const Bond w = te_bond(g, pos, spc, ns, tb, rc, ac, sa, jsr, has_s);
 
@@ -898,8 +899,7 @@ __global__ __launch_bounds__(256) void k_tersoff_trip(int n_chunks, int n, const
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
     uint64_t* __restrict__ amask) {
-  for (int cc = blockIdx.x; cc < n_chunks; cc += gridDim.x)
-    k_tersoff_trip_chunk<L, SP1>(xcd_remap(cc, n_chunks), n_chunks, n, pos, sid, geoms, single, spc, ns_arg, tb, tri, rc, cnt,
+  k_tersoff_trip_chunk<L, SP1>(xcd_remap((int)blockIdx.x, n_chunks), n_chunks, n, pos, sid, geoms, single, spc, ns_arg, tb, tri, rc, cnt,
                                 nbr, stride, chunk_rng, G, partial, amask);
 }
 
@@ -921,10 +921,11 @@ __device__ __forceinline__ void k_sw_site_chunk(int c, int n_chunks,
   const int seg = (threadIdx.x & 63) & ~(L - 1);
   const bool live = a < chunk_rng[2 * c + 1];
   const int ac = live ? a : 0;
-  const int nk = live ? cnt[ac] : 0;
+  const int nk_row = cnt[ac];  // loaded with the neighbour index
+  const int nk = live ? nk_row : 0;
   const bool has = live && k < nk;
   const size_t slot = (size_t)ac * stride + k;
-  const int jr = live ? nbr[slot] : 0;  // issued alongside cnt[]; used when has
+  const int jr = nbr[slot];  // used when has
   const int j = has ? jr : ac;
   const double eps = prm[0], sig = prm[1], Asw = prm[2], Bsw = prm[3], pp = prm[4],
                qq = prm[5], asw = prm[6], lam = prm[7], gam = prm[8], cth0 = prm[9];
@@ -933,7 +934,7 @@ __device__ __forceinline__ void k_sw_site_chunk(int c, int n_chunks,
   double ux = 0.0, uy = 0.0, uz = 0.0, r = 0.0, rinv = 0.0;
   bool act = false;
   if (has) {
-    const Geom& g = geoms[single ? 0 : sid[ac]];
+    const Geom& g = geoms[single ? 0 : sid[chunk_rng[2 * c]]];
     mic_fast(g, pos, ac, j, ux, uy, uz);
     r = sqrt(ux * ux + uy * uy + uz * uz);
     act = r < rc && r > 1e-12;
@@ -1012,8 +1013,7 @@ __global__ __launch_bounds__(256) void k_sw_site(
     const int* __restrict__ cnt, const int* __restrict__ nbr, int stride,
     const int* __restrict__ chunk_rng, double* __restrict__ G, double* __restrict__ partial,
     uint64_t* __restrict__ amask, double* __restrict__ esite) {
-  for (int cc = blockIdx.x; cc < n_chunks; cc += gridDim.x)
-    k_sw_site_chunk<L>(xcd_remap(cc, n_chunks), n_chunks, n, pos, sid, geoms, single, prm, cnt, nbr, stride,
+  k_sw_site_chunk<L>(xcd_remap((int)blockIdx.x, n_chunks), n_chunks, n, pos, sid, geoms, single, prm, cnt, nbr, stride,
                        chunk_rng, G, partial, amask, esite);
 }
 
@@ -1396,9 +1396,10 @@ __device__ __forceinline__ void k_sw_multi_chunk(
   const int a = chunk_rng[2 * c] + al;
   const bool live = a < chunk_rng[2 * c + 1];
   const int ac = live ? a : 0;
-  const int nk = live ? cnt[ac] : 0;
+  const int nk_row = cnt[ac];  // loaded with the neighbour index
+  const int nk = live ? nk_row : 0;
   const int si = spc[ac];
-  const Geom& g = geoms[single ? 0 : sid[ac]];
+  const Geom& g = geoms[single ? 0 : sid[chunk_rng[2 * c]]];
 
   // phase 1: legs inside a two-body or a three-body cutoff, in slot order
   int count = 0;
@@ -1536,8 +1537,7 @@ __global__ __launch_bounds__(256) void k_sw_multi(
   double* s_tri = lds_swm + (size_t)apb * L * 6 + ((size_t)apb * L * 3 * sizeof(int) + 7) / 8;
   for (int i = threadIdx.x; i < 2 * n2 * ns; i += blockDim.x) s_tri[i] = ttri[i];
   __syncthreads();
-  for (int cc = blockIdx.x; cc < n_chunks; cc += gridDim.x)
-    k_sw_multi_chunk<L, LA>(xcd_remap(cc, n_chunks), n_chunks, n, pos, sid, geoms, single, spc,
+  k_sw_multi_chunk<L, LA>(xcd_remap((int)blockIdx.x, n_chunks), n_chunks, n, pos, sid, geoms, single, spc,
                             ns, tpair, tleg, rc, cnt, nbr, stride, chunk_rng, G, partial, amask,
                             lds_swm, s_tri, esite);
 }
@@ -2117,7 +2117,8 @@ static void launch_bond_kernels(Potential& pot, NList& nl, const ForceWork& w,
   const int n = w.n;
   const int single = w.n_struct == 1 ? 1 : 0;
   double* G = w.G;
-  const dim3 grid = chunk_grid(pot, w.n_chunks), block(blk);
+  // one block per chunk (the bond kernels do not loop over chunks)
+  const dim3 grid(w.n_chunks > 0 ? w.n_chunks : 1), block(blk);
   const int* sid = w.sid;
   const Geom* geoms = w.geoms;
   const int* cnt = nl.cnt.as<int>();
