@@ -925,12 +925,10 @@ int ak_autotune_batched(ak_potential* P, int32_t S, const int64_t* off, const do
     P->batch_off.assign(off, off + S + 1);
 
     // candidate launch shapes {bond_tile, block, grid_cap, eam lanes}
-    int dev = 0, cus = 256;
-    AK_HIP(hipGetDevice(&dev));
-    AK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     std::vector<std::array<int, 4>> cand;
     const int blocks[3] = {64, 128, 256};
-    const int caps[3] = {0, 2 * cus, 4 * cus};
+    // the bond kernels and the Zhou-table EAM kernels run one block per chunk,
+    // so the grid cap is not a candidate
     if (P->pot.kind == KIND_EAM) {
       for (int lanes : {8, 16, 32, 64})
         for (int b : blocks)
@@ -941,8 +939,7 @@ int ak_autotune_batched(ak_potential* P, int32_t S, const int64_t* off, const do
       if (P->pot.kind == KIND_TERSOFF && L * L <= 64) tiles.push_back(2);
       for (int t : tiles)
         for (int b : blocks)
-          for (int c : caps)
-            if ((t == 2 ? L * L : L) <= b) cand.push_back({t, b, c, 0});
+          if ((t == 2 ? L * L : L) <= b) cand.push_back({t, b, 0, 0});
     }
     Potential& pot = P->pot;
     const int saved[4] = {pot.bond_tile, pot.block, pot.grid_cap, pot.eam_lanes};

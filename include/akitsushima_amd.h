@@ -197,14 +197,19 @@ int ak_potential_set_lanes(ak_potential *pot, int32_t lanes);
  *   block_threads 64, 128 or 256 threads per block (0 = 256); atoms per
  *                 block = block_threads / lanes per atom;
  *   grid_cap      0 = one block per chunk of atoms; > 0 = at most this many
- *                 blocks, each looping over chunks (persistent form). */
+ *                 blocks, each looping over chunks (persistent form).  Only
+ *                 the EAM force pass on general function tables (splines,
+ *                 rows other than the Zhou alloy form) loops; the bond-order
+ *                 kernels and the Zhou-table EAM kernels always run one block
+ *                 per chunk and accept the value without using it. */
 int ak_potential_set_launch(ak_potential *pot, int32_t bond_tile, int32_t block_threads,
                             int32_t grid_cap);
 
 /* Project-written autotuner (no counterpart in the reference): builds the
  * batch list for the given structures (as ak_compute_batched), times every
- * candidate launch shape -- Tersoff site form, threads per block, grid cap,
- * EAM lanes per atom -- over `reps` device force calls with HIP events, and
+ * candidate launch shape -- Tersoff site form and threads per block, or EAM
+ * lanes per atom and threads per block -- over `reps` device force calls
+ * with HIP events, and
  * leaves the fastest set on the potential.  best (4) = bond_tile,
  * block_threads, grid_cap, eam_lanes; *best_ms = its time per call.  When
  * shapes (capacity, 4) and times_ms (capacity) are given they receive every

@@ -69,10 +69,11 @@ def test_autotuner_picks_a_measured_shape_and_stays_exact():
     pot = TersoffPotential(TERSOFF_SI, species=["Si"])
     res = pot.autotune(structs, reps=3)
     tried = res["tried"]
-    assert len(tried) >= 6
+    assert len(tried) == 6
     assert all(r["ms"] > 0.0 for r in tried)
     assert res["best_ms"] == min(r["ms"] for r in tried)
-    assert any(r["bond_tile"] == 2 for r in tried) and any(r["grid_cap"] > 0 for r in tried)
+    # both Tersoff forms; the bond kernels run one block per chunk, so no grid cap
+    assert any(r["bond_tile"] == 2 for r in tried) and all(r["grid_cap"] == 0 for r in tried)
     refs = [O.tersoff_flat(l, p, TERSOFF_SI) for l, p, _ in structs]
     E, F, S = pot.compute_batched(structs)
     _compare(E, F, S, refs)

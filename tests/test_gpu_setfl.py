@@ -59,6 +59,19 @@ def test_setfl_matches_oracle(setfl_path, lanes):
            f"setfl lanes {lanes}")
 
 
+@pytest.mark.parametrize("block,cap", [(64, 3), (256, 5)])
+def test_setfl_persistent_force_grid(setfl_path, block, cap):
+    # general tables run the EAM force pass on at most `cap` blocks, each
+    # looping over chunks of atoms
+    pot = read_setfl(setfl_path)
+    pot.set_lanes(8)
+    pot.set_launch(0, block, cap)
+    tab, names = O.read_setfl(setfl_path)
+    lat, pos, t = ni3al(seed=9)
+    _check(pot.compute(lat, pos, t), O.eam_table(lat, pos, t, tab, names, nthreads=4),
+           f"setfl block {block} cap {cap}")
+
+
 def test_setfl_species_order_is_the_callers(setfl_path):
     pot = read_setfl(setfl_path, species=["Ni", "Al"])
     tab, names = O.read_setfl(setfl_path)
