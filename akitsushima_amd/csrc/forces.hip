@@ -2538,11 +2538,18 @@ __global__ __launch_bounds__(256) void k_heat_flux_final(int nb, const double* _
   __shared__ double smem[16 * 3];
   const int f = flux_frame(gate, every, phase);
   double J[3] = {0.0, 0.0, 0.0};
+  const int nt = blockDim.x;
   if (f >= 0)
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-      J[0] += part[3 * b];
-      J[1] += part[3 * b + 1];
-      J[2] += part[3 * b + 2];
+    for (int b = threadIdx.x; b < nb; b += 4 * nt) {  // four blocks' loads in flight
+      double x[4][3];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) x[u][c] = b + u * nt < nb ? part[3 * (b + u * nt) + c] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) J[c] += x[u][c];
     }
   block_sum<3>(J, smem);
   if (threadIdx.x == 0) {
@@ -2579,7 +2586,6 @@ __global__ __launch_bounds__(256) void k_heat_flux_groups(
     float* __restrict__ atoms, double inv_volume) {
   const int frame = flux_frame(gate, every, phase);
   if (frame < 0) return;
-  __shared__ double smem[16 * NC];
   const int a = (int)(blockIdx.x * blockDim.x + threadIdx.x) / LE;
   const int l = threadIdx.x & (LE - 1);
   double J[3] = {0.0, 0.0, 0.0};
@@ -2632,28 +2638,43 @@ __global__ __launch_bounds__(256) void k_heat_flux_groups(
   if (atoms && a < n && l == 0)
     for (int c = 0; c < 3; ++c)
       atoms[((size_t)frame * n + a) * 3 + c] = (float)(J[c] * inv_volume);
-  for (int q = 0; q < ng; ++q) {
+  // the block's atoms (blockDim / LE of them, usually in one or two groups)
+  // leave their records in LDS; thread q sums the records of group q in atom
+  // order and writes the block's partial of every group
+  constexpr int kApb = 256 / LE;
+  __shared__ double rec[kApb][NC];
+  __shared__ int rgrp[kApb];
+  const int al = (int)threadIdx.x / LE;
+  if (l == 0) {
+    rgrp[al] = grp;
+    rec[al][0] = J[0];
+    rec[al][1] = J[1];
+    rec[al][2] = J[2];
+    if constexpr (NC == 5) {
+      rec[al][3] = ke;
+      rec[al][4] = 1.0;
+    }
+  }
+  __syncthreads();
+  const int apb = (int)blockDim.x / LE;
+  for (int q = threadIdx.x; q < ng; q += blockDim.x) {
     double w[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) w[c] = 0.0;
-    if (grp == q && l == 0) {
-      w[0] = J[0];
-      w[1] = J[1];
-      w[2] = J[2];
-      if constexpr (NC == 5) {
-        w[3] = ke;
-        w[4] = 1.0;
-      }
-    }
-    block_sum<NC>(w, smem);
-    if (threadIdx.x == 0)
-      for (int c = 0; c < NC; ++c) part[((size_t)blockIdx.x * ng + q) * NC + c] = w[c];
+    for (int t = 0; t < apb; ++t)
+      if (rgrp[t] == q)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) w[c] += rec[t][c];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) part[((size_t)blockIdx.x * ng + q) * NC + c] = w[c];
   }
 }
 
 // Group sums of a record step: (sum of the block sums, fixed order) into
 // out[(frame * ng + g) * NC + c], the flux components divided by V; the gate
 // is read, not advanced.
+// One block per group q: each thread sums its blocks' partials in block
+// order, four blocks' loads in flight at a time.
 template <int NC>
 __global__ __launch_bounds__(256) void k_heat_flux_groups_final(
     int nb, int ng, const double* __restrict__ part, double inv_volume,
@@ -2662,18 +2683,26 @@ __global__ __launch_bounds__(256) void k_heat_flux_groups_final(
   const int frame = flux_frame(gate, every, phase);
   if (frame < 0) return;
   const size_t f = (size_t)frame;
-  for (int q = 0; q < ng; ++q) {
-    double J[NC];
+  const int q = blockIdx.x;
+  const int nt = blockDim.x;
+  double J[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) J[c] = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+  for (int c = 0; c < NC; ++c) J[c] = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 4 * nt) {
+    double x[4][NC];
 #pragma unroll
-      for (int c = 0; c < NC; ++c) J[c] += part[((size_t)b * ng + q) * NC + c];
-    block_sum<NC>(J, smem);
-    if (threadIdx.x == 0)
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
       for (int c = 0; c < NC; ++c)
-        out[(f * ng + q) * NC + c] = c < 3 ? J[c] * inv_volume : J[c];
+        x[u][c] = b + u * nt < nb ? part[((size_t)(b + u * nt) * ng + q) * NC + c] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) J[c] += x[u][c];
   }
+  block_sum<NC>(J, smem);
+  if (threadIdx.x == 0)
+    for (int c = 0; c < NC; ++c) out[(f * ng + q) * NC + c] = c < 3 ? J[c] * inv_volume : J[c];
 }
 
 bool site_masked(const Potential& pot, const NList& nl) {
@@ -2717,7 +2746,8 @@ void heat_flux(Potential& pot, NList& nl, SetDev& set, const double* d_pos, cons
                          site_masked(pot, nl) ? 1 : 0, conv, (const int*)d_gate, every, phase,
                          groups->labels, groups->axis, groups->bounds, groups->n_reg,
                          groups->ng, groups->part, groups->atoms, inv_v);
-      hipLaunchKernelGGL((k_heat_flux_groups_final<NC>), dim3(1), dim3(256), 0, st, (int)grid,
+      hipLaunchKernelGGL((k_heat_flux_groups_final<NC>), dim3(groups->ng), dim3(256), 0, st,
+                         (int)grid,
                          groups->ng, (const double*)groups->part, inv_v, (const int*)d_gate,
                          every, phase, groups->out);
     };
