@@ -438,6 +438,19 @@ struct FitState {
   // per-structure loss terms (kTerms each) and the Huber option
   std::vector<double> h_terms;
   DevBuf terms, fmul;
+  // An evaluation whose shape is unchanged replays a HIP graph of its
+  // device work (theta in from pinned memory, forward pass, adjoints,
+  // totals out to pinned memory): key = everything the captured launches
+  // bake in.  seen: the key of the last eager call (buffers allocated).
+  std::vector<double> graph_key, seen_key;
+  hipGraphExec_t graph = nullptr;
+  bool graph_off = false;
+  void drop_graph() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    graph = nullptr;
+    graph_key.clear();
+    seen_key.clear();
+  }
   int huber = 0;
   double hdelta[3] = {0.0, 0.0, 0.0};
   // pinned staging: [params 16 | theta 14 | loss, grad 15 | es 10 S]
@@ -454,6 +467,7 @@ struct FitState {
     return pin;
   }
   ~FitState() {
+    if (graph) (void)hipGraphExecDestroy(graph);
     if (pin) (void)hipHostFree(pin);
   }
 };
@@ -518,6 +532,7 @@ void tersoff_fit_set(Potential& pot, int S, const int64_t* off, const double* la
   F.huber = 0;
   F.nl.valid = false;
   F.loaded = true;
+  F.drop_graph();
 }
 
 // Per-structure loss terms of the loaded batch and the loss type
@@ -546,6 +561,7 @@ void tersoff_fit_terms(Potential& pot, int S, const double* terms, int huber,
       AK_REQUIRE(std::isfinite(deltas[q]) && deltas[q] >= 0.0, AK_ERR_ARG, "bad Huber delta");
   F.h_terms = h;
   AK_HIP(hipMemcpy(F.terms.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+  F.drop_graph();
   F.huber = huber;
   for (int q = 0; q < 3; ++q) F.hdelta[q] = huber ? deltas[q] : 0.0;
 }
@@ -578,7 +594,6 @@ void tersoff_fit_eval(Potential& pot, const double* theta, const double* weights
   std::memcpy(pin, pot.h_params.data(), sizeof(double) * 16);
   std::memcpy(pin + 16, theta, sizeof(double) * 14);
   F.cbuf.ensure(sizeof(double) * 30);
-  AK_HIP(hipMemcpyAsync(F.cbuf.p, pin, sizeof(double) * 30, hipMemcpyHostToDevice, st));
   pot.r_cut = rc;
   struct ParamView {  // pot.d_params -> cbuf while this call runs
     DevBuf& b;
@@ -603,56 +618,105 @@ void tersoff_fit_eval(Potential& pot, const double* theta, const double* weights
   }
   AK_REQUIRE(F.nl.stride <= 64, AK_ERR_CAPACITY, "more than 64 neighbours per atom");
 
-  // forward
-  F.F.ensure(sizeof(double) * 3 * n);
-  F.es.ensure(sizeof(double) * 10 * S);
-  compute_forces(pot, F.nl, F.set, F.pos.as<double>(), F.spc.as<int>(), F.F.as<double>(),
-                 F.es.as<double>(), st);
-  // loss and adjoints
   const double wE = weights[0], wF = weights[1], r0sq = weights[3] * weights[3],
                wS = weights[2], eps0 = weights[4];
-  F.lam.ensure(sizeof(double) * 3 * n);
-  F.lf.ensure(sizeof(double) * n);
-  F.loss_s.ensure(sizeof(double) * S);
-  F.kappa.ensure(sizeof(double) * S);
-  F.muv.ensure(sizeof(double) * 9 * S);
-  hipLaunchKernelGGL(k_fit_atoms, dim3((n + 255) / 256), dim3(256), 0, st, n,
-                     F.set.d_sid.as<int>(), F.set.d_geoms.as<Geom>(),
-                     (const double*)F.F.as<double>(), (const double*)F.fref.as<double>(), r0sq,
-                     (const double*)F.terms.as<double>(), F.lam.as<double>(), F.lf.as<double>());
-  hipLaunchKernelGGL(k_fit_structs, dim3(S), dim3(256), 0, st, F.set.d_geoms.as<Geom>(),
-                     (const double*)F.es.as<double>(), (const double*)F.ref.as<double>(),
-                     (const double*)F.lf.as<double>(), wE, wF, wS, eps0,
-                     (const double*)F.terms.as<double>(), F.huber, F.hdelta[0], F.hdelta[1],
-                     F.hdelta[2], F.loss_s.as<double>(), F.kappa.as<double>(),
-                     F.fmul.as<double>(), F.muv.as<double>());
-  if (grad) {
-    hipLaunchKernelGGL(k_fit_lam_scale, dim3((n + 255) / 256), dim3(256), 0, st, n,
-                       F.set.d_sid.as<int>(), (const double*)F.fmul.as<double>(),
-                       F.lam.as<double>());
-    F.dE.ensure(sizeof(double) * 14 * (size_t)n);
-    F.dD.ensure(sizeof(double) * 14 * (size_t)n);
-    F.grad_s.ensure(sizeof(double) * 14 * (size_t)S);
-    switch (F.nl.stride) {
-      case 4: launch_hd<4>(F, n, st); break;
-      case 8: launch_hd<8>(F, n, st); break;
-      case 16: launch_hd<16>(F, n, st); break;
-      case 32: launch_hd<32>(F, n, st); break;
-      case 64: launch_hd<64>(F, n, st); break;
-      default: throw Error(AK_ERR_STATE, "unsupported neighbour stride");
-    }
-    hipLaunchKernelGGL(k_fit_reduce, dim3(S, 14), dim3(256), 0, st, F.set.d_geoms.as<Geom>(),
-                       (const double*)F.dE.as<double>(), (const double*)F.dD.as<double>(),
-                       (const double*)F.kappa.as<double>(), n, F.grad_s.as<double>());
-  }
-  F.tot.ensure(sizeof(double) * (15 + 10 * (size_t)S));
   const bool want_es = energies || stresses;
-  hipLaunchKernelGGL(k_fit_total, dim3(1), dim3(256), 0, st, S,
-                     (const double*)F.loss_s.as<double>(),
-                     (const double*)(grad ? F.grad_s.as<double>() : nullptr),
-                     (const double*)(want_es ? F.es.as<double>() : nullptr), F.tot.as<double>());
-  AK_HIP(hipMemcpyAsync(pin_out, F.tot.p, sizeof(double) * (15 + (want_es ? 10 * (size_t)S : 0)),
-                        hipMemcpyDeviceToHost, st));
+  const size_t n_out = 15 + (want_es ? 10 * (size_t)S : 0);
+  // the device work of one evaluation; everything it launches is fixed by
+  // the key below, so it can be captured once and replayed
+  auto device_work = [&]() {
+    AK_HIP(hipMemcpyAsync(F.cbuf.p, pin, sizeof(double) * 30, hipMemcpyHostToDevice, st));
+    // forward
+    F.F.ensure(sizeof(double) * 3 * n);
+    F.es.ensure(sizeof(double) * 10 * S);
+    compute_forces(pot, F.nl, F.set, F.pos.as<double>(), F.spc.as<int>(), F.F.as<double>(),
+                   F.es.as<double>(), st);
+    // loss and adjoints
+    F.lam.ensure(sizeof(double) * 3 * n);
+    F.lf.ensure(sizeof(double) * n);
+    F.loss_s.ensure(sizeof(double) * S);
+    F.kappa.ensure(sizeof(double) * S);
+    F.muv.ensure(sizeof(double) * 9 * S);
+    hipLaunchKernelGGL(k_fit_atoms, dim3((n + 255) / 256), dim3(256), 0, st, n,
+                       F.set.d_sid.as<int>(), F.set.d_geoms.as<Geom>(),
+                       (const double*)F.F.as<double>(), (const double*)F.fref.as<double>(), r0sq,
+                       (const double*)F.terms.as<double>(), F.lam.as<double>(), F.lf.as<double>());
+    hipLaunchKernelGGL(k_fit_structs, dim3(S), dim3(256), 0, st, F.set.d_geoms.as<Geom>(),
+                       (const double*)F.es.as<double>(), (const double*)F.ref.as<double>(),
+                       (const double*)F.lf.as<double>(), wE, wF, wS, eps0,
+                       (const double*)F.terms.as<double>(), F.huber, F.hdelta[0], F.hdelta[1],
+                       F.hdelta[2], F.loss_s.as<double>(), F.kappa.as<double>(),
+                       F.fmul.as<double>(), F.muv.as<double>());
+    if (grad) {
+      hipLaunchKernelGGL(k_fit_lam_scale, dim3((n + 255) / 256), dim3(256), 0, st, n,
+                         F.set.d_sid.as<int>(), (const double*)F.fmul.as<double>(),
+                         F.lam.as<double>());
+      F.dE.ensure(sizeof(double) * 14 * (size_t)n);
+      F.dD.ensure(sizeof(double) * 14 * (size_t)n);
+      F.grad_s.ensure(sizeof(double) * 14 * (size_t)S);
+      switch (F.nl.stride) {
+        case 4: launch_hd<4>(F, n, st); break;
+        case 8: launch_hd<8>(F, n, st); break;
+        case 16: launch_hd<16>(F, n, st); break;
+        case 32: launch_hd<32>(F, n, st); break;
+        case 64: launch_hd<64>(F, n, st); break;
+        default: throw Error(AK_ERR_STATE, "unsupported neighbour stride");
+      }
+      hipLaunchKernelGGL(k_fit_reduce, dim3(S, 14), dim3(256), 0, st, F.set.d_geoms.as<Geom>(),
+                         (const double*)F.dE.as<double>(), (const double*)F.dD.as<double>(),
+                         (const double*)F.kappa.as<double>(), n, F.grad_s.as<double>());
+    }
+    F.tot.ensure(sizeof(double) * (15 + 10 * (size_t)S));
+    hipLaunchKernelGGL(k_fit_total, dim3(1), dim3(256), 0, st, S,
+                       (const double*)F.loss_s.as<double>(),
+                       (const double*)(grad ? F.grad_s.as<double>() : nullptr),
+                       (const double*)(want_es ? F.es.as<double>() : nullptr), F.tot.as<double>());
+    AK_HIP(hipGetLastError());
+    AK_HIP(hipMemcpyAsync(pin_out, F.tot.p, sizeof(double) * n_out, hipMemcpyDeviceToHost, st));
+  };
+
+  // key: the launches' fixed arguments (cutoff, weights, loss type, the
+  // forward kernel's form, list and launch shape, output set, staging)
+  const double* hp = pot.h_params.data();
+  const std::vector<double> key = {
+      rc, wE, wF, wS, eps0, r0sq, (double)F.huber, F.hdelta[0], F.hdelta[1], F.hdelta[2],
+      hp[13] == 0.0 ? 1.0 : 0.0, hp[14], hp[15], grad ? 1.0 : 0.0, want_es ? 1.0 : 0.0,
+      (double)F.nl.stride, (double)F.nl.n_builds, (double)pot.block, (double)pot.bond_tile,
+      (double)pot.eam_lanes, (double)(uintptr_t)pin, (double)(uintptr_t)F.cbuf.p};
+  const bool graphable = !forces && !pot.prof_on && !F.graph_off;
+  if (graphable && F.graph && key == F.graph_key) {
+    AK_HIP(hipGraphLaunch(F.graph, st));
+  } else if (graphable && key == F.seen_key) {
+    // second call of this shape: buffers exist, capture and replay
+    if (F.graph) (void)hipGraphExecDestroy(F.graph);
+    F.graph = nullptr;
+    F.graph_key.clear();
+    hipGraph_t g = nullptr;
+    AK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+      device_work();
+    } catch (...) {
+      (void)hipStreamEndCapture(st, &g);
+      if (g) (void)hipGraphDestroy(g);
+      F.graph_off = true;
+      throw;
+    }
+    AK_HIP(hipStreamEndCapture(st, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+      F.graph_off = true;
+      device_work();
+    } else {
+      F.graph = ex;
+      F.graph_key = key;
+      AK_HIP(hipGraphLaunch(F.graph, st));
+    }
+  } else {
+    device_work();
+    F.seen_key = key;
+  }
   if (forces)
     AK_HIP(hipMemcpyAsync(forces, F.F.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
   AK_HIP(hipStreamSynchronize(st));

@@ -81,3 +81,34 @@ def test_autograd_function_backward_is_engine_gradient():
     _, g = fit.loss_and_grad(theta)
     assert np.array_equal(th.grad.numpy(), g)
     assert abs(float(L.detach()) - fit.loss(theta)) == 0.0
+
+
+def test_replayed_evaluations_equal_fresh_ones():
+    """From the second evaluation of one shape on (same cutoff, weights and
+    outputs), the device work replays a captured graph with the new
+    parameters copied in: every replay equals a fresh, uncaptured evaluation
+    bit for bit (R and D held, so every fit builds the same list).  A change
+    of R moves the cutoff: other launches, evaluated and captured anew."""
+    theta, data = make_set(n_struct=3, seed=29, lambda3=1.2, gamma=1.0)
+    rng = np.random.default_rng(30)
+    thetas = []
+    for _ in range(4):
+        t = theta * (1 + rng.uniform(-0.02, 0.02, 14))
+        t[10:13] = theta[10:13]  # R, D, m held
+        thetas.append(t)
+    ref = [TersoffFit(data).loss_and_grad(t) for t in thetas]
+    ref_loss = [TersoffFit(data).loss(t) for t in thetas]
+    fit = TersoffFit(data)
+    for k in list(range(4)) * 2:  # eager, capture, then replays
+        L, g = fit.loss_and_grad(thetas[k])
+        assert L == ref[k][0] and np.array_equal(g, ref[k][1]), k
+    for k in (0, 1, 2):  # the loss alone: a graph of its own
+        assert fit.loss(thetas[k]) == ref_loss[k]
+    moved = thetas[0].copy()
+    moved[10] *= 1.01
+    La, ga = fit.loss_and_grad(moved)
+    Lb, gb = fit.loss_and_grad(moved)  # captured at the new cutoff
+    assert La == Lb and np.array_equal(ga, gb)
+    for k in (1, 2, 3):  # back at the first cutoff
+        L, g = fit.loss_and_grad(thetas[k])
+        assert L == ref[k][0] and np.array_equal(g, ref[k][1]), k
