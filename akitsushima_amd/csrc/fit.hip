@@ -442,7 +442,17 @@ struct FitState {
   // device work (theta in from pinned memory, forward pass, adjoints,
   // totals out to pinned memory): key = everything the captured launches
   // bake in.  seen: the key of the last eager call (buffers allocated).
-  std::vector<double> graph_key, seen_key;
+  struct GraphKey {
+    std::vector<double> values;  // cutoff, weights, loss type, kernel form
+    std::vector<int> shape;      // outputs, list and launch shape
+    const void* pin = nullptr;
+    const void* cbuf = nullptr;
+    bool operator==(const GraphKey& o) const {
+      return values == o.values && shape == o.shape && pin == o.pin && cbuf == o.cbuf;
+    }
+    void clear() { *this = GraphKey{}; }
+  };
+  GraphKey graph_key, seen_key;
   hipGraphExec_t graph = nullptr;
   bool graph_off = false;
   void drop_graph() {
@@ -678,11 +688,13 @@ void tersoff_fit_eval(Potential& pot, const double* theta, const double* weights
   // key: the launches' fixed arguments (cutoff, weights, loss type, the
   // forward kernel's form, list and launch shape, output set, staging)
   const double* hp = pot.h_params.data();
-  const std::vector<double> key = {
-      rc, wE, wF, wS, eps0, r0sq, (double)F.huber, F.hdelta[0], F.hdelta[1], F.hdelta[2],
-      hp[13] == 0.0 ? 1.0 : 0.0, hp[14], hp[15], grad ? 1.0 : 0.0, want_es ? 1.0 : 0.0,
-      (double)F.nl.stride, (double)F.nl.n_builds, (double)pot.block, (double)pot.bond_tile,
-      (double)pot.eam_lanes, (double)(uintptr_t)pin, (double)(uintptr_t)F.cbuf.p};
+  FitState::GraphKey key;
+  key.values = {rc, wE, wF, wS, eps0, r0sq, F.hdelta[0], F.hdelta[1], F.hdelta[2],
+                hp[13] == 0.0 ? 1.0 : 0.0, hp[14], hp[15]};
+  key.shape = {F.huber, grad ? 1 : 0, want_es ? 1 : 0, F.nl.stride, F.nl.n_builds, pot.block,
+               pot.bond_tile, pot.eam_lanes};
+  key.pin = pin;
+  key.cbuf = F.cbuf.p;
   const bool graphable = !forces && !pot.prof_on && !F.graph_off;
   if (graphable && F.graph && key == F.graph_key) {
     AK_HIP(hipGraphLaunch(F.graph, st));
