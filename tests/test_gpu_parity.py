@@ -11,6 +11,7 @@ import oracle as O
 from akitsushima_amd import (NeighbourList, StillingerWeberPotential, TersoffPotential,
                              zhou_eam)
 from akitsushima_amd.published import ZHOU_2004
+from akitsushima_amd.structures import FCC, cubic
 
 from _cases import (LEDGER, SW_SI, TERSOFF_SI, TERSOFF_XY, cuni_all, ni, rattled, rel_err,
                     sheared_si, si, xy)
@@ -165,6 +166,23 @@ def test_eam_cuni_matches_ledger_and_oracle(n):
     e0, f0, s0 = O.eam_zhou(lat, pos, t, {"Cu": O.ZHOU_CU, "Ni": O.ZHOU_NI}, ["Cu", "Ni"],
                             nthreads=8)
     _check(e, f, s, e0, f0, s0, N, f"eam cuni{n}")
+
+
+def test_eam_cuni_on_a_sheared_cell_matches_the_oracle():
+    # a triclinic cell: the Zhou kernels' general minimum image (a cubic
+    # cell takes the diagonal one)
+    lat, pos, _ = cubic(FCC, 3.58, 5, ["Cu"] * 4, rattle=0.0)
+    T = np.eye(3)
+    T[0, 1], T[2, 0], T[1, 2] = 0.06, -0.03, 0.02
+    frac = pos @ np.linalg.inv(lat)
+    lat = lat @ T
+    pos = frac @ lat + np.random.default_rng(11).normal(0.0, 0.04, pos.shape)
+    t = [str(x) for x in np.random.default_rng(12).choice(["Cu", "Ni"], len(pos))]
+    pot = zhou_eam({"Cu": ZHOU_2004["Cu"], "Ni": ZHOU_2004["Ni"]}, ["Cu", "Ni"])
+    e, f, s = pot.compute(lat, pos, t)
+    e0, f0, s0 = O.eam_zhou(lat, pos, t, {"Cu": O.ZHOU_CU, "Ni": O.ZHOU_NI}, ["Cu", "Ni"],
+                            nthreads=8)
+    _check(e, f, s, e0, f0, s0, len(pos), "eam cuni sheared")
 
 
 # --------------------------------------------------------------------------
