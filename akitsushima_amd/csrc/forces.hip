@@ -547,7 +547,7 @@ __device__ __forceinline__ double wave_sums8(const double (&v)[8], int& idx) {
 // the block calls it once (one chunk per block).
 __device__ __forceinline__ void block_partial7(const double (&v)[8], double* __restrict__ partial,
                                                int c, int n_chunks) {
-  __shared__ double sm[8][4];
+  __shared__ double sm[8][8];  // up to 8 wavefronts
   int idx;
   const double t = wave_sums8(v, idx);
   const int wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -596,7 +596,7 @@ __device__ __forceinline__ double qrot(double x) {
 }
 
 template <bool L3Z>
-__global__ __launch_bounds__(256) void k_tersoff_q4(
+__global__ __launch_bounds__(512) void k_tersoff_q4(
     int n_chunks, int n, const double* __restrict__ pos, const int* __restrict__ sid,
     const Geom* __restrict__ geoms, int single, const double* __restrict__ tb,
     const double* __restrict__ tri, double rc, const int* __restrict__ cnt,
@@ -2109,6 +2109,14 @@ static inline dim3 chunk_grid(const Potential& pot, int n_chunks) {
 // multiple of the XCD count (blocks are dispatched to the XCDs round-robin).
 static inline int gather_lead(int n_struct) { return (n_struct + kXcds - 1) / kXcds * kXcds; }
 
+// The four-slot Tersoff kernel runs: one species with the flat form's
+// record (m = 3, gamma = 1) on rows of four slots, one lane per bond.
+static bool q4_path(const Potential& pot, int lanes, bool trip) {
+  const double* hp = pot.h_params.data();
+  return pot.kind == KIND_TERSOFF && pot.ns == 1 && lanes == 4 && !trip && hp[0] != 0.0 &&
+         hp[9] != 0.0 && hp[14] == 3.0 && hp[15] == 1.0;
+}
+
 template <int L>
 static void launch_bond_kernels(Potential& pot, NList& nl, const ForceWork& w,
                                 const double* d_pos, const int* d_spc, double* d_F,
@@ -2144,7 +2152,7 @@ static void launch_bond_kernels(Potential& pot, NList& nl, const ForceWork& w,
       // the flat form (one valid record, m = 3, gamma = 1) on rows of four:
       // one block per chunk (the grid cap does not apply)
       const double* hp = pot.h_params.data();
-      if (!done && sp1 && hp[0] != 0.0 && hp[9] != 0.0 && hp[14] == 3.0 && hp[15] == 1.0) {
+      if (!done && q4_path(pot, L, false)) {
         launch_timed(pot, hp[13] == 0.0 ? k_tersoff_q4<true> : k_tersoff_q4<false>,
                      dim3(n_chunks > 0 ? n_chunks : 1), block,
                      st, (size_t)0, n_chunks, n, d_pos, sid, geoms, single, tb, tri, pot.r_cut, cnt,
@@ -2302,8 +2310,8 @@ bool eam_fast_table(int ns, const std::vector<double>& rows, const std::vector<i
 
 ForcePlan force_plan(const Potential& pot, const NList& nl, bool esite) {
   AK_REQUIRE(nl.valid, AK_ERR_STATE, "neighbour list has not been built");
-  AK_REQUIRE(pot.block == 64 || pot.block == 128 || pot.block == 256, AK_ERR_STATE,
-             "block size must be 64, 128 or 256 threads");
+  AK_REQUIRE(pot.block == 64 || pot.block == 128 || pot.block == 256 || pot.block == 512,
+             AK_ERR_STATE, "block size must be 64, 128, 256 or 512 threads");
   ForcePlan pl;
   if (pot.kind == KIND_EAM) {
     pl.lanes = pot.eam_lanes > 0 ? pot.eam_lanes : 16;
@@ -2322,6 +2330,9 @@ ForcePlan force_plan(const Potential& pot, const NList& nl, bool esite) {
   const int atom_lanes = pl.trip ? lanes * lanes
                                  : (sw_long || (swm && lanes > kRowLanes) ? kRowLanes : lanes);
   int blk = pot.block;
+  // 512 threads per block for the four-slot Tersoff kernel only (every other
+  // kernel is built for at most 256)
+  if (blk == 512 && !q4_path(pot, lanes, pl.trip)) blk = 256;
   // the long-row and multi-species SW kernels stage every leg of their atoms
   // in LDS: keep it within 48 KiB per block
   while (sw_long && blk > 64 && sw_c_lds(blk, lanes) > 48 * 1024) blk >>= 1;

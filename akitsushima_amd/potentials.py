@@ -73,7 +73,8 @@ class _GpuPotential:
     def set_launch(self, bond_tile=0, block_threads=0, grid_cap=0):
         """Launch shape of the site / force kernels (the autotuner's knobs):
         Tersoff site form (1 = lane per bond, 2 = lane per (bond, partner)),
-        threads per block (64, 128, 256), and a grid cap (0 = one block per
+        threads per block (64, 128, 256; 512 for the four-slot Tersoff
+        kernel, other kernels then run 256), and a grid cap (0 = one block per
         chunk of atoms, > 0 = persistent blocks looping over chunks)."""
         self._launch = (int(bond_tile), int(block_threads), int(grid_cap))
         h = getattr(self._tls, "h", None)

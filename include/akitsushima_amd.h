@@ -194,8 +194,10 @@ int ak_potential_set_lanes(ak_potential *pot, int32_t lanes);
  *   bond_tile     Tersoff site kernel form: 1 = one lane per bond, 2 = one
  *                 lane per (bond, partner leg) pair for list strides up to 8;
  *                 0 = automatic (one lane per bond);
- *   block_threads 64, 128 or 256 threads per block (0 = 256); atoms per
- *                 block = block_threads / lanes per atom;
+ *   block_threads 64, 128, 256 or 512 threads per block (0 = 256); atoms
+ *                 per block = block_threads / lanes per atom; 512 applies to
+ *                 the four-slot Tersoff kernel (one species, flat form, rows
+ *                 of four), every other kernel then runs 256;
  *   grid_cap      0 = one block per chunk of atoms; > 0 = at most this many
  *                 blocks, each looping over chunks (persistent form).  Only
  *                 the EAM force pass on general function tables (splines,
@@ -209,7 +211,7 @@ int ak_potential_set_launch(ak_potential *pot, int32_t bond_tile, int32_t block_
  * batch list for the given structures (as ak_compute_batched), times every
  * candidate launch shape -- Tersoff site form and threads per block, or EAM
  * lanes per atom and threads per block -- over `reps` device force calls
- * with HIP events, and
+ * with HIP events, in three rounds (each candidate keeps its best round), and
  * leaves the fastest set on the potential.  best (4) = bond_tile,
  * block_threads, grid_cap, eam_lanes; *best_ms = its time per call.  When
  * shapes (capacity, 4) and times_ms (capacity) are given they receive every

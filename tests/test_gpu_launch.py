@@ -16,7 +16,10 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-10
 
-SHAPES = [(1, 256, 0), (1, 64, 0), (1, 128, 3), (2, 256, 0), (2, 64, 5), (2, 128, 1)]
+# block 512: the four-slot kernel; with the bond-pair form (tile 2) the plan
+# falls back to 256
+SHAPES = [(1, 256, 0), (1, 64, 0), (1, 128, 3), (1, 512, 0), (2, 256, 0), (2, 64, 5),
+          (2, 128, 1), (2, 512, 0)]
 
 
 def _batch():
@@ -40,7 +43,7 @@ def test_tersoff_batch_every_launch_shape(shape):
     _compare(E, F, S, refs)
 
 
-@pytest.mark.parametrize("shape", [(0, 64, 0), (0, 128, 2), (0, 256, 7)])
+@pytest.mark.parametrize("shape", [(0, 64, 0), (0, 128, 2), (0, 256, 7), (0, 512, 0)])
 def test_sw_batch_launch_shapes(shape):
     structs = _batch()
     refs = [O.sw_flat(l, p, SW_SI) for l, p, _ in structs]
@@ -69,7 +72,7 @@ def test_autotuner_picks_a_measured_shape_and_stays_exact():
     pot = TersoffPotential(TERSOFF_SI, species=["Si"])
     res = pot.autotune(structs, reps=3)
     tried = res["tried"]
-    assert len(tried) == 6
+    assert len(tried) == 7  # two forms x three block sizes, and 512 for the four-slot kernel
     assert all(r["ms"] > 0.0 for r in tried)
     assert res["best_ms"] == min(r["ms"] for r in tried)
     # both Tersoff forms; the bond kernels run one block per chunk, so no grid cap
