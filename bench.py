@@ -671,6 +671,9 @@ def main():
     ap.add_argument("--bond-tile", type=int, default=0,
                     help="Tersoff site kernel form: 1 lane per bond, 2 lane per "
                          "(bond, partner), 0 automatic")
+    ap.add_argument("--block", type=int, default=0,
+                    help="threads per block of the headline force kernels (with --bond-tile: "
+                         "a fixed launch shape instead of the autotuner's, for profiling)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the SW MD, Zhou EAM and batched Tersoff lines")
     ap.add_argument("--leg", default="",
@@ -735,7 +738,9 @@ def main():
     # timed steps follow the tuned calls directly)
     tuned = None
     if args.bond_tile:
-        pot.set_bond_tile(args.bond_tile)
+        pot.set_launch(args.bond_tile, args.block, 0)
+        tuned = {"bond_tile": args.bond_tile, "block_threads": args.block or 256, "grid_cap": 0,
+                 "eam_lanes": 0}
     else:
         tuned = pot.autotune([(lat, pos, types)], reps=30)["best"]
 
