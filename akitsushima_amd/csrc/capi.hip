@@ -123,7 +123,23 @@ struct ak_potential {
   DevBuf d_pos, d_spc, d_F, d_out;
   std::vector<int32_t> last_spc;
   bool spc_valid = false;
+  // pinned host staging of the drop-in call: positions in, forces and the
+  // eleven result values out (DMA from and to pinned memory instead of the
+  // runtime's pageable path)
+  double* h_stage = nullptr;
+  size_t h_stage_n = 0;
+  double* pinned_stage(size_t n_doubles) {
+    if (n_doubles > h_stage_n) {
+      if (h_stage) AK_HIP(hipHostFree(h_stage));
+      h_stage = nullptr;
+      h_stage_n = 0;
+      AK_HIP(hipHostMalloc((void**)&h_stage, sizeof(double) * n_doubles, hipHostMallocDefault));
+      h_stage_n = n_doubles;
+    }
+    return h_stage;
+  }
   ~ak_potential() {
+    if (h_stage) (void)hipHostFree(h_stage);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -565,31 +581,44 @@ int ak_compute(ak_potential* P, ak_nlist* nl, int32_t n, const double* lattice,
                "bad arguments");
     AK_REQUIRE(positions || n == 0, AK_ERR_ARG, "null positions");
     check_finite(lattice, 9, "lattice");
-    check_finite(positions, 3 * (size_t)n, "positions");
     hipStream_t st = P->stream;
+    // pinned staging [positions 3n | forces 3n | out 11]; the positions are
+    // copied in and checked in one pass
+    const size_t m = 3 * (size_t)n;
+    double* hs = P->pinned_stage(2 * m + 11);
+    double* h_pos = hs;
+    double* h_F = hs + m;
+    double* out = hs + 2 * m;  // energy, stress, the owned list's largest displacement
+    {
+      bool ok = true;  // a branch-free pass (vectorised)
+      for (size_t k = 0; k < m; ++k) {
+        const double v = positions[k];
+        h_pos[k] = v;
+        ok &= std::isfinite(v);
+      }
+      AK_REQUIRE(ok, AK_ERR_ARG, "positions contains non-finite values");
+    }
     stage_species(P, n, species);
     P->d_pos.ensure(sizeof(double) * 3 * std::max(1, n));
     P->d_F.ensure(sizeof(double) * 3 * std::max(1, n));
     P->d_out.ensure(sizeof(double) * 11);
     if (n > 0)
-      AK_HIP(hipMemcpyAsync(P->d_pos.p, positions, sizeof(double) * 3 * n,
-                            hipMemcpyHostToDevice, st));
-    // out: energy, stress, and the owned list's largest displacement (valid
-    // once the stream has been synchronised; run returns a reference to it)
-    double out[11] = {};
+      AK_HIP(hipMemcpyAsync(P->d_pos.p, h_pos, sizeof(double) * m, hipMemcpyHostToDevice, st));
+    // the results are valid once the stream has been synchronised; run
+    // returns a reference to the largest displacement
     auto run = [&](ak_nlist* Lst) -> const double& {
       compute_forces(P->pot, Lst->nl, Lst->set, P->d_pos.as<double>(), P->d_spc.as<int>(),
                      P->d_F.as<double>(), P->d_out.as<double>(), st);
-      AK_HIP(hipMemcpyAsync(out, P->d_out.p, sizeof(out), hipMemcpyDeviceToHost, st));
+      AK_HIP(hipMemcpyAsync(out, P->d_out.p, sizeof(double) * 11, hipMemcpyDeviceToHost, st));
       if (n > 0)
-        AK_HIP(hipMemcpyAsync(forces, P->d_F.p, sizeof(double) * 3 * n,
-                              hipMemcpyDeviceToHost, st));
+        AK_HIP(hipMemcpyAsync(h_F, P->d_F.p, sizeof(double) * m, hipMemcpyDeviceToHost, st));
       return out[10];
     };
     if (nl || !owned_list_speculative(P, n, lattice, inv, st, run)) {
       run(select_list(P, nl, n, lattice, inv, st));
       AK_HIP(hipStreamSynchronize(st));
     }
+    if (n > 0) std::memcpy(forces, h_F, sizeof(double) * m);
     *energy = out[0];
     if (stress && need_stress) std::memcpy(stress, out + 1, 9 * sizeof(double));
   });
