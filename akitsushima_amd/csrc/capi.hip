@@ -164,7 +164,8 @@ static bool check_and_rebuild(ak_nlist& L, int n, const double* lattice, const d
                               const double* d_pos, hipStream_t st) {
   L.n_steps++;
   L.nl.n_checks++;
-  L.set.setup(1, std::vector<int64_t>{0, (int64_t)n}.data(), lattice, inv, L.nl.r_list, st);
+  const int64_t off[2] = {0, (int64_t)n};
+  L.set.setup(1, off, lattice, inv, L.nl.r_list, st);
   if (!L.nl.valid || L.nl.n_atoms != n) {
     nl_build(L.nl, L.set, d_pos, st);
     return true;
@@ -218,7 +219,8 @@ static ak_nlist* select_list(ak_potential* P, ak_nlist* L, int n, const double* 
   if (L) {
     AK_REQUIRE(L->nl.r_cut + 1e-12 >= P->pot.r_cut, AK_ERR_ARG,
                "neighbour list r_cut is smaller than the potential's cutoff");
-    L->set.setup(1, std::vector<int64_t>{0, (int64_t)n}.data(), lattice, inv, L->nl.r_list, st);
+    const int64_t off[2] = {0, (int64_t)n};
+    L->set.setup(1, off, lattice, inv, L->nl.r_list, st);
     AK_REQUIRE(L->nl.valid && L->nl.n_atoms == n, AK_ERR_STATE,
                "the neighbour list was not built for this structure");
     return L;
@@ -252,7 +254,8 @@ static bool owned_list_speculative(ak_potential* P, int n, const double* lattice
   const CellInfo ci = cell_info(lattice);
   const double skin = (P->pot.r_cut + 0.3 < ci.min_width / 2.0) ? 0.3 : 0.0;
   if (skin != P->owned_skin || !L.nl.valid || L.nl.n_atoms != n) return false;
-  L.set.setup(1, std::vector<int64_t>{0, (int64_t)n}.data(), lattice, inv, L.nl.r_list, st);
+  const int64_t off[2] = {0, (int64_t)n};
+  L.set.setup(1, off, lattice, inv, L.nl.r_list, st);
   const double* Lb = L.nl.built_geoms[0].lat;
   const double* Ib = L.nl.built_geoms[0].inv;
   double sig = 1.0;
@@ -744,7 +747,8 @@ int ak_compute_device(ak_potential* P, ak_nlist* L, int32_t n, const double* lat
     AK_REQUIRE(L->nl.r_cut + 1e-12 >= P->pot.r_cut, AK_ERR_ARG,
                "neighbour list r_cut is smaller than the potential's cutoff");
     hipStream_t st = stream ? (hipStream_t)stream : P->stream;
-    L->set.setup(1, std::vector<int64_t>{0, (int64_t)n}.data(), lattice, inv, L->nl.r_list, st);
+    const int64_t off[2] = {0, (int64_t)n};
+    L->set.setup(1, off, lattice, inv, L->nl.r_list, st);
     AK_REQUIRE(L->nl.valid && L->nl.n_atoms == n, AK_ERR_STATE,
                "the neighbour list was not built for this structure");
     const int* spc = d_spc;
