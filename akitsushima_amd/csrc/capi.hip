@@ -1039,6 +1039,17 @@ int ak_autotune_batched(ak_potential* P, int32_t S, const int64_t* off, const do
     pot.block = cand[bi][1];
     pot.grid_cap = cand[bi][2];
     pot.eam_lanes = cand[bi][3];
+    // The rounds above queue hundreds of calls between waits. On MI355X
+    // (ROCm 7) the next few dozen launches after such a burst cost about
+    // twice their usual host time (15 us instead of 8 per force call),
+    // whether or not the process pauses in between; calls that each wait
+    // for the stream bring the launch cost back, so the tuner ends with
+    // twenty of them at the chosen shape.
+    for (int r = 0; r < 20; ++r) {
+      compute_forces(pot, B.nl, B.set, P->d_pos.as<double>(), P->d_spc.as<int>(),
+                     P->d_F.as<double>(), P->d_out.as<double>(), st);
+      AK_HIP(hipStreamSynchronize(st));
+    }
     for (int q = 0; q < 4; ++q) best[q] = cand[bi][q];
     *best_ms = best_t;
     if (n_tried) *n_tried = tried;

@@ -596,7 +596,7 @@ __device__ __forceinline__ double qrot(double x) {
 }
 
 template <bool L3Z>
-__global__ __launch_bounds__(512) void k_tersoff_q4(
+__device__ __forceinline__ void q4_chunk(
     int n_chunks, int n, const double* __restrict__ pos, const int* __restrict__ sid,
     const Geom* __restrict__ geoms, int single, const double* __restrict__ tb,
     const double* __restrict__ tri, double rc, const int* __restrict__ cnt,
@@ -721,6 +721,31 @@ __global__ __launch_bounds__(512) void k_tersoff_q4(
                  -0.5 * (Gx * uz + Gz * ux), -Gy * uy, -0.5 * (Gy * uz + Gz * uy),
                  -Gz * uz, 0.0};
   block_partial7(v, partial, c, n_chunks);
+}
+
+// Batched sets: chunks from the chunk table, geometry by structure.
+template <bool L3Z>
+__global__ __launch_bounds__(512) void k_tersoff_q4(
+    int n_chunks, int n, const double* __restrict__ pos, const int* __restrict__ sid,
+    const Geom* __restrict__ geoms, const double* __restrict__ tb, double rc,
+    const int* __restrict__ cnt, const int* __restrict__ nbr, const int* __restrict__ chunk_rng,
+    double* __restrict__ G, double* __restrict__ partial, double* __restrict__ esite) {
+  q4_chunk<L3Z>(n_chunks, n, pos, sid, geoms, 0, tb, tb + 9, rc, cnt, nbr, chunk_rng, G,
+                partial, esite);
+}
+
+// One structure: chunk c is atoms c * blockDim / 4 onwards, one chunk per
+// block.  Fewer kernel arguments than the batched form: the host's launch
+// cost grows with the bytes of arguments, and a single force call on a
+// prebuilt list is paced by it.
+template <bool L3Z>
+__global__ __launch_bounds__(512) void k_tersoff_q4_one(
+    int n, const double* __restrict__ pos, const Geom* __restrict__ geoms,
+    const double* __restrict__ tb, double rc, const int* __restrict__ cnt,
+    const int* __restrict__ nbr, double* __restrict__ G, double* __restrict__ partial,
+    double* __restrict__ esite) {
+  q4_chunk<L3Z>((int)gridDim.x, n, pos, nullptr, geoms, 1, tb, tb + 9, rc, cnt, nbr, nullptr, G,
+                partial, esite);
 }
 
 // ---------------------------------------------------------------------------
@@ -1083,15 +1108,15 @@ __global__ __launch_bounds__(1024) void k_struct_final(const Geom* __restrict__ 
 // per-structure energy and stress (the site kernel has completed, so any
 // block may read them).
 template <int L>
-__global__ __launch_bounds__(256) void k_gather(int n, const int* __restrict__ cnt,
-                                                const int* __restrict__ rev, int stride,
-                                                const double* __restrict__ G,
-                                                double* __restrict__ F, int atom_blocks,
-                                                int n_struct, const Geom* __restrict__ geoms,
-                                                const int* __restrict__ first, int n_chunks,
-                                                const double* __restrict__ partial,
-                                                double* __restrict__ out,
-                                                const uint64_t* __restrict__ amask, int sym) {
+__device__ __forceinline__ void gather_blocks(int n, const int* __restrict__ cnt,
+                                              const int* __restrict__ rev, int stride,
+                                              const double* __restrict__ G,
+                                              double* __restrict__ F, int atom_blocks,
+                                              int n_struct, const Geom* __restrict__ geoms,
+                                              const int* __restrict__ first, int n_chunks,
+                                              const double* __restrict__ partial,
+                                              double* __restrict__ out,
+                                              const uint64_t* __restrict__ amask, int sym) {
   // the structure blocks come first, so that they are dispatched before
   // the atom blocks instead of forming the launch's tail; gather_lead pads
   // them to a whole number of XCD rounds, so that atom tile b runs on the
@@ -1145,6 +1170,35 @@ __global__ __launch_bounds__(256) void k_gather(int n, const int* __restrict__ c
     F[3 * (size_t)a + 1] = fy;
     F[3 * (size_t)a + 2] = fz;
   }
+}
+
+template <int L>
+__global__ __launch_bounds__(256) void k_gather(int n, const int* __restrict__ cnt,
+                                                const int* __restrict__ rev, int stride,
+                                                const double* __restrict__ G,
+                                                double* __restrict__ F, int atom_blocks,
+                                                int n_struct, const Geom* __restrict__ geoms,
+                                                const int* __restrict__ first, int n_chunks,
+                                                const double* __restrict__ partial,
+                                                double* __restrict__ out,
+                                                const uint64_t* __restrict__ amask, int sym) {
+  gather_blocks<L>(n, cnt, rev, stride, G, F, atom_blocks, n_struct, geoms, first, n_chunks,
+                   partial, out, amask, sym);
+}
+
+// One structure on rows of four slots, with the arguments of that case only
+// (the launch cost grows with the bytes of arguments).
+__global__ __launch_bounds__(256) void k_gather_one4(int n, int atom_blocks,
+                                                     const int* __restrict__ cnt,
+                                                     const int* __restrict__ rev,
+                                                     const double* __restrict__ G,
+                                                     double* __restrict__ F,
+                                                     const Geom* __restrict__ geoms,
+                                                     const int* __restrict__ first, int n_chunks,
+                                                     const double* __restrict__ partial,
+                                                     double* __restrict__ out) {
+  gather_blocks<4>(n, cnt, rev, 4, G, F, atom_blocks, 1, geoms, first, n_chunks, partial, out,
+                   nullptr, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -2153,10 +2207,15 @@ static void launch_bond_kernels(Potential& pot, NList& nl, const ForceWork& w,
       // one block per chunk (the grid cap does not apply)
       const double* hp = pot.h_params.data();
       if (!done && q4_path(pot, L, false)) {
-        launch_timed(pot, hp[13] == 0.0 ? k_tersoff_q4<true> : k_tersoff_q4<false>,
-                     dim3(n_chunks > 0 ? n_chunks : 1), block,
-                     st, (size_t)0, n_chunks, n, d_pos, sid, geoms, single, tb, tri, pot.r_cut, cnt,
-                     nbr, rng, G, part, esite);
+        const dim3 cg(n_chunks > 0 ? n_chunks : 1);
+        if (single)
+          launch_timed(pot, hp[13] == 0.0 ? k_tersoff_q4_one<true> : k_tersoff_q4_one<false>,
+                       cg, block, st, (size_t)0, n, d_pos, geoms, tb, pot.r_cut, cnt, nbr, G,
+                       part, esite);
+        else
+          launch_timed(pot, hp[13] == 0.0 ? k_tersoff_q4<true> : k_tersoff_q4<false>, cg, block,
+                       st, (size_t)0, n_chunks, n, d_pos, sid, geoms, tb, pot.r_cut, cnt, nbr,
+                       rng, G, part, esite);
         done = true;
       }
     }
@@ -2202,9 +2261,14 @@ static void launch_bond_kernels(Potential& pot, NList& nl, const ForceWork& w,
                        (const uint64_t*)amask, sym);
   } else {
     const int atom_blocks = (int)(((int64_t)n * L + 255) / 256);
-    hipLaunchKernelGGL(k_gather<L>, dim3(atom_blocks + lead), dim3(256), 0, st, n, cnt, rev,
-                       nl.stride, (const double*)G, d_F, atom_blocks, n_struct, geoms, first,
-                       n_chunks, (const double*)part, d_out, (const uint64_t*)amask, sym);
+    if (L == 4 && n_struct == 1)
+      hipLaunchKernelGGL(k_gather_one4, dim3(atom_blocks + lead), dim3(256), 0, st, n,
+                         atom_blocks, cnt, rev, (const double*)G, d_F, geoms, first, n_chunks,
+                         (const double*)part, d_out);
+    else
+      hipLaunchKernelGGL(k_gather<L>, dim3(atom_blocks + lead), dim3(256), 0, st, n, cnt, rev,
+                         nl.stride, (const double*)G, d_F, atom_blocks, n_struct, geoms, first,
+                         n_chunks, (const double*)part, d_out, (const uint64_t*)amask, sym);
   }
 }
 

@@ -114,12 +114,13 @@ def pair_counts(lat, pos, r_cut, host_pairs=None):
 
 
 def site_kernel_name(params, stride, tile):
-    """The Tersoff site kernel the engine launches for a flat (one-species,
-    m = 3, gamma = 1) parameter set on rows of `stride` slots."""
+    """The Tersoff site kernel the engine launches for one structure and a
+    flat (one-species, m = 3, gamma = 1) parameter set on rows of `stride`
+    slots."""
     if tile == 2 and stride <= 8:
         return f"k_tersoff_trip<{stride}, true>"
     if stride == 4:
-        return f"k_tersoff_q4<{'true' if params['lambda3'] == 0.0 else 'false'}>"
+        return f"k_tersoff_q4_one<{'true' if params['lambda3'] == 0.0 else 'false'}>"
     return f"k_tersoff_site<{stride}, true>"
 
 

@@ -212,7 +212,9 @@ int ak_potential_set_launch(ak_potential *pot, int32_t bond_tile, int32_t block_
  * candidate launch shape -- Tersoff site form and threads per block, or EAM
  * lanes per atom and threads per block -- over `reps` device force calls
  * with HIP events, in three rounds (each candidate keeps its best round), and
- * leaves the fastest set on the potential.  best (4) = bond_tile,
+ * leaves the fastest set on the potential.  It ends with twenty force calls
+ * at that shape on its own stream, each waited for, and returns with that
+ * stream idle.  best (4) = bond_tile,
  * block_threads, grid_cap, eam_lanes; *best_ms = its time per call.  When
  * shapes (capacity, 4) and times_ms (capacity) are given they receive every
  * candidate tried; *n_tried counts them. */
