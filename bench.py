@@ -14,7 +14,9 @@ The line also carries:
                 ``tersoff_flops``) over its average duration, timed with HIP
                 start/stop events of each of its dispatches on its launch
                 stream, over a second pass of the K timed steps (the events
-                cost host time, so the throughput pass runs without them);
+                cost host time, so the throughput pass runs without them;
+                the pass is queued behind a bounded GPU-side wait, so that
+                its calls run back to back as the throughput pass's do);
                 the kernel is bound by FP64 VALU issue (no matrix work);
                 ``traffic`` and the hardware FP64 operation count from the
                 committed rocprofv3 counter summary of the same kernel
@@ -769,18 +771,45 @@ def main():
     value = n * args.steps / elapsed
     # kernel duration: the same K steps again with start/stop events around
     # every site-kernel dispatch (hipExtLaunchKernelGGL on the launch stream).
-    # The events cost ~20 us of host time per call, so they get a pass of
-    # their own instead of slowing the throughput pass.
-    _lib.check(L.ak_profile_enable(h, 1))
-    torch.cuda.synchronize()
-    te0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    events_ms_per_step = (time.perf_counter() - te0) / args.steps * 1e3
+    # The events cost host time per call (about three times the GPU time of
+    # a call), so they get a pass of their own instead of slowing the
+    # throughput pass.  Outside profile mode (whose trace counts the
+    # dispatches), a first, unread pass creates the K event pairs
+    # (hipEventCreate) and measures the host time per event-timed call; the
+    # measured pass is then queued behind a GPU-side wait (torch's bounded
+    # spin kernel) that outlasts its enqueueing, so that its calls run back
+    # to back as in the throughput pass instead of each on an idle GPU.
     tot_ms = ctypes.c_double()
     min_ms = ctypes.c_double()
     cnt = ctypes.c_int64()
+    hold_ticks = 0
+    if not args.profile_only:
+        _lib.check(L.ak_profile_enable(h, 1))
+        tp0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        enqueue_s = time.perf_counter() - tp0
+        torch.cuda.synchronize()
+        _lib.check(L.ak_profile_read(h, ctypes.byref(tot_ms), ctypes.byref(min_ms),
+                                     ctypes.byref(cnt)))
+        # spin-kernel ticks per second on this GPU
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        torch.cuda._sleep(1_000_000)
+        ev1.record()
+        ev1.synchronize()
+        ticks_per_s = 1e6 / (ev0.elapsed_time(ev1) * 1e-3)
+        hold_ticks = int(ticks_per_s * (1.5 * enqueue_s + 1e-3))
+    _lib.check(L.ak_profile_enable(h, 1))
+    torch.cuda.synchronize()
+    if hold_ticks:
+        torch.cuda._sleep(hold_ticks)
+    te0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    events_host_us_per_call = (time.perf_counter() - te0) / args.steps * 1e6
+    torch.cuda.synchronize()
     _lib.check(L.ak_profile_read(h, ctypes.byref(tot_ms), ctypes.byref(min_ms),
                                  ctypes.byref(cnt)))
     _lib.check(L.ak_profile_enable(h, 0))
@@ -829,8 +858,9 @@ def main():
                 "kernel_avg_us": avg_site_s * 1e6,
                 "kernel_min_us": min_ms.value * 1e3, "launches_timed": int(cnt.value),
                 "timing": "HIP start/stop events of each site-kernel dispatch "
-                          "(hipExtLaunchKernelGGL) over a second pass of the K timed steps",
-                "events_pass_ms_per_step": events_ms_per_step,
+                          "(hipExtLaunchKernelGGL) over a second pass of the K timed steps, "
+                          "queued behind a GPU-side wait so that they run back to back",
+                "events_pass_host_us_per_call": events_host_us_per_call,
                 "flops_per_launch": flops,
                 "flops_per_atom": flops / n,
                 "compute_unit": "FP64 VALU (MI355X FP64 vector peak = 78.6 TFLOP/s)"}
