@@ -8,7 +8,7 @@ import pytest
 import oracle as O
 from akitsushima_amd import StillingerWeberPotential, TersoffPotential, zhou_eam
 from akitsushima_amd.published import ZHOU_2004
-from akitsushima_amd.structures import diamond_batch
+from akitsushima_amd.structures import DIAMOND, cubic, diamond_batch
 
 from _cases import SW_SI, TERSOFF_SI, cuni_all, rel_err
 
@@ -41,6 +41,25 @@ def test_tersoff_batch_every_launch_shape(shape):
     pot.set_launch(*shape)
     E, F, S = pot.compute_batched(structs)
     _compare(E, F, S, refs)
+
+
+@pytest.mark.parametrize("block", [256, 512])
+@pytest.mark.parametrize("cells", [4, 5])
+def test_single_structure_kernels_equal_the_batched_ones_bit_for_bit(cells, block):
+    """One structure on rows of four runs the single-structure forms of the
+    four-slot site kernel and its gather (fewer kernel arguments, chunks
+    from the block index); the same structure twice in a batch runs the
+    batched forms over the same chunks (the last one partial at 1000
+    atoms). Energy, forces and stress agree bit for bit."""
+    lat, pos, types = cubic(DIAMOND, 5.431, cells, ["Si"] * 8, rattle=0.05, seed=3)
+    pot = TersoffPotential(TERSOFF_SI, species=["Si"])
+    pot.set_launch(1, block, 0)
+    e, f, s = pot.compute(lat, pos, types)
+    E, F, S = pot.compute_batched([(lat, pos, types), (lat, pos, types)])
+    for k in range(2):
+        assert E[k] == e, (k, E[k], e)
+        assert np.array_equal(F[k], f), (k, np.abs(F[k] - f).max())
+        assert np.array_equal(S[k], s), (k, np.abs(S[k] - s).max())
 
 
 @pytest.mark.parametrize("shape", [(0, 64, 0), (0, 128, 2), (0, 256, 7), (0, 512, 0)])
