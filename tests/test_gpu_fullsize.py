@@ -30,11 +30,13 @@ def _check(e, f, s, e0, f0, s0, label):
     assert np.abs(s - s.T).max() < 1e-12 * max(1.0, np.abs(s).max())
 
 
-@pytest.mark.parametrize("tile", [1, 2])
-def test_tersoff_32768_atoms(tile):
+@pytest.mark.parametrize("tile,block", [(1, 256), (2, 256), (1, 512)])
+def test_tersoff_32768_atoms(tile, block):
+    """The headline workload; (1, 512) is the launch shape the autotuner
+    picks for it (the single-structure four-slot kernel, 256 blocks)."""
     lat, pos, t = cubic(DIAMOND, 5.431, 16, ["Si"] * 8, rattle=0.05, seed=0)
     pot = TersoffPotential(dict(TERSOFF_SI_T3), species=["Si"])
-    pot.set_bond_tile(tile)
+    pot.set_launch(tile, block, 0)
     e, f, s = pot.compute(lat, pos, t)
     e0, f0, s0 = O.tersoff_flat(lat, pos, dict(TERSOFF_SI_T3), nthreads=8)
     _check(e, f, s, e0, f0, s0, "tersoff 32768")
