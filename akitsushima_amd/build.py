@@ -28,6 +28,10 @@ HIPCC_FLAGS = [
     "-Wno-unused-variable",
     # no packed FP32 instructions in device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # kernel-argument preloading (gfx950): the first 14 dwords of a kernel's
+    # arguments arrive in SGPRs instead of through a load at the kernel's
+    # start; each kernel keeps the 256-byte compatibility prologue
+    "-mllvm", "-amdgpu-kernarg-preload-count=16",
 ]
 
 

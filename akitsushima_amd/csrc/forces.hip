@@ -597,13 +597,12 @@ __device__ __forceinline__ double qrot(double x) {
 
 template <bool L3Z>
 __device__ __forceinline__ void q4_chunk(
-    int n_chunks, int n, const double* __restrict__ pos, const int* __restrict__ sid,
+    int n_chunks, int apb, int n, const double* __restrict__ pos, const int* __restrict__ sid,
     const Geom* __restrict__ geoms, int single, const double* __restrict__ tb,
     const double* __restrict__ tri, double rc, const int* __restrict__ cnt,
     const int* __restrict__ nbr, const int* __restrict__ chunk_rng, double* __restrict__ G,
     double* __restrict__ partial, double* __restrict__ esite) {
   const int c = xcd_remap((int)blockIdx.x, n_chunks);
-  const int apb = (int)blockDim.x >> 2;
   const int a0 = single ? c * apb : chunk_rng[2 * c];
   const int a1 = single ? min(a0 + apb, n) : chunk_rng[2 * c + 1];
   const Geom& g = geoms[single ? 0 : sid[a0]];
@@ -730,23 +729,27 @@ __global__ __launch_bounds__(512) void k_tersoff_q4(
     const Geom* __restrict__ geoms, const double* __restrict__ tb, double rc,
     const int* __restrict__ cnt, const int* __restrict__ nbr, const int* __restrict__ chunk_rng,
     double* __restrict__ G, double* __restrict__ partial, double* __restrict__ esite) {
-  q4_chunk<L3Z>(n_chunks, n, pos, sid, geoms, 0, tb, tb + 9, rc, cnt, nbr, chunk_rng, G,
+  q4_chunk<L3Z>(n_chunks, 0, n, pos, sid, geoms, 0, tb, tb + 9, rc, cnt, nbr, chunk_rng, G,
                 partial, esite);
 }
 
-// One structure: chunk c is atoms c * blockDim / 4 onwards, one chunk per
-// block.  Fewer kernel arguments than the batched form: the host's launch
-// cost grows with the bytes of arguments, and a single force call on a
-// prebuilt list is paced by it.
-template <bool L3Z>
-__global__ __launch_bounds__(512) void k_tersoff_q4_one(
-    int n, const double* __restrict__ pos, const Geom* __restrict__ geoms,
+// One structure: chunk c is atoms c * BLK / 4 onwards, one chunk per block.
+// Fewer kernel arguments than the batched form: the host's launch cost
+// grows with the bytes of arguments, and a single force call on a prebuilt
+// list is paced by it.  The block size is a template parameter and the
+// chunk count an argument, so that nothing is read from the dispatch's
+// hidden arguments, and the arguments the first loads need come first
+// (with kernel-argument preloading they arrive in SGPRs).
+template <bool L3Z, int BLK>
+__global__ __launch_bounds__(BLK) void k_tersoff_q4_one(
+    int n, int n_chunks, const double* __restrict__ pos, const Geom* __restrict__ geoms,
     const double* __restrict__ tb, double rc, const int* __restrict__ cnt,
     const int* __restrict__ nbr, double* __restrict__ G, double* __restrict__ partial,
     double* __restrict__ esite) {
-  q4_chunk<L3Z>((int)gridDim.x, n, pos, nullptr, geoms, 1, tb, tb + 9, rc, cnt, nbr, nullptr, G,
-                partial, esite);
+  q4_chunk<L3Z>(n_chunks, BLK / 4, n, pos, nullptr, geoms, 1, tb, tb + 9, rc, cnt, nbr, nullptr,
+                G, partial, esite);
 }
+
 
 // ---------------------------------------------------------------------------
 // Tersoff site kernel, one lane per (bond, partner) pair.
@@ -1056,10 +1059,12 @@ __device__ __forceinline__ void struct_reduce_block(int s, const Geom* __restric
                                                     int n_chunks,
                                                     const double* __restrict__ partial,
                                                     double* __restrict__ out,
-                                                    const int* __restrict__ pair_flag = nullptr) {
+                                                    const int* __restrict__ pair_flag = nullptr,
+                                                    int threads = 0) {
   __shared__ double sm[10][16];
   const int c0 = first[s], c1 = first[s + 1];
-  const int nt = (int)blockDim.x, nw = nt >> 6;
+  // threads per block: blockDim, or the caller's compile-time block size
+  const int nt = threads > 0 ? threads : (int)blockDim.x, nw = nt >> 6;
   double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int c = c0 + (int)threadIdx.x; c < c1; c += U * nt) {
     double x[U][10];
@@ -1116,18 +1121,19 @@ __device__ __forceinline__ void gather_blocks(int n, const int* __restrict__ cnt
                                               const int* __restrict__ first, int n_chunks,
                                               const double* __restrict__ partial,
                                               double* __restrict__ out,
-                                              const uint64_t* __restrict__ amask, int sym) {
+                                              const uint64_t* __restrict__ amask, int sym,
+                                              int lead, int threads) {
   // the structure blocks come first, so that they are dispatched before
   // the atom blocks instead of forming the launch's tail; gather_lead pads
   // them to a whole number of XCD rounds, so that atom tile b runs on the
   // XCD whose L2 holds the site gradients of site-kernel chunk b
-  const int lead = (int)gridDim.x - atom_blocks;
   if ((int)blockIdx.x < lead) {
     if ((int)blockIdx.x < n_struct)
-      struct_reduce_block<2>((int)blockIdx.x, geoms, first, n_chunks, partial, out);
+      struct_reduce_block<2>((int)blockIdx.x, geoms, first, n_chunks, partial, out, nullptr,
+                             threads);
     return;
   }
-  const int gt = xcd_remap((int)blockIdx.x - lead, atom_blocks) * blockDim.x + threadIdx.x;
+  const int gt = xcd_remap((int)blockIdx.x - lead, atom_blocks) * threads + threadIdx.x;
   const int a = gt / L;
   const int k = gt & (L - 1);
   const bool live = a < n;
@@ -1183,22 +1189,25 @@ __global__ __launch_bounds__(256) void k_gather(int n, const int* __restrict__ c
                                                 double* __restrict__ out,
                                                 const uint64_t* __restrict__ amask, int sym) {
   gather_blocks<L>(n, cnt, rev, stride, G, F, atom_blocks, n_struct, geoms, first, n_chunks,
-                   partial, out, amask, sym);
+                   partial, out, amask, sym, (int)gridDim.x - atom_blocks, (int)blockDim.x);
 }
 
 // One structure on rows of four slots, with the arguments of that case only
-// (the launch cost grows with the bytes of arguments).
-__global__ __launch_bounds__(256) void k_gather_one4(int n, int atom_blocks,
-                                                     const int* __restrict__ cnt,
+// (the launch cost grows with the bytes of arguments), blocks of 256
+// threads.  The lead-block count is an argument, so that nothing is read
+// from the dispatch's hidden arguments, and the atom blocks' arguments come
+// first (with kernel-argument preloading they arrive in SGPRs).
+__global__ __launch_bounds__(256) void k_gather_one4(int n, int atom_blocks, int lead,
+                                                     int n_chunks, const int* __restrict__ cnt,
                                                      const int* __restrict__ rev,
                                                      const double* __restrict__ G,
                                                      double* __restrict__ F,
                                                      const Geom* __restrict__ geoms,
-                                                     const int* __restrict__ first, int n_chunks,
+                                                     const int* __restrict__ first,
                                                      const double* __restrict__ partial,
                                                      double* __restrict__ out) {
   gather_blocks<4>(n, cnt, rev, 4, G, F, atom_blocks, 1, geoms, first, n_chunks, partial, out,
-                   nullptr, 1);
+                   nullptr, 1, lead, 256);
 }
 
 // ---------------------------------------------------------------------------
@@ -2171,6 +2180,17 @@ static bool q4_path(const Potential& pot, int lanes, bool trip) {
          hp[9] != 0.0 && hp[14] == 3.0 && hp[15] == 1.0;
 }
 
+// The single-structure four-slot kernel at block size BLK.
+template <int BLK>
+static void launch_q4_one(Potential& pot, bool l3z, int n_chunks, hipStream_t st, int n,
+                          const double* pos, const Geom* geoms, const double* tb, double rc,
+                          const int* cnt, const int* nbr, double* G, double* part,
+                          double* esite) {
+  launch_timed(pot, l3z ? k_tersoff_q4_one<true, BLK> : k_tersoff_q4_one<false, BLK>,
+               dim3(n_chunks > 0 ? n_chunks : 1), dim3(BLK), st, (size_t)0, n, n_chunks, pos,
+               geoms, tb, rc, cnt, nbr, G, part, esite);
+}
+
 template <int L>
 static void launch_bond_kernels(Potential& pot, NList& nl, const ForceWork& w,
                                 const double* d_pos, const int* d_spc, double* d_F,
@@ -2208,11 +2228,15 @@ static void launch_bond_kernels(Potential& pot, NList& nl, const ForceWork& w,
       const double* hp = pot.h_params.data();
       if (!done && q4_path(pot, L, false)) {
         const dim3 cg(n_chunks > 0 ? n_chunks : 1);
-        if (single)
-          launch_timed(pot, hp[13] == 0.0 ? k_tersoff_q4_one<true> : k_tersoff_q4_one<false>,
-                       cg, block, st, (size_t)0, n, d_pos, geoms, tb, pot.r_cut, cnt, nbr, G,
-                       part, esite);
-        else
+        if (single) {
+          const bool l3z = hp[13] == 0.0;
+          switch (blk) {
+            case 64: launch_q4_one<64>(pot, l3z, n_chunks, st, n, d_pos, geoms, tb, pot.r_cut, cnt, nbr, G, part, esite); break;
+            case 128: launch_q4_one<128>(pot, l3z, n_chunks, st, n, d_pos, geoms, tb, pot.r_cut, cnt, nbr, G, part, esite); break;
+            case 256: launch_q4_one<256>(pot, l3z, n_chunks, st, n, d_pos, geoms, tb, pot.r_cut, cnt, nbr, G, part, esite); break;
+            default: launch_q4_one<512>(pot, l3z, n_chunks, st, n, d_pos, geoms, tb, pot.r_cut, cnt, nbr, G, part, esite); break;
+          }
+        } else
           launch_timed(pot, hp[13] == 0.0 ? k_tersoff_q4<true> : k_tersoff_q4<false>, cg, block,
                        st, (size_t)0, n_chunks, n, d_pos, sid, geoms, tb, pot.r_cut, cnt, nbr,
                        rng, G, part, esite);
@@ -2263,8 +2287,8 @@ static void launch_bond_kernels(Potential& pot, NList& nl, const ForceWork& w,
     const int atom_blocks = (int)(((int64_t)n * L + 255) / 256);
     if (L == 4 && n_struct == 1)
       hipLaunchKernelGGL(k_gather_one4, dim3(atom_blocks + lead), dim3(256), 0, st, n,
-                         atom_blocks, cnt, rev, (const double*)G, d_F, geoms, first, n_chunks,
-                         (const double*)part, d_out);
+                         atom_blocks, lead, n_chunks, cnt, rev, (const double*)G, d_F, geoms,
+                         first, (const double*)part, d_out);
     else
       hipLaunchKernelGGL(k_gather<L>, dim3(atom_blocks + lead), dim3(256), 0, st, n, cnt, rev,
                          nl.stride, (const double*)G, d_F, atom_blocks, n_struct, geoms, first,

@@ -115,14 +115,14 @@ def pair_counts(lat, pos, r_cut, host_pairs=None):
     return int(m.sum()), int(np.sum(cnt * (cnt - 1)) // 2)
 
 
-def site_kernel_name(params, stride, tile):
+def site_kernel_name(params, stride, tile, block=256):
     """The Tersoff site kernel the engine launches for one structure and a
     flat (one-species, m = 3, gamma = 1) parameter set on rows of `stride`
-    slots."""
+    slots, at `block` threads per block."""
     if tile == 2 and stride <= 8:
         return f"k_tersoff_trip<{stride}, true>"
     if stride == 4:
-        return f"k_tersoff_q4_one<{'true' if params['lambda3'] == 0.0 else 'false'}>"
+        return f"k_tersoff_q4_one<{'true' if params['lambda3'] == 0.0 else 'false'}, {block}>"
     return f"k_tersoff_site<{stride}, true>"
 
 
@@ -845,7 +845,8 @@ def main():
     while stride < int(np.bincount(np.concatenate([ci, cj]), minlength=n).max()):
         stride *= 2
     tile = args.bond_tile or (tuned or {}).get("bond_tile", 1)
-    site_kernel = site_kernel_name(params, stride, tile)
+    site_kernel = site_kernel_name(params, stride, tile,
+                                   (tuned or {}).get("block_threads", 256))
     n_pairs, n_trip = geometry_counts(lat, pos, pot.r_cut)
     flops = tersoff_flops(n_pairs, n_trip, params["lambda3"])
     avg_site_s = tot_ms.value / max(cnt.value, 1) * 1e-3

@@ -9,7 +9,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_counter_summary_names_the_headline_kernel():
-    name = bench.site_kernel_name({"lambda3": 0.0}, 4, 1)
+    name = bench.site_kernel_name({"lambda3": 0.0}, 4, 1, 512)
     c = bench.read_counters(name)
     assert c is not None, name
     assert abs(c["bytes_per_launch"] - c["fetch_bytes"] - c["write_bytes"]) <= 1
