@@ -722,13 +722,16 @@ __device__ __forceinline__ void q4_chunk(
   block_partial7(v, partial, c, n_chunks);
 }
 
-// Batched sets: chunks from the chunk table, geometry by structure.
+// Batched sets: chunks from the chunk table, geometry by structure.  The
+// arguments the first loads need (chunk table, structure ids, geometry,
+// positions, rows) come first, inside the preloaded 56 bytes.
 template <bool L3Z>
 __global__ __launch_bounds__(512) void k_tersoff_q4(
-    int n_chunks, int n, const double* __restrict__ pos, const int* __restrict__ sid,
-    const Geom* __restrict__ geoms, const double* __restrict__ tb, double rc,
-    const int* __restrict__ cnt, const int* __restrict__ nbr, const int* __restrict__ chunk_rng,
-    double* __restrict__ G, double* __restrict__ partial, double* __restrict__ esite) {
+    int n_chunks, int n, const int* __restrict__ chunk_rng, const int* __restrict__ sid,
+    const Geom* __restrict__ geoms, const double* __restrict__ pos,
+    const int* __restrict__ cnt, const int* __restrict__ nbr, const double* __restrict__ tb,
+    double rc, double* __restrict__ G, double* __restrict__ partial,
+    double* __restrict__ esite) {
   q4_chunk<L3Z>(n_chunks, 0, n, pos, sid, geoms, 0, tb, tb + 9, rc, cnt, nbr, chunk_rng, G,
                 partial, esite);
 }
@@ -2238,8 +2241,8 @@ static void launch_bond_kernels(Potential& pot, NList& nl, const ForceWork& w,
           }
         } else
           launch_timed(pot, hp[13] == 0.0 ? k_tersoff_q4<true> : k_tersoff_q4<false>, cg, block,
-                       st, (size_t)0, n_chunks, n, d_pos, sid, geoms, tb, pot.r_cut, cnt, nbr,
-                       rng, G, part, esite);
+                       st, (size_t)0, n_chunks, n, rng, sid, geoms, d_pos, cnt, nbr, tb,
+                       pot.r_cut, G, part, esite);
         done = true;
       }
     }
